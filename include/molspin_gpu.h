@@ -1,0 +1,261 @@
+/*
+ * molspin_gpu.h — C ABI of the MI355X (gfx950) Hilbert-space propagation backend.
+ *
+ * The boundary replaces the per-orientation time loop of MolSpin's HSGeneral task
+ * (RunSection/General/HS/TaskHSGeneral.cpp:470-576) together with the propagator it
+ * drives (RunSection/General/HS/HSPropagator.cpp:51-140) and the observable
+ * collector it samples (RunSection/General/HS/HSObservableCollector.cpp:276-326).
+ * A caller describes a spin system structurally (spins, interactions, states,
+ * transitions, relaxation operators — the SpinAPI object model), uploads it once,
+ * and then propagates batches of "items" (field orientations or field strengths)
+ * on the device.  The library owns the whole time loop: observable sampling at
+ * every grid point, the trapezoid and the optional finite-time correction.
+ *
+ * Conventions
+ *   - Plain C: pointers and sizes only.  No entry point throws; every entry point
+ *     returns an msg_status code and leaves a message retrievable with
+ *     msg_last_error() (mirrors the bool + std::string& error convention of
+ *     HSPropagator::Step, HSPropagator.h:40-41).
+ *   - Complex arrays are interleaved (re, im) doubles.  State vectors exchanged
+ *     with the caller use MolSpin's basis order: spins in declaration order, the
+ *     first spin most significant (SpinSpace::CreateOperator kron order,
+ *     SpinSpace_operators.cpp:84-102), and within a spin |+S> first
+ *     (SpinSpace_states.cpp:206-209).  Column-major D x M, like arma::cx_mat.
+ *   - Units are MolSpin's: fields and tensors as written in the input (T, or mT with
+ *     prefactor=0.001), rates in 1/ns, times in ns.  The common prefactor
+ *     87.9410005 rad ns^-1 T^-1 is applied where an interaction requests it
+ *     (SpinSpace_hamiltonians.cpp:746-748).
+ *   - One context per host thread at a time; a context owns one HIP stream.
+ */
+#ifndef MOLSPIN_GPU_H
+#define MOLSPIN_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum msg_status {
+    MSG_OK = 0,
+    MSG_INVALID_ARG = 1,      /* malformed description or parameters            */
+    MSG_UNSUPPORTED = 2,      /* valid for MolSpin but outside the device paths  */
+    MSG_OOM_ADMISSION = 3,    /* the request does not fit the device budget      */
+    MSG_DEVICE_ERROR = 4,     /* HIP runtime failure                             */
+    MSG_NUMERIC_BREAKDOWN = 5 /* non-finite state or Krylov breakdown            */
+} msg_status;
+
+/* ------------------------------------------------------------------------- */
+/* Structural system description (SpinAPI object model)                      */
+/* ------------------------------------------------------------------------- */
+
+/* SpinAPI::Spin (SpinAPI/Spin.cpp:29-53).  `tensor` is Tensor::LabFrame() of the
+ * spin tensor (default isotropic 2.0, Spin.cpp:29), row-major.  Quantisation axes
+ * are the laboratory axes. */
+typedef struct msg_spin_desc {
+    int32_t multiplicity;      /* 2S+1 */
+    int32_t is_electron;       /* type=electron */
+    int32_t tensor_isotropic;  /* IsIsotropic(tensor) (Tensor.cpp:834-840) */
+    int32_t reserved;
+    double  tensor[9];
+} msg_spin_desc;
+
+enum {
+    MSG_INTERACTION_ZEEMAN = 1,     /* onespin/singlespin/zeeman      (Interaction.cpp:91)  */
+    MSG_INTERACTION_DOUBLESPIN = 2, /* twospin/doublespin/hyperfine/dipole (Interaction.cpp:106) */
+    MSG_INTERACTION_EXCHANGE = 3    /* exchange                        (Interaction.cpp:116) */
+};
+
+/* SpinAPI::Interaction, the part InteractionOperator reads
+ * (SpinSpace_hamiltonians.cpp:438-777).  Spin groups index msg_system_desc.spins
+ * through msg_system_desc.index_pool. */
+typedef struct msg_interaction_desc {
+    int32_t type;
+    int32_t group1_offset, group1_count;
+    int32_t group2_offset, group2_count;
+    int32_t tensor_present;    /* coupling tensor given (else dot product)            */
+    int32_t tensor_isotropic;  /* IsIsotropic(coupling tensor)                         */
+    int32_t common_prefactor;  /* commonprefactor (default true, Interaction.cpp:25)  */
+    int32_t ignore_tensors;    /* ignoretensors   (default false)                      */
+    int32_t field_swept;       /* Zeeman field replaced by the batch item's field      */
+    double  prefactor;         /* prefactor (default 1)                                */
+    double  field[3];          /* Zeeman field as written                              */
+    double  tensor[9];         /* Tensor::LabFrame() of the coupling tensor, row-major */
+} msg_interaction_desc;
+
+/* SpinAPI::State restricted to the spins it fixes: a (not necessarily normalised)
+ * amplitude vector over `spins` in the listed order, first spin most significant.
+ * Products of independently specified spins are given as their kron product. */
+typedef struct msg_state_desc {
+    int32_t spins_offset, spins_count; /* into index_pool */
+    int32_t amp_offset;                /* into amp_pool, in complex elements */
+    int32_t reserved;
+} msg_state_desc;
+
+/* SpinAPI::Transition: a sink from `source_state` at `rate` (1/ns). */
+typedef struct msg_transition_desc {
+    int32_t source_state;
+    int32_t reserved;
+    double  rate;
+} msg_transition_desc;
+
+/* relaxationdephasing Operator on two spin-1/2 spins (SpinAPI/Operator.cpp:133,250;
+ * SpinSpace_relaxation.cpp:1144-1199). */
+typedef struct msg_relaxation_desc {
+    int32_t spin1, spin2;
+    double  rate;
+} msg_relaxation_desc;
+
+typedef struct msg_system_desc {
+    int32_t n_spins;
+    const msg_spin_desc *spins;
+    int32_t n_interactions;
+    const msg_interaction_desc *interactions;
+    int32_t n_states;
+    const msg_state_desc *states;
+    int32_t n_transitions;
+    const msg_transition_desc *transitions;
+    int32_t n_initial;                 /* initialstate list (Properties initialstate=)  */
+    const int32_t *initial_states;
+    const double *initial_weights;     /* NULL: equal weights (HSStatePreparation.cpp:96-103) */
+    int32_t n_relaxations;
+    const msg_relaxation_desc *relaxations;
+    int32_t index_pool_len;
+    const int32_t *index_pool;
+    int32_t amp_pool_len;              /* complex elements */
+    const double *amp_pool;            /* interleaved re, im */
+} msg_system_desc;
+
+/* ------------------------------------------------------------------------- */
+/* Run parameters (HSExecutionPlan, HSExecutionPlan.h:33-117)               */
+/* ------------------------------------------------------------------------- */
+
+enum {
+    MSG_CALC_YIELDS = 0,          /* calculation=yields        */
+    MSG_CALC_TIMEEVOLUTION = 1    /* calculation=timeevolution */
+};
+enum {
+    MSG_OBS_TRANSITION_YIELDS = 0, /* rate * Tr[P_source rho]  (HSObservableCollector.cpp:276-288) */
+    MSG_OBS_STATE_POPULATIONS = 1  /* Tr[P_state rho] for every State object                       */
+};
+enum {
+    MSG_METHOD_AUTO = 0,          /* device chooses: dense U for D<=512, matrix-free Krylov above */
+    MSG_METHOD_KRYLOV = 1,        /* propagationmethod=krylov, fixed krylovsize                   */
+    MSG_METHOD_EXPONENTIAL = 2    /* propagationmethod=normal: exact exp(dt G) applied as a GEMM  */
+};
+enum {
+    MSG_SAMPLING_DIRECT = 0,      /* factors of rho0 (HSStatePreparation.cpp:77-165)              */
+    MSG_SAMPLING_GIVEN = 1        /* caller-supplied columns (trace samples drawn on the host)    */
+};
+
+typedef struct msg_run_params {
+    int32_t  calculation;
+    int32_t  observables;
+    int32_t  method;
+    int32_t  krylov_size;         /* krylovsize (default 16) */
+    int32_t  sampling;
+    int32_t  yield_corrections;   /* yieldcorrections (default false) */
+    double   total_time;          /* totaltime (ns) */
+    double   time_step;           /* timestep  (ns) */
+    uint64_t seed;                /* key of the counter-based relaxation RNG */
+} msg_run_params;
+
+/* ------------------------------------------------------------------------- */
+/* Context and system handles                                               */
+/* ------------------------------------------------------------------------- */
+
+typedef struct msg_ctx msg_ctx;
+typedef struct msg_system msg_system;
+
+/* Creates a context on HIP device `device` with its own stream. */
+msg_status msg_ctx_create(int device, msg_ctx **out);
+msg_status msg_ctx_destroy(msg_ctx *ctx);
+/* Message of the last failed call on `ctx` (empty string if none).  Valid until
+ * the next call on the context.  With ctx == NULL: the last msg_ctx_create error. */
+const char *msg_last_error(const msg_ctx *ctx);
+
+/* Validates and resolves a system description (SpinSpace construction,
+ * SpinSpace.cpp:56-65; InteractionOperator resolution) and keeps its
+ * field-independent operator factors resident on the device. */
+msg_status msg_system_upload(msg_ctx *ctx, const msg_system_desc *desc, msg_system **out);
+msg_status msg_system_release(msg_ctx *ctx, msg_system *sys);
+/* Hilbert dimension of an uploaded system. */
+msg_status msg_system_dimension(const msg_system *sys, int64_t *dim);
+
+/* Device memory (bytes) one propagate call over `n_items` items with `n_columns`
+ * columns each would hold at peak; used for admission by HBM footprint. */
+msg_status msg_query_footprint(const msg_system *sys, const msg_run_params *params,
+                               int32_t n_items, int32_t n_columns, int64_t *bytes);
+
+/* Number of columns DIRECT sampling produces for `sys` (the rank of rho0). */
+msg_status msg_direct_columns(const msg_system *sys, int32_t *n_columns);
+
+/* ------------------------------------------------------------------------- */
+/* Propagation (replaces TaskHSGeneral.cpp:470-576 for a batch of items)    */
+/* ------------------------------------------------------------------------- */
+
+/* Number of observables written per sample for `params` (transition count or
+ * State count). */
+msg_status msg_observable_count(const msg_system *sys, const msg_run_params *params,
+                                int32_t *n_obs);
+/* Number of time samples of the free-evolution grid (BuildFreeEvolutionTimes,
+ * TaskHSGeneral.cpp:59-96): 0, dt, 2dt, ..., T with an exact endpoint. */
+msg_status msg_time_grid(double total_time, double time_step, int64_t *n_samples,
+                         double *times /* may be NULL; else n_samples entries */);
+
+/*
+ * Propagates `n_items` items of one system.
+ *   fields      n_items x 3 field vectors for the field_swept Zeeman interactions
+ *               (NULL: every item uses the fields as described).
+ *   columns     SAMPLING_GIVEN only: D x n_columns complex (MolSpin order), shared
+ *               by all items; scaled as given (HSStatePreparation.cpp:226 scales
+ *               trace samples by 1/sqrt(M) before they arrive here).
+ *   n_columns   SAMPLING_GIVEN only.
+ *   yields      n_items x n_obs (calculation=yields): the integrated observables
+ *               of each item, after the optional finite-time correction.
+ *   trace       n_items x n_samples x n_obs (calculation=timeevolution), or NULL.
+ * Blocks until the results are on the host.
+ */
+msg_status msg_propagate(msg_ctx *ctx, const msg_system *sys, const msg_run_params *params,
+                         int32_t n_items, const double *fields,
+                         const double *columns, int32_t n_columns,
+                         double *yields, double *trace);
+
+/*
+ * Per-step compatibility entry with the meaning of HSPropagator::StepStochastic
+ * (HSPropagator.cpp:131-140): advances `factors` (D x n_columns complex, MolSpin
+ * order, in place) by one interval `dt` under G = -iH - K of the system at
+ * `field` (NULL: as described), with the stochastic relaxation half-steps drawn
+ * from the counter-based RNG at (seed, item 0, step_index).
+ */
+msg_status msg_step(msg_ctx *ctx, const msg_system *sys, const msg_run_params *params,
+                    const double *field, double dt, int64_t step_index,
+                    double *factors, int32_t n_columns);
+
+/* ------------------------------------------------------------------------- */
+/* Serving: continuous batching of yield requests                           */
+/* ------------------------------------------------------------------------- */
+
+typedef struct msg_server msg_server;
+
+/* A server keeps up to `max_slots` requests resident and admits new ones as
+ * slots and the `hbm_budget_bytes` budget free up. */
+msg_status msg_server_create(msg_ctx *ctx, int32_t max_slots, int64_t hbm_budget_bytes,
+                             msg_server **out);
+msg_status msg_server_destroy(msg_server *srv);
+/* Queues a yield request: a system, its run parameters and `n_fields` field
+ * vectors.  Returns a request id.  The system must stay uploaded until done. */
+msg_status msg_server_submit(msg_server *srv, const msg_system *sys, const msg_run_params *params,
+                             int32_t n_fields, const double *fields, int64_t *request_id);
+/* Advances the device work by at most `max_steps` time steps of every resident
+ * request, admitting queued requests first.  Sets *n_active to the number of
+ * requests still queued or running. */
+msg_status msg_server_pump(msg_server *srv, int32_t max_steps, int32_t *n_active);
+/* Copies the yields (n_fields x n_obs) of a finished request and forgets it.
+ * Returns MSG_INVALID_ARG if the request is unknown, *done = 0 if unfinished. */
+msg_status msg_server_collect(msg_server *srv, int64_t request_id, double *yields, int32_t *done);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOLSPIN_GPU_H */
