@@ -232,6 +232,11 @@ msg_status msg_step(msg_ctx *ctx, const msg_system *sys, const msg_run_params *p
                     const double *field, double dt, int64_t step_index,
                     double *factors, int32_t n_columns);
 
+/* Device time (ms, HIP events on the context stream) and launch count of the
+ * propagation kernels of the last msg_propagate call on `ctx`.  Diagnostic
+ * counterpart of MolSpin's per-step wall clock (main.cpp:521-522,651-685). */
+msg_status msg_last_kernel_time(const msg_ctx *ctx, double *ms, int32_t *launches);
+
 /* ------------------------------------------------------------------------- */
 /* Serving: continuous batching of yield requests                           */
 /* ------------------------------------------------------------------------- */

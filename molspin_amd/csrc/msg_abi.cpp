@@ -1,0 +1,805 @@
+// C ABI of the MI355X propagation backend (include/molspin_gpu.h).
+//
+// A msg_propagate call replaces, for a whole batch of items (field
+// orientations / strengths), the orientation and time loops of
+// TaskHSGeneral::RunLocal (TaskHSGeneral.cpp:347-577): observable sampling
+// (HSObservableCollector::Evaluate), HSPropagator::StepStochastic and the
+// trapezoid / finite-time correction (HSObservableCollector.cpp:304-392).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/molspin_gpu.h"
+#include "msg_internal.hpp"
+#include "msg_resolve.hpp"
+
+namespace msg {
+hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_thread, int threads, int slots,
+                          hipStream_t st);
+hipError_t set_lanczos_lds_limits();
+hipError_t launch_dense(const StepArgs &a, const StepArgs *da, int slots, hipStream_t st);
+hipError_t launch_init_norms(const StepArgs &a, const StepArgs *da, hipStream_t st);
+hipError_t launch_zero_int(int *p, int n, hipStream_t st);
+hipError_t launch_reduce_columns(const double *in, double *out, int n_items, int ncols, int nk, int in_stride, int pick,
+                                 hipStream_t st);
+int lanczos_occupancy(int group_per_thread, int threads, size_t lds);
+int dense_occupancy(size_t lds);
+}  // namespace msg
+
+using namespace msg;
+
+struct msg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    std::string err;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double kernel_ms = 0.0;
+    int launches = 0;
+};
+
+struct msg_system {
+    msg_ctx *ctx = nullptr;
+    Resolved r;
+};
+
+namespace {
+
+std::string g_create_error;
+
+msg_status fail(msg_ctx *ctx, msg_status st, const std::string &m) {
+    if (ctx) ctx->err = m;
+    else g_create_error = m;
+    return st;
+}
+
+#define HIPCHECK(call)                                                                              \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            throw DeviceError(std::string(#call) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+struct DeviceError {
+    std::string what;
+    explicit DeviceError(std::string w) : what(std::move(w)) {}
+};
+struct ArgError {
+    msg_status st;
+    std::string what;
+};
+
+// RAII device allocation.
+struct DBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <class T>
+    T *alloc(size_t count) {
+        reset();
+        if (count == 0) return nullptr;
+        HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+        n = count * sizeof(T);
+        return static_cast<T *>(p);
+    }
+    template <class T>
+    T *get() const { return static_cast<T *>(p); }
+};
+
+enum class Path { Lanczos, Dense };
+
+// One propagation batch: all device state of the items of one call or request.
+struct Batch {
+    msg_ctx *ctx = nullptr;
+    const msg_system *sys = nullptr;
+    msg_run_params prm{};
+    Path path = Path::Lanczos;
+    int n_items = 0, ncols = 0, n_units = 0, nobs = 0, nsamples = 0;
+    int G = 1, T = 64, slots = 1;
+    std::vector<double> times, scale;
+    StepArgs a{};
+    DBuf coef, U, dts, wy, wt, thr, yacc, trace, state, basis, queue, status, given, norm0, yout, tout;
+    DBuf obs, evec, nflags, args;
+    int s_next = 0;
+    bool keep_state = false;
+    hipStream_t stream = nullptr;
+    size_t footprint = 0;
+    StepArgs *args_host = nullptr;  // pinned staging of the launch arguments
+    bool timed = false;             // bracket the propagation kernel with the context events
+    Batch() = default;
+    Batch(const Batch &) = delete;
+    ~Batch() {
+        if (args_host) (void)hipHostFree(args_host);
+    }
+};
+
+const int kObsMax = kMaxObs;
+
+void check_params(const msg_system *sys, const msg_run_params *p) {
+    if (!p) throw ArgError{MSG_INVALID_ARG, "null run parameters"};
+    if (p->calculation != MSG_CALC_YIELDS && p->calculation != MSG_CALC_TIMEEVOLUTION)
+        throw ArgError{MSG_INVALID_ARG, "calculation must be yields or timeevolution"};
+    if (p->observables != MSG_OBS_TRANSITION_YIELDS && p->observables != MSG_OBS_STATE_POPULATIONS)
+        throw ArgError{MSG_INVALID_ARG, "unknown observable mode"};
+    if (p->method != MSG_METHOD_AUTO && p->method != MSG_METHOD_KRYLOV && p->method != MSG_METHOD_EXPONENTIAL)
+        throw ArgError{MSG_INVALID_ARG, "propagationmethod must be auto, krylov or exponential on the device"};
+    if (p->krylov_size < 0 || p->krylov_size > kMaxKrylov) throw ArgError{MSG_UNSUPPORTED, "krylovsize above 32"};
+    if (p->sampling != MSG_SAMPLING_DIRECT && p->sampling != MSG_SAMPLING_GIVEN)
+        throw ArgError{MSG_INVALID_ARG, "unknown sampling mode"};
+    if (!sys->r.pair_ok)
+        throw ArgError{MSG_UNSUPPORTED,
+                       "the device paths need exactly two spin-1/2 electrons and States over those electrons"};
+}
+
+int observable_count(const Resolved &r, const msg_run_params *p) {
+    return p->observables == MSG_OBS_TRANSITION_YIELDS ? (int)r.P_trans.size() : (int)r.P_states.size();
+}
+
+Path choose_path(const Resolved &r, const msg_run_params *p) {
+    const bool dense_ok = r.D <= 512;
+    const bool lanczos_ok = r.fast_ok && r.symmetric && r.D >= 256 && r.D <= 8192;
+    if (p->method == MSG_METHOD_EXPONENTIAL) {
+        if (!dense_ok) throw ArgError{MSG_UNSUPPORTED, "propagationmethod=exponential needs D <= 512 on the device"};
+        return Path::Dense;
+    }
+    if (p->method == MSG_METHOD_AUTO && dense_ok) return Path::Dense;
+    if (lanczos_ok) return Path::Lanczos;
+    if (dense_ok) return Path::Dense;  // exact exponential: same result as Krylov within truncation
+    if (!r.symmetric)
+        throw ArgError{MSG_UNSUPPORTED, "unequal recombination rates need D <= 512 (dense exponential path)"};
+    if (!r.fast_ok) throw ArgError{MSG_UNSUPPORTED, "matrix-free path: " + r.why_not_fast};
+    throw ArgError{MSG_UNSUPPORTED, "Hilbert dimension outside the matrix-free range (256..8192)"};
+}
+
+const StepArgs *push_args(Batch &b, const StepArgs &a);
+
+// Builds a batch and uploads everything a launch needs.
+void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params *p, int n_items,
+             const double *fields, const double *columns, int n_columns, bool want_trace, bool keep_state,
+             hipStream_t stream) {
+    const Resolved &r = sys->r;
+    check_params(sys, p);
+    if (n_items < 1) throw ArgError{MSG_INVALID_ARG, "n_items must be positive"};
+    b.ctx = ctx;
+    b.sys = sys;
+    b.prm = *p;
+    b.stream = stream;
+    b.keep_state = keep_state;
+    b.path = choose_path(r, p);
+    b.n_items = n_items;
+    const int D = (int)r.D;
+    // columns
+    if (p->sampling == MSG_SAMPLING_GIVEN) {
+        if (!columns || n_columns < 1) throw ArgError{MSG_INVALID_ARG, "given sampling needs columns"};
+        b.ncols = n_columns;
+    } else {
+        if (r.evec.empty()) throw ArgError{MSG_INVALID_ARG, "the system defines no initial state"};
+        if ((int)r.evec.size() > kMaxInitial) throw ArgError{MSG_UNSUPPORTED, "initial density of rank above 4"};
+        if (r.zdim * (int64_t)r.evec.size() > (1 << 30)) throw ArgError{MSG_UNSUPPORTED, "too many columns"};
+        b.ncols = (int)(r.zdim * (int64_t)r.evec.size());
+    }
+    // observables
+    b.nobs = observable_count(r, p);
+    if (b.nobs < 1) throw ArgError{MSG_INVALID_ARG, "no observables (no transitions / States)"};
+    if (b.nobs > kObsMax) throw ArgError{MSG_UNSUPPORTED, "more than 8 observables"};
+    const std::vector<Mat4> &P = p->observables == MSG_OBS_TRANSITION_YIELDS ? r.P_trans : r.P_states;
+    b.scale.assign(b.nobs, 1.0);
+    if (p->observables == MSG_OBS_TRANSITION_YIELDS) b.scale = r.rates;
+    // time grid
+    const std::string te = free_times(p->total_time, p->time_step, b.times);
+    if (!te.empty()) throw ArgError{MSG_INVALID_ARG, te};
+    b.nsamples = (int)b.times.size();
+    const int ns = b.nsamples;
+    std::vector<double> dts(std::max(ns - 1, 1), 0.0), wyh((size_t)ns * b.nobs), wth((size_t)ns * b.nobs);
+    std::vector<uint32_t> thr(std::max(ns - 1, 1), 0u);
+    const bool factor_decay = r.symmetric;
+    for (int s = 0; s + 1 < ns; ++s) {
+        dts[s] = b.times[s + 1] - b.times[s];
+        if (r.relax) {
+            const double pr = -0.5 * std::expm1(-2.0 * (0.5 * r.relax_rate) * (0.5 * dts[s]));
+            thr[s] = (uint32_t)std::floor(pr * 4294967296.0);
+        }
+    }
+    for (int s = 0; s < ns; ++s) {
+        double w = 0.0;
+        if (s > 0) w += 0.5 * (b.times[s] - b.times[s - 1]);
+        if (s + 1 < ns) w += 0.5 * (b.times[s + 1] - b.times[s]);
+        const double decay = factor_decay ? std::exp(-2.0 * r.kappa * b.times[s]) : 1.0;
+        for (int c = 0; c < b.nobs; ++c) {
+            wyh[(size_t)s * b.nobs + c] = w * decay * b.scale[c];
+            wth[(size_t)s * b.nobs + c] = decay * b.scale[c];
+        }
+    }
+    StepArgs &a = b.a;
+    a = StepArgs();
+    a.dim = D;
+    a.nobs = b.nobs;
+    a.ncols = b.ncols;
+    a.nsamples = ns;
+    {
+        std::vector<float> ob((size_t)32 * b.nobs);
+        for (int c = 0; c < b.nobs; ++c)
+            for (int i = 0; i < 16; ++i) {
+                ob[32 * c + 2 * i] = (float)P[c][i].real();
+                ob[32 * c + 2 * i + 1] = (float)P[c][i].imag();
+            }
+        float *d = b.obs.alloc<float>(ob.size());
+        HIPCHECK(hipMemcpyAsync(d, ob.data(), ob.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        a.obs = d;
+    }
+    a.relax = r.relax ? 1 : 0;
+    a.seed = p->seed;
+    a.m = p->krylov_size > 0 ? p->krylov_size : 16;
+    a.renorm = 0;
+    // initial states
+    if (p->sampling == MSG_SAMPLING_GIVEN) {
+        a.init_mode = 1;
+        std::vector<float> g((size_t)n_columns * 2 * D);
+        for (int j = 0; j < n_columns; ++j)
+            for (int i = 0; i < D; ++i) {
+                const int64_t dev = r.perm[i];
+                const int64_t nu = dev >> 2, c = dev & 3;
+                const size_t off = (size_t)j * 2 * D + (c < 2 ? 4 * nu + 2 * c : D + 4 * nu + 2 * (c - 2));
+                g[off] = (float)columns[2 * ((size_t)j * D + i)];
+                g[off + 1] = (float)columns[2 * ((size_t)j * D + i) + 1];
+            }
+        float *dg = b.given.alloc<float>(g.size());
+        HIPCHECK(hipMemcpyAsync(dg, g.data(), g.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        a.given = dg;
+    } else {
+        a.init_mode = 0;
+        a.n_init = (int)r.evec.size();
+        a.zdim = (int)r.zdim;
+        const double inv = 1.0 / std::sqrt((double)r.zdim);
+        std::vector<float> ev((size_t)8 * a.n_init);
+        for (int i = 0; i < a.n_init; ++i)
+            for (int c = 0; c < 4; ++c) {
+                ev[i * 8 + 2 * c] = (float)(r.evec[i][c].real() * inv);
+                ev[i * 8 + 2 * c + 1] = (float)(r.evec[i][c].imag() * inv);
+            }
+        float *d = b.evec.alloc<float>(ev.size());
+        HIPCHECK(hipMemcpyAsync(d, ev.data(), ev.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        a.evec = d;
+    }
+    // operator
+    if (b.path == Path::Lanczos) {
+        const int ng = D / 4;
+        if (ng <= 1024) { b.T = std::min(ng, 256); b.G = ng / b.T; }
+        else { b.G = 2; b.T = ng / 2; }
+        if (b.T < 64 || b.T > 1024 || b.G * b.T != ng || (b.G != 1 && b.G != 2 && b.G != 4))
+            throw ArgError{MSG_UNSUPPORTED, "no matrix-free launch shape for this dimension"};
+        a.nnuc = (int)r.nuc.size();
+        std::vector<float> all, one;
+        int flags = 0;
+        std::vector<int> nfl(kMaxNuclei, 0);
+        for (int it = 0; it < n_items; ++it) {
+            int nf[kMaxNuclei];
+            fast_coefficients(r, fields ? fields + 3 * it : nullptr, one, flags, nf);
+            if (it == 0) {
+                a.coef_stride = (int)((one.size() + 15) / 16 * 16);
+                all.assign((size_t)n_items * a.coef_stride, 0.f);
+            }
+            for (int k = 0; k < kMaxNuclei; ++k) nfl[k] |= nf[k];
+            a.op_flags |= flags;
+            std::copy(one.begin(), one.end(), all.begin() + (size_t)it * a.coef_stride);
+        }
+        int *dfl = b.nflags.alloc<int>(nfl.size());
+        HIPCHECK(hipMemcpyAsync(dfl, nfl.data(), nfl.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+        a.nuc_flags = dfl;
+        float *dc = b.coef.alloc<float>(all.size());
+        HIPCHECK(hipMemcpyAsync(dc, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        a.coef = dc;
+        b.n_units = n_items * b.ncols;
+        a.renorm = r.symmetric ? 1 : 0;
+    } else {
+        const int Dp = (D + 15) / 16 * 16;
+        if (Dp > 512) throw ArgError{MSG_UNSUPPORTED, "dense path limited to D <= 512"};
+        a.dpad = Dp;
+        const bool two = ns >= 2 && std::fabs(dts[ns - 2] - dts[0]) > 0.0;
+        a.ulast = two ? 1 : 0;
+        const int per = two ? 2 : 1;
+        std::vector<float> Uh((size_t)n_items * per * 2 * Dp * Dp, 0.f);
+        std::vector<cd> H, Aexp, E;
+        for (int it = 0; it < n_items; ++it) {
+            dense_hamiltonian(r, fields ? fields + 3 * it : nullptr, H);
+            if (!r.symmetric)  // H_eff = H - i K (x) 1 on the electron pair
+                for (int64_t nu = 0; nu < D / 4; ++nu)
+                    for (int i = 0; i < 4; ++i)
+                        for (int j = 0; j < 4; ++j)
+                            H[(size_t)(nu * 4 + i) * D + nu * 4 + j] += cd(0.0, -1.0) * r.K[i * 4 + j];
+            for (int w = 0; w < per; ++w) {
+                const double dt = w == 0 ? dts[0] : dts[ns - 2];
+                Aexp.resize(H.size());
+                for (size_t i = 0; i < H.size(); ++i) Aexp[i] = cd(0.0, -dt) * H[i];
+                expm_dense(D, Aexp, E);
+                float *dst = &Uh[((size_t)it * per + w) * 2 * Dp * Dp];
+                // planes re, im; column-major U[k][row]
+                for (int row = 0; row < D; ++row)
+                    for (int k = 0; k < D; ++k) {
+                        dst[(size_t)k * Dp + row] = (float)E[(size_t)row * D + k].real();
+                        dst[(size_t)Dp * Dp + (size_t)k * Dp + row] = (float)E[(size_t)row * D + k].imag();
+                    }
+            }
+        }
+        float *dU = b.U.alloc<float>(Uh.size());
+        HIPCHECK(hipMemcpyAsync(dU, Uh.data(), Uh.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        a.U = dU;
+        a.ustride = per;
+        b.n_units = n_items * ((b.ncols + 15) / 16);
+    }
+    a.n_units = b.n_units;
+    // device arrays
+    double *ddts = b.dts.alloc<double>(dts.size());
+    HIPCHECK(hipMemcpyAsync(ddts, dts.data(), dts.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    a.dts = ddts;
+    double *dwy = b.wy.alloc<double>(wyh.size());
+    HIPCHECK(hipMemcpyAsync(dwy, wyh.data(), wyh.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    a.wy = dwy;
+    double *dwt = b.wt.alloc<double>(wth.size());
+    HIPCHECK(hipMemcpyAsync(dwt, wth.data(), wth.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    a.wt = dwt;
+    uint32_t *dthr = b.thr.alloc<uint32_t>(thr.size());
+    HIPCHECK(hipMemcpyAsync(dthr, thr.data(), thr.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    a.thr = dthr;
+    a.yacc = b.yacc.alloc<double>((size_t)b.n_units * b.nobs * 2);
+    a.trace = want_trace ? b.trace.alloc<double>((size_t)b.n_units * ns * b.nobs) : nullptr;
+    a.queue = b.queue.alloc<int>(1);
+    a.status = b.status.alloc<int>(1);
+    HIPCHECK(hipMemsetAsync(a.status, 0, sizeof(int), stream));
+    if (keep_state) {
+        const size_t per_unit = b.path == Path::Lanczos ? (size_t)2 * D : (size_t)2 * a.dpad * 16;
+        a.state = b.state.alloc<float>((size_t)b.n_units * per_unit);
+    }
+    // launch geometry
+    if (b.path == Path::Lanczos) {
+        const size_t lds = (size_t)8 * D + 4096;
+        const int occ = std::max(1, lanczos_occupancy(b.G, b.T, lds));
+        b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ));
+        a.basis = b.basis.alloc<float>((size_t)b.slots * a.m * 2 * D);
+        a.norm0 = b.norm0.alloc<double>(b.n_units);
+        if (a.renorm) HIPCHECK(launch_init_norms(a, push_args(b, a), stream));
+    } else {
+        const size_t lds = (size_t)8 * a.dpad * 16 + 4096;
+        const int occ = std::max(1, dense_occupancy(lds));
+        b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ));
+    }
+    b.footprint = b.coef.n + b.U.n + b.dts.n + b.wy.n + b.wt.n + b.thr.n + b.yacc.n + b.trace.n + b.state.n +
+                  b.basis.n + b.given.n + b.norm0.n;
+    b.s_next = 0;
+}
+
+// Copies the launch arguments to the device (ordered on the batch stream).
+const StepArgs *push_args(Batch &b, const StepArgs &a) {
+    if (!b.args.p) b.args.alloc<StepArgs>(1);
+    // staged through pinned memory owned by the batch so the async copy sees a stable source
+    if (!b.args_host) HIPCHECK(hipHostMalloc((void **)&b.args_host, sizeof(StepArgs), hipHostMallocDefault));
+    HIPCHECK(hipStreamSynchronize(b.stream));  // the previous launch has read its arguments
+    *b.args_host = a;
+    HIPCHECK(hipMemcpyAsync(b.args.p, b.args_host, sizeof(StepArgs), hipMemcpyHostToDevice, b.stream));
+    return b.args.get<StepArgs>();
+}
+
+// Runs intervals [s_next, min(s_next + steps, nsamples-1)).
+void run_chunk(Batch &b, int steps) {
+    StepArgs a = b.a;
+    const int last = b.nsamples - 1;
+    a.s_begin = b.s_next;
+    a.s_end = std::min(last, b.s_next + steps);
+    if (b.s_next > 0) {
+        if (!b.keep_state) throw DeviceError("chunked run without a state buffer");
+        a.init_mode = 2;
+    }
+    HIPCHECK(launch_zero_int(a.queue, 1, b.stream));
+    const StepArgs *da = push_args(b, a);
+    if (b.timed) HIPCHECK(hipEventRecord(b.ctx->ev0, b.stream));
+    if (b.path == Path::Lanczos) HIPCHECK(launch_lanczos(a, da, b.G, b.T, b.slots, b.stream));
+    else HIPCHECK(launch_dense(a, da, b.slots, b.stream));
+    if (b.timed) HIPCHECK(hipEventRecord(b.ctx->ev1, b.stream));
+    b.s_next = a.s_end;
+}
+
+// Copies results; per-item sums over columns in fixed order.
+void collect(Batch &b, double *yields, double *trace) {
+    const int cols_per_item = b.n_units / b.n_items;
+    if (yields) {
+        double *out = b.yout.alloc<double>((size_t)b.n_items * b.nobs);
+        HIPCHECK(launch_reduce_columns(b.a.yacc, out, b.n_items, cols_per_item, b.nobs, b.nobs * 2, 2, b.stream));
+        HIPCHECK(hipMemcpyAsync(yields, out, (size_t)b.n_items * b.nobs * sizeof(double), hipMemcpyDeviceToHost, b.stream));
+    }
+    if (trace && b.a.trace) {
+        const int nk = b.nsamples * b.nobs;
+        double *out = b.tout.alloc<double>((size_t)b.n_items * nk);
+        HIPCHECK(launch_reduce_columns(b.a.trace, out, b.n_items, cols_per_item, nk, nk, 1, b.stream));
+        HIPCHECK(hipMemcpyAsync(trace, out, (size_t)b.n_items * nk * sizeof(double), hipMemcpyDeviceToHost, b.stream));
+    }
+    int st = 0;
+    HIPCHECK(hipMemcpyAsync(&st, b.a.status, sizeof(int), hipMemcpyDeviceToHost, b.stream));
+    HIPCHECK(hipStreamSynchronize(b.stream));
+    if (st & 1) throw ArgError{MSG_NUMERIC_BREAKDOWN, "non-finite state during propagation"};
+    if (st & 2) throw ArgError{MSG_NUMERIC_BREAKDOWN, "time step too large for the Krylov exponential"};
+    if (yields && b.prm.calculation == MSG_CALC_YIELDS && b.prm.yield_corrections) {
+        const double den = 1.0 - std::exp(-b.prm.total_time * b.sys->r.kmax);
+        if (!(den > 0.0) || !std::isfinite(den))
+            throw ArgError{MSG_INVALID_ARG, "yieldcorrections=true needs a positive total time and rate"};
+        for (size_t i = 0; i < (size_t)b.n_items * b.nobs; ++i) yields[i] /= den;
+    }
+}
+
+template <class F>
+msg_status guarded(msg_ctx *ctx, F f) {
+    try {
+        f();
+        if (ctx) ctx->err.clear();
+        return MSG_OK;
+    } catch (const ArgError &e) {
+        return fail(ctx, e.st, e.what);
+    } catch (const DeviceError &e) {
+        return fail(ctx, MSG_DEVICE_ERROR, e.what);
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, MSG_OOM_ADMISSION, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(ctx, MSG_DEVICE_ERROR, e.what());
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Server (continuous batching of yield requests)
+// ---------------------------------------------------------------------------
+struct Request {
+    int64_t id = 0;
+    const msg_system *sys = nullptr;
+    msg_run_params prm{};
+    std::vector<double> fields;
+    int n_fields = 0;
+    std::unique_ptr<Batch> batch;
+    std::vector<double> result;
+    bool done = false;
+    bool failed = false;
+    std::string error;
+    int64_t bytes = 0;
+};
+
+struct msg_server {
+    msg_ctx *ctx = nullptr;
+    int max_slots = 0;
+    int64_t budget = 0, in_use = 0;
+    int64_t next_id = 1;
+    std::deque<std::unique_ptr<Request>> queued;
+    std::vector<std::unique_ptr<Request>> active;
+    std::map<int64_t, std::unique_ptr<Request>> finished;
+    std::vector<hipStream_t> streams;
+};
+
+extern "C" {
+
+const char *msg_last_error(const msg_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+msg_status msg_ctx_create(int device, msg_ctx **out) {
+    if (!out) return fail(nullptr, MSG_INVALID_ARG, "null output pointer");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return fail(nullptr, MSG_DEVICE_ERROR, "no HIP device");
+    if (device < 0 || device >= n) return fail(nullptr, MSG_INVALID_ARG, "device index out of range");
+    std::unique_ptr<msg_ctx> c(new (std::nothrow) msg_ctx());
+    if (!c) return fail(nullptr, MSG_OOM_ADMISSION, "host allocation failed");
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) return fail(nullptr, MSG_DEVICE_ERROR, "hipSetDevice failed");
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(nullptr, MSG_DEVICE_ERROR, "hipStreamCreate failed");
+    if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c->cus < 1)
+        c->cus = 1;
+    if (set_lanczos_lds_limits() != hipSuccess) return fail(nullptr, MSG_DEVICE_ERROR, "cannot raise the LDS limit");
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return fail(nullptr, MSG_DEVICE_ERROR, "hipEventCreate failed");
+    *out = c.release();
+    return MSG_OK;
+}
+
+msg_status msg_ctx_destroy(msg_ctx *ctx) {
+    if (!ctx) return MSG_INVALID_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    delete ctx;
+    return MSG_OK;
+}
+
+msg_status msg_system_upload(msg_ctx *ctx, const msg_system_desc *desc, msg_system **out) {
+    if (!ctx || !desc || !out) return fail(ctx, MSG_INVALID_ARG, "null argument");
+    return guarded(ctx, [&] {
+        std::unique_ptr<msg_system> s(new msg_system());
+        s->ctx = ctx;
+        const std::string e = resolve_system(desc, s->r);
+        if (!e.empty()) throw ArgError{MSG_INVALID_ARG, e};
+        *out = s.release();
+    });
+}
+
+msg_status msg_system_release(msg_ctx *ctx, msg_system *sys) {
+    if (!sys) return fail(ctx, MSG_INVALID_ARG, "null system");
+    delete sys;
+    return MSG_OK;
+}
+
+msg_status msg_system_dimension(const msg_system *sys, int64_t *dim) {
+    if (!sys || !dim) return MSG_INVALID_ARG;
+    *dim = sys->r.D;
+    return MSG_OK;
+}
+
+msg_status msg_direct_columns(const msg_system *sys, int32_t *n) {
+    if (!sys || !n) return MSG_INVALID_ARG;
+    *n = (int32_t)(sys->r.zdim * (int64_t)sys->r.evec.size());
+    return MSG_OK;
+}
+
+msg_status msg_observable_count(const msg_system *sys, const msg_run_params *p, int32_t *n) {
+    if (!sys || !p || !n) return MSG_INVALID_ARG;
+    *n = observable_count(sys->r, p);
+    return MSG_OK;
+}
+
+msg_status msg_time_grid(double total, double dt, int64_t *n, double *times) {
+    if (!n) return MSG_INVALID_ARG;
+    std::vector<double> t;
+    if (!free_times(total, dt, t).empty()) return MSG_INVALID_ARG;
+    *n = (int64_t)t.size();
+    if (times) std::copy(t.begin(), t.end(), times);
+    return MSG_OK;
+}
+
+msg_status msg_query_footprint(const msg_system *sys, const msg_run_params *p, int32_t n_items, int32_t n_columns,
+                               int64_t *bytes) {
+    if (!sys || !p || !bytes || n_items < 1) return MSG_INVALID_ARG;
+    const Resolved &r = sys->r;
+    std::vector<double> t;
+    if (!free_times(p->total_time, p->time_step, t).empty()) return MSG_INVALID_ARG;
+    const int ncols = n_columns > 0 ? n_columns : (int)(r.zdim * (int64_t)std::max<size_t>(1, r.evec.size()));
+    const int nobs = std::max(1, observable_count(r, p));
+    const int64_t D = r.D;
+    int64_t b = (int64_t)t.size() * nobs * 16 + (int64_t)t.size() * 12;
+    Path path;
+    try {
+        path = choose_path(r, p);
+    } catch (...) {
+        return MSG_UNSUPPORTED;
+    }
+    const int m = p->krylov_size > 0 ? p->krylov_size : 16;
+    if (path == Path::Lanczos) {
+        const int64_t units = (int64_t)n_items * ncols;
+        b += (int64_t)n_items * (kCoefHead + (int64_t)r.nuc.size() * kNucStride) * 4;
+        b += units * (nobs * 16 + 8);
+        b += std::min<int64_t>(units, 2048) * m * 8 * D;  // Krylov bases of the resident workgroups
+    } else {
+        const int64_t Dp = (D + 15) / 16 * 16;
+        const int64_t units = (int64_t)n_items * ((ncols + 15) / 16);
+        b += (int64_t)n_items * 2 * 8 * Dp * Dp;
+        b += units * nobs * 16;
+    }
+    if (p->sampling == MSG_SAMPLING_GIVEN) b += (int64_t)ncols * 8 * D;
+    *bytes = b;
+    return MSG_OK;
+}
+
+msg_status msg_propagate(msg_ctx *ctx, const msg_system *sys, const msg_run_params *params, int32_t n_items,
+                         const double *fields, const double *columns, int32_t n_columns, double *yields,
+                         double *trace) {
+    if (!ctx || !sys) return fail(ctx, MSG_INVALID_ARG, "null context or system");
+    return guarded(ctx, [&] {
+        HIPCHECK(hipSetDevice(ctx->device));
+        Batch b;
+        prepare(b, ctx, sys, params, n_items, fields, columns, n_columns, trace != nullptr, false, ctx->stream);
+        b.timed = true;
+        run_chunk(b, b.nsamples);
+        collect(b, yields, trace);
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->kernel_ms = ms;
+        ctx->launches = 1;
+    });
+}
+
+msg_status msg_step(msg_ctx *ctx, const msg_system *sys, const msg_run_params *params, const double *field, double dt,
+                    int64_t step_index, double *factors, int32_t n_columns) {
+    if (!ctx || !sys || !params || !factors || n_columns < 1) return fail(ctx, MSG_INVALID_ARG, "null argument");
+    return guarded(ctx, [&] {
+        HIPCHECK(hipSetDevice(ctx->device));
+        // one interval [0, dt] with the relaxation draws of step `step_index`
+        msg_run_params p = *params;
+        p.sampling = MSG_SAMPLING_GIVEN;
+        p.calculation = MSG_CALC_TIMEEVOLUTION;
+        p.total_time = dt;
+        p.time_step = dt;
+        p.yield_corrections = 0;
+        Batch b;
+        prepare(b, ctx, sys, &p, 1, field, factors, n_columns, false, true, ctx->stream);
+        if (b.nsamples != 2) throw ArgError{MSG_INVALID_ARG, "dt must be positive"};
+        if (step_index < 0 || step_index > 0x7fffffff) throw ArgError{MSG_INVALID_ARG, "step index out of range"};
+        if (step_index < 0 || step_index > 0x7fffffff) throw ArgError{MSG_INVALID_ARG, "step index out of range"};
+        b.a.rng_step0 = (int)step_index;  // relaxation draws of interval `step_index`
+        run_chunk(b, 1);
+        // read the state back (device order) into MolSpin order
+        const Resolved &r = sys->r;
+        const int D = (int)r.D;
+        std::vector<float> st(b.state.n / sizeof(float));
+        HIPCHECK(hipMemcpyAsync(st.data(), b.a.state, b.state.n, hipMemcpyDeviceToHost, b.stream));
+        int status = 0;
+        HIPCHECK(hipMemcpyAsync(&status, b.a.status, sizeof(int), hipMemcpyDeviceToHost, b.stream));
+        HIPCHECK(hipStreamSynchronize(b.stream));
+        if (status) throw ArgError{MSG_NUMERIC_BREAKDOWN, "non-finite state during the step"};
+        // the Lanczos path propagates the unitary part; apply the symmetric decay of K
+        const double decay = r.symmetric ? std::exp(-r.kappa * dt) : 1.0;
+        for (int j = 0; j < n_columns; ++j)
+            for (int i = 0; i < D; ++i) {
+                const int64_t dev = r.perm[i];
+                const int64_t nu = dev >> 2, c = dev & 3;
+                float re, im;
+                if (b.path == Path::Lanczos) {
+                    const size_t off = (size_t)j * 2 * D + (c < 2 ? 4 * nu + 2 * c : D + 4 * nu + 2 * (c - 2));
+                    re = st[off];
+                    im = st[off + 1];
+                } else {
+                    const int Dp = b.a.dpad, tile = j / 16, tc = j % 16;
+                    const float *u = &st[(size_t)tile * 2 * Dp * 16];
+                    re = u[dev * 16 + tc];
+                    im = u[Dp * 16 + dev * 16 + tc];
+                }
+                factors[2 * ((size_t)j * D + i)] = decay * re;
+                factors[2 * ((size_t)j * D + i) + 1] = decay * im;
+            }
+    });
+}
+
+msg_status msg_last_kernel_time(const msg_ctx *ctx, double *ms, int32_t *launches) {
+    if (!ctx || !ms) return MSG_INVALID_ARG;
+    *ms = ctx->kernel_ms;
+    if (launches) *launches = ctx->launches;
+    return MSG_OK;
+}
+
+msg_status msg_server_create(msg_ctx *ctx, int32_t max_slots, int64_t hbm_budget_bytes, msg_server **out) {
+    if (!ctx || !out || max_slots < 1 || hbm_budget_bytes < 1) return fail(ctx, MSG_INVALID_ARG, "bad server arguments");
+    return guarded(ctx, [&] {
+        std::unique_ptr<msg_server> s(new msg_server());
+        s->ctx = ctx;
+        s->max_slots = max_slots;
+        s->budget = hbm_budget_bytes;
+        const int ns = std::min(max_slots, 4);
+        for (int i = 0; i < ns; ++i) {
+            hipStream_t st;
+            HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            s->streams.push_back(st);
+        }
+        *out = s.release();
+    });
+}
+
+msg_status msg_server_destroy(msg_server *srv) {
+    if (!srv) return MSG_INVALID_ARG;
+    for (hipStream_t st : srv->streams) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
+    srv->active.clear();
+    delete srv;
+    return MSG_OK;
+}
+
+msg_status msg_server_submit(msg_server *srv, const msg_system *sys, const msg_run_params *params, int32_t n_fields,
+                             const double *fields, int64_t *request_id) {
+    if (!srv || !sys || !params || n_fields < 1 || !fields || !request_id)
+        return fail(srv ? srv->ctx : nullptr, MSG_INVALID_ARG, "bad request");
+    return guarded(srv->ctx, [&] {
+        check_params(sys, params);
+        if (params->calculation != MSG_CALC_YIELDS) throw ArgError{MSG_INVALID_ARG, "the server serves yield requests"};
+        if (params->sampling != MSG_SAMPLING_DIRECT) throw ArgError{MSG_UNSUPPORTED, "the server uses direct sampling"};
+        std::unique_ptr<Request> q(new Request());
+        q->id = srv->next_id++;
+        q->sys = sys;
+        q->prm = *params;
+        q->n_fields = n_fields;
+        q->fields.assign(fields, fields + 3 * (size_t)n_fields);
+        int64_t bytes = 0;
+        if (msg_query_footprint(sys, params, n_fields, 0, &bytes) != MSG_OK)
+            throw ArgError{MSG_UNSUPPORTED, "request not representable on the device"};
+        q->bytes = bytes;
+        if (bytes > srv->budget) throw ArgError{MSG_OOM_ADMISSION, "request exceeds the HBM budget"};
+        *request_id = q->id;
+        srv->queued.push_back(std::move(q));
+    });
+}
+
+msg_status msg_server_pump(msg_server *srv, int32_t max_steps, int32_t *n_active) {
+    if (!srv || max_steps < 1) return MSG_INVALID_ARG;
+    return guarded(srv->ctx, [&] {
+        msg_ctx *ctx = srv->ctx;
+        HIPCHECK(hipSetDevice(ctx->device));
+        // admission by slot count and HBM footprint, first come first served
+        while (!srv->queued.empty() && (int)srv->active.size() < srv->max_slots &&
+               srv->in_use + srv->queued.front()->bytes <= srv->budget) {
+            std::unique_ptr<Request> q = std::move(srv->queued.front());
+            srv->queued.pop_front();
+            q->batch.reset(new Batch());
+            const hipStream_t st = srv->streams[q->id % srv->streams.size()];
+            try {
+                prepare(*q->batch, ctx, q->sys, &q->prm, q->n_fields, q->fields.data(), nullptr, 0, false, true, st);
+            } catch (const ArgError &e) {
+                q->failed = true;
+                q->error = e.what;
+                q->done = true;
+                q->batch.reset();
+                srv->finished[q->id] = std::move(q);
+                continue;
+            }
+            srv->in_use += q->bytes;
+            srv->active.push_back(std::move(q));
+        }
+        // one chunk of every resident request, overlapping on the streams
+        for (auto &q : srv->active) run_chunk(*q->batch, max_steps);
+        for (auto it = srv->active.begin(); it != srv->active.end();) {
+            Request &q = **it;
+            if (q.batch->s_next >= q.batch->nsamples - 1) {
+                // the last chunk observed the final sample; collect
+                q.result.assign((size_t)q.n_fields * q.batch->nobs, 0.0);
+                try {
+                    collect(*q.batch, q.result.data(), nullptr);
+                } catch (const ArgError &e) {
+                    q.failed = true;
+                    q.error = e.what;
+                }
+                q.done = true;
+                srv->in_use -= q.bytes;
+                q.batch.reset();
+                srv->finished[q.id] = std::move(*it);
+                it = srv->active.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        if (n_active) *n_active = (int32_t)(srv->queued.size() + srv->active.size());
+    });
+}
+
+msg_status msg_server_collect(msg_server *srv, int64_t request_id, double *yields, int32_t *done) {
+    if (!srv || !done) return MSG_INVALID_ARG;
+    auto it = srv->finished.find(request_id);
+    if (it == srv->finished.end()) {
+        for (auto &q : srv->active)
+            if (q->id == request_id) { *done = 0; return MSG_OK; }
+        for (auto &q : srv->queued)
+            if (q->id == request_id) { *done = 0; return MSG_OK; }
+        return fail(srv->ctx, MSG_INVALID_ARG, "unknown request id");
+    }
+    Request &q = *it->second;
+    if (q.failed) {
+        const std::string e = q.error;
+        srv->finished.erase(it);
+        return fail(srv->ctx, MSG_NUMERIC_BREAKDOWN, e);
+    }
+    if (yields) std::copy(q.result.begin(), q.result.end(), yields);
+    *done = 1;
+    srv->finished.erase(it);
+    return MSG_OK;
+}
+
+}  // extern "C"

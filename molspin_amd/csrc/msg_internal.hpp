@@ -1,0 +1,81 @@
+// Shared host/device definitions of the MI355X propagation backend.
+//
+// Device basis order ("group-major"): index = nu * 4 + c, where c = 2*a + b is
+// the electron-pair index (a, b = 0 for spin up, 1 for down, electron 1 first)
+// and nu enumerates the nuclear configuration, nucleus 0 least significant
+// (mixed radix; for spin-1/2 nuclei bit k of nu is nucleus k, 0 = up).
+// A state column of the matrix-free path is stored as two planes of float4:
+//   plane A[nu] = (re c0, im c0, re c1, im c1), plane B[nu] = (re c2, im c2, re c3, im c3),
+// so one group is 32 bytes and a column of dimension D occupies 2*D floats.
+#pragma once
+#include <stdint.h>
+
+namespace msg {
+
+constexpr int kMaxKrylov = 32;
+constexpr int kMaxObs = 8;
+constexpr int kMaxNuclei = 16;     // matrix-free path: D = 4 * 2^N <= 262144
+constexpr int kMaxInitial = 4;
+constexpr int kChebMax = 96;
+
+// Per-item coefficient block of the matrix-free operator (floats):
+//   [0..2]  w1: field on electron 1 (H += w1 . S1), [3..5] w2: field on electron 2
+//   [6]     scalar shift (identity), [7] pad
+//   [8..39] E: 4x4 complex electron-pair matrix (row-major, interleaved), present if flag
+//   then per nucleus n (kNucStride floats):
+//     [0..2] h1, [3..5] h2, [6] r         diagonal: s_n (h1.S1 + h2.S2 + r), s_n = +1 up / -1 down
+//     [7..14] C1 (2x2 complex on e1), [15..22] C2 (2x2 complex on e2), [23..24] t (complex)
+//     off-diagonal <up_n| H |down_n> = C1 (x) 1 + 1 (x) C2 + t 1 on the electron pair
+constexpr int kCoefHead = 40;
+constexpr int kNucStride = 28;
+enum NucFlags { NUC_E1 = 1, NUC_E2 = 2, NUC_T = 4 };
+enum OpFlags { OP_EREST = 1 };
+
+struct StepArgs {
+    // geometry
+    int nnuc;          // number of spin-1/2 nuclei (matrix-free path)
+    int dim;           // Hilbert dimension D
+    int m;             // Krylov order
+    int nobs;
+    int n_units;       // (item, column) pairs
+    int ncols;         // columns per item
+    int op_flags;
+    const int *nuc_flags;  // [nnuc] device
+    const float *coef; // [n_items][coef_stride]
+    int coef_stride;
+    // dense path
+    const float *U;    // [n_items][ustride][2][Dp*Dp] planes re, im, column-major (Dp = padded dim)
+    int dpad;
+    int ustride;       // propagators per item (2 when the last interval is shorter)
+    int ulast;         // use the item's second propagator for interval nsamples-2
+    // observables: electron-pair 4x4 complex matrices (row-major interleaved), device
+    const float *obs;        // [nobs][32]
+    // time grid
+    int nsamples;
+    int s_begin, s_end;      // intervals processed by this launch
+    const double *dts;       // [nsamples-1]
+    const double *wy;        // [nsamples*nobs] yield weights
+    const double *wt;        // [nsamples*nobs] trace factors (NULL: no trace)
+    double *yacc;            // [n_units][nobs][2] Kahan sum / compensation
+    double *trace;           // [n_units][nsamples][nobs] or NULL
+    // relaxation (singlet-triplet dephasing of the electron pair)
+    int relax;
+    uint64_t seed;
+    const uint32_t *thr;     // [nsamples-1] flip thresholds of one half-step
+    int rng_step0;           // RNG counter of interval s is s + rng_step0
+    // state source / sink
+    int init_mode;           // 0 direct (generated), 1 given columns, 2 resume from state
+    int n_init;              // direct: number of electron vectors
+    const float *evec;       // [n_init][8] device
+    int zdim;                // direct: columns per electron vector (nuclear dimension)
+    const float *given;      // [ncols][2D] device order
+    float *state;            // [n_units][2D] saved state (NULL: not saved)
+    double *norm0;           // [n_units] norms for renormalisation (NULL: none)
+    int renorm;              // 1: rescale the Krylov result to norm0 (unitary part)
+    // scheduling
+    int *queue;              // work counter (zeroed before launch)
+    float *basis;            // [slots][m][2D]
+    int *status;             // device error flags (bit 0: non-finite, bit 1: dt too large)
+};
+
+}  // namespace msg

@@ -1,0 +1,823 @@
+// HIP kernels of the MI355X (gfx950) Hilbert-space propagation backend.
+//
+//   k_lanczos<G>   matrix-free Krylov propagation of radical-pair states with
+//                  spin-1/2 nuclei.  One workgroup owns one state column for a
+//                  whole range of time steps (persistent, work queue over
+//                  (item, column) units).  The Krylov vector lives in LDS as two
+//                  float4 planes; each thread owns G electron-pair groups
+//                  nu = g*T + t; the operator is applied matrix-free from
+//                  per-item Kronecker factors (electron fields, per-nucleus 2x2
+//                  blocks); inner products, the tridiagonal exponential and the
+//                  yield accumulation run in FP64.
+//   k_dense        D <= 512: U = exp(-i H dt) applied as a complex GEMM on the
+//                  FP32 matrix cores (v_mfma_f32_16x16x4f32), one workgroup per
+//                  (item, 16-column tile), persistent over the time steps.
+//   k_reduce_*     fixed-order reductions of per-column results.
+//
+// Replaces the serial loops of TaskHSGeneral.cpp:470-576 (observable sampling,
+// HSPropagator::StepStochastic HSPropagator.cpp:131-140, KrylovExpmGeneral
+// SpinSpace_operators.cpp:1257-1408) for whole batches of items.
+#include <hip/hip_runtime.h>
+
+#include "msg_internal.hpp"
+
+namespace msg {
+
+struct C4 {
+    float2 c[4];
+};
+
+__device__ __forceinline__ float2 cfma(float2 a, float2 b, float2 acc) {
+    acc.x = fmaf(a.x, b.x, fmaf(-a.y, b.y, acc.x));
+    acc.y = fmaf(a.x, b.y, fmaf(a.y, b.x, acc.y));
+    return acc;
+}
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+__device__ __forceinline__ C4 load_group(const float4 *A, const float4 *B, int nu) {
+    const float4 p = A[nu], q = B[nu];
+    C4 r;
+    r.c[0] = make_float2(p.x, p.y);
+    r.c[1] = make_float2(p.z, p.w);
+    r.c[2] = make_float2(q.x, q.y);
+    r.c[3] = make_float2(q.z, q.w);
+    return r;
+}
+__device__ __forceinline__ void store_group(float4 *A, float4 *B, int nu, const C4 &x) {
+    A[nu] = make_float4(x.c[0].x, x.c[0].y, x.c[1].x, x.c[1].y);
+    B[nu] = make_float4(x.c[2].x, x.c[2].y, x.c[3].x, x.c[3].y);
+}
+
+// y += M (x) 1 on electron 1 (pairs (0,2), (1,3)); M row-major 2x2 complex.
+__device__ __forceinline__ void apply_e1(const float2 m[4], const C4 &x, C4 &y) {
+    y.c[0] = cfma(m[0], x.c[0], cfma(m[1], x.c[2], y.c[0]));
+    y.c[2] = cfma(m[2], x.c[0], cfma(m[3], x.c[2], y.c[2]));
+    y.c[1] = cfma(m[0], x.c[1], cfma(m[1], x.c[3], y.c[1]));
+    y.c[3] = cfma(m[2], x.c[1], cfma(m[3], x.c[3], y.c[3]));
+}
+// y += 1 (x) M on electron 2 (pairs (0,1), (2,3)).
+__device__ __forceinline__ void apply_e2(const float2 m[4], const C4 &x, C4 &y) {
+    y.c[0] = cfma(m[0], x.c[0], cfma(m[1], x.c[1], y.c[0]));
+    y.c[1] = cfma(m[2], x.c[0], cfma(m[3], x.c[1], y.c[1]));
+    y.c[2] = cfma(m[0], x.c[2], cfma(m[1], x.c[3], y.c[2]));
+    y.c[3] = cfma(m[2], x.c[2], cfma(m[3], x.c[3], y.c[3]));
+}
+// 2x2 matrix f.S (S = sigma/2) for a real field f.
+__device__ __forceinline__ void field_matrix(float fx, float fy, float fz, float2 m[4]) {
+    m[0] = make_float2(0.5f * fz, 0.f);
+    m[1] = make_float2(0.5f * fx, -0.5f * fy);
+    m[2] = make_float2(0.5f * fx, 0.5f * fy);
+    m[3] = make_float2(-0.5f * fz, 0.f);
+}
+// Loads a 2x2 complex block, conjugate-transposed if `herm` (lane-selected).
+__device__ __forceinline__ void load_block(const float *p, bool herm, float2 m[4]) {
+    const float2 a = make_float2(p[0], p[1]), b = make_float2(p[2], p[3]);
+    const float2 c = make_float2(p[4], p[5]), d = make_float2(p[6], p[7]);
+    m[0] = herm ? conjf2(a) : a;
+    m[1] = herm ? conjf2(c) : b;
+    m[2] = herm ? conjf2(b) : c;
+    m[3] = herm ? conjf2(d) : d;
+}
+
+// H x for the group nu whose value is x; neighbours come from the LDS planes.
+// H = w1.S1 + w2.S2 + shift + [E] + sum_n (diagonal s_n(...) + flip-flop blocks).
+__device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict__ co, int nu, const C4 &x,
+                                      const float4 *LA, const float4 *LB) {
+    float f1x = co[0], f1y = co[1], f1z = co[2];
+    float f2x = co[3], f2y = co[4], f2z = co[5];
+    float rr = co[6];
+    C4 y;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) y.c[c] = make_float2(0.f, 0.f);
+    for (int n = 0; n < a.nnuc; ++n) {
+        const float *cn = co + kCoefHead + n * kNucStride;
+        const int fl = a.nuc_flags[n];
+        const bool down = (nu >> n) & 1;
+        const float s = down ? -1.f : 1.f;
+        const C4 nb = load_group(LA, LB, nu ^ (1 << n));
+        float2 m[4];
+        if (fl & NUC_E1) {
+            f1x = fmaf(s, cn[0], f1x); f1y = fmaf(s, cn[1], f1y); f1z = fmaf(s, cn[2], f1z);
+            load_block(cn + 7, down, m);
+            apply_e1(m, nb, y);
+        }
+        if (fl & NUC_E2) {
+            f2x = fmaf(s, cn[3], f2x); f2y = fmaf(s, cn[4], f2y); f2z = fmaf(s, cn[5], f2z);
+            load_block(cn + 15, down, m);
+            apply_e2(m, nb, y);
+        }
+        if (fl & NUC_T) {
+            rr = fmaf(s, cn[6], rr);
+            const float2 tt = make_float2(cn[23], down ? -cn[24] : cn[24]);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) y.c[c] = cfma(tt, nb.c[c], y.c[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        y.c[c].x = fmaf(rr, x.c[c].x, y.c[c].x);
+        y.c[c].y = fmaf(rr, x.c[c].y, y.c[c].y);
+    }
+    float2 m[4];
+    field_matrix(f1x, f1y, f1z, m);
+    apply_e1(m, x, y);
+    field_matrix(f2x, f2y, f2z, m);
+    apply_e2(m, x, y);
+    if (a.op_flags & OP_EREST) {
+        const float *E = co + 8;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                y.c[r] = cfma(make_float2(E[8 * r + 2 * c], E[8 * r + 2 * c + 1]), x.c[c], y.c[r]);
+    }
+    return y;
+}
+
+// Electron-pair density of a group, accumulated as 16 reals:
+// R[0..3] = |x_r|^2, then for r < c the complex x_c conj(x_r) (6 pairs).
+__device__ __forceinline__ void accum_rho(const C4 &x, float (&R)[16]) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) R[r] = fmaf(x.c[r].x, x.c[r].x, fmaf(x.c[r].y, x.c[r].y, R[r]));
+    int k = 4;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = r + 1; c < 4; ++c) {
+            // x_c conj(x_r)
+            R[k] = fmaf(x.c[c].x, x.c[r].x, fmaf(x.c[c].y, x.c[r].y, R[k]));
+            R[k + 1] = fmaf(x.c[c].y, x.c[r].x, fmaf(-x.c[c].x, x.c[r].y, R[k + 1]));
+            k += 2;
+        }
+}
+// Re Tr[P rho] = sum_r Re P_rr d_r + 2 sum_{r<c} Re(P_rc z_rc), z_rc = sum x_c conj(x_r).
+__device__ __forceinline__ double obs_from_rho(const float *P, const double *R) {
+    double v = 0.0;
+    for (int r = 0; r < 4; ++r) v += (double)P[8 * r + 2 * r] * R[r];
+    int k = 4;
+    for (int r = 0; r < 4; ++r)
+        for (int c = r + 1; c < 4; ++c) {
+            const double pr = P[8 * r + 2 * c], pi = P[8 * r + 2 * c + 1];
+            v += 2.0 * (pr * R[k] - pi * R[k + 1]);
+            k += 2;
+        }
+    return v;
+}
+
+// Samples every observable of one sample from the block-reduced density and
+// accumulates the Kahan-compensated yield and the trace (thread 0 only).
+__device__ __forceinline__ void record_sample(const StepArgs &a, int u, int s, const double (&R)[16], double *ys,
+                                              double *yc) {
+    for (int c = 0; c < a.nobs; ++c) {
+        const double v = obs_from_rho(a.obs + 32 * c, R);
+        const double term = a.wy[(size_t)s * a.nobs + c] * v - yc[c];
+        const double nsum = ys[c] + term;
+        yc[c] = (nsum - ys[c]) - term;
+        ys[c] = nsum;
+        if (a.trace) a.trace[((size_t)u * a.nsamples + s) * a.nobs + c] = a.wt[(size_t)s * a.nobs + c] * v;
+    }
+}
+
+// Sum over the workgroup of K doubles per thread, fixed order; every thread
+// receives the totals.  `scr` holds K*16 doubles.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double *scr) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) scr[k * 16 + wave] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += scr[k * 16 + w];
+        v[k] = s;
+    }
+}
+
+// Philox4x32-10 (Salmon et al., SC'11), first output word.
+__device__ __forceinline__ uint32_t philox_x0(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint64_t key) {
+    uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    return c0;
+}
+
+// Singlet-triplet dephasing event: psi -> (P_S - P_T) psi, i.e. the electron
+// pair (uu, ud, du, dd) -> (-uu, -du, -ud, -dd) (SpinSpace_relaxation.cpp:1171-1199).
+__device__ __forceinline__ void st_flip(C4 &x) {
+    const float2 a = x.c[1];
+    x.c[0] = make_float2(-x.c[0].x, -x.c[0].y);
+    x.c[1] = make_float2(-x.c[2].x, -x.c[2].y);
+    x.c[2] = make_float2(-a.x, -a.y);
+    x.c[3] = make_float2(-x.c[3].x, -x.c[3].y);
+}
+
+// exp(-i dt T) e1 for the real symmetric tridiagonal T (alpha, beta) of order
+// meff, in FP64 by a Chebyshev expansion on the Gershgorin interval, summed with
+// Clenshaw's recurrence while Miller's backward recurrence generates the Bessel
+// coefficients J_k(dt*rho) in the same order.  Executed by one wavefront; lane
+// i < meff writes component i to cf (re, im).
+__device__ void cheb_expm_wave(int meff, double dt, const double *al, const double *be, double *cf, int *status) {
+    const int i = threadIdx.x & 63;
+    const bool on = i < meff;
+    const double ai = on ? al[i] : 0.0;
+    const double bl = (on && i > 0) ? be[i - 1] : 0.0;
+    const double br = (i < meff - 1) ? be[i] : 0.0;
+    double lo = on ? ai - fabs(bl) - fabs(br) : 1e300;
+    double hi = on ? ai + fabs(bl) + fabs(br) : -1e300;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, off, 64));
+        hi = fmax(hi, __shfl_xor(hi, off, 64));
+    }
+    const double sigma = 0.5 * (lo + hi);
+    const double rho = 0.5 * (hi - lo);
+    const double z = dt * rho;
+    double re, im;
+    if (z < 1e-6) {
+        // second-order Taylor: error < z^3/6
+        const double x0 = on ? (ai - sigma) : 0.0;  // (T - sigma) e1 lives on lanes 0, 1
+        const double e = (i == 0) ? 1.0 : 0.0;
+        const double te = (i == 0) ? x0 : (i == 1 ? bl : 0.0);
+        // (T - sigma)^2 e1 = (T - sigma) te
+        const double tl = __shfl_up(te, 1, 64), tr = __shfl_down(te, 1, 64);
+        const double t2 = (on ? (ai - sigma) * te : 0.0) + (i > 0 ? bl * tl : 0.0) + br * tr;
+        re = e - 0.5 * dt * dt * t2;
+        im = -dt * te;
+    } else {
+        int K = (int)ceil(1.3 * z) + 18;
+        if (K > kChebMax) {
+            if (i == 0) atomicOr(status, 2);
+            K = kChebMax;
+        }
+        const double xd = on ? (ai - sigma) / rho : 0.0;
+        const double xl = bl / rho, xr = br / rho;
+        const double e = (i == 0) ? 1.0 : 0.0;
+        // Clenshaw state b_{k+1}, b_{k+2} (complex); Miller state j_{k+1}, j_k.
+        double b1r = 0, b1i = 0, b2r = 0, b2i = 0;
+        double jn1 = 0.0, jn = 1e-280, S = 0.0;
+        const int N = K + 24;
+        for (int k = N; k >= 0; --k) {
+            // jn holds the unnormalised J_k
+            if (k <= K) {
+                // c_k = (2 - delta_k0) (-i)^k J_k
+                const double mag = (k == 0 ? 1.0 : 2.0) * jn;
+                double cr = 0.0, ci = 0.0;
+                switch (k & 3) {
+                    case 0: cr = mag; break;
+                    case 1: ci = -mag; break;
+                    case 2: cr = -mag; break;
+                    default: ci = mag; break;
+                }
+                if (k > 0) {
+                    // b_k = c_k e + 2 x b_{k+1} - b_{k+2}
+                    const double lr = __shfl_up(b1r, 1, 64), li = __shfl_up(b1i, 1, 64);
+                    const double rr = __shfl_down(b1r, 1, 64), ri = __shfl_down(b1i, 1, 64);
+                    const double xbr = xd * b1r + (i > 0 ? xl * lr : 0.0) + xr * rr;
+                    const double xbi = xd * b1i + (i > 0 ? xl * li : 0.0) + xr * ri;
+                    const double nr = cr * e + 2.0 * xbr - b2r;
+                    const double ni = ci * e + 2.0 * xbi - b2i;
+                    b2r = b1r; b2i = b1i; b1r = nr; b1i = ni;
+                } else {
+                    // result = c_0 e + x b_1 - b_2
+                    const double lr = __shfl_up(b1r, 1, 64), li = __shfl_up(b1i, 1, 64);
+                    const double rr = __shfl_down(b1r, 1, 64), ri = __shfl_down(b1i, 1, 64);
+                    const double xbr = xd * b1r + (i > 0 ? xl * lr : 0.0) + xr * rr;
+                    const double xbi = xd * b1i + (i > 0 ? xl * li : 0.0) + xr * ri;
+                    b1r = cr * e + xbr - b2r;
+                    b1i = ci * e + xbi - b2i;
+                }
+            }
+            S += (k == 0) ? jn : ((k & 1) ? 0.0 : 2.0 * jn);
+            if (k > 0) {
+                const double jm = (2.0 * k / z) * jn - jn1;  // J_{k-1}
+                jn1 = jn;
+                jn = jm;
+                if (fabs(jn) > 1e200) {  // rescale everything linear in J
+                    jn *= 1e-200; jn1 *= 1e-200; S *= 1e-200;
+                    b1r *= 1e-200; b1i *= 1e-200; b2r *= 1e-200; b2i *= 1e-200;
+                }
+            }
+        }
+        // normalise (J_0 + 2 sum J_2k = 1) and apply the shift phase e^{-i dt sigma}
+        const double inv = 1.0 / S;
+        const double pr = cos(dt * sigma), pi = -sin(dt * sigma);
+        const double rr0 = b1r * inv, ri0 = b1i * inv;
+        re = rr0 * pr - ri0 * pi;
+        im = rr0 * pi + ri0 * pr;
+    }
+    if (on) {
+        cf[2 * i] = re;
+        cf[2 * i + 1] = im;
+    }
+}
+
+template <int G, int TMAX>
+__global__ __launch_bounds__(TMAX) void k_lanczos(const StepArgs *__restrict__ pa) {
+    const StepArgs &a = *pa;
+    extern __shared__ float4 smem[];
+    const int T = blockDim.x, t = threadIdx.x;
+    const int ng = G * T;
+    float4 *LA = smem;
+    float4 *LB = smem + ng;
+    double *scr = reinterpret_cast<double *>(smem + 2 * ng);  // 2 x 16 x 16 doubles
+    __shared__ double s_al[kMaxKrylov], s_be[kMaxKrylov], s_cf[2 * kMaxKrylov];
+    __shared__ double ys[kMaxObs], yc[kMaxObs];  // Kahan yield sums (thread 0)
+    __shared__ int s_unit;
+    const int D = a.dim;
+    const int D4 = D >> 2;
+    float *bas = a.basis + (size_t)blockIdx.x * a.m * 2 * D;
+    const int meff0 = a.m < D ? a.m : D;
+    int par = 0;
+    auto SCR = [&]() { par ^= 1; return scr + par * 256; };
+
+    for (;;) {
+        if (t == 0) s_unit = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        __syncthreads();
+        if (u >= a.n_units) break;
+        const int item = u / a.ncols, col = u - item * a.ncols;
+        const float *co = a.coef + (size_t)item * a.coef_stride;
+
+        // ---- initial state of this column
+        C4 psi[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int nu = g * T + t;
+            if (a.init_mode == 0) {
+                const int which = col / a.zdim, ncol = col - which * a.zdim;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    psi[g].c[c] = (nu == ncol) ? make_float2(a.evec[which * 8 + 2 * c], a.evec[which * 8 + 2 * c + 1])
+                                               : make_float2(0.f, 0.f);
+            } else {
+                const float *src = a.init_mode == 1 ? a.given + (size_t)col * 2 * D : a.state + (size_t)u * 2 * D;
+                const float4 *sA = reinterpret_cast<const float4 *>(src);
+                psi[g] = load_group(sA, sA + D4, nu);
+            }
+        }
+        if (t == 0)
+            for (int c = 0; c < a.nobs; ++c) {
+                ys[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2] : 0.0;
+                yc[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2 + 1] : 0.0;
+            }
+        const double n0 = a.norm0 ? a.norm0[u] : 0.0;
+
+        for (int s = a.s_begin;; ++s) {
+            const bool end = (s == a.s_end);
+            // ---- observables at sample s (HSObservableCollector::Evaluate)
+            if (!end || s == a.nsamples - 1) {
+                float Rf[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) Rf[k] = 0.f;
+#pragma unroll
+                for (int g = 0; g < G; ++g) accum_rho(psi[g], Rf);
+                double R[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) R[k] = Rf[k];
+                block_sum<16>(R, SCR());
+                if (t == 0) record_sample(a, u, s, R, ys, yc);
+            }
+            if (end) break;
+            const double dt = a.dts[s];
+            // ---- relaxation half-step before (HSPropagator.cpp:137)
+            if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s])
+#pragma unroll
+                for (int g = 0; g < G; ++g) st_flip(psi[g]);
+
+            // ---- Lanczos basis of H from psi (KrylovExpmGeneral, SpinSpace_operators.cpp:1257-1408).
+            // v_j lives in LDS, v_{j-1} in the basis store, the residual w in registers.
+            double nn[1] = {0.0};
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    nn[0] += (double)psi[g].c[c].x * psi[g].c[c].x + (double)psi[g].c[c].y * psi[g].c[c].y;
+            block_sum<1>(nn, SCR());
+            const double beta0 = sqrt(nn[0]);
+            if (beta0 > 0.0) {
+                {
+                    const float inv = (float)(1.0 / beta0);
+                    float4 *bA = reinterpret_cast<float4 *>(bas);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        C4 v0;
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) v0.c[c] = make_float2(psi[g].c[c].x * inv, psi[g].c[c].y * inv);
+                        const int nu = g * T + t;
+                        store_group(LA, LB, nu, v0);
+                        store_group(bA, bA + D4, nu, v0);
+                    }
+                }
+                double bprev = 0.0;
+                int meff = meff0;
+                for (int j = 0; j < meff0; ++j) {
+                    __syncthreads();
+                    C4 w[G];
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const int nu = g * T + t;
+                        w[g] = apply_h(a, co, nu, load_group(LA, LB, nu), LA, LB);
+                    }
+                    double al[1] = {0.0};
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const C4 vj = load_group(LA, LB, g * T + t);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            al[0] += (double)vj.c[c].x * w[g].c[c].x + (double)vj.c[c].y * w[g].c[c].y;
+                    }
+                    block_sum<1>(al, SCR());
+                    const float fa = (float)al[0], fb = (float)bprev;
+                    if (t == 0) s_al[j] = al[0];
+                    {
+                        const float4 *pA = reinterpret_cast<const float4 *>(bas + (size_t)(j > 0 ? j - 1 : 0) * 2 * D);
+#pragma unroll
+                        for (int g = 0; g < G; ++g) {
+                            const int nu = g * T + t;
+                            const C4 vj = load_group(LA, LB, nu);
+                            const C4 vp = j > 0 ? load_group(pA, pA + D4, nu) : C4{};
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                const float px = j > 0 ? vp.c[c].x : 0.f, py = j > 0 ? vp.c[c].y : 0.f;
+                                w[g].c[c].x = fmaf(-fa, vj.c[c].x, fmaf(-fb, px, w[g].c[c].x));
+                                w[g].c[c].y = fmaf(-fa, vj.c[c].y, fmaf(-fb, py, w[g].c[c].y));
+                            }
+                        }
+                    }
+                    if (j == meff0 - 1) break;
+                    double bb[1] = {0.0};
+#pragma unroll
+                    for (int g = 0; g < G; ++g)
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            bb[0] += (double)w[g].c[c].x * w[g].c[c].x + (double)w[g].c[c].y * w[g].c[c].y;
+                    block_sum<1>(bb, SCR());
+                    const double beta = sqrt(bb[0]);
+                    if (!(beta > 1e-30)) {  // invariant Krylov subspace (happy breakdown)
+                        meff = j + 1;
+                        break;
+                    }
+                    if (t == 0) s_be[j] = beta;
+                    const float ib = (float)(1.0 / beta);
+                    float4 *bA = reinterpret_cast<float4 *>(bas + (size_t)(j + 1) * 2 * D);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) w[g].c[c] = make_float2(w[g].c[c].x * ib, w[g].c[c].y * ib);
+                        const int nu = g * T + t;
+                        store_group(LA, LB, nu, w[g]);
+                        store_group(bA, bA + D4, nu, w[g]);
+                    }
+                    bprev = beta;
+                }
+                __syncthreads();
+                if (t < 64) cheb_expm_wave(meff, dt, s_al, s_be, s_cf, a.status);
+                __syncthreads();
+                // psi = beta * V c   (beta = norm0 when renormalising the unitary part)
+                const float sc = (float)(a.renorm ? n0 : beta0);
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) psi[g].c[c] = make_float2(0.f, 0.f);
+                for (int j = 0; j < meff; ++j) {
+                    const float2 cj = make_float2(sc * (float)s_cf[2 * j], sc * (float)s_cf[2 * j + 1]);
+                    const float4 *bA = reinterpret_cast<const float4 *>(bas + (size_t)j * 2 * D);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const C4 vj = load_group(bA, bA + D4, g * T + t);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) psi[g].c[c] = cfma(cj, vj.c[c], psi[g].c[c]);
+                    }
+                }
+            }
+            // ---- relaxation half-step after (HSPropagator.cpp:139)
+            if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
+#pragma unroll
+                for (int g = 0; g < G; ++g) st_flip(psi[g]);
+        }
+        // ---- results of this unit
+        if (t == 0)
+            for (int c = 0; c < a.nobs; ++c) {
+                a.yacc[((size_t)u * a.nobs + c) * 2] = ys[c];
+                a.yacc[((size_t)u * a.nobs + c) * 2 + 1] = yc[c];
+            }
+        if (a.state) {
+            float4 *dA = reinterpret_cast<float4 *>(a.state + (size_t)u * 2 * D);
+#pragma unroll
+            for (int g = 0; g < G; ++g) store_group(dA, dA + D4, g * T + t, psi[g]);
+        }
+        bool bad = false;
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bad |= !isfinite(psi[g].c[c].x) || !isfinite(psi[g].c[c].y);
+        if (bad) atomicOr(a.status, 1);
+    }
+}
+
+// Norms of the initial columns (renormalisation targets).
+__global__ void k_init_norms(const StepArgs *__restrict__ pa) {
+    const StepArgs &a = *pa;
+    // one wave per unit; initial columns follow init_mode as in k_lanczos
+    const int u = blockIdx.x;
+    if (u >= a.n_units) return;
+    const int item = u / a.ncols, col = u - item * a.ncols;
+    (void)item;
+    const int D = a.dim;
+    double acc = 0.0;
+    if (a.init_mode == 0) {
+        const int which = col / a.zdim;
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 8; ++k) acc += (double)a.evec[which * 8 + k] * a.evec[which * 8 + k];
+    } else {
+        const float *src = a.init_mode == 1 ? a.given + (size_t)col * 2 * D : a.state + (size_t)u * 2 * D;
+        for (int i = threadIdx.x; i < 2 * D; i += blockDim.x) acc += (double)src[i] * src[i];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (threadIdx.x == 0) a.norm0[u] = sqrt(acc);
+}
+
+__global__ void k_zero_int(int *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// out[item][k] = sum over the item's columns of in[unit][k] (unit = item*ncols + col),
+// columns summed in order.  `in_stride` doubles per unit, `k` picks every
+// `pick`-th entry (Kahan pairs store sum at even offsets).
+__global__ void k_reduce_columns(const double *in, double *out, int n_items, int ncols, int nk, int in_stride,
+                                 int pick) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_items * nk) return;
+    const int item = i / nk, k = i - item * nk;
+    double s = 0.0, comp = 0.0;
+    for (int c = 0; c < ncols; ++c) {
+        const double x = in[(size_t)(item * ncols + c) * in_stride + (size_t)k * pick] -
+                         (pick == 2 ? in[(size_t)(item * ncols + c) * in_stride + (size_t)k * pick + 1] : 0.0);
+        const double y = x - comp;
+        const double ns = s + y;
+        comp = (ns - s) - y;
+        s = ns;
+    }
+    out[i] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Dense path: B <- U B on the FP32 matrix cores.
+// ---------------------------------------------------------------------------
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+constexpr int kDenseMaxTiles = 8;  // row tiles per wave: Dp <= 4 waves * 8 * 16 = 512
+
+// Element (row, col) of the initial state of a dense-path unit.
+__device__ __forceinline__ float2 dense_init(const StepArgs &a, int u, int col, int row) {
+    const int D = a.dim;
+    if (row >= D || col >= a.ncols) return make_float2(0.f, 0.f);
+    if (a.init_mode == 0) {
+        const int which = col / a.zdim, ncol = col - which * a.zdim;
+        const int nu = row >> 2, c = row & 3;
+        return nu == ncol ? make_float2(a.evec[which * 8 + 2 * c], a.evec[which * 8 + 2 * c + 1]) : make_float2(0.f, 0.f);
+    }
+    if (a.init_mode == 1) {
+        const float *src = a.given + (size_t)col * 2 * D;
+        const int nu = row >> 2, c = row & 3;
+        const float *p = c < 2 ? src + 4 * nu + 2 * c : src + D + 4 * nu + 2 * (c - 2);
+        return make_float2(p[0], p[1]);
+    }
+    const float *st = a.state + (size_t)u * 2 * a.dpad * 16;
+    const int tcol = col & 15;
+    return make_float2(st[row * 16 + tcol], st[a.dpad * 16 + row * 16 + tcol]);
+}
+
+__global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) {
+    const StepArgs &a = *pa;
+    extern __shared__ float dsm[];
+    __shared__ double ys[kMaxObs], yc[kMaxObs];  // Kahan yield sums (thread 0)
+    const int Dp = a.dpad;
+    float *Bre = dsm, *Bim = dsm + Dp * 16;
+    double *scr = reinterpret_cast<double *>(dsm + 2 * Dp * 16);
+    __shared__ int s_unit;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int q = lane >> 4, cl = lane & 15;
+    const int ntile = Dp >> 4;
+    const int tiles_per_item = (a.ncols + 15) >> 4;
+    int par = 0;
+    auto SCR = [&]() { par ^= 1; return scr + par * 256; };
+    for (;;) {
+        if (t == 0) s_unit = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        __syncthreads();
+        if (u >= a.n_units) break;
+        const int item = u / tiles_per_item, tile = u - item * tiles_per_item;
+        const int col = tile * 16 + cl;  // the column this lane's accumulators hold
+        // register copy of the state in MFMA output layout
+        floatx4 xr[kDenseMaxTiles], xi[kDenseMaxTiles];
+#pragma unroll
+        for (int k = 0; k < kDenseMaxTiles; ++k) {
+            const int rt = wave + 4 * k;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float2 v = rt < ntile ? dense_init(a, u, col, rt * 16 + 4 * q + i) : make_float2(0.f, 0.f);
+                xr[k][i] = v.x;
+                xi[k][i] = v.y;
+            }
+        }
+        if (t == 0)
+            for (int c = 0; c < a.nobs; ++c) {
+                ys[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2] : 0.0;
+                yc[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2 + 1] : 0.0;
+            }
+        const bool colok = col < a.ncols;
+        for (int s = a.s_begin;; ++s) {
+            const bool end = s == a.s_end;
+            if (!end || s == a.nsamples - 1) {
+                float Rf[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) Rf[k] = 0.f;
+#pragma unroll
+                for (int k = 0; k < kDenseMaxTiles; ++k) {
+                    C4 g;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) g.c[i] = make_float2(xr[k][i], xi[k][i]);
+                    accum_rho(g, Rf);
+                }
+                double R[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) R[k] = Rf[k];
+                block_sum<16>(R, SCR());
+                if (t == 0) record_sample(a, u, s, R, ys, yc);
+            }
+            if (end) break;
+            if (a.relax && colok && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s])
+#pragma unroll
+                for (int k = 0; k < kDenseMaxTiles; ++k) {
+                    C4 g;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) g.c[i] = make_float2(xr[k][i], xi[k][i]);
+                    st_flip(g);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) { xr[k][i] = g.c[i].x; xi[k][i] = g.c[i].y; }
+                }
+            // publish B to LDS ([row][16 columns] planes)
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kDenseMaxTiles; ++k) {
+                const int rt = wave + 4 * k;
+                if (rt < ntile)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        Bre[(rt * 16 + 4 * q + i) * 16 + cl] = xr[k][i];
+                        Bim[(rt * 16 + 4 * q + i) * 16 + cl] = xi[k][i];
+                    }
+            }
+            __syncthreads();
+            // B' = U B: real MFMA form of the complex product
+            const int uw = (a.ulast && s == a.nsamples - 2) ? 1 : 0;
+            const float *Ur = a.U + ((size_t)item * a.ustride + uw) * 2 * Dp * Dp, *Ui = Ur + (size_t)Dp * Dp;
+#pragma unroll
+            for (int k = 0; k < kDenseMaxTiles; ++k) {
+                xr[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+                xi[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+            }
+            for (int kk = 0; kk < ntile * 4; ++kk) {
+                const int kr = kk * 4 + q;
+                const float br = Bre[kr * 16 + cl], bi = Bim[kr * 16 + cl];
+#pragma unroll
+                for (int k = 0; k < kDenseMaxTiles; ++k) {
+                    const int rt = wave + 4 * k;
+                    if (rt < ntile) {
+                        const float ar = Ur[(size_t)kr * Dp + rt * 16 + cl];
+                        const float ai = Ui[(size_t)kr * Dp + rt * 16 + cl];
+                        xr[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, xr[k], 0, 0, 0);
+                        xr[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, bi, xr[k], 0, 0, 0);
+                        xi[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, xi[k], 0, 0, 0);
+                        xi[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, xi[k], 0, 0, 0);
+                    }
+                }
+            }
+            if (a.relax && colok && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
+#pragma unroll
+                for (int k = 0; k < kDenseMaxTiles; ++k) {
+                    C4 g;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) g.c[i] = make_float2(xr[k][i], xi[k][i]);
+                    st_flip(g);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) { xr[k][i] = g.c[i].x; xi[k][i] = g.c[i].y; }
+                }
+        }
+        if (t == 0)
+            for (int c = 0; c < a.nobs; ++c) {
+                a.yacc[((size_t)u * a.nobs + c) * 2] = ys[c];
+                a.yacc[((size_t)u * a.nobs + c) * 2 + 1] = yc[c];
+            }
+        bool bad = false;
+#pragma unroll
+        for (int k = 0; k < kDenseMaxTiles; ++k) {
+            const int rt = wave + 4 * k;
+            if (rt < ntile)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    bad |= !isfinite(xr[k][i]) || !isfinite(xi[k][i]);
+                    if (a.state) {
+                        float *st = a.state + (size_t)u * 2 * Dp * 16;
+                        st[(rt * 16 + 4 * q + i) * 16 + cl] = xr[k][i];
+                        st[Dp * 16 + (rt * 16 + 4 * q + i) * 16 + cl] = xi[k][i];
+                    }
+                }
+        }
+        if (bad) atomicOr(a.status, 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers (called from the host part of the library)
+// ---------------------------------------------------------------------------
+#define MSG_LANCZOS_VARIANTS(X) X(1, 256) X(2, 256) X(4, 256) X(2, 1024)
+
+hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_thread, int threads, int slots,
+                          hipStream_t st) {
+    const size_t lds = (size_t)2 * a.dim * sizeof(float) + 2 * 256 * sizeof(double);
+#define X(G_, T_)                                                                        \
+    if (group_per_thread == G_ && threads <= T_) {                                       \
+        hipLaunchKernelGGL((k_lanczos<G_, T_>), dim3(slots), dim3(threads), lds, st, da); \
+        return hipGetLastError();                                                         \
+    }
+    MSG_LANCZOS_VARIANTS(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+// Raises the dynamic-LDS limit of the large-LDS kernels to what the device
+// allows per workgroup (minus their static LDS); best effort.
+hipError_t set_lanczos_lds_limits() {
+    int dev = 0, maxb = 65536;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    const int lim = maxb - 2048;
+#define X(G_, T_) (void)hipFuncSetAttribute((const void *)k_lanczos<G_, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    MSG_LANCZOS_VARIANTS(X)
+#undef X
+    (void)hipFuncSetAttribute((const void *)k_dense, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipGetLastError();
+    return hipSuccess;
+}
+
+hipError_t launch_dense(const StepArgs &a, const StepArgs *da, int slots, hipStream_t st) {
+    const size_t lds = (size_t)2 * a.dpad * 16 * sizeof(float) + 2 * 256 * sizeof(double);
+    hipLaunchKernelGGL(k_dense, dim3(slots), dim3(256), lds, st, da);
+    return hipGetLastError();
+}
+
+int lanczos_occupancy(int group_per_thread, int threads, size_t lds) {
+    int nb = 0;
+#define X(G_, T_)                                                                                        \
+    if (group_per_thread == G_ && threads <= T_)                                                         \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos<G_, T_>, threads, lds) == hipSuccess \
+                   ? nb : 1;
+    MSG_LANCZOS_VARIANTS(X)
+#undef X
+    return 1;
+}
+
+int dense_occupancy(size_t lds) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_dense, 256, lds) != hipSuccess) return 1;
+    return nb;
+}
+
+hipError_t launch_init_norms(const StepArgs &a, const StepArgs *da, hipStream_t st) {
+    hipLaunchKernelGGL(k_init_norms, dim3(a.n_units), dim3(64), 0, st, da);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_int(int *p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_columns(const double *in, double *out, int n_items, int ncols, int nk, int in_stride, int pick,
+                                 hipStream_t st) {
+    const int n = n_items * nk;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_columns, dim3((n + 127) / 128), dim3(128), 0, st, in, out, n_items, ncols, nk, in_stride,
+                       pick);
+    return hipGetLastError();
+}
+
+}  // namespace msg

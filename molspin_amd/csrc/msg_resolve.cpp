@@ -1,0 +1,538 @@
+// Host-side resolution of spin systems into device operator formats.
+#include "msg_resolve.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace msg {
+
+namespace {
+
+const double kCommon = 87.9410005;  // rad ns^-1 T^-1 (SpinSpace_hamiltonians.cpp:746-748)
+
+// T_a = sum_c g[a][c] S_c (Spin::Tx/Ty/Tz with laboratory quantisation axes).
+void t_matrix(const msg_spin_desc &sp, bool ignore, double g[9]) {
+    for (int i = 0; i < 9; ++i) g[i] = ignore ? ((i % 4) == 0 ? 1.0 : 0.0) : sp.tensor[i];
+}
+
+// Spin-S matrices in the |+S>-first basis, row-major mult x mult, axis x/y/z.
+std::vector<cd> spin_op(int mult, int axis) {
+    const double S = 0.5 * (mult - 1);
+    std::vector<cd> m((size_t)mult * mult, 0.0);
+    for (int i = 0; i < mult; ++i) {
+        if (axis == 2) { m[(size_t)i * mult + i] = S - i; continue; }
+        // <i-1| S+ |i> = sqrt(S(S+1) - m(m+1)), m = S - i
+        if (i > 0) {
+            const double mm = S - i;
+            const double v = std::sqrt(S * (S + 1) - mm * (mm + 1));
+            // Sx = (S+ + S-)/2, Sy = (S+ - S-)/(2i)
+            if (axis == 0) {
+                m[(size_t)(i - 1) * mult + i] += 0.5 * v;
+                m[(size_t)i * mult + (i - 1)] += 0.5 * v;
+            } else {
+                m[(size_t)(i - 1) * mult + i] += cd(0.0, -0.5 * v);
+                m[(size_t)i * mult + (i - 1)] += cd(0.0, 0.5 * v);
+            }
+        }
+    }
+    return m;
+}
+
+// Electron-pair projector of a State (|v><v|/<v|v> on the listed electrons,
+// identity on the other electron), or false if the State involves a nucleus.
+bool pair_projector(const msg_system_desc *d, const Resolved &r, int state, Mat4 &P) {
+    const msg_state_desc &st = d->states[state];
+    const int32_t *sp = d->index_pool + st.spins_offset;
+    const int ns = st.spins_count;
+    if (ns < 1 || ns > 2) return false;
+    int which[2] = {-1, -1};
+    for (int k = 0; k < ns; ++k) {
+        if (sp[k] == r.e1) which[k] = 0;
+        else if (sp[k] == r.e2) which[k] = 1;
+        else return false;
+    }
+    if (ns == 2 && which[0] == which[1]) return false;
+    const int L = ns == 2 ? 4 : 2;
+    std::vector<cd> v(L);
+    double nn = 0.0;
+    for (int i = 0; i < L; ++i) {
+        v[i] = cd(d->amp_pool[2 * (st.amp_offset + i)], d->amp_pool[2 * (st.amp_offset + i) + 1]);
+        nn += std::norm(v[i]);
+    }
+    if (!(nn > 0.0)) return false;
+    P.fill(0.0);
+    for (int row = 0; row < 4; ++row)
+        for (int col = 0; col < 4; ++col) {
+            const int ra[2] = {row >> 1, row & 1}, ca[2] = {col >> 1, col & 1};
+            if (ns == 2) {
+                // listed order (first most significant) -> sub index
+                const int rs = ra[which[0]] * 2 + ra[which[1]];
+                const int cs = ca[which[0]] * 2 + ca[which[1]];
+                P[row * 4 + col] = v[rs] * std::conj(v[cs]) / nn;
+            } else {
+                const int other = 1 - which[0];
+                if (ra[other] != ca[other]) continue;
+                P[row * 4 + col] = v[ra[which[0]]] * std::conj(v[ca[which[0]]]) / nn;
+            }
+        }
+    return true;
+}
+
+// Jacobi eigen-decomposition of a 4x4 Hermitian matrix (row-major).
+void eig4(const Mat4 &Ain, double w[4], cd V[16]) {
+    cd A[16];
+    for (int i = 0; i < 16; ++i) A[i] = Ain[i];
+    for (int i = 0; i < 16; ++i) V[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j)
+                if (i != j) off += std::norm(A[i * 4 + j]);
+        if (off < 1e-32) break;
+        for (int p = 0; p < 3; ++p)
+            for (int q = p + 1; q < 4; ++q) {
+                const double g = std::abs(A[p * 4 + q]);
+                if (g < 1e-300) continue;
+                const cd ph = A[p * 4 + q] / g;
+                const double th = 0.5 * std::atan2(2.0 * g, A[q * 4 + q].real() - A[p * 4 + p].real());
+                const double c = std::cos(th), s = std::sin(th);
+                const cd r00 = c, r01 = s, r10 = -s * std::conj(ph), r11 = c * std::conj(ph);
+                for (int k = 0; k < 4; ++k) {
+                    const cd x = A[k * 4 + p], y = A[k * 4 + q];
+                    A[k * 4 + p] = x * r00 + y * r10;
+                    A[k * 4 + q] = x * r01 + y * r11;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const cd x = A[p * 4 + k], y = A[q * 4 + k];
+                    A[p * 4 + k] = std::conj(r00) * x + std::conj(r10) * y;
+                    A[q * 4 + k] = std::conj(r01) * x + std::conj(r11) * y;
+                }
+                for (int k = 0; k < 4; ++k) {
+                    const cd x = V[k * 4 + p], y = V[k * 4 + q];
+                    V[k * 4 + p] = x * r00 + y * r10;
+                    V[k * 4 + q] = x * r01 + y * r11;
+                }
+            }
+    }
+    for (int i = 0; i < 4; ++i) w[i] = A[i * 5].real();
+}
+
+void matmul(int n, const std::vector<cd> &A, const std::vector<cd> &B, std::vector<cd> &C) {
+    C.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) {
+            const cd a = A[(size_t)i * n + k];
+            if (a == 0.0) continue;
+            const cd *b = &B[(size_t)k * n];
+            cd *c = &C[(size_t)i * n];
+            for (int j = 0; j < n; ++j) c[j] += a * b[j];
+        }
+}
+
+}  // namespace
+
+std::string free_times(double total, double dt, std::vector<double> &t) {
+    t.assign(1, 0.0);
+    if (!(dt > 0.0) || !std::isfinite(dt)) return "timestep must be finite and positive";
+    if (!(total >= 0.0) || !std::isfinite(total)) return "totaltime must be finite and non-negative";
+    if (total == 0.0) return "";
+    const double ratio = total / dt;
+    if (!std::isfinite(ratio) || ratio > 1e9) return "too many time intervals";
+    const double nearest = std::round(ratio);
+    const double tol = 64.0 * 2.220446049250313e-16 * std::max(1.0, std::fabs(ratio));
+    const int64_t n = (int64_t)((nearest >= 1.0 && std::fabs(ratio - nearest) <= tol) ? nearest : std::ceil(ratio));
+    for (int64_t s = 1; s <= n; ++s) {
+        const double target = s == n ? total : std::min(total, (double)s * dt);
+        if (!(target > t.back())) return "non-increasing time grid";
+        t.push_back(target);
+    }
+    return "";
+}
+
+std::string resolve_system(const msg_system_desc *d, Resolved &r) {
+    r = Resolved();
+    if (!d || d->n_spins < 1 || !d->spins) return "system has no spins";
+    r.nspins = d->n_spins;
+    r.mult.resize(r.nspins);
+    r.D = 1;
+    for (int i = 0; i < r.nspins; ++i) {
+        r.mult[i] = d->spins[i].multiplicity;
+        if (r.mult[i] < 1 || r.mult[i] > 16) return "invalid spin multiplicity";
+        r.D *= r.mult[i];
+        if (r.D > (int64_t)1 << 30) return "Hilbert space too large";
+    }
+    auto pool_ok = [&](int off, int cnt) {
+        return off >= 0 && cnt >= 0 && off + cnt <= d->index_pool_len;
+    };
+    // electrons
+    std::vector<int> el;
+    for (int i = 0; i < r.nspins; ++i)
+        if (d->spins[i].is_electron) el.push_back(i);
+    if (el.size() == 2 && r.mult[el[0]] == 2 && r.mult[el[1]] == 2) {
+        r.e1 = el[0];
+        r.e2 = el[1];
+        r.pair_ok = true;
+    }
+    if (!r.pair_ok) return "";  // structure not representable on the device paths; caller reports
+    for (int i = 0; i < r.nspins; ++i)
+        if (i != r.e1 && i != r.e2) r.nuc.push_back(i);
+    r.zdim = r.D / 4;
+    // MolSpin index -> device index (nu * 4 + 2a + b, nucleus 0 least significant)
+    r.perm.resize(r.D);
+    std::vector<int64_t> stride(r.nspins);
+    {
+        int64_t s = 1;
+        for (int i = r.nspins - 1; i >= 0; --i) { stride[i] = s; s *= r.mult[i]; }
+    }
+    std::vector<int64_t> nstride(r.nuc.size());
+    {
+        int64_t s = 1;
+        for (size_t k = 0; k < r.nuc.size(); ++k) { nstride[k] = s; s *= r.mult[r.nuc[k]]; }
+    }
+    for (int64_t idx = 0; idx < r.D; ++idx) {
+        const int a = (int)((idx / stride[r.e1]) % 2), b = (int)((idx / stride[r.e2]) % 2);
+        int64_t nu = 0;
+        for (size_t k = 0; k < r.nuc.size(); ++k)
+            nu += ((idx / stride[r.nuc[k]]) % r.mult[r.nuc[k]]) * nstride[k];
+        r.perm[idx] = nu * 4 + 2 * a + b;
+    }
+    // interactions -> one- and two-spin terms (InteractionOperator)
+    for (int it = 0; it < d->n_interactions; ++it) {
+        const msg_interaction_desc &in = d->interactions[it];
+        if (!pool_ok(in.group1_offset, in.group1_count) || !pool_ok(in.group2_offset, in.group2_count))
+            return "interaction spin group out of range";
+        double sc = in.prefactor;
+        if (in.tensor_present && in.tensor_isotropic) sc *= in.tensor[0];
+        if (in.common_prefactor) sc *= kCommon;
+        const int32_t *g1 = d->index_pool + in.group1_offset;
+        const int32_t *g2 = d->index_pool + in.group2_offset;
+        for (int k = 0; k < in.group1_count; ++k)
+            if (g1[k] < 0 || g1[k] >= r.nspins) return "interaction spin index out of range";
+        for (int k = 0; k < in.group2_count; ++k)
+            if (g2[k] < 0 || g2[k] >= r.nspins) return "interaction spin index out of range";
+        if (in.type == MSG_INTERACTION_ZEEMAN) {
+            for (int k = 0; k < in.group1_count; ++k) {
+                Term1 t{};
+                t.s = g1[k];
+                t.swept = in.field_swept != 0;
+                double G[9];
+                t_matrix(d->spins[t.s], in.ignore_tensors != 0, G);
+                for (int i = 0; i < 9; ++i) t.Mz[i] = sc * G[i];
+                for (int a = 0; a < 3; ++a)
+                    t.c[a] = t.Mz[3 * a] * in.field[0] + t.Mz[3 * a + 1] * in.field[1] + t.Mz[3 * a + 2] * in.field[2];
+                r.t1.push_back(t);
+            }
+        } else if (in.type == MSG_INTERACTION_DOUBLESPIN || in.type == MSG_INTERACTION_EXCHANGE) {
+            const bool ex = in.type == MSG_INTERACTION_EXCHANGE;
+            const bool aniso = in.tensor_present && !in.tensor_isotropic;
+            for (int i = 0; i < in.group1_count; ++i)
+                for (int j = 0; j < in.group2_count; ++j) {
+                    Term2 t{};
+                    t.s1 = g1[i];
+                    t.s2 = g2[j];
+                    if (t.s1 == t.s2) return "a two-spin interaction couples a spin to itself";
+                    double ga[9], gb[9];
+                    t_matrix(d->spins[t.s1], in.ignore_tensors != 0, ga);
+                    t_matrix(d->spins[t.s2], in.ignore_tensors != 0, gb);
+                    // S1_c S2_e coefficient of sum_ab A_ab T1a T2b
+                    for (int c = 0; c < 3; ++c)
+                        for (int e = 0; e < 3; ++e) {
+                            double m = 0.0;
+                            for (int aa = 0; aa < 3; ++aa)
+                                for (int bb = 0; bb < 3; ++bb)
+                                    m += (aniso ? in.tensor[3 * aa + bb] : (aa == bb ? 1.0 : 0.0)) * ga[3 * aa + c] *
+                                         gb[3 * bb + e];
+                            t.M[3 * c + e] = sc * (ex ? 2.0 : 1.0) * m;
+                        }
+                    r.t2.push_back(t);
+                    if (ex) r.shift += 0.5 * sc;
+                }
+        } else {
+            return "interaction type not supported";
+        }
+    }
+    // structure of the matrix-free operator
+    r.fast_ok = true;
+    for (int k : r.nuc)
+        if (r.mult[k] != 2) { r.fast_ok = false; r.why_not_fast = "nucleus with spin > 1/2"; }
+    if ((int)r.nuc.size() > kMaxNuclei) { r.fast_ok = false; r.why_not_fast = "too many nuclei"; }
+    for (const Term2 &t : r.t2) {
+        const bool e_a = t.s1 == r.e1 || t.s1 == r.e2, e_b = t.s2 == r.e1 || t.s2 == r.e2;
+        if (e_a && e_b) r.has_ee = true;
+        if (!e_a && !e_b) { r.fast_ok = false; r.why_not_fast = "nucleus-nucleus coupling"; }
+    }
+    // states, transitions, reaction operator
+    for (int s = 0; s < d->n_states; ++s) {
+        if (!pool_ok(d->states[s].spins_offset, d->states[s].spins_count)) return "state spin list out of range";
+        Mat4 P;
+        if (!pair_projector(d, r, s, P)) {
+            r.pair_ok = false;
+            return "";
+        }
+        r.P_states.push_back(P);
+    }
+    r.K.fill(0.0);
+    for (int t = 0; t < d->n_transitions; ++t) {
+        const int src = d->transitions[t].source_state;
+        if (src < 0) continue;
+        if (src >= d->n_states) return "transition source out of range";
+        r.P_trans.push_back(r.P_states[src]);
+        r.rates.push_back(d->transitions[t].rate);
+        r.kmax = std::max(r.kmax, d->transitions[t].rate);
+        for (int i = 0; i < 16; ++i) r.K[i] += 0.5 * d->transitions[t].rate * r.P_states[src][i];
+    }
+    {
+        const cd kap = 0.25 * (r.K[0] + r.K[5] + r.K[10] + r.K[15]);
+        double dev = 0.0;
+        for (int i = 0; i < 16; ++i) dev += std::abs(r.K[i] - ((i % 5) == 0 ? kap : cd(0.0)));
+        r.symmetric = dev <= 1e-13 * std::max(1.0, std::abs(kap)) && std::fabs(kap.imag()) < 1e-300 + 1e-13;
+        r.kappa = kap.real();
+    }
+    // relaxation
+    if (d->n_relaxations > 1) return "only one relaxationdephasing operator is supported";
+    if (d->n_relaxations == 1 && d->relaxations[0].rate != 0.0) {
+        const auto &rl = d->relaxations[0];
+        if (!((rl.spin1 == r.e1 && rl.spin2 == r.e2) || (rl.spin1 == r.e2 && rl.spin2 == r.e1)))
+            return "relaxationdephasing must act on the radical-pair electrons";
+        r.relax = true;
+        r.relax_rate = rl.rate;
+    }
+    // direct factors of rho0 = rho_e (x) 1/Z (HSStatePreparation.cpp:77-165)
+    if (d->n_initial >= 1) {
+        if (d->n_initial > 16) return "too many initial states";
+        std::vector<double> w(d->n_initial, 1.0 / d->n_initial);
+        if (d->initial_weights) {
+            double tot = 0.0;
+            for (int i = 0; i < d->n_initial; ++i) {
+                w[i] = d->initial_weights[i];
+                if (!(w[i] >= 0.0)) return "initial-state weights must be non-negative";
+                tot += w[i];
+            }
+            if (!(tot > 0.0)) return "initial-state weights must have a positive sum";
+            for (double &x : w) x /= tot;
+        }
+        Mat4 rho{};
+        for (int i = 0; i < d->n_initial; ++i) {
+            const int s = d->initial_states[i];
+            if (s < 0 || s >= d->n_states) return "initial state out of range";
+            const Mat4 &P = r.P_states[s];
+            const double tr = (P[0] + P[5] + P[10] + P[15]).real();
+            for (int k = 0; k < 16; ++k) rho[k] += w[i] * P[k] / tr;
+        }
+        double ev[4];
+        cd V[16];
+        eig4(rho, ev, V);
+        for (int k = 0; k < 4; ++k)
+            if (ev[k] > 1e-12) {
+                std::array<cd, 4> v;
+                for (int i = 0; i < 4; ++i) v[i] = std::sqrt(ev[k]) * V[i * 4 + k];
+                r.evec.push_back(v);
+            }
+    }
+    return "";
+}
+
+void fast_coefficients(const Resolved &r, const double *field, std::vector<float> &out, int &op_flags,
+                       int nuc_flags[kMaxNuclei]) {
+    const int N = (int)r.nuc.size();
+    std::vector<double> c((size_t)kCoefHead + (size_t)N * kNucStride, 0.0);
+    std::vector<int> nucpos(r.nspins, -1);
+    for (int k = 0; k < N; ++k) nucpos[r.nuc[k]] = k;
+    for (int k = 0; k < kMaxNuclei; ++k) nuc_flags[k] = 0;
+    op_flags = 0;
+    c[6] = r.shift;
+    // electron-pair matrix E (e-e couplings); S1 acts on the high bit of c
+    std::vector<cd> sx = spin_op(2, 0), sy = spin_op(2, 1), sz = spin_op(2, 2);
+    const std::vector<cd> *S[3] = {&sx, &sy, &sz};
+    cd E[16] = {};
+    for (const Term1 &t : r.t1) {
+        double v[3];
+        for (int a = 0; a < 3; ++a)
+            v[a] = (t.swept && field) ? t.Mz[3 * a] * field[0] + t.Mz[3 * a + 1] * field[1] + t.Mz[3 * a + 2] * field[2]
+                                      : t.c[a];
+        if (t.s == r.e1) for (int a = 0; a < 3; ++a) c[a] += v[a];
+        else if (t.s == r.e2) for (int a = 0; a < 3; ++a) c[3 + a] += v[a];
+        else {
+            const int k = nucpos[t.s];
+            double *cn = &c[kCoefHead + (size_t)k * kNucStride];
+            cn[6] += 0.5 * v[2];            // <up|c.I|up> = c_z/2
+            cn[23] += 0.5 * v[0];           // <up|c.I|down> = (c_x - i c_y)/2
+            cn[24] += -0.5 * v[1];
+            nuc_flags[k] |= NUC_T;
+        }
+    }
+    for (const Term2 &t : r.t2) {
+        const bool e1a = t.s1 == r.e1 || t.s1 == r.e2, e2b = t.s2 == r.e1 || t.s2 == r.e2;
+        if (e1a && e2b) {
+            // sum_ab M_ab S_{s1,a} S_{s2,b} on the pair, kron(e1 op, e2 op)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    const double m = t.M[3 * a + b];
+                    if (m == 0.0) continue;
+                    const std::vector<cd> &A1 = t.s1 == r.e1 ? *S[a] : *S[b];
+                    const std::vector<cd> &A2 = t.s1 == r.e1 ? *S[b] : *S[a];
+                    for (int i = 0; i < 4; ++i)
+                        for (int j = 0; j < 4; ++j)
+                            E[i * 4 + j] += m * A1[(i >> 1) * 2 + (j >> 1)] * A2[(i & 1) * 2 + (j & 1)];
+                }
+            op_flags |= OP_EREST;
+            continue;
+        }
+        // electron-nucleus: M_ab with a on the electron, b on the nucleus
+        int e, n;
+        double M[9];
+        if (e1a) { e = t.s1; n = t.s2; for (int i = 0; i < 9; ++i) M[i] = t.M[i]; }
+        else { e = t.s2; n = t.s1; for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) M[3 * a + b] = t.M[3 * b + a]; }
+        const int k = nucpos[n];
+        double *cn = &c[kCoefHead + (size_t)k * kNucStride];
+        const int hoff = e == r.e1 ? 0 : 3, boff = e == r.e1 ? 7 : 15;
+        nuc_flags[k] |= e == r.e1 ? NUC_E1 : NUC_E2;
+        for (int a = 0; a < 3; ++a) cn[hoff + a] += 0.5 * M[3 * a + 2];
+        // u_a = (M_ax - i M_ay)/2;  block = sum_a u_a sigma_a / 2
+        cd u[3];
+        for (int a = 0; a < 3; ++a) u[a] = cd(0.5 * M[3 * a + 0], -0.5 * M[3 * a + 1]);
+        const cd I(0.0, 1.0);
+        const cd b00 = 0.5 * u[2], b01 = 0.5 * (u[0] - I * u[1]), b10 = 0.5 * (u[0] + I * u[1]), b11 = -0.5 * u[2];
+        const cd blk[4] = {b00, b01, b10, b11};
+        for (int q = 0; q < 4; ++q) {
+            cn[boff + 2 * q] += blk[q].real();
+            cn[boff + 2 * q + 1] += blk[q].imag();
+        }
+    }
+    for (int i = 0; i < 16; ++i) {
+        c[8 + 2 * i] = E[i].real();
+        c[8 + 2 * i + 1] = E[i].imag();
+    }
+    out.resize(c.size());
+    for (size_t i = 0; i < c.size(); ++i) out[i] = (float)c[i];
+}
+
+void dense_hamiltonian(const Resolved &r, const double *field, std::vector<cd> &H) {
+    const int64_t D = r.D;
+    H.assign((size_t)(D * D), 0.0);
+    std::vector<int64_t> stride(r.nspins);
+    {
+        int64_t s = 1;
+        for (int i = r.nspins - 1; i >= 0; --i) { stride[i] = s; s *= r.mult[i]; }
+    }
+    std::vector<std::vector<cd>> ops;  // cached per (mult, axis)
+    auto op = [&](int mult, int axis) -> const std::vector<cd> & {
+        const size_t key = (size_t)mult * 3 + axis;
+        if (ops.size() <= key) ops.resize(key + 1);
+        if (ops[key].empty()) ops[key] = spin_op(mult, axis);
+        return ops[key];
+    };
+    auto digit = [&](int64_t idx, int s) { return (int)((idx / stride[s]) % r.mult[s]); };
+    // in MolSpin order, then permuted
+    auto add = [&](int64_t row, int64_t col, cd v) { H[(size_t)(r.perm[row] * D + r.perm[col])] += v; };
+    for (int64_t row = 0; row < D; ++row) {
+        add(row, row, r.shift);
+        for (const Term1 &t : r.t1) {
+            double v[3];
+            for (int a = 0; a < 3; ++a)
+                v[a] = (t.swept && field) ? t.Mz[3 * a] * field[0] + t.Mz[3 * a + 1] * field[1] + t.Mz[3 * a + 2] * field[2]
+                                          : t.c[a];
+            const int m = r.mult[t.s], dr = digit(row, t.s);
+            for (int a = 0; a < 3; ++a) {
+                if (v[a] == 0.0) continue;
+                const std::vector<cd> &S = op(m, a);
+                for (int dc = 0; dc < m; ++dc) {
+                    const cd x = S[(size_t)dr * m + dc];
+                    if (x != 0.0) add(row, row + (int64_t)(dc - dr) * stride[t.s], v[a] * x);
+                }
+            }
+        }
+        for (const Term2 &t : r.t2) {
+            const int m1 = r.mult[t.s1], m2 = r.mult[t.s2];
+            const int d1 = digit(row, t.s1), d2 = digit(row, t.s2);
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    const double mab = t.M[3 * a + b];
+                    if (mab == 0.0) continue;
+                    const std::vector<cd> &A = op(m1, a), &B = op(m2, b);
+                    for (int c1 = 0; c1 < m1; ++c1) {
+                        const cd x = A[(size_t)d1 * m1 + c1];
+                        if (x == 0.0) continue;
+                        for (int c2 = 0; c2 < m2; ++c2) {
+                            const cd y = B[(size_t)d2 * m2 + c2];
+                            if (y == 0.0) continue;
+                            add(row, row + (int64_t)(c1 - d1) * stride[t.s1] + (int64_t)(c2 - d2) * stride[t.s2],
+                                mab * x * y);
+                        }
+                    }
+                }
+        }
+    }
+}
+
+void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E) {
+    // Pade(13) coefficients and theta_13 (Higham, SIAM J. Matrix Anal. Appl. 26 (2005) 1179).
+    static const double b[14] = {64764752532480000.0, 32382376266240000.0, 7771770303897600.0, 1187353796428800.0,
+                                 129060195264000.0,   10559470521600.0,    670442572800.0,     33522128640.0,
+                                 1323241920.0,        40840800.0,          960960.0,           16380.0,
+                                 182.0,               1.0};
+    const double theta = 5.371920351148152;
+    double nrm = 0.0;
+    for (int j = 0; j < n; ++j) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += std::abs(A[(size_t)i * n + j]);
+        nrm = std::max(nrm, s);
+    }
+    int sq = 0;
+    if (nrm > theta) sq = (int)std::ceil(std::log2(nrm / theta));
+    const double f = std::ldexp(1.0, -sq);
+    std::vector<cd> X(A.size());
+    for (size_t i = 0; i < A.size(); ++i) X[i] = A[i] * f;
+    std::vector<cd> X2, X4, X6;
+    matmul(n, X, X, X2);
+    matmul(n, X2, X2, X4);
+    matmul(n, X4, X2, X6);
+    const size_t nn = (size_t)n * n;
+    std::vector<cd> t1(nn), t2(nn), U(nn), V(nn), tmp;
+    for (size_t i = 0; i < nn; ++i) t1[i] = b[13] * X6[i] + b[11] * X4[i] + b[9] * X2[i];
+    matmul(n, X6, t1, tmp);
+    for (size_t i = 0; i < nn; ++i) t2[i] = tmp[i] + b[7] * X6[i] + b[5] * X4[i] + b[3] * X2[i];
+    for (int i = 0; i < n; ++i) t2[(size_t)i * n + i] += b[1];
+    matmul(n, X, t2, U);
+    for (size_t i = 0; i < nn; ++i) t1[i] = b[12] * X6[i] + b[10] * X4[i] + b[8] * X2[i];
+    matmul(n, X6, t1, tmp);
+    for (size_t i = 0; i < nn; ++i) V[i] = tmp[i] + b[6] * X6[i] + b[4] * X4[i] + b[2] * X2[i];
+    for (int i = 0; i < n; ++i) V[(size_t)i * n + i] += b[0];
+    // solve (V - U) E = (V + U) by Gaussian elimination with partial pivoting
+    std::vector<cd> L(nn), R(nn);
+    for (size_t i = 0; i < nn; ++i) { L[i] = V[i] - U[i]; R[i] = V[i] + U[i]; }
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i)
+            if (std::abs(L[(size_t)i * n + k]) > std::abs(L[(size_t)p * n + k])) p = i;
+        if (p != k)
+            for (int j = 0; j < n; ++j) {
+                std::swap(L[(size_t)k * n + j], L[(size_t)p * n + j]);
+                std::swap(R[(size_t)k * n + j], R[(size_t)p * n + j]);
+            }
+        const cd piv = L[(size_t)k * n + k];
+        for (int i = k + 1; i < n; ++i) {
+            const cd fct = L[(size_t)i * n + k] / piv;
+            if (fct == 0.0) continue;
+            for (int j = k; j < n; ++j) L[(size_t)i * n + j] -= fct * L[(size_t)k * n + j];
+            for (int j = 0; j < n; ++j) R[(size_t)i * n + j] -= fct * R[(size_t)k * n + j];
+        }
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        for (int k = i + 1; k < n; ++k) {
+            const cd fct = L[(size_t)i * n + k];
+            if (fct == 0.0) continue;
+            for (int j = 0; j < n; ++j) R[(size_t)i * n + j] -= fct * R[(size_t)k * n + j];
+        }
+        const cd inv = 1.0 / L[(size_t)i * n + i];
+        for (int j = 0; j < n; ++j) R[(size_t)i * n + j] *= inv;
+    }
+    E.swap(R);
+    for (int i = 0; i < sq; ++i) {
+        matmul(n, E, E, tmp);
+        E.swap(tmp);
+    }
+}
+
+}  // namespace msg

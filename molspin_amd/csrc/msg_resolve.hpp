@@ -1,0 +1,80 @@
+// Host-side resolution of a structural spin-system description into the
+// device operator formats.  Restates the operator semantics of SpinAPI
+// (SpinSpace::InteractionOperator, SpinSpace_hamiltonians.cpp:438-777;
+// Spin::Tx/Ty/Tz, Spin.cpp:360-408; SpinSpace::GetState,
+// SpinSpace_states.cpp:626-700; reaction operators,
+// SpinSpace_transitions.cpp:201-229) without building Hilbert-space matrices
+// for the matrix-free path.
+#pragma once
+#include <array>
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "../../include/molspin_gpu.h"
+#include "msg_internal.hpp"
+
+namespace msg {
+
+using cd = std::complex<double>;
+using Mat4 = std::array<cd, 16>;  // 4x4 electron-pair matrix, row-major, c = 2a + b
+
+struct Term1 {       // H += c . S_s, c = Mz B (swept) or fixed c
+    int s;
+    bool swept;
+    double c[3];
+    double Mz[9];
+};
+struct Term2 {       // H += sum_ab M_ab S_{s1,a} S_{s2,b}
+    int s1, s2;
+    double M[9];
+};
+
+struct Resolved {
+    int nspins = 0;
+    std::vector<int> mult;
+    int64_t D = 0;
+    int e1 = -1, e2 = -1;         // the radical-pair electrons (declaration order)
+    std::vector<int> nuc;         // nuclei in device order (nucleus 0 least significant)
+    std::vector<int64_t> perm;    // MolSpin index -> device index
+    std::vector<Term1> t1;
+    std::vector<Term2> t2;
+    double shift = 0.0;
+    bool pair_ok = false;         // two spin-1/2 electrons, electron-only states
+    bool fast_ok = false;         // + all nuclei spin-1/2 and no nucleus-nucleus terms
+    bool has_ee = false;          // electron-electron couplings present
+    std::string why_not_fast;
+    // observables
+    std::vector<Mat4> P_states;        // every State object (electron-pair projector)
+    std::vector<Mat4> P_trans;         // transition source projectors
+    std::vector<double> rates;         // transition rates
+    double kmax = 0.0;
+    Mat4 K{};                          // Haberkorn K on the electron pair
+    bool symmetric = false;            // K = kappa * 1
+    double kappa = 0.0;
+    // relaxation
+    bool relax = false;
+    double relax_rate = 0.0;
+    // direct factors: rho0 = rho_e (x) 1/Z; columns evec_i (x) e_nu / sqrt(Z)
+    std::vector<std::array<cd, 4>> evec;
+    int64_t zdim = 0;
+};
+
+// Returns an empty string on success, else the reason.
+std::string resolve_system(const msg_system_desc *d, Resolved &r);
+
+// Per-item coefficient block of the matrix-free operator (layout in msg_internal.hpp).
+void fast_coefficients(const Resolved &r, const double *field, std::vector<float> &out, int &op_flags,
+                       int nuc_flags[kMaxNuclei]);
+
+// Dense D x D Hamiltonian (rad/ns) in device order, row-major complex.
+void dense_hamiltonian(const Resolved &r, const double *field, std::vector<cd> &H);
+
+// exp(A) for a dense complex matrix (row-major), Pade(13) with scaling and
+// squaring (Higham 2005).
+void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E);
+
+// BuildFreeEvolutionTimes (TaskHSGeneral.cpp:59-96).
+std::string free_times(double total, double dt, std::vector<double> &t);
+
+}  // namespace msg

@@ -1,0 +1,158 @@
+"""Device (HIP, through the C ABI) against the FP64 oracle on the same seeded
+inputs.  Tolerances follow the specification of the path: singlet yields to
+|dPhi| <= 1e-5 and P(t) traces to 1e-4 max-relative (against the trace maximum)."""
+import math
+
+import numpy as np
+import pytest
+
+from molspin_amd import abi
+from molspin_amd import workloads
+from molspin_amd.system import SpinSystem
+from oracle import binding as ref
+
+pytestmark = pytest.mark.gpu
+
+YIELD_TOL = 1e-5
+TRACE_TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from molspin_amd import backend
+    c = backend.Context(0)
+    yield c
+    c.close()
+
+
+def trace_err(dev, ora):
+    return np.abs(dev - ora).max() / np.abs(ora).max()
+
+
+@pytest.mark.parametrize("method", [abi.MSG_METHOD_AUTO, abi.MSG_METHOD_EXPONENTIAL])
+def test_config1_yields_and_trace(ctx, method):
+    """Config 1: D = 48 (spin-1 nucleus), 0-5000 ns at dt = 1 ns, dense path."""
+    s, cfg = workloads.config1()
+    p = s.pack()
+    sysd = ctx.upload(p)
+    prm = abi.run_params(total_time=cfg["total_time"], time_step=cfg["time_step"], method=method)
+    y, _ = sysd.propagate(prm)
+    yo, _ = ref.run_hsgeneral(p, abi.run_params(total_time=cfg["total_time"], time_step=cfg["time_step"],
+                                                method=abi.MSG_METHOD_KRYLOV), threads=8)
+    print("cfg1 yields dev", y[0], "oracle", yo[0])
+    assert np.abs(y - yo).max() <= YIELD_TOL
+    prm_t = abi.run_params(calculation=abi.MSG_CALC_TIMEEVOLUTION, observables=abi.MSG_OBS_STATE_POPULATIONS,
+                           total_time=cfg["total_time"], time_step=cfg["time_step"], method=method)
+    _, tr = sysd.propagate(prm_t, trace=True)
+    _, tro = ref.run_hsgeneral(p, prm_t, trace=True, threads=8)
+    assert tr.shape == tro.shape
+    assert trace_err(tr[0, :, 0], tro[0, :, 0]) <= TRACE_TOL
+
+
+def test_golden_system_dense(ctx):
+    """The reference's golden symmetric-decay system (D = 16) through HSGeneral:
+    a long finite-time run approaches the infinite-time golden value."""
+    from tests.test_oracle import golden_system
+    p = golden_system().pack()
+    sysd = ctx.upload(p)
+    prm = abi.run_params(observables=abi.MSG_OBS_STATE_POPULATIONS, total_time=2000.0, time_step=5.0)
+    # populations integrated with k e^{-kt}: state-population yields need the rate weight, so compare
+    # directly with the oracle's finite-time run instead
+    y, _ = sysd.propagate(prm)
+    yo, _ = ref.run_hsgeneral(p, prm)
+    assert np.abs(y - yo).max() <= YIELD_TOL * 100  # populations integrate to O(T) values
+
+
+def test_config2_orientations_short(ctx):
+    """Config 2 (D = 4096, anisotropic hyperfine) on 2 Fibonacci orientations,
+    all 1024 direct columns, 0-20 ns: matrix-free Lanczos path."""
+    s, cfg = workloads.config2(n_orientations=1024)
+    p = s.pack()
+    sysd = ctx.upload(p)
+    fields = [cfg["fields"][0], cfg["fields"][517]]
+    prm = abi.run_params(total_time=20.0, time_step=2.0, method=abi.MSG_METHOD_KRYLOV)
+    y, _ = sysd.propagate(prm, fields=fields)
+    yo, _ = ref.run_hsgeneral(p, prm, fields=fields, threads=8)
+    print("cfg2 short dev", y, "oracle", yo)
+    assert np.abs(y - yo).max() <= YIELD_TOL
+
+
+def test_config2_long_given_columns(ctx):
+    """Config 2 over the full 0-3000 ns grid (1500 steps) on 16 direct columns:
+    FP32 drift of the matrix-free path stays within the yield tolerance."""
+    s, cfg = workloads.config2(n_orientations=1024)
+    p = s.pack()
+    sysd = ctx.upload(p)
+    B = ref.direct_factors(p)
+    cols = B[:, ::64] * math.sqrt(B.shape[1] / 16.0)  # 16 columns, rho renormalised to trace 1
+    field = cfg["fields"][300]
+    prm = abi.run_params(total_time=3000.0, time_step=2.0, sampling=abi.MSG_SAMPLING_GIVEN)
+    y, _ = sysd.propagate(prm, fields=[field], columns=cols)
+    yo, _ = ref.run_hsgeneral(p, prm, fields=[field], columns=cols, threads=8)
+    print("cfg2 long dev", y, "oracle", yo, "diff", y - yo)
+    assert np.abs(y - yo).max() <= YIELD_TOL
+
+
+def st_system(n_a, n_b, rate, seed):
+    rng = workloads.SplitMix64(77, seed)
+    hf = [math.exp(rng.uniform(math.log(0.05), math.log(1.5))) for _ in range(n_a + n_b)]
+    s = workloads._radical_pair("st", ["1/2"] * n_a, ["1/2"] * n_b, hf, 1e-3)
+    s.add_st_dephasing("E1", "E2", rate)
+    return s
+
+
+@pytest.mark.parametrize("na,nb", [(2, 2), (5, 3)])
+def test_stochastic_trajectories_st_dephasing(ctx, na, nb):
+    """Config-3-style trajectories (coherent nuclear states, singlet electrons,
+    ST dephasing) with the counter-based flips shared by device and oracle:
+    per-trajectory parity, so yields agree deterministically."""
+    s = st_system(na, nb, 0.02, na * 10 + nb)
+    p = s.pack()
+    sysd = ctx.upload(p)
+    M = 64
+    cols = ref.coherent_samples(p, s.state_index("S"), 4242, M) / math.sqrt(M)
+    prm = abi.run_params(total_time=400.0, time_step=1.0, sampling=abi.MSG_SAMPLING_GIVEN, seed=99,
+                         method=abi.MSG_METHOD_KRYLOV)
+    y, _ = sysd.propagate(prm, columns=cols)
+    yo, _ = ref.run_hsgeneral(p, prm, columns=cols, threads=8)
+    print("st dev", y, "oracle", yo)
+    assert np.abs(y - yo).max() <= YIELD_TOL
+
+
+def test_singlet_only_recombination_dense(ctx):
+    """Unequal Haberkorn rates (singlet and T0 sinks only): the dense path
+    exponentiates -iH - K exactly."""
+    s = SpinSystem("asym")
+    s.add_spin("E1", "1/2", electron=True, tensor=2.0023)
+    s.add_spin("E2", "1/2", electron=True, tensor=2.0023)
+    for i, a in enumerate([0.9, 0.4, 1.3]):
+        s.add_spin("N%d" % i, "1/2", tensor=1.0)
+        s.add_hyperfine(["E1" if i < 2 else "E2"], ["N%d" % i], tensor=a, prefactor=0.001)
+    s.add_zeeman(["E1", "E2"], (0.0, 0.02, 0.05), prefactor=0.001)
+    s.add_radical_pair_states("E1", "E2")
+    s.add_transition("kS", "S", 3e-3)
+    s.add_transition("kT0", "T0", 1e-3)
+    s.set_initial(["S"])
+    p = s.pack()
+    sysd = ctx.upload(p)
+    prm = abi.run_params(total_time=800.0, time_step=1.0, method=abi.MSG_METHOD_EXPONENTIAL)
+    y, _ = sysd.propagate(prm)
+    yo, _ = ref.run_hsgeneral(p, prm)
+    assert np.abs(y - yo).max() <= YIELD_TOL
+
+
+def test_step_entry_matches_oracle(ctx):
+    """msg_step (HSPropagator::StepStochastic) on one interval."""
+    s = st_system(4, 3, 0.05, 5)
+    p = s.pack()
+    sysd = ctx.upload(p)
+    M = 8
+    cols = ref.coherent_samples(p, s.state_index("S"), 11, M) / math.sqrt(M)
+    prm = abi.run_params(sampling=abi.MSG_SAMPLING_GIVEN, method=abi.MSG_METHOD_KRYLOV, seed=5)
+    new = sysd.step(prm, cols, 2.0, step_index=0)
+    # oracle: one interval of the same loop, observed through state populations
+    prm_t = abi.run_params(calculation=abi.MSG_CALC_TIMEEVOLUTION, observables=abi.MSG_OBS_STATE_POPULATIONS,
+                           total_time=2.0, time_step=2.0, sampling=abi.MSG_SAMPLING_GIVEN, seed=5)
+    _, tro = ref.run_hsgeneral(p, prm_t, columns=cols, trace=True)
+    pops = [np.real(np.einsum("ij,ik,jk->", new.conj(), ref.projector(p, st), new)) for st in range(4)]
+    assert np.abs(np.array(pops) - tro[0, 1]).max() < 1e-6
