@@ -111,14 +111,14 @@ def cpu_baseline(packed, cfg, seconds):
     from molspin_amd import abi
     from oracle import binding as ref
     threads = min(16, os.cpu_count() or 1)
-    prm = abi.run_params(total_time=cfg["time_step"] * 400, time_step=cfg["time_step"], method=abi.MSG_METHOD_KRYLOV)
+    prm = abi.run_params(total_time=cfg["time_step"] * 4000, time_step=cfg["time_step"], method=abi.MSG_METHOD_KRYLOV)
     cols = 2 * threads
     steps = 4
     # calibrate to about `seconds` of work
     t0 = time.perf_counter()
     ref.run_columns(packed, prm, field=cfg["fields"][0], col_range=(0, cols), max_steps=steps, threads=threads)
     t1 = time.perf_counter() - t0
-    steps = max(4, min(400, int(steps * seconds / max(t1, 1e-3))))
+    steps = max(4, min(4000, int(steps * seconds / max(t1, 1e-3))))
     t0 = time.perf_counter()
     ref.run_columns(packed, prm, field=cfg["fields"][0], col_range=(0, cols), max_steps=steps, threads=threads)
     el = time.perf_counter() - t0

@@ -342,6 +342,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         HIPCHECK(hipMemcpyAsync(dU, Uh.data(), Uh.size() * sizeof(float), hipMemcpyHostToDevice, stream));
         a.U = dU;
         a.ustride = per;
+        a.renorm = r.symmetric ? 1 : 0;  // U unitary: keep each column at its initial norm
         b.n_units = n_items * ((b.ncols + 15) / 16);
     }
     a.n_units = b.n_units;

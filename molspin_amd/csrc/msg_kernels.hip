@@ -607,6 +607,7 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
     const StepArgs &a = *pa;
     extern __shared__ float dsm[];
     __shared__ double ys[kMaxObs], yc[kMaxObs];  // Kahan yield sums (thread 0)
+    __shared__ double s_col[2][64];              // per-wave column norms (double-buffered)
     const int Dp = a.dpad;
     float *Bre = dsm, *Bim = dsm + Dp * 16;
     double *scr = reinterpret_cast<double *>(dsm + 2 * Dp * 16);
@@ -615,8 +616,23 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
     const int q = lane >> 4, cl = lane & 15;
     const int ntile = Dp >> 4;
     const int tiles_per_item = (a.ncols + 15) >> 4;
-    int par = 0;
+    int par = 0, cpar = 0;
     auto SCR = [&]() { par ^= 1; return scr + par * 256; };
+    // squared norm of this lane's column: lanes cl + 16 q of every wave hold its rows
+    auto colnorm2 = [&](const floatx4 *xr_, const floatx4 *xi_) -> double {
+        float p = 0.f;
+#pragma unroll
+        for (int k = 0; k < kDenseMaxTiles; ++k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) p = fmaf(xr_[k][i], xr_[k][i], fmaf(xi_[k][i], xi_[k][i], p));
+        double d = p;
+        d += __shfl_xor(d, 16, 64);
+        d += __shfl_xor(d, 32, 64);
+        cpar ^= 1;
+        if (q == 0) s_col[cpar][wave * 16 + cl] = d;
+        __syncthreads();
+        return s_col[cpar][cl] + s_col[cpar][16 + cl] + s_col[cpar][32 + cl] + s_col[cpar][48 + cl];
+    };
     for (;;) {
         if (t == 0) s_unit = atomicAdd(a.queue, 1);
         __syncthreads();
@@ -643,6 +659,8 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
                 yc[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2 + 1] : 0.0;
             }
         const bool colok = col < a.ncols;
+        // target norm of the column for the renormalised unitary part (symmetric recombination)
+        const double cn0 = a.renorm ? sqrt(colnorm2(xr, xi)) : 0.0;
         for (int s = a.s_begin;; ++s) {
             const bool end = s == a.s_end;
             if (!end || s == a.nsamples - 1) {
@@ -708,6 +726,15 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
                         xi[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, xi[k], 0, 0, 0);
                         xi[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, xi[k], 0, 0, 0);
                     }
+                }
+            }
+            if (a.renorm) {
+                const double n2 = colnorm2(xr, xi);
+                const float f = n2 > 0.0 ? (float)(cn0 / sqrt(n2)) : 0.f;
+#pragma unroll
+                for (int k = 0; k < kDenseMaxTiles; ++k) {
+                    xr[k] *= f;
+                    xi[k] *= f;
                 }
             }
             if (a.relax && colok && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])

@@ -416,10 +416,10 @@ void dense_hamiltonian(const Resolved &r, const double *field, std::vector<cd> &
         int64_t s = 1;
         for (int i = r.nspins - 1; i >= 0; --i) { stride[i] = s; s *= r.mult[i]; }
     }
-    std::vector<std::vector<cd>> ops;  // cached per (mult, axis)
+    // cached per (mult, axis); sized once so references stay valid
+    std::vector<std::vector<cd>> ops((size_t)17 * 3);
     auto op = [&](int mult, int axis) -> const std::vector<cd> & {
         const size_t key = (size_t)mult * 3 + axis;
-        if (ops.size() <= key) ops.resize(key + 1);
         if (ops[key].empty()) ops[key] = spin_op(mult, axis);
         return ops[key];
     };
