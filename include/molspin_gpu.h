@@ -187,6 +187,22 @@ msg_status msg_system_dimension(const msg_system *sys, int64_t *dim);
 msg_status msg_query_footprint(const msg_system *sys, const msg_run_params *params,
                                int32_t n_items, int32_t n_columns, int64_t *bytes);
 
+enum {
+    MSG_TRACE_COHERENT = 0,   /* samplingmethod=coherent: spin-coherent product states */
+    MSG_TRACE_SUZ = 1         /* samplingmethod=suz: fixed State (x) random SU(Z) vector */
+};
+
+/* Trace samples of State `state` (SpinSpace::BuildTraceSamples,
+ * SpinSpace_states.cpp:376-568) drawn on the host from a counter-based stream
+ * (Philox4x32-10 keyed by `seed`): coherent - every spin in
+ * exp(-i phi Sz) exp(-i theta Sy)|+S>, theta = acos(1-2u), phi = 2 pi u, then
+ * projected on the State's support and normalised; suz - the State's vector (x)
+ * a normalised complex Gaussian vector over the open spins.  Columns are unit
+ * norm and in MolSpin order (D x m complex); scale by 1/sqrt(m) before
+ * propagating (HSStatePreparation.cpp:226).  Host-only: needs no device. */
+msg_status msg_trace_samples(const msg_system *sys, int32_t state, int32_t method, uint64_t seed,
+                             int32_t m, double *columns);
+
 /* Number of columns DIRECT sampling produces for `sys` (the rank of rho0). */
 msg_status msg_direct_columns(const msg_system *sys, int32_t *n_columns);
 

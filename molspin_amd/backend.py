@@ -19,6 +19,7 @@ EXPORTS = [
     "msg_system_dimension", "msg_query_footprint", "msg_direct_columns", "msg_observable_count",
     "msg_time_grid", "msg_propagate", "msg_step", "msg_server_create", "msg_server_destroy",
     "msg_server_submit", "msg_server_pump", "msg_server_collect", "msg_last_kernel_time",
+    "msg_trace_samples",
 ]
 
 
@@ -55,6 +56,7 @@ def lib():
         L.msg_server_pump.argtypes = [vp, C.c_int32, P(C.c_int32)]
         L.msg_server_collect.argtypes = [vp, C.c_int64, vp, P(C.c_int32)]
         L.msg_last_kernel_time.argtypes = [vp, P(C.c_double), P(C.c_int32)]
+        L.msg_trace_samples.argtypes = [vp, C.c_int32, C.c_int32, C.c_uint64, C.c_int32, vp]
         for name in EXPORTS:
             getattr(L, name).restype = C.c_int if name != "msg_last_error" else C.c_char_p
         _LIB = L
@@ -134,6 +136,13 @@ class System:
         n = C.c_int32()
         self.ctx.check(lib().msg_observable_count(self.h, C.byref(params), C.byref(n)))
         return n.value
+
+    def trace_samples(self, state, m, seed, method=0):
+        """Unit-norm trace samples of State `state` (D x m, MolSpin order); method 0
+        coherent, 1 SU(Z)."""
+        out = np.zeros(2 * self.dim * m)
+        self.ctx.check(lib().msg_trace_samples(self.h, int(state), int(method), int(seed), int(m), _ptr(out)))
+        return out.view(np.complex128).reshape((self.dim, m), order="F")
 
     def direct_columns(self):
         n = C.c_int32()

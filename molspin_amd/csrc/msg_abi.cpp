@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -154,7 +155,7 @@ int observable_count(const Resolved &r, const msg_run_params *p) {
 
 Path choose_path(const Resolved &r, const msg_run_params *p) {
     const bool dense_ok = r.D <= 512;
-    const bool lanczos_ok = r.fast_ok && r.symmetric && r.D >= 256 && r.D <= 8192;
+    const bool lanczos_ok = r.fast_ok && r.symmetric && r.D >= 256 && r.D <= (int64_t)4 << kMaxNuclei;
     if (p->method == MSG_METHOD_EXPONENTIAL) {
         if (!dense_ok) throw ArgError{MSG_UNSUPPORTED, "propagationmethod=exponential needs D <= 512 on the device"};
         return Path::Dense;
@@ -165,7 +166,7 @@ Path choose_path(const Resolved &r, const msg_run_params *p) {
     if (!r.symmetric)
         throw ArgError{MSG_UNSUPPORTED, "unequal recombination rates need D <= 512 (dense exponential path)"};
     if (!r.fast_ok) throw ArgError{MSG_UNSUPPORTED, "matrix-free path: " + r.why_not_fast};
-    throw ArgError{MSG_UNSUPPORTED, "Hilbert dimension outside the matrix-free range (256..8192)"};
+    throw ArgError{MSG_UNSUPPORTED, "Hilbert dimension outside the matrix-free range"};
 }
 
 const StepArgs *push_args(Batch &b, const StepArgs &a);
@@ -281,9 +282,16 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     // operator
     if (b.path == Path::Lanczos) {
         const int ng = D / 4;
-        if (ng <= 1024) { b.T = std::min(ng, 256); b.G = ng / b.T; }
-        else { b.G = 2; b.T = ng / 2; }
-        if (b.T < 64 || b.T > 1024 || b.G * b.T != ng || (b.G != 1 && b.G != 2 && b.G != 4))
+        // launch shape: groups per thread G, threads T (one workgroup per column);
+        // G = 0 selects the streaming variant once two Krylov vectors exceed LDS
+        if (ng <= 512) { b.T = ng; b.G = 1; }
+        else if (ng <= 2048) { b.G = 2; b.T = ng / 2; }
+        else { b.G = 0; b.T = 1024; }
+        if (const char *env = std::getenv("MSG_LANCZOS_G")) {  // launch-shape override (tuning runs)
+            const int g = std::atoi(env);
+            if ((g == 1 || g == 2) && ng / g >= 64 && ng / g <= 1024) { b.G = g; b.T = ng / g; }
+        }
+        if (b.G != 0 && (b.T < 64 || b.T > 1024 || b.G * b.T != ng))
             throw ArgError{MSG_UNSUPPORTED, "no matrix-free launch shape for this dimension"};
         a.nnuc = (int)r.nuc.size();
         std::vector<float> all, one;
@@ -370,10 +378,11 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     }
     // launch geometry
     if (b.path == Path::Lanczos) {
-        const size_t lds = (size_t)8 * D + 4096;
+        const size_t lds = b.G == 0 ? 0 : (size_t)16 * D + 4096;
         const int occ = std::max(1, lanczos_occupancy(b.G, b.T, lds));
         b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ));
-        a.basis = b.basis.alloc<float>((size_t)b.slots * a.m * 2 * D);
+        const int vecs = b.G == 0 ? a.m + 2 : a.m;  // streaming variant: + residual and state
+        a.basis = b.basis.alloc<float>((size_t)b.slots * vecs * 2 * D);
         a.norm0 = b.norm0.alloc<double>(b.n_units);
         if (a.renorm) HIPCHECK(launch_init_norms(a, push_args(b, a), stream));
     } else {
@@ -551,6 +560,19 @@ msg_status msg_system_dimension(const msg_system *sys, int64_t *dim) {
 msg_status msg_direct_columns(const msg_system *sys, int32_t *n) {
     if (!sys || !n) return MSG_INVALID_ARG;
     *n = (int32_t)(sys->r.zdim * (int64_t)sys->r.evec.size());
+    return MSG_OK;
+}
+
+msg_status msg_trace_samples(const msg_system *sys, int32_t state, int32_t method, uint64_t seed, int32_t m,
+                             double *columns) {
+    if (!sys || !columns || m < 1 || (method != MSG_TRACE_COHERENT && method != MSG_TRACE_SUZ))
+        return fail(sys ? sys->ctx : nullptr, MSG_INVALID_ARG, "bad trace-sample arguments");
+    try {
+        const std::string e = trace_samples(sys->r, state, method, seed, m, columns);
+        if (!e.empty()) return fail(sys->ctx, MSG_INVALID_ARG, e);
+    } catch (const std::bad_alloc &) {
+        return fail(sys->ctx, MSG_OOM_ADMISSION, "host allocation failed");
+    }
     return MSG_OK;
 }
 

@@ -166,7 +166,7 @@ __device__ __forceinline__ double obs_from_rho(const float *P, const double *R) 
 
 // Samples every observable of one sample from the block-reduced density and
 // accumulates the Kahan-compensated yield and the trace (thread 0 only).
-__device__ __forceinline__ void record_sample(const StepArgs &a, int u, int s, const double (&R)[16], double *ys,
+__device__ __forceinline__ void record_sample(const StepArgs &a, int u, int s, const double *R, double *ys,
                                               double *yc) {
     for (int c = 0; c < a.nobs; ++c) {
         const double v = obs_from_rho(a.obs + 32 * c, R);
@@ -224,17 +224,19 @@ __device__ __forceinline__ void st_flip(C4 &x) {
     x.c[3] = make_float2(-x.c[3].x, -x.c[3].y);
 }
 
-// exp(-i dt T) e1 for the real symmetric tridiagonal T (alpha, beta) of order
-// meff, in FP64 by a Chebyshev expansion on the Gershgorin interval, summed with
+// exp(-i dt T) e1 for the real symmetric tridiagonal T of order meff, in FP64,
+// by a Chebyshev expansion on the Gershgorin interval of T, summed with
 // Clenshaw's recurrence while Miller's backward recurrence generates the Bessel
-// coefficients J_k(dt*rho) in the same order.  Executed by one wavefront; lane
-// i < meff writes component i to cf (re, im).
-__device__ void cheb_expm_wave(int meff, double dt, const double *al, const double *be, double *cf, int *status) {
+// coefficients J_k(dt*rho) in the same order.  Every wavefront evaluates it
+// redundantly: lane i holds ai = alpha_i, bi = beta_i (coupling of i and i+1),
+// and receives component i of the result in (re, im).  Lanes >= meff are zero.
+__device__ __forceinline__ void cheb_expm_lanes(int meff, double dt, double ai_in, double bi_in, double &re,
+                                                double &im, int *status) {
     const int i = threadIdx.x & 63;
     const bool on = i < meff;
-    const double ai = on ? al[i] : 0.0;
-    const double bl = (on && i > 0) ? be[i - 1] : 0.0;
-    const double br = (i < meff - 1) ? be[i] : 0.0;
+    const double ai = on ? ai_in : 0.0;
+    const double br = (i < meff - 1) ? bi_in : 0.0;
+    const double bl = __shfl_up(br, 1, 64) * (i > 0 ? 1.0 : 0.0);
     double lo = on ? ai - fabs(bl) - fabs(br) : 1e300;
     double hi = on ? ai + fabs(bl) + fabs(br) : -1e300;
 #pragma unroll
@@ -245,17 +247,15 @@ __device__ void cheb_expm_wave(int meff, double dt, const double *al, const doub
     const double sigma = 0.5 * (lo + hi);
     const double rho = 0.5 * (hi - lo);
     const double z = dt * rho;
-    double re, im;
+    const double e = (i == 0) ? 1.0 : 0.0;
+    double rr, ri;
     if (z < 1e-6) {
-        // second-order Taylor: error < z^3/6
-        const double x0 = on ? (ai - sigma) : 0.0;  // (T - sigma) e1 lives on lanes 0, 1
-        const double e = (i == 0) ? 1.0 : 0.0;
-        const double te = (i == 0) ? x0 : (i == 1 ? bl : 0.0);
-        // (T - sigma)^2 e1 = (T - sigma) te
+        // second order Taylor of exp(-i dt (T - sigma)) e1; error < z^3/6
+        const double te = (i == 0) ? (ai - sigma) : (i == 1 ? bl : 0.0);
         const double tl = __shfl_up(te, 1, 64), tr = __shfl_down(te, 1, 64);
         const double t2 = (on ? (ai - sigma) * te : 0.0) + (i > 0 ? bl * tl : 0.0) + br * tr;
-        re = e - 0.5 * dt * dt * t2;
-        im = -dt * te;
+        rr = e - 0.5 * dt * dt * t2;
+        ri = -dt * te;
     } else {
         int K = (int)ceil(1.3 * z) + 18;
         if (K > kChebMax) {
@@ -264,16 +264,12 @@ __device__ void cheb_expm_wave(int meff, double dt, const double *al, const doub
         }
         const double xd = on ? (ai - sigma) / rho : 0.0;
         const double xl = bl / rho, xr = br / rho;
-        const double e = (i == 0) ? 1.0 : 0.0;
-        // Clenshaw state b_{k+1}, b_{k+2} (complex); Miller state j_{k+1}, j_k.
         double b1r = 0, b1i = 0, b2r = 0, b2i = 0;
         double jn1 = 0.0, jn = 1e-280, S = 0.0;
         const int N = K + 24;
         for (int k = N; k >= 0; --k) {
-            // jn holds the unnormalised J_k
             if (k <= K) {
-                // c_k = (2 - delta_k0) (-i)^k J_k
-                const double mag = (k == 0 ? 1.0 : 2.0) * jn;
+                const double mag = (k == 0 ? 1.0 : 2.0) * jn;  // c_k = (2 - delta_k0) (-i)^k J_k
                 double cr = 0.0, ci = 0.0;
                 switch (k & 3) {
                     case 0: cr = mag; break;
@@ -281,21 +277,14 @@ __device__ void cheb_expm_wave(int meff, double dt, const double *al, const doub
                     case 2: cr = -mag; break;
                     default: ci = mag; break;
                 }
-                if (k > 0) {
-                    // b_k = c_k e + 2 x b_{k+1} - b_{k+2}
-                    const double lr = __shfl_up(b1r, 1, 64), li = __shfl_up(b1i, 1, 64);
-                    const double rr = __shfl_down(b1r, 1, 64), ri = __shfl_down(b1i, 1, 64);
-                    const double xbr = xd * b1r + (i > 0 ? xl * lr : 0.0) + xr * rr;
-                    const double xbi = xd * b1i + (i > 0 ? xl * li : 0.0) + xr * ri;
-                    const double nr = cr * e + 2.0 * xbr - b2r;
-                    const double ni = ci * e + 2.0 * xbi - b2i;
+                const double lr = __shfl_up(b1r, 1, 64), li = __shfl_up(b1i, 1, 64);
+                const double qr = __shfl_down(b1r, 1, 64), qi = __shfl_down(b1i, 1, 64);
+                const double xbr = xd * b1r + (i > 0 ? xl * lr : 0.0) + xr * qr;
+                const double xbi = xd * b1i + (i > 0 ? xl * li : 0.0) + xr * qi;
+                if (k > 0) {  // b_k = c_k e + 2 x b_{k+1} - b_{k+2}
+                    const double nr = cr * e + 2.0 * xbr - b2r, ni = ci * e + 2.0 * xbi - b2i;
                     b2r = b1r; b2i = b1i; b1r = nr; b1i = ni;
-                } else {
-                    // result = c_0 e + x b_1 - b_2
-                    const double lr = __shfl_up(b1r, 1, 64), li = __shfl_up(b1i, 1, 64);
-                    const double rr = __shfl_down(b1r, 1, 64), ri = __shfl_down(b1i, 1, 64);
-                    const double xbr = xd * b1r + (i > 0 ? xl * lr : 0.0) + xr * rr;
-                    const double xbi = xd * b1i + (i > 0 ? xl * li : 0.0) + xr * ri;
+                } else {      // f = c_0 e + x b_1 - b_2
                     b1r = cr * e + xbr - b2r;
                     b1i = ci * e + xbi - b2i;
                 }
@@ -305,36 +294,39 @@ __device__ void cheb_expm_wave(int meff, double dt, const double *al, const doub
                 const double jm = (2.0 * k / z) * jn - jn1;  // J_{k-1}
                 jn1 = jn;
                 jn = jm;
-                if (fabs(jn) > 1e200) {  // rescale everything linear in J
+                if (fabs(jn) > 1e200) {
                     jn *= 1e-200; jn1 *= 1e-200; S *= 1e-200;
                     b1r *= 1e-200; b1i *= 1e-200; b2r *= 1e-200; b2i *= 1e-200;
                 }
             }
         }
-        // normalise (J_0 + 2 sum J_2k = 1) and apply the shift phase e^{-i dt sigma}
-        const double inv = 1.0 / S;
-        const double pr = cos(dt * sigma), pi = -sin(dt * sigma);
-        const double rr0 = b1r * inv, ri0 = b1i * inv;
-        re = rr0 * pr - ri0 * pi;
-        im = rr0 * pi + ri0 * pr;
+        rr = b1r / S;
+        ri = b1i / S;
     }
-    if (on) {
-        cf[2 * i] = re;
-        cf[2 * i + 1] = im;
-    }
+    // shift phase e^{-i dt sigma}
+    const double pr = cos(dt * sigma), pi = -sin(dt * sigma);
+    re = on ? rr * pr - ri * pi : 0.0;
+    im = on ? rr * pi + ri * pr : 0.0;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 template <int G, int TMAX>
-__global__ __launch_bounds__(TMAX) void k_lanczos(const StepArgs *__restrict__ pa) {
+__global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict__ pa) {
     const StepArgs &a = *pa;
     extern __shared__ float4 smem[];
-    const int T = blockDim.x, t = threadIdx.x;
+    const int T = blockDim.x, t = threadIdx.x, lane = t & 63;
     const int ng = G * T;
-    float4 *LA = smem;
-    float4 *LB = smem + ng;
-    double *scr = reinterpret_cast<double *>(smem + 2 * ng);  // 2 x 16 x 16 doubles
-    __shared__ double s_al[kMaxKrylov], s_be[kMaxKrylov], s_cf[2 * kMaxKrylov];
-    __shared__ double ys[kMaxObs], yc[kMaxObs];  // Kahan yield sums (thread 0)
+    // two LDS buffers (A/B planes each): v_j and v_{j-1}, swapped every iteration
+    float4 *LA[2] = {smem, smem + 2 * ng};
+    float4 *LB[2] = {smem + ng, smem + 3 * ng};
+    double *scr = reinterpret_cast<double *>(smem + 4 * ng);  // 2 x 16 x 16 doubles
+    __shared__ double ys[kMaxObs], yc[kMaxObs];                // Kahan yield sums (thread 0)
     __shared__ int s_unit;
     const int D = a.dim;
     const int D4 = D >> 2;
@@ -398,8 +390,7 @@ __global__ __launch_bounds__(TMAX) void k_lanczos(const StepArgs *__restrict__ p
 #pragma unroll
                 for (int g = 0; g < G; ++g) st_flip(psi[g]);
 
-            // ---- Lanczos basis of H from psi (KrylovExpmGeneral, SpinSpace_operators.cpp:1257-1408).
-            // v_j lives in LDS, v_{j-1} in the basis store, the residual w in registers.
+            // ---- Lanczos basis of H from psi (KrylovExpmGeneral, SpinSpace_operators.cpp:1257-1408)
             double nn[1] = {0.0};
 #pragma unroll
             for (int g = 0; g < G; ++g)
@@ -418,60 +409,73 @@ __global__ __launch_bounds__(TMAX) void k_lanczos(const StepArgs *__restrict__ p
 #pragma unroll
                         for (int c = 0; c < 4; ++c) v0.c[c] = make_float2(psi[g].c[c].x * inv, psi[g].c[c].y * inv);
                         const int nu = g * T + t;
-                        store_group(LA, LB, nu, v0);
+                        store_group(LA[0], LB[0], nu, v0);
                         store_group(bA, bA + D4, nu, v0);
                     }
                 }
+                double my_al = 0.0, my_be = 0.0;  // lane j keeps alpha_j and beta_j
                 double bprev = 0.0;
                 int meff = meff0;
+                int cur = 0;
                 for (int j = 0; j < meff0; ++j) {
-                    __syncthreads();
+                    __syncthreads();  // v_j complete in buffer `cur`
+                    const float4 *cA = LA[cur], *cB = LB[cur];
+                    float4 *pA = LA[cur ^ 1], *pB = LB[cur ^ 1];
+                    const float fb = (float)bprev;
+                    // w = H v_j - beta_{j-1} v_{j-1}; one reduction gives alpha and |w|^2
                     C4 w[G];
+                    double red[2] = {0.0, 0.0};
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
                         const int nu = g * T + t;
-                        w[g] = apply_h(a, co, nu, load_group(LA, LB, nu), LA, LB);
-                    }
-                    double al[1] = {0.0};
-#pragma unroll
-                    for (int g = 0; g < G; ++g) {
-                        const C4 vj = load_group(LA, LB, g * T + t);
-#pragma unroll
-                        for (int c = 0; c < 4; ++c)
-                            al[0] += (double)vj.c[c].x * w[g].c[c].x + (double)vj.c[c].y * w[g].c[c].y;
-                    }
-                    block_sum<1>(al, SCR());
-                    const float fa = (float)al[0], fb = (float)bprev;
-                    if (t == 0) s_al[j] = al[0];
-                    {
-                        const float4 *pA = reinterpret_cast<const float4 *>(bas + (size_t)(j > 0 ? j - 1 : 0) * 2 * D);
-#pragma unroll
-                        for (int g = 0; g < G; ++g) {
-                            const int nu = g * T + t;
-                            const C4 vj = load_group(LA, LB, nu);
-                            const C4 vp = j > 0 ? load_group(pA, pA + D4, nu) : C4{};
+                        const C4 vj = load_group(cA, cB, nu);
+                        w[g] = apply_h(a, co, nu, vj, cA, cB);
+                        if (j > 0) {
+                            const C4 vp = load_group(pA, pB, nu);
 #pragma unroll
                             for (int c = 0; c < 4; ++c) {
-                                const float px = j > 0 ? vp.c[c].x : 0.f, py = j > 0 ? vp.c[c].y : 0.f;
-                                w[g].c[c].x = fmaf(-fa, vj.c[c].x, fmaf(-fb, px, w[g].c[c].x));
-                                w[g].c[c].y = fmaf(-fa, vj.c[c].y, fmaf(-fb, py, w[g].c[c].y));
+                                w[g].c[c].x = fmaf(-fb, vp.c[c].x, w[g].c[c].x);
+                                w[g].c[c].y = fmaf(-fb, vp.c[c].y, w[g].c[c].y);
                             }
+                        }
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            red[0] += (double)vj.c[c].x * w[g].c[c].x + (double)vj.c[c].y * w[g].c[c].y;
+                            red[1] += (double)w[g].c[c].x * w[g].c[c].x + (double)w[g].c[c].y * w[g].c[c].y;
+                        }
+                    }
+                    block_sum<2>(red, SCR());
+                    const double alpha = red[0];
+                    if (lane == j) my_al = alpha;
+                    const float fa = (float)alpha;
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const C4 vj = load_group(cA, cB, g * T + t);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            w[g].c[c].x = fmaf(-fa, vj.c[c].x, w[g].c[c].x);
+                            w[g].c[c].y = fmaf(-fa, vj.c[c].y, w[g].c[c].y);
                         }
                     }
                     if (j == meff0 - 1) break;
-                    double bb[1] = {0.0};
+                    // |w - alpha v|^2 = |w|^2 - alpha^2 for orthonormal v; recomputed when it cancels
+                    double beta2 = red[1] - alpha * alpha;
+                    if (!(beta2 > 0.02 * red[1])) {
+                        double bb[1] = {0.0};
 #pragma unroll
-                    for (int g = 0; g < G; ++g)
+                        for (int g = 0; g < G; ++g)
 #pragma unroll
-                        for (int c = 0; c < 4; ++c)
-                            bb[0] += (double)w[g].c[c].x * w[g].c[c].x + (double)w[g].c[c].y * w[g].c[c].y;
-                    block_sum<1>(bb, SCR());
-                    const double beta = sqrt(bb[0]);
+                            for (int c = 0; c < 4; ++c)
+                                bb[0] += (double)w[g].c[c].x * w[g].c[c].x + (double)w[g].c[c].y * w[g].c[c].y;
+                        block_sum<1>(bb, SCR());
+                        beta2 = bb[0];
+                    }
+                    const double beta = sqrt(fmax(beta2, 0.0));
                     if (!(beta > 1e-30)) {  // invariant Krylov subspace (happy breakdown)
                         meff = j + 1;
                         break;
                     }
-                    if (t == 0) s_be[j] = beta;
+                    if (lane == j) my_be = beta;
                     const float ib = (float)(1.0 / beta);
                     float4 *bA = reinterpret_cast<float4 *>(bas + (size_t)(j + 1) * 2 * D);
 #pragma unroll
@@ -479,22 +483,23 @@ __global__ __launch_bounds__(TMAX) void k_lanczos(const StepArgs *__restrict__ p
 #pragma unroll
                         for (int c = 0; c < 4; ++c) w[g].c[c] = make_float2(w[g].c[c].x * ib, w[g].c[c].y * ib);
                         const int nu = g * T + t;
-                        store_group(LA, LB, nu, w[g]);
+                        store_group(pA, pB, nu, w[g]);  // v_{j+1} replaces v_{j-1} (own elements only)
                         store_group(bA, bA + D4, nu, w[g]);
                     }
                     bprev = beta;
+                    cur ^= 1;
                 }
-                __syncthreads();
-                if (t < 64) cheb_expm_wave(meff, dt, s_al, s_be, s_cf, a.status);
-                __syncthreads();
+                // every wavefront evaluates the small exponential for itself
+                double cre, cim;
+                cheb_expm_lanes(meff, dt, my_al, my_be, cre, cim, a.status);
                 // psi = beta * V c   (beta = norm0 when renormalising the unitary part)
-                const float sc = (float)(a.renorm ? n0 : beta0);
+                const double sc = a.renorm ? n0 : beta0;
 #pragma unroll
                 for (int g = 0; g < G; ++g)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) psi[g].c[c] = make_float2(0.f, 0.f);
                 for (int j = 0; j < meff; ++j) {
-                    const float2 cj = make_float2(sc * (float)s_cf[2 * j], sc * (float)s_cf[2 * j + 1]);
+                    const float2 cj = make_float2((float)(sc * readlane_f64(cre, j)), (float)(sc * readlane_f64(cim, j)));
                     const float4 *bA = reinterpret_cast<const float4 *>(bas + (size_t)j * 2 * D);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
@@ -526,6 +531,205 @@ __global__ __launch_bounds__(TMAX) void k_lanczos(const StepArgs *__restrict__ p
 #pragma unroll
             for (int c = 0; c < 4; ++c) bad |= !isfinite(psi[g].c[c].x) || !isfinite(psi[g].c[c].y);
         if (bad) atomicOr(a.status, 1);
+    }
+}
+
+// Matrix-free Lanczos for Hilbert dimensions whose Krylov vector exceeds LDS
+// (D >= 16384): one 1024-thread workgroup per column; the state, the residual
+// and the Krylov basis stream through global memory (per-slot scratch), and
+// the operator gathers neighbour groups from the basis vector in L2/MALL.
+// Per iteration: pass A computes w = H v_j - beta v_{j-1} with alpha and |w|^2
+// in one reduction; pass B writes v_{j+1} = (w - alpha v_j) / beta.
+__global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict__ pa) {
+    const StepArgs &a = *pa;
+    __shared__ double scr[2 * 17 * 16];  // block_sum of up to 17 values, double-buffered
+    __shared__ double ys[kMaxObs], yc[kMaxObs];
+    __shared__ int s_unit;
+    const int T = blockDim.x, t = threadIdx.x, lane = t & 63;
+    const int D = a.dim, D4 = D >> 2, ng = D >> 2;
+    // per-slot scratch: basis (m vectors), residual W, state P
+    float *bas = a.basis + (size_t)blockIdx.x * (a.m + 2) * 2 * D;
+    float4 *WA = reinterpret_cast<float4 *>(bas + (size_t)a.m * 2 * D), *WB = WA + D4;
+    float4 *PA = reinterpret_cast<float4 *>(bas + (size_t)(a.m + 1) * 2 * D), *PB = PA + D4;
+    auto BA = [&](int j) { return reinterpret_cast<float4 *>(bas + (size_t)j * 2 * D); };
+    const int meff0 = a.m < D ? a.m : D;
+    int par = 0;
+    auto SCR = [&]() { par ^= 1; return scr + par * 17 * 16; };
+    for (;;) {
+        if (t == 0) s_unit = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        __syncthreads();
+        if (u >= a.n_units) break;
+        const int item = u / a.ncols, col = u - item * a.ncols;
+        const float *co = a.coef + (size_t)item * a.coef_stride;
+        for (int nu = t; nu < ng; nu += T) {
+            C4 x;
+            if (a.init_mode == 0) {
+                const int which = col / a.zdim, ncol = col - which * a.zdim;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    x.c[c] = (nu == ncol) ? make_float2(a.evec[which * 8 + 2 * c], a.evec[which * 8 + 2 * c + 1])
+                                          : make_float2(0.f, 0.f);
+            } else {
+                const float *src = a.init_mode == 1 ? a.given + (size_t)col * 2 * D : a.state + (size_t)u * 2 * D;
+                const float4 *sA = reinterpret_cast<const float4 *>(src);
+                x = load_group(sA, sA + D4, nu);
+            }
+            store_group(PA, PB, nu, x);
+        }
+        if (t == 0)
+            for (int c = 0; c < a.nobs; ++c) {
+                ys[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2] : 0.0;
+                yc[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2 + 1] : 0.0;
+            }
+        const double n0 = a.norm0 ? a.norm0[u] : 0.0;
+        for (int s = a.s_begin;; ++s) {
+            const bool end = (s == a.s_end);
+            const bool flip0 = !end && a.relax &&
+                               philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s];
+            // observables at sample s, then the first relaxation half-step and |psi|^2
+            {
+                float Rf[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) Rf[k] = 0.f;
+                double nn = 0.0;
+                for (int nu = t; nu < ng; nu += T) {
+                    C4 x = load_group(PA, PB, nu);
+                    accum_rho(x, Rf);
+                    if (flip0) {
+                        st_flip(x);
+                        store_group(PA, PB, nu, x);
+                    }
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) nn += (double)x.c[c].x * x.c[c].x + (double)x.c[c].y * x.c[c].y;
+                }
+                double R[17];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) R[k] = Rf[k];
+                R[16] = nn;
+                block_sum<17>(R, SCR());
+                if (t == 0 && (!end || s == a.nsamples - 1)) record_sample(a, u, s, R, ys, yc);
+                if (end) break;
+                const double dt = a.dts[s];
+                const double beta0 = sqrt(R[16]);
+                if (beta0 > 0.0) {
+                    const float inv = (float)(1.0 / beta0);
+                    for (int nu = t; nu < ng; nu += T) {
+                        C4 x = load_group(PA, PB, nu);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) x.c[c] = make_float2(x.c[c].x * inv, x.c[c].y * inv);
+                        store_group(BA(0), BA(0) + D4, nu, x);
+                    }
+                    double my_al = 0.0, my_be = 0.0, bprev = 0.0;
+                    int meff = meff0;
+                    for (int j = 0; j < meff0; ++j) {
+                        __syncthreads();  // v_j written by the whole workgroup
+                        const float4 *vA = BA(j), *vB = BA(j) + D4;
+                        const float4 *pA = BA(j > 0 ? j - 1 : 0), *pB = pA + D4;
+                        const float fb = (float)bprev;
+                        double red[2] = {0.0, 0.0};
+                        for (int nu = t; nu < ng; nu += T) {
+                            const C4 vj = load_group(vA, vB, nu);
+                            C4 w = apply_h(a, co, nu, vj, vA, vB);
+                            if (j > 0) {
+                                const C4 vp = load_group(pA, pB, nu);
+#pragma unroll
+                                for (int c = 0; c < 4; ++c) {
+                                    w.c[c].x = fmaf(-fb, vp.c[c].x, w.c[c].x);
+                                    w.c[c].y = fmaf(-fb, vp.c[c].y, w.c[c].y);
+                                }
+                            }
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                red[0] += (double)vj.c[c].x * w.c[c].x + (double)vj.c[c].y * w.c[c].y;
+                                red[1] += (double)w.c[c].x * w.c[c].x + (double)w.c[c].y * w.c[c].y;
+                            }
+                            store_group(WA, WB, nu, w);
+                        }
+                        block_sum<2>(red, SCR());
+                        const double alpha = red[0];
+                        if (lane == j) my_al = alpha;
+                        if (j == meff0 - 1) break;
+                        const float fa = (float)alpha;
+                        double beta2 = red[1] - alpha * alpha;
+                        if (!(beta2 > 0.02 * red[1])) {
+                            double bb[1] = {0.0};
+                            for (int nu = t; nu < ng; nu += T) {
+                                const C4 vj = load_group(vA, vB, nu);
+                                const C4 w = load_group(WA, WB, nu);
+#pragma unroll
+                                for (int c = 0; c < 4; ++c) {
+                                    const float wx = fmaf(-fa, vj.c[c].x, w.c[c].x), wy = fmaf(-fa, vj.c[c].y, w.c[c].y);
+                                    bb[0] += (double)wx * wx + (double)wy * wy;
+                                }
+                            }
+                            block_sum<1>(bb, SCR());
+                            beta2 = bb[0];
+                        }
+                        const double beta = sqrt(fmax(beta2, 0.0));
+                        if (!(beta > 1e-30)) {
+                            meff = j + 1;
+                            break;
+                        }
+                        if (lane == j) my_be = beta;
+                        const float ib = (float)(1.0 / beta);
+                        float4 *nA = BA(j + 1), *nB = nA + D4;
+                        for (int nu = t; nu < ng; nu += T) {
+                            const C4 vj = load_group(vA, vB, nu);
+                            C4 w = load_group(WA, WB, nu);
+#pragma unroll
+                            for (int c = 0; c < 4; ++c)
+                                w.c[c] = make_float2(fmaf(-fa, vj.c[c].x, w.c[c].x) * ib, fmaf(-fa, vj.c[c].y, w.c[c].y) * ib);
+                            store_group(nA, nB, nu, w);
+                        }
+                        bprev = beta;
+                    }
+                    double cre, cim;
+                    cheb_expm_lanes(meff, dt, my_al, my_be, cre, cim, a.status);
+                    const double sc = a.renorm ? n0 : beta0;
+                    for (int nu = t; nu < ng; nu += T) {
+                        C4 x;
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) x.c[c] = make_float2(0.f, 0.f);
+                        for (int j = 0; j < meff; ++j) {
+                            const float2 cj = make_float2((float)(sc * readlane_f64(cre, j)), (float)(sc * readlane_f64(cim, j)));
+                            const C4 vj = load_group(BA(j), BA(j) + D4, nu);
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) x.c[c] = cfma(cj, vj.c[c], x.c[c]);
+                        }
+                        if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
+                            st_flip(x);
+                        store_group(PA, PB, nu, x);
+                    }
+                } else if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) <
+                                          a.thr[s]) {
+                    for (int nu = t; nu < ng; nu += T) {
+                        C4 x = load_group(PA, PB, nu);
+                        st_flip(x);
+                        store_group(PA, PB, nu, x);
+                    }
+                }
+                __syncthreads();  // psi complete before the next sample
+            }
+        }
+        if (t == 0)
+            for (int c = 0; c < a.nobs; ++c) {
+                a.yacc[((size_t)u * a.nobs + c) * 2] = ys[c];
+                a.yacc[((size_t)u * a.nobs + c) * 2 + 1] = yc[c];
+            }
+        bool bad = false;
+        for (int nu = t; nu < ng; nu += T) {
+            const C4 x = load_group(PA, PB, nu);
+            if (a.state) {
+                float4 *dA = reinterpret_cast<float4 *>(a.state + (size_t)u * 2 * D);
+                store_group(dA, dA + D4, nu, x);
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bad |= !isfinite(x.c[c].x) || !isfinite(x.c[c].y);
+        }
+        if (bad) atomicOr(a.status, 1);
+        __syncthreads();
     }
 }
 
@@ -775,11 +979,15 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
 // ---------------------------------------------------------------------------
 // Launchers (called from the host part of the library)
 // ---------------------------------------------------------------------------
-#define MSG_LANCZOS_VARIANTS(X) X(1, 256) X(2, 256) X(4, 256) X(2, 1024)
+#define MSG_LANCZOS_VARIANTS(X) X(1, 256) X(1, 512) X(1, 1024) X(2, 512) X(2, 1024)
 
 hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_thread, int threads, int slots,
                           hipStream_t st) {
-    const size_t lds = (size_t)2 * a.dim * sizeof(float) + 2 * 256 * sizeof(double);
+    if (group_per_thread == 0) {  // streaming variant for large D
+        hipLaunchKernelGGL(k_lanczos_big, dim3(slots), dim3(1024), 0, st, da);
+        return hipGetLastError();
+    }
+    const size_t lds = (size_t)4 * a.dim * sizeof(float) + 2 * 256 * sizeof(double);
 #define X(G_, T_)                                                                        \
     if (group_per_thread == G_ && threads <= T_) {                                       \
         hipLaunchKernelGGL((k_lanczos<G_, T_>), dim3(slots), dim3(threads), lds, st, da); \
@@ -813,6 +1021,8 @@ hipError_t launch_dense(const StepArgs &a, const StepArgs *da, int slots, hipStr
 
 int lanczos_occupancy(int group_per_thread, int threads, size_t lds) {
     int nb = 0;
+    if (group_per_thread == 0)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big, 1024, 0) == hipSuccess ? nb : 1;
 #define X(G_, T_)                                                                                        \
     if (group_per_thread == G_ && threads <= T_)                                                         \
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos<G_, T_>, threads, lds) == hipSuccess \

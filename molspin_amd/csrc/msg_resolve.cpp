@@ -150,6 +150,135 @@ std::string free_times(double total, double dt, std::vector<double> &t) {
     return "";
 }
 
+
+namespace {
+inline void philox4x32(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint64_t key, uint32_t o[4]) {
+    uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    o[0] = c0; o[1] = c1; o[2] = c2; o[3] = c3;
+}
+// two uniforms in (0, 1) with 53-bit resolution from one Philox block
+inline void two_uniforms(const uint32_t o[4], double &u1, double &u2) {
+    u1 = ((double)(((uint64_t)o[0] << 21) ^ (o[1] >> 11)) + 0.5) * 0x1p-53;
+    u2 = ((double)(((uint64_t)o[2] << 21) ^ (o[3] >> 11)) + 0.5) * 0x1p-53;
+}
+}  // namespace
+
+std::string trace_samples(const Resolved &r, int state, int method, uint64_t seed, int m, double *out) {
+    if (state < 0 || state >= (int)r.state_spins.size()) return "state index out of range";
+    if (m < 1) return "the trace-sample count must be positive";
+    const int64_t D = r.D;
+    std::vector<int64_t> stride(r.nspins);
+    {
+        int64_t s = 1;
+        for (int i = r.nspins - 1; i >= 0; --i) { stride[i] = s; s *= r.mult[i]; }
+    }
+    const std::vector<int> &sp = r.state_spins[state];
+    const std::vector<cd> &v = r.state_amps[state];
+    double vn = 0.0;
+    for (const cd &x : v) vn += std::norm(x);
+    if (!(vn > 0.0)) return "state with zero norm";
+    std::vector<char> fixed(r.nspins, 0);
+    for (int x : sp) fixed[x] = 1;
+    std::vector<int> open;
+    for (int i = 0; i < r.nspins; ++i)
+        if (!fixed[i]) open.push_back(i);
+    const int64_t L = (int64_t)v.size(), Z = D / L;
+    // listed-spin sub-index <-> offset, open-spin index <-> offset
+    std::vector<int64_t> off_listed(L), off_open(Z);
+    for (int64_t c = 0; c < L; ++c) {
+        int64_t rem = c, idx = 0;
+        for (int q = (int)sp.size() - 1; q >= 0; --q) {
+            idx += (rem % r.mult[sp[q]]) * stride[sp[q]];
+            rem /= r.mult[sp[q]];
+        }
+        off_listed[c] = idx;
+    }
+    for (int64_t k = 0; k < Z; ++k) {
+        int64_t rem = k, idx = 0;
+        for (int q = (int)open.size() - 1; q >= 0; --q) {
+            idx += (rem % r.mult[open[q]]) * stride[open[q]];
+            rem /= r.mult[open[q]];
+        }
+        off_open[k] = idx;
+    }
+    std::vector<cd> col(D);
+    for (int j = 0; j < m; ++j) {
+        if (method == 0) {
+            // product over every spin of exp(-i phi Sz) exp(-i theta Sy)|+S>: Wigner d^S_{mS}(theta)
+            col.assign(1, cd(1.0));
+            for (int s = 0; s < r.nspins; ++s) {
+                uint32_t o[4];
+                philox4x32((uint32_t)j, (uint32_t)s, 0u, 0x434f4852u, seed, o);
+                double u1, u2;
+                two_uniforms(o, u1, u2);
+                const double th = std::acos(1.0 - 2.0 * u1), ph = 2.0 * M_PI * u2;
+                const int mult = r.mult[s];
+                const double S = 0.5 * (mult - 1);
+                const double c = std::cos(0.5 * th), sn = std::sin(0.5 * th);
+                std::vector<cd> loc(mult);
+                for (int i = 0; i < mult; ++i) {
+                    const double mz = S - i;
+                    // sqrt(binomial(2S, S + m)) cos^{S+m} sin^{S-m}
+                    const double lb = std::lgamma(2 * S + 1) - std::lgamma(S + mz + 1) - std::lgamma(S - mz + 1);
+                    const double mag = std::exp(0.5 * lb) * std::pow(c, S + mz) * std::pow(sn, S - mz);
+                    loc[i] = mag * std::polar(1.0, -ph * mz);
+                }
+                std::vector<cd> nxt(col.size() * mult);
+                for (size_t a = 0; a < col.size(); ++a)
+                    for (int b = 0; b < mult; ++b) nxt[a * mult + b] = col[a] * loc[b];
+                col.swap(nxt);
+            }
+            // project on the State's support: (v v^dagger / |v|^2) (x) 1
+            double nrm = 0.0;
+            for (int64_t k = 0; k < Z; ++k) {
+                cd ov = 0.0;
+                for (int64_t c = 0; c < L; ++c) ov += std::conj(v[c]) * col[off_open[k] + off_listed[c]];
+                ov /= vn;
+                for (int64_t c = 0; c < L; ++c) {
+                    col[off_open[k] + off_listed[c]] = ov * v[c];
+                    nrm += std::norm(ov * v[c]);
+                }
+            }
+            if (!(nrm > 1e-28)) return "a coherent sample has no overlap with the State";
+            const double inv = 1.0 / std::sqrt(nrm);
+            for (int64_t i = 0; i < D; ++i) col[i] *= inv;
+        } else {
+            // |v> (x) normalise(x + i y), x, y ~ N(0,1) by Box-Muller
+            std::vector<cd> chi(Z);
+            double cn = 0.0;
+            for (int64_t k = 0; k < Z; ++k) {
+                uint32_t o[4];
+                philox4x32((uint32_t)j, (uint32_t)k, 1u, 0x53555A00u, seed, o);
+                double u1, u2;
+                two_uniforms(o, u1, u2);
+                const double rr = std::sqrt(-2.0 * std::log(u1));
+                chi[k] = cd(rr * std::cos(2.0 * M_PI * u2), rr * std::sin(2.0 * M_PI * u2));
+                cn += std::norm(chi[k]);
+            }
+            const double sc = 1.0 / std::sqrt(cn * vn);
+            std::fill(col.begin(), col.end(), cd(0.0));
+            col.resize(D);
+            for (int64_t k = 0; k < Z; ++k)
+                for (int64_t c = 0; c < L; ++c) col[off_open[k] + off_listed[c]] = v[c] * chi[k] * sc;
+        }
+        for (int64_t i = 0; i < D; ++i) {
+            out[2 * ((size_t)j * D + i)] = col[i].real();
+            out[2 * ((size_t)j * D + i) + 1] = col[i].imag();
+        }
+    }
+    return "";
+}
+
 std::string resolve_system(const msg_system_desc *d, Resolved &r) {
     r = Resolved();
     if (!d || d->n_spins < 1 || !d->spins) return "system has no spins";
@@ -173,6 +302,22 @@ std::string resolve_system(const msg_system_desc *d, Resolved &r) {
         r.e1 = el[0];
         r.e2 = el[1];
         r.pair_ok = true;
+    }
+    for (int st = 0; st < d->n_states; ++st) {
+        const msg_state_desc &sd = d->states[st];
+        if (!pool_ok(sd.spins_offset, sd.spins_count) || sd.amp_offset < 0) return "state out of range";
+        std::vector<int> sp(d->index_pool + sd.spins_offset, d->index_pool + sd.spins_offset + sd.spins_count);
+        int64_t L = 1;
+        for (int x : sp) {
+            if (x < 0 || x >= r.nspins) return "state spin index out of range";
+            L *= r.mult[x];
+        }
+        if (sd.amp_offset + L > d->amp_pool_len) return "state amplitudes out of range";
+        std::vector<cd> amps(L);
+        for (int64_t i = 0; i < L; ++i)
+            amps[i] = cd(d->amp_pool[2 * (sd.amp_offset + i)], d->amp_pool[2 * (sd.amp_offset + i) + 1]);
+        r.state_spins.push_back(sp);
+        r.state_amps.push_back(amps);
     }
     if (!r.pair_ok) return "";  // structure not representable on the device paths; caller reports
     for (int i = 0; i < r.nspins; ++i)

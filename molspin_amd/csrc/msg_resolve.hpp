@@ -58,6 +58,9 @@ struct Resolved {
     // direct factors: rho0 = rho_e (x) 1/Z; columns evec_i (x) e_nu / sqrt(Z)
     std::vector<std::array<cd, 4>> evec;
     int64_t zdim = 0;
+    // State objects as described (spins in listed order, amplitudes), for trace sampling
+    std::vector<std::vector<int>> state_spins;
+    std::vector<std::vector<cd>> state_amps;
 };
 
 // Returns an empty string on success, else the reason.
@@ -73,6 +76,10 @@ void dense_hamiltonian(const Resolved &r, const double *field, std::vector<cd> &
 // exp(A) for a dense complex matrix (row-major), Pade(13) with scaling and
 // squaring (Higham 2005).
 void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E);
+
+// Trace samples of State `state` (SpinSpace::BuildTraceSamples): method 0
+// coherent, 1 SU(Z); unit-norm columns in MolSpin order, D x m interleaved.
+std::string trace_samples(const Resolved &r, int state, int method, uint64_t seed, int m, double *out);
 
 // BuildFreeEvolutionTimes (TaskHSGeneral.cpp:59-96).
 std::string free_times(double total, double dt, std::vector<double> &t);

@@ -40,6 +40,7 @@ def lib():
         L.ref_legacy_stoch_yields.argtypes = [P(abi.SystemDesc), C.c_void_p, C.c_double, C.c_double,
                                               C.c_void_p, C.c_int32, C.c_int, C.c_void_p]
         L.ref_coherent_samples.argtypes = [P(abi.SystemDesc), C.c_int, C.c_uint64, C.c_int32, C.c_void_p]
+        L.ref_suz_samples.argtypes = [P(abi.SystemDesc), C.c_int, C.c_uint64, C.c_int32, C.c_void_p]
         L.ref_fibonacci_point.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_void_p]
         L.ref_logspace.argtypes = [C.c_double, C.c_double, C.c_int32, C.c_void_p]
         L.ref_time_grid.argtypes = [C.c_double, C.c_double, C.c_void_p, C.c_void_p]
@@ -182,6 +183,13 @@ def coherent_samples(packed, state, seed, m):
     D = packed.dim
     out = np.zeros(2 * D * m)
     _check(lib().ref_coherent_samples(packed.ref(), int(state), int(seed), int(m), _p(out)))
+    return _cplx(out, (D, m))
+
+
+def suz_samples(packed, state, seed, m):
+    D = packed.dim
+    out = np.zeros(2 * D * m)
+    _check(lib().ref_suz_samples(packed.ref(), int(state), int(seed), int(m), _p(out)))
     return _cplx(out, (D, m))
 
 

@@ -1091,6 +1091,64 @@ int ref_coherent_samples(const msg_system_desc *sys, int state, uint64_t seed, i
     });
 }
 
+int ref_suz_samples(const msg_system_desc *sys, int state, uint64_t seed, int32_t m, double *out) {
+    return guarded([&] {
+        System S(sys);
+        const msg_state_desc &st = sys->states[state];
+        const int32_t *sp = sys->index_pool + st.spins_offset;
+        const int ns = st.spins_count;
+        std::vector<char> fixed(S.nspins, 0);
+        int64_t L = 1;
+        for (int k = 0; k < ns; ++k) { fixed[sp[k]] = 1; L *= S.mult[sp[k]]; }
+        std::vector<cd> v(L);
+        double vn = 0.0;
+        for (int64_t i = 0; i < L; ++i) {
+            v[i] = cd(sys->amp_pool[2 * (st.amp_offset + i)], sys->amp_pool[2 * (st.amp_offset + i) + 1]);
+            vn += std::norm(v[i]);
+        }
+        std::vector<int> open;
+        for (int s = 0; s < S.nspins; ++s) if (!fixed[s]) open.push_back(s);
+        const int64_t Z = S.D / L;
+        std::vector<cd> chi(Z);
+        for (int j = 0; j < m; ++j) {
+            double cn = 0.0;
+            for (int64_t k = 0; k < Z; ++k) {
+                const uint32_t ctr[4] = {(uint32_t)j, (uint32_t)k, 1u, 0x53555A00u};
+                const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+                uint32_t o[4];
+                philox(ctr, key, o);
+                const double u1 = ((double)(((uint64_t)o[0] << 21) ^ (o[1] >> 11)) + 0.5) * std::ldexp(1.0, -53);
+                const double u2 = ((double)(((uint64_t)o[2] << 21) ^ (o[3] >> 11)) + 0.5) * std::ldexp(1.0, -53);
+                const double rr = std::sqrt(-2.0 * std::log(u1));
+                chi[k] = cd(rr * std::cos(2.0 * M_PI * u2), rr * std::sin(2.0 * M_PI * u2));
+                cn += std::norm(chi[k]);
+            }
+            const double sc = 1.0 / std::sqrt(cn * vn);
+            for (int64_t i = 0; i < S.D; ++i) { out[2 * ((size_t)j * S.D + i)] = 0.0; out[2 * ((size_t)j * S.D + i) + 1] = 0.0; }
+            for (int64_t k = 0; k < Z; ++k) {
+                // open-spin digits of k, first open spin most significant
+                int64_t rem = k, base = 0;
+                for (int q = (int)open.size() - 1; q >= 0; --q) {
+                    const int mm = S.mult[open[q]];
+                    base += (rem % mm) * S.stride[open[q]];
+                    rem /= mm;
+                }
+                for (int64_t c = 0; c < L; ++c) {
+                    int64_t r2 = c, idx = base;
+                    for (int q = ns - 1; q >= 0; --q) {
+                        const int mm = S.mult[sp[q]];
+                        idx += (r2 % mm) * S.stride[sp[q]];
+                        r2 /= mm;
+                    }
+                    const cd val = v[c] * chi[k] * sc;
+                    out[2 * ((size_t)j * S.D + idx)] = val.real();
+                    out[2 * ((size_t)j * S.D + idx) + 1] = val.imag();
+                }
+            }
+        }
+    });
+}
+
 void ref_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox(ctr, key, out); }
 
 void ref_fibonacci_point(int32_t n, int32_t index, double magnitude, double out[3]) {

@@ -86,6 +86,11 @@ int ref_legacy_stoch_yields(const msg_system_desc *sys, const double *field,
  * `state` and normalised.  Not scaled by 1/sqrt(M).  out: D x m complex. */
 int ref_coherent_samples(const msg_system_desc *sys, int state, uint64_t seed, int32_t m, double *out);
 
+/* SU(Z) trace samples (SpinSpace_states.cpp:452-486): the State's normalised vector
+ * (x) a normalised complex Gaussian vector over the open spins, Gaussians by
+ * Box-Muller from Philox at counter (sample, component, 1, 0x53555A00).  out: D x m. */
+int ref_suz_samples(const msg_system_desc *sys, int state, uint64_t seed, int32_t m, double *out);
+
 /* Philox4x32-10 (Salmon et al., SC'11). */
 void ref_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

@@ -29,6 +29,12 @@ int shim_resolve(const msg_system_desc *d, int64_t *dim, int *fast_ok, int *pair
     return 0;
 }
 
+int shim_trace_samples(const msg_system_desc *d, int state, int method, uint64_t seed, int m, double *out) {
+    Resolved r;
+    if (!resolve_system(d, r).empty()) return 1;
+    return trace_samples(r, state, method, seed, m, out).empty() ? 0 : 2;
+}
+
 int shim_perm(const msg_system_desc *d, int64_t *perm) {
     Resolved r;
     if (!resolve_system(d, r).empty() || !r.pair_ok) return 1;

@@ -166,3 +166,38 @@ def test_step_entry_matches_oracle(ctx):
     _, tro = ref.run_hsgeneral(p, prm_t, columns=cols, trace=True)
     pops = [np.real(np.trace(new.conj().T @ ref.projector(p, st) @ new)) for st in range(4)]
     assert np.abs(np.array(pops) - tro[0, 1]).max() < 1e-6
+
+
+def test_large_dimension_streaming_path(ctx):
+    """D = 16384 (7 + 5 anisotropic spin-1/2 nuclei): the streaming matrix-free
+    variant on 6 direct columns and 2 orientations over 0-60 ns."""
+    rng = workloads.SplitMix64(41, 3)
+    hf = [workloads._aniso_hf(rng) for _ in range(12)]
+    s = workloads._radical_pair("big", ["1/2"] * 7, ["1/2"] * 5, hf, 1e-3, swept=True)
+    p = s.pack()
+    sysd = ctx.upload(p)
+    assert sysd.dim == 16384
+    B = ref.direct_factors(p)
+    cols = B[:, [0, 77, 1023, 2048, 3001, 4095]] * math.sqrt(B.shape[1] / 6.0)
+    fields = [workloads.fibonacci_point(64, i, 0.05) for i in (5, 40)]
+    prm = abi.run_params(total_time=60.0, time_step=2.0, sampling=abi.MSG_SAMPLING_GIVEN)
+    y, _ = sysd.propagate(prm, fields=fields, columns=cols)
+    yo, _ = ref.run_hsgeneral(p, prm, fields=fields, columns=cols, threads=8)
+    print("D=16384 dev", y, "oracle", yo)
+    assert np.abs(y - yo).max() <= YIELD_TOL
+
+
+def test_config3_trajectories(ctx):
+    """Config 3 (D = 65536, 14 isotropic nuclei, ST dephasing 1e-4 /ns): 4 spin-
+    coherent trajectories drawn by the backend's sampler, 0-30 ns, against the
+    oracle on the same columns and the same counter-based flips."""
+    s, cfg = workloads.config3(n_traj=4)
+    p = s.pack()
+    sysd = ctx.upload(p)
+    M = 4
+    cols = sysd.trace_samples(s.state_index("S"), M, cfg["sample_seed"]) / math.sqrt(M)
+    prm = abi.run_params(total_time=30.0, time_step=1.0, sampling=abi.MSG_SAMPLING_GIVEN, seed=2026)
+    y, _ = sysd.propagate(prm, columns=cols)
+    yo, _ = ref.run_hsgeneral(p, prm, columns=cols, threads=8)
+    print("cfg3 dev", y, "oracle", yo)
+    assert np.abs(y - yo).max() <= YIELD_TOL

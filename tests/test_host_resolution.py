@@ -32,6 +32,7 @@ def shim():
     L.shim_perm.argtypes = [P(abi.SystemDesc), C.c_void_p]
     L.shim_dense_hamiltonian.argtypes = [P(abi.SystemDesc), C.c_void_p, C.c_void_p]
     L.shim_fast_apply.argtypes = [P(abi.SystemDesc), C.c_void_p, C.c_void_p, C.c_void_p]
+    L.shim_trace_samples.argtypes = [P(abi.SystemDesc), C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_void_p]
     return L
 
 
@@ -111,3 +112,22 @@ def test_resolution_flags(shim):
     s2, _ = workloads.config2(n_orientations=4)
     assert shim.shim_resolve(s2.pack().ref(), C.byref(dim), C.byref(fast), C.byref(pair), C.byref(sym)) == 0
     assert dim.value == 4096 and fast.value == 1 and sym.value == 1
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_trace_samples_match_oracle_sampler(shim, method):
+    """The backend's host sampler (BuildTraceSamples restated, coherent and SU(Z))
+    draws the same columns as the oracle's from the same counter-based stream,
+    for spin-1/2 and spin-1 nuclei and a singlet support."""
+    s, _ = workloads.config1()
+    p = s.pack()
+    M = 16
+    out = np.zeros(2 * p.dim * M)
+    assert shim.shim_trace_samples(p.ref(), s.state_index("S"), method, 777, M, _p(out)) == 0
+    got = out.view(np.complex128).reshape((p.dim, M), order="F")
+    want = (ref.coherent_samples if method == 0 else ref.suz_samples)(p, s.state_index("S"), 777, M)
+    # columns agree up to a global phase of each column
+    for j in range(M):
+        ph = np.vdot(want[:, j], got[:, j])
+        assert abs(abs(ph) - 1.0) < 1e-12
+        assert np.abs(got[:, j] - ph * want[:, j]).max() < 1e-12

@@ -339,3 +339,22 @@ def test_expm_and_eig_pins():
     U = ref.expm(-1j * H * 0.7)
     lam, Q = np.linalg.eigh(H)
     assert np.abs(U - Q @ np.diag(np.exp(-0.7j * lam)) @ Q.conj().T).max() < 1e-12
+
+
+def test_suz_samples_statistics():
+    """SU(Z) trace samples: unit norm, fixed singlet part and E|chi><chi| = 1/Z
+    on the open spins (STOCHASTIC_RELAXATION.md:322-337)."""
+    s = SpinSystem("c")
+    s.add_spin("e1", "1/2", electron=True)
+    s.add_spin("e2", "1/2", electron=True)
+    s.add_spin("n1", "1/2", tensor=1.0)
+    s.add_spin("n2", "1", tensor=1.0)
+    s.add_radical_pair_states("e1", "e2")
+    p = s.pack()
+    M = 8192
+    B = ref.suz_samples(p, 0, 4321, M)
+    assert np.allclose(np.linalg.norm(B, axis=0), 1.0, atol=1e-14)
+    P = ref.projector(p, 0)
+    assert np.abs(P @ B - B).max() < 1e-14
+    rho = B @ B.conj().T / M
+    assert np.abs(rho - P / np.trace(P).real).max() < 6.0 / math.sqrt(M)
