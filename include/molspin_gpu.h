@@ -148,6 +148,12 @@ enum {
     MSG_SAMPLING_GIVEN = 1        /* caller-supplied columns (trace samples drawn on the host)    */
 };
 
+enum {
+    MSG_TASK_HSGENERAL = 0,            /* type=HSGeneral (TaskHSGeneral.cpp:267-652)               */
+    MSG_TASK_LEGACY_DIRECT_YIELDS = 1, /* type=statichs-direct-yields (TaskStaticHSDirectYields.cpp) */
+    MSG_TASK_LEGACY_STOCH_YIELDS = 2   /* type=statichs-stoch-yields (TaskStaticHSStochYields.cpp)   */
+};
+
 typedef struct msg_run_params {
     int32_t  calculation;
     int32_t  observables;
@@ -158,6 +164,11 @@ typedef struct msg_run_params {
     double   total_time;          /* totaltime (ns) */
     double   time_step;           /* timestep  (ns) */
     uint64_t seed;                /* key of the counter-based relaxation RNG */
+    int32_t  task;                /* MSG_TASK_*: the legacy tasks sample k dt for k < ceil(T/dt)
+                                     (endpoint excluded, TaskStaticHSDirectYields.cpp:347) and treat
+                                     equal rates as symmetric recombination with the decay
+                                     e^{-k t} applied analytically (:257-292) */
+    int32_t  reserved;
 } msg_run_params;
 
 /* ------------------------------------------------------------------------- */
@@ -215,7 +226,8 @@ msg_status msg_direct_columns(const msg_system *sys, int32_t *n_columns);
 msg_status msg_observable_count(const msg_system *sys, const msg_run_params *params,
                                 int32_t *n_obs);
 /* Number of time samples of the free-evolution grid (BuildFreeEvolutionTimes,
- * TaskHSGeneral.cpp:59-96): 0, dt, 2dt, ..., T with an exact endpoint. */
+ * TaskHSGeneral.cpp:59-96): 0, dt, 2dt, ..., T with an exact endpoint.  (The
+ * legacy tasks' grid has ceil(T/dt) samples k dt.) */
 msg_status msg_time_grid(double total_time, double time_step, int64_t *n_samples,
                          double *times /* may be NULL; else n_samples entries */);
 

@@ -31,6 +31,11 @@ MSG_METHOD_EXPONENTIAL = 2
 MSG_METHOD_RK4 = 3  # oracle only
 MSG_SAMPLING_DIRECT = 0
 MSG_SAMPLING_GIVEN = 1
+MSG_TASK_HSGENERAL = 0
+MSG_TASK_LEGACY_DIRECT_YIELDS = 1
+MSG_TASK_LEGACY_STOCH_YIELDS = 2
+MSG_TRACE_COHERENT = 0
+MSG_TRACE_SUZ = 1
 
 
 class SpinDesc(C.Structure):
@@ -79,12 +84,12 @@ class RunParams(C.Structure):
                 ("method", C.c_int32), ("krylov_size", C.c_int32),
                 ("sampling", C.c_int32), ("yield_corrections", C.c_int32),
                 ("total_time", C.c_double), ("time_step", C.c_double),
-                ("seed", C.c_uint64)]
+                ("seed", C.c_uint64), ("task", C.c_int32), ("reserved", C.c_int32)]
 
 
 def run_params(calculation=MSG_CALC_YIELDS, observables=MSG_OBS_TRANSITION_YIELDS,
                method=MSG_METHOD_KRYLOV, krylov_size=16, sampling=MSG_SAMPLING_DIRECT,
-               yield_corrections=False, total_time=10000.0, time_step=1.0, seed=1):
+               yield_corrections=False, total_time=10000.0, time_step=1.0, seed=1, task=MSG_TASK_HSGENERAL):
     """RunParams with HSExecutionPlan defaults (HSExecutionPlan.h:60-79)."""
     return RunParams(calculation, observables, method, krylov_size, sampling,
-                     int(bool(yield_corrections)), float(total_time), float(time_step), int(seed))
+                     int(bool(yield_corrections)), float(total_time), float(time_step), int(seed), int(task), 0)

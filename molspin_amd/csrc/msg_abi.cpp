@@ -153,9 +153,29 @@ int observable_count(const Resolved &r, const msg_run_params *p) {
     return p->observables == MSG_OBS_TRANSITION_YIELDS ? (int)r.P_trans.size() : (int)r.P_states.size();
 }
 
+// Symmetric recombination: K = kappa 1 (HSGeneral); the legacy tasks call any set of
+// equal rates symmetric unless a source State is the identity
+// (TaskStaticHSDirectYields.cpp:257-292) and then use kappa = k/2.
+bool sym_for(const Resolved &r, const msg_run_params *p) {
+    if (p->task == MSG_TASK_HSGENERAL) return r.symmetric;
+    if (r.rates.empty()) return true;
+    for (double k : r.rates)
+        if (k != r.rates[0]) return false;
+    for (const Mat4 &P : r.P_trans) {
+        bool ident = true;
+        for (int i = 0; i < 16; ++i) ident &= std::abs(P[i] - ((i % 5) == 0 ? cd(1.0) : cd(0.0))) < 1e-14;
+        if (ident) return false;
+    }
+    return true;
+}
+double kappa_for(const Resolved &r, const msg_run_params *p) {
+    if (p->task == MSG_TASK_HSGENERAL) return r.kappa;
+    return r.rates.empty() ? 0.0 : 0.5 * r.rates[0];
+}
+
 Path choose_path(const Resolved &r, const msg_run_params *p) {
     const bool dense_ok = r.D <= 512;
-    const bool lanczos_ok = r.fast_ok && r.symmetric && r.D >= 256 && r.D <= (int64_t)4 << kMaxNuclei;
+    const bool lanczos_ok = r.fast_ok && sym_for(r, p) && r.D >= 256 && r.D <= (int64_t)4 << kMaxNuclei;
     if (p->method == MSG_METHOD_EXPONENTIAL) {
         if (!dense_ok) throw ArgError{MSG_UNSUPPORTED, "propagationmethod=exponential needs D <= 512 on the device"};
         return Path::Dense;
@@ -163,7 +183,7 @@ Path choose_path(const Resolved &r, const msg_run_params *p) {
     if (p->method == MSG_METHOD_AUTO && dense_ok) return Path::Dense;
     if (lanczos_ok) return Path::Lanczos;
     if (dense_ok) return Path::Dense;  // exact exponential: same result as Krylov within truncation
-    if (!r.symmetric)
+    if (!sym_for(r, p))
         throw ArgError{MSG_UNSUPPORTED, "unequal recombination rates need D <= 512 (dense exponential path)"};
     if (!r.fast_ok) throw ArgError{MSG_UNSUPPORTED, "matrix-free path: " + r.why_not_fast};
     throw ArgError{MSG_UNSUPPORTED, "Hilbert dimension outside the matrix-free range"};
@@ -204,13 +224,17 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     b.scale.assign(b.nobs, 1.0);
     if (p->observables == MSG_OBS_TRANSITION_YIELDS) b.scale = r.rates;
     // time grid
-    const std::string te = free_times(p->total_time, p->time_step, b.times);
+    const bool legacy = p->task == MSG_TASK_LEGACY_DIRECT_YIELDS || p->task == MSG_TASK_LEGACY_STOCH_YIELDS;
+    if (!(p->task == MSG_TASK_HSGENERAL || legacy)) throw ArgError{MSG_INVALID_ARG, "unknown task"};
+    const std::string te = legacy ? legacy_times(p->total_time, p->time_step, b.times)
+                                  : free_times(p->total_time, p->time_step, b.times);
     if (!te.empty()) throw ArgError{MSG_INVALID_ARG, te};
     b.nsamples = (int)b.times.size();
     const int ns = b.nsamples;
     std::vector<double> dts(std::max(ns - 1, 1), 0.0), wyh((size_t)ns * b.nobs), wth((size_t)ns * b.nobs);
     std::vector<uint32_t> thr(std::max(ns - 1, 1), 0u);
-    const bool factor_decay = r.symmetric;
+    const bool factor_decay = sym_for(r, p);
+    const double kappa = factor_decay ? kappa_for(r, p) : 0.0;
     for (int s = 0; s + 1 < ns; ++s) {
         dts[s] = b.times[s + 1] - b.times[s];
         if (r.relax) {
@@ -222,7 +246,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         double w = 0.0;
         if (s > 0) w += 0.5 * (b.times[s] - b.times[s - 1]);
         if (s + 1 < ns) w += 0.5 * (b.times[s + 1] - b.times[s]);
-        const double decay = factor_decay ? std::exp(-2.0 * r.kappa * b.times[s]) : 1.0;
+        const double decay = factor_decay ? std::exp(-2.0 * kappa * b.times[s]) : 1.0;
         for (int c = 0; c < b.nobs; ++c) {
             wyh[(size_t)s * b.nobs + c] = w * decay * b.scale[c];
             wth[(size_t)s * b.nobs + c] = decay * b.scale[c];
@@ -315,7 +339,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         HIPCHECK(hipMemcpyAsync(dc, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice, stream));
         a.coef = dc;
         b.n_units = n_items * b.ncols;
-        a.renorm = r.symmetric ? 1 : 0;
+        a.renorm = factor_decay ? 1 : 0;
     } else {
         const int Dp = (D + 15) / 16 * 16;
         if (Dp > 512) throw ArgError{MSG_UNSUPPORTED, "dense path limited to D <= 512"};
@@ -327,7 +351,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         std::vector<cd> H, Aexp, E;
         for (int it = 0; it < n_items; ++it) {
             dense_hamiltonian(r, fields ? fields + 3 * it : nullptr, H);
-            if (!r.symmetric)  // H_eff = H - i K (x) 1 on the electron pair
+            if (!factor_decay)  // H_eff = H - i K (x) 1 on the electron pair
                 for (int64_t nu = 0; nu < D / 4; ++nu)
                     for (int i = 0; i < 4; ++i)
                         for (int j = 0; j < 4; ++j)
@@ -350,7 +374,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         HIPCHECK(hipMemcpyAsync(dU, Uh.data(), Uh.size() * sizeof(float), hipMemcpyHostToDevice, stream));
         a.U = dU;
         a.ustride = per;
-        a.renorm = r.symmetric ? 1 : 0;  // U unitary: keep each column at its initial norm
+        a.renorm = factor_decay ? 1 : 0;  // U unitary: keep each column at its initial norm
         b.n_units = n_items * ((b.ncols + 15) / 16);
     }
     a.n_units = b.n_units;

@@ -279,6 +279,16 @@ std::string trace_samples(const Resolved &r, int state, int method, uint64_t see
     return "";
 }
 
+std::string legacy_times(double total, double dt, std::vector<double> &t) {
+    t.clear();
+    if (!(dt > 0.0) || !std::isfinite(dt)) return "timestep must be finite and positive";
+    if (!(total > 0.0) || !std::isfinite(total)) return "totaltime must be finite and positive";
+    const double n = std::ceil(total / dt);
+    if (!(n >= 1.0) || n > 1e9) return "too many time intervals";
+    for (int64_t k = 0; k < (int64_t)n; ++k) t.push_back((double)k * dt);
+    return "";
+}
+
 std::string resolve_system(const msg_system_desc *d, Resolved &r) {
     r = Resolved();
     if (!d || d->n_spins < 1 || !d->spins) return "system has no spins";

@@ -84,4 +84,7 @@ std::string trace_samples(const Resolved &r, int state, int method, uint64_t see
 // BuildFreeEvolutionTimes (TaskHSGeneral.cpp:59-96).
 std::string free_times(double total, double dt, std::vector<double> &t);
 
+// Legacy StaticHS grid: k dt for k < ceil(T/dt) (TaskStaticHSDirectYields.cpp:347).
+std::string legacy_times(double total, double dt, std::vector<double> &t);
+
 }  // namespace msg

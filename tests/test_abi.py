@@ -34,7 +34,7 @@ def test_struct_layouts_match_header_sizes():
     # sizes of the POD structs as the C compiler lays them out (natural alignment)
     assert C.sizeof(abi.SpinDesc) == 16 + 72
     assert C.sizeof(abi.InteractionDesc) == 40 + 8 + 24 + 72
-    assert C.sizeof(abi.RunParams) == 24 + 16 + 8
+    assert C.sizeof(abi.RunParams) == 24 + 16 + 8 + 8
 
 
 def test_time_grid_without_device():

@@ -201,3 +201,43 @@ def test_config3_trajectories(ctx):
     yo, _ = ref.run_hsgeneral(p, prm, columns=cols, threads=8)
     print("cfg3 dev", y, "oracle", yo)
     assert np.abs(y - yo).max() <= YIELD_TOL
+
+
+def test_legacy_direct_yields(ctx):
+    """statichs-direct-yields (endpoint excluded, Lanczos with basis reuse in the
+    reference): device task mode against the oracle's legacy loop."""
+    s, _ = workloads.config1()
+    p = s.pack()
+    sysd = ctx.upload(p)
+    T, dt = 600.0, 1.0
+    prm = abi.run_params(total_time=T, time_step=dt, task=abi.MSG_TASK_LEGACY_DIRECT_YIELDS, yield_corrections=True)
+    y, _ = sysd.propagate(prm)
+    yo = ref.legacy_direct_yields(p, T, dt, correction=True)
+    print("legacy direct dev", y[0], "oracle", yo)
+    assert np.abs(y[0] - yo).max() <= YIELD_TOL
+
+
+def test_legacy_stoch_yields_singlet_only_sink(ctx):
+    """statichs-stoch-yields with a single singlet sink: the legacy rule treats
+    equal rates as symmetric and applies e^{-kt} analytically."""
+    s = SpinSystem("leg")
+    s.add_spin("E1", "1/2", electron=True, tensor=2.0023)
+    s.add_spin("E2", "1/2", electron=True, tensor=2.0023)
+    for i, a in enumerate([0.9, 0.4, 1.3, 0.7]):
+        s.add_spin("N%d" % i, "1/2", tensor=1.0)
+        s.add_hyperfine(["E1" if i < 2 else "E2"], ["N%d" % i], tensor=a, prefactor=0.001)
+    s.add_zeeman(["E1", "E2"], (0.0, 0.0, 0.05), prefactor=0.001)
+    s.add_radical_pair_states("E1", "E2")
+    s.add_transition("kS", "S", 2e-3)
+    s.set_initial(["S"])
+    p = s.pack()
+    sysd = ctx.upload(p)
+    M = 32
+    cols = sysd.trace_samples(s.state_index("S"), M, 31337)
+    T, dt = 500.0, 1.0
+    prm = abi.run_params(total_time=T, time_step=dt, sampling=abi.MSG_SAMPLING_GIVEN,
+                         task=abi.MSG_TASK_LEGACY_STOCH_YIELDS)
+    y, _ = sysd.propagate(prm, columns=cols / math.sqrt(M))
+    yo = ref.legacy_stoch_yields(p, T, dt, cols)
+    print("legacy stoch dev", y[0], "oracle", yo)
+    assert np.abs(y[0] - yo).max() <= YIELD_TOL
