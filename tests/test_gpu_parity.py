@@ -241,3 +241,31 @@ def test_legacy_stoch_yields_singlet_only_sink(ctx):
     yo = ref.legacy_stoch_yields(p, T, dt, cols)
     print("legacy stoch dev", y[0], "oracle", yo)
     assert np.abs(y[0] - yo).max() <= YIELD_TOL
+
+
+def test_server_alone_equals_mixed(ctx):
+    """Continuous batching: requests of different radical pairs (dense and
+    matrix-free paths) admitted while others run give the same yields as each
+    request propagated alone, and match the oracle."""
+    from molspin_amd import backend
+    reqs = []
+    for idx in (4, 3, 20):  # D = 1024 and 2048 (matrix-free), 512 (dense)
+        s, cfg = workloads.config4_pair(idx)
+        p = s.pack()
+        reqs.append((s, p, ctx.upload(p), cfg["fields"][::32]))
+    prm = abi.run_params(total_time=40.0, time_step=2.0)
+    alone = [sd.propagate(prm, fields=f)[0] for _, _, sd, f in reqs]
+    srv = backend.Server(ctx, max_slots=2, hbm_budget_bytes=8 << 30)
+    ids = [srv.submit(sd, prm, f) for _, _, sd, f in reqs[:2]]
+    srv.pump(5)  # first two resident and part-way
+    ids.append(srv.submit(reqs[2][2], prm, reqs[2][3]))
+    active = 3
+    while active:
+        active = srv.pump(5)
+    mixed = [srv.collect(i) for i in ids]
+    srv.close()
+    for (s, p, sd, f), a, m in zip(reqs, alone, mixed):
+        print("D", sd.dim, "alone", a[0], "mixed", m[0])
+        assert np.abs(a - m).max() <= 1e-12
+        yo, _ = ref.run_hsgeneral(p, prm, fields=f, threads=8)
+        assert np.abs(m - yo).max() <= YIELD_TOL
