@@ -31,6 +31,9 @@ def main():
     ap.add_argument("--intervals", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--serve-intervals", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -78,6 +81,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(packed, cfg, args.cpu_seconds)
+    serving = None if args.no_serving else serve(ctx, args.requests, args.serve_intervals)
 
     line = {
         "metric": "propagated state-steps/s + p99 yield-request latency at |dPhiS|<=1e-5 vs FP64",
@@ -101,8 +105,67 @@ def main():
                      "frac": achieved / HBM_PEAK, "traffic": None,
                      "kernel_ms": kernel_s * 1e3, "bytes_per_state_step": bytes_per_ss},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
+
+
+def serve(ctx, n_requests, intervals):
+    """Serving loop: a seeded Poisson stream (mean 200 requests/s) of radical-pair
+    field-sweep yield requests (n ~ U{2..12} nuclei, 16-128 fields each), each
+    propagated over `intervals` steps of 2 ns; requests are admitted by HBM
+    footprint into a continuous-batching server (one persistent launch per request
+    on its own stream, up to 8 overlapping).
+    Latency = completion - arrival.  Returns p50/p99 and state-steps/s."""
+    import numpy as np
+    from molspin_amd import abi, backend, workloads
+    reqs, t_arr = [], 0.0
+    for i in range(n_requests):
+        s, p, gap = workloads.serving_request(i)
+        t_arr += gap
+        packed = s.pack()
+        sysd = ctx.upload(packed)
+        prm = abi.run_params(total_time=intervals * p["time_step"], time_step=p["time_step"])
+        reqs.append(dict(arrival=t_arr, sys=sysd, packed=packed, prm=prm, fields=p["fields"],
+                         steps=sysd.direct_columns() * len(p["fields"]) * intervals))
+    srv = backend.Server(ctx, max_slots=8, hbm_budget_bytes=64 << 30)
+    t0 = time.perf_counter()
+    pending = list(range(n_requests))
+    live = {}
+    lat = []
+    while pending or live:
+        now = time.perf_counter() - t0
+        while pending and reqs[pending[0]]["arrival"] <= now:
+            i = pending.pop(0)
+            r = reqs[i]
+            live[srv.submit(r["sys"], r["prm"], r["fields"])] = i
+        if not live:
+            time.sleep(max(0.0, reqs[pending[0]]["arrival"] - now))
+            continue
+        if srv.pump(0) == len(live) and not (pending and reqs[pending[0]]["arrival"] <= now):
+            time.sleep(0.0005)
+        for rid in list(live):
+            y = srv.collect(rid)
+            if y is not None:
+                i = live.pop(rid)
+                lat.append(time.perf_counter() - t0 - reqs[i]["arrival"])
+    elapsed = time.perf_counter() - t0
+    srv.close()
+    lat = np.array(lat)
+    total = sum(r["steps"] for r in reqs)
+    # plain one-system-at-a-time form: the first 6 requests, one field per call
+    t1 = time.perf_counter()
+    plain_steps = 0
+    for r in reqs[:6]:
+        for f in r["fields"]:
+            r["sys"].propagate(r["prm"], fields=[f])
+        plain_steps += r["steps"]
+    plain = plain_steps / (time.perf_counter() - t1)
+    return {"requests": n_requests, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3), "state_steps_per_s": total / elapsed,
+            "plain_one_at_a_time_state_steps_per_s": plain, "served_over_plain": total / elapsed / plain,
+            "elapsed_s": elapsed, "intervals_per_request": intervals,
+            "workload": "Poisson 200 req/s, config-4-style pairs (D 16..16384), 16-128 fields/request"}
 
 
 def cpu_baseline(packed, cfg, seconds):

@@ -280,9 +280,12 @@ msg_status msg_server_destroy(msg_server *srv);
  * vectors.  Returns a request id.  The system must stay uploaded until done. */
 msg_status msg_server_submit(msg_server *srv, const msg_system *sys, const msg_run_params *params,
                              int32_t n_fields, const double *fields, int64_t *request_id);
-/* Advances the device work by at most `max_steps` time steps of every resident
- * request, admitting queued requests first.  Sets *n_active to the number of
- * requests still queued or running. */
+/* Collects finished requests (without blocking), then admits queued requests
+ * while a slot and the HBM budget allow.  max_steps <= 0: every admitted request
+ * is propagated over its whole grid in one persistent launch on its own stream,
+ * so requests overlap on the device.  max_steps > 0: resident requests advance
+ * in chunks of max_steps intervals per call, their states kept on the device.
+ * Sets *n_active to the number of requests still queued or running. */
 msg_status msg_server_pump(msg_server *srv, int32_t max_steps, int32_t *n_active);
 /* Copies the yields (n_fields x n_obs) of a finished request and forgets it.
  * Returns MSG_INVALID_ARG if the request is unknown, *done = 0 if unfinished. */

@@ -33,6 +33,9 @@ hipError_t launch_zero_int(int *p, int n, hipStream_t st);
 hipError_t launch_reduce_columns(const double *in, double *out, int n_items, int ncols, int nk, int in_stride, int pick,
                                  hipStream_t st);
 int lanczos_occupancy(int group_per_thread, int threads, size_t lds);
+hipError_t launch_zgemm(const void *A, const void *B, void *C, int n, int batch, double coef, double diag_add,
+                        hipStream_t st);
+hipError_t launch_to_planes(const void *U, float *dst, int n, int Dp, size_t stride, int batch, hipStream_t st);
 int dense_occupancy(size_t lds);
 }  // namespace msg
 
@@ -79,24 +82,27 @@ struct ArgError {
     std::string what;
 };
 
-// RAII device allocation.
+// RAII device allocation, stream-ordered so that a batch never synchronises the
+// device (other requests keep running while this one allocates or frees).
 struct DBuf {
     void *p = nullptr;
     size_t n = 0;
+    hipStream_t st = nullptr;
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { reset(); }
     void reset() {
-        if (p) (void)hipFree(p);
+        if (p) (void)hipFreeAsync(p, st);
         p = nullptr;
         n = 0;
     }
     template <class T>
-    T *alloc(size_t count) {
+    T *alloc(size_t count, hipStream_t stream) {
         reset();
         if (count == 0) return nullptr;
-        HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+        st = stream;
+        HIPCHECK(hipMallocAsync(&p, count * sizeof(T), stream));
         n = count * sizeof(T);
         return static_cast<T *>(p);
     }
@@ -122,13 +128,10 @@ struct Batch {
     bool keep_state = false;
     hipStream_t stream = nullptr;
     size_t footprint = 0;
-    StepArgs *args_host = nullptr;  // pinned staging of the launch arguments
-    bool timed = false;             // bracket the propagation kernel with the context events
+    StepArgs args_host{};  // host copy of the last launch arguments
+    bool timed = false;    // bracket the propagation kernel with the context events
     Batch() = default;
     Batch(const Batch &) = delete;
-    ~Batch() {
-        if (args_host) (void)hipHostFree(args_host);
-    }
 };
 
 const int kObsMax = kMaxObs;
@@ -265,7 +268,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                 ob[32 * c + 2 * i] = (float)P[c][i].real();
                 ob[32 * c + 2 * i + 1] = (float)P[c][i].imag();
             }
-        float *d = b.obs.alloc<float>(ob.size());
+        float *d = b.obs.alloc<float>(ob.size(), stream);
         HIPCHECK(hipMemcpyAsync(d, ob.data(), ob.size() * sizeof(float), hipMemcpyHostToDevice, stream));
         a.obs = d;
     }
@@ -285,7 +288,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                 g[off] = (float)columns[2 * ((size_t)j * D + i)];
                 g[off + 1] = (float)columns[2 * ((size_t)j * D + i) + 1];
             }
-        float *dg = b.given.alloc<float>(g.size());
+        float *dg = b.given.alloc<float>(g.size(), stream);
         HIPCHECK(hipMemcpyAsync(dg, g.data(), g.size() * sizeof(float), hipMemcpyHostToDevice, stream));
         a.given = dg;
     } else {
@@ -299,7 +302,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                 ev[i * 8 + 2 * c] = (float)(r.evec[i][c].real() * inv);
                 ev[i * 8 + 2 * c + 1] = (float)(r.evec[i][c].imag() * inv);
             }
-        float *d = b.evec.alloc<float>(ev.size());
+        float *d = b.evec.alloc<float>(ev.size(), stream);
         HIPCHECK(hipMemcpyAsync(d, ev.data(), ev.size() * sizeof(float), hipMemcpyHostToDevice, stream));
         a.evec = d;
     }
@@ -332,10 +335,10 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
             a.op_flags |= flags;
             std::copy(one.begin(), one.end(), all.begin() + (size_t)it * a.coef_stride);
         }
-        int *dfl = b.nflags.alloc<int>(nfl.size());
+        int *dfl = b.nflags.alloc<int>(nfl.size(), stream);
         HIPCHECK(hipMemcpyAsync(dfl, nfl.data(), nfl.size() * sizeof(int), hipMemcpyHostToDevice, stream));
         a.nuc_flags = dfl;
-        float *dc = b.coef.alloc<float>(all.size());
+        float *dc = b.coef.alloc<float>(all.size(), stream);
         HIPCHECK(hipMemcpyAsync(dc, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice, stream));
         a.coef = dc;
         b.n_units = n_items * b.ncols;
@@ -347,31 +350,55 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         const bool two = ns >= 2 && std::fabs(dts[ns - 2] - dts[0]) > 0.0;
         a.ulast = two ? 1 : 0;
         const int per = two ? 2 : 1;
-        std::vector<float> Uh((size_t)n_items * per * 2 * Dp * Dp, 0.f);
-        std::vector<cd> H, Aexp, E;
-        for (int it = 0; it < n_items; ++it) {
-            dense_hamiltonian(r, fields ? fields + 3 * it : nullptr, H);
-            if (!factor_decay)  // H_eff = H - i K (x) 1 on the electron pair
-                for (int64_t nu = 0; nu < D / 4; ++nu)
-                    for (int i = 0; i < 4; ++i)
-                        for (int j = 0; j < 4; ++j)
-                            H[(size_t)(nu * 4 + i) * D + nu * 4 + j] += cd(0.0, -1.0) * r.K[i * 4 + j];
-            for (int w = 0; w < per; ++w) {
-                const double dt = w == 0 ? dts[0] : dts[ns - 2];
-                Aexp.resize(H.size());
-                for (size_t i = 0; i < H.size(); ++i) Aexp[i] = cd(0.0, -dt) * H[i];
-                expm_dense(D, Aexp, E);
-                float *dst = &Uh[((size_t)it * per + w) * 2 * Dp * Dp];
-                // planes re, im; column-major U[k][row]
-                for (int row = 0; row < D; ++row)
-                    for (int k = 0; k < D; ++k) {
-                        dst[(size_t)k * Dp + row] = (float)E[(size_t)row * D + k].real();
-                        dst[(size_t)Dp * Dp + (size_t)k * Dp + row] = (float)E[(size_t)row * D + k].imag();
+        // exp(-i H_eff dt) on the device in FP64: Taylor(18) of X = A / 2^s, then s squarings;
+        // items are processed in chunks to bound the FP64 workspace
+        float *dU = b.U.alloc<float>((size_t)n_items * per * 2 * Dp * Dp, stream);
+        HIPCHECK(hipMemsetAsync(dU, 0, (size_t)n_items * per * 2 * Dp * Dp * sizeof(float), stream));
+        const int chunk = std::max(1, std::min(n_items, (int)((size_t)256 << 20) / (D * D * 16)));
+        DBuf X, P, Q;
+        cd *dX = X.alloc<cd>((size_t)chunk * D * D, stream);
+        cd *dP = P.alloc<cd>((size_t)chunk * D * D, stream);
+        cd *dQ = Q.alloc<cd>((size_t)chunk * D * D, stream);
+        std::vector<cd> H, Hs((size_t)chunk * D * D);
+        for (int w = 0; w < per; ++w) {
+            const double dt = w == 0 ? dts[0] : dts[ns - 2];
+            for (int it0 = 0; it0 < n_items; it0 += chunk) {
+                const int nb = std::min(chunk, n_items - it0);
+                double nrm = 0.0;
+                for (int q = 0; q < nb; ++q) {
+                    dense_hamiltonian(r, fields ? fields + 3 * (it0 + q) : nullptr, H);
+                    if (!factor_decay)  // H_eff = H - i K (x) 1 on the electron pair
+                        for (int64_t nu = 0; nu < D / 4; ++nu)
+                            for (int i = 0; i < 4; ++i)
+                                for (int j = 0; j < 4; ++j)
+                                    H[(size_t)(nu * 4 + i) * D + nu * 4 + j] += cd(0.0, -1.0) * r.K[i * 4 + j];
+                    for (int j = 0; j < D; ++j) {  // 1-norm of A = -i dt H_eff
+                        double cs = 0.0;
+                        for (int i = 0; i < D; ++i) cs += std::abs(H[(size_t)i * D + j]);
+                        nrm = std::max(nrm, cs * dt);
                     }
+                    for (size_t k = 0; k < H.size(); ++k) Hs[(size_t)q * D * D + k] = cd(0.0, -dt) * H[k];
+                }
+                int sq = 0;
+                while (nrm > 0.5) { nrm *= 0.5; ++sq; }
+                const double scale = std::ldexp(1.0, -sq);
+                for (auto &x : Hs) x *= scale;
+                HIPCHECK(hipMemcpyAsync(dX, Hs.data(), (size_t)nb * D * D * sizeof(cd), hipMemcpyHostToDevice, stream));
+                // Horner: P = I + X/18; P = I + X P / k, k = 17..1
+                HIPCHECK(launch_zgemm(dX, dX, dP, D, nb, 0.0, 1.0, stream));  // P = I
+                for (int k = 18; k >= 1; --k) {
+                    HIPCHECK(launch_zgemm(dX, dP, dQ, D, nb, 1.0 / k, 1.0, stream));
+                    std::swap(dP, dQ);
+                }
+                for (int q = 0; q < sq; ++q) {
+                    HIPCHECK(launch_zgemm(dP, dP, dQ, D, nb, 1.0, 0.0, stream));
+                    std::swap(dP, dQ);
+                }
+                HIPCHECK(launch_to_planes(dP, dU + ((size_t)it0 * per + w) * 2 * Dp * Dp, D, Dp,
+                                          (size_t)per * 2 * Dp * Dp, nb, stream));
             }
         }
-        float *dU = b.U.alloc<float>(Uh.size());
-        HIPCHECK(hipMemcpyAsync(dU, Uh.data(), Uh.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        // X, P, Q are released (stream-ordered) after the conversion
         a.U = dU;
         a.ustride = per;
         a.renorm = factor_decay ? 1 : 0;  // U unitary: keep each column at its initial norm
@@ -379,26 +406,26 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     }
     a.n_units = b.n_units;
     // device arrays
-    double *ddts = b.dts.alloc<double>(dts.size());
+    double *ddts = b.dts.alloc<double>(dts.size(), stream);
     HIPCHECK(hipMemcpyAsync(ddts, dts.data(), dts.size() * sizeof(double), hipMemcpyHostToDevice, stream));
     a.dts = ddts;
-    double *dwy = b.wy.alloc<double>(wyh.size());
+    double *dwy = b.wy.alloc<double>(wyh.size(), stream);
     HIPCHECK(hipMemcpyAsync(dwy, wyh.data(), wyh.size() * sizeof(double), hipMemcpyHostToDevice, stream));
     a.wy = dwy;
-    double *dwt = b.wt.alloc<double>(wth.size());
+    double *dwt = b.wt.alloc<double>(wth.size(), stream);
     HIPCHECK(hipMemcpyAsync(dwt, wth.data(), wth.size() * sizeof(double), hipMemcpyHostToDevice, stream));
     a.wt = dwt;
-    uint32_t *dthr = b.thr.alloc<uint32_t>(thr.size());
+    uint32_t *dthr = b.thr.alloc<uint32_t>(thr.size(), stream);
     HIPCHECK(hipMemcpyAsync(dthr, thr.data(), thr.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     a.thr = dthr;
-    a.yacc = b.yacc.alloc<double>((size_t)b.n_units * b.nobs * 2);
-    a.trace = want_trace ? b.trace.alloc<double>((size_t)b.n_units * ns * b.nobs) : nullptr;
-    a.queue = b.queue.alloc<int>(1);
-    a.status = b.status.alloc<int>(1);
+    a.yacc = b.yacc.alloc<double>((size_t)b.n_units * b.nobs * 2, stream);
+    a.trace = want_trace ? b.trace.alloc<double>((size_t)b.n_units * ns * b.nobs, stream) : nullptr;
+    a.queue = b.queue.alloc<int>(1, stream);
+    a.status = b.status.alloc<int>(1, stream);
     HIPCHECK(hipMemsetAsync(a.status, 0, sizeof(int), stream));
     if (keep_state) {
         const size_t per_unit = b.path == Path::Lanczos ? (size_t)2 * D : (size_t)2 * a.dpad * 16;
-        a.state = b.state.alloc<float>((size_t)b.n_units * per_unit);
+        a.state = b.state.alloc<float>((size_t)b.n_units * per_unit, stream);
     }
     // launch geometry
     if (b.path == Path::Lanczos) {
@@ -406,8 +433,8 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         const int occ = std::max(1, lanczos_occupancy(b.G, b.T, lds));
         b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ));
         const int vecs = b.G == 0 ? a.m + 2 : a.m;  // streaming variant: + residual and state
-        a.basis = b.basis.alloc<float>((size_t)b.slots * vecs * 2 * D);
-        a.norm0 = b.norm0.alloc<double>(b.n_units);
+        a.basis = b.basis.alloc<float>((size_t)b.slots * vecs * 2 * D, stream);
+        a.norm0 = b.norm0.alloc<double>(b.n_units, stream);
         if (a.renorm) HIPCHECK(launch_init_norms(a, push_args(b, a), stream));
     } else {
         const size_t lds = (size_t)8 * a.dpad * 16 + 4096;
@@ -419,14 +446,12 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     b.s_next = 0;
 }
 
-// Copies the launch arguments to the device (ordered on the batch stream).
+// Copies the launch arguments to the device, ordered on the batch stream (a
+// pageable source is staged before the call returns, so it may be reused).
 const StepArgs *push_args(Batch &b, const StepArgs &a) {
-    if (!b.args.p) b.args.alloc<StepArgs>(1);
-    // staged through pinned memory owned by the batch so the async copy sees a stable source
-    if (!b.args_host) HIPCHECK(hipHostMalloc((void **)&b.args_host, sizeof(StepArgs), hipHostMallocDefault));
-    HIPCHECK(hipStreamSynchronize(b.stream));  // the previous launch has read its arguments
-    *b.args_host = a;
-    HIPCHECK(hipMemcpyAsync(b.args.p, b.args_host, sizeof(StepArgs), hipMemcpyHostToDevice, b.stream));
+    if (!b.args.p) b.args.alloc<StepArgs>(1, b.stream);
+    b.args_host = a;
+    HIPCHECK(hipMemcpyAsync(b.args.p, &b.args_host, sizeof(StepArgs), hipMemcpyHostToDevice, b.stream));
     return b.args.get<StepArgs>();
 }
 
@@ -453,13 +478,13 @@ void run_chunk(Batch &b, int steps) {
 void collect(Batch &b, double *yields, double *trace) {
     const int cols_per_item = b.n_units / b.n_items;
     if (yields) {
-        double *out = b.yout.alloc<double>((size_t)b.n_items * b.nobs);
+        double *out = b.yout.alloc<double>((size_t)b.n_items * b.nobs, b.stream);
         HIPCHECK(launch_reduce_columns(b.a.yacc, out, b.n_items, cols_per_item, b.nobs, b.nobs * 2, 2, b.stream));
         HIPCHECK(hipMemcpyAsync(yields, out, (size_t)b.n_items * b.nobs * sizeof(double), hipMemcpyDeviceToHost, b.stream));
     }
     if (trace && b.a.trace) {
         const int nk = b.nsamples * b.nobs;
-        double *out = b.tout.alloc<double>((size_t)b.n_items * nk);
+        double *out = b.tout.alloc<double>((size_t)b.n_items * nk, b.stream);
         HIPCHECK(launch_reduce_columns(b.a.trace, out, b.n_items, cols_per_item, nk, nk, 1, b.stream));
         HIPCHECK(hipMemcpyAsync(trace, out, (size_t)b.n_items * nk * sizeof(double), hipMemcpyDeviceToHost, b.stream));
     }
@@ -506,10 +531,12 @@ struct Request {
     int n_fields = 0;
     std::unique_ptr<Batch> batch;
     std::vector<double> result;
-    bool done = false;
     bool failed = false;
     std::string error;
     int64_t bytes = 0;
+    int stream = -1;             // index of the server stream the request owns
+    bool whole = false;          // launched to completion in one persistent launch
+    hipEvent_t done_ev = nullptr;
 };
 
 struct msg_server {
@@ -521,6 +548,7 @@ struct msg_server {
     std::vector<std::unique_ptr<Request>> active;
     std::map<int64_t, std::unique_ptr<Request>> finished;
     std::vector<hipStream_t> streams;
+    std::vector<char> stream_busy;
 };
 
 extern "C" {
@@ -731,23 +759,24 @@ msg_status msg_server_create(msg_ctx *ctx, int32_t max_slots, int64_t hbm_budget
         s->ctx = ctx;
         s->max_slots = max_slots;
         s->budget = hbm_budget_bytes;
-        const int ns = std::min(max_slots, 4);
-        for (int i = 0; i < ns; ++i) {
+        for (int i = 0; i < max_slots; ++i) {
             hipStream_t st;
             HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
             s->streams.push_back(st);
         }
+        s->stream_busy.assign(max_slots, 0);
         *out = s.release();
     });
 }
 
 msg_status msg_server_destroy(msg_server *srv) {
     if (!srv) return MSG_INVALID_ARG;
-    for (hipStream_t st : srv->streams) {
-        (void)hipStreamSynchronize(st);
-        (void)hipStreamDestroy(st);
-    }
+    for (hipStream_t st : srv->streams) (void)hipStreamSynchronize(st);
+    for (auto &q : srv->active)
+        if (q->done_ev) (void)hipEventDestroy(q->done_ev);
     srv->active.clear();
+    srv->finished.clear();
+    for (hipStream_t st : srv->streams) (void)hipStreamDestroy(st);
     delete srv;
     return MSG_OK;
 }
@@ -777,50 +806,80 @@ msg_status msg_server_submit(msg_server *srv, const msg_system *sys, const msg_r
 }
 
 msg_status msg_server_pump(msg_server *srv, int32_t max_steps, int32_t *n_active) {
-    if (!srv || max_steps < 1) return MSG_INVALID_ARG;
+    if (!srv) return MSG_INVALID_ARG;
     return guarded(srv->ctx, [&] {
         msg_ctx *ctx = srv->ctx;
         HIPCHECK(hipSetDevice(ctx->device));
-        // admission by slot count and HBM footprint, first come first served
-        while (!srv->queued.empty() && (int)srv->active.size() < srv->max_slots &&
-               srv->in_use + srv->queued.front()->bytes <= srv->budget) {
-            std::unique_ptr<Request> q = std::move(srv->queued.front());
-            srv->queued.pop_front();
-            q->batch.reset(new Batch());
-            const hipStream_t st = srv->streams[q->id % srv->streams.size()];
+        auto finish = [&](std::unique_ptr<Request> &q) {
+            q->result.assign((size_t)q->n_fields * q->batch->nobs, 0.0);
             try {
-                prepare(*q->batch, ctx, q->sys, &q->prm, q->n_fields, q->fields.data(), nullptr, 0, false, true, st);
+                collect(*q->batch, q->result.data(), nullptr);
             } catch (const ArgError &e) {
                 q->failed = true;
                 q->error = e.what;
-                q->done = true;
-                q->batch.reset();
-                srv->finished[q->id] = std::move(q);
-                continue;
             }
-            srv->in_use += q->bytes;
-            srv->active.push_back(std::move(q));
-        }
-        // one chunk of every resident request, overlapping on the streams
-        for (auto &q : srv->active) run_chunk(*q->batch, max_steps);
+            srv->in_use -= q->bytes;
+            q->batch.reset();
+            if (q->done_ev) (void)hipEventDestroy(q->done_ev);
+            q->done_ev = nullptr;
+            srv->stream_busy[q->stream] = 0;
+            const int64_t id = q->id;
+            srv->finished[id] = std::move(q);
+        };
+        // 1. completions (non-blocking)
         for (auto it = srv->active.begin(); it != srv->active.end();) {
             Request &q = **it;
-            if (q.batch->s_next >= q.batch->nsamples - 1) {
-                // the last chunk observed the final sample; collect
-                q.result.assign((size_t)q.n_fields * q.batch->nobs, 0.0);
-                try {
-                    collect(*q.batch, q.result.data(), nullptr);
-                } catch (const ArgError &e) {
-                    q.failed = true;
-                    q.error = e.what;
-                }
-                q.done = true;
-                srv->in_use -= q.bytes;
-                q.batch.reset();
-                srv->finished[q.id] = std::move(*it);
+            const bool done = q.whole ? hipEventQuery(q.done_ev) == hipSuccess : q.batch->s_next >= q.batch->nsamples - 1;
+            if (done) {
+                finish(*it);
                 it = srv->active.erase(it);
             } else {
                 ++it;
+            }
+        }
+        // 2. admission by free stream and HBM footprint, first come first served
+        while (!srv->queued.empty() && srv->in_use + srv->queued.front()->bytes <= srv->budget) {
+            int slot = -1;
+            for (int i = 0; i < (int)srv->streams.size(); ++i)
+                if (!srv->stream_busy[i]) { slot = i; break; }
+            if (slot < 0) break;
+            std::unique_ptr<Request> q = std::move(srv->queued.front());
+            srv->queued.pop_front();
+            q->stream = slot;
+            q->batch.reset(new Batch());
+            q->whole = max_steps <= 0;
+            try {
+                prepare(*q->batch, ctx, q->sys, &q->prm, q->n_fields, q->fields.data(), nullptr, 0, false, !q->whole,
+                        srv->streams[slot]);
+            } catch (const ArgError &e) {
+                q->failed = true;
+                q->error = e.what;
+                q->batch.reset();
+                const int64_t id = q->id;
+                srv->finished[id] = std::move(q);
+                continue;
+            }
+            srv->stream_busy[slot] = 1;
+            srv->in_use += q->bytes;
+            if (q->whole) {
+                // one persistent launch over the whole time grid; completion signalled by an event
+                run_chunk(*q->batch, q->batch->nsamples);
+                HIPCHECK(hipEventCreateWithFlags(&q->done_ev, hipEventDisableTiming));
+                HIPCHECK(hipEventRecord(q->done_ev, srv->streams[slot]));
+            }
+            srv->active.push_back(std::move(q));
+        }
+        // 3. chunked mode: one chunk of every resident request, overlapping on the streams
+        if (max_steps > 0) {
+            for (auto &q : srv->active)
+                if (!q->whole) run_chunk(*q->batch, max_steps);
+            for (auto it = srv->active.begin(); it != srv->active.end();) {
+                if (!(*it)->whole && (*it)->batch->s_next >= (*it)->batch->nsamples - 1) {
+                    finish(*it);
+                    it = srv->active.erase(it);
+                } else {
+                    ++it;
+                }
             }
         }
         if (n_active) *n_active = (int32_t)(srv->queued.size() + srv->active.size());

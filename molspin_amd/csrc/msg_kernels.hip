@@ -19,6 +19,8 @@
 // SpinSpace_operators.cpp:1257-1408) for whole batches of items.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "msg_internal.hpp"
 
 namespace msg {
@@ -316,16 +318,18 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-template <int G, int TMAX>
+// DBUF: two LDS buffers hold v_j and v_{j-1} (swapped every iteration); without
+// DBUF one buffer holds v_j and v_{j-1} is re-read from the basis store, which
+// keeps D = 16384 (128 KiB per vector) resident in LDS.
+template <int G, int TMAX, bool DBUF>
 __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict__ pa) {
     const StepArgs &a = *pa;
     extern __shared__ float4 smem[];
     const int T = blockDim.x, t = threadIdx.x, lane = t & 63;
     const int ng = G * T;
-    // two LDS buffers (A/B planes each): v_j and v_{j-1}, swapped every iteration
-    float4 *LA[2] = {smem, smem + 2 * ng};
-    float4 *LB[2] = {smem + ng, smem + 3 * ng};
-    double *scr = reinterpret_cast<double *>(smem + 4 * ng);  // 2 x 16 x 16 doubles
+    float4 *LA[2] = {smem, DBUF ? smem + 2 * ng : smem};
+    float4 *LB[2] = {smem + ng, DBUF ? smem + 3 * ng : smem + ng};
+    double *scr = reinterpret_cast<double *>(smem + (DBUF ? 4 : 2) * ng);  // 2 x 16 x 16 doubles
     __shared__ double ys[kMaxObs], yc[kMaxObs];                // Kahan yield sums (thread 0)
     __shared__ int s_unit;
     const int D = a.dim;
@@ -421,6 +425,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict_
                     __syncthreads();  // v_j complete in buffer `cur`
                     const float4 *cA = LA[cur], *cB = LB[cur];
                     float4 *pA = LA[cur ^ 1], *pB = LB[cur ^ 1];
+                    const float4 *gA = reinterpret_cast<const float4 *>(bas + (size_t)(j > 0 ? j - 1 : 0) * 2 * D);
                     const float fb = (float)bprev;
                     // w = H v_j - beta_{j-1} v_{j-1}; one reduction gives alpha and |w|^2
                     C4 w[G];
@@ -431,7 +436,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict_
                         const C4 vj = load_group(cA, cB, nu);
                         w[g] = apply_h(a, co, nu, vj, cA, cB);
                         if (j > 0) {
-                            const C4 vp = load_group(pA, pB, nu);
+                            const C4 vp = DBUF ? load_group(pA, pB, nu) : load_group(gA, gA + D4, nu);
 #pragma unroll
                             for (int c = 0; c < 4; ++c) {
                                 w[g].c[c].x = fmaf(-fb, vp.c[c].x, w[g].c[c].x);
@@ -483,11 +488,11 @@ __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict_
 #pragma unroll
                         for (int c = 0; c < 4; ++c) w[g].c[c] = make_float2(w[g].c[c].x * ib, w[g].c[c].y * ib);
                         const int nu = g * T + t;
-                        store_group(pA, pB, nu, w[g]);  // v_{j+1} replaces v_{j-1} (own elements only)
+                        store_group(pA, pB, nu, w[g]);  // v_{j+1} replaces v_{j-1} (DBUF) or v_j (own elements only)
                         store_group(bA, bA + D4, nu, w[g]);
                     }
                     bprev = beta;
-                    cur ^= 1;
+                    if (DBUF) cur ^= 1;
                 }
                 // every wavefront evaluates the small exponential for itself
                 double cre, cim;
@@ -977,9 +982,66 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
 }
 
 // ---------------------------------------------------------------------------
+// Device FP64 matrix exponential for the dense path: exp(X) of a batch of
+// n x n complex matrices by a degree-18 Taylor polynomial (Horner form) of
+// X / 2^s followed by s squarings.  Row-major double2 matrices.
+// ---------------------------------------------------------------------------
+constexpr int kZT = 32;  // output tile (kZT x kZT) per workgroup of 256 threads
+
+// C_b = diag_add * I + coef * A_b B_b  for every batch entry b.
+__global__ __launch_bounds__(256) void k_zgemm(const double2 *__restrict__ A, const double2 *__restrict__ B,
+                                               double2 *__restrict__ C, int n, double coef, double diag_add) {
+    __shared__ double2 sA[kZT][kZT + 1], sB[kZT][kZT + 1];
+    const size_t off = (size_t)blockIdx.z * n * n;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads, 4 rows each
+    const int row0 = blockIdx.y * kZT, col0 = blockIdx.x * kZT;
+    double2 acc[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] = make_double2(0.0, 0.0);
+    for (int k0 = 0; k0 < n; k0 += kZT) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = ty + 8 * r;
+            sA[i][tx] = (row0 + i < n && k0 + tx < n) ? A[off + (size_t)(row0 + i) * n + k0 + tx] : make_double2(0.0, 0.0);
+            sB[i][tx] = (k0 + i < n && col0 + tx < n) ? B[off + (size_t)(k0 + i) * n + col0 + tx] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int k = 0; k < kZT; ++k) {
+            const double2 b = sB[k][tx];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double2 a = sA[ty + 8 * r][k];
+                acc[r].x = fma(a.x, b.x, fma(-a.y, b.y, acc[r].x));
+                acc[r].y = fma(a.x, b.y, fma(a.y, b.x, acc[r].y));
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int i = row0 + ty + 8 * r, j = col0 + tx;
+        if (i < n && j < n)
+            C[off + (size_t)i * n + j] = make_double2(coef * acc[r].x + (i == j ? diag_add : 0.0), coef * acc[r].y);
+    }
+}
+
+// Planes of the dense-path operand: U[b] row-major n x n (double2) ->
+// dst[b*stride + k*Dp + row] = re, dst[b*stride + Dp*Dp + k*Dp + row] = im.
+__global__ void k_to_planes(const double2 *__restrict__ U, float *__restrict__ dst, int n, int Dp, size_t stride) {
+    const size_t b = blockIdx.y;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n * n; idx += gridDim.x * blockDim.x) {
+        const int row = idx / n, k = idx - row * n;
+        const double2 v = U[b * n * n + idx];
+        dst[b * stride + (size_t)k * Dp + row] = (float)v.x;
+        dst[b * stride + (size_t)Dp * Dp + (size_t)k * Dp + row] = (float)v.y;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers (called from the host part of the library)
 // ---------------------------------------------------------------------------
-#define MSG_LANCZOS_VARIANTS(X) X(1, 256) X(1, 512) X(1, 1024) X(2, 512) X(2, 1024)
+#define MSG_LANCZOS_VARIANTS(X) X(1, 256, true) X(1, 512, true) X(1, 1024, true) X(2, 512, true) X(2, 1024, true)
 
 hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_thread, int threads, int slots,
                           hipStream_t st) {
@@ -987,11 +1049,11 @@ hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_t
         hipLaunchKernelGGL(k_lanczos_big, dim3(slots), dim3(1024), 0, st, da);
         return hipGetLastError();
     }
-    const size_t lds = (size_t)4 * a.dim * sizeof(float) + 2 * 256 * sizeof(double);
-#define X(G_, T_)                                                                        \
-    if (group_per_thread == G_ && threads <= T_) {                                       \
-        hipLaunchKernelGGL((k_lanczos<G_, T_>), dim3(slots), dim3(threads), lds, st, da); \
-        return hipGetLastError();                                                         \
+    const size_t lds = (size_t)(group_per_thread == 4 ? 2 : 4) * a.dim * sizeof(float) + 2 * 256 * sizeof(double);
+#define X(G_, T_, B_)                                                                         \
+    if (group_per_thread == G_ && threads <= T_) {                                            \
+        hipLaunchKernelGGL((k_lanczos<G_, T_, B_>), dim3(slots), dim3(threads), lds, st, da); \
+        return hipGetLastError();                                                              \
     }
     MSG_LANCZOS_VARIANTS(X)
 #undef X
@@ -1005,7 +1067,7 @@ hipError_t set_lanczos_lds_limits() {
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     const int lim = maxb - 2048;
-#define X(G_, T_) (void)hipFuncSetAttribute((const void *)k_lanczos<G_, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+#define X(G_, T_, B_) (void)hipFuncSetAttribute((const void *)k_lanczos<G_, T_, B_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     MSG_LANCZOS_VARIANTS(X)
 #undef X
     (void)hipFuncSetAttribute((const void *)k_dense, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
@@ -1023,9 +1085,9 @@ int lanczos_occupancy(int group_per_thread, int threads, size_t lds) {
     int nb = 0;
     if (group_per_thread == 0)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big, 1024, 0) == hipSuccess ? nb : 1;
-#define X(G_, T_)                                                                                        \
-    if (group_per_thread == G_ && threads <= T_)                                                         \
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos<G_, T_>, threads, lds) == hipSuccess \
+#define X(G_, T_, B_)                                                                                         \
+    if (group_per_thread == G_ && threads <= T_)                                                              \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos<G_, T_, B_>, threads, lds) == hipSuccess \
                    ? nb : 1;
     MSG_LANCZOS_VARIANTS(X)
 #undef X
@@ -1036,6 +1098,20 @@ int dense_occupancy(size_t lds) {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_dense, 256, lds) != hipSuccess) return 1;
     return nb;
+}
+
+hipError_t launch_zgemm(const void *A, const void *B, void *C, int n, int batch, double coef, double diag_add,
+                        hipStream_t st) {
+    const int nt = (n + kZT - 1) / kZT;
+    hipLaunchKernelGGL(k_zgemm, dim3(nt, nt, batch), dim3(256), 0, st, (const double2 *)A, (const double2 *)B,
+                       (double2 *)C, n, coef, diag_add);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_planes(const void *U, float *dst, int n, int Dp, size_t stride, int batch, hipStream_t st) {
+    const int blocks = std::min(1024, (n * n + 255) / 256);
+    hipLaunchKernelGGL(k_to_planes, dim3(blocks, batch), dim3(256), 0, st, (const double2 *)U, dst, n, Dp, stride);
+    return hipGetLastError();
 }
 
 hipError_t launch_init_norms(const StepArgs &a, const StepArgs *da, hipStream_t st) {
