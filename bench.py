@@ -1,16 +1,24 @@
-"""Benchmark: propagated state-steps/s of the orientation batch (D = 4096).
+"""Benchmark of the Hilbert-space propagation path on one GPU.
 
-Workload: the anisotropic radical pair with 6 + 4 spin-1/2 nuclei (D = 4096),
-field directions from the 1024-point Fibonacci sphere (|B| = 50 uT), symmetric
-recombination k = 1e-3 /ns, dt = 2 ns, Lanczos order 16, complex FP32 states
-with FP64 inner products / exponential / yield sums.  One bench step
-propagates a batch of `--orientations` orientations x 1024 direct columns over
-`--intervals` time steps (one msg_propagate call: upload, persistent kernel,
-reduction, yields back on the host).  A state-step is one column advanced by
-dt including its observable sample.
+Default workload (the headline line): the orientation batch -- the anisotropic
+radical pair with 6 + 4 spin-1/2 nuclei (D = 4096), field directions from the
+1024-point Fibonacci sphere (|B| = 50 uT), symmetric recombination k = 1e-3 /ns,
+dt = 2 ns, Lanczos order 16, complex FP32 states with FP64 inner products /
+exponential / yield sums.  One bench step propagates `--orientations`
+orientations x 1024 direct columns over `--intervals` time steps (one
+msg_propagate call: upload, persistent kernel, reduction, yields back on the
+host).  A state-step is one column advanced by dt including its observable
+sample.
+
+Other workloads (`--workload`), same JSON line format:
+  trajectories  8 + 6 nuclei (D = 65536), `--trajectories` spin-coherent columns,
+                singlet-triplet dephasing 1e-4 /ns, dt = 1 ns (streaming Lanczos);
+  dense         a 7-nucleus pair (D = 512) swept over 64 field strengths on the
+                dense MFMA path (FP64 propagator built once per field).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -19,7 +27,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-HBM_PEAK = 8.0e12  # MI355X, bytes/s
+HBM_PEAK = 8.0e12      # MI355X, bytes/s
+MFMA_F32_PEAK = 157.3e12  # MI355X dense FP32 matrix, flop/s
+METRIC = "propagated state-steps/s + p99 yield-request latency at |dPhiS|<=1e-5 vs FP64"
 
 
 def main():
@@ -27,13 +37,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["orientations", "trajectories", "dense"], default="orientations")
     ap.add_argument("--orientations", type=int, default=32)
-    ap.add_argument("--intervals", type=int, default=50)
+    ap.add_argument("--trajectories", type=int, default=512)
+    ap.add_argument("--intervals", type=int, default=None, help="time steps per bench step")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--requests", type=int, default=40)
-    ap.add_argument("--serve-intervals", type=int, default=20)
+    ap.add_argument("--serve-intervals", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -44,47 +56,37 @@ def main():
 
     from molspin_amd import abi, backend, workloads
 
-    sysobj, cfg = workloads.config2(n_orientations=1024)
-    packed = sysobj.pack()
     ctx = backend.Context(0)
-    sysd = ctx.upload(packed)
-    D = sysd.dim
-    m = cfg["krylov_size"]
-    dt = cfg["time_step"]
-    B = args.orientations
-    T = args.intervals * dt
-    prm = abi.run_params(total_time=T, time_step=dt, method=abi.MSG_METHOD_KRYLOV, krylov_size=m)
-    ncols = sysd.direct_columns()
-    state_steps = B * ncols * args.intervals
-
-    def step(i):
-        first = (i * B) % 1024
-        fields = [cfg["fields"][(first + j) % 1024] for j in range(B)]
-        y, _ = sysd.propagate(prm, fields=fields)
-        return y
+    if args.workload == "orientations":
+        run = orientation_batch(ctx, args)
+    elif args.workload == "trajectories":
+        run = trajectory_batch(ctx, args)
+    else:
+        run = dense_sweep(ctx, args)
 
     for i in range(args.warmup):
-        step(i)
+        run["step"](i)
     torch.cuda.synchronize() if torch.cuda.is_available() else None
     kms = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        y = step(args.warmup + i)
+        run["step"](args.warmup + i)
         kms.append(ctx.last_kernel_ms())
     torch.cuda.synchronize() if torch.cuda.is_available() else None
     wall = time.perf_counter() - t0
-    value = state_steps * args.steps / wall
+    value = run["state_steps"] * args.steps / wall
     kernel_s = float(np.mean(kms)) * 1e-3
-    bytes_per_ss = 8.0 * D * (4 * m + 1)  # streamed-basis Lanczos, complex FP32
-    achieved = bytes_per_ss * state_steps / kernel_s
+    roof = run["roofline"](kernel_s)
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(packed, cfg, args.cpu_seconds)
-    serving = None if args.no_serving else serve(ctx, args.requests, args.serve_intervals)
+        cpu = cpu_baseline(run["packed"], run["cpu_prm"], run["cpu_field"], run["cpu_columns"], args.cpu_seconds)
+    serving = None
+    if not args.no_serving and args.workload == "orientations":
+        serving = serve(ctx, args.requests, args.serve_intervals)
 
     line = {
-        "metric": "propagated state-steps/s + p99 yield-request latency at |dPhiS|<=1e-5 vs FP64",
+        "metric": METRIC,
         "value": value,
         "unit": "state-steps/s",
         "n_gpus": 1,
@@ -95,19 +97,131 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "complex-f32 states, f64 reductions",
-        "data": "synthetic (seeded radical pair, Fibonacci field directions)",
-        "config": {"workload": "orientation batch: D=4096 (6+4 spin-1/2 nuclei, anisotropic HF), "
-                               "%d orientations x %d columns x %d steps of 2 ns per step, Lanczos m=%d"
-                               % (B, ncols, args.intervals, m),
-                   "orientations_per_step": B, "columns": ncols, "intervals": args.intervals,
-                   "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel_ms": kernel_s * 1e3, "bytes_per_state_step": bytes_per_ss},
+        "data": run["data"],
+        "config": run["config"],
+        "roofline": roof,
         "cpu_baseline": cpu,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)
+
+
+def lanczos_roofline(D, m, state_steps, kernel_s, traffic):
+    """Matrix-free Lanczos: algorithmic HBM bytes per state-step 8 D (4m + 1)
+    (m iterations reading v_j, v_{j-1} and writing v_{j+1}, plus the m + 1
+    vectors of the final combination, complex FP32)."""
+    bytes_per_ss = 8.0 * D * (4 * m + 1)
+    achieved = bytes_per_ss * state_steps / kernel_s
+    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": traffic,
+            "kernel_ms": kernel_s * 1e3, "bytes_per_state_step": bytes_per_ss,
+            "state_steps_per_launch": state_steps}
+
+
+# HBM bytes per state-step of the dominant kernel, from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE runs of each workload at a smaller launch (calibrated against a
+# device copy: the FETCH_SIZE count is doubled, WRITE_SIZE taken as is); the
+# traffic of a launch scales with its state-steps at fixed D.  The counter runs
+# are summarised in profiles/k_lanczos_orientation_batch_D4096.md,
+# profiles/k_lanczos_big_trajectories_D65536.md and profiles/k_dense_field_sweep_D512.md.
+MEASURED_TRAFFIC_PER_STATE_STEP = {"orientations": 1.073e+06, "trajectories": 9.309e+07, "dense": 3.648e+04}
+
+
+def orientation_batch(ctx, args):
+    from molspin_amd import abi, workloads
+    sysobj, cfg = workloads.config2(n_orientations=1024)
+    packed = sysobj.pack()
+    sysd = ctx.upload(packed)
+    D, m, dt = sysd.dim, cfg["krylov_size"], cfg["time_step"]
+    B = args.orientations
+    intervals = args.intervals or 50
+    prm = abi.run_params(total_time=intervals * dt, time_step=dt, method=abi.MSG_METHOD_KRYLOV, krylov_size=m)
+    ncols = sysd.direct_columns()
+    state_steps = B * ncols * intervals
+
+    def step(i):
+        first = (i * B) % 1024
+        sysd.propagate(prm, fields=[cfg["fields"][(first + j) % 1024] for j in range(B)])
+
+    def roof(kernel_s):
+        tr = MEASURED_TRAFFIC_PER_STATE_STEP.get("orientations")
+        return lanczos_roofline(D, m, state_steps, kernel_s, None if tr is None else tr * state_steps)
+
+    return dict(step=step, state_steps=state_steps, roofline=roof, packed=packed,
+                cpu_prm=abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_KRYLOV),
+                cpu_field=cfg["fields"][0], cpu_columns=None,
+                data="synthetic (seeded radical pair, Fibonacci field directions)",
+                config={"workload": "orientation batch: D=4096 (6+4 spin-1/2 nuclei, anisotropic HF), "
+                                    "%d orientations x %d columns x %d steps of 2 ns per step, Lanczos m=%d"
+                                    % (B, ncols, intervals, m),
+                        "orientations_per_step": B, "columns": ncols, "intervals": intervals,
+                        "parallelism": "single GPU"})
+
+
+def trajectory_batch(ctx, args):
+    from molspin_amd import abi, workloads
+    M = args.trajectories
+    sysobj, cfg = workloads.config3(n_traj=M)
+    packed = sysobj.pack()
+    sysd = ctx.upload(packed)
+    D, m, dt = sysd.dim, cfg["krylov_size"], cfg["time_step"]
+    intervals = args.intervals or 20
+    cols = sysd.trace_samples(sysobj.state_index("S"), M, cfg["sample_seed"]) / math.sqrt(M)
+    prm = abi.run_params(total_time=intervals * dt, time_step=dt, method=abi.MSG_METHOD_KRYLOV, krylov_size=m,
+                         sampling=abi.MSG_SAMPLING_GIVEN, seed=2026)
+    state_steps = M * intervals
+
+    def step(i):
+        sysd.propagate(prm, columns=cols)
+
+    def roof(kernel_s):
+        tr = MEASURED_TRAFFIC_PER_STATE_STEP.get("trajectories")
+        return lanczos_roofline(D, m, state_steps, kernel_s, None if tr is None else tr * state_steps)
+
+    cpu_prm = abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_KRYLOV,
+                             sampling=abi.MSG_SAMPLING_GIVEN, seed=2026)
+    return dict(step=step, state_steps=state_steps, roofline=roof, packed=packed, cpu_prm=cpu_prm,
+                cpu_field=None, cpu_columns=cols,
+                data="synthetic (seeded radical pair, host-drawn spin-coherent nuclear states)",
+                config={"workload": "stochastic trajectories: D=65536 (8+6 spin-1/2 nuclei, isotropic HF), "
+                                    "%d trajectories x %d steps of 1 ns, ST dephasing 1e-4/ns, Lanczos m=%d"
+                                    % (M, intervals, m),
+                        "trajectories": M, "intervals": intervals, "parallelism": "single GPU"})
+
+
+def dense_sweep(ctx, args):
+    from molspin_amd import abi, workloads
+    idx = next(i for i in range(1000) if workloads.config4_pair(i)[1]["n"] == 7)
+    sysobj, cfg = workloads.config4_pair(idx)
+    packed = sysobj.pack()
+    sysd = ctx.upload(packed)
+    D, dt = sysd.dim, cfg["time_step"]
+    intervals = args.intervals or 500
+    prm = abi.run_params(total_time=intervals * dt, time_step=dt, method=abi.MSG_METHOD_EXPONENTIAL)
+    ncols = sysd.direct_columns()
+    nf = len(cfg["fields"])
+    state_steps = nf * ncols * intervals
+
+    def step(i):
+        sysd.propagate(prm, fields=cfg["fields"])
+
+    def roof(kernel_s):
+        flops = 8.0 * D * D * state_steps
+        tr = MEASURED_TRAFFIC_PER_STATE_STEP.get("dense")
+        return {"bound": "mfma", "achieved": flops / kernel_s / 1e12, "peak": MFMA_F32_PEAK / 1e12,
+                "unit": "TFLOP/s", "frac": flops / kernel_s / MFMA_F32_PEAK,
+                "traffic": None if tr is None else tr * state_steps,
+                "kernel_ms": kernel_s * 1e3, "flops_per_state_step": 8.0 * D * D,
+                "state_steps_per_launch": state_steps}
+
+    return dict(step=step, state_steps=state_steps, roofline=roof, packed=packed,
+                cpu_prm=abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_EXPONENTIAL),
+                cpu_field=cfg["fields"][0], cpu_columns=None,
+                data="synthetic (seeded radical pair, log-spaced field strengths)",
+                config={"workload": "dense field sweep: D=512 (7 spin-1/2 nuclei, anisotropic HF, pair %d), "
+                                    "%d fields x %d columns x %d steps of 2 ns, FP32 MFMA propagation"
+                                    % (idx, nf, ncols, intervals),
+                        "fields": nf, "columns": ncols, "intervals": intervals, "parallelism": "single GPU"})
 
 
 def serve(ctx, n_requests, intervals):
@@ -115,8 +229,12 @@ def serve(ctx, n_requests, intervals):
     field-sweep yield requests (n ~ U{2..12} nuclei, 16-128 fields each), each
     propagated over `intervals` steps of 2 ns; requests are admitted by HBM
     footprint into a continuous-batching server (one persistent launch per request
-    on its own stream, up to 8 overlapping).
-    Latency = completion - arrival.  Returns p50/p99 and state-steps/s."""
+    on its own stream, up to 8 overlapping).  Latency = completion - arrival.
+
+    The plain form is the same requests one system and one field per call,
+    sequentially; its time is estimated per request from two timed single-field
+    calls (scaled by the request's field count) so the comparison covers the
+    same mix without doubling the run time."""
     import numpy as np
     from molspin_amd import abi, backend, workloads
     reqs, t_arr = [], 0.0
@@ -153,41 +271,46 @@ def serve(ctx, n_requests, intervals):
     srv.close()
     lat = np.array(lat)
     total = sum(r["steps"] for r in reqs)
-    # plain one-system-at-a-time form: the first 6 requests, one field per call
-    t1 = time.perf_counter()
-    plain_steps = 0
-    for r in reqs[:6]:
-        for f in r["fields"]:
+    plain_s = 0.0
+    for r in reqs:
+        r["sys"].propagate(r["prm"], fields=[r["fields"][0]])  # first call per system uploads factors
+        t1 = time.perf_counter()
+        for f in r["fields"][1:3]:
             r["sys"].propagate(r["prm"], fields=[f])
-        plain_steps += r["steps"]
-    plain = plain_steps / (time.perf_counter() - t1)
+        plain_s += (time.perf_counter() - t1) / 2 * len(r["fields"])
+    plain = total / plain_s
     return {"requests": n_requests, "p50_ms": float(np.percentile(lat, 50) * 1e3),
             "p99_ms": float(np.percentile(lat, 99) * 1e3), "state_steps_per_s": total / elapsed,
             "plain_one_at_a_time_state_steps_per_s": plain, "served_over_plain": total / elapsed / plain,
             "elapsed_s": elapsed, "intervals_per_request": intervals,
-            "workload": "Poisson 200 req/s, config-4-style pairs (D 16..16384), 16-128 fields/request"}
+            "workload": "Poisson 200 req/s, seeded radical pairs (D 16..16384), 16-128 fields/request"}
 
 
-def cpu_baseline(packed, cfg, seconds):
-    """FP64 oracle (the reference algorithm: Arnoldi m=16 + Pade expm, CSR
-    matvecs) on a bounded sample of the same workload, all host threads."""
-    from molspin_amd import abi
+def cpu_baseline(packed, prm, field, columns, seconds):
+    """FP64 oracle (the reference algorithm: Arnoldi m=16 + Pade expm on the
+    Krylov matrix, CSR matvecs; dense Pade propagator on the dense workload) on
+    a bounded sample of the same workload, OpenMP over columns on the host
+    threads."""
     from oracle import binding as ref
     threads = min(16, os.cpu_count() or 1)
-    prm = abi.run_params(total_time=cfg["time_step"] * 4000, time_step=cfg["time_step"], method=abi.MSG_METHOD_KRYLOV)
     cols = 2 * threads
+    if columns is not None:
+        cols = min(cols, columns.shape[1])
     steps = 4
-    # calibrate to about `seconds` of work
+    kw = dict(col_range=(0, cols), threads=threads)
+    if field is not None:
+        kw["field"] = field
+    if columns is not None:
+        kw["columns"] = columns
     t0 = time.perf_counter()
-    ref.run_columns(packed, prm, field=cfg["fields"][0], col_range=(0, cols), max_steps=steps, threads=threads)
+    ref.run_columns(packed, prm, max_steps=steps, **kw)
     t1 = time.perf_counter() - t0
     steps = max(4, min(4000, int(steps * seconds / max(t1, 1e-3))))
     t0 = time.perf_counter()
-    ref.run_columns(packed, prm, field=cfg["fields"][0], col_range=(0, cols), max_steps=steps, threads=threads)
+    ref.run_columns(packed, prm, max_steps=steps, **kw)
     el = time.perf_counter() - t0
     return {"value": cols * steps / el, "unit": "state-steps/s", "cores": threads, "kind": "port",
-            "sample": "1 orientation x %d direct columns x %d steps (FP64 Arnoldi m=16, %d OpenMP threads)"
-                      % (cols, steps, threads)}
+            "sample": "1 item x %d columns x %d steps (FP64 oracle, %d OpenMP threads)" % (cols, steps, threads)}
 
 
 if __name__ == "__main__":

@@ -151,7 +151,9 @@ enum {
 enum {
     MSG_TASK_HSGENERAL = 0,            /* type=HSGeneral (TaskHSGeneral.cpp:267-652)               */
     MSG_TASK_LEGACY_DIRECT_YIELDS = 1, /* type=statichs-direct-yields (TaskStaticHSDirectYields.cpp) */
-    MSG_TASK_LEGACY_STOCH_YIELDS = 2   /* type=statichs-stoch-yields (TaskStaticHSStochYields.cpp)   */
+    MSG_TASK_LEGACY_STOCH_YIELDS = 2,  /* type=statichs-stoch-yields (TaskStaticHSStochYields.cpp)   */
+    MSG_TASK_SYMMETRIC_DECAY = 3       /* type=statichs-symmetricdecay: msg_symmetric_decay, not
+                                          msg_propagate (TaskStaticHSSymmetricDecay.cpp:33-171)     */
 };
 
 typedef struct msg_run_params {
@@ -170,6 +172,36 @@ typedef struct msg_run_params {
                                      e^{-k t} applied analytically (:257-292) */
     int32_t  reserved;
 } msg_run_params;
+
+/* propagationmethod as written in the plan (HSExecutionPlan.h:29) */
+enum {
+    MSG_PROP_EXPONENTIAL = 0,     /* normal / exp / exponential / matrixexponential */
+    MSG_PROP_AUTOEXPM = 1,        /* autoexpm                                       */
+    MSG_PROP_KRYLOV = 2,          /* krylov                                         */
+    MSG_PROP_RK4 = 3              /* rk4 / explicit (not on the device path)        */
+};
+
+/* A resolved Task property list (ResolveExecutionPlan, HSExecutionPlan.cpp:89-615). */
+typedef struct msg_plan {
+    msg_run_params params;        /* run parameters for msg_propagate / msg_symmetric_decay */
+    int32_t montecarlo_samples;   /* montecarlosamples (default 1): columns for sampling=stochastic */
+    int32_t sampling_method;      /* samplingmethod: MSG_TRACE_SUZ (default) or MSG_TRACE_COHERENT */
+    int32_t auto_seed;            /* autoseed (default true) */
+    int32_t device_backend;       /* backend=mi355x (default cpu) */
+    int32_t time_infinity;        /* method=timeinf (calculation=yields) */
+    int32_t propagation;          /* MSG_PROP_* */
+} msg_plan;
+
+/*
+ * Resolves the property list of an MSD Task block, "key=value; key=value; ...",
+ * with ResolveExecutionPlan's keys, aliases, defaults and error messages.  Keys
+ * selecting physics off this path (dynamics=dynamic, powder grids, secular
+ * approximation, spin-polarisation/CIDSP output, pulse sequences, rk4) return
+ * MSG_UNSUPPORTED naming the key; invalid values return MSG_INVALID_ARG with the
+ * reference's message.  `error` (error_len bytes, may be NULL) receives the message.
+ * Host-only: needs no device.
+ */
+msg_status msg_resolve_plan(const char *task_properties, msg_plan *plan, char *error, int32_t error_len);
 
 /* ------------------------------------------------------------------------- */
 /* Context and system handles                                               */
@@ -248,6 +280,18 @@ msg_status msg_propagate(msg_ctx *ctx, const msg_system *sys, const msg_run_para
                          int32_t n_items, const double *fields,
                          const double *columns, int32_t n_columns,
                          double *yields, double *trace);
+
+/*
+ * Infinite-time yields of every State object under symmetric decay at rate
+ * `rate` (TaskStaticHSSymmetricDecay::RunLocal, TaskStaticHSSymmetricDecay.cpp:33-171):
+ *   Phi_P = sum_{g,h} Re(<g|P|h><h|rho0|g>) k^2 / (k^2 + (lambda_g - lambda_h)^2)
+ * over the eigenpairs of H (FP64 on the device), rho0 = sum of the initial-state
+ * projectors / trace.  rate = 0 takes the reference's rule: the first Transition's
+ * rate, else 1e-3 /ns.  fields: n_items x 3 (NULL: as described).
+ * yields: n_items x n_states.  States on the radical-pair electrons, D <= 4096.
+ */
+msg_status msg_symmetric_decay(msg_ctx *ctx, const msg_system *sys, int32_t n_items, const double *fields,
+                               double rate, double *yields);
 
 /*
  * Per-step compatibility entry with the meaning of HSPropagator::StepStochastic

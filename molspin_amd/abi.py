@@ -34,6 +34,11 @@ MSG_SAMPLING_GIVEN = 1
 MSG_TASK_HSGENERAL = 0
 MSG_TASK_LEGACY_DIRECT_YIELDS = 1
 MSG_TASK_LEGACY_STOCH_YIELDS = 2
+MSG_TASK_SYMMETRIC_DECAY = 3
+MSG_PROP_EXPONENTIAL = 0
+MSG_PROP_AUTOEXPM = 1
+MSG_PROP_KRYLOV = 2
+MSG_PROP_RK4 = 3
 MSG_TRACE_COHERENT = 0
 MSG_TRACE_SUZ = 1
 
@@ -85,6 +90,12 @@ class RunParams(C.Structure):
                 ("sampling", C.c_int32), ("yield_corrections", C.c_int32),
                 ("total_time", C.c_double), ("time_step", C.c_double),
                 ("seed", C.c_uint64), ("task", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Plan(C.Structure):
+    _fields_ = [("params", RunParams), ("montecarlo_samples", C.c_int32), ("sampling_method", C.c_int32),
+                ("auto_seed", C.c_int32), ("device_backend", C.c_int32), ("time_infinity", C.c_int32),
+                ("propagation", C.c_int32)]
 
 
 def run_params(calculation=MSG_CALC_YIELDS, observables=MSG_OBS_TRANSITION_YIELDS,

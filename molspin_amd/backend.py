@@ -19,7 +19,7 @@ EXPORTS = [
     "msg_system_dimension", "msg_query_footprint", "msg_direct_columns", "msg_observable_count",
     "msg_time_grid", "msg_propagate", "msg_step", "msg_server_create", "msg_server_destroy",
     "msg_server_submit", "msg_server_pump", "msg_server_collect", "msg_last_kernel_time",
-    "msg_trace_samples",
+    "msg_trace_samples", "msg_symmetric_decay", "msg_resolve_plan",
 ]
 
 
@@ -57,6 +57,8 @@ def lib():
         L.msg_server_collect.argtypes = [vp, C.c_int64, vp, P(C.c_int32)]
         L.msg_last_kernel_time.argtypes = [vp, P(C.c_double), P(C.c_int32)]
         L.msg_trace_samples.argtypes = [vp, C.c_int32, C.c_int32, C.c_uint64, C.c_int32, vp]
+        L.msg_symmetric_decay.argtypes = [vp, vp, C.c_int32, vp, C.c_double, vp]
+        L.msg_resolve_plan.argtypes = [C.c_char_p, P(abi.Plan), C.c_char_p, C.c_int32]
         for name in EXPORTS:
             getattr(L, name).restype = C.c_int if name != "msg_last_error" else C.c_char_p
         _LIB = L
@@ -70,6 +72,17 @@ def _ptr(a):
 def _columns(columns):
     cc = np.asfortranarray(columns, dtype=np.complex128)
     return cc.reshape(-1, order="F").view(np.float64).copy(), cc.shape[1]
+
+
+def resolve_plan(task_properties):
+    """msg_resolve_plan: an MSD Task property list -> abi.Plan (host only).
+    Raises BackendError with the resolver's message."""
+    plan = abi.Plan()
+    err = C.create_string_buffer(512)
+    st = lib().msg_resolve_plan(task_properties.encode(), C.byref(plan), err, len(err))
+    if st != abi.MSG_OK:
+        raise BackendError(st, err.value.decode())
+    return plan
 
 
 class Context:
@@ -166,6 +179,15 @@ class System:
         self.ctx.check(lib().msg_propagate(self.ctx.h, self.h, C.byref(params), n_items, _ptr(f), _ptr(cols), ncol,
                                            _ptr(y), _ptr(tr)))
         return y.reshape(n_items, nobs), (tr.reshape(n_items, ns, nobs) if trace else None)
+
+    def symmetric_decay(self, fields=None, rate=0.0):
+        """Infinite-time yields of every State under symmetric decay
+        (TaskStaticHSSymmetricDecay); returns [n_items, n_states]."""
+        n_items = 1 if fields is None else len(fields)
+        f = None if fields is None else np.ascontiguousarray(fields, dtype=np.float64).reshape(-1)
+        y = np.zeros(n_items * self.packed.desc.n_states)
+        self.ctx.check(lib().msg_symmetric_decay(self.ctx.h, self.h, n_items, _ptr(f), float(rate), _ptr(y)))
+        return y.reshape(n_items, -1)
 
     def step(self, params, factors, dt, step_index=0, field=None):
         """HSPropagator::StepStochastic equivalent; returns the new factors."""

@@ -17,7 +17,10 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
              "-Wall", "-Wno-unused-parameter", "-Wno-unused-variable"]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 
-SOURCES = [("msg_kernels.hip", "hip"), ("msg_abi.cpp", "host"), ("msg_resolve.cpp", "host")]
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+LIBS = ["-L" + os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas", "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+
+SOURCES = [("msg_kernels.hip", "hip"), ("msg_abi.cpp", "host"), ("msg_resolve.cpp", "host"), ("msg_plan.cpp", "host")]
 
 
 def build(verbose=False):
@@ -36,7 +39,8 @@ def build(verbose=False):
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = ["hipcc", "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    # rocSOLVER (Hermitian eigensolver of the symmetric-decay entry) and its rocBLAS handle
+    cmd = ["hipcc", "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + LIBS
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

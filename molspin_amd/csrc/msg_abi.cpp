@@ -6,6 +6,8 @@
 // (HSObservableCollector::Evaluate), HSPropagator::StepStochastic and the
 // trapezoid / finite-time correction (HSObservableCollector.cpp:304-392).
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
@@ -34,7 +36,10 @@ hipError_t launch_reduce_columns(const double *in, double *out, int n_items, int
                                  hipStream_t st);
 int lanczos_occupancy(int group_per_thread, int threads, size_t lds);
 hipError_t launch_zgemm(const void *A, const void *B, void *C, int n, int batch, double coef, double diag_add,
-                        hipStream_t st);
+                        hipStream_t st, bool conj_b = false);
+hipError_t launch_blockdiag_right(const void *M, const double *B4, double scale, void *Y, int n, hipStream_t st);
+hipError_t launch_lorentz_sum(const void *Pt, const void *Rt, const double *lam, double k, int n, double *rows,
+                              double *out, hipStream_t st);
 hipError_t launch_to_planes(const void *U, float *dst, int n, int Dp, size_t stride, int batch, hipStream_t st);
 int dense_occupancy(size_t lds);
 }  // namespace msg
@@ -49,6 +54,7 @@ struct msg_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double kernel_ms = 0.0;
     int launches = 0;
+    rocblas_handle blas = nullptr;  // created on first use (eigensolver), bound to `stream`
 };
 
 struct msg_system {
@@ -310,15 +316,18 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     if (b.path == Path::Lanczos) {
         const int ng = D / 4;
         // launch shape: groups per thread G, threads T (one workgroup per column);
-        // G = 0 selects the streaming variant once two Krylov vectors exceed LDS
+        // G = 0 selects the streaming variant once two Krylov vectors exceed LDS,
+        // G = -1 the streaming variant with v_j mirrored in LDS (one vector fits)
         if (ng <= 512) { b.T = ng; b.G = 1; }
         else if (ng <= 2048) { b.G = 2; b.T = ng / 2; }
+        else if (ng <= 4096) { b.G = -1; b.T = 1024; }
         else { b.G = 0; b.T = 1024; }
         if (const char *env = std::getenv("MSG_LANCZOS_G")) {  // launch-shape override (tuning runs)
             const int g = std::atoi(env);
             if ((g == 1 || g == 2) && ng / g >= 64 && ng / g <= 1024) { b.G = g; b.T = ng / g; }
+            if (g == 0 && ng > 2048) { b.G = 0; b.T = 1024; }
         }
-        if (b.G != 0 && (b.T < 64 || b.T > 1024 || b.G * b.T != ng))
+        if (b.G > 0 && (b.T < 64 || b.T > 1024 || b.G * b.T != ng))
             throw ArgError{MSG_UNSUPPORTED, "no matrix-free launch shape for this dimension"};
         a.nnuc = (int)r.nuc.size();
         std::vector<float> all, one;
@@ -429,10 +438,10 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     }
     // launch geometry
     if (b.path == Path::Lanczos) {
-        const size_t lds = b.G == 0 ? 0 : (size_t)16 * D + 4096;
+        const size_t lds = b.G == 0 ? 0 : b.G < 0 ? (size_t)8 * D : (size_t)16 * D + 4096;
         const int occ = std::max(1, lanczos_occupancy(b.G, b.T, lds));
         b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ));
-        const int vecs = b.G == 0 ? a.m + 2 : a.m;  // streaming variant: + residual and state
+        const int vecs = b.G <= 0 ? a.m + 2 : a.m;  // streaming variants: + residual and state
         a.basis = b.basis.alloc<float>((size_t)b.slots * vecs * 2 * D, stream);
         a.norm0 = b.norm0.alloc<double>(b.n_units, stream);
         if (a.renorm) HIPCHECK(launch_init_norms(a, push_args(b, a), stream));
@@ -582,6 +591,7 @@ msg_status msg_ctx_destroy(msg_ctx *ctx) {
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     delete ctx;
     return MSG_OK;
 }
@@ -664,7 +674,8 @@ msg_status msg_query_footprint(const msg_system *sys, const msg_run_params *p, i
         const int64_t units = (int64_t)n_items * ncols;
         b += (int64_t)n_items * (kCoefHead + (int64_t)r.nuc.size() * kNucStride) * 4;
         b += units * (nobs * 16 + 8);
-        b += std::min<int64_t>(units, 2048) * m * 8 * D;  // Krylov bases of the resident workgroups
+        const int vecs = D > 8192 ? m + 2 : m;  // streaming variants keep residual and state beside the basis
+        b += std::min<int64_t>(units, 2048) * vecs * 8 * D;  // Krylov bases of the resident workgroups
     } else {
         const int64_t Dp = (D + 15) / 16 * 16;
         const int64_t units = (int64_t)n_items * ((ncols + 15) / 16);
@@ -742,6 +753,83 @@ msg_status msg_step(msg_ctx *ctx, const msg_system *sys, const msg_run_params *p
                 factors[2 * ((size_t)j * D + i)] = decay * re;
                 factors[2 * ((size_t)j * D + i) + 1] = decay * im;
             }
+    });
+}
+
+msg_status msg_symmetric_decay(msg_ctx *ctx, const msg_system *sys, int32_t n_items, const double *fields,
+                               double rate, double *yields) {
+    if (!ctx || !sys || !yields || n_items < 1) return fail(ctx, MSG_INVALID_ARG, "null argument or no items");
+    return guarded(ctx, [&] {
+        const Resolved &r = sys->r;
+        if (!r.pair_ok)
+            throw ArgError{MSG_UNSUPPORTED, "symmetric decay on the device needs States on the radical-pair electrons"};
+        if (r.initial.empty()) throw ArgError{MSG_INVALID_ARG, "no initial state"};
+        if (r.D > 4096) throw ArgError{MSG_UNSUPPORTED, "symmetric decay on the device is limited to D <= 4096"};
+        if (rate < 0.0 || std::isnan(rate)) throw ArgError{MSG_INVALID_ARG, "rate must be positive (0: from the system)"};
+        // the reference's rule when no rate is given: first Transition's rate, else 1e-3
+        const double k = rate > 0.0 ? rate : (r.first_rate >= 0.0 ? r.first_rate : 1e-3);
+        if (!(k > 0.0)) throw ArgError{MSG_INVALID_ARG, "decay rate must be positive"};
+        const int D = (int)r.D;
+        const int S = (int)r.P_states.size();
+        hipStream_t st = ctx->stream;
+        if (!ctx->blas) {
+            if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
+                throw ArgError{MSG_DEVICE_ERROR, "rocblas_create_handle failed"};
+        }
+        if (rocblas_set_stream(ctx->blas, st) != rocblas_status_success)
+            throw ArgError{MSG_DEVICE_ERROR, "rocblas_set_stream failed"};
+        // rho0 = (sum_i P_i) (x) 1 / tr(...)  (TaskStaticHSSymmetricDecay.cpp:100-112)
+        double rho[32] = {0.0};
+        double tr = 0.0;
+        for (int s0 : r.initial) {
+            const Mat4 &P = r.P_states[s0];
+            for (int i = 0; i < 16; ++i) {
+                rho[2 * i] += P[i].real();
+                rho[2 * i + 1] += P[i].imag();
+            }
+            tr += (P[0] + P[5] + P[10] + P[15]).real();
+        }
+        if (!(tr > 0.0)) throw ArgError{MSG_INVALID_ARG, "initial state has zero trace"};
+        const double rho_scale = 1.0 / (tr * (double)(r.D / 4));
+        DBuf A, Y, Rt, Pt, lam, off, info, rows, res;
+        cd *dA = A.alloc<cd>((size_t)D * D, st);
+        cd *dY = Y.alloc<cd>((size_t)D * D, st);
+        cd *dR = Rt.alloc<cd>((size_t)D * D, st);
+        cd *dP = Pt.alloc<cd>((size_t)D * D, st);
+        double *dl = lam.alloc<double>(D, st);
+        double *de = off.alloc<double>(D, st);
+        rocblas_int *di = info.alloc<rocblas_int>(1, st);
+        double *drows = rows.alloc<double>(D, st);
+        double *dres = res.alloc<double>((size_t)n_items * S, st);
+        std::vector<cd> H;
+        for (int it = 0; it < n_items; ++it) {
+            dense_hamiltonian(r, fields ? fields + 3 * it : nullptr, H);  // device order, row-major
+            HIPCHECK(hipMemcpyAsync(dA, H.data(), (size_t)D * D * sizeof(cd), hipMemcpyHostToDevice, st));
+            // The solver reads the row-major H as column-major, i.e. conj(H) = W L W^dagger; the
+            // row-major view of W is then M = W^T = V^dagger with H = V L V^dagger.
+            if (rocsolver_zheevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, D,
+                                 reinterpret_cast<rocblas_double_complex *>(dA), D, dl, de, di) !=
+                rocblas_status_success)
+                throw ArgError{MSG_DEVICE_ERROR, "rocsolver_zheevd failed"};
+            rocblas_int hinfo = 0;
+            HIPCHECK(hipMemcpyAsync(&hinfo, di, sizeof(hinfo), hipMemcpyDeviceToHost, st));
+            HIPCHECK(launch_blockdiag_right(dA, rho, rho_scale, dY, D, st));       // Y = M rho0
+            HIPCHECK(launch_zgemm(dY, dA, dR, D, 1, 1.0, 0.0, st, true));          // Rt = M rho0 M^dagger
+            for (int s0 = 0; s0 < S; ++s0) {
+                double P[32];
+                for (int i = 0; i < 16; ++i) {
+                    P[2 * i] = r.P_states[s0][i].real();
+                    P[2 * i + 1] = r.P_states[s0][i].imag();
+                }
+                HIPCHECK(launch_blockdiag_right(dA, P, 1.0, dY, D, st));           // Y = M P
+                HIPCHECK(launch_zgemm(dY, dA, dP, D, 1, 1.0, 0.0, st, true));      // Pt = M P M^dagger
+                HIPCHECK(launch_lorentz_sum(dP, dR, dl, k, D, drows, dres + (size_t)it * S + s0, st));
+            }
+            HIPCHECK(hipStreamSynchronize(st));
+            if (hinfo != 0) throw ArgError{MSG_NUMERIC_BREAKDOWN, "eigensolver did not converge"};
+        }
+        HIPCHECK(hipMemcpyAsync(yields, dres, (size_t)n_items * S * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
     });
 }
 

@@ -545,8 +545,13 @@ __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict_
 // the operator gathers neighbour groups from the basis vector in L2/MALL.
 // Per iteration: pass A computes w = H v_j - beta v_{j-1} with alpha and |w|^2
 // in one reduction; pass B writes v_{j+1} = (w - alpha v_j) / beta.
+// LV: v_j is also mirrored in LDS (8 D bytes, D <= 16384) so the matvec's
+// neighbour reads stay on chip; the Krylov basis, residual and state stay in
+// the per-slot global scratch either way.
+template <bool LV>
 __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict__ pa) {
     const StepArgs &a = *pa;
+    extern __shared__ float4 lvs[];
     __shared__ double scr[2 * 17 * 16];  // block_sum of up to 17 values, double-buffered
     __shared__ double ys[kMaxObs], yc[kMaxObs];
     __shared__ int s_unit;
@@ -625,12 +630,13 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
 #pragma unroll
                         for (int c = 0; c < 4; ++c) x.c[c] = make_float2(x.c[c].x * inv, x.c[c].y * inv);
                         store_group(BA(0), BA(0) + D4, nu, x);
+                        if (LV) store_group(lvs, lvs + D4, nu, x);
                     }
                     double my_al = 0.0, my_be = 0.0, bprev = 0.0;
                     int meff = meff0;
                     for (int j = 0; j < meff0; ++j) {
                         __syncthreads();  // v_j written by the whole workgroup
-                        const float4 *vA = BA(j), *vB = BA(j) + D4;
+                        const float4 *vA = LV ? lvs : BA(j), *vB = vA + D4;
                         const float4 *pA = BA(j > 0 ? j - 1 : 0), *pB = pA + D4;
                         const float fb = (float)bprev;
                         double red[2] = {0.0, 0.0};
@@ -687,6 +693,7 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
                             for (int c = 0; c < 4; ++c)
                                 w.c[c] = make_float2(fmaf(-fa, vj.c[c].x, w.c[c].x) * ib, fmaf(-fa, vj.c[c].y, w.c[c].y) * ib);
                             store_group(nA, nB, nu, w);
+                            if (LV) store_group(lvs, lvs + D4, nu, w);  // own elements: read above by this thread only
                         }
                         bprev = beta;
                     }
@@ -988,9 +995,11 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
 // ---------------------------------------------------------------------------
 constexpr int kZT = 32;  // output tile (kZT x kZT) per workgroup of 256 threads
 
-// C_b = diag_add * I + coef * A_b B_b  for every batch entry b.
+// C_b = diag_add * I + coef * A_b op(B_b) for every batch entry b; op(B) = B, or
+// B^dagger when conj_b is set.
 __global__ __launch_bounds__(256) void k_zgemm(const double2 *__restrict__ A, const double2 *__restrict__ B,
-                                               double2 *__restrict__ C, int n, double coef, double diag_add) {
+                                               double2 *__restrict__ C, int n, double coef, double diag_add,
+                                               int conj_b) {
     __shared__ double2 sA[kZT][kZT + 1], sB[kZT][kZT + 1];
     const size_t off = (size_t)blockIdx.z * n * n;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads, 4 rows each
@@ -1003,7 +1012,13 @@ __global__ __launch_bounds__(256) void k_zgemm(const double2 *__restrict__ A, co
         for (int r = 0; r < 4; ++r) {
             const int i = ty + 8 * r;
             sA[i][tx] = (row0 + i < n && k0 + tx < n) ? A[off + (size_t)(row0 + i) * n + k0 + tx] : make_double2(0.0, 0.0);
-            sB[i][tx] = (k0 + i < n && col0 + tx < n) ? B[off + (size_t)(k0 + i) * n + col0 + tx] : make_double2(0.0, 0.0);
+            if (conj_b) {  // sB[k][c] = conj(B[col0 + c][k0 + k]), read along rows of B
+                const double2 v = (col0 + i < n && k0 + tx < n) ? B[off + (size_t)(col0 + i) * n + k0 + tx]
+                                                                 : make_double2(0.0, 0.0);
+                sB[tx][i] = make_double2(v.x, -v.y);
+            } else {
+                sB[i][tx] = (k0 + i < n && col0 + tx < n) ? B[off + (size_t)(k0 + i) * n + col0 + tx] : make_double2(0.0, 0.0);
+            }
         }
         __syncthreads();
 #pragma unroll 8
@@ -1026,6 +1041,71 @@ __global__ __launch_bounds__(256) void k_zgemm(const double2 *__restrict__ A, co
     }
 }
 
+// ---------------------------------------------------------------------------
+// Infinite-time symmetric-decay yields (TaskStaticHSSymmetricDecay.cpp:114-158)
+// from the eigenvectors of H: with M = V^dagger (row-major), Pt = M P M^dagger and
+// Rt = M rho0 M^dagger,  Phi = sum_gh Re(Pt_gh conj(Rt_gh)) k^2 / (k^2 + (l_g - l_h)^2).
+// ---------------------------------------------------------------------------
+struct Block4 {
+    double2 m[16];  // 4x4 electron-pair matrix, row-major
+};
+
+// Y = s * M (B (x) 1) in the device basis (index = nu * 4 + c): each row of M is
+// multiplied on the right by the block-diagonal matrix.
+__global__ void k_blockdiag_right(const double2 *__restrict__ M, Block4 B, double scale, double2 *__restrict__ Y,
+                                  int n) {
+    const size_t total = (size_t)n * n;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t row = idx / n;
+        const int col = (int)(idx - row * n), c = col & 3, base = col - c;
+        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const double2 x = M[row * n + base + a], b = B.m[a * 4 + c];
+            acc.x = fma(x.x, b.x, fma(-x.y, b.y, acc.x));
+            acc.y = fma(x.x, b.y, fma(x.y, b.x, acc.y));
+        }
+        Y[idx] = make_double2(scale * acc.x, scale * acc.y);
+    }
+}
+
+// rows[g] = sum_h Re(Pt_gh conj(Rt_gh)) k^2 / (k^2 + (lam_g - lam_h)^2); one workgroup per row.
+__global__ __launch_bounds__(256) void k_lorentz_rows(const double2 *__restrict__ Pt, const double2 *__restrict__ Rt,
+                                                      const double *__restrict__ lam, double k2, int n,
+                                                      double *__restrict__ rows) {
+    __shared__ double red[256];
+    const int g = blockIdx.x, t = threadIdx.x;
+    const double lg = lam[g];
+    double acc = 0.0;
+    for (int h = t; h < n; h += blockDim.x) {
+        const double2 p = Pt[(size_t)g * n + h], r = Rt[(size_t)g * n + h];
+        const double w = lg - lam[h];
+        acc += (p.x * r.x + p.y * r.y) * (k2 / (k2 + w * w));
+    }
+    red[t] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) rows[g] = red[0];
+}
+
+// out[0] = sum of rows[0..n) in a fixed order (one workgroup).
+__global__ __launch_bounds__(256) void k_sum_rows(const double *__restrict__ rows, int n, double *__restrict__ out) {
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    for (int i = t; i < n; i += blockDim.x) acc += rows[i];
+    red[t] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) out[0] = red[0];
+}
+
 // Planes of the dense-path operand: U[b] row-major n x n (double2) ->
 // dst[b*stride + k*Dp + row] = re, dst[b*stride + Dp*Dp + k*Dp + row] = im.
 __global__ void k_to_planes(const double2 *__restrict__ U, float *__restrict__ dst, int n, int Dp, size_t stride) {
@@ -1046,10 +1126,14 @@ __global__ void k_to_planes(const double2 *__restrict__ U, float *__restrict__ d
 hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_thread, int threads, int slots,
                           hipStream_t st) {
     if (group_per_thread == 0) {  // streaming variant for large D
-        hipLaunchKernelGGL(k_lanczos_big, dim3(slots), dim3(1024), 0, st, da);
+        hipLaunchKernelGGL(k_lanczos_big<false>, dim3(slots), dim3(1024), 0, st, da);
         return hipGetLastError();
     }
-    const size_t lds = (size_t)(group_per_thread == 4 ? 2 : 4) * a.dim * sizeof(float) + 2 * 256 * sizeof(double);
+    if (group_per_thread == -1) {  // streaming, v_j mirrored in LDS
+        hipLaunchKernelGGL(k_lanczos_big<true>, dim3(slots), dim3(1024), (size_t)8 * a.dim, st, da);
+        return hipGetLastError();
+    }
+    const size_t lds = (size_t)4 * a.dim * sizeof(float) + 2 * 256 * sizeof(double);
 #define X(G_, T_, B_)                                                                         \
     if (group_per_thread == G_ && threads <= T_) {                                            \
         hipLaunchKernelGGL((k_lanczos<G_, T_, B_>), dim3(slots), dim3(threads), lds, st, da); \
@@ -1071,6 +1155,7 @@ hipError_t set_lanczos_lds_limits() {
     MSG_LANCZOS_VARIANTS(X)
 #undef X
     (void)hipFuncSetAttribute((const void *)k_dense, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)k_lanczos_big<true>, hipFuncAttributeMaxDynamicSharedMemorySize, maxb - 8192);
     (void)hipGetLastError();
     return hipSuccess;
 }
@@ -1084,7 +1169,9 @@ hipError_t launch_dense(const StepArgs &a, const StepArgs *da, int slots, hipStr
 int lanczos_occupancy(int group_per_thread, int threads, size_t lds) {
     int nb = 0;
     if (group_per_thread == 0)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big, 1024, 0) == hipSuccess ? nb : 1;
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<false>, 1024, 0) == hipSuccess ? nb : 1;
+    if (group_per_thread == -1)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<true>, 1024, lds) == hipSuccess ? nb : 1;
 #define X(G_, T_, B_)                                                                                         \
     if (group_per_thread == G_ && threads <= T_)                                                              \
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos<G_, T_, B_>, threads, lds) == hipSuccess \
@@ -1101,10 +1188,27 @@ int dense_occupancy(size_t lds) {
 }
 
 hipError_t launch_zgemm(const void *A, const void *B, void *C, int n, int batch, double coef, double diag_add,
-                        hipStream_t st) {
+                        hipStream_t st, bool conj_b) {
     const int nt = (n + kZT - 1) / kZT;
     hipLaunchKernelGGL(k_zgemm, dim3(nt, nt, batch), dim3(256), 0, st, (const double2 *)A, (const double2 *)B,
-                       (double2 *)C, n, coef, diag_add);
+                       (double2 *)C, n, coef, diag_add, conj_b ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_blockdiag_right(const void *M, const double *B4, double scale, void *Y, int n, hipStream_t st) {
+    Block4 b;
+    for (int i = 0; i < 16; ++i) b.m[i] = make_double2(B4[2 * i], B4[2 * i + 1]);
+    const size_t total = (size_t)n * n;
+    const int blocks = (int)std::min<size_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_blockdiag_right, dim3(blocks), dim3(256), 0, st, (const double2 *)M, b, scale, (double2 *)Y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_lorentz_sum(const void *Pt, const void *Rt, const double *lam, double k, int n, double *rows,
+                              double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_lorentz_rows, dim3(n), dim3(256), 0, st, (const double2 *)Pt, (const double2 *)Rt, lam, k * k,
+                       n, rows);
+    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, st, (const double *)rows, n, out);
     return hipGetLastError();
 }
 

@@ -453,6 +453,7 @@ std::string resolve_system(const msg_system_desc *d, Resolved &r) {
         r.relax = true;
         r.relax_rate = rl.rate;
     }
+    if (d->n_transitions > 0) r.first_rate = d->transitions[0].rate;
     // direct factors of rho0 = rho_e (x) 1/Z (HSStatePreparation.cpp:77-165)
     if (d->n_initial >= 1) {
         if (d->n_initial > 16) return "too many initial states";
@@ -471,6 +472,7 @@ std::string resolve_system(const msg_system_desc *d, Resolved &r) {
         for (int i = 0; i < d->n_initial; ++i) {
             const int s = d->initial_states[i];
             if (s < 0 || s >= d->n_states) return "initial state out of range";
+            r.initial.push_back(s);
             const Mat4 &P = r.P_states[s];
             const double tr = (P[0] + P[5] + P[10] + P[15]).real();
             for (int k = 0; k < 16; ++k) rho[k] += w[i] * P[k] / tr;

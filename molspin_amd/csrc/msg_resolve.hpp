@@ -58,6 +58,8 @@ struct Resolved {
     // direct factors: rho0 = rho_e (x) 1/Z; columns evec_i (x) e_nu / sqrt(Z)
     std::vector<std::array<cd, 4>> evec;
     int64_t zdim = 0;
+    std::vector<int> initial;          // initial State indices as listed
+    double first_rate = -1.0;          // rate of the first Transition object (< 0: none)
     // State objects as described (spins in listed order, amplitudes), for trace sampling
     std::vector<std::vector<int>> state_spins;
     std::vector<std::vector<cd>> state_amps;

@@ -25,6 +25,13 @@ def ctx():
     c.close()
 
 
+def load_fixture(name):
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name + ".json")) as f:
+        return json.load(f)
+
+
 def trace_err(dev, ora):
     return np.abs(dev - ora).max() / np.abs(ora).max()
 
@@ -269,3 +276,95 @@ def test_server_alone_equals_mixed(ctx):
         assert np.abs(a - m).max() <= 1e-12
         yo, _ = ref.run_hsgeneral(p, prm, fields=f, threads=8)
         assert np.abs(m - yo).max() <= YIELD_TOL
+
+
+def test_symmetric_decay_golden_values(ctx):
+    """Device eigenbasis yields (TaskStaticHSSymmetricDecay) reproduce the
+    reference's golden values at k = 1e-4 (tests_TaskStaticHSSymmetricDecay.cpp:64-69)
+    and the oracle to 1e-10; rate 0 takes the default 1e-3 (no transitions)."""
+    from tests.test_oracle import golden_system
+    p = golden_system().pack()
+    sysd = ctx.upload(p)
+    y = sysd.symmetric_decay(rate=1e-4)[0]
+    print("golden", y)
+    assert "%g %g" % (y[0], y[1]) == "0.325757 0.174256"
+    assert np.abs(y - ref.symmetric_decay(p, 1e-4)).max() <= 1e-10
+    y0 = sysd.symmetric_decay(rate=0.0)[0]
+    assert np.abs(y0 - ref.symmetric_decay(p, 1e-3)).max() <= 1e-10
+
+
+def test_symmetric_decay_golden_sweep(ctx):
+    """The 13-row rotated-field sweep (tests_TaskStaticHSSymmetricDecay.cpp:110-174)
+    as one batch of fields."""
+    from tests.test_oracle import GOLDEN_SWEEP, golden_system
+    p = golden_system().pack()
+    sysd = ctx.upload(p)
+    fields = []
+    for step in range(len(GOLDEN_SWEEP)):
+        th = math.radians(15.0 * step)
+        fields.append((5e-5 * math.sin(th), 0.0, 5e-5 * math.cos(th)))
+    y = sysd.symmetric_decay(fields=fields, rate=1e-3)
+    for step, (gs, gt) in enumerate(GOLDEN_SWEEP):
+        assert "%g %g" % (y[step, 0], y[step, 1]) == "%g %g" % (gs, gt), step
+
+
+def test_symmetric_decay_dense_pair_and_completeness(ctx):
+    """A 7-nucleus anisotropic pair (D = 512) against the oracle's eigensolver, and
+    at D = 4096 (config 2 system) the completeness law: the yields of S, T0, T+, T-
+    (projectors summing to the identity) add up to Tr rho0 = 1."""
+    fx = load_fixture("symdecay_d512")
+    s, cfg = workloads.config4_pair(fx["pair"])
+    sysd = ctx.upload(s.pack())
+    y = sysd.symmetric_decay(fields=fx["fields"], rate=fx["rate"])
+    print("D=512 dev", y, "oracle", fx["yields"])
+    assert np.abs(y - np.array(fx["yields"])).max() <= 1e-9
+    s2, cfg2 = workloads.config2(n_orientations=1024)
+    sysd2 = ctx.upload(s2.pack())
+    y2 = sysd2.symmetric_decay(fields=[cfg2["fields"][0]], rate=1e-3)[0]
+    print("D=4096 yields", y2, "sum", y2.sum())
+    assert abs(y2.sum() - 1.0) <= 1e-9
+    assert (y2 > 0).all()
+
+
+def test_config2_against_fixture(ctx):
+    """Config 2 (D = 4096): 16 Fibonacci orientations, 0-50 ns, all 1024 direct
+    columns, against the committed FP64 fixture."""
+    fx = load_fixture("cfg2_subset")
+    s, cfg = workloads.config2(n_orientations=1024)
+    sysd = ctx.upload(s.pack())
+    prm = abi.run_params(total_time=fx["total_time"], time_step=fx["time_step"], method=abi.MSG_METHOD_KRYLOV,
+                         krylov_size=fx["krylov_size"])
+    y, _ = sysd.propagate(prm, fields=[cfg["fields"][i] for i in fx["orientations"]])
+    err = np.abs(y - np.array(fx["yields"])).max()
+    print("cfg2 fixture max |dPhi|", err)
+    assert err <= YIELD_TOL
+
+
+def test_config3_against_fixture(ctx):
+    """Config 3 (D = 65536): 32 spin-coherent trajectories drawn by the backend's
+    sampler (same columns as the oracle's up to a phase per column), ST dephasing
+    with flip seed 2026, 0-200 ns, against the committed FP64 fixture."""
+    fx = load_fixture("cfg3_subset")
+    M = fx["trajectories"]
+    s, cfg = workloads.config3(n_traj=M)
+    sysd = ctx.upload(s.pack())
+    cols = sysd.trace_samples(s.state_index("S"), M, fx["sample_seed"]) / math.sqrt(M)
+    prm = abi.run_params(total_time=fx["total_time"], time_step=fx["time_step"], method=abi.MSG_METHOD_KRYLOV,
+                         sampling=abi.MSG_SAMPLING_GIVEN, seed=fx["flip_seed"], krylov_size=fx["krylov_size"])
+    y, _ = sysd.propagate(prm, columns=cols)
+    err = np.abs(y[0] - np.array(fx["yields"])).max()
+    print("cfg3 fixture max |dPhi|", err, y[0])
+    assert err <= YIELD_TOL
+
+
+def test_config1_trace_against_fixture(ctx):
+    """Config 1 P(t) traces of S, T0, T+, T- at 5001 points against the fixture."""
+    fx = load_fixture("cfg1")
+    s, cfg = workloads.config1()
+    sysd = ctx.upload(s.pack())
+    prm = abi.run_params(calculation=abi.MSG_CALC_TIMEEVOLUTION, observables=abi.MSG_OBS_STATE_POPULATIONS,
+                         total_time=fx["total_time"], time_step=fx["time_step"])
+    _, tr = sysd.propagate(prm, trace=True)
+    want = np.array(fx["trace"]).T
+    for c in range(4):
+        assert trace_err(tr[0, :, c], want[:, c]) <= TRACE_TOL, c
