@@ -324,8 +324,12 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         else { b.G = 0; b.T = 1024; }
         if (const char *env = std::getenv("MSG_LANCZOS_G")) {  // launch-shape override (tuning runs)
             const int g = std::atoi(env);
-            if ((g == 1 || g == 2) && ng / g >= 64 && ng / g <= 1024) { b.G = g; b.T = ng / g; }
+            if ((g == 1 && ng >= 64 && ng <= 1024) || (g == 2 && ng >= 1024 && ng <= 2048)) { b.G = g; b.T = ng / g; }
             if (g == 0 && ng > 2048) { b.G = 0; b.T = 1024; }
+        }
+        if (const char *env = std::getenv("MSG_LANCZOS_THREADS")) {  // streaming workgroup size (tuning runs)
+            const int th = std::atoi(env);
+            if (b.G == 0 && th >= 64 && th <= 1024 && th % 64 == 0) b.T = th;
         }
         if (b.G > 0 && (b.T < 64 || b.T > 1024 || b.G * b.T != ng))
             throw ArgError{MSG_UNSUPPORTED, "no matrix-free launch shape for this dimension"};
@@ -441,6 +445,10 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         const size_t lds = b.G == 0 ? 0 : b.G < 0 ? (size_t)8 * D : (size_t)16 * D + 4096;
         const int occ = std::max(1, lanczos_occupancy(b.G, b.T, lds));
         b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ));
+        if (const char *env = std::getenv("MSG_LANCZOS_SLOTS")) {  // resident-workgroup override (tuning runs)
+            const int sl = std::atoi(env);
+            if (sl >= 1) b.slots = std::max(1, std::min(b.slots, sl));
+        }
         const int vecs = b.G <= 0 ? a.m + 2 : a.m;  // streaming variants: + residual and state
         a.basis = b.basis.alloc<float>((size_t)b.slots * vecs * 2 * D, stream);
         a.norm0 = b.norm0.alloc<double>(b.n_units, stream);

@@ -325,12 +325,15 @@ template <int G, int TMAX, bool DBUF>
 __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict__ pa) {
     const StepArgs &a = *pa;
     extern __shared__ float4 smem[];
-    const int T = blockDim.x, t = threadIdx.x, lane = t & 63;
-    const int ng = G * T;
-    float4 *LA[2] = {smem, DBUF ? smem + 2 * ng : smem};
-    float4 *LB[2] = {smem + ng, DBUF ? smem + 3 * ng : smem + ng};
+    constexpr int T = TMAX;  // launched with exactly TMAX threads
+    const int t = threadIdx.x, lane = t & 63;
+    constexpr int ng = G * T;
+    // LDS planes of buffer i (i = 0, 1; a single buffer without DBUF)
+    auto LA = [&](int i) -> float4 * { return smem + ((DBUF && i) ? 2 * ng : 0); };
+    auto LB = [&](int i) -> float4 * { return smem + ng + ((DBUF && i) ? 2 * ng : 0); };
     double *scr = reinterpret_cast<double *>(smem + (DBUF ? 4 : 2) * ng);  // 2 x 16 x 16 doubles
     __shared__ double ys[kMaxObs], yc[kMaxObs];                // Kahan yield sums (thread 0)
+    __shared__ float2 s_coef[kMaxKrylov];                        // exp(-i dt T) e1, scaled
     __shared__ int s_unit;
     const int D = a.dim;
     const int D4 = D >> 2;
@@ -413,7 +416,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict_
 #pragma unroll
                         for (int c = 0; c < 4; ++c) v0.c[c] = make_float2(psi[g].c[c].x * inv, psi[g].c[c].y * inv);
                         const int nu = g * T + t;
-                        store_group(LA[0], LB[0], nu, v0);
+                        store_group(LA(0), LB(0), nu, v0);
                         store_group(bA, bA + D4, nu, v0);
                     }
                 }
@@ -423,8 +426,8 @@ __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict_
                 int cur = 0;
                 for (int j = 0; j < meff0; ++j) {
                     __syncthreads();  // v_j complete in buffer `cur`
-                    const float4 *cA = LA[cur], *cB = LB[cur];
-                    float4 *pA = LA[cur ^ 1], *pB = LB[cur ^ 1];
+                    const float4 *cA = LA(cur), *cB = LB(cur);
+                    float4 *pA = LA(cur ^ 1), *pB = LB(cur ^ 1);
                     const float4 *gA = reinterpret_cast<const float4 *>(bas + (size_t)(j > 0 ? j - 1 : 0) * 2 * D);
                     const float fb = (float)bprev;
                     // w = H v_j - beta_{j-1} v_{j-1}; one reduction gives alpha and |w|^2
@@ -494,17 +497,21 @@ __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict_
                     bprev = beta;
                     if (DBUF) cur ^= 1;
                 }
-                // every wavefront evaluates the small exponential for itself
-                double cre, cim;
-                cheb_expm_lanes(meff, dt, my_al, my_be, cre, cim, a.status);
-                // psi = beta * V c   (beta = norm0 when renormalising the unitary part)
-                const double sc = a.renorm ? n0 : beta0;
+                // wavefront 0 evaluates the small exponential; psi = beta * V c
+                // (beta = norm0 when renormalising the unitary part)
+                if (t < 64) {
+                    double cre, cim;
+                    cheb_expm_lanes(meff, dt, my_al, my_be, cre, cim, a.status);
+                    const double sc = a.renorm ? n0 : beta0;
+                    if (t < meff) s_coef[t] = make_float2((float)(sc * cre), (float)(sc * cim));
+                }
+                __syncthreads();
 #pragma unroll
                 for (int g = 0; g < G; ++g)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) psi[g].c[c] = make_float2(0.f, 0.f);
                 for (int j = 0; j < meff; ++j) {
-                    const float2 cj = make_float2((float)(sc * readlane_f64(cre, j)), (float)(sc * readlane_f64(cim, j)));
+                    const float2 cj = s_coef[j];
                     const float4 *bA = reinterpret_cast<const float4 *>(bas + (size_t)j * 2 * D);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
@@ -554,6 +561,7 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
     extern __shared__ float4 lvs[];
     __shared__ double scr[2 * 17 * 16];  // block_sum of up to 17 values, double-buffered
     __shared__ double ys[kMaxObs], yc[kMaxObs];
+    __shared__ float2 s_coef[kMaxKrylov];
     __shared__ int s_unit;
     const int T = blockDim.x, t = threadIdx.x, lane = t & 63;
     const int D = a.dim, D4 = D >> 2, ng = D >> 2;
@@ -697,15 +705,20 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
                         }
                         bprev = beta;
                     }
-                    double cre, cim;
-                    cheb_expm_lanes(meff, dt, my_al, my_be, cre, cim, a.status);
-                    const double sc = a.renorm ? n0 : beta0;
+                    if (t < 64) {  // wavefront 0 evaluates the small exponential
+                        double cre, cim;
+                        cheb_expm_lanes(meff, dt, my_al, my_be, cre, cim, a.status);
+                        const double sc = a.renorm ? n0 : beta0;
+                        if (t < meff) s_coef[t] = make_float2((float)(sc * cre), (float)(sc * cim));
+                    }
+                    __syncthreads();
                     for (int nu = t; nu < ng; nu += T) {
                         C4 x;
 #pragma unroll
                         for (int c = 0; c < 4; ++c) x.c[c] = make_float2(0.f, 0.f);
+#pragma unroll 2
                         for (int j = 0; j < meff; ++j) {
-                            const float2 cj = make_float2((float)(sc * readlane_f64(cre, j)), (float)(sc * readlane_f64(cim, j)));
+                            const float2 cj = s_coef[j];
                             const C4 vj = load_group(BA(j), BA(j) + D4, nu);
 #pragma unroll
                             for (int c = 0; c < 4; ++c) x.c[c] = cfma(cj, vj.c[c], x.c[c]);
@@ -1121,12 +1134,14 @@ __global__ void k_to_planes(const double2 *__restrict__ U, float *__restrict__ d
 // ---------------------------------------------------------------------------
 // Launchers (called from the host part of the library)
 // ---------------------------------------------------------------------------
-#define MSG_LANCZOS_VARIANTS(X) X(1, 256, true) X(1, 512, true) X(1, 1024, true) X(2, 512, true) X(2, 1024, true)
+#define MSG_LANCZOS_VARIANTS(X) \
+    X(1, 64, true) X(1, 128, true) X(1, 256, true) X(1, 512, true) X(1, 1024, true) X(2, 512, true) X(2, 1024, true)
 
 hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_thread, int threads, int slots,
                           hipStream_t st) {
     if (group_per_thread == 0) {  // streaming variant for large D
-        hipLaunchKernelGGL(k_lanczos_big<false>, dim3(slots), dim3(1024), 0, st, da);
+        if (threads < 64 || threads > 1024 || (threads & 63)) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_lanczos_big<false>, dim3(slots), dim3(threads), 0, st, da);
         return hipGetLastError();
     }
     if (group_per_thread == -1) {  // streaming, v_j mirrored in LDS
@@ -1135,7 +1150,7 @@ hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_t
     }
     const size_t lds = (size_t)4 * a.dim * sizeof(float) + 2 * 256 * sizeof(double);
 #define X(G_, T_, B_)                                                                         \
-    if (group_per_thread == G_ && threads <= T_) {                                            \
+    if (group_per_thread == G_ && threads == T_) {                                            \
         hipLaunchKernelGGL((k_lanczos<G_, T_, B_>), dim3(slots), dim3(threads), lds, st, da); \
         return hipGetLastError();                                                              \
     }
@@ -1169,11 +1184,11 @@ hipError_t launch_dense(const StepArgs &a, const StepArgs *da, int slots, hipStr
 int lanczos_occupancy(int group_per_thread, int threads, size_t lds) {
     int nb = 0;
     if (group_per_thread == 0)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<false>, 1024, 0) == hipSuccess ? nb : 1;
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<false>, threads, 0) == hipSuccess ? nb : 1;
     if (group_per_thread == -1)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<true>, 1024, lds) == hipSuccess ? nb : 1;
 #define X(G_, T_, B_)                                                                                         \
-    if (group_per_thread == G_ && threads <= T_)                                                              \
+    if (group_per_thread == G_ && threads == T_)                                                              \
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos<G_, T_, B_>, threads, lds) == hipSuccess \
                    ? nb : 1;
     MSG_LANCZOS_VARIANTS(X)
