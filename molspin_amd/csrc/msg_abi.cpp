@@ -327,10 +327,6 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
             if ((g == 1 && ng >= 64 && ng <= 1024) || (g == 2 && ng >= 1024 && ng <= 2048)) { b.G = g; b.T = ng / g; }
             if (g == 0 && ng > 2048) { b.G = 0; b.T = 1024; }
         }
-        if (const char *env = std::getenv("MSG_LANCZOS_THREADS")) {  // streaming workgroup size (tuning runs)
-            const int th = std::atoi(env);
-            if (b.G == 0 && th >= 64 && th <= 1024 && th % 64 == 0) b.T = th;
-        }
         if (b.G > 0 && (b.T < 64 || b.T > 1024 || b.G * b.T != ng))
             throw ArgError{MSG_UNSUPPORTED, "no matrix-free launch shape for this dimension"};
         a.nnuc = (int)r.nuc.size();

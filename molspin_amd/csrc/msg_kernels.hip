@@ -29,6 +29,15 @@ struct C4 {
     float2 c[4];
 };
 
+template <int N>
+struct Log2 {
+    static constexpr int value = 1 + Log2<N / 2>::value;
+};
+template <>
+struct Log2<1> {
+    static constexpr int value = 0;
+};
+
 __device__ __forceinline__ float2 cfma(float2 a, float2 b, float2 acc) {
     acc.x = fmaf(a.x, b.x, fmaf(-a.y, b.y, acc.x));
     acc.y = fmaf(a.x, b.y, fmaf(a.y, b.x, acc.y));
@@ -83,6 +92,9 @@ __device__ __forceinline__ void load_block(const float *p, bool herm, float2 m[4
 
 // H x for the group nu whose value is x; neighbours come from the LDS planes.
 // H = w1.S1 + w2.S2 + shift + [E] + sum_n (diagonal s_n(...) + flip-flop blocks).
+// NN >= 0: the number of nuclei is a compile-time constant (the loop unrolls);
+// NN < 0: a.nnuc at run time.
+template <int NN = -1>
 __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict__ co, int nu, const C4 &x,
                                       const float4 *LA, const float4 *LB) {
     float f1x = co[0], f1y = co[1], f1z = co[2];
@@ -91,7 +103,10 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
     C4 y;
 #pragma unroll
     for (int c = 0; c < 4; ++c) y.c[c] = make_float2(0.f, 0.f);
-    for (int n = 0; n < a.nnuc; ++n) {
+    const int nn = NN >= 0 ? NN : a.nnuc;
+#pragma unroll
+    for (int n = 0; n < (NN >= 0 ? NN : kMaxNuclei); ++n) {
+        if (NN < 0 && n >= nn) break;
         const float *cn = co + kCoefHead + n * kNucStride;
         const int fl = a.nuc_flags[n];
         const bool down = (nu >> n) & 1;
@@ -437,7 +452,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict_
                     for (int g = 0; g < G; ++g) {
                         const int nu = g * T + t;
                         const C4 vj = load_group(cA, cB, nu);
-                        w[g] = apply_h(a, co, nu, vj, cA, cB);
+                        w[g] = apply_h<Log2<ng>::value>(a, co, nu, vj, cA, cB);
                         if (j > 0) {
                             const C4 vp = DBUF ? load_group(pA, pB, nu) : load_group(gA, gA + D4, nu);
 #pragma unroll
@@ -555,6 +570,8 @@ __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict_
 // LV: v_j is also mirrored in LDS (8 D bytes, D <= 16384) so the matvec's
 // neighbour reads stay on chip; the Krylov basis, residual and state stay in
 // the per-slot global scratch either way.
+// (Unrolling the group and nuclear loops at compile time was measured slower
+// here: these variants are bound by memory latency, not instruction issue.)
 template <bool LV>
 __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict__ pa) {
     const StepArgs &a = *pa;
@@ -563,7 +580,8 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
     __shared__ double ys[kMaxObs], yc[kMaxObs];
     __shared__ float2 s_coef[kMaxKrylov];
     __shared__ int s_unit;
-    const int T = blockDim.x, t = threadIdx.x, lane = t & 63;
+    constexpr int T = 1024;  // launched with exactly 1024 threads
+    const int t = threadIdx.x, lane = t & 63;
     const int D = a.dim, D4 = D >> 2, ng = D >> 2;
     // per-slot scratch: basis (m vectors), residual W, state P
     float *bas = a.basis + (size_t)blockIdx.x * (a.m + 2) * 2 * D;
@@ -1139,13 +1157,12 @@ __global__ void k_to_planes(const double2 *__restrict__ U, float *__restrict__ d
 
 hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_thread, int threads, int slots,
                           hipStream_t st) {
-    if (group_per_thread == 0) {  // streaming variant for large D
-        if (threads < 64 || threads > 1024 || (threads & 63)) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_lanczos_big<false>, dim3(slots), dim3(threads), 0, st, da);
-        return hipGetLastError();
-    }
-    if (group_per_thread == -1) {  // streaming, v_j mirrored in LDS
-        hipLaunchKernelGGL(k_lanczos_big<true>, dim3(slots), dim3(1024), (size_t)8 * a.dim, st, da);
+    if (group_per_thread <= 0) {  // streaming variants for large D (v_j mirrored in LDS when G = -1)
+        if (threads != 1024) return hipErrorInvalidValue;
+        if (group_per_thread == -1)
+            hipLaunchKernelGGL(k_lanczos_big<true>, dim3(slots), dim3(1024), (size_t)8 * a.dim, st, da);
+        else
+            hipLaunchKernelGGL(k_lanczos_big<false>, dim3(slots), dim3(1024), 0, st, da);
         return hipGetLastError();
     }
     const size_t lds = (size_t)4 * a.dim * sizeof(float) + 2 * 256 * sizeof(double);
@@ -1170,7 +1187,8 @@ hipError_t set_lanczos_lds_limits() {
     MSG_LANCZOS_VARIANTS(X)
 #undef X
     (void)hipFuncSetAttribute((const void *)k_dense, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-    (void)hipFuncSetAttribute((const void *)k_lanczos_big<true>, hipFuncAttributeMaxDynamicSharedMemorySize, maxb - 8192);
+    (void)hipFuncSetAttribute((const void *)k_lanczos_big<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              maxb - 8192);
     (void)hipGetLastError();
     return hipSuccess;
 }
@@ -1184,7 +1202,7 @@ hipError_t launch_dense(const StepArgs &a, const StepArgs *da, int slots, hipStr
 int lanczos_occupancy(int group_per_thread, int threads, size_t lds) {
     int nb = 0;
     if (group_per_thread == 0)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<false>, threads, 0) == hipSuccess ? nb : 1;
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<false>, 1024, 0) == hipSuccess ? nb : 1;
     if (group_per_thread == -1)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<true>, 1024, lds) == hipSuccess ? nb : 1;
 #define X(G_, T_, B_)                                                                                         \
