@@ -321,11 +321,12 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         if (ng <= 512) { b.T = ng; b.G = 1; }
         else if (ng <= 2048) { b.G = 2; b.T = ng / 2; }
         else if (ng <= 4096) { b.G = -1; b.T = 1024; }
-        else { b.G = 0; b.T = 1024; }
+        else { b.G = -2; b.T = 1024; }  // tiled streaming
         if (const char *env = std::getenv("MSG_LANCZOS_G")) {  // launch-shape override (tuning runs)
             const int g = std::atoi(env);
             if ((g == 1 && ng >= 64 && ng <= 1024) || (g == 2 && ng >= 1024 && ng <= 2048)) { b.G = g; b.T = ng / g; }
             if (g == 0 && ng > 2048) { b.G = 0; b.T = 1024; }
+            if (g == -2 && ng > kTileGroups) { b.G = -2; b.T = 1024; }
         }
         if (b.G > 0 && (b.T < 64 || b.T > 1024 || b.G * b.T != ng))
             throw ArgError{MSG_UNSUPPORTED, "no matrix-free launch shape for this dimension"};
@@ -438,7 +439,8 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     }
     // launch geometry
     if (b.path == Path::Lanczos) {
-        const size_t lds = b.G == 0 ? 0 : b.G < 0 ? (size_t)8 * D : (size_t)16 * D + 4096;
+        const size_t lds = b.G == 0 ? 0 : b.G == -1 ? (size_t)8 * D : b.G == -2 ? (size_t)32 * kTileGroups
+                                                                              : (size_t)16 * D + 4096;
         const int occ = std::max(1, lanczos_occupancy(b.G, b.T, lds));
         b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ));
         if (const char *env = std::getenv("MSG_LANCZOS_SLOTS")) {  // resident-workgroup override (tuning runs)

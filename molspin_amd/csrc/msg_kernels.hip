@@ -92,11 +92,19 @@ __device__ __forceinline__ void load_block(const float *p, bool herm, float2 m[4
 
 // H x for the group nu whose value is x; neighbours come from the LDS planes.
 // H = w1.S1 + w2.S2 + shift + [E] + sum_n (diagonal s_n(...) + flip-flop blocks).
+// Groups per LDS tile of the tiled streaming kernel: index bits 0-11 of nu.
+constexpr int kTileBits = 12;
+constexpr int kTileG = 1 << kTileBits;
+static_assert(kTileG == kTileGroups, "tile size shared with the host");
+
 // NN >= 0: the number of nuclei is a compile-time constant (the loop unrolls);
-// NN < 0: a.nnuc at run time.
-template <int NN = -1>
+// NN < 0: a.nnuc at run time.  TILED: neighbours across index bits below
+// kTileBits come from the LDS tile (LA/LB indexed by nu mod kTileG), the others
+// from the vector in global memory (GA/GB).
+template <int NN = -1, bool TILED = false>
 __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict__ co, int nu, const C4 &x,
-                                      const float4 *LA, const float4 *LB) {
+                                      const float4 *LA, const float4 *LB, const float4 *GA = nullptr,
+                                      const float4 *GB = nullptr) {
     float f1x = co[0], f1y = co[1], f1z = co[2];
     float f2x = co[3], f2y = co[4], f2z = co[5];
     float rr = co[6];
@@ -111,7 +119,9 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
         const int fl = a.nuc_flags[n];
         const bool down = (nu >> n) & 1;
         const float s = down ? -1.f : 1.f;
-        const C4 nb = load_group(LA, LB, nu ^ (1 << n));
+        const C4 nb = !TILED ? load_group(LA, LB, nu ^ (1 << n))
+                             : (n < kTileBits ? load_group(LA, LB, (nu ^ (1 << n)) & (kTileG - 1))
+                                              : load_group(GA, GB, nu ^ (1 << n)));
         float2 m[4];
         if (fl & NUC_E1) {
             f1x = fmaf(s, cn[0], f1x); f1y = fmaf(s, cn[1], f1y); f1z = fmaf(s, cn[2], f1z);
@@ -570,9 +580,12 @@ __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict_
 // LV: v_j is also mirrored in LDS (8 D bytes, D <= 16384) so the matvec's
 // neighbour reads stay on chip; the Krylov basis, residual and state stay in
 // the per-slot global scratch either way.
+// TILED (D >= 32768, LV false): the matvec stages v_j through LDS in tiles of
+// kTileG groups, so only the nuclei on index bits >= kTileBits gather from
+// global memory.
 // (Unrolling the group and nuclear loops at compile time was measured slower
 // here: these variants are bound by memory latency, not instruction issue.)
-template <bool LV>
+template <bool LV, bool TILED = false>
 __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict__ pa) {
     const StepArgs &a = *pa;
     extern __shared__ float4 lvs[];
@@ -666,9 +679,8 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
                         const float4 *pA = BA(j > 0 ? j - 1 : 0), *pB = pA + D4;
                         const float fb = (float)bprev;
                         double red[2] = {0.0, 0.0};
-                        for (int nu = t; nu < ng; nu += T) {
-                            const C4 vj = load_group(vA, vB, nu);
-                            C4 w = apply_h(a, co, nu, vj, vA, vB);
+                        auto pass_a = [&](int nu, const C4 &vj, const C4 &w0) {
+                            C4 w = w0;
                             if (j > 0) {
                                 const C4 vp = load_group(pA, pB, nu);
 #pragma unroll
@@ -683,6 +695,27 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
                                 red[1] += (double)w.c[c].x * w.c[c].x + (double)w.c[c].y * w.c[c].y;
                             }
                             store_group(WA, WB, nu, w);
+                        };
+                        if (TILED) {
+                            float4 *tA = lvs, *tB = lvs + kTileG;
+                            for (int tb = 0; tb < ng; tb += kTileG) {
+                                if (tb > 0) __syncthreads();  // the previous tile is no longer read
+                                for (int k = t; k < kTileG; k += T) {
+                                    tA[k] = vA[tb + k];
+                                    tB[k] = vB[tb + k];
+                                }
+                                __syncthreads();
+                                for (int k = t; k < kTileG; k += T) {
+                                    const int nu = tb + k;
+                                    const C4 vj = load_group(tA, tB, k);
+                                    pass_a(nu, vj, apply_h<-1, true>(a, co, nu, vj, tA, tB, vA, vB));
+                                }
+                            }
+                        } else {
+                            for (int nu = t; nu < ng; nu += T) {
+                                const C4 vj = load_group(vA, vB, nu);
+                                pass_a(nu, vj, apply_h(a, co, nu, vj, vA, vB));
+                            }
                         }
                         block_sum<2>(red, SCR());
                         const double alpha = red[0];
@@ -1161,6 +1194,8 @@ hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_t
         if (threads != 1024) return hipErrorInvalidValue;
         if (group_per_thread == -1)
             hipLaunchKernelGGL(k_lanczos_big<true>, dim3(slots), dim3(1024), (size_t)8 * a.dim, st, da);
+        else if (group_per_thread == -2)
+            hipLaunchKernelGGL((k_lanczos_big<false, true>), dim3(slots), dim3(1024), (size_t)32 * kTileG, st, da);
         else
             hipLaunchKernelGGL(k_lanczos_big<false>, dim3(slots), dim3(1024), 0, st, da);
         return hipGetLastError();
@@ -1189,6 +1224,8 @@ hipError_t set_lanczos_lds_limits() {
     (void)hipFuncSetAttribute((const void *)k_dense, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     (void)hipFuncSetAttribute((const void *)k_lanczos_big<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               maxb - 8192);
+    (void)hipFuncSetAttribute((const void *)k_lanczos_big<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              maxb - 8192);
     (void)hipGetLastError();
     return hipSuccess;
 }
@@ -1205,6 +1242,9 @@ int lanczos_occupancy(int group_per_thread, int threads, size_t lds) {
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<false>, 1024, 0) == hipSuccess ? nb : 1;
     if (group_per_thread == -1)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<true>, 1024, lds) == hipSuccess ? nb : 1;
+    if (group_per_thread == -2)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<false, true>, 1024, lds) == hipSuccess
+                   ? nb : 1;
 #define X(G_, T_, B_)                                                                                         \
     if (group_per_thread == G_ && threads == T_)                                                              \
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos<G_, T_, B_>, threads, lds) == hipSuccess \
