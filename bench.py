@@ -138,6 +138,7 @@ def orientation_batch(ctx, args):
     prm = abi.run_params(total_time=intervals * dt, time_step=dt, method=abi.MSG_METHOD_KRYLOV, krylov_size=m)
     ncols = sysd.direct_columns()
     state_steps = B * ncols * intervals
+    shape = sysd.autotune(prm, fields=cfg["fields"][:B], sample_steps=5)
 
     def step(i):
         first = (i * B) % 1024
@@ -155,7 +156,9 @@ def orientation_batch(ctx, args):
                                     "%d orientations x %d columns x %d steps of 2 ns per step, Lanczos m=%d"
                                     % (B, ncols, intervals, m),
                         "orientations_per_step": B, "columns": ncols, "intervals": intervals,
-                        "parallelism": "single GPU"})
+                        "parallelism": "single GPU",
+                        "launch_shape": {"groups_per_thread": shape.groups_per_thread, "threads": shape.threads,
+                                         "slots": shape.slots, "chosen_by": "msg_autotune"}})
 
 
 def trajectory_batch(ctx, args):
@@ -170,6 +173,7 @@ def trajectory_batch(ctx, args):
     prm = abi.run_params(total_time=intervals * dt, time_step=dt, method=abi.MSG_METHOD_KRYLOV, krylov_size=m,
                          sampling=abi.MSG_SAMPLING_GIVEN, seed=2026)
     state_steps = M * intervals
+    shape = sysd.autotune(prm, columns=cols[:, :min(M, 256)], sample_steps=2)
 
     def step(i):
         sysd.propagate(prm, columns=cols)
@@ -186,7 +190,9 @@ def trajectory_batch(ctx, args):
                 config={"workload": "stochastic trajectories: D=65536 (8+6 spin-1/2 nuclei, isotropic HF), "
                                     "%d trajectories x %d steps of 1 ns, ST dephasing 1e-4/ns, Lanczos m=%d"
                                     % (M, intervals, m),
-                        "trajectories": M, "intervals": intervals, "parallelism": "single GPU"})
+                        "trajectories": M, "intervals": intervals, "parallelism": "single GPU",
+                        "launch_shape": {"groups_per_thread": shape.groups_per_thread, "threads": shape.threads,
+                                         "slots": shape.slots, "chosen_by": "msg_autotune"}})
 
 
 def dense_sweep(ctx, args):

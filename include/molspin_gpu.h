@@ -293,6 +293,30 @@ msg_status msg_propagate(msg_ctx *ctx, const msg_system *sys, const msg_run_para
 msg_status msg_symmetric_decay(msg_ctx *ctx, const msg_system *sys, int32_t n_items, const double *fields,
                                double rate, double *yields);
 
+/* Launch shape chosen by msg_autotune. */
+typedef struct msg_launch_shape {
+    int32_t path;                 /* 1 matrix-free Lanczos, 2 dense MFMA */
+    int32_t groups_per_thread;    /* Lanczos: electron-pair groups per thread (1, 2), -1 streaming with the
+                                     Krylov vector in LDS, -2 streaming with LDS tiles, 0 plain streaming */
+    int32_t threads;              /* workgroup size */
+    int32_t slots;                /* resident workgroups (persistent launch) */
+    double kernel_ms;             /* device time of the chosen shape on the sample */
+} msg_launch_shape;
+
+/*
+ * Autotuner (no counterpart in the reference: the reference's CPU loop has no
+ * launch shapes).  Times the candidate launch shapes of this system's path --
+ * workgroup shape and Krylov-vector placement (LDS, LDS tiles or streamed), and
+ * the number of resident workgroups -- on a sample of `params` (n_items items
+ * of `fields`/`columns` as for msg_propagate, `sample_steps` time steps) and
+ * keeps the fastest for this context and Hilbert dimension: later
+ * msg_propagate, msg_step and server launches on `ctx` use it.  Results do not
+ * depend on the shape beyond FP32 rounding.  `best` may be NULL.
+ */
+msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_params *params, int32_t n_items,
+                        const double *fields, const double *columns, int32_t n_columns, int32_t sample_steps,
+                        msg_launch_shape *best);
+
 /*
  * Per-step compatibility entry with the meaning of HSPropagator::StepStochastic
  * (HSPropagator.cpp:131-140): advances `factors` (D x n_columns complex, MolSpin

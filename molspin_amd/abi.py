@@ -98,6 +98,11 @@ class Plan(C.Structure):
                 ("propagation", C.c_int32)]
 
 
+class LaunchShape(C.Structure):
+    _fields_ = [("path", C.c_int32), ("groups_per_thread", C.c_int32), ("threads", C.c_int32),
+                ("slots", C.c_int32), ("kernel_ms", C.c_double)]
+
+
 def run_params(calculation=MSG_CALC_YIELDS, observables=MSG_OBS_TRANSITION_YIELDS,
                method=MSG_METHOD_KRYLOV, krylov_size=16, sampling=MSG_SAMPLING_DIRECT,
                yield_corrections=False, total_time=10000.0, time_step=1.0, seed=1, task=MSG_TASK_HSGENERAL):

@@ -19,7 +19,7 @@ EXPORTS = [
     "msg_system_dimension", "msg_query_footprint", "msg_direct_columns", "msg_observable_count",
     "msg_time_grid", "msg_propagate", "msg_step", "msg_server_create", "msg_server_destroy",
     "msg_server_submit", "msg_server_pump", "msg_server_collect", "msg_last_kernel_time",
-    "msg_trace_samples", "msg_symmetric_decay", "msg_resolve_plan",
+    "msg_trace_samples", "msg_symmetric_decay", "msg_resolve_plan", "msg_autotune",
 ]
 
 
@@ -59,6 +59,8 @@ def lib():
         L.msg_trace_samples.argtypes = [vp, C.c_int32, C.c_int32, C.c_uint64, C.c_int32, vp]
         L.msg_symmetric_decay.argtypes = [vp, vp, C.c_int32, vp, C.c_double, vp]
         L.msg_resolve_plan.argtypes = [C.c_char_p, P(abi.Plan), C.c_char_p, C.c_int32]
+        L.msg_autotune.argtypes = [vp, vp, P(abi.RunParams), C.c_int32, vp, vp, C.c_int32, C.c_int32,
+                                   P(abi.LaunchShape)]
         for name in EXPORTS:
             getattr(L, name).restype = C.c_int if name != "msg_last_error" else C.c_char_p
         _LIB = L
@@ -70,8 +72,10 @@ def _ptr(a):
 
 
 def _columns(columns):
+    """Column-major complex128 D x M as a flat float64 array (no copy when the
+    input already has that layout)."""
     cc = np.asfortranarray(columns, dtype=np.complex128)
-    return cc.reshape(-1, order="F").view(np.float64).copy(), cc.shape[1]
+    return cc.reshape(-1, order="F").view(np.float64), cc.shape[1]
 
 
 def resolve_plan(task_properties):
@@ -180,6 +184,17 @@ class System:
                                            _ptr(y), _ptr(tr)))
         return y.reshape(n_items, nobs), (tr.reshape(n_items, ns, nobs) if trace else None)
 
+    def autotune(self, params, fields=None, columns=None, sample_steps=4):
+        """msg_autotune: times the candidate launch shapes on this sample and keeps
+        the fastest for the context; returns the chosen abi.LaunchShape."""
+        n_items = 1 if fields is None else len(fields)
+        f = None if fields is None else np.ascontiguousarray(fields, dtype=np.float64).reshape(-1)
+        cols, ncol = (None, 0) if columns is None else _columns(columns)
+        shape = abi.LaunchShape()
+        self.ctx.check(lib().msg_autotune(self.ctx.h, self.h, C.byref(params), n_items, _ptr(f), _ptr(cols), ncol,
+                                          int(sample_steps), C.byref(shape)))
+        return shape
+
     def symmetric_decay(self, fields=None, rate=0.0):
         """Infinite-time yields of every State under symmetric decay
         (TaskStaticHSSymmetricDecay); returns [n_items, n_states]."""
@@ -192,6 +207,7 @@ class System:
     def step(self, params, factors, dt, step_index=0, field=None):
         """HSPropagator::StepStochastic equivalent; returns the new factors."""
         cols, ncol = _columns(factors)
+        cols = cols.copy()  # msg_step writes the factors in place; keep the caller's array
         f = None if field is None else np.ascontiguousarray(field, dtype=np.float64)
         self.ctx.check(lib().msg_step(self.ctx.h, self.h, C.byref(params), _ptr(f), float(dt), int(step_index),
                                       _ptr(cols), ncol))

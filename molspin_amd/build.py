@@ -15,7 +15,8 @@ ARCH = os.environ.get("MSG_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-parameter", "-Wno-unused-variable"]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+# host translation units go through hipcc too (HIP runtime API); they carry no kernels
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Wall", "-Wextra", "-Wno-unused-parameter"]
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 LIBS = ["-L" + os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas", "-Wl,-rpath," + os.path.join(ROCM, "lib")]

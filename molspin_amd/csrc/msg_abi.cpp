@@ -19,6 +19,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/molspin_gpu.h"
@@ -55,6 +56,12 @@ struct msg_ctx {
     double kernel_ms = 0.0;
     int launches = 0;
     rocblas_handle blas = nullptr;  // created on first use (eigensolver), bound to `stream`
+    // Lanczos launch shapes chosen by msg_autotune, per Hilbert dimension
+    struct Shape {
+        int G = 0, T = 0, slots = 0;
+    };
+    std::map<int64_t, Shape> tuned;
+    const Shape *force = nullptr;  // candidate under trial inside msg_autotune
 };
 
 struct msg_system {
@@ -215,6 +222,12 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     b.path = choose_path(r, p);
     b.n_items = n_items;
     const int D = (int)r.D;
+    // launch shape from the autotuner (a candidate under trial, or the tuned one)
+    const msg_ctx::Shape *ts = ctx->force;
+    if (!ts) {
+        auto it = ctx->tuned.find(r.D);
+        if (it != ctx->tuned.end()) ts = &it->second;
+    }
     // columns
     if (p->sampling == MSG_SAMPLING_GIVEN) {
         if (!columns || n_columns < 1) throw ArgError{MSG_INVALID_ARG, "given sampling needs columns"};
@@ -286,14 +299,28 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     if (p->sampling == MSG_SAMPLING_GIVEN) {
         a.init_mode = 1;
         std::vector<float> g((size_t)n_columns * 2 * D);
-        for (int j = 0; j < n_columns; ++j)
-            for (int i = 0; i < D; ++i) {
-                const int64_t dev = r.perm[i];
-                const int64_t nu = dev >> 2, c = dev & 3;
-                const size_t off = (size_t)j * 2 * D + (c < 2 ? 4 * nu + 2 * c : D + 4 * nu + 2 * (c - 2));
-                g[off] = (float)columns[2 * ((size_t)j * D + i)];
-                g[off + 1] = (float)columns[2 * ((size_t)j * D + i) + 1];
-            }
+        // MolSpin order (complex128) -> device planes (complex64); columns split over host threads
+        auto convert = [&](int j0, int j1) {
+            for (int j = j0; j < j1; ++j)
+                for (int i = 0; i < D; ++i) {
+                    const int64_t dev = r.perm[i];
+                    const int64_t nu = dev >> 2, c = dev & 3;
+                    const size_t off = (size_t)j * 2 * D + (c < 2 ? 4 * nu + 2 * c : D + 4 * nu + 2 * (c - 2));
+                    g[off] = (float)columns[2 * ((size_t)j * D + i)];
+                    g[off + 1] = (float)columns[2 * ((size_t)j * D + i) + 1];
+                }
+        };
+        const int64_t work = (int64_t)n_columns * D;
+        const int nth = (int)std::min<int64_t>({16, std::max(1u, std::thread::hardware_concurrency()),
+                                                 std::max<int64_t>(1, work >> 20), (int64_t)n_columns});
+        if (nth <= 1) {
+            convert(0, n_columns);
+        } else {
+            std::vector<std::thread> pool;
+            for (int k = 0; k < nth; ++k)
+                pool.emplace_back(convert, (int)((int64_t)n_columns * k / nth), (int)((int64_t)n_columns * (k + 1) / nth));
+            for (auto &th : pool) th.join();
+        }
         float *dg = b.given.alloc<float>(g.size(), stream);
         HIPCHECK(hipMemcpyAsync(dg, g.data(), g.size() * sizeof(float), hipMemcpyHostToDevice, stream));
         a.given = dg;
@@ -322,6 +349,10 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         else if (ng <= 2048) { b.G = 2; b.T = ng / 2; }
         else if (ng <= 4096) { b.G = -1; b.T = 1024; }
         else { b.G = -2; b.T = 1024; }  // tiled streaming
+        if (ts) {
+            b.G = ts->G;
+            b.T = ts->T;
+        }
         if (const char *env = std::getenv("MSG_LANCZOS_G")) {  // launch-shape override (tuning runs)
             const int g = std::atoi(env);
             if ((g == 1 && ng >= 64 && ng <= 1024) || (g == 2 && ng >= 1024 && ng <= 2048)) { b.G = g; b.T = ng / g; }
@@ -443,6 +474,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                                                                               : (size_t)16 * D + 4096;
         const int occ = std::max(1, lanczos_occupancy(b.G, b.T, lds));
         b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ));
+        if (ts && ts->slots > 0) b.slots = std::max(1, std::min(b.slots, ts->slots));
         if (const char *env = std::getenv("MSG_LANCZOS_SLOTS")) {  // resident-workgroup override (tuning runs)
             const int sl = std::atoi(env);
             if (sl >= 1) b.slots = std::max(1, std::min(b.slots, sl));
@@ -711,6 +743,89 @@ msg_status msg_propagate(msg_ctx *ctx, const msg_system *sys, const msg_run_para
     });
 }
 
+msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_params *params, int32_t n_items,
+                        const double *fields, const double *columns, int32_t n_columns, int32_t sample_steps,
+                        msg_launch_shape *best) {
+    if (!ctx || !sys || !params || n_items < 1 || sample_steps < 1)
+        return fail(ctx, MSG_INVALID_ARG, "null argument, no items or no sample steps");
+    return guarded(ctx, [&] {
+        HIPCHECK(hipSetDevice(ctx->device));
+        const Resolved &r = sys->r;
+        msg_run_params p = *params;
+        p.calculation = MSG_CALC_YIELDS;
+        p.yield_corrections = 0;
+        p.total_time = sample_steps * p.time_step;
+        const Path path = choose_path(r, &p);
+        msg_launch_shape out{};
+        out.path = path == Path::Lanczos ? 1 : 2;
+        std::vector<msg_ctx::Shape> cand;
+        if (path == Path::Lanczos) {
+            const int ng = (int)(r.D / 4);
+            std::vector<msg_ctx::Shape> base;
+            if (ng <= 512) base.push_back({1, ng, 0});
+            else if (ng == 1024) { base.push_back({2, 512, 0}); base.push_back({1, 1024, 0}); }
+            else if (ng == 2048) base.push_back({2, 1024, 0});
+            else if (ng == 4096) { base.push_back({-1, 1024, 0}); base.push_back({0, 1024, 0}); }
+            else { base.push_back({-2, 1024, 0}); base.push_back({0, 1024, 0}); }
+            for (const auto &c : base) {
+                cand.push_back(c);  // every resident workgroup the occupancy allows
+                msg_ctx::Shape h = c;
+                h.slots = -2;       // half of them (fewer columns in flight, more cache per column)
+                cand.push_back(h);
+            }
+        } else {
+            cand.push_back({0, 0, 0});  // the dense path has a single launch shape
+        }
+        int nobs = 0;
+        {
+            int32_t n = 0;
+            if (msg_observable_count(sys, &p, &n) != MSG_OK) throw ArgError{MSG_INVALID_ARG, "observables"};
+            nobs = n;
+        }
+        std::vector<double> y((size_t)n_items * std::max(1, nobs));
+        double best_ms = 1e300;
+        msg_ctx::Shape best_shape{};
+        for (auto c : cand) {
+            double ms_min = 1e300;
+            int slots_used = 0;
+            for (int rep = 0; rep < 2; ++rep) {
+                msg_ctx::Shape trial = c;
+                Batch b;
+                if (path == Path::Lanczos && c.slots == -2) {
+                    // resolve "half the resident workgroups" against this launch's occupancy
+                    trial.slots = 0;
+                    ctx->force = &trial;
+                    Batch probe;
+                    prepare(probe, ctx, sys, &p, n_items, fields, columns, n_columns, false, false, ctx->stream);
+                    trial.slots = std::max(1, probe.slots / 2);
+                }
+                ctx->force = path == Path::Lanczos ? &trial : nullptr;
+                prepare(b, ctx, sys, &p, n_items, fields, columns, n_columns, false, false, ctx->stream);
+                ctx->force = nullptr;
+                b.timed = true;
+                run_chunk(b, b.nsamples);
+                collect(b, y.data(), nullptr);
+                float ms = 0.f;
+                HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+                ms_min = std::min(ms_min, (double)ms);
+                slots_used = b.slots;
+                if (path == Path::Lanczos && c.slots == -2) c.slots = trial.slots;
+            }
+            if (ms_min < best_ms) {
+                best_ms = ms_min;
+                best_shape = c;
+                out.groups_per_thread = c.G;
+                out.threads = path == Path::Lanczos ? c.T : 256;
+                out.slots = slots_used;
+            }
+        }
+        ctx->force = nullptr;
+        if (path == Path::Lanczos) ctx->tuned[r.D] = best_shape;
+        out.kernel_ms = best_ms;
+        if (best) *best = out;
+    });
+}
+
 msg_status msg_step(msg_ctx *ctx, const msg_system *sys, const msg_run_params *params, const double *field, double dt,
                     int64_t step_index, double *factors, int32_t n_columns) {
     if (!ctx || !sys || !params || !factors || n_columns < 1) return fail(ctx, MSG_INVALID_ARG, "null argument");
@@ -726,7 +841,6 @@ msg_status msg_step(msg_ctx *ctx, const msg_system *sys, const msg_run_params *p
         Batch b;
         prepare(b, ctx, sys, &p, 1, field, factors, n_columns, false, true, ctx->stream);
         if (b.nsamples != 2) throw ArgError{MSG_INVALID_ARG, "dt must be positive"};
-        if (step_index < 0 || step_index > 0x7fffffff) throw ArgError{MSG_INVALID_ARG, "step index out of range"};
         if (step_index < 0 || step_index > 0x7fffffff) throw ArgError{MSG_INVALID_ARG, "step index out of range"};
         b.a.rng_step0 = (int)step_index;  // relaxation draws of interval `step_index`
         run_chunk(b, 1);

@@ -111,10 +111,7 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
     C4 y;
 #pragma unroll
     for (int c = 0; c < 4; ++c) y.c[c] = make_float2(0.f, 0.f);
-    const int nn = NN >= 0 ? NN : a.nnuc;
-#pragma unroll
-    for (int n = 0; n < (NN >= 0 ? NN : kMaxNuclei); ++n) {
-        if (NN < 0 && n >= nn) break;
+    auto nucleus = [&](int n) {
         const float *cn = co + kCoefHead + n * kNucStride;
         const int fl = a.nuc_flags[n];
         const bool down = (nu >> n) & 1;
@@ -139,6 +136,12 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
 #pragma unroll
             for (int c = 0; c < 4; ++c) y.c[c] = cfma(tt, nb.c[c], y.c[c]);
         }
+    };
+    if constexpr (NN >= 0) {
+#pragma unroll
+        for (int n = 0; n < NN; ++n) nucleus(n);
+    } else {
+        for (int n = 0; n < a.nnuc; ++n) nucleus(n);
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -767,7 +770,6 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
                         C4 x;
 #pragma unroll
                         for (int c = 0; c < 4; ++c) x.c[c] = make_float2(0.f, 0.f);
-#pragma unroll 2
                         for (int j = 0; j < meff; ++j) {
                             const float2 cj = s_coef[j];
                             const C4 vj = load_group(BA(j), BA(j) + D4, nu);

@@ -368,3 +368,35 @@ def test_config1_trace_against_fixture(ctx):
     want = np.array(fx["trace"]).T
     for c in range(4):
         assert trace_err(tr[0, :, c], want[:, c]) <= TRACE_TOL, c
+
+
+def test_autotune_keeps_results(ctx):
+    """msg_autotune picks a launch shape per Hilbert dimension; yields computed
+    with the tuned shape agree with the default shape's to FP32 rounding, for the
+    LDS path (D = 4096) and the tiled streaming path (D = 65536)."""
+    from molspin_amd import backend
+    s, cfg = workloads.config2(n_orientations=1024)
+    fields = cfg["fields"][:4]
+    prm = abi.run_params(total_time=20.0, time_step=2.0, method=abi.MSG_METHOD_KRYLOV)
+    c2 = backend.Context(0)
+    try:
+        sysd = c2.upload(s.pack())
+        y0, _ = sysd.propagate(prm, fields=fields)
+        shape = sysd.autotune(prm, fields=fields, sample_steps=3)
+        print("D=4096 tuned", shape.groups_per_thread, shape.threads, shape.slots, shape.kernel_ms)
+        assert shape.path == 1 and shape.threads in (512, 1024) and shape.slots >= 1
+        y1, _ = sysd.propagate(prm, fields=fields)
+        assert np.abs(y1 - y0).max() <= 1e-7
+        s3, cfg3 = workloads.config3(n_traj=8)
+        sys3 = c2.upload(s3.pack())
+        cols = sys3.trace_samples(s3.state_index("S"), 8, cfg3["sample_seed"]) / math.sqrt(8)
+        p3 = abi.run_params(total_time=5.0, time_step=1.0, method=abi.MSG_METHOD_KRYLOV,
+                            sampling=abi.MSG_SAMPLING_GIVEN, seed=2026)
+        z0, _ = sys3.propagate(p3, columns=cols)
+        sh3 = sys3.autotune(p3, columns=cols, sample_steps=2)
+        print("D=65536 tuned", sh3.groups_per_thread, sh3.threads, sh3.slots, sh3.kernel_ms)
+        assert sh3.groups_per_thread in (-2, 0)
+        z1, _ = sys3.propagate(p3, columns=cols)
+        assert np.abs(z1 - z0).max() <= 1e-7
+    finally:
+        c2.close()
