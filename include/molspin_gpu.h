@@ -349,7 +349,8 @@ msg_status msg_server_destroy(msg_server *srv);
 msg_status msg_server_submit(msg_server *srv, const msg_system *sys, const msg_run_params *params,
                              int32_t n_fields, const double *fields, int64_t *request_id);
 /* Collects finished requests (without blocking), then admits queued requests
- * while a slot and the HBM budget allow.  max_steps <= 0: every admitted request
+ * while a slot and the HBM budget allow, shortest first (state-steps x D; arrival
+ * order among equals).  max_steps <= 0: every admitted request
  * is propagated over its whole grid in one persistent launch on its own stream,
  * so requests overlap on the device.  max_steps > 0: resident requests advance
  * in chunks of max_steps intervals per call, their states kept on the device.

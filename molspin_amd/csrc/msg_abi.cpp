@@ -133,6 +133,7 @@ struct Batch {
     Path path = Path::Lanczos;
     int n_items = 0, ncols = 0, n_units = 0, nobs = 0, nsamples = 0;
     int G = 1, T = 64, slots = 1;
+    int share = 1;  // the launch keeps 1/share of the resident workgroups it could hold (server space sharing)
     std::vector<double> times, scale;
     StepArgs a{};
     DBuf coef, U, dts, wy, wt, thr, yacc, trace, state, basis, queue, status, given, norm0, yout, tout;
@@ -475,6 +476,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         const int occ = std::max(1, lanczos_occupancy(b.G, b.T, lds));
         b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ));
         if (ts && ts->slots > 0) b.slots = std::max(1, std::min(b.slots, ts->slots));
+        if (b.share > 1) b.slots = std::max(1, std::min(b.slots, ctx->cus * occ / b.share));
         if (const char *env = std::getenv("MSG_LANCZOS_SLOTS")) {  // resident-workgroup override (tuning runs)
             const int sl = std::atoi(env);
             if (sl >= 1) b.slots = std::max(1, std::min(b.slots, sl));
@@ -487,6 +489,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         const size_t lds = (size_t)8 * a.dpad * 16 + 4096;
         const int occ = std::max(1, dense_occupancy(lds));
         b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ));
+        if (b.share > 1) b.slots = std::max(1, std::min(b.slots, ctx->cus * occ / b.share));
     }
     b.footprint = b.coef.n + b.U.n + b.dts.n + b.wy.n + b.wt.n + b.thr.n + b.yacc.n + b.trace.n + b.state.n +
                   b.basis.n + b.given.n + b.norm0.n;
@@ -584,6 +587,7 @@ struct Request {
     int stream = -1;             // index of the server stream the request owns
     bool whole = false;          // launched to completion in one persistent launch
     hipEvent_t done_ev = nullptr;
+    double work = 0.0;           // state-steps x D: admission order (shortest first)
 };
 
 struct msg_server {
@@ -1008,6 +1012,10 @@ msg_status msg_server_submit(msg_server *srv, const msg_system *sys, const msg_r
             throw ArgError{MSG_UNSUPPORTED, "request not representable on the device"};
         q->bytes = bytes;
         if (bytes > srv->budget) throw ArgError{MSG_OOM_ADMISSION, "request exceeds the HBM budget"};
+        std::vector<double> t;
+        (void)free_times(params->total_time, params->time_step, t);
+        q->work = (double)sys->r.D * (double)sys->r.zdim * std::max<size_t>(1, sys->r.evec.size()) * n_fields *
+                  (double)t.size();
         *request_id = q->id;
         srv->queued.push_back(std::move(q));
     });
@@ -1045,14 +1053,21 @@ msg_status msg_server_pump(msg_server *srv, int32_t max_steps, int32_t *n_active
                 ++it;
             }
         }
-        // 2. admission by free stream and HBM footprint, first come first served
-        while (!srv->queued.empty() && srv->in_use + srv->queued.front()->bytes <= srv->budget) {
+        // 2. admission by free stream and HBM footprint, shortest job first (arrival order
+        //    among equals).  (Capping each launch's resident workgroups while others wait
+        //    was measured slower: a persistent launch cannot grow once the others finish.)
+        for (;;) {
             int slot = -1;
             for (int i = 0; i < (int)srv->streams.size(); ++i)
                 if (!srv->stream_busy[i]) { slot = i; break; }
             if (slot < 0) break;
-            std::unique_ptr<Request> q = std::move(srv->queued.front());
-            srv->queued.pop_front();
+            auto pick = srv->queued.end();
+            for (auto it = srv->queued.begin(); it != srv->queued.end(); ++it)
+                if (srv->in_use + (*it)->bytes <= srv->budget && (pick == srv->queued.end() || (*it)->work < (*pick)->work))
+                    pick = it;
+            if (pick == srv->queued.end()) break;
+            std::unique_ptr<Request> q = std::move(*pick);
+            srv->queued.erase(pick);
             q->stream = slot;
             q->batch.reset(new Batch());
             q->whole = max_steps <= 0;
