@@ -285,9 +285,27 @@ def serve(ctx, n_requests, intervals):
             r["sys"].propagate(r["prm"], fields=[f])
         plain_s += (time.perf_counter() - t1) / 2 * len(r["fields"])
     plain = total / plain_s
+    # small systems (D <= 512, dense path): the whole field sweep in one call against
+    # one field per call, both timed in full on the same requests
+    small = [r for r in reqs if r["sys"].dim <= 512]
+    by_class = None
+    if small:
+        steps_small = sum(r["steps"] for r in small)
+        t1 = time.perf_counter()
+        for r in small:
+            for f in r["fields"]:
+                r["sys"].propagate(r["prm"], fields=[f])
+        t_plain = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        for r in small:
+            r["sys"].propagate(r["prm"], fields=r["fields"])
+        t_batched = time.perf_counter() - t1
+        by_class = {"requests": len(small), "plain_state_steps_per_s": steps_small / t_plain,
+                    "batched_state_steps_per_s": steps_small / t_batched, "batched_over_plain": t_plain / t_batched}
     return {"requests": n_requests, "p50_ms": float(np.percentile(lat, 50) * 1e3),
             "p99_ms": float(np.percentile(lat, 99) * 1e3), "state_steps_per_s": total / elapsed,
             "plain_one_at_a_time_state_steps_per_s": plain, "served_over_plain": total / elapsed / plain,
+            "small_systems_D_le_512": by_class,
             "elapsed_s": elapsed, "intervals_per_request": intervals,
             "workload": "Poisson 200 req/s, seeded radical pairs (D 16..16384), 16-128 fields/request"}
 

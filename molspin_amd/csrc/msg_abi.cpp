@@ -42,7 +42,7 @@ hipError_t launch_blockdiag_right(const void *M, const double *B4, double scale,
 hipError_t launch_lorentz_sum(const void *Pt, const void *Rt, const double *lam, double k, int n, double *rows,
                               double *out, hipStream_t st);
 hipError_t launch_to_planes(const void *U, float *dst, int n, int Dp, size_t stride, int batch, hipStream_t st);
-int dense_occupancy(size_t lds);
+int dense_occupancy(int dpad, size_t lds);
 }  // namespace msg
 
 using namespace msg;
@@ -207,6 +207,14 @@ Path choose_path(const Resolved &r, const msg_run_params *p) {
 }
 
 const StepArgs *push_args(Batch &b, const StepArgs &a);
+
+// Padded dimension of the dense path: 16 x (one of the row-tile counts the
+// kernel is compiled for); 0 above 512.
+int dense_padding(int D) {
+    for (int nt : {1, 2, 3, 4, 6, 8, 12, 16, 24, 32})
+        if (16 * nt >= D) return 16 * nt;
+    return 0;
+}
 
 // Builds a batch and uploads everything a launch needs.
 void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params *p, int n_items,
@@ -386,8 +394,8 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         b.n_units = n_items * b.ncols;
         a.renorm = factor_decay ? 1 : 0;
     } else {
-        const int Dp = (D + 15) / 16 * 16;
-        if (Dp > 512) throw ArgError{MSG_UNSUPPORTED, "dense path limited to D <= 512"};
+        const int Dp = dense_padding(D);
+        if (Dp == 0) throw ArgError{MSG_UNSUPPORTED, "dense path limited to D <= 512"};
         a.dpad = Dp;
         const bool two = ns >= 2 && std::fabs(dts[ns - 2] - dts[0]) > 0.0;
         a.ulast = two ? 1 : 0;
@@ -487,7 +495,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         if (a.renorm) HIPCHECK(launch_init_norms(a, push_args(b, a), stream));
     } else {
         const size_t lds = (size_t)8 * a.dpad * 16 + 4096;
-        const int occ = std::max(1, dense_occupancy(lds));
+        const int occ = std::max(1, dense_occupancy(a.dpad, lds));
         b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ));
         if (b.share > 1) b.slots = std::max(1, std::min(b.slots, ctx->cus * occ / b.share));
     }
@@ -719,7 +727,7 @@ msg_status msg_query_footprint(const msg_system *sys, const msg_run_params *p, i
         const int vecs = D > 8192 ? m + 2 : m;  // streaming variants keep residual and state beside the basis
         b += std::min<int64_t>(units, 2048) * vecs * 8 * D;  // Krylov bases of the resident workgroups
     } else {
-        const int64_t Dp = (D + 15) / 16 * 16;
+        const int64_t Dp = dense_padding((int)D);
         const int64_t units = (int64_t)n_items * ((ncols + 15) / 16);
         b += (int64_t)n_items * 2 * 8 * Dp * Dp;
         b += units * nobs * 16;

@@ -863,7 +863,6 @@ __global__ void k_reduce_columns(const double *in, double *out, int n_items, int
 // Dense path: B <- U B on the FP32 matrix cores.
 // ---------------------------------------------------------------------------
 typedef float floatx4 __attribute__((ext_vector_type(4)));
-constexpr int kDenseMaxTiles = 8;  // row tiles per wave: Dp <= 4 waves * 8 * 16 = 512
 
 // Element (row, col) of the initial state of a dense-path unit.
 __device__ __forceinline__ float2 dense_init(const StepArgs &a, int u, int col, int row) {
@@ -885,18 +884,22 @@ __device__ __forceinline__ float2 dense_init(const StepArgs &a, int u, int col, 
     return make_float2(st[row * 16 + tcol], st[a.dpad * 16 + row * 16 + tcol]);
 }
 
+// NT: row tiles of 16 (padded dimension Dp = 16 NT, a compile-time constant so the
+// product loop and the per-wave tile loops unroll); each wave owns tiles wave + 4k.
+template <int NT>
 __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) {
     const StepArgs &a = *pa;
     extern __shared__ float dsm[];
     __shared__ double ys[kMaxObs], yc[kMaxObs];  // Kahan yield sums (thread 0)
     __shared__ double s_col[2][64];              // per-wave column norms (double-buffered)
-    const int Dp = a.dpad;
+    constexpr int Dp = 16 * NT;
+    constexpr int KT = (NT + 3) / 4;  // row tiles per wave
     float *Bre = dsm, *Bim = dsm + Dp * 16;
     double *scr = reinterpret_cast<double *>(dsm + 2 * Dp * 16);
     __shared__ int s_unit;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int q = lane >> 4, cl = lane & 15;
-    const int ntile = Dp >> 4;
+    constexpr int ntile = NT;
     const int tiles_per_item = (a.ncols + 15) >> 4;
     int par = 0, cpar = 0;
     auto SCR = [&]() { par ^= 1; return scr + par * 256; };
@@ -904,7 +907,7 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
     auto colnorm2 = [&](const floatx4 *xr_, const floatx4 *xi_) -> double {
         float p = 0.f;
 #pragma unroll
-        for (int k = 0; k < kDenseMaxTiles; ++k)
+        for (int k = 0; k < KT; ++k)
 #pragma unroll
             for (int i = 0; i < 4; ++i) p = fmaf(xr_[k][i], xr_[k][i], fmaf(xi_[k][i], xi_[k][i], p));
         double d = p;
@@ -924,9 +927,9 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
         const int item = u / tiles_per_item, tile = u - item * tiles_per_item;
         const int col = tile * 16 + cl;  // the column this lane's accumulators hold
         // register copy of the state in MFMA output layout
-        floatx4 xr[kDenseMaxTiles], xi[kDenseMaxTiles];
+        floatx4 xr[KT], xi[KT];
 #pragma unroll
-        for (int k = 0; k < kDenseMaxTiles; ++k) {
+        for (int k = 0; k < KT; ++k) {
             const int rt = wave + 4 * k;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -950,7 +953,7 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
 #pragma unroll
                 for (int k = 0; k < 16; ++k) Rf[k] = 0.f;
 #pragma unroll
-                for (int k = 0; k < kDenseMaxTiles; ++k) {
+                for (int k = 0; k < KT; ++k) {
                     C4 g;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) g.c[i] = make_float2(xr[k][i], xi[k][i]);
@@ -965,7 +968,7 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
             if (end) break;
             if (a.relax && colok && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s])
 #pragma unroll
-                for (int k = 0; k < kDenseMaxTiles; ++k) {
+                for (int k = 0; k < KT; ++k) {
                     C4 g;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) g.c[i] = make_float2(xr[k][i], xi[k][i]);
@@ -976,7 +979,7 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
             // publish B to LDS ([row][16 columns] planes)
             __syncthreads();
 #pragma unroll
-            for (int k = 0; k < kDenseMaxTiles; ++k) {
+            for (int k = 0; k < KT; ++k) {
                 const int rt = wave + 4 * k;
                 if (rt < ntile)
 #pragma unroll
@@ -990,7 +993,7 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
             const int uw = (a.ulast && s == a.nsamples - 2) ? 1 : 0;
             const float *Ur = a.U + ((size_t)item * a.ustride + uw) * 2 * Dp * Dp, *Ui = Ur + (size_t)Dp * Dp;
 #pragma unroll
-            for (int k = 0; k < kDenseMaxTiles; ++k) {
+            for (int k = 0; k < KT; ++k) {
                 xr[k] = floatx4{0.f, 0.f, 0.f, 0.f};
                 xi[k] = floatx4{0.f, 0.f, 0.f, 0.f};
             }
@@ -998,7 +1001,7 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
                 const int kr = kk * 4 + q;
                 const float br = Bre[kr * 16 + cl], bi = Bim[kr * 16 + cl];
 #pragma unroll
-                for (int k = 0; k < kDenseMaxTiles; ++k) {
+                for (int k = 0; k < KT; ++k) {
                     const int rt = wave + 4 * k;
                     if (rt < ntile) {
                         const float ar = Ur[(size_t)kr * Dp + rt * 16 + cl];
@@ -1014,14 +1017,14 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
                 const double n2 = colnorm2(xr, xi);
                 const float f = n2 > 0.0 ? (float)(cn0 / sqrt(n2)) : 0.f;
 #pragma unroll
-                for (int k = 0; k < kDenseMaxTiles; ++k) {
+                for (int k = 0; k < KT; ++k) {
                     xr[k] *= f;
                     xi[k] *= f;
                 }
             }
             if (a.relax && colok && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
 #pragma unroll
-                for (int k = 0; k < kDenseMaxTiles; ++k) {
+                for (int k = 0; k < KT; ++k) {
                     C4 g;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) g.c[i] = make_float2(xr[k][i], xi[k][i]);
@@ -1037,7 +1040,7 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
             }
         bool bad = false;
 #pragma unroll
-        for (int k = 0; k < kDenseMaxTiles; ++k) {
+        for (int k = 0; k < KT; ++k) {
             const int rt = wave + 4 * k;
             if (rt < ntile)
 #pragma unroll
@@ -1187,6 +1190,7 @@ __global__ void k_to_planes(const double2 *__restrict__ U, float *__restrict__ d
 // ---------------------------------------------------------------------------
 // Launchers (called from the host part of the library)
 // ---------------------------------------------------------------------------
+#define MSG_DENSE_VARIANTS(X) X(1) X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32)
 #define MSG_LANCZOS_VARIANTS(X) \
     X(1, 64, true) X(1, 128, true) X(1, 256, true) X(1, 512, true) X(1, 1024, true) X(2, 512, true) X(2, 1024, true)
 
@@ -1223,7 +1227,9 @@ hipError_t set_lanczos_lds_limits() {
 #define X(G_, T_, B_) (void)hipFuncSetAttribute((const void *)k_lanczos<G_, T_, B_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     MSG_LANCZOS_VARIANTS(X)
 #undef X
-    (void)hipFuncSetAttribute((const void *)k_dense, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+#define X(NT_) (void)hipFuncSetAttribute((const void *)k_dense<NT_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    MSG_DENSE_VARIANTS(X)
+#undef X
     (void)hipFuncSetAttribute((const void *)k_lanczos_big<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               maxb - 8192);
     (void)hipFuncSetAttribute((const void *)k_lanczos_big<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1234,8 +1240,14 @@ hipError_t set_lanczos_lds_limits() {
 
 hipError_t launch_dense(const StepArgs &a, const StepArgs *da, int slots, hipStream_t st) {
     const size_t lds = (size_t)2 * a.dpad * 16 * sizeof(float) + 2 * 256 * sizeof(double);
-    hipLaunchKernelGGL(k_dense, dim3(slots), dim3(256), lds, st, da);
-    return hipGetLastError();
+#define X(NT_)                                                                  \
+    if (a.dpad == 16 * NT_) {                                                   \
+        hipLaunchKernelGGL(k_dense<NT_>, dim3(slots), dim3(256), lds, st, da); \
+        return hipGetLastError();                                               \
+    }
+    MSG_DENSE_VARIANTS(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
 int lanczos_occupancy(int group_per_thread, int threads, size_t lds) {
@@ -1256,10 +1268,14 @@ int lanczos_occupancy(int group_per_thread, int threads, size_t lds) {
     return 1;
 }
 
-int dense_occupancy(size_t lds) {
+int dense_occupancy(int dpad, size_t lds) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_dense, 256, lds) != hipSuccess) return 1;
-    return nb;
+#define X(NT_)                                                                                      \
+    if (dpad == 16 * NT_)                                                                           \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_dense<NT_>, 256, lds) == hipSuccess ? nb : 1;
+    MSG_DENSE_VARIANTS(X)
+#undef X
+    return 1;
 }
 
 hipError_t launch_zgemm(const void *A, const void *B, void *C, int n, int batch, double coef, double diag_add,
