@@ -37,7 +37,9 @@ hipError_t launch_reduce_columns(const double *in, double *out, int n_items, int
                                  hipStream_t st);
 int lanczos_occupancy(int group_per_thread, int threads, size_t lds);
 hipError_t launch_zgemm(const void *A, const void *B, void *C, int n, int batch, double coef, double diag_add,
-                        hipStream_t st, bool conj_b = false);
+                        hipStream_t st, bool conj_b = false, const void *E = nullptr);
+hipError_t launch_zlincomb(const void *X, const void *X2, const void *X3, void *out, int n, int batch,
+                           const double *c, hipStream_t st);
 hipError_t launch_blockdiag_right(const void *M, const double *B4, double scale, void *Y, int n, hipStream_t st);
 hipError_t launch_lorentz_sum(const void *Pt, const void *Rt, const double *lam, double k, int n, double *rows,
                               double *out, hipStream_t st);
@@ -400,15 +402,26 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         const bool two = ns >= 2 && std::fabs(dts[ns - 2] - dts[0]) > 0.0;
         a.ulast = two ? 1 : 0;
         const int per = two ? 2 : 1;
-        // exp(-i H_eff dt) on the device in FP64: Taylor(18) of X = A / 2^s, then s squarings;
-        // items are processed in chunks to bound the FP64 workspace
+        // exp(-i H_eff dt) on the device in FP64: Taylor(18) of X = A / 2^s evaluated by
+        // Paterson-Stockmeyer in X^4 (7 complex GEMMs), then s squarings; items are
+        // processed in chunks to bound the FP64 workspace
         float *dU = b.U.alloc<float>((size_t)n_items * per * 2 * Dp * Dp, stream);
         HIPCHECK(hipMemsetAsync(dU, 0, (size_t)n_items * per * 2 * Dp * Dp * sizeof(float), stream));
-        const int chunk = std::max(1, std::min(n_items, (int)((size_t)256 << 20) / (D * D * 16)));
-        DBuf X, P, Q;
+        const int chunk = std::max(1, std::min(n_items, (int)((size_t)128 << 20) / (D * D * 16)));
+        DBuf X, P, Q, X2, X3, X4, Bk;
         cd *dX = X.alloc<cd>((size_t)chunk * D * D, stream);
         cd *dP = P.alloc<cd>((size_t)chunk * D * D, stream);
         cd *dQ = Q.alloc<cd>((size_t)chunk * D * D, stream);
+        cd *dX2 = X2.alloc<cd>((size_t)chunk * D * D, stream);
+        cd *dX3 = X3.alloc<cd>((size_t)chunk * D * D, stream);
+        cd *dX4 = X4.alloc<cd>((size_t)chunk * D * D, stream);
+        cd *dB = Bk.alloc<cd>((size_t)chunk * D * D, stream);
+        double fact[19];  // 1/j!
+        fact[0] = 1.0;
+        for (int j = 1; j <= 18; ++j) fact[j] = fact[j - 1] / j;
+        auto taylor_block = [&](int k, double c[4]) {  // coefficients of X^0..X^3 in block k
+            for (int i = 0; i < 4; ++i) c[i] = 4 * k + i <= 18 ? fact[4 * k + i] : 0.0;
+        };
         std::vector<cd> H, Hs((size_t)chunk * D * D);
         for (int w = 0; w < per; ++w) {
             const double dt = w == 0 ? dts[0] : dts[ns - 2];
@@ -434,10 +447,18 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                 const double scale = std::ldexp(1.0, -sq);
                 for (auto &x : Hs) x *= scale;
                 HIPCHECK(hipMemcpyAsync(dX, Hs.data(), (size_t)nb * D * D * sizeof(cd), hipMemcpyHostToDevice, stream));
-                // Horner: P = I + X/18; P = I + X P / k, k = 17..1
-                HIPCHECK(launch_zgemm(dX, dX, dP, D, nb, 0.0, 1.0, stream));  // P = I
-                for (int k = 18; k >= 1; --k) {
-                    HIPCHECK(launch_zgemm(dX, dP, dQ, D, nb, 1.0 / k, 1.0, stream));
+                // powers X^2, X^3, X^4; then P = B_4, P = P X^4 + B_k for k = 3..0 with
+                // B_k = sum_i X^i / (4k+i)!  (the same degree-18 Taylor polynomial)
+                HIPCHECK(launch_zgemm(dX, dX, dX2, D, nb, 1.0, 0.0, stream));
+                HIPCHECK(launch_zgemm(dX2, dX, dX3, D, nb, 1.0, 0.0, stream));
+                HIPCHECK(launch_zgemm(dX2, dX2, dX4, D, nb, 1.0, 0.0, stream));
+                double c[4];
+                taylor_block(4, c);
+                HIPCHECK(launch_zlincomb(dX, dX2, dX3, dP, D, nb, c, stream));
+                for (int k = 3; k >= 0; --k) {
+                    taylor_block(k, c);
+                    HIPCHECK(launch_zlincomb(dX, dX2, dX3, dB, D, nb, c, stream));
+                    HIPCHECK(launch_zgemm(dP, dX4, dQ, D, nb, 1.0, 0.0, stream, false, dB));
                     std::swap(dP, dQ);
                 }
                 for (int q = 0; q < sq; ++q) {

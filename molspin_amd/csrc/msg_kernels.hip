@@ -1064,11 +1064,11 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
 // ---------------------------------------------------------------------------
 constexpr int kZT = 32;  // output tile (kZT x kZT) per workgroup of 256 threads
 
-// C_b = diag_add * I + coef * A_b op(B_b) for every batch entry b; op(B) = B, or
-// B^dagger when conj_b is set.
+// C_b = diag_add * I + coef * A_b op(B_b) (+ E_b when E is given) for every batch
+// entry b; op(B) = B, or B^dagger when conj_b is set.
 __global__ __launch_bounds__(256) void k_zgemm(const double2 *__restrict__ A, const double2 *__restrict__ B,
                                                double2 *__restrict__ C, int n, double coef, double diag_add,
-                                               int conj_b) {
+                                               int conj_b, const double2 *__restrict__ E) {
     __shared__ double2 sA[kZT][kZT + 1], sB[kZT][kZT + 1];
     const size_t off = (size_t)blockIdx.z * n * n;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads, 4 rows each
@@ -1105,8 +1105,28 @@ __global__ __launch_bounds__(256) void k_zgemm(const double2 *__restrict__ A, co
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int i = row0 + ty + 8 * r, j = col0 + tx;
-        if (i < n && j < n)
-            C[off + (size_t)i * n + j] = make_double2(coef * acc[r].x + (i == j ? diag_add : 0.0), coef * acc[r].y);
+        if (i < n && j < n) {
+            double2 v = make_double2(coef * acc[r].x + (i == j ? diag_add : 0.0), coef * acc[r].y);
+            if (E) {
+                const double2 e = E[off + (size_t)i * n + j];
+                v.x += e.x;
+                v.y += e.y;
+            }
+            C[off + (size_t)i * n + j] = v;
+        }
+    }
+}
+
+// out_b = c0 I + c1 X_b + c2 X2_b + c3 X3_b (a block of the Paterson-Stockmeyer
+// evaluation of the Taylor polynomial); batch entries of n x n, contiguous.
+__global__ void k_zlincomb(const double2 *__restrict__ X, const double2 *__restrict__ X2,
+                           const double2 *__restrict__ X3, double2 *__restrict__ out, int n, size_t total, double c0,
+                           double c1, double c2, double c3) {
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t e = idx % ((size_t)n * n);
+        const bool diag = (e / n) == (e % n);
+        const double2 x = X[idx], x2 = X2[idx], x3 = X3[idx];
+        out[idx] = make_double2((diag ? c0 : 0.0) + c1 * x.x + c2 * x2.x + c3 * x3.x, c1 * x.y + c2 * x2.y + c3 * x3.y);
     }
 }
 
@@ -1279,10 +1299,19 @@ int dense_occupancy(int dpad, size_t lds) {
 }
 
 hipError_t launch_zgemm(const void *A, const void *B, void *C, int n, int batch, double coef, double diag_add,
-                        hipStream_t st, bool conj_b) {
+                        hipStream_t st, bool conj_b, const void *E) {
     const int nt = (n + kZT - 1) / kZT;
     hipLaunchKernelGGL(k_zgemm, dim3(nt, nt, batch), dim3(256), 0, st, (const double2 *)A, (const double2 *)B,
-                       (double2 *)C, n, coef, diag_add, conj_b ? 1 : 0);
+                       (double2 *)C, n, coef, diag_add, conj_b ? 1 : 0, (const double2 *)E);
+    return hipGetLastError();
+}
+
+hipError_t launch_zlincomb(const void *X, const void *X2, const void *X3, void *out, int n, int batch,
+                           const double *c, hipStream_t st) {
+    const size_t total = (size_t)batch * n * n;
+    const int blocks = (int)std::min<size_t>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_zlincomb, dim3(blocks), dim3(256), 0, st, (const double2 *)X, (const double2 *)X2,
+                       (const double2 *)X3, (double2 *)out, n, total, c[0], c[1], c[2], c[3]);
     return hipGetLastError();
 }
 
