@@ -141,7 +141,10 @@ enum {
 enum {
     MSG_METHOD_AUTO = 0,          /* device chooses: dense U for D<=512, matrix-free Krylov above */
     MSG_METHOD_KRYLOV = 1,        /* propagationmethod=krylov, fixed krylovsize                   */
-    MSG_METHOD_EXPONENTIAL = 2    /* propagationmethod=normal: exact exp(dt G) applied as a GEMM  */
+    MSG_METHOD_EXPONENTIAL = 2,   /* propagationmethod=normal: exact exp(dt G) applied as a GEMM  */
+    MSG_METHOD_RK4 = 3            /* propagationmethod=rk4 (HSPropagator.cpp:80-90): for a static G
+                                     one RK4 step is the degree-4 Taylor polynomial of dt G, applied
+                                     as a GEMM (D <= 512)                                          */
 };
 enum {
     MSG_SAMPLING_DIRECT = 0,      /* factors of rho0 (HSStatePreparation.cpp:77-165)              */
@@ -178,7 +181,7 @@ enum {
     MSG_PROP_EXPONENTIAL = 0,     /* normal / exp / exponential / matrixexponential */
     MSG_PROP_AUTOEXPM = 1,        /* autoexpm                                       */
     MSG_PROP_KRYLOV = 2,          /* krylov                                         */
-    MSG_PROP_RK4 = 3              /* rk4 / explicit (not on the device path)        */
+    MSG_PROP_RK4 = 3              /* rk4 / explicit                                 */
 };
 
 /* A resolved Task property list (ResolveExecutionPlan, HSExecutionPlan.cpp:89-615). */
@@ -196,7 +199,7 @@ typedef struct msg_plan {
  * Resolves the property list of an MSD Task block, "key=value; key=value; ...",
  * with ResolveExecutionPlan's keys, aliases, defaults and error messages.  Keys
  * selecting physics off this path (dynamics=dynamic, powder grids, secular
- * approximation, spin-polarisation/CIDSP output, pulse sequences, rk4) return
+ * approximation, spin-polarisation/CIDSP output, pulse sequences) return
  * MSG_UNSUPPORTED naming the key; invalid values return MSG_INVALID_ARG with the
  * reference's message.  `error` (error_len bytes, may be NULL) receives the message.
  * Host-only: needs no device.

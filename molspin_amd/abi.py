@@ -28,7 +28,7 @@ MSG_OBS_STATE_POPULATIONS = 1
 MSG_METHOD_AUTO = 0
 MSG_METHOD_KRYLOV = 1
 MSG_METHOD_EXPONENTIAL = 2
-MSG_METHOD_RK4 = 3  # oracle only
+MSG_METHOD_RK4 = 3
 MSG_SAMPLING_DIRECT = 0
 MSG_SAMPLING_GIVEN = 1
 MSG_TASK_HSGENERAL = 0

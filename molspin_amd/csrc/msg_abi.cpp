@@ -158,8 +158,9 @@ void check_params(const msg_system *sys, const msg_run_params *p) {
         throw ArgError{MSG_INVALID_ARG, "calculation must be yields or timeevolution"};
     if (p->observables != MSG_OBS_TRANSITION_YIELDS && p->observables != MSG_OBS_STATE_POPULATIONS)
         throw ArgError{MSG_INVALID_ARG, "unknown observable mode"};
-    if (p->method != MSG_METHOD_AUTO && p->method != MSG_METHOD_KRYLOV && p->method != MSG_METHOD_EXPONENTIAL)
-        throw ArgError{MSG_INVALID_ARG, "propagationmethod must be auto, krylov or exponential on the device"};
+    if (p->method != MSG_METHOD_AUTO && p->method != MSG_METHOD_KRYLOV && p->method != MSG_METHOD_EXPONENTIAL &&
+        p->method != MSG_METHOD_RK4)
+        throw ArgError{MSG_INVALID_ARG, "propagationmethod must be auto, krylov, exponential or rk4 on the device"};
     if (p->krylov_size < 0 || p->krylov_size > kMaxKrylov) throw ArgError{MSG_UNSUPPORTED, "krylovsize above 32"};
     if (p->sampling != MSG_SAMPLING_DIRECT && p->sampling != MSG_SAMPLING_GIVEN)
         throw ArgError{MSG_INVALID_ARG, "unknown sampling mode"};
@@ -195,8 +196,9 @@ double kappa_for(const Resolved &r, const msg_run_params *p) {
 Path choose_path(const Resolved &r, const msg_run_params *p) {
     const bool dense_ok = r.D <= 512;
     const bool lanczos_ok = r.fast_ok && sym_for(r, p) && r.D >= 256 && r.D <= (int64_t)4 << kMaxNuclei;
-    if (p->method == MSG_METHOD_EXPONENTIAL) {
-        if (!dense_ok) throw ArgError{MSG_UNSUPPORTED, "propagationmethod=exponential needs D <= 512 on the device"};
+    if (p->method == MSG_METHOD_EXPONENTIAL || p->method == MSG_METHOD_RK4) {
+        if (!dense_ok)
+            throw ArgError{MSG_UNSUPPORTED, "propagationmethod=exponential/rk4 needs D <= 512 on the device"};
         return Path::Dense;
     }
     if (p->method == MSG_METHOD_AUTO && dense_ok) return Path::Dense;
@@ -266,7 +268,9 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     const int ns = b.nsamples;
     std::vector<double> dts(std::max(ns - 1, 1), 0.0), wyh((size_t)ns * b.nobs), wth((size_t)ns * b.nobs);
     std::vector<uint32_t> thr(std::max(ns - 1, 1), 0u);
-    const bool factor_decay = sym_for(r, p);
+    // symmetric K = kappa 1 is factored out as e^{-2 kappa t} -- exact for the exponential
+    // and Krylov propagators, not for RK4's polynomial, which keeps K inside
+    const bool factor_decay = sym_for(r, p) && p->method != MSG_METHOD_RK4;
     const double kappa = factor_decay ? kappa_for(r, p) : 0.0;
     for (int s = 0; s + 1 < ns; ++s) {
         dts[s] = b.times[s + 1] - b.times[s];
@@ -443,7 +447,8 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                     for (size_t k = 0; k < H.size(); ++k) Hs[(size_t)q * D * D + k] = cd(0.0, -dt) * H[k];
                 }
                 int sq = 0;
-                while (nrm > 0.5) { nrm *= 0.5; ++sq; }
+                const bool rk4 = p->method == MSG_METHOD_RK4;
+                while (!rk4 && nrm > 0.5) { nrm *= 0.5; ++sq; }
                 const double scale = std::ldexp(1.0, -sq);
                 for (auto &x : Hs) x *= scale;
                 HIPCHECK(hipMemcpyAsync(dX, Hs.data(), (size_t)nb * D * D * sizeof(cd), hipMemcpyHostToDevice, stream));
@@ -451,15 +456,22 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                 // B_k = sum_i X^i / (4k+i)!  (the same degree-18 Taylor polynomial)
                 HIPCHECK(launch_zgemm(dX, dX, dX2, D, nb, 1.0, 0.0, stream));
                 HIPCHECK(launch_zgemm(dX2, dX, dX3, D, nb, 1.0, 0.0, stream));
-                HIPCHECK(launch_zgemm(dX2, dX2, dX4, D, nb, 1.0, 0.0, stream));
                 double c[4];
-                taylor_block(4, c);
-                HIPCHECK(launch_zlincomb(dX, dX2, dX3, dP, D, nb, c, stream));
-                for (int k = 3; k >= 0; --k) {
-                    taylor_block(k, c);
+                if (rk4) {
+                    // one RK4 step of d psi/dt = G psi: I + X + X^2/2 + X^3/6 + X^4/24
+                    taylor_block(0, c);
                     HIPCHECK(launch_zlincomb(dX, dX2, dX3, dB, D, nb, c, stream));
-                    HIPCHECK(launch_zgemm(dP, dX4, dQ, D, nb, 1.0, 0.0, stream, false, dB));
-                    std::swap(dP, dQ);
+                    HIPCHECK(launch_zgemm(dX2, dX2, dP, D, nb, 1.0 / 24.0, 0.0, stream, false, dB));
+                } else {
+                    HIPCHECK(launch_zgemm(dX2, dX2, dX4, D, nb, 1.0, 0.0, stream));
+                    taylor_block(4, c);
+                    HIPCHECK(launch_zlincomb(dX, dX2, dX3, dP, D, nb, c, stream));
+                    for (int k = 3; k >= 0; --k) {
+                        taylor_block(k, c);
+                        HIPCHECK(launch_zlincomb(dX, dX2, dX3, dB, D, nb, c, stream));
+                        HIPCHECK(launch_zgemm(dP, dX4, dQ, D, nb, 1.0, 0.0, stream, false, dB));
+                        std::swap(dP, dQ);
+                    }
                 }
                 for (int q = 0; q < sq; ++q) {
                     HIPCHECK(launch_zgemm(dP, dP, dQ, D, nb, 1.0, 0.0, stream));

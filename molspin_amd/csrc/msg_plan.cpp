@@ -283,8 +283,8 @@ void resolve(const Props &p, msg_plan &out) {
         r.method = MSG_METHOD_KRYLOV;
         out.propagation = MSG_PROP_KRYLOV;
     } else if (one_of(v, {"rk4", "explicit"})) {
+        r.method = MSG_METHOD_RK4;
         out.propagation = MSG_PROP_RK4;
-        off("propagationmethod=rk4 is not on the device path");
     } else {
         throw invalid("propagationmethod must be normal/exp, autoexpm, krylov, or rk4");
     }

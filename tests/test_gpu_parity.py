@@ -400,3 +400,26 @@ def test_autotune_keeps_results(ctx):
         assert np.abs(z1 - z0).max() <= 1e-7
     finally:
         c2.close()
+
+
+@pytest.mark.parametrize("singlet_only", [False, True])
+def test_rk4_matches_oracle(ctx, singlet_only):
+    """propagationmethod=rk4 (HSPropagator.cpp:80-90): config 1 (D = 48) with the
+    four equal sinks, and with a singlet-only sink (K not a multiple of the
+    identity), yields and P(t) traces against the oracle's RK4 on the same grid."""
+    s, cfg = workloads.config1()
+    if singlet_only:
+        s.transitions = [t for t in s.transitions if t["name"] == "kS"] or s.transitions[:1]
+    p = s.pack()
+    sysd = ctx.upload(p)
+    prm = abi.run_params(total_time=400.0, time_step=1.0, method=abi.MSG_METHOD_RK4)
+    y, _ = sysd.propagate(prm)
+    yo, _ = ref.run_hsgeneral(p, prm, threads=8)
+    print("rk4 yields dev", y[0], "oracle", yo[0])
+    assert np.abs(y - yo).max() <= YIELD_TOL
+    prm_t = abi.run_params(calculation=abi.MSG_CALC_TIMEEVOLUTION, observables=abi.MSG_OBS_STATE_POPULATIONS,
+                           total_time=400.0, time_step=1.0, method=abi.MSG_METHOD_RK4)
+    _, tr = sysd.propagate(prm_t, trace=True)
+    _, tro = ref.run_hsgeneral(p, prm_t, trace=True, threads=8)
+    for c in range(tr.shape[2]):
+        assert trace_err(tr[0, :, c], tro[0, :, c]) <= TRACE_TOL, c

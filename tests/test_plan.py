@@ -96,7 +96,9 @@ def test_reference_rejections():
 
 
 def test_valid_but_off_the_device_path():
-    for text in ("dynamics=dynamic;calculation=yields;", "propagationmethod=rk4;",
+    p = resolve("type=HSGeneral;propagationmethod=rk4;")
+    assert p.params.method == abi.MSG_METHOD_RK4 and p.propagation == abi.MSG_PROP_RK4
+    for text in ("dynamics=dynamic;calculation=yields;",
                  "calculation=timeevolution;observables=spins;spinlist=E;",
                  "calculation=yields;approximation=secular;hamiltonianh0list=H0;",
                  "calculation=timeevolution;hamiltonianh0list=H0;powdergrid=uniform;powdersamplingpoints=8;",
