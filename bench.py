@@ -37,7 +37,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["orientations", "trajectories", "dense"], default="orientations")
+    ap.add_argument("--workload", choices=["orientations", "trajectories", "dense", "uncoupled"],
+                    default="orientations")
     ap.add_argument("--orientations", type=int, default=32)
     ap.add_argument("--trajectories", type=int, default=512)
     ap.add_argument("--intervals", type=int, default=None, help="time steps per bench step")
@@ -61,6 +62,8 @@ def main():
         run = orientation_batch(ctx, args)
     elif args.workload == "trajectories":
         run = trajectory_batch(ctx, args)
+    elif args.workload == "uncoupled":
+        run = uncoupled_batch(ctx, args)
     else:
         run = dense_sweep(ctx, args)
 
@@ -193,6 +196,42 @@ def trajectory_batch(ctx, args):
                         "trajectories": M, "intervals": intervals, "parallelism": "single GPU",
                         "launch_shape": {"groups_per_thread": shape.groups_per_thread, "threads": shape.threads,
                                          "slots": shape.slots, "chosen_by": "msg_autotune"}})
+
+
+def uncoupled_batch(ctx, args):
+    """The orientation batch through the exact uncoupled-radical method (radicals of
+    dimension 128 and 32): all 1024 orientations over the full 0-3000 ns grid per
+    step; state-steps counted as for the Lanczos form (1024 columns per orientation)."""
+    from molspin_amd import abi, workloads
+    sysobj, cfg = workloads.config2(n_orientations=1024)
+    packed = sysobj.pack()
+    sysd = ctx.upload(packed)
+    D, dt = sysd.dim, cfg["time_step"]
+    intervals = args.intervals or int(round(cfg["total_time"] / dt))
+    prm = abi.run_params(total_time=intervals * dt, time_step=dt, method=abi.MSG_METHOD_UNCOUPLED)
+    B = 1024
+    ncols = sysd.direct_columns()
+    state_steps = B * ncols * intervals
+    ns = intervals + 1
+    flops = B * (128 * 128 + 32 * 32) * ns * 24.0  # correlation sums: 9 products + 1 phase update per pair and sample
+
+    def step(i):
+        sysd.propagate(prm, fields=cfg["fields"])
+
+    def roof(kernel_s):
+        return {"bound": "fp64-valu", "achieved": flops / kernel_s / 1e12, "peak": 78.6, "unit": "TFLOP/s",
+                "frac": flops / kernel_s / 78.6e12, "traffic": None, "kernel_ms": kernel_s * 1e3,
+                "state_steps_per_launch": state_steps,
+                "note": "device time of the whole method (eigensolver, GEMMs, correlation and combination kernels)"}
+
+    return dict(step=step, state_steps=state_steps, roofline=roof, packed=packed,
+                cpu_prm=abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_KRYLOV),
+                cpu_field=cfg["fields"][0], cpu_columns=None,
+                data="synthetic (seeded radical pair, Fibonacci field directions)",
+                config={"workload": "orientation batch, exact uncoupled-radical method: D=4096 as radicals of 128 and "
+                                    "32, %d orientations x %d columns x %d steps of 2 ns per step" % (B, ncols, intervals),
+                        "orientations_per_step": B, "columns": ncols, "intervals": intervals,
+                        "parallelism": "single GPU"})
 
 
 def dense_sweep(ctx, args):

@@ -142,9 +142,15 @@ enum {
     MSG_METHOD_AUTO = 0,          /* device chooses: dense U for D<=512, matrix-free Krylov above */
     MSG_METHOD_KRYLOV = 1,        /* propagationmethod=krylov, fixed krylovsize                   */
     MSG_METHOD_EXPONENTIAL = 2,   /* propagationmethod=normal: exact exp(dt G) applied as a GEMM  */
-    MSG_METHOD_RK4 = 3            /* propagationmethod=rk4 (HSPropagator.cpp:80-90): for a static G
+    MSG_METHOD_RK4 = 3,           /* propagationmethod=rk4 (HSPropagator.cpp:80-90): for a static G
                                      one RK4 step is the degree-4 Taylor polynomial of dt G, applied
                                      as a GEMM (D <= 512)                                          */
+    MSG_METHOD_UNCOUPLED = 4      /* exact propagation of uncoupled radicals (no electron-electron
+                                     coupling, symmetric recombination, no relaxation, direct
+                                     sampling): H = H_A (x) 1 + 1 (x) H_B, the observables follow from
+                                     per-radical electron-spin correlation functions computed from
+                                     the radicals' eigenpairs (TaskStaticHSDirectYieldsSymmUncoupled
+                                     .cpp:150-470 restated exactly); radicals of dimension <= 512 */
 };
 enum {
     MSG_SAMPLING_DIRECT = 0,      /* factors of rho0 (HSStatePreparation.cpp:77-165)              */

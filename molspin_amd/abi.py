@@ -29,6 +29,7 @@ MSG_METHOD_AUTO = 0
 MSG_METHOD_KRYLOV = 1
 MSG_METHOD_EXPONENTIAL = 2
 MSG_METHOD_RK4 = 3
+MSG_METHOD_UNCOUPLED = 4
 MSG_SAMPLING_DIRECT = 0
 MSG_SAMPLING_GIVEN = 1
 MSG_TASK_HSGENERAL = 0

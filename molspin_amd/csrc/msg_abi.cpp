@@ -43,6 +43,12 @@ hipError_t launch_zlincomb(const void *X, const void *X2, const void *X3, void *
 hipError_t launch_blockdiag_right(const void *M, const double *B4, double scale, void *Y, int n, hipStream_t st);
 hipError_t launch_lorentz_sum(const void *Pt, const void *Rt, const double *lam, double k, int n, double *rows,
                               double *out, hipStream_t st);
+hipError_t launch_pauli_right(const void *M, void *Y, int n, int mu, int batch, hipStream_t st);
+hipError_t launch_corr(const void *S, size_t plane, const double *lam, const double *times, int n, int ns, int batch,
+                       double *C, hipStream_t st);
+hipError_t launch_unc_combine(const double *CA, const double *CB, const double *Q, const double *wy, const double *wt,
+                              int n_items, int ns, int nobs, double dimA, double dimB, double *yields, double *trace,
+                              hipStream_t st);
 hipError_t launch_to_planes(const void *U, float *dst, int n, int Dp, size_t stride, int batch, hipStream_t st);
 int dense_occupancy(int dpad, size_t lds);
 }  // namespace msg
@@ -159,8 +165,8 @@ void check_params(const msg_system *sys, const msg_run_params *p) {
     if (p->observables != MSG_OBS_TRANSITION_YIELDS && p->observables != MSG_OBS_STATE_POPULATIONS)
         throw ArgError{MSG_INVALID_ARG, "unknown observable mode"};
     if (p->method != MSG_METHOD_AUTO && p->method != MSG_METHOD_KRYLOV && p->method != MSG_METHOD_EXPONENTIAL &&
-        p->method != MSG_METHOD_RK4)
-        throw ArgError{MSG_INVALID_ARG, "propagationmethod must be auto, krylov, exponential or rk4 on the device"};
+        p->method != MSG_METHOD_RK4 && p->method != MSG_METHOD_UNCOUPLED)
+        throw ArgError{MSG_INVALID_ARG, "propagationmethod must be auto, krylov, exponential, rk4 or uncoupled"};
     if (p->krylov_size < 0 || p->krylov_size > kMaxKrylov) throw ArgError{MSG_UNSUPPORTED, "krylovsize above 32"};
     if (p->sampling != MSG_SAMPLING_DIRECT && p->sampling != MSG_SAMPLING_GIVEN)
         throw ArgError{MSG_INVALID_ARG, "unknown sampling mode"};
@@ -194,6 +200,8 @@ double kappa_for(const Resolved &r, const msg_run_params *p) {
 }
 
 Path choose_path(const Resolved &r, const msg_run_params *p) {
+    if (p->method == MSG_METHOD_UNCOUPLED)
+        throw ArgError{MSG_UNSUPPORTED, "propagationmethod=uncoupled is available through msg_propagate only"};
     const bool dense_ok = r.D <= 512;
     const bool lanczos_ok = r.fast_ok && sym_for(r, p) && r.D >= 256 && r.D <= (int64_t)4 << kMaxNuclei;
     if (p->method == MSG_METHOD_EXPONENTIAL || p->method == MSG_METHOD_RK4) {
@@ -770,12 +778,183 @@ msg_status msg_query_footprint(const msg_system *sys, const msg_run_params *p, i
     return MSG_OK;
 }
 
+namespace {
+
+// Pauli coefficients Tr[X (sigma_mu (x) sigma_nu)] of a 4x4 electron-pair matrix
+// (pair index 2a + b, electron 1 first; sigma_0 = 1, x, y, z), real for Hermitian X.
+void pauli_coefficients(const Mat4 &X, double out[16]) {
+    const cd I(0.0, 1.0);
+    const cd sg[4][4] = {{1.0, 0.0, 0.0, 1.0}, {0.0, 1.0, 1.0, 0.0}, {0.0, -I, I, 0.0}, {1.0, 0.0, 0.0, -1.0}};
+    for (int m = 0; m < 4; ++m)
+        for (int n = 0; n < 4; ++n) {
+            cd tr = 0.0;
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) {
+                    // (sigma_m (x) sigma_n)[j][i] with j = 2 a + b, i = 2 a' + b'
+                    const cd e = sg[m][(j >> 1) * 2 + (i >> 1)] * sg[n][(j & 1) * 2 + (i & 1)];
+                    tr += X[i * 4 + j] * e;
+                }
+            out[m * 4 + n] = tr.real();
+        }
+}
+
+// msg_propagate for MSG_METHOD_UNCOUPLED.
+void propagate_uncoupled(msg_ctx *ctx, const msg_system *sys, const msg_run_params *p, int n_items,
+                         const double *fields, double *yields, double *trace) {
+    const Resolved &r = sys->r;
+    check_params(sys, p);
+    std::vector<int> SA, SB;
+    const std::string why = uncoupled_split(r, SA, SB);
+    if (!why.empty()) throw ArgError{MSG_UNSUPPORTED, "uncoupled propagation: " + why};
+    if (p->sampling != MSG_SAMPLING_DIRECT) throw ArgError{MSG_UNSUPPORTED, "uncoupled propagation uses direct sampling"};
+    if (r.relax) throw ArgError{MSG_UNSUPPORTED, "relaxation couples the radicals"};
+    if (!sym_for(r, p)) throw ArgError{MSG_UNSUPPORTED, "uncoupled propagation needs symmetric recombination"};
+    if (r.evec.empty()) throw ArgError{MSG_INVALID_ARG, "the system defines no initial state"};
+    int nA = 1, nB = 1;
+    for (int sp : SA) nA *= r.mult[sp];
+    for (int sp : SB) nB *= r.mult[sp];
+    if (nA > 512 || nB > 512) throw ArgError{MSG_UNSUPPORTED, "uncoupled propagation: a radical above dimension 512"};
+    const int nobs = observable_count(r, p);
+    if (nobs < 1) throw ArgError{MSG_INVALID_ARG, "no observables (no transitions / States)"};
+    if (nobs > kObsMax) throw ArgError{MSG_UNSUPPORTED, "more than 8 observables"};
+    // time grid and weights, as for the other paths (decay factored out)
+    const bool legacy = p->task == MSG_TASK_LEGACY_DIRECT_YIELDS || p->task == MSG_TASK_LEGACY_STOCH_YIELDS;
+    if (!(p->task == MSG_TASK_HSGENERAL || legacy)) throw ArgError{MSG_INVALID_ARG, "unknown task"};
+    std::vector<double> times;
+    const std::string te = legacy ? legacy_times(p->total_time, p->time_step, times) : free_times(p->total_time, p->time_step, times);
+    if (!te.empty()) throw ArgError{MSG_INVALID_ARG, te};
+    const int ns = (int)times.size();
+    const double kappa = kappa_for(r, p);
+    const std::vector<Mat4> &P = p->observables == MSG_OBS_TRANSITION_YIELDS ? r.P_trans : r.P_states;
+    std::vector<double> scale(nobs, 1.0);
+    if (p->observables == MSG_OBS_TRANSITION_YIELDS) scale = r.rates;
+    std::vector<double> wyh((size_t)ns * nobs), wth((size_t)ns * nobs);
+    for (int s = 0; s < ns; ++s) {
+        double w = 0.0;
+        if (s > 0) w += 0.5 * (times[s] - times[s - 1]);
+        if (s + 1 < ns) w += 0.5 * (times[s + 1] - times[s]);
+        const double decay = std::exp(-2.0 * kappa * times[s]);
+        for (int c = 0; c < nobs; ++c) {
+            wyh[(size_t)s * nobs + c] = w * decay * scale[c];
+            wth[(size_t)s * nobs + c] = decay * scale[c];
+        }
+    }
+    // Q_c[mu alpha][nu beta] = p^c_{mu nu} r_{alpha beta} / (16 Z_A Z_B), rho_e = sum_i evec_i evec_i^dagger
+    Mat4 rho{};
+    for (const auto &v : r.evec)
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) rho[i * 4 + j] += v[i] * std::conj(v[j]);
+    double rc[16];
+    pauli_coefficients(rho, rc);
+    const double norm = 1.0 / (16.0 * (nA / 2) * (nB / 2));
+    std::vector<double> Q((size_t)nobs * 256);
+    for (int c = 0; c < nobs; ++c) {
+        double pc[16];
+        pauli_coefficients(P[c], pc);
+        for (int mu = 0; mu < 4; ++mu)
+            for (int al = 0; al < 4; ++al)
+                for (int nu = 0; nu < 4; ++nu)
+                    for (int be = 0; be < 4; ++be)
+                        Q[(size_t)c * 256 + (mu * 4 + al) * 16 + nu * 4 + be] = pc[mu * 4 + nu] * rc[al * 4 + be] * norm;
+    }
+    hipStream_t st = ctx->stream;
+    if (!ctx->blas) {
+        if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
+            throw ArgError{MSG_DEVICE_ERROR, "rocblas_create_handle failed"};
+    }
+    if (rocblas_set_stream(ctx->blas, st) != rocblas_status_success)
+        throw ArgError{MSG_DEVICE_ERROR, "rocblas_set_stream failed"};
+    DBuf dQ, dwy, dwt, dtimes, dy, dtr;
+    double *q = dQ.alloc<double>(Q.size(), st);
+    double *wy = dwy.alloc<double>(wyh.size(), st);
+    double *wt = dwt.alloc<double>(wth.size(), st);
+    double *tm = dtimes.alloc<double>(times.size(), st);
+    HIPCHECK(hipMemcpyAsync(q, Q.data(), Q.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(wy, wyh.data(), wyh.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(wt, wth.data(), wth.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(tm, times.data(), times.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    double *yo = dy.alloc<double>((size_t)n_items * nobs, st);
+    double *to = trace ? dtr.alloc<double>((size_t)n_items * ns * nobs, st) : nullptr;
+    // items in chunks bounding the FP64 workspace (5 n^2 complex per item and radical)
+    const int nmax = std::max(nA, nB);
+    const int chunk = std::max(1, std::min(n_items, (int)(((size_t)512 << 20) / ((size_t)5 * nmax * nmax * 16))));
+    DBuf hA, hB, yA, yB, sA, sB, lA, lB, eA, eB, iA, iB, cA, cB;
+    cd *dHA = hA.alloc<cd>((size_t)chunk * nA * nA, st), *dHB = hB.alloc<cd>((size_t)chunk * nB * nB, st);
+    cd *dYA = yA.alloc<cd>((size_t)chunk * nA * nA, st), *dYB = yB.alloc<cd>((size_t)chunk * nB * nB, st);
+    cd *dSA = sA.alloc<cd>((size_t)chunk * 3 * nA * nA, st), *dSB = sB.alloc<cd>((size_t)chunk * 3 * nB * nB, st);
+    double *dlA = lA.alloc<double>((size_t)chunk * nA, st), *dlB = lB.alloc<double>((size_t)chunk * nB, st);
+    double *deA = eA.alloc<double>((size_t)chunk * nA, st), *deB = eB.alloc<double>((size_t)chunk * nB, st);
+    rocblas_int *diA = iA.alloc<rocblas_int>(chunk, st), *diB = iB.alloc<rocblas_int>(chunk, st);
+    double *dCA = cA.alloc<double>((size_t)chunk * ns * 9, st), *dCB = cB.alloc<double>((size_t)chunk * ns * 9, st);
+    std::vector<cd> hostA((size_t)chunk * nA * nA), hostB((size_t)chunk * nB * nB), H;
+    std::vector<rocblas_int> info(2 * (size_t)chunk);
+    HIPCHECK(hipEventRecord(ctx->ev0, st));
+    for (int it0 = 0; it0 < n_items; it0 += chunk) {
+        const int nb = std::min(chunk, n_items - it0);
+        for (int q0 = 0; q0 < nb; ++q0) {
+            const double *f = fields ? fields + 3 * (size_t)(it0 + q0) : nullptr;
+            subsystem_hamiltonian(r, SA, f, H);
+            std::copy(H.begin(), H.end(), hostA.begin() + (size_t)q0 * nA * nA);
+            subsystem_hamiltonian(r, SB, f, H);
+            std::copy(H.begin(), H.end(), hostB.begin() + (size_t)q0 * nB * nB);
+        }
+        HIPCHECK(hipMemcpyAsync(dHA, hostA.data(), (size_t)nb * nA * nA * sizeof(cd), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(dHB, hostB.data(), (size_t)nb * nB * nB * sizeof(cd), hipMemcpyHostToDevice, st));
+        // eigenpairs (the row-major view of the solver's output is M = V^dagger, as in
+        // msg_symmetric_decay), then S_mu = M (sigma_mu (x) 1) M^dagger and the correlations
+        struct Rad { int n; cd *Hm, *Y, *S; double *lam, *off, *C; rocblas_int *inf; };
+        for (const Rad &rd : {Rad{nA, dHA, dYA, dSA, dlA, deA, dCA, diA}, Rad{nB, dHB, dYB, dSB, dlB, deB, dCB, diB}}) {
+            if (rocsolver_zheevd_strided_batched(ctx->blas, rocblas_evect_original, rocblas_fill_upper, rd.n,
+                                                 reinterpret_cast<rocblas_double_complex *>(rd.Hm), rd.n,
+                                                 (rocblas_stride)rd.n * rd.n, rd.lam, rd.n, rd.off, rd.n, rd.inf,
+                                                 nb) != rocblas_status_success)
+                throw ArgError{MSG_DEVICE_ERROR, "rocsolver_zheevd_strided_batched failed"};
+            const size_t plane = (size_t)nb * rd.n * rd.n;  // S_mu of all items of the chunk
+            for (int mu = 0; mu < 3; ++mu) {
+                HIPCHECK(launch_pauli_right(rd.Hm, rd.Y, rd.n, mu, nb, st));
+                HIPCHECK(launch_zgemm(rd.Y, rd.Hm, rd.S + mu * plane, rd.n, nb, 1.0, 0.0, st, true));
+            }
+            HIPCHECK(launch_corr(rd.S, plane, rd.lam, tm, rd.n, ns, nb, rd.C, st));
+        }
+        HIPCHECK(launch_unc_combine(dCA, dCB, q, wy, wt, nb, ns, nobs, (double)nA, (double)nB,
+                                    yo + (size_t)it0 * nobs, to ? to + (size_t)it0 * ns * nobs : nullptr, st));
+        HIPCHECK(hipMemcpyAsync(info.data(), diA, nb * sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(info.data() + chunk, diB, nb * sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));  // host buffers are refilled by the next chunk
+        for (int q0 = 0; q0 < nb; ++q0)
+            if (info[q0] != 0 || info[chunk + q0] != 0) throw ArgError{MSG_NUMERIC_BREAKDOWN, "eigensolver did not converge"};
+    }
+    HIPCHECK(hipEventRecord(ctx->ev1, st));
+    if (yields)
+        HIPCHECK(hipMemcpyAsync(yields, yo, (size_t)n_items * nobs * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (trace)
+        HIPCHECK(hipMemcpyAsync(trace, to, (size_t)n_items * ns * nobs * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (yields && p->calculation == MSG_CALC_YIELDS && p->yield_corrections) {
+        const double den = 1.0 - std::exp(-p->total_time * r.kmax);
+        if (!(den > 0.0) || !std::isfinite(den))
+            throw ArgError{MSG_INVALID_ARG, "yieldcorrections=true needs a positive total time and rate"};
+        for (size_t i = 0; i < (size_t)n_items * nobs; ++i) yields[i] /= den;
+    }
+}
+
+}  // namespace
+
 msg_status msg_propagate(msg_ctx *ctx, const msg_system *sys, const msg_run_params *params, int32_t n_items,
                          const double *fields, const double *columns, int32_t n_columns, double *yields,
                          double *trace) {
     if (!ctx || !sys) return fail(ctx, MSG_INVALID_ARG, "null context or system");
     return guarded(ctx, [&] {
         HIPCHECK(hipSetDevice(ctx->device));
+        if (params && params->method == MSG_METHOD_UNCOUPLED) {
+            if (n_items < 1) throw ArgError{MSG_INVALID_ARG, "n_items must be positive"};
+            propagate_uncoupled(ctx, sys, params, n_items, fields, yields, trace);
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            ctx->kernel_ms = ms;
+            ctx->launches = 1;
+            return;
+        }
         Batch b;
         prepare(b, ctx, sys, params, n_items, fields, columns, n_columns, trace != nullptr, false, ctx->stream);
         b.timed = true;

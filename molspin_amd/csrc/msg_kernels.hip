@@ -1195,6 +1195,145 @@ __global__ __launch_bounds__(256) void k_sum_rows(const double *__restrict__ row
     if (t == 0) out[0] = red[0];
 }
 
+// ---------------------------------------------------------------------------
+// Uncoupled radicals (H = H_A (x) 1 + 1 (x) H_B, no electron-electron coupling):
+// per radical, the electron-spin correlation functions
+//   C_{mu,alpha}(t) = Tr[(sigma_mu (x) 1) U(t) (sigma_alpha (x) 1) U(t)^dagger]
+// from the eigenpairs of the radical's Hamiltonian (M = V^dagger, row-major):
+//   S_mu = M (sigma_mu (x) 1) M^dagger,  C(t) = sum_gh S_mu[g,h] conj(S_alpha[g,h]) e^{i(l_g - l_h) t}.
+// ---------------------------------------------------------------------------
+
+// Y = M (sigma_mu (x) 1_Z) for every batch entry; basis index e * Z + nu (electron first).
+__global__ void k_pauli_right(const double2 *__restrict__ M, double2 *__restrict__ Y, int n, int mu, size_t total) {
+    const int Z = n >> 1;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t rowbase = idx - idx % n;
+        const int col = (int)(idx % n), e = col / Z, nu = col - e * Z;
+        const double2 m0 = M[rowbase + nu], m1 = M[rowbase + Z + nu];  // electron up / down columns
+        double2 v;
+        if (mu == 0) v = e == 0 ? m1 : m0;                                               // sigma_x
+        else if (mu == 1) v = e == 0 ? make_double2(-m1.y, m1.x) : make_double2(m0.y, -m0.x);  // sigma_y
+        else v = e == 0 ? m0 : make_double2(-m1.x, -m1.y);                               // sigma_z
+        Y[idx] = v;
+    }
+}
+
+constexpr int kCorrBlock = 64;  // samples per LDS block of k_corr
+
+// C[item][s][3 mu + alpha] for the samples `times[0..ns)`; one 1024-thread
+// workgroup per item, threads over the (g, h) pairs, phases advanced by
+// recurrence within a block of samples and re-anchored with sincos per block.
+// Partial sums per wavefront are kept apart and added in a fixed order.
+// S holds three planes (x, y, z) of `plane` elements each; item i's S_mu is at
+// S + mu * plane + i * n^2.
+__global__ __launch_bounds__(1024) void k_corr(const double2 *__restrict__ S, size_t plane, const double *__restrict__ lam,
+                                               const double *__restrict__ times, int n, int ns,
+                                               double *__restrict__ C) {
+    __shared__ double acc[16 * kCorrBlock * 9];
+    const int item = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
+    const size_t nn = (size_t)n * n;
+    const double2 *Sx = S + (size_t)item * nn, *Sy = Sx + plane, *Sz = Sy + plane;
+    const double *l = lam + (size_t)item * n;
+    const int K = n * n;
+    for (int s0 = 0; s0 < ns; s0 += kCorrBlock) {
+        const int nb = min(kCorrBlock, ns - s0);
+        for (int i = t; i < 16 * kCorrBlock * 9; i += blockDim.x) acc[i] = 0.0;
+        __syncthreads();
+        for (int kb = 0; kb < K; kb += blockDim.x) {
+            const int k = kb + t;
+            double2 F[9];
+            double w = 0.0;
+            if (k < K) {
+                const int g = k / n, h = k - g * n;
+                w = l[g] - l[h];
+                const double2 sv[3] = {Sx[k], Sy[k], Sz[k]};
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+#pragma unroll
+                    for (int q = 0; q < 3; ++q)  // S_m conj(S_q)
+                        F[3 * m + q] = make_double2(sv[m].x * sv[q].x + sv[m].y * sv[q].y,
+                                                    sv[m].y * sv[q].x - sv[m].x * sv[q].y);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) F[q] = make_double2(0.0, 0.0);
+            }
+            double ar, ai, zr = 1.0, zi = 0.0, zdt = -1.0;
+            sincos(w * times[s0], &ai, &ar);
+            for (int j = 0; j < nb; ++j) {
+                double v[9];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) v[q] = F[q].x * ar - F[q].y * ai;
+#pragma unroll
+                for (int q = 0; q < 9; ++q)
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off, 64);
+                if (lane == 0)
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) acc[(wave * kCorrBlock + j) * 9 + q] += v[q];
+                if (j + 1 < nb) {
+                    const double dtj = times[s0 + j + 1] - times[s0 + j];
+                    if (!(fabs(dtj - zdt) <= 1e-13 * fabs(dtj))) {
+                        sincos(w * dtj, &zi, &zr);
+                        zdt = dtj;
+                    }
+                    const double nr = ar * zr - ai * zi, ni = ar * zi + ai * zr;
+                    ar = nr;
+                    ai = ni;
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = t; i < nb * 9; i += blockDim.x) {
+            double sum = 0.0;
+            for (int wv = 0; wv < nw; ++wv) sum += acc[wv * kCorrBlock * 9 + i];
+            C[((size_t)item * ns + s0) * 9 + i] = sum;
+        }
+        __syncthreads();
+    }
+}
+
+// Observables of the uncoupled pair from the radicals' correlations:
+//   T_c(s) = sum_{mu nu alpha beta} Q_c[mu][alpha][nu][beta] CA_{mu alpha}(s) CB_{nu beta}(s)
+// with the identity rows/columns of C filled in (C_00 = dimension, C_0a = C_m0 = 0);
+// Q carries p^c_{mu nu} r_{alpha beta} / (16 Z_A Z_B) and the rate of observable c.
+// One thread per item sums the Kahan-compensated yields in sample order.
+__global__ void k_unc_combine(const double *__restrict__ CA, const double *__restrict__ CB, const double *__restrict__ Q,
+                              const double *__restrict__ wy, const double *__restrict__ wt, int n_items, int ns,
+                              int nobs, double dimA, double dimB, double *__restrict__ yields, double *__restrict__ trace) {
+    const int item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= n_items) return;
+    double ys[kMaxObs], yc[kMaxObs];
+    for (int c = 0; c < nobs; ++c) ys[c] = yc[c] = 0.0;
+    for (int s = 0; s < ns; ++s) {
+        double a[16], b[16];
+        const double *ca = CA + ((size_t)item * ns + s) * 9, *cb = CB + ((size_t)item * ns + s) * 9;
+        for (int i = 0; i < 16; ++i) a[i] = b[i] = 0.0;
+        a[0] = dimA;
+        b[0] = dimB;
+        for (int m = 0; m < 3; ++m)
+            for (int q = 0; q < 3; ++q) {
+                a[(m + 1) * 4 + q + 1] = ca[3 * m + q];
+                b[(m + 1) * 4 + q + 1] = cb[3 * m + q];
+            }
+        for (int c = 0; c < nobs; ++c) {
+            const double *Qc = Q + (size_t)c * 256;
+            double v = 0.0;
+            for (int ma = 0; ma < 16; ++ma) {
+                if (a[ma] == 0.0) continue;
+                double inner = 0.0;
+                for (int nb2 = 0; nb2 < 16; ++nb2) inner += Qc[ma * 16 + nb2] * b[nb2];
+                v += a[ma] * inner;
+            }
+            const double term = wy[(size_t)s * nobs + c] * v - yc[c];
+            const double nsum = ys[c] + term;
+            yc[c] = (nsum - ys[c]) - term;
+            ys[c] = nsum;
+            if (trace) trace[((size_t)item * ns + s) * nobs + c] = wt[(size_t)s * nobs + c] * v;
+        }
+    }
+    for (int c = 0; c < nobs; ++c) yields[(size_t)item * nobs + c] = ys[c];
+}
+
 // Planes of the dense-path operand: U[b] row-major n x n (double2) ->
 // dst[b*stride + k*Dp + row] = re, dst[b*stride + Dp*Dp + k*Dp + row] = im.
 __global__ void k_to_planes(const double2 *__restrict__ U, float *__restrict__ dst, int n, int Dp, size_t stride) {
@@ -1321,6 +1460,27 @@ hipError_t launch_blockdiag_right(const void *M, const double *B4, double scale,
     const size_t total = (size_t)n * n;
     const int blocks = (int)std::min<size_t>((total + 255) / 256, 4096);
     hipLaunchKernelGGL(k_blockdiag_right, dim3(blocks), dim3(256), 0, st, (const double2 *)M, b, scale, (double2 *)Y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pauli_right(const void *M, void *Y, int n, int mu, int batch, hipStream_t st) {
+    const size_t total = (size_t)batch * n * n;
+    const int blocks = (int)std::min<size_t>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_pauli_right, dim3(blocks), dim3(256), 0, st, (const double2 *)M, (double2 *)Y, n, mu, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_corr(const void *S, size_t plane, const double *lam, const double *times, int n, int ns, int batch,
+                       double *C, hipStream_t st) {
+    hipLaunchKernelGGL(k_corr, dim3(batch), dim3(1024), 0, st, (const double2 *)S, plane, lam, times, n, ns, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_unc_combine(const double *CA, const double *CB, const double *Q, const double *wy, const double *wt,
+                              int n_items, int ns, int nobs, double dimA, double dimB, double *yields, double *trace,
+                              hipStream_t st) {
+    hipLaunchKernelGGL(k_unc_combine, dim3((n_items + 63) / 64), dim3(64), 0, st, CA, CB, Q, wy, wt, n_items, ns, nobs,
+                       dimA, dimB, yields, trace);
     return hipGetLastError();
 }
 

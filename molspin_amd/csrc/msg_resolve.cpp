@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <cstring>
 
 namespace msg {
@@ -616,6 +617,84 @@ void dense_hamiltonian(const Resolved &r, const double *field, std::vector<cd> &
                             if (y == 0.0) continue;
                             add(row, row + (int64_t)(c1 - d1) * stride[t.s1] + (int64_t)(c2 - d2) * stride[t.s2],
                                 mab * x * y);
+                        }
+                    }
+                }
+        }
+    }
+}
+
+std::string uncoupled_split(const Resolved &r, std::vector<int> &A, std::vector<int> &B) {
+    if (!r.pair_ok) return "needs two spin-1/2 electrons with States on them";
+    if (r.has_ee) return "the electrons are coupled (exchange or electron-electron terms)";
+    // connected components of the two-spin coupling graph
+    std::vector<int> comp(r.nspins);
+    for (int i = 0; i < r.nspins; ++i) comp[i] = i;
+    std::function<int(int)> find = [&](int x) { return comp[x] == x ? x : comp[x] = find(comp[x]); };
+    for (const Term2 &t : r.t2) comp[find(t.s1)] = find(t.s2);
+    const int ca = find(r.e1), cb = find(r.e2);
+    if (ca == cb) return "a chain of couplings links the two electrons";
+    A.assign(1, r.e1);
+    B.assign(1, r.e2);
+    for (int i = 0; i < r.nspins; ++i) {
+        if (i == r.e1 || i == r.e2) continue;
+        (find(i) == cb ? B : A).push_back(i);
+    }
+    return "";
+}
+
+void subsystem_hamiltonian(const Resolved &r, const std::vector<int> &spins, const double *field,
+                           std::vector<cd> &H) {
+    const int ns = (int)spins.size();
+    std::vector<int> pos(r.nspins, -1);
+    for (int i = 0; i < ns; ++i) pos[spins[i]] = i;
+    std::vector<int64_t> stride(ns);
+    int64_t n = 1;
+    for (int i = ns - 1; i >= 0; --i) { stride[i] = n; n *= r.mult[spins[i]]; }
+    H.assign((size_t)(n * n), 0.0);
+    std::vector<std::vector<cd>> ops((size_t)17 * 3);
+    auto op = [&](int mult, int axis) -> const std::vector<cd> & {
+        const size_t key = (size_t)mult * 3 + axis;
+        if (ops[key].empty()) ops[key] = spin_op(mult, axis);
+        return ops[key];
+    };
+    auto digit = [&](int64_t idx, int i) { return (int)((idx / stride[i]) % r.mult[spins[i]]); };
+    for (int64_t row = 0; row < n; ++row) {
+        for (const Term1 &t : r.t1) {
+            const int i = pos[t.s];
+            if (i < 0) continue;
+            double v[3];
+            for (int a = 0; a < 3; ++a)
+                v[a] = (t.swept && field) ? t.Mz[3 * a] * field[0] + t.Mz[3 * a + 1] * field[1] + t.Mz[3 * a + 2] * field[2]
+                                          : t.c[a];
+            const int m = r.mult[t.s], dr = digit(row, i);
+            for (int a = 0; a < 3; ++a) {
+                if (v[a] == 0.0) continue;
+                const std::vector<cd> &S = op(m, a);
+                for (int dc = 0; dc < m; ++dc) {
+                    const cd x = S[(size_t)dr * m + dc];
+                    if (x != 0.0) H[(size_t)(row * n + row + (int64_t)(dc - dr) * stride[i])] += v[a] * x;
+                }
+            }
+        }
+        for (const Term2 &t : r.t2) {
+            const int i1 = pos[t.s1], i2 = pos[t.s2];
+            if (i1 < 0 || i2 < 0) continue;
+            const int m1 = r.mult[t.s1], m2 = r.mult[t.s2];
+            const int d1 = digit(row, i1), d2 = digit(row, i2);
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    const double mab = t.M[3 * a + b];
+                    if (mab == 0.0) continue;
+                    const std::vector<cd> &Sa = op(m1, a), &Sb = op(m2, b);
+                    for (int c1 = 0; c1 < m1; ++c1) {
+                        const cd x = Sa[(size_t)d1 * m1 + c1];
+                        if (x == 0.0) continue;
+                        for (int c2 = 0; c2 < m2; ++c2) {
+                            const cd y = Sb[(size_t)d2 * m2 + c2];
+                            if (y == 0.0) continue;
+                            H[(size_t)(row * n + row + (int64_t)(c1 - d1) * stride[i1] + (int64_t)(c2 - d2) * stride[i2])] +=
+                                mab * x * y;
                         }
                     }
                 }

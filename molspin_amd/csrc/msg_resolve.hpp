@@ -75,6 +75,18 @@ void fast_coefficients(const Resolved &r, const double *field, std::vector<float
 // Dense D x D Hamiltonian (rad/ns) in device order, row-major complex.
 void dense_hamiltonian(const Resolved &r, const double *field, std::vector<cd> &H);
 
+// Split of a radical pair without electron-electron couplings into the two
+// uncoupled radicals (TaskStaticHSDirectYieldsSymmUncoupled.cpp:100-150): A = e1
+// plus every spin coupled to it, B = e2 plus every spin coupled to it; spins coupled
+// to neither go to A.  Within each list the electron comes first.  Returns an empty
+// string on success, else why the Hamiltonian does not factor as H_A (x) 1 + 1 (x) H_B.
+std::string uncoupled_split(const Resolved &r, std::vector<int> &A, std::vector<int> &B);
+
+// Dense Hamiltonian (rad/ns, row-major) of the spins in `spins` (first spin most
+// significant): the one-spin terms of those spins and the two-spin terms among them.
+void subsystem_hamiltonian(const Resolved &r, const std::vector<int> &spins, const double *field,
+                           std::vector<cd> &H);
+
 // exp(A) for a dense complex matrix (row-major), Pade(13) with scaling and
 // squaring (Higham 2005).
 void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E);

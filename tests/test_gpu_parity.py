@@ -423,3 +423,57 @@ def test_rk4_matches_oracle(ctx, singlet_only):
     _, tro = ref.run_hsgeneral(p, prm_t, trace=True, threads=8)
     for c in range(tr.shape[2]):
         assert trace_err(tr[0, :, c], tro[0, :, c]) <= TRACE_TOL, c
+
+
+def test_uncoupled_config1_matches_oracle(ctx):
+    """propagationmethod=uncoupled (H = H_A (x) 1 + 1 (x) H_B, correlation functions
+    from the radicals' eigenpairs): config 1 (radical B holds a spin-1 nucleus)
+    yields over 0-5000 ns and the four State traces against the oracle's exact
+    exponential propagation."""
+    s, cfg = workloads.config1()
+    p = s.pack()
+    sysd = ctx.upload(p)
+    prm = abi.run_params(total_time=cfg["total_time"], time_step=cfg["time_step"], method=abi.MSG_METHOD_UNCOUPLED)
+    y, _ = sysd.propagate(prm)
+    yo, _ = ref.run_hsgeneral(p, abi.run_params(total_time=cfg["total_time"], time_step=cfg["time_step"],
+                                                method=abi.MSG_METHOD_KRYLOV), threads=8)
+    print("uncoupled cfg1", y[0], "oracle", yo[0])
+    assert np.abs(y - yo).max() <= YIELD_TOL
+    fx = load_fixture("cfg1")
+    prm_t = abi.run_params(calculation=abi.MSG_CALC_TIMEEVOLUTION, observables=abi.MSG_OBS_STATE_POPULATIONS,
+                           total_time=fx["total_time"], time_step=fx["time_step"], method=abi.MSG_METHOD_UNCOUPLED)
+    _, tr = sysd.propagate(prm_t, trace=True)
+    want = np.array(fx["trace"]).T
+    for c in range(4):
+        assert trace_err(tr[0, :, c], want[:, c]) <= TRACE_TOL, c
+
+
+def test_uncoupled_config2_against_fixture(ctx):
+    """Config 2 (D = 4096 as radicals of 128 and 32): the 16 fixture orientations."""
+    fx = load_fixture("cfg2_subset")
+    s, cfg = workloads.config2(n_orientations=1024)
+    sysd = ctx.upload(s.pack())
+    prm = abi.run_params(total_time=fx["total_time"], time_step=fx["time_step"], method=abi.MSG_METHOD_UNCOUPLED)
+    y, _ = sysd.propagate(prm, fields=[cfg["fields"][i] for i in fx["orientations"]])
+    err = np.abs(y - np.array(fx["yields"])).max()
+    print("uncoupled cfg2 max |dPhi|", err)
+    assert err <= YIELD_TOL
+
+
+def test_uncoupled_rejects_coupled_electrons(ctx):
+    from molspin_amd import backend
+    s = SpinSystem("ex")
+    s.add_spin("E1", "1/2", electron=True, tensor=2.0023)
+    s.add_spin("E2", "1/2", electron=True, tensor=2.0023)
+    s.add_spin("N", "1/2", tensor=1.0)
+    s.add_hyperfine(["E1"], ["N"], tensor=1.0, prefactor=0.001)
+    s.add_exchange(["E1"], ["E2"], 0.2, prefactor=0.001)
+    s.add_zeeman(["E1", "E2"], (0.0, 0.0, 0.05), prefactor=0.001)
+    s.add_radical_pair_states("E1", "E2")
+    for st in ("S", "T0", "Tp", "Tm"):
+        s.add_transition("k" + st, st, 1e-3)
+    s.set_initial(["S"])
+    sysd = ctx.upload(s.pack())
+    with pytest.raises(backend.BackendError) as e:
+        sysd.propagate(abi.run_params(total_time=10.0, time_step=1.0, method=abi.MSG_METHOD_UNCOUPLED))
+    assert e.value.status == abi.MSG_UNSUPPORTED
