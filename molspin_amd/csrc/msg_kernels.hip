@@ -1218,78 +1218,71 @@ __global__ void k_pauli_right(const double2 *__restrict__ M, double2 *__restrict
     }
 }
 
-constexpr int kCorrBlock = 64;  // samples per LDS block of k_corr
+constexpr int kCorrSamples = 4;  // samples per lane in k_corr
 
-// C[item][s][3 mu + alpha] for the samples `times[0..ns)`; one 1024-thread
-// workgroup per item, threads over the (g, h) pairs, phases advanced by
-// recurrence within a block of samples and re-anchored with sincos per block.
-// Partial sums per wavefront are kept apart and added in a fixed order.
-// S holds three planes (x, y, z) of `plane` elements each; item i's S_mu is at
-// S + mu * plane + i * n^2.
-__global__ __launch_bounds__(1024) void k_corr(const double2 *__restrict__ S, size_t plane, const double *__restrict__ lam,
-                                               const double *__restrict__ times, int n, int ns,
-                                               double *__restrict__ C) {
-    __shared__ double acc[16 * kCorrBlock * 9];
-    const int item = blockIdx.x, t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
+// C[item][s][3 mu + alpha] for the samples times[0..ns): every lane owns
+// kCorrSamples consecutive samples and runs over the pairs g <= h in a fixed order
+// (the pair (h, g) adds the conjugate of (g, h), so off-diagonal pairs count
+// twice); the pair data are the same in every lane.  Phases: e^{i w t} of the
+// lane's first sample by sincos, then the step factor e^{i w dt} (recomputed
+// when a step differs from the uniform one).  Grid: (sample blocks of
+// 256 kCorrSamples, items); no cross-lane traffic, deterministic sums.
+__global__ __launch_bounds__(256) void k_corr(const double2 *__restrict__ S, size_t plane, const double *__restrict__ lam,
+                                              const double *__restrict__ times, int n, int ns,
+                                              double *__restrict__ C) {
+    const int item = blockIdx.y;
+    const int s_first = (blockIdx.x * blockDim.x + threadIdx.x) * kCorrSamples;
     const size_t nn = (size_t)n * n;
     const double2 *Sx = S + (size_t)item * nn, *Sy = Sx + plane, *Sz = Sy + plane;
     const double *l = lam + (size_t)item * n;
-    const int K = n * n;
-    for (int s0 = 0; s0 < ns; s0 += kCorrBlock) {
-        const int nb = min(kCorrBlock, ns - s0);
-        for (int i = t; i < 16 * kCorrBlock * 9; i += blockDim.x) acc[i] = 0.0;
-        __syncthreads();
-        for (int kb = 0; kb < K; kb += blockDim.x) {
-            const int k = kb + t;
+    double t_s[kCorrSamples];
+#pragma unroll
+    for (int j = 0; j < kCorrSamples; ++j) t_s[j] = times[min(s_first + j, ns - 1)];
+    const double dt0 = ns > 1 ? times[1] - times[0] : 0.0;
+    double acc[kCorrSamples][9];
+#pragma unroll
+    for (int j = 0; j < kCorrSamples; ++j)
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc[j][q] = 0.0;
+    for (int g = 0; g < n; ++g) {
+        const double lg = l[g];
+        for (int h = g; h < n; ++h) {
+            const size_t k = (size_t)g * n + h;
+            const double wt2 = h > g ? 2.0 : 1.0;
+            const double2 sv[3] = {Sx[k], Sy[k], Sz[k]};
             double2 F[9];
-            double w = 0.0;
-            if (k < K) {
-                const int g = k / n, h = k - g * n;
-                w = l[g] - l[h];
-                const double2 sv[3] = {Sx[k], Sy[k], Sz[k]};
 #pragma unroll
-                for (int m = 0; m < 3; ++m)
+            for (int m = 0; m < 3; ++m)
 #pragma unroll
-                    for (int q = 0; q < 3; ++q)  // S_m conj(S_q)
-                        F[3 * m + q] = make_double2(sv[m].x * sv[q].x + sv[m].y * sv[q].y,
-                                                    sv[m].y * sv[q].x - sv[m].x * sv[q].y);
-            } else {
+                for (int q = 0; q < 3; ++q)  // S_m conj(S_q)
+                    F[3 * m + q] = make_double2(wt2 * (sv[m].x * sv[q].x + sv[m].y * sv[q].y),
+                                                wt2 * (sv[m].y * sv[q].x - sv[m].x * sv[q].y));
+            const double w = lg - l[h];
+            double ar, ai, zr, zi;
+            sincos(w * t_s[0], &ai, &ar);
+            sincos(w * dt0, &zi, &zr);
 #pragma unroll
-                for (int q = 0; q < 9; ++q) F[q] = make_double2(0.0, 0.0);
-            }
-            double ar, ai, zr = 1.0, zi = 0.0, zdt = -1.0;
-            sincos(w * times[s0], &ai, &ar);
-            for (int j = 0; j < nb; ++j) {
-                double v[9];
+            for (int j = 0; j < kCorrSamples; ++j) {
 #pragma unroll
-                for (int q = 0; q < 9; ++q) v[q] = F[q].x * ar - F[q].y * ai;
-#pragma unroll
-                for (int q = 0; q < 9; ++q)
-#pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off, 64);
-                if (lane == 0)
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) acc[(wave * kCorrBlock + j) * 9 + q] += v[q];
-                if (j + 1 < nb) {
-                    const double dtj = times[s0 + j + 1] - times[s0 + j];
-                    if (!(fabs(dtj - zdt) <= 1e-13 * fabs(dtj))) {
-                        sincos(w * dtj, &zi, &zr);
-                        zdt = dtj;
+                for (int q = 0; q < 9; ++q) acc[j][q] = fma(F[q].x, ar, fma(-F[q].y, ai, acc[j][q]));
+                if (j + 1 < kCorrSamples) {
+                    const double dtj = t_s[j + 1] - t_s[j];
+                    if (dtj == dt0) {
+                        const double nr = ar * zr - ai * zi, ni = ar * zi + ai * zr;
+                        ar = nr;
+                        ai = ni;
+                    } else {
+                        sincos(w * t_s[j + 1], &ai, &ar);
                     }
-                    const double nr = ar * zr - ai * zi, ni = ar * zi + ai * zr;
-                    ar = nr;
-                    ai = ni;
                 }
             }
         }
-        __syncthreads();
-        for (int i = t; i < nb * 9; i += blockDim.x) {
-            double sum = 0.0;
-            for (int wv = 0; wv < nw; ++wv) sum += acc[wv * kCorrBlock * 9 + i];
-            C[((size_t)item * ns + s0) * 9 + i] = sum;
-        }
-        __syncthreads();
     }
+#pragma unroll
+    for (int j = 0; j < kCorrSamples; ++j)
+        if (s_first + j < ns)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) C[((size_t)item * ns + s_first + j) * 9 + q] = acc[j][q];
 }
 
 // Observables of the uncoupled pair from the radicals' correlations:
@@ -1472,7 +1465,9 @@ hipError_t launch_pauli_right(const void *M, void *Y, int n, int mu, int batch, 
 
 hipError_t launch_corr(const void *S, size_t plane, const double *lam, const double *times, int n, int ns, int batch,
                        double *C, hipStream_t st) {
-    hipLaunchKernelGGL(k_corr, dim3(batch), dim3(1024), 0, st, (const double2 *)S, plane, lam, times, n, ns, C);
+    const int per_block = 256 * kCorrSamples;
+    hipLaunchKernelGGL(k_corr, dim3((ns + per_block - 1) / per_block, batch), dim3(256), 0, st, (const double2 *)S, plane,
+                       lam, times, n, ns, C);
     return hipGetLastError();
 }
 

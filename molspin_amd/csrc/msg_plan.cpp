@@ -285,6 +285,9 @@ void resolve(const Props &p, msg_plan &out) {
     } else if (one_of(v, {"rk4", "explicit"})) {
         r.method = MSG_METHOD_RK4;
         out.propagation = MSG_PROP_RK4;
+    } else if (v == "uncoupled") {  // this backend's exact uncoupled-radical propagator
+        r.method = MSG_METHOD_UNCOUPLED;
+        out.propagation = MSG_PROP_EXPONENTIAL;
     } else {
         throw invalid("propagationmethod must be normal/exp, autoexpm, krylov, or rk4");
     }

@@ -96,6 +96,8 @@ def test_reference_rejections():
 
 
 def test_valid_but_off_the_device_path():
+    p = resolve("type=HSGeneral;propagationmethod=uncoupled;calculation=yields;")
+    assert p.params.method == abi.MSG_METHOD_UNCOUPLED
     p = resolve("type=HSGeneral;propagationmethod=rk4;")
     assert p.params.method == abi.MSG_METHOD_RK4 and p.propagation == abi.MSG_PROP_RK4
     for text in ("dynamics=dynamic;calculation=yields;",
