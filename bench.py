@@ -87,6 +87,9 @@ def main():
     serving = None
     if not args.no_serving and args.workload == "orientations":
         serving = serve(ctx, args.requests, args.serve_intervals)
+        # the same request stream through the exact uncoupled-radical method
+        serving["uncoupled_method"] = serve(ctx, args.requests, args.serve_intervals, method="uncoupled",
+                                            compare_plain=False)
 
     line = {
         "metric": METRIC,
@@ -269,7 +272,7 @@ def dense_sweep(ctx, args):
                         "fields": nf, "columns": ncols, "intervals": intervals, "parallelism": "single GPU"})
 
 
-def serve(ctx, n_requests, intervals):
+def serve(ctx, n_requests, intervals, method="auto", compare_plain=True):
     """Serving loop: a seeded Poisson stream (mean 200 requests/s) of radical-pair
     field-sweep yield requests (n ~ U{2..12} nuclei, 16-128 fields each), each
     propagated over `intervals` steps of 2 ns; requests are admitted by HBM
@@ -288,7 +291,8 @@ def serve(ctx, n_requests, intervals):
         t_arr += gap
         packed = s.pack()
         sysd = ctx.upload(packed)
-        prm = abi.run_params(total_time=intervals * p["time_step"], time_step=p["time_step"])
+        prm = abi.run_params(total_time=intervals * p["time_step"], time_step=p["time_step"],
+                             method=abi.MSG_METHOD_UNCOUPLED if method == "uncoupled" else abi.MSG_METHOD_AUTO)
         reqs.append(dict(arrival=t_arr, sys=sysd, packed=packed, prm=prm, fields=p["fields"],
                          steps=sysd.direct_columns() * len(p["fields"]) * intervals))
     srv = backend.Server(ctx, max_slots=8, hbm_budget_bytes=64 << 30)
@@ -316,6 +320,10 @@ def serve(ctx, n_requests, intervals):
     srv.close()
     lat = np.array(lat)
     total = sum(r["steps"] for r in reqs)
+    if not compare_plain:
+        return {"requests": n_requests, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_ms": float(np.percentile(lat, 99) * 1e3), "state_steps_per_s": total / elapsed,
+                "elapsed_s": elapsed, "method": method}
     plain_s = 0.0
     for r in reqs:
         r["sys"].propagate(r["prm"], fields=[r["fields"][0]])  # first call per system uploads factors

@@ -359,7 +359,8 @@ msg_status msg_server_submit(msg_server *srv, const msg_system *sys, const msg_r
                              int32_t n_fields, const double *fields, int64_t *request_id);
 /* Collects finished requests (without blocking), then admits queued requests
  * while a slot and the HBM budget allow, shortest first (state-steps x D; arrival
- * order among equals).  max_steps <= 0: every admitted request
+ * order among equals).  A request with method MSG_METHOD_UNCOUPLED is computed
+ * within the call (at most one per call).  max_steps <= 0: every admitted request
  * is propagated over its whole grid in one persistent launch on its own stream,
  * so requests overlap on the device.  max_steps > 0: resident requests advance
  * in chunks of max_steps intervals per call, their states kept on the device.

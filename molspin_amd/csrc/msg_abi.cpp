@@ -754,6 +754,19 @@ msg_status msg_query_footprint(const msg_system *sys, const msg_run_params *p, i
     const int nobs = std::max(1, observable_count(r, p));
     const int64_t D = r.D;
     int64_t b = (int64_t)t.size() * nobs * 16 + (int64_t)t.size() * 12;
+    if (p->method == MSG_METHOD_UNCOUPLED) {  // per radical: 5 n^2 FP64 complex per item of a chunk + correlations
+        std::vector<int> SA, SB;
+        if (!uncoupled_split(r, SA, SB).empty()) return MSG_UNSUPPORTED;
+        int64_t nA = 1, nB = 1;
+        for (int sp : SA) nA *= r.mult[sp];
+        for (int sp : SB) nB *= r.mult[sp];
+        const int64_t nmax = std::max(nA, nB);
+        const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(n_items, ((int64_t)512 << 20) / (5 * nmax * nmax * 16)));
+        b += chunk * 5 * (nA * nA + nB * nB) * 16 + chunk * 2 * (int64_t)t.size() * 9 * 8 +
+             (int64_t)n_items * nobs * 8;
+        *bytes = b;
+        return MSG_OK;
+    }
     Path path;
     try {
         path = choose_path(r, p);
@@ -1288,6 +1301,19 @@ msg_status msg_server_pump(msg_server *srv, int32_t max_steps, int32_t *n_active
             if (pick == srv->queued.end()) break;
             std::unique_ptr<Request> q = std::move(*pick);
             srv->queued.erase(pick);
+            if (q->prm.method == MSG_METHOD_UNCOUPLED) {
+                // exact uncoupled-radical propagation: a few short launches, completed here
+                q->result.assign((size_t)q->n_fields * std::max(1, observable_count(q->sys->r, &q->prm)), 0.0);
+                try {
+                    propagate_uncoupled(ctx, q->sys, &q->prm, q->n_fields, q->fields.data(), q->result.data(), nullptr);
+                } catch (const ArgError &e) {
+                    q->failed = true;
+                    q->error = e.what;
+                }
+                const int64_t id = q->id;
+                srv->finished[id] = std::move(q);
+                break;  // one synchronous request per call, so callers collect between them
+            }
             q->stream = slot;
             q->batch.reset(new Batch());
             q->whole = max_steps <= 0;

@@ -477,3 +477,32 @@ def test_uncoupled_rejects_coupled_electrons(ctx):
     with pytest.raises(backend.BackendError) as e:
         sysd.propagate(abi.run_params(total_time=10.0, time_step=1.0, method=abi.MSG_METHOD_UNCOUPLED))
     assert e.value.status == abi.MSG_UNSUPPORTED
+
+
+def test_server_uncoupled_requests(ctx):
+    """Server requests with propagationmethod=uncoupled, mixed with Krylov-path
+    requests: each result equals the same request run alone, and the uncoupled
+    yields match the oracle."""
+    from molspin_amd import backend
+    reqs = []
+    for idx in (4, 20):
+        s, cfg = workloads.config4_pair(idx)
+        p = s.pack()
+        reqs.append((p, ctx.upload(p), cfg["fields"][::32]))
+    pu = abi.run_params(total_time=40.0, time_step=2.0, method=abi.MSG_METHOD_UNCOUPLED)
+    pk = abi.run_params(total_time=40.0, time_step=2.0)
+    alone = [sd.propagate(pu, fields=f)[0] for _, sd, f in reqs]
+    srv = backend.Server(ctx, max_slots=2, hbm_budget_bytes=8 << 30)
+    ids = [srv.submit(reqs[0][1], pk, reqs[0][2]), srv.submit(reqs[0][1], pu, reqs[0][2]),
+           srv.submit(reqs[1][1], pu, reqs[1][2])]
+    active = 3
+    while active:
+        active = srv.pump(0)
+    got = [srv.collect(i) for i in ids]
+    srv.close()
+    assert np.abs(got[1] - alone[0]).max() <= 1e-12
+    assert np.abs(got[2] - alone[1]).max() <= 1e-12
+    assert np.abs(got[0] - got[1]).max() <= YIELD_TOL  # Krylov path and exact uncoupled agree
+    for (p, sd, f), a in zip(reqs, alone):
+        yo, _ = ref.run_hsgeneral(p, pk, fields=f, threads=8)
+        assert np.abs(a - yo).max() <= YIELD_TOL
