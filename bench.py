@@ -130,7 +130,7 @@ def lanczos_roofline(D, m, state_steps, kernel_s, traffic):
 # traffic of a launch scales with its state-steps at fixed D.  The counter runs
 # are summarised in profiles/k_lanczos_orientation_batch_D4096.md,
 # profiles/k_lanczos_big_trajectories_D65536.md and profiles/k_dense_field_sweep_D512.md.
-MEASURED_TRAFFIC_PER_STATE_STEP = {"orientations": 1.073e+06, "trajectories": 9.309e+07, "dense": 3.648e+04}
+MEASURED_TRAFFIC_PER_STATE_STEP = {"orientations": 1.066e+06, "trajectories": 7.523e+07, "dense": 3.25e+04}
 
 
 def orientation_batch(ctx, args):
