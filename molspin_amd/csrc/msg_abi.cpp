@@ -418,7 +418,6 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         // Paterson-Stockmeyer in X^4 (7 complex GEMMs), then s squarings; items are
         // processed in chunks to bound the FP64 workspace
         float *dU = b.U.alloc<float>((size_t)n_items * per * 2 * Dp * Dp, stream);
-        HIPCHECK(hipMemsetAsync(dU, 0, (size_t)n_items * per * 2 * Dp * Dp * sizeof(float), stream));
         const int chunk = std::max(1, std::min(n_items, (int)((size_t)128 << 20) / (D * D * 16)));
         DBuf X, P, Q, X2, X3, X4, Bk;
         cd *dX = X.alloc<cd>((size_t)chunk * D * D, stream);

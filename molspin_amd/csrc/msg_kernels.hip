@@ -997,19 +997,30 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
                 xr[k] = floatx4{0.f, 0.f, 0.f, 0.f};
                 xi[k] = floatx4{0.f, 0.f, 0.f, 0.f};
             }
-            for (int kk = 0; kk < ntile * 4; ++kk) {
-                const int kr = kk * 4 + q;
-                const float br = Bre[kr * 16 + cl], bi = Bim[kr * 16 + cl];
+            // k-groups of 16: one float4 per lane and plane carries four k-steps of A
+            for (int g = 0; g < ntile; ++g) {
+                float br[4], bi[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int kr = 16 * g + 4 * u + q;
+                    br[u] = Bre[kr * 16 + cl];
+                    bi[u] = Bim[kr * 16 + cl];
+                }
 #pragma unroll
                 for (int k = 0; k < KT; ++k) {
                     const int rt = wave + 4 * k;
                     if (rt < ntile) {
-                        const float ar = Ur[(size_t)kr * Dp + rt * 16 + cl];
-                        const float ai = Ui[(size_t)kr * Dp + rt * 16 + cl];
-                        xr[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, xr[k], 0, 0, 0);
-                        xr[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, bi, xr[k], 0, 0, 0);
-                        xi[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, xi[k], 0, 0, 0);
-                        xi[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, xi[k], 0, 0, 0);
+                        const size_t o = ((size_t)(rt * ntile + g) * 64 + lane) * 4;
+                        const float4 ar4 = *reinterpret_cast<const float4 *>(Ur + o);
+                        const float4 ai4 = *reinterpret_cast<const float4 *>(Ui + o);
+                        const float ar[4] = {ar4.x, ar4.y, ar4.z, ar4.w}, ai[4] = {ai4.x, ai4.y, ai4.z, ai4.w};
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            xr[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[u], br[u], xr[k], 0, 0, 0);
+                            xr[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai[u], bi[u], xr[k], 0, 0, 0);
+                            xi[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[u], bi[u], xi[k], 0, 0, 0);
+                            xi[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[u], br[u], xi[k], 0, 0, 0);
+                        }
                     }
                 }
             }
@@ -1329,13 +1340,21 @@ __global__ void k_unc_combine(const double *__restrict__ CA, const double *__res
 
 // Planes of the dense-path operand: U[b] row-major n x n (double2) ->
 // dst[b*stride + k*Dp + row] = re, dst[b*stride + Dp*Dp + k*Dp + row] = im.
+// The planes are stored in the MFMA operand order of k_dense: for row tile rt and
+// k-group g (16 columns), lane l = 16 q + i holds rows rt*16 + i and columns
+// 16 g + 4 u + q for u = 0..3 as one float4, so a lane fetches four k-steps of
+// its A operand with one 16-byte load (index ((rt * Dp/16 + g) * 64 + l) * 4 + u).
+// Padding rows and columns are written as zero.
 __global__ void k_to_planes(const double2 *__restrict__ U, float *__restrict__ dst, int n, int Dp, size_t stride) {
     const size_t b = blockIdx.y;
-    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n * n; idx += gridDim.x * blockDim.x) {
-        const int row = idx / n, k = idx - row * n;
-        const double2 v = U[b * n * n + idx];
-        dst[b * stride + (size_t)k * Dp + row] = (float)v.x;
-        dst[b * stride + (size_t)Dp * Dp + (size_t)k * Dp + row] = (float)v.y;
+    const int ng = Dp >> 4;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < Dp * Dp; idx += gridDim.x * blockDim.x) {
+        const int u = idx & 3, l = (idx >> 2) & 63, tg = idx >> 8;
+        const int rt = tg / ng, g = tg - rt * ng;
+        const int row = rt * 16 + (l & 15), col = 16 * g + 4 * u + (l >> 4);
+        const double2 v = (row < n && col < n) ? U[b * n * n + (size_t)row * n + col] : make_double2(0.0, 0.0);
+        dst[b * stride + idx] = (float)v.x;
+        dst[b * stride + (size_t)Dp * Dp + idx] = (float)v.y;
     }
 }
 
@@ -1488,7 +1507,7 @@ hipError_t launch_lorentz_sum(const void *Pt, const void *Rt, const double *lam,
 }
 
 hipError_t launch_to_planes(const void *U, float *dst, int n, int Dp, size_t stride, int batch, hipStream_t st) {
-    const int blocks = std::min(1024, (n * n + 255) / 256);
+    const int blocks = std::min(1024, (Dp * Dp + 255) / 256);
     hipLaunchKernelGGL(k_to_planes, dim3(blocks, batch), dim3(256), 0, st, (const double2 *)U, dst, n, Dp, stride);
     return hipGetLastError();
 }
