@@ -997,7 +997,16 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
                 xr[k] = floatx4{0.f, 0.f, 0.f, 0.f};
                 xi[k] = floatx4{0.f, 0.f, 0.f, 0.f};
             }
-            // k-groups of 16: one float4 per lane and plane carries four k-steps of A
+            // k-groups of 16: one float4 per lane and plane carries four k-steps of A;
+            // the A operands of group g + 1 are fetched while group g multiplies
+            float4 nr4[KT], ni4[KT];
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                const int rt = wave + 4 * k;
+                const size_t o = ((size_t)(rt * ntile) * 64 + lane) * 4;
+                nr4[k] = rt < ntile ? *reinterpret_cast<const float4 *>(Ur + o) : float4{0.f, 0.f, 0.f, 0.f};
+                ni4[k] = rt < ntile ? *reinterpret_cast<const float4 *>(Ui + o) : float4{0.f, 0.f, 0.f, 0.f};
+            }
             for (int g = 0; g < ntile; ++g) {
                 float br[4], bi[4];
 #pragma unroll
@@ -1006,13 +1015,23 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
                     br[u] = Bre[kr * 16 + cl];
                     bi[u] = Bim[kr * 16 + cl];
                 }
+                float4 cr4[KT], ci4[KT];
+#pragma unroll
+                for (int k = 0; k < KT; ++k) {
+                    cr4[k] = nr4[k];
+                    ci4[k] = ni4[k];
+                    const int rt = wave + 4 * k;
+                    if (rt < ntile && g + 1 < ntile) {
+                        const size_t o = ((size_t)(rt * ntile + g + 1) * 64 + lane) * 4;
+                        nr4[k] = *reinterpret_cast<const float4 *>(Ur + o);
+                        ni4[k] = *reinterpret_cast<const float4 *>(Ui + o);
+                    }
+                }
 #pragma unroll
                 for (int k = 0; k < KT; ++k) {
                     const int rt = wave + 4 * k;
                     if (rt < ntile) {
-                        const size_t o = ((size_t)(rt * ntile + g) * 64 + lane) * 4;
-                        const float4 ar4 = *reinterpret_cast<const float4 *>(Ur + o);
-                        const float4 ai4 = *reinterpret_cast<const float4 *>(Ui + o);
+                        const float4 ar4 = cr4[k], ai4 = ci4[k];
                         const float ar[4] = {ar4.x, ar4.y, ar4.z, ar4.w}, ai[4] = {ai4.x, ai4.y, ai4.z, ai4.w};
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
