@@ -682,7 +682,10 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
                         const float4 *pA = BA(j > 0 ? j - 1 : 0), *pB = pA + D4;
                         const float fb = (float)bprev;
                         double red[2] = {0.0, 0.0};
-                        auto pass_a = [&](int nu, const C4 &vj, const C4 &w0) {
+                        // LV (D = 16384, four groups per thread): the residual stays in
+                        // registers; otherwise it goes through the per-slot scratch W
+                        C4 wr[LV ? 4 : 1];
+                        auto pass_a = [&](int nu, const C4 &vj, const C4 &w0, C4 *keep) {
                             C4 w = w0;
                             if (j > 0) {
                                 const C4 vp = load_group(pA, pB, nu);
@@ -697,7 +700,8 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
                                 red[0] += (double)vj.c[c].x * w.c[c].x + (double)vj.c[c].y * w.c[c].y;
                                 red[1] += (double)w.c[c].x * w.c[c].x + (double)w.c[c].y * w.c[c].y;
                             }
-                            store_group(WA, WB, nu, w);
+                            if (keep) *keep = w;
+                            else store_group(WA, WB, nu, w);
                         };
                         if (TILED) {
                             float4 *tA = lvs, *tB = lvs + kTileG;
@@ -711,13 +715,20 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
                                 for (int k = t; k < kTileG; k += T) {
                                     const int nu = tb + k;
                                     const C4 vj = load_group(tA, tB, k);
-                                    pass_a(nu, vj, apply_h<-1, true>(a, co, nu, vj, tA, tB, vA, vB));
+                                    pass_a(nu, vj, apply_h<-1, true>(a, co, nu, vj, tA, tB, vA, vB), nullptr);
                                 }
+                            }
+                        } else if (LV) {
+#pragma unroll
+                            for (int g = 0; g < (LV ? 4 : 1); ++g) {
+                                const int nu = g * T + t;
+                                const C4 vj = load_group(vA, vB, nu);
+                                pass_a(nu, vj, apply_h(a, co, nu, vj, vA, vB), &wr[g]);
                             }
                         } else {
                             for (int nu = t; nu < ng; nu += T) {
                                 const C4 vj = load_group(vA, vB, nu);
-                                pass_a(nu, vj, apply_h(a, co, nu, vj, vA, vB));
+                                pass_a(nu, vj, apply_h(a, co, nu, vj, vA, vB), nullptr);
                             }
                         }
                         block_sum<2>(red, SCR());
@@ -728,14 +739,18 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
                         double beta2 = red[1] - alpha * alpha;
                         if (!(beta2 > 0.02 * red[1])) {
                             double bb[1] = {0.0};
-                            for (int nu = t; nu < ng; nu += T) {
-                                const C4 vj = load_group(vA, vB, nu);
-                                const C4 w = load_group(WA, WB, nu);
+                            auto acc_bb = [&](const C4 &vj, const C4 &w) {
 #pragma unroll
                                 for (int c = 0; c < 4; ++c) {
                                     const float wx = fmaf(-fa, vj.c[c].x, w.c[c].x), wy = fmaf(-fa, vj.c[c].y, w.c[c].y);
                                     bb[0] += (double)wx * wx + (double)wy * wy;
                                 }
+                            };
+                            if (LV) {
+#pragma unroll
+                                for (int g = 0; g < (LV ? 4 : 1); ++g) acc_bb(load_group(vA, vB, g * T + t), wr[g]);
+                            } else {
+                                for (int nu = t; nu < ng; nu += T) acc_bb(load_group(vA, vB, nu), load_group(WA, WB, nu));
                             }
                             block_sum<1>(bb, SCR());
                             beta2 = bb[0];
@@ -748,14 +763,19 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
                         if (lane == j) my_be = beta;
                         const float ib = (float)(1.0 / beta);
                         float4 *nA = BA(j + 1), *nB = nA + D4;
-                        for (int nu = t; nu < ng; nu += T) {
+                        auto pass_b = [&](int nu, C4 w) {
                             const C4 vj = load_group(vA, vB, nu);
-                            C4 w = load_group(WA, WB, nu);
 #pragma unroll
                             for (int c = 0; c < 4; ++c)
                                 w.c[c] = make_float2(fmaf(-fa, vj.c[c].x, w.c[c].x) * ib, fmaf(-fa, vj.c[c].y, w.c[c].y) * ib);
                             store_group(nA, nB, nu, w);
                             if (LV) store_group(lvs, lvs + D4, nu, w);  // own elements: read above by this thread only
+                        };
+                        if (LV) {
+#pragma unroll
+                            for (int g = 0; g < (LV ? 4 : 1); ++g) pass_b(g * T + t, wr[g]);
+                        } else {
+                            for (int nu = t; nu < ng; nu += T) pass_b(nu, load_group(WA, WB, nu));
                         }
                         bprev = beta;
                     }
