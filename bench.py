@@ -14,7 +14,10 @@ Other workloads (`--workload`), same JSON line format:
   trajectories  8 + 6 nuclei (D = 65536), `--trajectories` spin-coherent columns,
                 singlet-triplet dephasing 1e-4 /ns, dt = 1 ns (streaming Lanczos);
   dense         a 7-nucleus pair (D = 512) swept over 64 field strengths on the
-                dense MFMA path (FP64 propagator built once per field).
+                dense MFMA path (FP64 propagator built once per field);
+  pair          a 12-nucleus pair (D = 16384) at `--fields` field points x 4096
+                direct columns (Lanczos with the Krylov vector mirrored in LDS);
+  uncoupled     the orientation batch through the exact uncoupled-radical method.
 """
 import argparse
 import json
@@ -37,10 +40,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["orientations", "trajectories", "dense", "uncoupled"],
+    ap.add_argument("--workload", choices=["orientations", "trajectories", "dense", "pair", "uncoupled"],
                     default="orientations")
     ap.add_argument("--orientations", type=int, default=32)
     ap.add_argument("--trajectories", type=int, default=512)
+    ap.add_argument("--fields", type=int, default=2, help="field points of the pair workload")
     ap.add_argument("--intervals", type=int, default=None, help="time steps per bench step")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -64,6 +68,8 @@ def main():
         run = trajectory_batch(ctx, args)
     elif args.workload == "uncoupled":
         run = uncoupled_batch(ctx, args)
+    elif args.workload == "pair":
+        run = pair_batch(ctx, args)
     else:
         run = dense_sweep(ctx, args)
 
@@ -130,7 +136,8 @@ def lanczos_roofline(D, m, state_steps, kernel_s, traffic):
 # traffic of a launch scales with its state-steps at fixed D.  The counter runs
 # are summarised in profiles/k_lanczos_orientation_batch_D4096.md,
 # profiles/k_lanczos_big_trajectories_D65536.md and profiles/k_dense_field_sweep_D512.md.
-MEASURED_TRAFFIC_PER_STATE_STEP = {"orientations": 1.066e+06, "trajectories": 7.523e+07, "dense": 3.25e+04}
+MEASURED_TRAFFIC_PER_STATE_STEP = {"orientations": 1.066e+06, "trajectories": 7.523e+07, "dense": 2.51e+04,
+                                   "pair": 6.60e+06}
 
 
 def orientation_batch(ctx, args):
@@ -197,6 +204,42 @@ def trajectory_batch(ctx, args):
                                     "%d trajectories x %d steps of 1 ns, ST dephasing 1e-4/ns, Lanczos m=%d"
                                     % (M, intervals, m),
                         "trajectories": M, "intervals": intervals, "parallelism": "single GPU",
+                        "launch_shape": {"groups_per_thread": shape.groups_per_thread, "threads": shape.threads,
+                                         "slots": shape.slots, "chosen_by": "msg_autotune"}})
+
+
+def pair_batch(ctx, args):
+    """A D = 16384 radical pair (12 anisotropic spin-1/2 nuclei) of the serving mix
+    at `--fields` field points, all 4096 direct columns per field."""
+    from molspin_amd import abi, workloads
+    idx = next(i for i in range(1000) if workloads.config4_pair(i)[1]["n"] == 12)
+    sysobj, cfg = workloads.config4_pair(idx)
+    packed = sysobj.pack()
+    sysd = ctx.upload(packed)
+    D, m, dt = sysd.dim, cfg["krylov_size"], cfg["time_step"]
+    intervals = args.intervals or 10
+    prm = abi.run_params(total_time=intervals * dt, time_step=dt, method=abi.MSG_METHOD_KRYLOV, krylov_size=m)
+    fields = cfg["fields"][:args.fields]
+    ncols = sysd.direct_columns()
+    state_steps = len(fields) * ncols * intervals
+    shape = sysd.autotune(prm, fields=fields, sample_steps=2)
+
+    def step(i):
+        sysd.propagate(prm, fields=fields)
+
+    def roof(kernel_s):
+        tr = MEASURED_TRAFFIC_PER_STATE_STEP.get("pair")
+        return lanczos_roofline(D, m, state_steps, kernel_s, None if tr is None else tr * state_steps)
+
+    return dict(step=step, state_steps=state_steps, roofline=roof, packed=packed,
+                cpu_prm=abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_KRYLOV),
+                cpu_field=fields[0], cpu_columns=None,
+                data="synthetic (seeded radical pair, log-spaced field strengths)",
+                config={"workload": "radical pair D=16384 (12 spin-1/2 nuclei, anisotropic HF, pair %d), "
+                                    "%d fields x %d columns x %d steps of %g ns, Lanczos m=%d"
+                                    % (idx, len(fields), ncols, intervals, dt, m),
+                        "fields": len(fields), "columns": ncols, "intervals": intervals,
+                        "parallelism": "single GPU",
                         "launch_shape": {"groups_per_thread": shape.groups_per_thread, "threads": shape.threads,
                                          "slots": shape.slots, "chosen_by": "msg_autotune"}})
 
