@@ -197,7 +197,7 @@ typedef struct msg_plan {
     int32_t sampling_method;      /* samplingmethod: MSG_TRACE_SUZ (default) or MSG_TRACE_COHERENT */
     int32_t auto_seed;            /* autoseed (default true) */
     int32_t device_backend;       /* backend=mi355x (default cpu) */
-    int32_t time_infinity;        /* method=timeinf (calculation=yields) */
+    int32_t time_infinity;        /* method=timeinf (calculation=yields): msg_time_infinity */
     int32_t propagation;          /* MSG_PROP_* */
 } msg_plan;
 
@@ -301,6 +301,34 @@ msg_status msg_propagate(msg_ctx *ctx, const msg_system *sys, const msg_run_para
  */
 msg_status msg_symmetric_decay(msg_ctx *ctx, const msg_system *sys, int32_t n_items, const double *fields,
                                double rate, double *yields);
+
+/*
+ * Time-infinity yields of HSGeneral (calculation=yields; method=timeinf): replaces
+ * HSPropagator::SolveTimeInfinity + HSObservableCollector::EvaluateDensity
+ * (TaskHSGeneral.cpp:453-467, HSPropagator.cpp:259-271, SpinSpace_operators.cpp:142-201,
+ * HSObservableCollector.cpp:290-301).  Per item,
+ *   X = int_0^inf rho(t) dt,   G X + X G^dagger = -rho0,   G = -iH - K,
+ *   y_c = scale_c Re Tr[P_c X]  (scale = rate for transition yields, 1 for States),
+ * with rho0 = rho_e (x) 1/Z (SAMPLING_DIRECT) or columns columns^dagger (SAMPLING_GIVEN,
+ * n_columns <= D, MolSpin basis order, column-major).  FP64 on the device:
+ *   - K = kappa 1 and D <= 4096 (method AUTO / KRYLOV): the eigenpairs of H give
+ *     X_gh = rho0_gh / (2 kappa + i (lambda_g - lambda_h)) in the eigenbasis;
+ *   - otherwise D <= 512 (and always with method EXPONENTIAL): X by doubling,
+ *     X <- X + U X U^dagger, U <- U^2 from U = e^{G tau} and the Taylor-series
+ *     integral over [0, tau], until the added integral over [T, 2T] is below 1e-10
+ *     of X (Frobenius).
+ * Systems with relaxation operators are MSG_UNSUPPORTED (the reference adds a
+ * Liouville-space relaxation term there).  A part of the space that never decays
+ * adds nothing while rho(t) never reaches it, as the minimum-norm solution the
+ * reference's singular solve falls back to; once populated it makes the integral
+ * diverge: MSG_NUMERIC_BREAKDOWN (the doubling route reports it when the integral has
+ * not converged by T = tau 2^36, with tau = 1/(4 ||G||_1)).  total_time, time_step, krylov_size and yield_corrections are not used
+ * (HSObservableCollector.cpp:366 applies no correction at time infinity).
+ * fields: n_items x 3 (NULL: as described).  yields: n_items x n_obs.
+ */
+msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_params *params,
+                             int32_t n_items, const double *fields,
+                             const double *columns, int32_t n_columns, double *yields);
 
 /* Launch shape chosen by msg_autotune. */
 typedef struct msg_launch_shape {

@@ -19,7 +19,7 @@ EXPORTS = [
     "msg_system_dimension", "msg_query_footprint", "msg_direct_columns", "msg_observable_count",
     "msg_time_grid", "msg_propagate", "msg_step", "msg_server_create", "msg_server_destroy",
     "msg_server_submit", "msg_server_pump", "msg_server_collect", "msg_last_kernel_time",
-    "msg_trace_samples", "msg_symmetric_decay", "msg_resolve_plan", "msg_autotune",
+    "msg_trace_samples", "msg_symmetric_decay", "msg_resolve_plan", "msg_autotune", "msg_time_infinity",
 ]
 
 
@@ -58,6 +58,7 @@ def lib():
         L.msg_last_kernel_time.argtypes = [vp, P(C.c_double), P(C.c_int32)]
         L.msg_trace_samples.argtypes = [vp, C.c_int32, C.c_int32, C.c_uint64, C.c_int32, vp]
         L.msg_symmetric_decay.argtypes = [vp, vp, C.c_int32, vp, C.c_double, vp]
+        L.msg_time_infinity.argtypes = [vp, vp, P(abi.RunParams), C.c_int32, vp, vp, C.c_int32, vp]
         L.msg_resolve_plan.argtypes = [C.c_char_p, P(abi.Plan), C.c_char_p, C.c_int32]
         L.msg_autotune.argtypes = [vp, vp, P(abi.RunParams), C.c_int32, vp, vp, C.c_int32, C.c_int32,
                                    P(abi.LaunchShape)]
@@ -202,6 +203,16 @@ class System:
         f = None if fields is None else np.ascontiguousarray(fields, dtype=np.float64).reshape(-1)
         y = np.zeros(n_items * self.packed.desc.n_states)
         self.ctx.check(lib().msg_symmetric_decay(self.ctx.h, self.h, n_items, _ptr(f), float(rate), _ptr(y)))
+        return y.reshape(n_items, -1)
+
+    def time_infinity(self, params, fields=None, columns=None):
+        """HSGeneral method=timeinf yields (msg_time_infinity); returns [n_items, n_obs]."""
+        n_items = 1 if fields is None else len(fields)
+        f = None if fields is None else np.ascontiguousarray(fields, dtype=np.float64).reshape(-1)
+        cols, ncol = (None, 0) if columns is None else _columns(columns)
+        y = np.zeros(n_items * self.n_obs(params))
+        self.ctx.check(lib().msg_time_infinity(self.ctx.h, self.h, C.byref(params), n_items, _ptr(f), _ptr(cols),
+                                               ncol, _ptr(y)))
         return y.reshape(n_items, -1)
 
     def step(self, params, factors, dt, step_index=0, field=None):

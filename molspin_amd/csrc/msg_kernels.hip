@@ -1208,18 +1208,23 @@ __global__ void k_blockdiag_right(const double2 *__restrict__ M, Block4 B, doubl
     }
 }
 
-// rows[g] = sum_h Re(Pt_gh conj(Rt_gh)) k^2 / (k^2 + (lam_g - lam_h)^2); one workgroup per row.
+// rows[g] = sum_h (a Re z_gh + disp w_gh Im z_gh) sc / (a^2 + w_gh^2) with
+// z_gh = conj(Pt_gh) Rt_gh and w_gh = lam_g - lam_h; one workgroup per row.
+//  * symmetric decay: a = sc = k, disp = 0, i.e. Re z k^2 / (k^2 + w^2);
+//  * time-infinity integral of Tr[P rho(t)] under the decay e^{-a t}: a = 2 kappa,
+//    sc = 1, disp = 1, i.e. Re(z / (a + i w)).
 __global__ __launch_bounds__(256) void k_lorentz_rows(const double2 *__restrict__ Pt, const double2 *__restrict__ Rt,
-                                                      const double *__restrict__ lam, double k2, int n,
-                                                      double *__restrict__ rows) {
+                                                      const double *__restrict__ lam, double a, double disp, double sc,
+                                                      int n, double *__restrict__ rows) {
     __shared__ double red[256];
     const int g = blockIdx.x, t = threadIdx.x;
-    const double lg = lam[g];
+    const double lg = lam[g], a2 = a * a;
     double acc = 0.0;
     for (int h = t; h < n; h += blockDim.x) {
         const double2 p = Pt[(size_t)g * n + h], r = Rt[(size_t)g * n + h];
         const double w = lg - lam[h];
-        acc += (p.x * r.x + p.y * r.y) * (k2 / (k2 + w * w));
+        const double re = p.x * r.x + p.y * r.y, im = p.x * r.y - p.y * r.x;
+        acc += (a * re + disp * w * im) * (sc / (a2 + w * w));
     }
     red[t] = acc;
     __syncthreads();
@@ -1243,6 +1248,68 @@ __global__ __launch_bounds__(256) void k_sum_rows(const double *__restrict__ row
         __syncthreads();
     }
     if (t == 0) out[0] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// Time-infinity integral X = int_0^inf e^{Gt} rho0 e^{G^dagger t} dt by doubling
+// (batches of n x n row-major matrices, contiguous):
+//   X_tau = tau S_0 with S_n = rho0 + tau/(n+2) L(S_{n+1}), L(S) = G S + (G S)^dagger,
+//   then X <- X + U X U^dagger and U <- U^2 (U = e^{G tau}) until U has decayed.
+// ---------------------------------------------------------------------------
+
+// S = s (R + c (Y + Y^dagger)) for every batch entry; Y = G S_prev (Hermitian S).
+__global__ void k_herm_update(const double2 *__restrict__ Y, const double2 *__restrict__ R, double2 *__restrict__ S,
+                              int n, size_t total, double c, double s) {
+    const size_t nn = (size_t)n * n;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t e = idx % nn, base = idx - e;
+        const size_t i = e / n, j = e - i * n;
+        const double2 y = Y[idx], yt = Y[base + j * n + i], r = R[idx];
+        S[idx] = make_double2(s * (r.x + c * (y.x + yt.x)), s * (r.y + c * (y.y - yt.y)));
+    }
+}
+
+// out[b] = sum_ij |U_b[i][j]|^2; one workgroup per batch entry, fixed reduction order.
+__global__ __launch_bounds__(256) void k_fro2(const double2 *__restrict__ U, int n, double *__restrict__ out) {
+    __shared__ double red[256];
+    const size_t nn = (size_t)n * n;
+    const double2 *u = U + (size_t)blockIdx.x * nn;
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    for (size_t i = t; i < nn; i += blockDim.x) acc += u[i].x * u[i].x + u[i].y * u[i].y;
+    red[t] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) out[blockIdx.x] = red[0];
+}
+
+// Electron-pair block of the partial trace over the nuclei (device index 4 nu + c):
+// out[b][a][c] = sum_nu X_b[4 nu + a][4 nu + c]; grid (batch, 16).
+__global__ __launch_bounds__(256) void k_pair_block_trace(const double2 *__restrict__ X, int n,
+                                                          double2 *__restrict__ out) {
+    __shared__ double rx[256], ry[256];
+    const int b = blockIdx.x, ac = blockIdx.y, a = ac >> 2, c = ac & 3, t = threadIdx.x;
+    const double2 *x = X + (size_t)b * n * n;
+    double sx = 0.0, sy = 0.0;
+    for (int nu = t; nu < n / 4; nu += blockDim.x) {
+        const double2 v = x[(size_t)(4 * nu + a) * n + 4 * nu + c];
+        sx += v.x;
+        sy += v.y;
+    }
+    rx[t] = sx;
+    ry[t] = sy;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            rx[t] += rx[t + w];
+            ry[t] += ry[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) out[(size_t)b * 16 + ac] = make_double2(rx[0], ry[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1539,9 +1606,36 @@ hipError_t launch_unc_combine(const double *CA, const double *CB, const double *
 
 hipError_t launch_lorentz_sum(const void *Pt, const void *Rt, const double *lam, double k, int n, double *rows,
                               double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_lorentz_rows, dim3(n), dim3(256), 0, st, (const double2 *)Pt, (const double2 *)Rt, lam, k * k,
-                       n, rows);
+    hipLaunchKernelGGL(k_lorentz_rows, dim3(n), dim3(256), 0, st, (const double2 *)Pt, (const double2 *)Rt, lam, k, 0.0,
+                       k, n, rows);
     hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, st, (const double *)rows, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_timeinf_sum(const void *Pt, const void *Rt, const double *lam, double a, int n, double *rows,
+                              double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_lorentz_rows, dim3(n), dim3(256), 0, st, (const double2 *)Pt, (const double2 *)Rt, lam, a, 1.0,
+                       1.0, n, rows);
+    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, st, (const double *)rows, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_herm_update(const void *Y, const void *R, void *S, int n, int batch, double c, double s,
+                              hipStream_t st) {
+    const size_t total = (size_t)batch * n * n;
+    const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_herm_update, dim3(blocks), dim3(256), 0, st, (const double2 *)Y, (const double2 *)R,
+                       (double2 *)S, n, total, c, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_fro2(const void *U, int n, int batch, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_fro2, dim3(batch), dim3(256), 0, st, (const double2 *)U, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_block_trace(const void *X, int n, int batch, void *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_pair_block_trace, dim3(batch, 16), dim3(256), 0, st, (const double2 *)X, n, (double2 *)out);
     return hipGetLastError();
 }
 

@@ -35,6 +35,9 @@ def lib():
         L.ref_run_columns.argtypes = [P(abi.SystemDesc), P(abi.RunParams), C.c_void_p, C.c_void_p,
                                       C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_void_p, C.c_int]
         L.ref_symmetric_decay.argtypes = [P(abi.SystemDesc), C.c_void_p, C.c_double, C.c_void_p]
+        L.ref_time_infinity.argtypes = [P(abi.SystemDesc), P(abi.RunParams), C.c_void_p, C.c_void_p,
+                                        C.c_int32, C.c_void_p]
+        L.ref_lyapunov_integral.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ref_legacy_direct_yields.argtypes = [P(abi.SystemDesc), C.c_void_p, C.c_double, C.c_double,
                                                C.c_int, C.c_double, C.c_int, C.c_void_p]
         L.ref_legacy_stoch_yields.argtypes = [P(abi.SystemDesc), C.c_void_p, C.c_double, C.c_double,
@@ -160,6 +163,31 @@ def symmetric_decay(packed, k, field=None):
     y = np.zeros(packed.desc.n_states)
     _check(lib().ref_symmetric_decay(packed.ref(), _p(f), float(k), _p(y)))
     return y
+
+
+def time_infinity(packed, params, field=None, columns=None):
+    """HSGeneral method=timeinf yields of one item (the D^2 x D^2 time-integral solve)."""
+    f = None if field is None else np.ascontiguousarray(field, dtype=np.float64)
+    cols, ncol = None, 0
+    if columns is not None:
+        cc = np.asfortranarray(columns, dtype=np.complex128)
+        ncol = cc.shape[1]
+        cols = cc.reshape(-1, order="F").view(np.float64).copy()
+    n_obs = len(packed_transitions(packed)) if params.observables == abi.MSG_OBS_TRANSITION_YIELDS \
+        else packed.desc.n_states
+    y = np.zeros(n_obs)
+    _check(lib().ref_time_infinity(packed.ref(), C.byref(params), _p(f), _p(cols), ncol, _p(y)))
+    return y
+
+
+def lyapunov_integral(g, rho0):
+    """X with G X + X G^dagger = -rho0 by the oracle's Kronecker-form solve."""
+    n = g.shape[0]
+    gg = np.asfortranarray(g, dtype=np.complex128).reshape(-1, order="F").view(np.float64).copy()
+    rr = np.asfortranarray(rho0, dtype=np.complex128).reshape(-1, order="F").view(np.float64).copy()
+    out = np.zeros(2 * n * n)
+    _check(lib().ref_lyapunov_integral(n, _p(gg), _p(rr), _p(out)))
+    return _cplx(out, (n, n))
 
 
 def legacy_direct_yields(packed, total, dt, krylov=16, tol=0.0, correction=False, field=None):

@@ -12,6 +12,9 @@
 // Tests/tests_HSGeneral.cpp (trapezoid/grid/correction); Breit-Rabi and
 // eigen-decomposition closed forms for the time loop; Philox known-answer
 // vectors; Fibonacci/LogSpace vectors of Tests/tests_actions.cpp:266-335.
+// Time-infinity yields (ref_time_infinity) are pinned by the closed forms of
+// Tests/tests_HSGeneral.cpp:1410-1431,1704-1730, by scipy's Bartels-Stewart Lyapunov
+// solver and by the eigen closed form for K = kappa 1.
 // Parity unpinned: the SplitMix64/Philox sample streams of ref_coherent_samples
 // (checked statistically only).
 #include "refcpu.h"
@@ -732,6 +735,108 @@ void run_block(const System &S, const ItemModel &M, const ObsSet &O, const std::
     }
 }
 
+// Minimum-norm least-squares solution of L x = b (row-major n x n), through the
+// eigenpairs of L^dagger L: x = W diag(1/lambda) W^dagger L^dagger b over the
+// eigenvalues above 1e-12 lambda_max (singular values above 1e-6 sigma_max; the
+// eigenvalues of the null space come out at the rounding level, ~n eps lambda_max).
+std::vector<cd> min_norm_solve(const std::vector<cd> &L, const std::vector<cd> &b, int n) {
+    Mat A(n, n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            cd acc = 0.0;
+            for (int k = 0; k < n; ++k) acc += std::conj(L[(size_t)i * n + k]) * L[(size_t)j * n + k];
+            A(i, j) = acc;
+        }
+    std::vector<double> lam;
+    Mat W;
+    eig_hermitian(A, lam, W);
+    double lmax = 0.0;
+    for (double l : lam) lmax = std::max(lmax, std::fabs(l));
+    std::vector<cd> c(n, cd(0.0)), x(n, cd(0.0));  // c = L^dagger b
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) c[i] += std::conj(L[(size_t)i * n + k]) * b[k];
+    for (int m = 0; m < n; ++m) {
+        if (!(lam[m] > 1e-12 * lmax)) continue;
+        cd proj = 0.0;
+        for (int i = 0; i < n; ++i) proj += std::conj(W(i, m)) * c[i];
+        proj /= lam[m];
+        for (int i = 0; i < n; ++i) x[i] += W(i, m) * proj;
+    }
+    return x;
+}
+
+// X = int_0^inf e^{Gt} rho0 e^{G^dagger t} dt from G X + X G^dagger = -rho0, solved as
+// the D^2 x D^2 linear system of SpinSpace::SolveHilbertTimeIntegral
+// (SpinSpace_operators.cpp:142-201): Liouvillian kron(G, 1) + kron(1, conj G) over the
+// row-major vectorisation X_ij -> x[i D + j], right-hand side -vec(rho0), LU with partial
+// pivoting (the reference's arma::solve).  Column-oriented elimination with OpenMP over
+// the trailing columns.  A singular system (a part of the space that never decays,
+// e.g. triplets without a sink or mixing; detected as a pivot below 1e-13 of the largest
+// entry) takes the minimum-norm least-squares solution, arma::solve's fallback for
+// singular systems (D <= 32 here): the never-decaying part then stays empty.
+Mat lyapunov_integral(const Mat &G, const Mat &rho0) {
+    const int D = G.r;
+    const int n = D * D;
+    if (n > 4096) throw std::runtime_error("time-integral oracle limited to D <= 64");
+    std::vector<cd> L((size_t)n * n, cd(0.0)), x(n);
+    auto at = [&](int i, int j) -> cd & { return L[(size_t)j * n + i]; };  // column-major
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) {
+            const cd g = G(i, j), gc = std::conj(G(i, j));
+            for (int k = 0; k < D; ++k) {
+                at(i * D + k, j * D + k) += g;   // kron(G, 1)
+                at(k * D + i, k * D + j) += gc;  // kron(1, conj G)
+            }
+        }
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) x[(size_t)i * D + j] = -rho0(i, j);
+    const std::vector<cd> L0 = n <= 1024 ? L : std::vector<cd>(), b0 = x;
+    auto to_mat = [&](const std::vector<cd> &v) {
+        Mat X(D, D);
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j < D; ++j) X(i, j) = v[(size_t)i * D + j];
+        return X;
+    };
+    double amax = 0.0;
+    for (const cd &v : L) amax = std::max(amax, std::abs(v));
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = std::abs(at(k, k));
+        for (int i = k + 1; i < n; ++i)
+            if (std::abs(at(i, k)) > best) { best = std::abs(at(i, k)); p = i; }
+        if (best <= 1e-13 * amax) {  // singular up to rounding
+            if (L0.empty()) throw std::runtime_error("failed to solve the static Hilbert time-integral equation");
+            std::vector<cd> Lt((size_t)n * n);  // row-major copy for min_norm_solve
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) Lt[(size_t)i * n + j] = L0[(size_t)j * n + i];
+            return to_mat(min_norm_solve(Lt, b0, n));
+        }
+        if (p != k) {
+            for (int j = 0; j < n; ++j) std::swap(at(k, j), at(p, j));
+            std::swap(x[k], x[p]);
+        }
+        const cd inv = 1.0 / at(k, k);
+        for (int i = k + 1; i < n; ++i) at(i, k) *= inv;
+        const cd *lk = &L[(size_t)k * n];
+#pragma omp parallel for schedule(static)
+        for (int j = k + 1; j < n; ++j) {
+            cd *cj = &L[(size_t)j * n];
+            const cd akj = cj[k];
+            if (akj == 0.0) continue;
+            for (int i = k + 1; i < n; ++i) cj[i] -= lk[i] * akj;
+        }
+        const cd xk = x[k];
+        if (xk != 0.0)
+            for (int i = k + 1; i < n; ++i) x[i] -= lk[i] * xk;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        cd s = x[i];
+        for (int k = i + 1; k < n; ++k) s -= at(i, k) * x[k];
+        x[i] = s / at(i, i);
+    }
+    return to_mat(x);
+}
+
 Mat load_columns(int64_t D, const double *cols, int m) {
     Mat B((int)D, m);
     for (size_t i = 0; i < B.a.size(); ++i) B.a[i] = cd(cols[2 * i], cols[2 * i + 1]);
@@ -898,6 +1003,43 @@ int ref_symmetric_decay(const msg_system_desc *sys, const double *field, double 
                 }
             yields[st] = acc;
         }
+    });
+}
+
+int ref_time_infinity(const msg_system_desc *sys, const msg_run_params *p, const double *field,
+                      const double *columns, int32_t n_columns, double *yields) {
+    return guarded([&] {
+        System S(sys);
+        if (sys->n_relaxations > 0) throw std::runtime_error("the time-integral oracle covers systems without relaxation");
+        const ObsSet O = observables(S, p->observables);
+        // rho0 = factors factors^dagger (TaskHSGeneral.cpp:455-456)
+        const Mat B = p->sampling == MSG_SAMPLING_GIVEN ? load_columns(S.D, columns, n_columns) : direct_factors(S);
+        Mat Bh(B.c, B.r);
+        for (int i = 0; i < B.r; ++i)
+            for (int j = 0; j < B.c; ++j) Bh(j, i) = std::conj(B(i, j));
+        const Mat rho0 = matmul(B, Bh);
+        // G = -iH - K (SpinSpace_operators.cpp:165-166)
+        const Mat G = add(scale(dense_of(build_hamiltonian(S, field)), -I1), dense_of(reaction_operator(S)), -1.0);
+        const Mat X = lyapunov_integral(G, rho0);
+        // EvaluateDensity (HSObservableCollector.cpp:290-301): rate * Re Tr[P X]
+        for (size_t o = 0; o < O.ops.size(); ++o) {
+            const Mat P = dense_of(O.ops[o]);
+            cd tr = 0.0;
+            for (int i = 0; i < P.r; ++i)
+                for (int k = 0; k < P.c; ++k) tr += P(i, k) * X(k, i);
+            yields[o] = O.scale[o] * tr.real();
+        }
+    });
+}
+
+int ref_lyapunov_integral(int32_t n, const double *g, const double *rho0, double *out) {
+    return guarded([&] {
+        Mat G(n, n), R(n, n);
+        for (size_t i = 0; i < G.a.size(); ++i) {
+            G.a[i] = cd(g[2 * i], g[2 * i + 1]);
+            R.a[i] = cd(rho0[2 * i], rho0[2 * i + 1]);
+        }
+        store(lyapunov_integral(G, R), out);
     });
 }
 

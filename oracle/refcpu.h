@@ -67,6 +67,18 @@ int ref_run_columns(const msg_system_desc *sys, const msg_run_params *params,
  * (TaskStaticHSSymmetricDecay.cpp:114-158): Phi_P = sum_{g,h} Re<g|P|h><h|rho0|g> k^2/(k^2+w_gh^2). */
 int ref_symmetric_decay(const msg_system_desc *sys, const double *field, double k, double *yields);
 
+/* Time-infinity yields of HSGeneral (method=timeinf, TaskHSGeneral.cpp:453-467):
+ * X = int_0^inf rho(t) dt from the D^2 x D^2 solve of SpinSpace::SolveHilbertTimeIntegral
+ * (SpinSpace_operators.cpp:142-201) with rho0 = factors factors^dagger, then
+ * EvaluateDensity (HSObservableCollector.cpp:290-301): scale_c Re Tr[P_c X].  No
+ * relaxation; D <= 64.  yields: n_obs values. */
+int ref_time_infinity(const msg_system_desc *sys, const msg_run_params *params, const double *field,
+                      const double *columns, int32_t n_columns, double *yields);
+
+/* The same solve for a given generator: X with G X + X G^dagger = -rho0 (column-major
+ * n x n complex, interleaved). */
+int ref_lyapunov_integral(int32_t n, const double *g, const double *rho0, double *out);
+
 /* Legacy StaticHS-Direct-Yields, symmetric Krylov/Lanczos branch with basis reuse
  * (TaskStaticHSDirectYields.cpp:190-292,569-652,753-807).  yields: one per transition. */
 int ref_legacy_direct_yields(const msg_system_desc *sys, const double *field,
