@@ -81,7 +81,23 @@ typedef struct msg_interaction_desc {
     double  prefactor;         /* prefactor (default 1)                                */
     double  field[3];          /* Zeeman field as written                              */
     double  tensor[9];         /* Tensor::LabFrame() of the coupling tensor, row-major */
+    /* Time dependence of a Zeeman field (fieldtype / timedependence, Interaction.cpp:
+     * 500-560, 1240-1262, 1900-1933): MSG_TD_*.  Used by dynamics=dynamic runs only;
+     * a static run takes `field` as written. */
+    int32_t time_dependence;
+    int32_t td_perpendicular;  /* perpendicularoscillations (circular; default false)  */
+    double  td_frequency;      /* frequency, rad/ns (default 1)                        */
+    double  td_phase;          /* phase, rad (default 0)                               */
+    double  td_axis[3];        /* axis (circular; default 0 0 1, used as given)        */
 } msg_interaction_desc;
+
+enum {
+    MSG_TD_NONE = 0,
+    MSG_TD_LINEAR = 1,         /* linearpolarized: B(t) = field cos(w t + phase)            */
+    MSG_TD_CIRCULAR = 2        /* circularpolarized: B(t) = R(w t + phase) b with the Rodrigues
+                                  matrix R = 1 + sin(x) W + 2 sin^2(x/2) W^2 of the axis, b =
+                                  field or, with perpendicularoscillations, field - (axis.field) axis */
+};
 
 /* SpinAPI::State restricted to the spins it fixes: a (not necessarily normalised)
  * amplitude vector over `spins` in the listed order, first spin most significant.
@@ -179,7 +195,10 @@ typedef struct msg_run_params {
                                      (endpoint excluded, TaskStaticHSDirectYields.cpp:347) and treat
                                      equal rates as symmetric recombination with the decay
                                      e^{-k t} applied analytically (:257-292) */
-    int32_t  reserved;
+    int32_t  dynamics;            /* dynamics=dynamic (HSExecutionPlan.cpp:96-101): time-dependent
+                                     Zeeman fields at their time-dependent value; each interval
+                                     uses H at its midpoint (TaskHSGeneral.cpp:527-537), or the RK4
+                                     stages at t, t+dt/2, t+dt with method RK4 (:493-519) */
 } msg_run_params;
 
 /* propagationmethod as written in the plan (HSExecutionPlan.h:29) */
@@ -204,7 +223,7 @@ typedef struct msg_plan {
 /*
  * Resolves the property list of an MSD Task block, "key=value; key=value; ...",
  * with ResolveExecutionPlan's keys, aliases, defaults and error messages.  Keys
- * selecting physics off this path (dynamics=dynamic, powder grids, secular
+ * selecting physics off this path (powder grids, secular
  * approximation, spin-polarisation/CIDSP output, pulse sequences) return
  * MSG_UNSUPPORTED naming the key; invalid values return MSG_INVALID_ARG with the
  * reference's message.  `error` (error_len bytes, may be NULL) receives the message.

@@ -40,6 +40,9 @@ MSG_PROP_EXPONENTIAL = 0
 MSG_PROP_AUTOEXPM = 1
 MSG_PROP_KRYLOV = 2
 MSG_PROP_RK4 = 3
+MSG_TD_NONE = 0
+MSG_TD_LINEAR = 1
+MSG_TD_CIRCULAR = 2
 MSG_TRACE_COHERENT = 0
 MSG_TRACE_SUZ = 1
 
@@ -57,7 +60,9 @@ class InteractionDesc(C.Structure):
                 ("tensor_present", C.c_int32), ("tensor_isotropic", C.c_int32),
                 ("common_prefactor", C.c_int32), ("ignore_tensors", C.c_int32),
                 ("field_swept", C.c_int32),
-                ("prefactor", C.c_double), ("field", C.c_double * 3), ("tensor", C.c_double * 9)]
+                ("prefactor", C.c_double), ("field", C.c_double * 3), ("tensor", C.c_double * 9),
+                ("time_dependence", C.c_int32), ("td_perpendicular", C.c_int32),
+                ("td_frequency", C.c_double), ("td_phase", C.c_double), ("td_axis", C.c_double * 3)]
 
 
 class StateDesc(C.Structure):
@@ -90,7 +95,7 @@ class RunParams(C.Structure):
                 ("method", C.c_int32), ("krylov_size", C.c_int32),
                 ("sampling", C.c_int32), ("yield_corrections", C.c_int32),
                 ("total_time", C.c_double), ("time_step", C.c_double),
-                ("seed", C.c_uint64), ("task", C.c_int32), ("reserved", C.c_int32)]
+                ("seed", C.c_uint64), ("task", C.c_int32), ("dynamics", C.c_int32)]
 
 
 class Plan(C.Structure):
@@ -106,7 +111,9 @@ class LaunchShape(C.Structure):
 
 def run_params(calculation=MSG_CALC_YIELDS, observables=MSG_OBS_TRANSITION_YIELDS,
                method=MSG_METHOD_KRYLOV, krylov_size=16, sampling=MSG_SAMPLING_DIRECT,
-               yield_corrections=False, total_time=10000.0, time_step=1.0, seed=1, task=MSG_TASK_HSGENERAL):
+               yield_corrections=False, total_time=10000.0, time_step=1.0, seed=1, task=MSG_TASK_HSGENERAL,
+               dynamics=False):
     """RunParams with HSExecutionPlan defaults (HSExecutionPlan.h:60-79)."""
     return RunParams(calculation, observables, method, krylov_size, sampling,
-                     int(bool(yield_corrections)), float(total_time), float(time_step), int(seed), int(task), 0)
+                     int(bool(yield_corrections)), float(total_time), float(time_step), int(seed), int(task),
+                     int(bool(dynamics)))

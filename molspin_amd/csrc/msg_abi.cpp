@@ -47,6 +47,8 @@ hipError_t launch_timeinf_sum(const void *Pt, const void *Rt, const double *lam,
                               double *out, hipStream_t st);
 hipError_t launch_herm_update(const void *Y, const void *R, void *S, int n, int batch, double c, double s,
                               hipStream_t st);
+hipError_t launch_dyn_generator(const void *G, const void *Z, const double *coef, const int *item, const double *dt,
+                               int nz, int n, int batch, void *out, hipStream_t st);
 hipError_t launch_fro2(const void *U, int n, int batch, double *out, hipStream_t st);
 hipError_t launch_pair_block_trace(const void *X, int n, int batch, void *out, hipStream_t st);
 hipError_t launch_pauli_right(const void *M, void *Y, int n, int mu, int batch, hipStream_t st);
@@ -158,6 +160,14 @@ struct Batch {
     size_t footprint = 0;
     StepArgs args_host{};  // host copy of the last launch arguments
     bool timed = false;    // bracket the propagation kernel with the context events
+    // dynamics=dynamic on the dense path: one propagator per interval, built per chunk
+    // of intervals from G_item (static generator) + sum_k c_k(t) Z_k
+    bool dyn_dense = false;
+    int dyn_chunk = 0;                      // intervals per propagator chunk
+    DBuf dynG, dynZ, dyn_head;
+    std::vector<double> dyn_gnorm, dyn_znorm, item_fields;
+    std::vector<std::pair<int, int>> dyn_terms;  // (spin, axis) of each Z_k
+    bool field_given = false;
     Batch() = default;
     Batch(const Batch &) = delete;
 };
@@ -176,6 +186,9 @@ void check_params(const msg_system *sys, const msg_run_params *p) {
     if (p->krylov_size < 0 || p->krylov_size > kMaxKrylov) throw ArgError{MSG_UNSUPPORTED, "krylovsize above 32"};
     if (p->sampling != MSG_SAMPLING_DIRECT && p->sampling != MSG_SAMPLING_GIVEN)
         throw ArgError{MSG_INVALID_ARG, "unknown sampling mode"};
+    if (p->dynamics != 0 && p->dynamics != 1) throw ArgError{MSG_INVALID_ARG, "dynamics must be static or dynamic"};
+    if (p->dynamics && p->task != MSG_TASK_HSGENERAL)
+        throw ArgError{MSG_INVALID_ARG, "the legacy StaticHS tasks take static Hamiltonians"};
     if (!sys->r.pair_ok)
         throw ArgError{MSG_UNSUPPORTED,
                        "the device paths need exactly two spin-1/2 electrons and States over those electrons"};
@@ -209,7 +222,13 @@ Path choose_path(const Resolved &r, const msg_run_params *p) {
     if (p->method == MSG_METHOD_UNCOUPLED)
         throw ArgError{MSG_UNSUPPORTED, "propagationmethod=uncoupled is available through msg_propagate only"};
     const bool dense_ok = r.D <= 512;
-    const bool lanczos_ok = r.fast_ok && sym_for(r, p) && r.D >= 256 && r.D <= (int64_t)4 << kMaxNuclei;
+    // a time-dependent field on a nucleus changes per-nucleus coefficients, which the
+    // matrix-free head cannot carry; the dense path builds each interval's propagator
+    const bool dyn_nuclear = p->dynamics && r.has_td && r.td_nuclear;
+    const bool lanczos_ok =
+        r.fast_ok && sym_for(r, p) && r.D >= 256 && r.D <= (int64_t)4 << kMaxNuclei && !dyn_nuclear;
+    if (dyn_nuclear && !dense_ok)
+        throw ArgError{MSG_UNSUPPORTED, "a time-dependent field on a nucleus needs D <= 512 (dense path)"};
     if (p->method == MSG_METHOD_EXPONENTIAL || p->method == MSG_METHOD_RK4) {
         if (!dense_ok)
             throw ArgError{MSG_UNSUPPORTED, "propagationmethod=exponential/rk4 needs D <= 512 on the device"};
@@ -427,9 +446,12 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         std::vector<float> all, one;
         int flags = 0;
         std::vector<int> nfl(kMaxNuclei, 0);
+        const bool dyn = p->dynamics && r.has_td;
+        TimePoint static_part;
+        static_part.skip = true;
         for (int it = 0; it < n_items; ++it) {
             int nf[kMaxNuclei];
-            fast_coefficients(r, fields ? fields + 3 * it : nullptr, one, flags, nf);
+            fast_coefficients(r, fields ? fields + 3 * it : nullptr, one, flags, nf, dyn ? &static_part : nullptr);
             if (it == 0) {
                 a.coef_stride = (int)((one.size() + 15) / 16 * 16);
                 all.assign((size_t)n_items * a.coef_stride, 0.f);
@@ -443,9 +465,99 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         a.nuc_flags = dfl;
         float *dc = b.coef.alloc<float>(all.size(), stream);
         HIPCHECK(hipMemcpyAsync(dc, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        if (dyn) {
+            // per-interval head: the static w1, w2, shift plus the time-dependent electron
+            // fields at the interval's midpoint (TaskHSGeneral.cpp:527-537)
+            std::vector<float> heads((size_t)n_items * (ns - 1) * 8, 0.f);
+            for (int it = 0; it < n_items; ++it) {
+                const double *f = fields ? fields + 3 * it : nullptr;
+                for (int s = 0; s + 1 < ns; ++s) {
+                    TimePoint tp;
+                    tp.t = b.times[s] + 0.5 * (b.times[s + 1] - b.times[s]);
+                    double h[8];
+                    for (int k = 0; k < 8; ++k) h[k] = all[(size_t)it * a.coef_stride + k];
+                    for (const Term1 &t : r.t1) {
+                        if (!t.td) continue;
+                        double v[3];
+                        term1_value(t, f, &tp, v);
+                        const int off = t.s == r.e1 ? 0 : 3;
+                        for (int k = 0; k < 3; ++k) h[off + k] += v[k];
+                    }
+                    float *dst = &heads[((size_t)it * (ns - 1) + s) * 8];
+                    for (int k = 0; k < 8; ++k) dst[k] = (float)h[k];
+                }
+            }
+            float *dh = b.dyn_head.alloc<float>(heads.size(), stream);
+            HIPCHECK(hipMemcpyAsync(dh, heads.data(), heads.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+            HIPCHECK(hipStreamSynchronize(stream));  // `heads` is released at scope end
+            a.dyn_head = dh;
+        }
         a.coef = dc;
         b.n_units = n_items * b.ncols;
         a.renorm = factor_decay ? 1 : 0;
+    } else if (p->dynamics && r.has_td) {
+        // dense path, time-dependent H: one propagator per interval, built chunk by chunk
+        // in advance(); here the static generator of every item and the time-dependent
+        // operators Z_k = -i S_a (spin s) of the spins with time-dependent fields
+        const int Dp = dense_padding(D);
+        if (Dp == 0) throw ArgError{MSG_UNSUPPORTED, "dense path limited to D <= 512"};
+        a.dpad = Dp;
+        a.ustep = 1;
+        b.dyn_dense = true;
+        b.keep_state = true;
+        const size_t DD = (size_t)D * D;
+        std::vector<cd> H, all(DD * n_items);
+        TimePoint static_part;
+        static_part.skip = true;
+        b.dyn_gnorm.assign(n_items, 0.0);
+        for (int it = 0; it < n_items; ++it) {
+            dense_hamiltonian(r, fields ? fields + 3 * it : nullptr, H, &static_part);
+            cd *g = &all[(size_t)it * DD];
+            for (size_t k = 0; k < DD; ++k) g[k] = cd(H[k].imag(), -H[k].real());  // -i H
+            if (!factor_decay)
+                for (int nu = 0; nu < D / 4; ++nu)
+                    for (int i = 0; i < 4; ++i)
+                        for (int j = 0; j < 4; ++j) g[(size_t)(4 * nu + i) * D + 4 * nu + j] -= r.K[i * 4 + j];
+            for (int j = 0; j < D; ++j) {
+                double cs = 0.0;
+                for (int i = 0; i < D; ++i) cs += std::abs(g[(size_t)i * D + j]);
+                b.dyn_gnorm[it] = std::max(b.dyn_gnorm[it], cs);
+            }
+        }
+        cd *dG = b.dynG.alloc<cd>(all.size(), stream);
+        HIPCHECK(hipMemcpyAsync(dG, all.data(), all.size() * sizeof(cd), hipMemcpyHostToDevice, stream));
+        std::vector<int> td_spins;
+        for (const Term1 &t : r.t1)
+            if (t.td && std::find(td_spins.begin(), td_spins.end(), t.s) == td_spins.end()) td_spins.push_back(t.s);
+        b.dyn_terms.clear();
+        std::vector<cd> Z(3 * td_spins.size() * DD);
+        b.dyn_znorm.assign(3 * td_spins.size(), 0.0);
+        for (size_t q = 0; q < td_spins.size(); ++q)
+            for (int ax = 0; ax < 3; ++ax) {
+                const size_t k = 3 * q + ax;
+                b.dyn_terms.push_back({td_spins[q], ax});
+                spin_operator_dense(r, td_spins[q], ax, H);
+                cd *z = &Z[k * DD];
+                for (size_t e = 0; e < DD; ++e) z[e] = cd(H[e].imag(), -H[e].real());
+                for (int j = 0; j < D; ++j) {
+                    double cs = 0.0;
+                    for (int i = 0; i < D; ++i) cs += std::abs(z[(size_t)i * D + j]);
+                    b.dyn_znorm[k] = std::max(b.dyn_znorm[k], cs);
+                }
+            }
+        cd *dZ = b.dynZ.alloc<cd>(Z.size(), stream);
+        HIPCHECK(hipMemcpyAsync(dZ, Z.data(), Z.size() * sizeof(cd), hipMemcpyHostToDevice, stream));
+        HIPCHECK(hipStreamSynchronize(stream));  // host staging is released at scope end
+        b.field_given = fields != nullptr;
+        if (fields) b.item_fields.assign(fields, fields + 3 * (size_t)n_items);
+        // propagator planes of one chunk: at most ~1 GiB
+        const size_t per = (size_t)n_items * 2 * Dp * Dp * sizeof(float);
+        b.dyn_chunk = (int)std::max<size_t>(1, std::min<size_t>(ns - 1, ((size_t)1 << 30) / per));
+        if (const char *env = std::getenv("MSG_DYN_CHUNK"))  // smaller chunks (tests of the resume)
+            if (std::atoi(env) >= 1) b.dyn_chunk = std::min(b.dyn_chunk, std::atoi(env));
+        a.ustride = b.dyn_chunk;
+        a.renorm = factor_decay ? 1 : 0;
+        b.n_units = n_items * ((b.ncols + 15) / 16);
     } else {
         const int Dp = dense_padding(D);
         if (Dp == 0) throw ArgError{MSG_UNSUPPORTED, "dense path limited to D <= 512"};
@@ -526,7 +638,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     a.queue = b.queue.alloc<int>(1, stream);
     a.status = b.status.alloc<int>(1, stream);
     HIPCHECK(hipMemsetAsync(a.status, 0, sizeof(int), stream));
-    if (keep_state) {
+    if (b.keep_state) {
         const size_t per_unit = b.path == Path::Lanczos ? (size_t)2 * D : (size_t)2 * a.dpad * 16;
         a.state = b.state.alloc<float>((size_t)b.n_units * per_unit, stream);
     }
@@ -583,6 +695,112 @@ void run_chunk(Batch &b, int steps) {
     else HIPCHECK(launch_dense(a, da, b.slots, b.stream));
     if (b.timed) HIPCHECK(hipEventRecord(b.ctx->ev1, b.stream));
     b.s_next = a.s_end;
+}
+
+// Propagators of intervals [s0, s1) of a dynamic dense batch, into b.U with index
+// item * (s1 - s0) + (s - s0): the midpoint exponential exp(dt G(t + dt/2)) (Taylor 18
+// of dt G / 2^q, then q squarings; TaskHSGeneral.cpp:527-537) or, with RK4, the stage
+// map I + (K1 + 2 K2 + 2 K3 + K4)/6 of G at t, t + dt/2, t + dt (HSPropagator.cpp:104-129),
+// K1 = dt G0, K2 = dt Gm (1 + K1/2), K3 = dt Gm (1 + K2/2), K4 = dt G1 (1 + K3).
+void build_dyn_chunk(Batch &b, int s0, int s1) {
+    const Resolved &r = b.sys->r;
+    const int D = (int)r.D, Dp = b.a.dpad, nsc = s1 - s0;
+    const size_t DD = (size_t)D * D;
+    hipStream_t st = b.stream;
+    const bool rk4 = b.prm.method == MSG_METHOD_RK4;
+    const int nz = (int)b.dyn_terms.size(), stages = rk4 ? 3 : 1;
+    float *dU = b.U.p ? b.U.get<float>() : b.U.alloc<float>((size_t)b.n_items * b.dyn_chunk * 2 * Dp * Dp, st);
+    b.a.U = dU;  // the launch reads the chunk's propagators from here
+    const int total = b.n_items * nsc;
+    const int nbuf = rk4 ? 10 : 7;
+    const int sub = (int)std::max<size_t>(1, std::min<size_t>(total, ((size_t)1 << 30) / (nbuf * DD * sizeof(cd))));
+    DBuf bX, bX2, bX3, bX4, bB, bP, bQ, bc, bi, bd;
+    cd *dX = bX.alloc<cd>((size_t)stages * sub * DD, st);
+    cd *dX2 = bX2.alloc<cd>((size_t)sub * DD, st), *dX3 = bX3.alloc<cd>((size_t)sub * DD, st);
+    cd *dX4 = bX4.alloc<cd>((size_t)sub * DD, st), *dB = bB.alloc<cd>((size_t)sub * DD, st);
+    cd *dP = bP.alloc<cd>((size_t)sub * DD, st), *dQ = bQ.alloc<cd>((size_t)sub * DD, st);
+    double *dcoef = bc.alloc<double>((size_t)stages * sub * std::max(nz, 1), st);
+    int *ditem = bi.alloc<int>((size_t)stages * sub, st);
+    double *ddt = bd.alloc<double>((size_t)stages * sub, st);
+    std::vector<double> coef((size_t)stages * sub * std::max(nz, 1)), dts((size_t)stages * sub);
+    std::vector<int> items((size_t)stages * sub);
+    for (int q0 = 0; q0 < total; q0 += sub) {
+        const int nb = std::min(sub, total - q0);
+        double nrm = 0.0;
+        std::fill(coef.begin(), coef.end(), 0.0);
+        for (int q = 0; q < nb; ++q) {
+            const int it = (q0 + q) / nsc, s = s0 + (q0 + q) % nsc;
+            const double t0 = b.times[s], h = b.times[s + 1] - b.times[s];
+            const double *f = b.field_given ? &b.item_fields[3 * (size_t)it] : nullptr;
+            for (int g = 0; g < stages; ++g) {
+                const size_t e = (size_t)g * nb + q;  // stage-major
+                TimePoint tp;
+                tp.t = rk4 ? t0 + 0.5 * h * g : t0 + 0.5 * h;
+                double *c = &coef[e * std::max(nz, 1)];
+                for (const Term1 &t : r.t1) {
+                    if (!t.td) continue;
+                    double v[3];
+                    term1_value(t, f, &tp, v);
+                    for (int k = 0; k < nz; ++k)
+                        if (b.dyn_terms[k].first == t.s) c[k] += v[b.dyn_terms[k].second];
+                }
+                double bound = b.dyn_gnorm[it];
+                for (int k = 0; k < nz; ++k) bound += std::fabs(c[k]) * b.dyn_znorm[k];
+                nrm = std::max(nrm, h * bound);
+                items[e] = it;
+                dts[e] = h;
+            }
+        }
+        int sq = 0;
+        if (!rk4) {
+            while (nrm > 0.5) { nrm *= 0.5; ++sq; }
+            for (int q = 0; q < nb; ++q) dts[q] = std::ldexp(dts[q], -sq);
+        }
+        const size_t ne = (size_t)stages * nb;
+        HIPCHECK(hipMemcpyAsync(dcoef, coef.data(), ne * std::max(nz, 1) * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(ditem, items.data(), ne * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(ddt, dts.data(), ne * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHECK(launch_dyn_generator(b.dynG.p, b.dynZ.p, dcoef, ditem, ddt, nz, D, (int)ne, dX, st));
+        cd *res = nullptr;
+        if (!rk4) {
+            cd *p = dP, *qq = dQ;
+            taylor_batch(dX, dX2, dX3, dX4, dB, p, qq, D, nb, false, st);
+            for (int k = 0; k < sq; ++k) {
+                HIPCHECK(launch_zgemm(p, p, qq, D, nb, 1.0, 0.0, st));
+                std::swap(p, qq);
+            }
+            res = p;
+        } else {
+            const cd *a0 = dX, *am = dX + (size_t)nb * DD, *a1 = dX + (size_t)2 * nb * DD;
+            HIPCHECK(launch_zgemm(am, a0, dX2, D, nb, 0.5, 0.0, st, false, am));   // K2 = am + am K1 / 2
+            HIPCHECK(launch_zgemm(am, dX2, dX3, D, nb, 0.5, 0.0, st, false, am));  // K3 = am + am K2 / 2
+            HIPCHECK(launch_zgemm(a1, dX3, dX4, D, nb, 1.0, 0.0, st, false, a1));  // K4 = a1 + a1 K3
+            const double c1[4] = {1.0, 1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0};
+            HIPCHECK(launch_zlincomb(a0, dX2, dX3, dB, D, nb, c1, st));            // 1 + K1/6 + K2/3 + K3/3
+            const double c2[4] = {0.0, 1.0, 1.0 / 6.0, 0.0};
+            HIPCHECK(launch_zlincomb(dB, dX4, dX4, dP, D, nb, c2, st));            // + K4/6
+            res = dP;
+        }
+        HIPCHECK(launch_to_planes(res, dU + (size_t)q0 * 2 * Dp * Dp, D, Dp, (size_t)2 * Dp * Dp, nb, st));
+        HIPCHECK(hipStreamSynchronize(st));  // host staging reused by the next sub-batch
+    }
+}
+
+// Runs `steps` intervals; a dynamic dense batch builds its propagators chunk by chunk.
+void advance(Batch &b, int steps) {
+    if (!b.dyn_dense || b.nsamples <= 1) {
+        run_chunk(b, steps);
+        return;
+    }
+    const int last = b.nsamples - 1;
+    int left = steps;
+    while (left > 0 && b.s_next < last) {
+        const int s0 = b.s_next, s1 = std::min({last, s0 + b.dyn_chunk, s0 + left});
+        build_dyn_chunk(b, s0, s1);
+        b.a.ustride = s1 - s0;
+        run_chunk(b, s1 - s0);
+        left -= s1 - s0;
+    }
 }
 
 // Copies results; per-item sums over columns in fixed order.
@@ -833,6 +1051,7 @@ void propagate_uncoupled(msg_ctx *ctx, const msg_system *sys, const msg_run_para
     if (!why.empty()) throw ArgError{MSG_UNSUPPORTED, "uncoupled propagation: " + why};
     if (p->sampling != MSG_SAMPLING_DIRECT) throw ArgError{MSG_UNSUPPORTED, "uncoupled propagation uses direct sampling"};
     if (r.relax) throw ArgError{MSG_UNSUPPORTED, "relaxation couples the radicals"};
+    if (p->dynamics && r.has_td) throw ArgError{MSG_UNSUPPORTED, "uncoupled propagation takes a static Hamiltonian"};
     if (!sym_for(r, p)) throw ArgError{MSG_UNSUPPORTED, "uncoupled propagation needs symmetric recombination"};
     if (r.evec.empty()) throw ArgError{MSG_INVALID_ARG, "the system defines no initial state"};
     int nA = 1, nB = 1;
@@ -983,7 +1202,7 @@ msg_status msg_propagate(msg_ctx *ctx, const msg_system *sys, const msg_run_para
         Batch b;
         prepare(b, ctx, sys, params, n_items, fields, columns, n_columns, trace != nullptr, false, ctx->stream);
         b.timed = true;
-        run_chunk(b, b.nsamples);
+        advance(b, b.nsamples);
         collect(b, yields, trace);
         float ms = 0.f;
         HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -1052,7 +1271,7 @@ msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_param
                 prepare(b, ctx, sys, &p, n_items, fields, columns, n_columns, false, false, ctx->stream);
                 ctx->force = nullptr;
                 b.timed = true;
-                run_chunk(b, b.nsamples);
+                advance(b, b.nsamples);
                 collect(b, y.data(), nullptr);
                 float ms = 0.f;
                 HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -1080,6 +1299,8 @@ msg_status msg_step(msg_ctx *ctx, const msg_system *sys, const msg_run_params *p
     if (!ctx || !sys || !params || !factors || n_columns < 1) return fail(ctx, MSG_INVALID_ARG, "null argument");
     return guarded(ctx, [&] {
         HIPCHECK(hipSetDevice(ctx->device));
+        if (params->dynamics && sys->r.has_td)
+            throw ArgError{MSG_UNSUPPORTED, "msg_step takes a static Hamiltonian (dynamics=static)"};
         // one interval [0, dt] with the relaxation draws of step `step_index`
         msg_run_params p = *params;
         p.sampling = MSG_SAMPLING_GIVEN;
@@ -1211,6 +1432,9 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
         const Resolved &r = sys->r;
         if (p->calculation != MSG_CALC_YIELDS) throw ArgError{MSG_INVALID_ARG, "method=timeinf needs calculation=yields"};
         if (p->task != MSG_TASK_HSGENERAL) throw ArgError{MSG_INVALID_ARG, "time-infinity yields are an HSGeneral mode"};
+        if (p->dynamics)
+            throw ArgError{MSG_INVALID_ARG, "dynamic yields require finite-time propagation; method=timeinf is only "
+                                            "defined for static Hamiltonians"};
         if (r.relax)
             throw ArgError{MSG_UNSUPPORTED, "time-infinity yields with relaxation operators need the Liouville-space solve"};
         const int nobs = observable_count(r, p);
@@ -1462,6 +1686,8 @@ msg_status msg_server_submit(msg_server *srv, const msg_system *sys, const msg_r
         check_params(sys, params);
         if (params->calculation != MSG_CALC_YIELDS) throw ArgError{MSG_INVALID_ARG, "the server serves yield requests"};
         if (params->sampling != MSG_SAMPLING_DIRECT) throw ArgError{MSG_UNSUPPORTED, "the server uses direct sampling"};
+        if (params->dynamics && sys->r.has_td && choose_path(sys->r, params) == Path::Dense)
+            throw ArgError{MSG_UNSUPPORTED, "time-dependent requests on the dense path run through msg_propagate"};
         std::unique_ptr<Request> q(new Request());
         q->id = srv->next_id++;
         q->sys = sys;

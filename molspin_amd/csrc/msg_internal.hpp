@@ -44,11 +44,14 @@ struct StepArgs {
     const int *nuc_flags;  // [nnuc] device
     const float *coef; // [n_items][coef_stride]
     int coef_stride;
+    const float *dyn_head;  // [n_items][nsamples-1][8] head (w1, w2, shift) of each interval of a
+                            // time-dependent operator, or NULL (static: the head of `coef`)
     // dense path
     const float *U;    // [n_items][ustride][2][Dp*Dp] planes re, im, column-major (Dp = padded dim)
     int dpad;
     int ustride;       // propagators per item (2 when the last interval is shorter)
     int ulast;         // use the item's second propagator for interval nsamples-2
+    int ustep;         // one propagator per interval of the launch: item * ustride + (s - s_begin)
     // observables: electron-pair 4x4 complex matrices (row-major interleaved), device
     const float *obs;        // [nobs][32]
     // time grid

@@ -91,7 +91,9 @@ __device__ __forceinline__ void load_block(const float *p, bool herm, float2 m[4
 }
 
 // H x for the group nu whose value is x; neighbours come from the LDS planes.
-// H = w1.S1 + w2.S2 + shift + [E] + sum_n (diagonal s_n(...) + flip-flop blocks).
+// H = w1.S1 + w2.S2 + shift + [E] + sum_n (diagonal s_n(...) + flip-flop blocks);
+// w1, w2 and the shift come from the head `hd` (the item's block, or the interval's
+// head of a time-dependent operator), the rest from the item's block `co`.
 // Groups per LDS tile of the tiled streaming kernel: index bits 0-11 of nu.
 constexpr int kTileBits = 12;
 constexpr int kTileG = 1 << kTileBits;
@@ -102,12 +104,12 @@ static_assert(kTileG == kTileGroups, "tile size shared with the host");
 // kTileBits come from the LDS tile (LA/LB indexed by nu mod kTileG), the others
 // from the vector in global memory (GA/GB).
 template <int NN = -1, bool TILED = false>
-__device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict__ co, int nu, const C4 &x,
-                                      const float4 *LA, const float4 *LB, const float4 *GA = nullptr,
-                                      const float4 *GB = nullptr) {
-    float f1x = co[0], f1y = co[1], f1z = co[2];
-    float f2x = co[3], f2y = co[4], f2z = co[5];
-    float rr = co[6];
+__device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict__ co, const float *__restrict__ hd,
+                                      int nu, const C4 &x, const float4 *LA, const float4 *LB,
+                                      const float4 *GA = nullptr, const float4 *GB = nullptr) {
+    float f1x = hd[0], f1y = hd[1], f1z = hd[2];
+    float f2x = hd[3], f2y = hd[4], f2z = hd[5];
+    float rr = hd[6];
     C4 y;
 #pragma unroll
     for (int c = 0; c < 4; ++c) y.c[c] = make_float2(0.f, 0.f);
@@ -420,6 +422,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict_
             }
             if (end) break;
             const double dt = a.dts[s];
+            const float *hd = a.dyn_head ? a.dyn_head + ((size_t)item * (a.nsamples - 1) + s) * 8 : co;
             // ---- relaxation half-step before (HSPropagator.cpp:137)
             if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s])
 #pragma unroll
@@ -465,7 +468,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict_
                     for (int g = 0; g < G; ++g) {
                         const int nu = g * T + t;
                         const C4 vj = load_group(cA, cB, nu);
-                        w[g] = apply_h<Log2<ng>::value>(a, co, nu, vj, cA, cB);
+                        w[g] = apply_h<Log2<ng>::value>(a, co, hd, nu, vj, cA, cB);
                         if (j > 0) {
                             const C4 vp = DBUF ? load_group(pA, pB, nu) : load_group(gA, gA + D4, nu);
 #pragma unroll
@@ -664,6 +667,7 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
                 if (t == 0 && (!end || s == a.nsamples - 1)) record_sample(a, u, s, R, ys, yc);
                 if (end) break;
                 const double dt = a.dts[s];
+                const float *hd = a.dyn_head ? a.dyn_head + ((size_t)item * (a.nsamples - 1) + s) * 8 : co;
                 const double beta0 = sqrt(R[16]);
                 if (beta0 > 0.0) {
                     const float inv = (float)(1.0 / beta0);
@@ -715,7 +719,7 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
                                 for (int k = t; k < kTileG; k += T) {
                                     const int nu = tb + k;
                                     const C4 vj = load_group(tA, tB, k);
-                                    pass_a(nu, vj, apply_h<-1, true>(a, co, nu, vj, tA, tB, vA, vB), nullptr);
+                                    pass_a(nu, vj, apply_h<-1, true>(a, co, hd, nu, vj, tA, tB, vA, vB), nullptr);
                                 }
                             }
                         } else if (LV) {
@@ -723,12 +727,12 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
                             for (int g = 0; g < (LV ? 4 : 1); ++g) {
                                 const int nu = g * T + t;
                                 const C4 vj = load_group(vA, vB, nu);
-                                pass_a(nu, vj, apply_h(a, co, nu, vj, vA, vB), &wr[g]);
+                                pass_a(nu, vj, apply_h(a, co, hd, nu, vj, vA, vB), &wr[g]);
                             }
                         } else {
                             for (int nu = t; nu < ng; nu += T) {
                                 const C4 vj = load_group(vA, vB, nu);
-                                pass_a(nu, vj, apply_h(a, co, nu, vj, vA, vB), nullptr);
+                                pass_a(nu, vj, apply_h(a, co, hd, nu, vj, vA, vB), nullptr);
                             }
                         }
                         block_sum<2>(red, SCR());
@@ -1010,7 +1014,7 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
             }
             __syncthreads();
             // B' = U B: real MFMA form of the complex product
-            const int uw = (a.ulast && s == a.nsamples - 2) ? 1 : 0;
+            const int uw = a.ustep ? s - a.s_begin : ((a.ulast && s == a.nsamples - 2) ? 1 : 0);
             const float *Ur = a.U + ((size_t)item * a.ustride + uw) * 2 * Dp * Dp, *Ui = Ur + (size_t)Dp * Dp;
 #pragma unroll
             for (int k = 0; k < KT; ++k) {
@@ -1248,6 +1252,31 @@ __global__ __launch_bounds__(256) void k_sum_rows(const double *__restrict__ row
         __syncthreads();
     }
     if (t == 0) out[0] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// Time-dependent generators of the dense path (dynamics=dynamic):
+//   out_b = dt_b (G[item_b] + sum_k coef_b[k] Z_k)
+// with G the static generator of each item and Z_k the time-dependent operators
+// (-i S_a of the spins with time-dependent fields); n x n row-major, contiguous.
+// ---------------------------------------------------------------------------
+__global__ void k_dyn_generator(const double2 *__restrict__ G, const double2 *__restrict__ Z,
+                                const double *__restrict__ coef, const int *__restrict__ item,
+                                const double *__restrict__ dt, int nz, int n, size_t total,
+                                double2 *__restrict__ out) {
+    const size_t nn = (size_t)n * n;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t b = idx / nn, e = idx - b * nn;
+        double2 acc = G[(size_t)item[b] * nn + e];
+        for (int k = 0; k < nz; ++k) {
+            const double c = coef[b * nz + k];
+            const double2 z = Z[(size_t)k * nn + e];
+            acc.x = fma(c, z.x, acc.x);
+            acc.y = fma(c, z.y, acc.y);
+        }
+        const double h = dt[b];
+        out[idx] = make_double2(h * acc.x, h * acc.y);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1617,6 +1646,15 @@ hipError_t launch_timeinf_sum(const void *Pt, const void *Rt, const double *lam,
     hipLaunchKernelGGL(k_lorentz_rows, dim3(n), dim3(256), 0, st, (const double2 *)Pt, (const double2 *)Rt, lam, a, 1.0,
                        1.0, n, rows);
     hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, st, (const double *)rows, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_dyn_generator(const void *G, const void *Z, const double *coef, const int *item, const double *dt,
+                               int nz, int n, int batch, void *out, hipStream_t st) {
+    const size_t total = (size_t)batch * n * n;
+    const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_dyn_generator, dim3(blocks), dim3(256), 0, st, (const double2 *)G, (const double2 *)Z, coef,
+                       item, dt, nz, n, total, (double2 *)out);
     return hipGetLastError();
 }
 

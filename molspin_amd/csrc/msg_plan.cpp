@@ -4,8 +4,8 @@
 // Restates ResolveExecutionPlan (RunSection/General/HS/HSExecutionPlan.cpp:89-615)
 // for the keys of the static Hilbert-space path, with the reference's defaults
 // (HSExecutionPlan.h:33-117), aliases, value spellings and error messages.  Keys
-// that select physics outside this path (dynamic Hamiltonians, powder grids,
-// secular approximations, spin polarisation / CIDSP output, pulse sequences)
+// that select physics outside this path (powder grids, secular approximations,
+// spin polarisation / CIDSP output, pulse sequences)
 // are reported as MSG_UNSUPPORTED with the key named, so a caller can fall back
 // to its CPU path.  Property splitting follows ObjectParser
 // (MSDParser/ObjectParser.cpp:37-86): ';' ends a value unless inside ( ).
@@ -206,7 +206,7 @@ void resolve(const Props &p, msg_plan &out) {
     if (one_of(v, {"timedependent", "time-dependent"})) v = "dynamic";
     const bool dynamic = v == "dynamic";
     if (!dynamic && v != "static") throw invalid("dynamics must be static or dynamic");
-    if (dynamic) off("dynamics=dynamic is not on the device path");
+    out.params.dynamics = dynamic ? 1 : 0;
 
     v = "timeevolution";
     p.str({"calculation", "calculationmode", "calculation_mode"}, v);

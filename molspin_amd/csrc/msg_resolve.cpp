@@ -377,6 +377,20 @@ std::string resolve_system(const msg_system_desc *d, Resolved &r) {
                 for (int i = 0; i < 9; ++i) t.Mz[i] = sc * G[i];
                 for (int a = 0; a < 3; ++a)
                     t.c[a] = t.Mz[3 * a] * in.field[0] + t.Mz[3 * a + 1] * in.field[1] + t.Mz[3 * a + 2] * in.field[2];
+                if (in.time_dependence != MSG_TD_NONE) {
+                    if (in.time_dependence != MSG_TD_LINEAR && in.time_dependence != MSG_TD_CIRCULAR)
+                        return "unknown field time dependence (linearpolarized or circularpolarized)";
+                    if (in.field_swept) return "a time-dependent Zeeman field cannot be the swept field";
+                    t.td = in.time_dependence;
+                    t.td_perp = in.td_perpendicular != 0;
+                    t.td_freq = in.td_frequency;
+                    t.td_phase = in.td_phase;
+                    for (int a = 0; a < 3; ++a) {
+                        t.td_axis[a] = in.td_axis[a];
+                        t.field[a] = in.field[a];
+                    }
+                    r.has_td = true;
+                }
                 r.t1.push_back(t);
             }
         } else if (in.type == MSG_INTERACTION_DOUBLESPIN || in.type == MSG_INTERACTION_EXCHANGE) {
@@ -455,6 +469,8 @@ std::string resolve_system(const msg_system_desc *d, Resolved &r) {
         r.relax_rate = rl.rate;
     }
     if (d->n_transitions > 0) r.first_rate = d->transitions[0].rate;
+    for (const Term1 &t : r.t1)
+        if (t.td && t.s != r.e1 && t.s != r.e2) r.td_nuclear = true;
     // direct factors of rho0 = rho_e (x) 1/Z (HSStatePreparation.cpp:77-165)
     if (d->n_initial >= 1) {
         if (d->n_initial > 16) return "too many initial states";
@@ -491,8 +507,48 @@ std::string resolve_system(const msg_system_desc *d, Resolved &r) {
     return "";
 }
 
+void td_field(const Term1 &t, double time, double B[3]) {
+    if (t.td == MSG_TD_LINEAR) {  // linearpolarized: the field scaled by cos(w t + phase)
+        const double c = std::cos(t.td_freq * time + t.td_phase);
+        for (int a = 0; a < 3; ++a) B[a] = t.field[a] * c;
+        return;
+    }
+    // circularpolarized: Rodrigues rotation by x = w t + phase about the axis as given
+    const double *u = t.td_axis;
+    const double x = time * t.td_freq + t.td_phase, sx = std::sin(x), hx = 2.0 * std::sin(0.5 * x) * std::sin(0.5 * x);
+    const double W[3][3] = {{0.0, -u[2], u[1]}, {u[2], 0.0, -u[0]}, {-u[1], u[0], 0.0}};
+    double b[3] = {t.field[0], t.field[1], t.field[2]};
+    if (t.td_perp) {
+        const double d = u[0] * b[0] + u[1] * b[1] + u[2] * b[2];
+        for (int a = 0; a < 3; ++a) b[a] -= d * u[a];
+    }
+    for (int i = 0; i < 3; ++i) {
+        double acc = b[i];
+        for (int j = 0; j < 3; ++j) {
+            double w2 = 0.0;
+            for (int k = 0; k < 3; ++k) w2 += W[i][k] * W[k][j];
+            acc += (sx * W[i][j] + hx * w2) * b[j];
+        }
+        B[i] = acc;
+    }
+}
+
+bool term1_value(const Term1 &t, const double *field, const TimePoint *tp, double v[3]) {
+    if (t.td && tp) {
+        if (tp->skip) return false;
+        double B[3];
+        td_field(t, tp->t, B);
+        for (int a = 0; a < 3; ++a) v[a] = t.Mz[3 * a] * B[0] + t.Mz[3 * a + 1] * B[1] + t.Mz[3 * a + 2] * B[2];
+        return true;
+    }
+    for (int a = 0; a < 3; ++a)
+        v[a] = (t.swept && field) ? t.Mz[3 * a] * field[0] + t.Mz[3 * a + 1] * field[1] + t.Mz[3 * a + 2] * field[2]
+                                  : t.c[a];
+    return true;
+}
+
 void fast_coefficients(const Resolved &r, const double *field, std::vector<float> &out, int &op_flags,
-                       int nuc_flags[kMaxNuclei]) {
+                       int nuc_flags[kMaxNuclei], const TimePoint *tp) {
     const int N = (int)r.nuc.size();
     std::vector<double> c((size_t)kCoefHead + (size_t)N * kNucStride, 0.0);
     std::vector<int> nucpos(r.nspins, -1);
@@ -506,9 +562,7 @@ void fast_coefficients(const Resolved &r, const double *field, std::vector<float
     cd E[16] = {};
     for (const Term1 &t : r.t1) {
         double v[3];
-        for (int a = 0; a < 3; ++a)
-            v[a] = (t.swept && field) ? t.Mz[3 * a] * field[0] + t.Mz[3 * a + 1] * field[1] + t.Mz[3 * a + 2] * field[2]
-                                      : t.c[a];
+        if (!term1_value(t, field, tp, v)) continue;
         if (t.s == r.e1) for (int a = 0; a < 3; ++a) c[a] += v[a];
         else if (t.s == r.e2) for (int a = 0; a < 3; ++a) c[3 + a] += v[a];
         else {
@@ -566,7 +620,23 @@ void fast_coefficients(const Resolved &r, const double *field, std::vector<float
     for (size_t i = 0; i < c.size(); ++i) out[i] = (float)c[i];
 }
 
-void dense_hamiltonian(const Resolved &r, const double *field, std::vector<cd> &H) {
+void spin_operator_dense(const Resolved &r, int spin, int axis, std::vector<cd> &H) {
+    const int64_t D = r.D;
+    H.assign((size_t)(D * D), 0.0);
+    int64_t stride = 1;
+    for (int i = r.nspins - 1; i > spin; --i) stride *= r.mult[i];
+    const int m = r.mult[spin];
+    const std::vector<cd> S = spin_op(m, axis);
+    for (int64_t row = 0; row < D; ++row) {
+        const int dr = (int)((row / stride) % m);
+        for (int dc = 0; dc < m; ++dc) {
+            const cd x = S[(size_t)dr * m + dc];
+            if (x != 0.0) H[(size_t)(r.perm[row] * D + r.perm[row + (int64_t)(dc - dr) * stride])] += x;
+        }
+    }
+}
+
+void dense_hamiltonian(const Resolved &r, const double *field, std::vector<cd> &H, const TimePoint *tp) {
     const int64_t D = r.D;
     H.assign((size_t)(D * D), 0.0);
     std::vector<int64_t> stride(r.nspins);
@@ -588,9 +658,7 @@ void dense_hamiltonian(const Resolved &r, const double *field, std::vector<cd> &
         add(row, row, r.shift);
         for (const Term1 &t : r.t1) {
             double v[3];
-            for (int a = 0; a < 3; ++a)
-                v[a] = (t.swept && field) ? t.Mz[3 * a] * field[0] + t.Mz[3 * a + 1] * field[1] + t.Mz[3 * a + 2] * field[2]
-                                          : t.c[a];
+            if (!term1_value(t, field, tp, v)) continue;
             const int m = r.mult[t.s], dr = digit(row, t.s);
             for (int a = 0; a < 3; ++a) {
                 if (v[a] == 0.0) continue;
@@ -664,9 +732,7 @@ void subsystem_hamiltonian(const Resolved &r, const std::vector<int> &spins, con
             const int i = pos[t.s];
             if (i < 0) continue;
             double v[3];
-            for (int a = 0; a < 3; ++a)
-                v[a] = (t.swept && field) ? t.Mz[3 * a] * field[0] + t.Mz[3 * a + 1] * field[1] + t.Mz[3 * a + 2] * field[2]
-                                          : t.c[a];
+            term1_value(t, field, nullptr, v);
             const int m = r.mult[t.s], dr = digit(row, i);
             for (int a = 0; a < 3; ++a) {
                 if (v[a] == 0.0) continue;

@@ -24,7 +24,24 @@ struct Term1 {       // H += c . S_s, c = Mz B (swept) or fixed c
     bool swept;
     double c[3];
     double Mz[9];
+    // time-dependent Zeeman field (MSG_TD_*): B(t) from the field as written
+    int td = 0;
+    bool td_perp = false;
+    double td_freq = 1.0, td_phase = 0.0, td_axis[3] = {0.0, 0.0, 1.0}, field[3] = {0.0, 0.0, 0.0};
 };
+
+// Time point of a dynamics=dynamic evaluation (SpinSpace::StaticHamiltonian +
+// DynamicHamiltonian at time t, SpinSpace_hamiltonians.cpp:1590-1740): time-dependent
+// Zeeman terms take their field at `t`, or are left out (`skip`: the static part).
+// Without a time point (dynamics=static) they keep the field as written.
+struct TimePoint {
+    double t = 0.0;
+    bool skip = false;
+};
+
+// Field of a time-dependent Zeeman term at time t (Interaction.cpp:1900-1933).
+void td_field(const Term1 &t, double time, double B[3]);
+
 struct Term2 {       // H += sum_ab M_ab S_{s1,a} S_{s2,b}
     int s1, s2;
     double M[9];
@@ -43,6 +60,8 @@ struct Resolved {
     bool pair_ok = false;         // two spin-1/2 electrons, electron-only states
     bool fast_ok = false;         // + all nuclei spin-1/2 and no nucleus-nucleus terms
     bool has_ee = false;          // electron-electron couplings present
+    bool has_td = false;          // time-dependent Zeeman terms present
+    bool td_nuclear = false;      // ... acting on a nucleus
     std::string why_not_fast;
     // observables
     std::vector<Mat4> P_states;        // every State object (electron-pair projector)
@@ -70,10 +89,17 @@ std::string resolve_system(const msg_system_desc *d, Resolved &r);
 
 // Per-item coefficient block of the matrix-free operator (layout in msg_internal.hpp).
 void fast_coefficients(const Resolved &r, const double *field, std::vector<float> &out, int &op_flags,
-                       int nuc_flags[kMaxNuclei]);
+                       int nuc_flags[kMaxNuclei], const TimePoint *tp = nullptr);
+
+// One-spin coefficient c of term t (H += c . S) at the item's field and time point;
+// false when the term is left out (time-dependent term of a static part).
+bool term1_value(const Term1 &t, const double *field, const TimePoint *tp, double v[3]);
+
+// S_axis of spin `spin` (x 1 on the others), dense D x D in device order, row-major.
+void spin_operator_dense(const Resolved &r, int spin, int axis, std::vector<cd> &H);
 
 // Dense D x D Hamiltonian (rad/ns) in device order, row-major complex.
-void dense_hamiltonian(const Resolved &r, const double *field, std::vector<cd> &H);
+void dense_hamiltonian(const Resolved &r, const double *field, std::vector<cd> &H, const TimePoint *tp = nullptr);
 
 // Split of a radical pair without electron-electron couplings into the two
 // uncoupled radicals (TaskStaticHSDirectYieldsSymmUncoupled.cpp:100-150): A = e1

@@ -79,11 +79,19 @@ class SpinSystem:
         return x if isinstance(x, int) else self.spin_index(x)
 
     def add_zeeman(self, spins, field, prefactor=1.0, common_prefactor=True,
-                   ignore_tensors=False, swept=False, name=None):
+                   ignore_tensors=False, swept=False, name=None, time_dependence=None,
+                   frequency=1.0, phase=0.0, axis=(0.0, 0.0, 1.0), perpendicular=False):
+        """Zeeman interaction; time_dependence None, "linearpolarized" or
+        "circularpolarized" (fieldtype, Interaction.cpp:500-560) with frequency (rad/ns),
+        phase (rad), axis and perpendicularoscillations."""
+        td = {None: abi.MSG_TD_NONE, "linearpolarized": abi.MSG_TD_LINEAR,
+              "circularpolarized": abi.MSG_TD_CIRCULAR}[time_dependence]
         self.interactions.append(dict(
             type=abi.MSG_INTERACTION_ZEEMAN, g1=[self._idx(s) for s in spins], g2=[],
             field=[float(x) for x in field], tensor=None, prefactor=float(prefactor),
-            common=common_prefactor, ignore=ignore_tensors, swept=swept, name=name))
+            common=common_prefactor, ignore=ignore_tensors, swept=swept, name=name,
+            td=td, freq=float(frequency), phase=float(phase), axis=[float(x) for x in axis],
+            perp=bool(perpendicular)))
 
     def add_hyperfine(self, group1, group2, tensor=None, prefactor=1.0, common_prefactor=True,
                       ignore_tensors=False, name=None, exchange=False):
@@ -176,6 +184,12 @@ class Packed:
             d.prefactor = it["prefactor"]
             for a in range(3):
                 d.field[a] = it["field"][a]
+            d.time_dependence = it.get("td", abi.MSG_TD_NONE)
+            d.td_frequency = it.get("freq", 1.0)
+            d.td_phase = it.get("phase", 0.0)
+            d.td_perpendicular = int(it.get("perp", False))
+            for a, x in enumerate(it.get("axis", (0.0, 0.0, 1.0))):
+                d.td_axis[a] = x
             if it["tensor"] is not None:
                 iso, m = it["tensor"]
                 d.tensor_present = 1

@@ -313,7 +313,40 @@ void spin_t_matrix(const msg_spin_desc &sp, bool ignore, double g[9]) {
     for (int i = 0; i < 9; ++i) g[i] = ignore ? (i % 4 == 0 ? 1.0 : 0.0) : sp.tensor[i];
 }
 
-CSR build_hamiltonian(const System &S, const double *field_override) {
+// Field of a time-dependent Zeeman interaction at time t (Interaction.cpp:1900-1933):
+// linear  B = v cos(w t + phase);
+// circular B = R (v or v - (a.v) a), R = 1 + sin(x) W + 2 sin^2(x/2) W^2, x = w t + phase,
+// W the cross-product matrix of the axis a as given.
+void td_field(const msg_interaction_desc &in, double t, double B[3]) {
+    const double *v = in.field;
+    if (in.time_dependence == MSG_TD_LINEAR) {
+        const double c = std::cos(in.td_frequency * t + in.td_phase);
+        for (int a = 0; a < 3; ++a) B[a] = v[a] * c;
+        return;
+    }
+    const double *ax = in.td_axis;
+    const double x = t * in.td_frequency + in.td_phase;
+    const double W[9] = {0.0, -ax[2], ax[1], ax[2], 0.0, -ax[0], -ax[1], ax[0], 0.0};
+    double R[9];
+    const double sx = std::sin(x), h = 2.0 * std::sin(0.5 * x) * std::sin(0.5 * x);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double w2 = 0.0;
+            for (int k = 0; k < 3; ++k) w2 += W[3 * i + k] * W[3 * k + j];
+            R[3 * i + j] = (i == j ? 1.0 : 0.0) + sx * W[3 * i + j] + h * w2;
+        }
+    double b[3] = {v[0], v[1], v[2]};
+    if (in.td_perpendicular) {
+        const double d = ax[0] * v[0] + ax[1] * v[1] + ax[2] * v[2];
+        for (int a = 0; a < 3; ++a) b[a] -= d * ax[a];
+    }
+    for (int i = 0; i < 3; ++i) B[i] = R[3 * i] * b[0] + R[3 * i + 1] * b[1] + R[3 * i + 2] * b[2];
+}
+
+// H at a field override; `time` (dynamics=dynamic) puts time-dependent Zeeman fields at
+// their value at that time (SpinSpace::StaticHamiltonian + DynamicHamiltonian,
+// SpinSpace_hamiltonians.cpp:1590-1740); without it they keep the field as written.
+CSR build_hamiltonian(const System &S, const double *field_override, const double *time = nullptr) {
     Triplets T(S.D);
     const msg_system_desc *d = S.d;
     Mat ops[3][8];  // cached single-spin matrices per multiplicity (<= 8)
@@ -334,7 +367,12 @@ CSR build_hamiltonian(const System &S, const double *field_override) {
         const int32_t *g2 = d->index_pool + in.group2_offset;
         if (in.type == MSG_INTERACTION_ZEEMAN) {
             // S . (G B) (SpinSpace_hamiltonians.cpp:450-479); G = spin tensor unless ignoretensors.
+            double Bt[3];
             const double *B = (in.field_swept && field_override) ? field_override : in.field;
+            if (time && in.time_dependence != MSG_TD_NONE) {
+                td_field(in, *time, Bt);
+                B = Bt;
+            }
             for (int k = 0; k < in.group1_count; ++k) {
                 const int s = g1[k];
                 double G[9];
@@ -650,9 +688,9 @@ struct ItemModel {
     double relax_rate = 0.0;
 };
 
-ItemModel make_model(const System &S, const double *field, int method) {
+ItemModel make_model(const System &S, const double *field, int method, const double *time = nullptr) {
     ItemModel m;
-    const CSR H = build_hamiltonian(S, field);
+    const CSR H = build_hamiltonian(S, field, time);
     const CSR K = reaction_operator(S);
     Triplets T(S.D);
     for (int64_t r = 0; r < H.n; ++r)
@@ -673,9 +711,11 @@ ItemModel make_model(const System &S, const double *field, int method) {
 
 // Propagates columns [c0, c1) of B0 over the grid; adds sum-over-columns
 // observables into vals ((nsteps+1) x nobs).
+// dynamic (dynamics=dynamic, TaskHSGeneral.cpp:487-537): each interval uses the model at
+// its midpoint, or with RK4 the stages at t, t+dt/2, t+dt; `field` is the item's field.
 void run_block(const System &S, const ItemModel &M, const ObsSet &O, const std::vector<double> &times,
                int64_t nsamples, const Mat &B0, int c0, int c1, int method, int krylov, uint64_t seed,
-               uint32_t item, std::vector<double> &vals) {
+               uint32_t item, std::vector<double> &vals, bool dynamic = false, const double *field = nullptr) {
     const int64_t D = S.D;
     const int nobs = (int)O.ops.size();
     const int ncol = c1 - c0;
@@ -698,6 +738,20 @@ void run_block(const System &S, const ItemModel &M, const ObsSet &O, const std::
         if (s + 1 >= nsamples) break;
         const double dt = times[s + 1] - times[s];
         const uint32_t thr = M.has_relax ? flip_threshold(M.relax_rate, 0.5 * dt) : 0u;
+        ItemModel Mt[3];  // dynamic: midpoint (or RK4 start, midpoint, end)
+        const ItemModel *Gs[3] = {&M, &M, &M};
+        if (dynamic) {
+            const double tm = times[s] + 0.5 * dt;
+            if (method == MSG_METHOD_RK4) {
+                const double ts[3] = {times[s], tm, times[s + 1]};
+                for (int k = 0; k < 3; ++k) { Mt[k] = make_model(S, field, method, &ts[k]); Gs[k] = &Mt[k]; }
+            } else {
+                Mt[1] = make_model(S, field, method, &tm);
+                Gs[0] = Gs[1] = Gs[2] = &Mt[1];
+                Udt = -1.0;  // a new propagator every interval
+            }
+        }
+        const ItemModel &Mm = *Gs[1];
         for (int c = 0; c < ncol; ++c) {
             cd *b = &B[(size_t)c * D];
             // StepStochastic: R(dt/2) Step(dt) R(dt/2) (HSPropagator.cpp:131-140)
@@ -705,11 +759,11 @@ void run_block(const System &S, const ItemModel &M, const ObsSet &O, const std::
                 M.U.matvec(b, tmp.data()); std::copy(tmp.begin(), tmp.end(), b);
             }
             if (method == MSG_METHOD_KRYLOV) {
-                krylov_expm_general(M.G, b, dt, krylov, y.data());
+                krylov_expm_general(Mm.G, b, dt, krylov, y.data());
                 std::copy(y.begin(), y.end(), b);
             } else if (method == MSG_METHOD_EXPONENTIAL) {
                 if (dt != Udt) {
-                    U = expm(scale(M.Gdense, dt));
+                    U = expm(scale(Mm.Gdense, dt));
                     Udt = dt;
                 }
                 for (int64_t i = 0; i < D; ++i) {
@@ -718,14 +772,14 @@ void run_block(const System &S, const ItemModel &M, const ObsSet &O, const std::
                     y[i] = acc;
                 }
                 std::copy(y.begin(), y.end(), b);
-            } else {  // RK4 (HSPropagator.cpp:80-90)
-                M.G.matvec(b, k1.data());
+            } else {  // RK4 (HSPropagator.cpp:80-90; time-dependent stages :104-129)
+                Gs[0]->G.matvec(b, k1.data());
                 for (int64_t i = 0; i < D; ++i) tmp[i] = b[i] + 0.5 * dt * k1[i];
-                M.G.matvec(tmp.data(), k2.data());
+                Gs[1]->G.matvec(tmp.data(), k2.data());
                 for (int64_t i = 0; i < D; ++i) tmp[i] = b[i] + 0.5 * dt * k2[i];
-                M.G.matvec(tmp.data(), k3.data());
+                Gs[1]->G.matvec(tmp.data(), k3.data());
                 for (int64_t i = 0; i < D; ++i) tmp[i] = b[i] + dt * k3[i];
-                M.G.matvec(tmp.data(), k4.data());
+                Gs[2]->G.matvec(tmp.data(), k4.data());
                 for (int64_t i = 0; i < D; ++i) b[i] += (dt / 6.0) * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
             }
             if (M.has_relax && flip_draw(seed, item, (uint32_t)(c0 + c), (uint32_t)s, 1u, thr)) {
@@ -915,7 +969,8 @@ int ref_run_hsgeneral(const msg_system_desc *sys, const msg_run_params *p, int32
                 if (c1 <= c0) continue;
                 std::vector<double> v((size_t)ns * nobs, 0.0);
                 try {
-                    run_block(S, models[it], O, times, ns, B0, c0, c1, method, kry, p->seed, (uint32_t)it, v);
+                    run_block(S, models[it], O, times, ns, B0, c0, c1, method, kry, p->seed, (uint32_t)it, v,
+                              p->dynamics != 0, fields ? fields + 3 * it : nullptr);
                 } catch (const std::exception &e) {
 #pragma omp critical
                     err = e.what();
@@ -968,7 +1023,7 @@ int ref_run_columns(const msg_system_desc *sys, const msg_run_params *p, const d
             const int c0 = col_begin + (int)((int64_t)ncol * blk / blocks);
             const int c1 = col_begin + (int)((int64_t)ncol * (blk + 1) / blocks);
             std::vector<double> v((size_t)ns * nobs, 0.0);
-            if (c1 > c0) run_block(S, M, O, times, ns, B0, c0, c1, method, kry, p->seed, 0u, v);
+            if (c1 > c0) run_block(S, M, O, times, ns, B0, c0, c1, method, kry, p->seed, 0u, v, p->dynamics != 0, field);
 #pragma omp critical
             for (size_t k = 0; k < v.size(); ++k) total[k] += v[k];
         }

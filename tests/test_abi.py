@@ -30,11 +30,36 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(backend.EXPORTS)
 
 
-def test_struct_layouts_match_header_sizes():
-    # sizes of the POD structs as the C compiler lays them out (natural alignment)
-    assert C.sizeof(abi.SpinDesc) == 16 + 72
-    assert C.sizeof(abi.InteractionDesc) == 40 + 8 + 24 + 72
-    assert C.sizeof(abi.RunParams) == 24 + 16 + 8 + 8
+STRUCTS = [("msg_spin_desc", "SpinDesc"), ("msg_interaction_desc", "InteractionDesc"),
+           ("msg_state_desc", "StateDesc"), ("msg_transition_desc", "TransitionDesc"),
+           ("msg_relaxation_desc", "RelaxationDesc"), ("msg_system_desc", "SystemDesc"),
+           ("msg_run_params", "RunParams"), ("msg_plan", "Plan"), ("msg_launch_shape", "LaunchShape")]
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes mirror has the size and field offsets the C compiler gives the
+    header's struct (gcc on include/molspin_gpu.h)."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "molspin_gpu.h"', 'int main(void) {']
+    for cname, pyname in STRUCTS:
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (pyname, cname))
+        for f in getattr(abi, pyname)._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (pyname, f[0], cname, f[0]))
+    lines += ['return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in out:
+        if not line:
+            continue
+        pyname, field, value = line.split()
+        cls = getattr(abi, pyname)
+        got = C.sizeof(cls) if field == "size" else getattr(cls, field).offset
+        assert got == int(value), (pyname, field, got, value)
 
 
 def test_time_grid_without_device():

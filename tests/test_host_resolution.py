@@ -17,11 +17,13 @@ from oracle import binding as ref
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = [os.path.join(ROOT, "tests", "host_shim.cpp"), os.path.join(ROOT, "molspin_amd", "csrc", "msg_resolve.cpp")]
+HEADERS = [os.path.join(ROOT, "include", "molspin_gpu.h"), os.path.join(ROOT, "molspin_amd", "csrc", "msg_resolve.hpp"),
+           os.path.join(ROOT, "molspin_amd", "csrc", "msg_internal.hpp")]
 
 
 @pytest.fixture(scope="module")
 def shim():
-    h = hashlib.sha1(b"".join(open(p, "rb").read() for p in SRC)).hexdigest()[:12]
+    h = hashlib.sha1(b"".join(open(p, "rb").read() for p in SRC + HEADERS)).hexdigest()[:12]
     out = "/tmp/msg_host_shim_%s.so" % h
     if not os.path.exists(out):
         subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-o", out + ".tmp"] + SRC, check=True)

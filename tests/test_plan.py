@@ -100,8 +100,10 @@ def test_valid_but_off_the_device_path():
     assert p.params.method == abi.MSG_METHOD_UNCOUPLED
     p = resolve("type=HSGeneral;propagationmethod=rk4;")
     assert p.params.method == abi.MSG_METHOD_RK4 and p.propagation == abi.MSG_PROP_RK4
-    for text in ("dynamics=dynamic;calculation=yields;",
-                 "calculation=timeevolution;observables=spins;spinlist=E;",
+    p = resolve("type=HSGeneral;dynamics=timedependent;calculation=yields;")
+    assert p.params.dynamics == 1
+    assert resolve("type=HSGeneral;").params.dynamics == 0
+    for text in ("calculation=timeevolution;observables=spins;spinlist=E;",
                  "calculation=yields;approximation=secular;hamiltonianh0list=H0;",
                  "calculation=timeevolution;hamiltonianh0list=H0;powdergrid=uniform;powdersamplingpoints=8;",
                  "krylovsize=64;", 'pulsesequence=["p1 10"];'):
