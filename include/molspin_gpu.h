@@ -232,6 +232,37 @@ typedef struct msg_plan {
 msg_status msg_resolve_plan(const char *task_properties, msg_plan *plan, char *error, int32_t error_len);
 
 /* ------------------------------------------------------------------------- */
+/* MSD front-end (host only)                                                 */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * Compiles the text of an MSD input file: one SpinSystem (Spin, Interaction, State,
+ * Transition, Operator relaxationdephasing, Properties initialstate), the Settings
+ * (steps; RotateVector, FibonacciSphere and LogSpace actions on one Zeeman field or
+ * prefactor) and the Run tasks.  It replaces MSDParser::Parse + RunSection set-up
+ * (MSDParser/MSDParser.cpp:50-270, main.cpp:600-680) for this path: the RunSection
+ * step loop (tasks at step i, then Action::Step(i + 1), grid actions installing point
+ * zero before step 1) is replayed on the host, so the swept field of every step is
+ * known and the whole sweep runs as one msg_propagate batch, one item per step.
+ * Objects off this path return MSG_UNSUPPORTED naming them; malformed input
+ * MSG_INVALID_ARG.  Host-only: needs no device.
+ */
+typedef struct msg_msd msg_msd;
+msg_status msg_msd_compile(const char *text, msg_msd **out, char *error, int32_t error_len);
+/* The compiled system, for msg_system_upload; valid until msg_msd_destroy.  A swept
+ * Zeeman interaction has field_swept set and its prefactor folded into the fields. */
+msg_status msg_msd_system(const msg_msd *msd, const msg_system_desc **desc);
+/* Calculation steps; *swept = 1 when an action drives a Zeeman field, and then
+ * `fields` (n_steps x 3, may be NULL) receives each step's field (prefactor x field). */
+msg_status msg_msd_steps(const msg_msd *msd, int32_t *n_steps, int32_t *swept, double *fields);
+msg_status msg_msd_task_count(const msg_msd *msd, int32_t *n);
+/* Task `index` in file order: its name, its plan (msg_resolve_plan of its properties)
+ * and the resolver's status and message, which this call returns. */
+msg_status msg_msd_task(const msg_msd *msd, int32_t index, msg_plan *plan, char *name, int32_t name_len,
+                        char *why, int32_t why_len);
+msg_status msg_msd_destroy(msg_msd *msd);
+
+/* ------------------------------------------------------------------------- */
 /* Context and system handles                                               */
 /* ------------------------------------------------------------------------- */
 

@@ -20,6 +20,7 @@ EXPORTS = [
     "msg_time_grid", "msg_propagate", "msg_step", "msg_server_create", "msg_server_destroy",
     "msg_server_submit", "msg_server_pump", "msg_server_collect", "msg_last_kernel_time",
     "msg_trace_samples", "msg_symmetric_decay", "msg_resolve_plan", "msg_autotune", "msg_time_infinity",
+    "msg_msd_compile", "msg_msd_system", "msg_msd_steps", "msg_msd_task_count", "msg_msd_task", "msg_msd_destroy",
 ]
 
 
@@ -59,6 +60,12 @@ def lib():
         L.msg_trace_samples.argtypes = [vp, C.c_int32, C.c_int32, C.c_uint64, C.c_int32, vp]
         L.msg_symmetric_decay.argtypes = [vp, vp, C.c_int32, vp, C.c_double, vp]
         L.msg_time_infinity.argtypes = [vp, vp, P(abi.RunParams), C.c_int32, vp, vp, C.c_int32, vp]
+        L.msg_msd_compile.argtypes = [C.c_char_p, P(vp), C.c_char_p, C.c_int32]
+        L.msg_msd_system.argtypes = [vp, P(P(abi.SystemDesc))]
+        L.msg_msd_steps.argtypes = [vp, P(C.c_int32), P(C.c_int32), vp]
+        L.msg_msd_task_count.argtypes = [vp, P(C.c_int32)]
+        L.msg_msd_task.argtypes = [vp, C.c_int32, P(abi.Plan), C.c_char_p, C.c_int32, C.c_char_p, C.c_int32]
+        L.msg_msd_destroy.argtypes = [vp]
         L.msg_resolve_plan.argtypes = [C.c_char_p, P(abi.Plan), C.c_char_p, C.c_int32]
         L.msg_autotune.argtypes = [vp, vp, P(abi.RunParams), C.c_int32, vp, vp, C.c_int32, C.c_int32,
                                    P(abi.LaunchShape)]
@@ -77,6 +84,57 @@ def _columns(columns):
     input already has that layout)."""
     cc = np.asfortranarray(columns, dtype=np.complex128)
     return cc.reshape(-1, order="F").view(np.float64), cc.shape[1]
+
+
+class Msd:
+    """A compiled MSD file (msg_msd_compile): `packed`-like system handle for
+    Context.upload, the per-step swept fields and the tasks with their plans."""
+
+    def __init__(self, text):
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        st = lib().msg_msd_compile(text.encode(), C.byref(h), err, 512)
+        if st != abi.MSG_OK:
+            raise BackendError(st, err.value.decode())
+        self.h = h
+        d = C.POINTER(abi.SystemDesc)()
+        lib().msg_msd_system(self.h, C.byref(d))
+        self.desc = d.contents
+        n, sw = C.c_int32(), C.c_int32()
+        lib().msg_msd_steps(self.h, C.byref(n), C.byref(sw), None)
+        self.steps = n.value
+        self.fields = None
+        if sw.value:
+            f = np.zeros(3 * n.value)
+            lib().msg_msd_steps(self.h, C.byref(n), C.byref(sw), _ptr(f))
+            self.fields = f.reshape(-1, 3)
+        cnt = C.c_int32()
+        lib().msg_msd_task_count(self.h, C.byref(cnt))
+        self.tasks = []
+        for i in range(cnt.value):
+            plan = abi.Plan()
+            name = C.create_string_buffer(256)
+            why = C.create_string_buffer(512)
+            st = lib().msg_msd_task(self.h, i, C.byref(plan), name, 256, why, 512)
+            self.tasks.append(dict(name=name.value.decode(), status=st, plan=plan, why=why.value.decode()))
+        d = self.desc
+        self.dim = 1
+        for i in range(d.n_spins):
+            self.dim *= d.spins[i].multiplicity
+
+    def ref(self):
+        return C.byref(self.desc)
+
+    def close(self):
+        if self.h:
+            lib().msg_msd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def resolve_plan(task_properties):

@@ -17,102 +17,14 @@
 #include <vector>
 
 #include "../../include/molspin_gpu.h"
+#include "msg_props.hpp"
 
 namespace {
 
-std::string trim(const std::string &s) {
-    size_t b = 0, e = s.size();
-    while (b < e && std::isspace((unsigned char)s[b])) ++b;
-    while (e > b && std::isspace((unsigned char)s[e - 1])) --e;
-    return s.substr(b, e - b);
-}
-
-std::string lower(std::string s) {
-    for (char &c : s) c = (char)std::tolower((unsigned char)c);
-    return s;
-}
-
-struct Props {
-    std::map<std::string, std::string> f;
-
-    explicit Props(const char *text) {
-        std::string key, value;
-        bool is_key = true;
-        int paren = 0;
-        auto flush = [&]() {
-            std::string k = lower(trim(key)), v = trim(value);
-            if (v.size() >= 2 && v.front() == '"' && v.back() == '"') v = v.substr(1, v.size() - 2);
-            if (!k.empty() && !v.empty()) f[k] = v;
-            key.clear();
-            value.clear();
-            is_key = true;
-            paren = 0;
-        };
-        for (const char *p = text; *p; ++p) {
-            const char c = *p;
-            if (is_key) {
-                if (c == '=') is_key = false;
-                else if (c == ';') key.clear();
-                else key += c;
-            } else {
-                if (c == '(') ++paren;
-                else if (c == ')') --paren;
-                if (c == ';' && paren < 1) flush();
-                else value += c;
-            }
-        }
-        if (!is_key) flush();  // a last value without ';'
-    }
-    bool has(const char *k) const { return f.count(k) != 0; }
-    // ReadString: first key present, lower-cased (HSExecutionPlan.cpp:52-64)
-    bool str(std::initializer_list<const char *> keys, std::string &out) const {
-        for (const char *k : keys) {
-            auto it = f.find(k);
-            if (it != f.end()) {
-                out = lower(it->second);
-                return true;
-            }
-        }
-        return false;
-    }
-    // ObjectParser::Get(double): std::stod; an unparsable value reads as absent
-    bool num(const char *k, double &out) const {
-        auto it = f.find(k);
-        if (it == f.end()) return false;
-        try {
-            out = std::stod(it->second);
-        } catch (...) {
-            return false;
-        }
-        return true;
-    }
-    bool integer(const char *k, int &out) const {
-        auto it = f.find(k);
-        if (it == f.end()) return false;
-        try {
-            out = std::stoi(it->second);
-        } catch (...) {
-            return false;
-        }
-        return true;
-    }
-    // ObjectParser::Get(bool): true/yes/1, false/no/0 (MSDParser/ObjectParser.cpp:187-202)
-    bool boolean(const char *k, bool &out) const {
-        auto it = f.find(k);
-        if (it == f.end()) return false;
-        const std::string v = it->second;
-        if (v == "true" || v == "yes" || v == "1") out = true;
-        else if (v == "false" || v == "no" || v == "0") out = false;
-        else return false;
-        return true;
-    }
-};
-
-bool one_of(const std::string &v, std::initializer_list<const char *> c) {
-    for (const char *x : c)
-        if (v == x) return true;
-    return false;
-}
+using msg::lower;
+using msg::one_of;
+using msg::Props;
+using msg::trim;
 
 struct PlanError {
     msg_status st;
