@@ -578,12 +578,93 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         cd *dX3 = X3.alloc<cd>((size_t)chunk * D * D, stream);
         cd *dX4 = X4.alloc<cd>((size_t)chunk * D * D, stream);
         cd *dB = Bk.alloc<cd>((size_t)chunk * D * D, stream);
-        std::vector<cd> H, Hs((size_t)chunk * D * D);
+        std::vector<cd> H, Hs;
+        // A field sweep: H(f) = H(0) + sum_a f_a Z_a is linear in the swept field, so the
+        // host builds H(0) and the three unit-field operators once and the device forms
+        // every item's generator (k_dyn_generator) instead of one host H per item.
+        const bool linear = fields != nullptr && n_items > 1;
+        const size_t DD = (size_t)D * D;
+        DBuf bG0, bZ, bcoef, bitem, bdt;
+        double g0norm = 0.0, znorm[3] = {0.0, 0.0, 0.0};
+        std::vector<double> lcoef, ldt;
+        std::vector<int> litem;
+        if (linear) {
+            const double zero[3] = {0.0, 0.0, 0.0};
+            dense_hamiltonian(r, zero, H);
+            std::vector<cd> G0(DD), Z(3 * DD), H1;
+            for (size_t k = 0; k < DD; ++k) G0[k] = cd(H[k].imag(), -H[k].real());  // -i H(0)
+            if (!factor_decay)
+                for (int nu = 0; nu < D / 4; ++nu)
+                    for (int i = 0; i < 4; ++i)
+                        for (int j = 0; j < 4; ++j) G0[(size_t)(4 * nu + i) * D + 4 * nu + j] -= r.K[i * 4 + j];
+            auto norm1 = [&](const cd *g) {
+                double m1 = 0.0;
+                for (int j = 0; j < D; ++j) {
+                    double cs = 0.0;
+                    for (int i = 0; i < D; ++i) cs += std::abs(g[(size_t)i * D + j]);
+                    m1 = std::max(m1, cs);
+                }
+                return m1;
+            };
+            g0norm = norm1(G0.data());
+            for (int ax = 0; ax < 3; ++ax) {
+                double e[3] = {0.0, 0.0, 0.0};
+                e[ax] = 1.0;
+                dense_hamiltonian(r, e, H1);
+                cd *z = &Z[(size_t)ax * DD];
+                for (size_t k = 0; k < DD; ++k) {
+                    const cd d = H1[k] - H[k];
+                    z[k] = cd(d.imag(), -d.real());  // -i Z_a
+                }
+                znorm[ax] = norm1(z);
+            }
+            cd *dG0 = bG0.alloc<cd>(DD, stream), *dZ = bZ.alloc<cd>(3 * DD, stream);
+            HIPCHECK(hipMemcpyAsync(dG0, G0.data(), DD * sizeof(cd), hipMemcpyHostToDevice, stream));
+            HIPCHECK(hipMemcpyAsync(dZ, Z.data(), 3 * DD * sizeof(cd), hipMemcpyHostToDevice, stream));
+            HIPCHECK(hipStreamSynchronize(stream));  // G0 and Z are released at scope end
+            bcoef.alloc<double>((size_t)3 * chunk, stream);
+            bitem.alloc<int>(chunk, stream);
+            bdt.alloc<double>(chunk, stream);
+            HIPCHECK(hipMemsetAsync(bitem.p, 0, (size_t)chunk * sizeof(int), stream));  // every entry uses G0
+            lcoef.resize((size_t)3 * chunk);
+            ldt.resize(chunk);
+        } else {
+            Hs.resize((size_t)chunk * DD);
+        }
         for (int w = 0; w < per; ++w) {
             const double dt = w == 0 ? dts[0] : dts[ns - 2];
             for (int it0 = 0; it0 < n_items; it0 += chunk) {
                 const int nb = std::min(chunk, n_items - it0);
                 double nrm = 0.0;
+                if (linear) {
+                    for (int q = 0; q < nb; ++q) {
+                        const double *f = fields + 3 * (size_t)(it0 + q);
+                        double bound = g0norm;
+                        for (int ax = 0; ax < 3; ++ax) {
+                            lcoef[3 * (size_t)q + ax] = f[ax];
+                            bound += std::fabs(f[ax]) * znorm[ax];
+                        }
+                        nrm = std::max(nrm, dt * bound);
+                    }
+                    int sq = 0;
+                    const bool rk4 = p->method == MSG_METHOD_RK4;
+                    while (!rk4 && nrm > 0.5) { nrm *= 0.5; ++sq; }
+                    for (int q = 0; q < nb; ++q) ldt[q] = std::ldexp(dt, -sq);
+                    HIPCHECK(hipMemcpyAsync(bcoef.p, lcoef.data(), (size_t)3 * nb * sizeof(double), hipMemcpyHostToDevice,
+                                            stream));
+                    HIPCHECK(hipMemcpyAsync(bdt.p, ldt.data(), (size_t)nb * sizeof(double), hipMemcpyHostToDevice, stream));
+                    HIPCHECK(launch_dyn_generator(bG0.p, bZ.p, bcoef.get<double>(), bitem.get<int>(), bdt.get<double>(), 3,
+                                                  D, nb, dX, stream));
+                    HIPCHECK(hipStreamSynchronize(stream));  // lcoef / ldt are refilled for the next chunk
+                    taylor_batch(dX, dX2, dX3, dX4, dB, dP, dQ, D, nb, rk4, stream);
+                    for (int q = 0; q < sq; ++q) {
+                        HIPCHECK(launch_zgemm(dP, dP, dQ, D, nb, 1.0, 0.0, stream));
+                        std::swap(dP, dQ);
+                    }
+                    HIPCHECK(launch_to_planes(dP, dU + ((size_t)it0 * per + w) * 2 * Dp * Dp, D, Dp,
+                                              (size_t)per * 2 * Dp * Dp, nb, stream));
+                    continue;
+                }
                 for (int q = 0; q < nb; ++q) {
                     dense_hamiltonian(r, fields ? fields + 3 * (it0 + q) : nullptr, H);
                     if (!factor_decay)  // H_eff = H - i K (x) 1 on the electron pair
