@@ -351,7 +351,9 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 // DBUF: two LDS buffers hold v_j and v_{j-1} (swapped every iteration); without
 // DBUF one buffer holds v_j and v_{j-1} is re-read from the basis store, which
 // keeps D = 16384 (128 KiB per vector) resident in LDS.
-template <int G, int TMAX, bool DBUF>
+// DYN: the operator head (electron fields, shift) comes per interval from
+// a.dyn_head (time-dependent Zeeman fields); otherwise from the item's block.
+template <int G, int TMAX, bool DBUF, bool DYN = false>
 __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict__ pa) {
     const StepArgs &a = *pa;
     extern __shared__ float4 smem[];
@@ -422,7 +424,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict_
             }
             if (end) break;
             const double dt = a.dts[s];
-            const float *hd = a.dyn_head ? a.dyn_head + ((size_t)item * (a.nsamples - 1) + s) * 8 : co;
+            const float *hd = DYN ? a.dyn_head + ((size_t)item * (a.nsamples - 1) + s) * 8 : co;
             // ---- relaxation half-step before (HSPropagator.cpp:137)
             if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s])
 #pragma unroll
@@ -1513,10 +1515,13 @@ hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_t
         return hipGetLastError();
     }
     const size_t lds = (size_t)4 * a.dim * sizeof(float) + 2 * 256 * sizeof(double);
-#define X(G_, T_, B_)                                                                         \
-    if (group_per_thread == G_ && threads == T_) {                                            \
-        hipLaunchKernelGGL((k_lanczos<G_, T_, B_>), dim3(slots), dim3(threads), lds, st, da); \
-        return hipGetLastError();                                                              \
+#define X(G_, T_, B_)                                                                                   \
+    if (group_per_thread == G_ && threads == T_) {                                                      \
+        if (a.dyn_head)                                                                                  \
+            hipLaunchKernelGGL((k_lanczos<G_, T_, B_, true>), dim3(slots), dim3(threads), lds, st, da);  \
+        else                                                                                             \
+            hipLaunchKernelGGL((k_lanczos<G_, T_, B_, false>), dim3(slots), dim3(threads), lds, st, da); \
+        return hipGetLastError();                                                                        \
     }
     MSG_LANCZOS_VARIANTS(X)
 #undef X
@@ -1530,7 +1535,9 @@ hipError_t set_lanczos_lds_limits() {
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     const int lim = maxb - 2048;
-#define X(G_, T_, B_) (void)hipFuncSetAttribute((const void *)k_lanczos<G_, T_, B_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+#define X(G_, T_, B_)                                                                                               \
+    (void)hipFuncSetAttribute((const void *)k_lanczos<G_, T_, B_, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+    (void)hipFuncSetAttribute((const void *)k_lanczos<G_, T_, B_, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     MSG_LANCZOS_VARIANTS(X)
 #undef X
 #define X(NT_) (void)hipFuncSetAttribute((const void *)k_dense<NT_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
@@ -1567,7 +1574,7 @@ int lanczos_occupancy(int group_per_thread, int threads, size_t lds) {
                    ? nb : 1;
 #define X(G_, T_, B_)                                                                                         \
     if (group_per_thread == G_ && threads == T_)                                                              \
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos<G_, T_, B_>, threads, lds) == hipSuccess \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos<G_, T_, B_, false>, threads, lds) == hipSuccess \
                    ? nb : 1;
     MSG_LANCZOS_VARIANTS(X)
 #undef X
