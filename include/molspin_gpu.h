@@ -156,7 +156,8 @@ enum {
 };
 enum {
     MSG_METHOD_AUTO = 0,          /* device chooses: dense U for D<=512, matrix-free Krylov above */
-    MSG_METHOD_KRYLOV = 1,        /* propagationmethod=krylov, fixed krylovsize                   */
+    MSG_METHOD_KRYLOV = 1,        /* propagationmethod=krylov, fixed krylovsize: Lanczos for K = kappa 1,
+                                     Arnoldi on -iH - K for unequal rates (D <= 8192)             */
     MSG_METHOD_EXPONENTIAL = 2,   /* propagationmethod=normal: exact exp(dt G) applied as a GEMM  */
     MSG_METHOD_RK4 = 3,           /* propagationmethod=rk4 (HSPropagator.cpp:80-90): for a static G
                                      one RK4 step is the degree-4 Taylor polynomial of dt G, applied

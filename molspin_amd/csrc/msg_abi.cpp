@@ -35,7 +35,7 @@ hipError_t launch_init_norms(const StepArgs &a, const StepArgs *da, hipStream_t 
 hipError_t launch_zero_int(int *p, int n, hipStream_t st);
 hipError_t launch_reduce_columns(const double *in, double *out, int n_items, int ncols, int nk, int in_stride, int pick,
                                  hipStream_t st);
-int lanczos_occupancy(int group_per_thread, int threads, size_t lds);
+int lanczos_occupancy(int group_per_thread, int threads, size_t lds, bool arnoldi);
 hipError_t launch_zgemm(const void *A, const void *B, void *C, int n, int batch, double coef, double diag_add,
                         hipStream_t st, bool conj_b = false, const void *E = nullptr);
 hipError_t launch_zlincomb(const void *X, const void *X2, const void *X3, void *out, int n, int batch,
@@ -162,6 +162,7 @@ struct Batch {
     bool timed = false;    // bracket the propagation kernel with the context events
     // dynamics=dynamic on the dense path: one propagator per interval, built per chunk
     // of intervals from G_item (static generator) + sum_k c_k(t) Z_k
+    bool arnoldi = false;  // matrix-free path on G = -iH - K (unequal recombination rates)
     bool dyn_dense = false;
     int dyn_chunk = 0;                      // intervals per propagator chunk
     DBuf dynG, dynZ, dyn_head;
@@ -237,8 +238,12 @@ Path choose_path(const Resolved &r, const msg_run_params *p) {
     if (p->method == MSG_METHOD_AUTO && dense_ok) return Path::Dense;
     if (lanczos_ok) return Path::Lanczos;
     if (dense_ok) return Path::Dense;  // exact exponential: same result as Krylov within truncation
+    // unequal rates: matrix-free Arnoldi on G = -iH - K while the Krylov vector fits in LDS
+    const bool arnoldi_ok = r.fast_ok && !sym_for(r, p) && r.D <= 8192 && !dyn_nuclear;
+    if (arnoldi_ok) return Path::Lanczos;
     if (!sym_for(r, p))
-        throw ArgError{MSG_UNSUPPORTED, "unequal recombination rates need D <= 512 (dense exponential path)"};
+        throw ArgError{MSG_UNSUPPORTED, "unequal recombination rates need D <= 8192 (matrix-free Arnoldi) with "
+                                        "spin-1/2 nuclei, or D <= 512 (dense exponential path)"};
     if (!r.fast_ok) throw ArgError{MSG_UNSUPPORTED, "matrix-free path: " + r.why_not_fast};
     throw ArgError{MSG_UNSUPPORTED, "Hilbert dimension outside the matrix-free range"};
 }
@@ -439,6 +444,17 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
             if ((g == 1 && ng >= 64 && ng <= 1024) || (g == 2 && ng >= 1024 && ng <= 2048)) { b.G = g; b.T = ng / g; }
             if (g == 0 && ng > 2048) { b.G = 0; b.T = 1024; }
             if (g == -2 && ng > kTileGroups) { b.G = -2; b.T = 1024; }
+        }
+        b.arnoldi = !sym_for(r, p);
+        if (b.arnoldi) {  // G = -iH - K: Arnoldi with the Krylov vector in LDS, one group per thread up to D = 4096
+            if (ng > 2048) throw ArgError{MSG_UNSUPPORTED, "matrix-free Arnoldi is limited to D <= 8192"};
+            b.G = ng <= 1024 ? 1 : 2;
+            b.T = ng / b.G;
+            a.arnoldi = 1;
+            for (int i = 0; i < 16; ++i) {
+                a.kmat[2 * i] = (float)r.K[i].real();
+                a.kmat[2 * i + 1] = (float)r.K[i].imag();
+            }
         }
         if (b.G > 0 && (b.T < 64 || b.T > 1024 || b.G * b.T != ng))
             throw ArgError{MSG_UNSUPPORTED, "no matrix-free launch shape for this dimension"};
@@ -726,8 +742,8 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     // launch geometry
     if (b.path == Path::Lanczos) {
         const size_t lds = b.G == 0 ? 0 : b.G == -1 ? (size_t)8 * D : b.G == -2 ? (size_t)32 * kTileGroups
-                                                                              : (size_t)16 * D + 4096;
-        const int occ = std::max(1, lanczos_occupancy(b.G, b.T, lds));
+                                     : b.arnoldi ? (size_t)8 * D + 4096 : (size_t)16 * D + 4096;
+        const int occ = std::max(1, lanczos_occupancy(b.G, b.T, lds, b.arnoldi));
         b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ));
         if (ts && ts->slots > 0) b.slots = std::max(1, std::min(b.slots, ts->slots));
         if (b.share > 1) b.slots = std::max(1, std::min(b.slots, ctx->cus * occ / b.share));

@@ -44,6 +44,8 @@ struct StepArgs {
     const int *nuc_flags;  // [nnuc] device
     const float *coef; // [n_items][coef_stride]
     int coef_stride;
+    int arnoldi;            // 1: Arnoldi on G = -iH - K (unequal recombination rates)
+    float kmat[32];         // K on the electron pair, 4x4 complex row-major (Arnoldi)
     const float *dyn_head;  // [n_items][nsamples-1][8] head (w1, w2, shift) of each interval of a
                             // time-dependent operator, or NULL (static: the head of `coef`)
     // dense path

@@ -579,6 +579,264 @@ __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict_
     }
 }
 
+// Matrix-free Arnoldi for the non-Hermitian generator G = -iH - K (x) 1 of unequal
+// recombination rates (KrylovExpmGeneral / ArnoldiProcess, SpinSpace_operators.cpp:
+// 1257-1408): psi <- beta V_m exp(dt H_m) e1 with the m x m upper Hessenberg H_m of
+// modified Gram-Schmidt (FP64 inner products; happy breakdown below 1e-14).  One
+// workgroup per (item, column); v_j in LDS for the operator's neighbour reads, the
+// basis in the slot's global scratch, w in registers.  Wavefront 0 evaluates
+// exp(dt H_m) e1 in FP64 as N applications of the Taylor series of exp(dt H_m / N)
+// to the vector, N = ceil(2 ||dt H_m||_1).
+template <int G, int TMAX>
+__global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict__ pa) {
+    const StepArgs &a = *pa;
+    extern __shared__ float4 smem[];
+    constexpr int T = TMAX;
+    constexpr int ng = G * T;
+    const int t = threadIdx.x, lane = t & 63;
+    float4 *LA = smem, *LB = smem + ng;
+    double *scr = reinterpret_cast<double *>(smem + 2 * ng);  // 2 x 256 doubles
+    __shared__ double2 s_h[kMaxKrylov * kMaxKrylov];          // H_m, row-major
+    __shared__ float2 s_coef[kMaxKrylov];
+    __shared__ double ys[kMaxObs], yc[kMaxObs];
+    __shared__ int s_unit;
+    const int D = a.dim, D4 = D >> 2;
+    float *bas = a.basis + (size_t)blockIdx.x * a.m * 2 * D;
+    const int meff0 = a.m < D ? a.m : D;
+    int par = 0;
+    auto SCR = [&]() { par ^= 1; return scr + par * 256; };
+    for (;;) {
+        if (t == 0) s_unit = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        __syncthreads();
+        if (u >= a.n_units) break;
+        const int item = u / a.ncols, col = u - item * a.ncols;
+        const float *co = a.coef + (size_t)item * a.coef_stride;
+        C4 psi[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int nu = g * T + t;
+            if (a.init_mode == 0) {
+                const int which = col / a.zdim, ncol = col - which * a.zdim;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    psi[g].c[c] = (nu == ncol) ? make_float2(a.evec[which * 8 + 2 * c], a.evec[which * 8 + 2 * c + 1])
+                                               : make_float2(0.f, 0.f);
+            } else {
+                const float *src = a.init_mode == 1 ? a.given + (size_t)col * 2 * D : a.state + (size_t)u * 2 * D;
+                const float4 *sA = reinterpret_cast<const float4 *>(src);
+                psi[g] = load_group(sA, sA + D4, nu);
+            }
+        }
+        if (t == 0)
+            for (int c = 0; c < a.nobs; ++c) {
+                ys[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2] : 0.0;
+                yc[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2 + 1] : 0.0;
+            }
+        for (int s = a.s_begin;; ++s) {
+            const bool end = (s == a.s_end);
+            if (!end || s == a.nsamples - 1) {  // observables at sample s
+                float Rf[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) Rf[k] = 0.f;
+#pragma unroll
+                for (int g = 0; g < G; ++g) accum_rho(psi[g], Rf);
+                double R[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) R[k] = Rf[k];
+                block_sum<16>(R, SCR());
+                if (t == 0) record_sample(a, u, s, R, ys, yc);
+            }
+            if (end) break;
+            const double dt = a.dts[s];
+            const float *hd = a.dyn_head ? a.dyn_head + ((size_t)item * (a.nsamples - 1) + s) * 8 : co;
+            if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s])
+#pragma unroll
+                for (int g = 0; g < G; ++g) st_flip(psi[g]);
+            double nn[1] = {0.0};
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    nn[0] += (double)psi[g].c[c].x * psi[g].c[c].x + (double)psi[g].c[c].y * psi[g].c[c].y;
+            block_sum<1>(nn, SCR());
+            const double beta0 = sqrt(nn[0]);
+            if (beta0 > 0.0) {
+                for (int k = t; k < kMaxKrylov * kMaxKrylov; k += T) s_h[k] = make_double2(0.0, 0.0);
+                {
+                    const float inv = (float)(1.0 / beta0);
+                    float4 *bA = reinterpret_cast<float4 *>(bas);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        C4 v0;
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) v0.c[c] = make_float2(psi[g].c[c].x * inv, psi[g].c[c].y * inv);
+                        const int nu = g * T + t;
+                        store_group(LA, LB, nu, v0);
+                        store_group(bA, bA + D4, nu, v0);
+                    }
+                }
+                int meff = meff0;
+                for (int j = 0; j < meff0; ++j) {
+                    __syncthreads();  // v_j complete in LDS
+                    C4 w[G];
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const int nu = g * T + t;
+                        const C4 vj = load_group(LA, LB, nu);
+                        const C4 hv = apply_h<Log2<ng>::value>(a, co, hd, nu, vj, LA, LB);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {  // w = -i H v - K v
+                            float2 kv = make_float2(0.f, 0.f);
+#pragma unroll
+                            for (int c = 0; c < 4; ++c)
+                                kv = cfma(make_float2(a.kmat[8 * r + 2 * c], a.kmat[8 * r + 2 * c + 1]), vj.c[c], kv);
+                            w[g].c[r] = make_float2(hv.c[r].y - kv.x, -hv.c[r].x - kv.y);
+                        }
+                    }
+                    // modified Gram-Schmidt: h_ij = <v_i, w>, w -= h_ij v_i
+                    for (int i = 0; i <= j; ++i) {
+                        // v_j from LDS, earlier basis vectors from the slot's global scratch
+                        const float4 *gA = reinterpret_cast<const float4 *>(bas + (size_t)i * 2 * D), *gB = gA + D4;
+                        double h[2] = {0.0, 0.0};
+#pragma unroll
+                        for (int g = 0; g < G; ++g) {
+                            const int nu = g * T + t;
+                            const C4 vi = i == j ? load_group(LA, LB, nu) : load_group(gA, gB, nu);
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                h[0] += (double)vi.c[c].x * w[g].c[c].x + (double)vi.c[c].y * w[g].c[c].y;
+                                h[1] += (double)vi.c[c].x * w[g].c[c].y - (double)vi.c[c].y * w[g].c[c].x;
+                            }
+                        }
+                        block_sum<2>(h, SCR());
+                        if (t == 0) s_h[i * kMaxKrylov + j] = make_double2(h[0], h[1]);
+                        const float hr = (float)h[0], hi = (float)h[1];
+#pragma unroll
+                        for (int g = 0; g < G; ++g) {
+                            const int nu = g * T + t;
+                            const C4 vi = i == j ? load_group(LA, LB, nu) : load_group(gA, gB, nu);
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                w[g].c[c].x -= hr * vi.c[c].x - hi * vi.c[c].y;
+                                w[g].c[c].y -= hr * vi.c[c].y + hi * vi.c[c].x;
+                            }
+                        }
+                    }
+                    if (j == meff0 - 1) break;
+                    double wn[1] = {0.0};
+#pragma unroll
+                    for (int g = 0; g < G; ++g)
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            wn[0] += (double)w[g].c[c].x * w[g].c[c].x + (double)w[g].c[c].y * w[g].c[c].y;
+                    block_sum<1>(wn, SCR());
+                    const double hn = sqrt(wn[0]);
+                    if (!(hn >= 1e-14)) {  // invariant Krylov subspace (happy breakdown)
+                        meff = j + 1;
+                        break;
+                    }
+                    if (t == 0) s_h[(j + 1) * kMaxKrylov + j] = make_double2(hn, 0.0);
+                    // v_{j+1}: the operator's neighbour reads of v_j finished before the
+                    // first reduction above, so the LDS copy can be overwritten
+                    const float ib = (float)(1.0 / hn);
+                    float4 *bA = reinterpret_cast<float4 *>(bas + (size_t)(j + 1) * 2 * D);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) w[g].c[c] = make_float2(w[g].c[c].x * ib, w[g].c[c].y * ib);
+                        const int nu = g * T + t;
+                        store_group(LA, LB, nu, w[g]);
+                        store_group(bA, bA + D4, nu, w[g]);
+                    }
+                }
+                __syncthreads();  // H_m complete
+                if (t < 64) {
+                    // exp(dt H_m) e1 in FP64: lane r owns component r
+                    const int mm = meff;
+                    double nrm = 0.0;  // 1-norm of dt H_m (column sums)
+                    if (lane < mm)
+                        for (int r = 0; r < mm; ++r) {
+                            const double2 h = s_h[r * kMaxKrylov + lane];
+                            nrm += dt * sqrt(h.x * h.x + h.y * h.y);
+                        }
+                    for (int off = 32; off > 0; off >>= 1) nrm = fmax(nrm, __shfl_xor(nrm, off, 64));
+                    int nrep = (int)ceil(2.0 * nrm);
+                    if (nrep < 1) nrep = 1;
+                    if (nrep > 4096) {
+                        if (lane == 0) atomicOr(a.status, 2);
+                        nrep = 4096;
+                    }
+                    const double sc = dt / nrep;
+                    double xr = lane == 0 ? 1.0 : 0.0, xi = 0.0;  // x = e1
+                    for (int rep = 0; rep < nrep; ++rep) {
+                        double sr = xr, si = xi, tr = xr, ti = xi;  // sum and current term
+                        for (int k = 1; k <= 40; ++k) {
+                            double nr = 0.0, ni = 0.0;  // (sc H_m term)_lane / k
+                            for (int c = 0; c < mm; ++c) {
+                                const double cr = __shfl(tr, c, 64), ci = __shfl(ti, c, 64);
+                                if (lane < mm) {
+                                    const double2 h = s_h[lane * kMaxKrylov + c];
+                                    nr += h.x * cr - h.y * ci;
+                                    ni += h.x * ci + h.y * cr;
+                                }
+                            }
+                            tr = lane < mm ? nr * sc / k : 0.0;
+                            ti = lane < mm ? ni * sc / k : 0.0;
+                            sr += tr;
+                            si += ti;
+                            double tn = fabs(tr) + fabs(ti), sn = fabs(sr) + fabs(si);
+                            for (int off = 32; off > 0; off >>= 1) {
+                                tn = fmax(tn, __shfl_xor(tn, off, 64));
+                                sn = fmax(sn, __shfl_xor(sn, off, 64));
+                            }
+                            if (tn <= 1e-18 * sn) break;
+                        }
+                        xr = sr;
+                        xi = si;
+                    }
+                    if (lane < mm) s_coef[lane] = make_float2((float)(beta0 * xr), (float)(beta0 * xi));
+                }
+                __syncthreads();
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) psi[g].c[c] = make_float2(0.f, 0.f);
+                for (int j = 0; j < meff; ++j) {
+                    const float2 cj = s_coef[j];
+                    const float4 *bA = reinterpret_cast<const float4 *>(bas + (size_t)j * 2 * D);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const C4 vj = load_group(bA, bA + D4, g * T + t);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) psi[g].c[c] = cfma(cj, vj.c[c], psi[g].c[c]);
+                    }
+                }
+            }
+            if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
+#pragma unroll
+                for (int g = 0; g < G; ++g) st_flip(psi[g]);
+        }
+        if (t == 0)
+            for (int c = 0; c < a.nobs; ++c) {
+                a.yacc[((size_t)u * a.nobs + c) * 2] = ys[c];
+                a.yacc[((size_t)u * a.nobs + c) * 2 + 1] = yc[c];
+            }
+        if (a.state) {
+            float4 *dA = reinterpret_cast<float4 *>(a.state + (size_t)u * 2 * D);
+#pragma unroll
+            for (int g = 0; g < G; ++g) store_group(dA, dA + D4, g * T + t, psi[g]);
+        }
+        bool bad = false;
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bad |= !isfinite(psi[g].c[c].x) || !isfinite(psi[g].c[c].y);
+        if (bad) atomicOr(a.status, 1);
+    }
+}
+
 // Matrix-free Lanczos for Hilbert dimensions whose Krylov vector exceeds LDS
 // (D >= 16384): one 1024-thread workgroup per column; the state, the residual
 // and the Krylov basis stream through global memory (per-slot scratch), and
@@ -1514,6 +1772,17 @@ hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_t
             hipLaunchKernelGGL(k_lanczos_big<false>, dim3(slots), dim3(1024), 0, st, da);
         return hipGetLastError();
     }
+    if (a.arnoldi) {
+        const size_t lds_a = (size_t)2 * a.dim * sizeof(float) + 2 * 256 * sizeof(double);
+#define X(G_, T_, B_)                                                                                  \
+        if (group_per_thread == G_ && threads == T_) {                                                 \
+            hipLaunchKernelGGL((k_arnoldi<G_, T_>), dim3(slots), dim3(threads), lds_a, st, da);        \
+            return hipGetLastError();                                                                   \
+        }
+        MSG_LANCZOS_VARIANTS(X)
+#undef X
+        return hipErrorInvalidValue;
+    }
     const size_t lds = (size_t)4 * a.dim * sizeof(float) + 2 * 256 * sizeof(double);
 #define X(G_, T_, B_)                                                                                   \
     if (group_per_thread == G_ && threads == T_) {                                                      \
@@ -1537,7 +1806,8 @@ hipError_t set_lanczos_lds_limits() {
     const int lim = maxb - 2048;
 #define X(G_, T_, B_)                                                                                               \
     (void)hipFuncSetAttribute((const void *)k_lanczos<G_, T_, B_, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
-    (void)hipFuncSetAttribute((const void *)k_lanczos<G_, T_, B_, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)k_lanczos<G_, T_, B_, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+    (void)hipFuncSetAttribute((const void *)k_arnoldi<G_, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     MSG_LANCZOS_VARIANTS(X)
 #undef X
 #define X(NT_) (void)hipFuncSetAttribute((const void *)k_dense<NT_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
@@ -1563,8 +1833,16 @@ hipError_t launch_dense(const StepArgs &a, const StepArgs *da, int slots, hipStr
     return hipErrorInvalidValue;
 }
 
-int lanczos_occupancy(int group_per_thread, int threads, size_t lds) {
+int lanczos_occupancy(int group_per_thread, int threads, size_t lds, bool arnoldi) {
     int nb = 0;
+    if (arnoldi) {
+#define X(G_, T_, B_)                                                                                         \
+        if (group_per_thread == G_ && threads == T_)                                                          \
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_arnoldi<G_, T_>, threads, lds) == hipSuccess ? nb : 1;
+        MSG_LANCZOS_VARIANTS(X)
+#undef X
+        return 1;
+    }
     if (group_per_thread == 0)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<false>, 1024, 0) == hipSuccess ? nb : 1;
     if (group_per_thread == -1)
