@@ -17,7 +17,13 @@ Other workloads (`--workload`), same JSON line format:
                 dense MFMA path (FP64 propagator built once per field);
   pair          a 12-nucleus pair (D = 16384) at `--fields` field points x 4096
                 direct columns (Lanczos with the Krylov vector mirrored in LDS);
-  uncoupled     the orientation batch through the exact uncoupled-radical method.
+  uncoupled     the orientation batch through the exact uncoupled-radical method;
+  rydmr         the orientation batch with a linearly polarized 5 MHz field on both
+                electrons (dynamics=dynamic: per-interval electron fields);
+  arnoldi       the orientation batch with singlet-only recombination (unequal rates:
+                matrix-free Arnoldi on -iH - K);
+  timeinf       time-infinity yields (method=timeinf) of a D = 512 pair with unequal
+                sinks over 64 field strengths (propagator doubling, FP64).
 """
 import argparse
 import json
@@ -40,7 +46,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["orientations", "trajectories", "dense", "pair", "uncoupled"],
+    ap.add_argument("--workload", choices=["orientations", "trajectories", "dense", "pair", "uncoupled", "rydmr", "arnoldi",
+                             "timeinf"],
                     default="orientations")
     ap.add_argument("--orientations", type=int, default=32)
     ap.add_argument("--trajectories", type=int, default=512)
@@ -70,6 +77,10 @@ def main():
         run = uncoupled_batch(ctx, args)
     elif args.workload == "pair":
         run = pair_batch(ctx, args)
+    elif args.workload in ("rydmr", "arnoldi"):
+        run = variant_batch(ctx, args)
+    elif args.workload == "timeinf":
+        run = timeinf_batch(ctx, args)
     else:
         run = dense_sweep(ctx, args)
 
@@ -88,7 +99,7 @@ def main():
     roof = run["roofline"](kernel_s)
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and run.get("packed") is not None:
         cpu = cpu_baseline(run["packed"], run["cpu_prm"], run["cpu_field"], run["cpu_columns"], args.cpu_seconds)
     serving = None
     if not args.no_serving and args.workload == "orientations":
@@ -100,7 +111,7 @@ def main():
     line = {
         "metric": METRIC,
         "value": value,
-        "unit": "state-steps/s",
+        "unit": run.get("unit", "state-steps/s"),
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -242,6 +253,82 @@ def pair_batch(ctx, args):
                         "parallelism": "single GPU",
                         "launch_shape": {"groups_per_thread": shape.groups_per_thread, "threads": shape.threads,
                                          "slots": shape.slots, "chosen_by": "msg_autotune"}})
+
+
+def variant_batch(ctx, args):
+    """The orientation batch (D = 4096, 6 + 4 anisotropic nuclei) with either a
+    linearly polarized 5 MHz field of 50 uT on both electrons (rydmr) or
+    singlet-only recombination at 2e-3 /ns (arnoldi)."""
+    from molspin_amd import abi, workloads
+    rydmr = args.workload == "rydmr"
+    rng = workloads.SplitMix64(2, 0)
+    hf = [workloads._aniso_hf(rng) for _ in range(10)]
+    sysobj = workloads._radical_pair("cfg2", ["1/2"] * 6, ["1/2"] * 4, hf, 1e-3, swept=True)
+    if rydmr:
+        sysobj.add_zeeman(["E1", "E2"], (0.05, 0.0, 0.0), prefactor=0.001, time_dependence="linearpolarized",
+                          frequency=2 * math.pi * 0.005)
+    else:
+        for t in sysobj.transitions:
+            t["rate"] = 2e-3 if sysobj.states[t["source"]]["name"] == "S" else 0.0
+    packed = sysobj.pack()
+    sysd = ctx.upload(packed)
+    D, m, dt = sysd.dim, 16, 2.0
+    B = args.orientations if rydmr else min(args.orientations, 8)
+    intervals = args.intervals or (50 if rydmr else 20)
+    fields = [workloads.fibonacci_point(1024, i, 0.05) for i in range(B)]
+    prm = abi.run_params(total_time=intervals * dt, time_step=dt, method=abi.MSG_METHOD_KRYLOV, krylov_size=m,
+                         dynamics=rydmr)
+    ncols = sysd.direct_columns()
+    state_steps = B * ncols * intervals
+
+    def step(i):
+        sysd.propagate(prm, fields=fields)
+
+    def roof(kernel_s):
+        if rydmr:
+            return lanczos_roofline(D, m, state_steps, kernel_s, None)
+        # Arnoldi: per state-step m basis stores, the Gram-Schmidt reads of the earlier
+        # vectors (twice each), and m reads in the final combination: 8 D m (m + 1) bytes
+        bss = 8.0 * D * m * (m + 1)
+        return {"bound": "hbm", "achieved": bss * state_steps / kernel_s / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": bss * state_steps / kernel_s / HBM_PEAK, "traffic": None,
+                "kernel_ms": kernel_s * 1e3, "bytes_per_state_step": bss, "state_steps_per_launch": state_steps}
+
+    what = ("linearly polarized 5 MHz, 50 uT field on both electrons (dynamics=dynamic)" if rydmr
+            else "singlet-only recombination 2e-3/ns (Arnoldi on -iH - K)")
+    return dict(step=step, state_steps=state_steps, roofline=roof, packed=packed,
+                cpu_prm=abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_KRYLOV,
+                                       dynamics=rydmr),
+                cpu_field=fields[0], cpu_columns=None,
+                data="synthetic (seeded radical pair, Fibonacci field directions)",
+                config={"workload": "orientation batch D=4096 with %s: %d orientations x %d columns x %d steps of "
+                                    "2 ns, Krylov m=%d" % (what, B, ncols, intervals, m),
+                        "orientations_per_step": B, "columns": ncols, "intervals": intervals,
+                        "parallelism": "single GPU"})
+
+
+def timeinf_batch(ctx, args):
+    """Time-infinity yields (msg_time_infinity, doubling route) of a 7-nucleus pair
+    (D = 512) with unequal sinks over the 64 field strengths of the dense sweep."""
+    from molspin_amd import abi, workloads
+    idx = next(i for i in range(1000) if workloads.config4_pair(i)[1]["n"] == 7)
+    sysobj, cfg = workloads.config4_pair(idx)
+    for t in sysobj.transitions:
+        t["rate"] = 3e-3 if sysobj.states[t["source"]]["name"] == "S" else 1e-3
+    packed = sysobj.pack()
+    sysd = ctx.upload(packed)
+    prm = abi.run_params()
+    nf = len(cfg["fields"])
+
+    def step(i):
+        sysd.time_infinity(prm, fields=cfg["fields"])
+
+    return dict(step=step, state_steps=nf, roofline=lambda kernel_s: None, packed=None,
+                data="synthetic (seeded radical pair, log-spaced field strengths)",
+                unit="yield sets/s",
+                config={"workload": "time-infinity yields: D=512 pair %d (7 spin-1/2 nuclei), unequal sinks "
+                                    "k_S=3e-3, k_T=1e-3 /ns, %d field strengths per step (FP64 doubling)" % (idx, nf),
+                        "fields": nf, "parallelism": "single GPU"})
 
 
 def uncoupled_batch(ctx, args):

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <new>
 #include <string>
@@ -36,8 +37,6 @@ hipError_t launch_zero_int(int *p, int n, hipStream_t st);
 hipError_t launch_reduce_columns(const double *in, double *out, int n_items, int ncols, int nk, int in_stride, int pick,
                                  hipStream_t st);
 int lanczos_occupancy(int group_per_thread, int threads, size_t lds, bool arnoldi);
-hipError_t launch_zgemm(const void *A, const void *B, void *C, int n, int batch, double coef, double diag_add,
-                        hipStream_t st, bool conj_b = false, const void *E = nullptr);
 hipError_t launch_zlincomb(const void *X, const void *X2, const void *X3, void *out, int n, int batch,
                            const double *c, hipStream_t st);
 hipError_t launch_blockdiag_right(const void *M, const double *B4, double scale, void *Y, int n, hipStream_t st);
@@ -250,6 +249,32 @@ Path choose_path(const Resolved &r, const msg_run_params *p) {
 
 const StepArgs *push_args(Batch &b, const StepArgs &a);
 
+// Batched complex FP64 GEMM of row-major n x n operands (contiguous batches) on
+// rocBLAS: C = coef A op(B) + E with op(B) = B or B^dagger.  A row-major array read
+// column-major is the transpose, so C^T = op(B)^T A^T is one column-major call with
+// the operands swapped (conj(B) A^T for op(B) = B^dagger).  One process-wide handle,
+// set to the caller's stream for each call.
+hipError_t zgemm(const void *A, const void *B, void *C, int n, int batch, double coef, hipStream_t st,
+                 bool conj_b = false, const void *E = nullptr) {
+    static std::mutex mu;
+    static rocblas_handle h = nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!h && rocblas_create_handle(&h) != rocblas_status_success) return hipErrorUnknown;
+    if (rocblas_set_stream(h, st) != rocblas_status_success) return hipErrorUnknown;
+    const size_t nn = (size_t)n * n;
+    if (E && E != C) {
+        const hipError_t e = hipMemcpyAsync(C, E, (size_t)batch * nn * sizeof(cd), hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return e;
+    }
+    const rocblas_double_complex alpha(coef, 0.0), beta(E ? 1.0 : 0.0, 0.0);
+    const rocblas_status rs = rocblas_zgemm_strided_batched(
+        h, conj_b ? rocblas_operation_conjugate_transpose : rocblas_operation_none, rocblas_operation_none, n, n, n,
+        &alpha, static_cast<const rocblas_double_complex *>(B), n, (rocblas_stride)nn,
+        static_cast<const rocblas_double_complex *>(A), n, (rocblas_stride)nn, &beta,
+        static_cast<rocblas_double_complex *>(C), n, (rocblas_stride)nn, batch);
+    return rs == rocblas_status_success ? hipSuccess : hipErrorUnknown;
+}
+
 // exp(X) for a batch of nb row-major D x D matrices scaled to ||X||_1 <= 1/2: the
 // degree-18 Taylor polynomial by Paterson-Stockmeyer in X^4 (powers X^2, X^3, X^4;
 // then P = B_4 and P = P X^4 + B_k for k = 3..0 with B_k = sum_i X^i / (4k+i)!).
@@ -263,22 +288,22 @@ void taylor_batch(cd *dX, cd *dX2, cd *dX3, cd *dX4, cd *dB, cd *&dP, cd *&dQ, i
     auto taylor_block = [&](int k, double c[4]) {  // coefficients of X^0..X^3 in block k
         for (int i = 0; i < 4; ++i) c[i] = 4 * k + i <= 18 ? fact[4 * k + i] : 0.0;
     };
-    HIPCHECK(launch_zgemm(dX, dX, dX2, D, nb, 1.0, 0.0, stream));
-    HIPCHECK(launch_zgemm(dX2, dX, dX3, D, nb, 1.0, 0.0, stream));
+    HIPCHECK(zgemm(dX, dX, dX2, D, nb, 1.0, stream));
+    HIPCHECK(zgemm(dX2, dX, dX3, D, nb, 1.0, stream));
     double c[4];
     if (rk4) {
         taylor_block(0, c);
         HIPCHECK(launch_zlincomb(dX, dX2, dX3, dB, D, nb, c, stream));
-        HIPCHECK(launch_zgemm(dX2, dX2, dP, D, nb, 1.0 / 24.0, 0.0, stream, false, dB));
+        HIPCHECK(zgemm(dX2, dX2, dP, D, nb, 1.0 / 24.0, stream, false, dB));
         return;
     }
-    HIPCHECK(launch_zgemm(dX2, dX2, dX4, D, nb, 1.0, 0.0, stream));
+    HIPCHECK(zgemm(dX2, dX2, dX4, D, nb, 1.0, stream));
     taylor_block(4, c);
     HIPCHECK(launch_zlincomb(dX, dX2, dX3, dP, D, nb, c, stream));
     for (int k = 3; k >= 0; --k) {
         taylor_block(k, c);
         HIPCHECK(launch_zlincomb(dX, dX2, dX3, dB, D, nb, c, stream));
-        HIPCHECK(launch_zgemm(dP, dX4, dQ, D, nb, 1.0, 0.0, stream, false, dB));
+        HIPCHECK(zgemm(dP, dX4, dQ, D, nb, 1.0, stream, false, dB));
         std::swap(dP, dQ);
     }
 }
@@ -674,7 +699,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                     HIPCHECK(hipStreamSynchronize(stream));  // lcoef / ldt are refilled for the next chunk
                     taylor_batch(dX, dX2, dX3, dX4, dB, dP, dQ, D, nb, rk4, stream);
                     for (int q = 0; q < sq; ++q) {
-                        HIPCHECK(launch_zgemm(dP, dP, dQ, D, nb, 1.0, 0.0, stream));
+                        HIPCHECK(zgemm(dP, dP, dQ, D, nb, 1.0, stream));
                         std::swap(dP, dQ);
                     }
                     HIPCHECK(launch_to_planes(dP, dU + ((size_t)it0 * per + w) * 2 * Dp * Dp, D, Dp,
@@ -703,7 +728,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                 HIPCHECK(hipMemcpyAsync(dX, Hs.data(), (size_t)nb * D * D * sizeof(cd), hipMemcpyHostToDevice, stream));
                 taylor_batch(dX, dX2, dX3, dX4, dB, dP, dQ, D, nb, rk4, stream);
                 for (int q = 0; q < sq; ++q) {
-                    HIPCHECK(launch_zgemm(dP, dP, dQ, D, nb, 1.0, 0.0, stream));
+                    HIPCHECK(zgemm(dP, dP, dQ, D, nb, 1.0, stream));
                     std::swap(dP, dQ);
                 }
                 HIPCHECK(launch_to_planes(dP, dU + ((size_t)it0 * per + w) * 2 * Dp * Dp, D, Dp,
@@ -863,15 +888,15 @@ void build_dyn_chunk(Batch &b, int s0, int s1) {
             cd *p = dP, *qq = dQ;
             taylor_batch(dX, dX2, dX3, dX4, dB, p, qq, D, nb, false, st);
             for (int k = 0; k < sq; ++k) {
-                HIPCHECK(launch_zgemm(p, p, qq, D, nb, 1.0, 0.0, st));
+                HIPCHECK(zgemm(p, p, qq, D, nb, 1.0, st));
                 std::swap(p, qq);
             }
             res = p;
         } else {
             const cd *a0 = dX, *am = dX + (size_t)nb * DD, *a1 = dX + (size_t)2 * nb * DD;
-            HIPCHECK(launch_zgemm(am, a0, dX2, D, nb, 0.5, 0.0, st, false, am));   // K2 = am + am K1 / 2
-            HIPCHECK(launch_zgemm(am, dX2, dX3, D, nb, 0.5, 0.0, st, false, am));  // K3 = am + am K2 / 2
-            HIPCHECK(launch_zgemm(a1, dX3, dX4, D, nb, 1.0, 0.0, st, false, a1));  // K4 = a1 + a1 K3
+            HIPCHECK(zgemm(am, a0, dX2, D, nb, 0.5, st, false, am));   // K2 = am + am K1 / 2
+            HIPCHECK(zgemm(am, dX2, dX3, D, nb, 0.5, st, false, am));  // K3 = am + am K2 / 2
+            HIPCHECK(zgemm(a1, dX3, dX4, D, nb, 1.0, st, false, a1));  // K4 = a1 + a1 K3
             const double c1[4] = {1.0, 1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0};
             HIPCHECK(launch_zlincomb(a0, dX2, dX3, dB, D, nb, c1, st));            // 1 + K1/6 + K2/3 + K3/3
             const double c2[4] = {0.0, 1.0, 1.0 / 6.0, 0.0};
@@ -1253,7 +1278,7 @@ void propagate_uncoupled(msg_ctx *ctx, const msg_system *sys, const msg_run_para
             const size_t plane = (size_t)nb * rd.n * rd.n;  // S_mu of all items of the chunk
             for (int mu = 0; mu < 3; ++mu) {
                 HIPCHECK(launch_pauli_right(rd.Hm, rd.Y, rd.n, mu, nb, st));
-                HIPCHECK(launch_zgemm(rd.Y, rd.Hm, rd.S + mu * plane, rd.n, nb, 1.0, 0.0, st, true));
+                HIPCHECK(zgemm(rd.Y, rd.Hm, rd.S + mu * plane, rd.n, nb, 1.0, st, true));
             }
             HIPCHECK(launch_corr(rd.S, plane, rd.lam, tm, rd.n, ns, nb, rd.C, st));
         }
@@ -1501,7 +1526,7 @@ msg_status msg_symmetric_decay(msg_ctx *ctx, const msg_system *sys, int32_t n_it
             rocblas_int hinfo = 0;
             HIPCHECK(hipMemcpyAsync(&hinfo, di, sizeof(hinfo), hipMemcpyDeviceToHost, st));
             HIPCHECK(launch_blockdiag_right(dA, rho, rho_scale, dY, D, st));       // Y = M rho0
-            HIPCHECK(launch_zgemm(dY, dA, dR, D, 1, 1.0, 0.0, st, true));          // Rt = M rho0 M^dagger
+            HIPCHECK(zgemm(dY, dA, dR, D, 1, 1.0, st, true));          // Rt = M rho0 M^dagger
             for (int s0 = 0; s0 < S; ++s0) {
                 double P[32];
                 for (int i = 0; i < 16; ++i) {
@@ -1509,7 +1534,7 @@ msg_status msg_symmetric_decay(msg_ctx *ctx, const msg_system *sys, int32_t n_it
                     P[2 * i + 1] = r.P_states[s0][i].imag();
                 }
                 HIPCHECK(launch_blockdiag_right(dA, P, 1.0, dY, D, st));           // Y = M P
-                HIPCHECK(launch_zgemm(dY, dA, dP, D, 1, 1.0, 0.0, st, true));      // Pt = M P M^dagger
+                HIPCHECK(zgemm(dY, dA, dP, D, 1, 1.0, st, true));      // Pt = M P M^dagger
                 HIPCHECK(launch_lorentz_sum(dP, dR, dl, k, D, drows, dres + (size_t)it * S + s0, st));
             }
             HIPCHECK(hipStreamSynchronize(st));
@@ -1601,11 +1626,11 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
                 rocblas_int hinfo = 0;
                 HIPCHECK(hipMemcpyAsync(&hinfo, di, sizeof(hinfo), hipMemcpyDeviceToHost, st));
                 if (given) {
-                    HIPCHECK(launch_zgemm(dA, dC, dY, D, 1, 1.0, 0.0, st));         // Y = M C
-                    HIPCHECK(launch_zgemm(dY, dY, dR, D, 1, 1.0, 0.0, st, true));   // Rt = Y Y^dagger
+                    HIPCHECK(zgemm(dA, dC, dY, D, 1, 1.0, st));         // Y = M C
+                    HIPCHECK(zgemm(dY, dY, dR, D, 1, 1.0, st, true));   // Rt = Y Y^dagger
                 } else {
                     HIPCHECK(launch_blockdiag_right(dA, rho_e, inv_z, dY, D, st));  // Y = M rho0
-                    HIPCHECK(launch_zgemm(dY, dA, dR, D, 1, 1.0, 0.0, st, true));   // Rt = M rho0 M^dagger
+                    HIPCHECK(zgemm(dY, dA, dR, D, 1, 1.0, st, true));   // Rt = M rho0 M^dagger
                 }
                 for (int c = 0; c < nobs; ++c) {
                     double P[32];
@@ -1614,7 +1639,7 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
                         P[2 * i + 1] = Pobs[c][i].imag();
                     }
                     HIPCHECK(launch_blockdiag_right(dA, P, 1.0, dY, D, st));           // Y = M P
-                    HIPCHECK(launch_zgemm(dY, dA, dP, D, 1, 1.0, 0.0, st, true));      // Pt = M P M^dagger
+                    HIPCHECK(zgemm(dY, dA, dP, D, 1, 1.0, st, true));      // Pt = M P M^dagger
                     HIPCHECK(launch_timeinf_sum(dP, dR, dl, 2.0 * r.kappa, D, drows, dres + (size_t)it * nobs + c, st));
                 }
                 HIPCHECK(hipStreamSynchronize(st));
@@ -1636,7 +1661,7 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
             // rho0 (device order, row-major), replicated over the chunk
             if (given) {
                 HIPCHECK(hipMemcpyAsync(dX, Cpad.data(), DD * sizeof(cd), hipMemcpyHostToDevice, st));
-                HIPCHECK(launch_zgemm(dX, dX, dR, D, 1, 1.0, 0.0, st, true));  // C C^dagger
+                HIPCHECK(zgemm(dX, dX, dR, D, 1, 1.0, st, true));  // C C^dagger
             } else {
                 std::vector<cd> rho0(DD, cd(0.0));
                 for (int nu = 0; nu < D / 4; ++nu)
@@ -1681,7 +1706,7 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
                 cd *dS = dX3, *dY = dX4;
                 const cd *src = dR;
                 for (int n = N - 1; n >= 0; --n) {
-                    HIPCHECK(launch_zgemm(dG, src, dY, D, nb, 1.0, 0.0, st));
+                    HIPCHECK(zgemm(dG, src, dY, D, nb, 1.0, st));
                     HIPCHECK(launch_herm_update(dY, dR, dS, D, nb, tau / (n + 2), n == 0 ? tau : 1.0, st));
                     src = dS;
                 }
@@ -1699,13 +1724,13 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
                 const double one_one[4] = {0.0, 1.0, 1.0, 0.0};
                 bool done = false;
                 for (int d = 0; d < kMaxDoublings && !done; ++d) {
-                    HIPCHECK(launch_zgemm(dU, dXc, dW, D, nb, 1.0, 0.0, st));              // W = U X
-                    HIPCHECK(launch_zgemm(dW, dU, dXn, D, nb, 1.0, 0.0, st, true));        // Delta = W U^dagger
+                    HIPCHECK(zgemm(dU, dXc, dW, D, nb, 1.0, st));              // W = U X
+                    HIPCHECK(zgemm(dW, dU, dXn, D, nb, 1.0, st, true));        // Delta = W U^dagger
                     HIPCHECK(launch_fro2(dXn, D, nb, dn, st));
                     HIPCHECK(launch_fro2(dXc, D, nb, dn + chunk, st));
                     HIPCHECK(launch_zlincomb(dXc, dXn, dXn, dW, D, nb, one_one, st));      // X + Delta
                     std::swap(dXc, dW);
-                    HIPCHECK(launch_zgemm(dU, dU, dU2, D, nb, 1.0, 0.0, st));              // U^2
+                    HIPCHECK(zgemm(dU, dU, dU2, D, nb, 1.0, st));              // U^2
                     std::swap(dU, dU2);
                     HIPCHECK(hipMemcpyAsync(fro.data(), dn, 2 * (size_t)chunk * sizeof(double), hipMemcpyDeviceToHost, st));
                     HIPCHECK(hipStreamSynchronize(st));

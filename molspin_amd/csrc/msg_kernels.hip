@@ -1372,64 +1372,10 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
 }
 
 // ---------------------------------------------------------------------------
-// Device FP64 matrix exponential for the dense path: exp(X) of a batch of
-// n x n complex matrices by a degree-18 Taylor polynomial (Horner form) of
-// X / 2^s followed by s squarings.  Row-major double2 matrices.
+// Device FP64 matrix exponential for the dense path: exp(X) of a batch of n x n
+// complex matrices by a degree-18 Taylor polynomial (Paterson-Stockmeyer, the
+// products on rocBLAS) of X / 2^s followed by s squarings.  Row-major double2.
 // ---------------------------------------------------------------------------
-constexpr int kZT = 32;  // output tile (kZT x kZT) per workgroup of 256 threads
-
-// C_b = diag_add * I + coef * A_b op(B_b) (+ E_b when E is given) for every batch
-// entry b; op(B) = B, or B^dagger when conj_b is set.
-__global__ __launch_bounds__(256) void k_zgemm(const double2 *__restrict__ A, const double2 *__restrict__ B,
-                                               double2 *__restrict__ C, int n, double coef, double diag_add,
-                                               int conj_b, const double2 *__restrict__ E) {
-    __shared__ double2 sA[kZT][kZT + 1], sB[kZT][kZT + 1];
-    const size_t off = (size_t)blockIdx.z * n * n;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads, 4 rows each
-    const int row0 = blockIdx.y * kZT, col0 = blockIdx.x * kZT;
-    double2 acc[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc[r] = make_double2(0.0, 0.0);
-    for (int k0 = 0; k0 < n; k0 += kZT) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = ty + 8 * r;
-            sA[i][tx] = (row0 + i < n && k0 + tx < n) ? A[off + (size_t)(row0 + i) * n + k0 + tx] : make_double2(0.0, 0.0);
-            if (conj_b) {  // sB[k][c] = conj(B[col0 + c][k0 + k]), read along rows of B
-                const double2 v = (col0 + i < n && k0 + tx < n) ? B[off + (size_t)(col0 + i) * n + k0 + tx]
-                                                                 : make_double2(0.0, 0.0);
-                sB[tx][i] = make_double2(v.x, -v.y);
-            } else {
-                sB[i][tx] = (k0 + i < n && col0 + tx < n) ? B[off + (size_t)(k0 + i) * n + col0 + tx] : make_double2(0.0, 0.0);
-            }
-        }
-        __syncthreads();
-#pragma unroll 8
-        for (int k = 0; k < kZT; ++k) {
-            const double2 b = sB[k][tx];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const double2 a = sA[ty + 8 * r][k];
-                acc[r].x = fma(a.x, b.x, fma(-a.y, b.y, acc[r].x));
-                acc[r].y = fma(a.x, b.y, fma(a.y, b.x, acc[r].y));
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int i = row0 + ty + 8 * r, j = col0 + tx;
-        if (i < n && j < n) {
-            double2 v = make_double2(coef * acc[r].x + (i == j ? diag_add : 0.0), coef * acc[r].y);
-            if (E) {
-                const double2 e = E[off + (size_t)i * n + j];
-                v.x += e.x;
-                v.y += e.y;
-            }
-            C[off + (size_t)i * n + j] = v;
-        }
-    }
-}
 
 // out_b = c0 I + c1 X_b + c2 X2_b + c3 X3_b (a block of the Paterson-Stockmeyer
 // evaluation of the Taylor polynomial); batch entries of n x n, contiguous.
@@ -1558,9 +1504,10 @@ __global__ void k_herm_update(const double2 *__restrict__ Y, const double2 *__re
     }
 }
 
-// out[b] = sum_ij |U_b[i][j]|^2; one workgroup per batch entry, fixed reduction order.
-__global__ __launch_bounds__(256) void k_fro2(const double2 *__restrict__ U, int n, double *__restrict__ out) {
-    __shared__ double red[256];
+// out[b] = sum_ij |U_b[i][j]|^2; one 1024-thread workgroup per batch entry, fixed
+// reduction order.
+__global__ __launch_bounds__(1024) void k_fro2(const double2 *__restrict__ U, int n, double *__restrict__ out) {
+    __shared__ double red[1024];
     const size_t nn = (size_t)n * n;
     const double2 *u = U + (size_t)blockIdx.x * nn;
     const int t = threadIdx.x;
@@ -1869,13 +1816,6 @@ int dense_occupancy(int dpad, size_t lds) {
     return 1;
 }
 
-hipError_t launch_zgemm(const void *A, const void *B, void *C, int n, int batch, double coef, double diag_add,
-                        hipStream_t st, bool conj_b, const void *E) {
-    const int nt = (n + kZT - 1) / kZT;
-    hipLaunchKernelGGL(k_zgemm, dim3(nt, nt, batch), dim3(256), 0, st, (const double2 *)A, (const double2 *)B,
-                       (double2 *)C, n, coef, diag_add, conj_b ? 1 : 0, (const double2 *)E);
-    return hipGetLastError();
-}
 
 hipError_t launch_zlincomb(const void *X, const void *X2, const void *X3, void *out, int n, int batch,
                            const double *c, hipStream_t st) {
@@ -1953,7 +1893,7 @@ hipError_t launch_herm_update(const void *Y, const void *R, void *S, int n, int 
 }
 
 hipError_t launch_fro2(const void *U, int n, int batch, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_fro2, dim3(batch), dim3(256), 0, st, (const double2 *)U, n, out);
+    hipLaunchKernelGGL(k_fro2, dim3(batch), dim3(1024), 0, st, (const double2 *)U, n, out);
     return hipGetLastError();
 }
 
