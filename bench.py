@@ -146,9 +146,10 @@ def lanczos_roofline(D, m, state_steps, kernel_s, traffic):
 # device copy: the FETCH_SIZE count is doubled, WRITE_SIZE taken as is); the
 # traffic of a launch scales with its state-steps at fixed D.  The counter runs
 # are summarised in profiles/k_lanczos_orientation_batch_D4096.md,
-# profiles/k_lanczos_big_trajectories_D65536.md and profiles/k_dense_field_sweep_D512.md.
+# profiles/k_lanczos_big_trajectories_D65536.md, profiles/k_lanczos_big_pair_D16384.md,
+# profiles/k_dense_field_sweep_D512.md and profiles/k_arnoldi_singlet_sink_D4096.md.
 MEASURED_TRAFFIC_PER_STATE_STEP = {"orientations": 1.066e+06, "trajectories": 7.523e+07, "dense": 2.51e+04,
-                                   "pair": 6.60e+06}
+                                   "pair": 6.60e+06, "arnoldi": 4.825e+06}
 
 
 def orientation_batch(ctx, args):
@@ -290,8 +291,9 @@ def variant_batch(ctx, args):
         # Arnoldi: per state-step m basis stores, the Gram-Schmidt reads of the earlier
         # vectors (twice each), and m reads in the final combination: 8 D m (m + 1) bytes
         bss = 8.0 * D * m * (m + 1)
+        tr = MEASURED_TRAFFIC_PER_STATE_STEP["arnoldi"] * state_steps
         return {"bound": "hbm", "achieved": bss * state_steps / kernel_s / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": bss * state_steps / kernel_s / HBM_PEAK, "traffic": None,
+                "unit": "GB/s", "frac": bss * state_steps / kernel_s / HBM_PEAK, "traffic": tr,
                 "kernel_ms": kernel_s * 1e3, "bytes_per_state_step": bss, "state_steps_per_launch": state_steps}
 
     what = ("linearly polarized 5 MHz, 50 uT field on both electrons (dynamics=dynamic)" if rydmr

@@ -308,6 +308,72 @@ void taylor_batch(cd *dX, cd *dX2, cd *dX3, cd *dX4, cd *dB, cd *&dP, cd *&dQ, i
     }
 }
 
+double norm1(const cd *g, int D) {  // max column sum of a row-major D x D matrix
+    std::vector<double> cs(D, 0.0);
+    for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) cs[j] += std::abs(g[(size_t)i * D + j]);
+    return *std::max_element(cs.begin(), cs.end());
+}
+
+// Generators of a field sweep.  H(f) = H(0) + sum_a f_a Z_a is linear in the swept
+// field, so the host builds H(0) and the three unit-field operators once and the device
+// forms every item's scaled generator s_q (G0 + sum_a f_a Z_a) (k_dyn_generator), with
+// G0 = -i H(0) - K (x) 1 (with_k) or -i H(0), and Z_a = -i (H(e_a) - H(0)).
+struct SweepGenerators {
+    DBuf G0, Z, coef, item, scale;
+    double g0norm = 0.0, znorm[3] = {0.0, 0.0, 0.0};
+    std::vector<double> hcoef, hscale;
+
+    void build(const Resolved &r, int D, bool with_k, int chunk, hipStream_t st) {
+        const size_t DD = (size_t)D * D;
+        const double zero[3] = {0.0, 0.0, 0.0};
+        std::vector<cd> H, H1, g(DD), z(3 * DD);
+        dense_hamiltonian(r, zero, H);
+        for (size_t k = 0; k < DD; ++k) g[k] = cd(H[k].imag(), -H[k].real());  // -i H(0)
+        if (with_k)
+            for (int nu = 0; nu < D / 4; ++nu)
+                for (int i = 0; i < 4; ++i)
+                    for (int j = 0; j < 4; ++j) g[(size_t)(4 * nu + i) * D + 4 * nu + j] -= r.K[i * 4 + j];
+        g0norm = norm1(g.data(), D);
+        for (int ax = 0; ax < 3; ++ax) {
+            double e[3] = {0.0, 0.0, 0.0};
+            e[ax] = 1.0;
+            dense_hamiltonian(r, e, H1);
+            cd *za = &z[(size_t)ax * DD];
+            for (size_t k = 0; k < DD; ++k) {
+                const cd d = H1[k] - H[k];
+                za[k] = cd(d.imag(), -d.real());  // -i Z_a
+            }
+            znorm[ax] = norm1(za, D);
+        }
+        HIPCHECK(hipMemcpyAsync(G0.alloc<cd>(DD, st), g.data(), DD * sizeof(cd), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(Z.alloc<cd>(3 * DD, st), z.data(), 3 * DD * sizeof(cd), hipMemcpyHostToDevice, st));
+        coef.alloc<double>((size_t)3 * chunk, st);
+        scale.alloc<double>(chunk, st);
+        item.alloc<int>(chunk, st);
+        HIPCHECK(hipMemsetAsync(item.p, 0, (size_t)chunk * sizeof(int), st));  // every entry uses G0
+        HIPCHECK(hipStreamSynchronize(st));  // g and z are released at scope end
+        hcoef.resize((size_t)3 * chunk);
+        hscale.resize(chunk);
+    }
+    // upper bound of ||G(f)||_1
+    double bound(const double *f) const {
+        return g0norm + std::fabs(f[0]) * znorm[0] + std::fabs(f[1]) * znorm[1] + std::fabs(f[2]) * znorm[2];
+    }
+    // out[q] = s (G0 + sum_a f_a Z_a), f = fields + 3 q, for q < nb
+    void form(const double *fields, int nb, double s, int D, cd *out, hipStream_t st) {
+        for (int q = 0; q < nb; ++q) {
+            for (int ax = 0; ax < 3; ++ax) hcoef[3 * (size_t)q + ax] = fields[3 * (size_t)q + ax];
+            hscale[q] = s;
+        }
+        HIPCHECK(hipMemcpyAsync(coef.p, hcoef.data(), (size_t)3 * nb * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(scale.p, hscale.data(), (size_t)nb * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHECK(launch_dyn_generator(G0.p, Z.p, coef.get<double>(), item.get<int>(), scale.get<double>(), 3, D, nb, out,
+                                      st));
+        HIPCHECK(hipStreamSynchronize(st));  // hcoef / hscale are refilled by the next call
+    }
+};
+
 // Padded dimension of the dense path: 16 x (one of the row-tile counts the
 // kernel is compiled for); 0 above 512.
 int dense_padding(int D) {
@@ -559,11 +625,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                 for (int nu = 0; nu < D / 4; ++nu)
                     for (int i = 0; i < 4; ++i)
                         for (int j = 0; j < 4; ++j) g[(size_t)(4 * nu + i) * D + 4 * nu + j] -= r.K[i * 4 + j];
-            for (int j = 0; j < D; ++j) {
-                double cs = 0.0;
-                for (int i = 0; i < D; ++i) cs += std::abs(g[(size_t)i * D + j]);
-                b.dyn_gnorm[it] = std::max(b.dyn_gnorm[it], cs);
-            }
+            b.dyn_gnorm[it] = norm1(g, D);
         }
         cd *dG = b.dynG.alloc<cd>(all.size(), stream);
         HIPCHECK(hipMemcpyAsync(dG, all.data(), all.size() * sizeof(cd), hipMemcpyHostToDevice, stream));
@@ -580,11 +642,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                 spin_operator_dense(r, td_spins[q], ax, H);
                 cd *z = &Z[k * DD];
                 for (size_t e = 0; e < DD; ++e) z[e] = cd(H[e].imag(), -H[e].real());
-                for (int j = 0; j < D; ++j) {
-                    double cs = 0.0;
-                    for (int i = 0; i < D; ++i) cs += std::abs(z[(size_t)i * D + j]);
-                    b.dyn_znorm[k] = std::max(b.dyn_znorm[k], cs);
-                }
+                b.dyn_znorm[k] = norm1(z, D);
             }
         cd *dZ = b.dynZ.alloc<cd>(Z.size(), stream);
         HIPCHECK(hipMemcpyAsync(dZ, Z.data(), Z.size() * sizeof(cd), hipMemcpyHostToDevice, stream));
@@ -620,83 +678,25 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         cd *dX4 = X4.alloc<cd>((size_t)chunk * D * D, stream);
         cd *dB = Bk.alloc<cd>((size_t)chunk * D * D, stream);
         std::vector<cd> H, Hs;
-        // A field sweep: H(f) = H(0) + sum_a f_a Z_a is linear in the swept field, so the
-        // host builds H(0) and the three unit-field operators once and the device forms
-        // every item's generator (k_dyn_generator) instead of one host H per item.
+        // a field sweep forms every item's generator on the device (SweepGenerators)
         const bool linear = fields != nullptr && n_items > 1;
         const size_t DD = (size_t)D * D;
-        DBuf bG0, bZ, bcoef, bitem, bdt;
-        double g0norm = 0.0, znorm[3] = {0.0, 0.0, 0.0};
-        std::vector<double> lcoef, ldt;
-        std::vector<int> litem;
-        if (linear) {
-            const double zero[3] = {0.0, 0.0, 0.0};
-            dense_hamiltonian(r, zero, H);
-            std::vector<cd> G0(DD), Z(3 * DD), H1;
-            for (size_t k = 0; k < DD; ++k) G0[k] = cd(H[k].imag(), -H[k].real());  // -i H(0)
-            if (!factor_decay)
-                for (int nu = 0; nu < D / 4; ++nu)
-                    for (int i = 0; i < 4; ++i)
-                        for (int j = 0; j < 4; ++j) G0[(size_t)(4 * nu + i) * D + 4 * nu + j] -= r.K[i * 4 + j];
-            auto norm1 = [&](const cd *g) {
-                double m1 = 0.0;
-                for (int j = 0; j < D; ++j) {
-                    double cs = 0.0;
-                    for (int i = 0; i < D; ++i) cs += std::abs(g[(size_t)i * D + j]);
-                    m1 = std::max(m1, cs);
-                }
-                return m1;
-            };
-            g0norm = norm1(G0.data());
-            for (int ax = 0; ax < 3; ++ax) {
-                double e[3] = {0.0, 0.0, 0.0};
-                e[ax] = 1.0;
-                dense_hamiltonian(r, e, H1);
-                cd *z = &Z[(size_t)ax * DD];
-                for (size_t k = 0; k < DD; ++k) {
-                    const cd d = H1[k] - H[k];
-                    z[k] = cd(d.imag(), -d.real());  // -i Z_a
-                }
-                znorm[ax] = norm1(z);
-            }
-            cd *dG0 = bG0.alloc<cd>(DD, stream), *dZ = bZ.alloc<cd>(3 * DD, stream);
-            HIPCHECK(hipMemcpyAsync(dG0, G0.data(), DD * sizeof(cd), hipMemcpyHostToDevice, stream));
-            HIPCHECK(hipMemcpyAsync(dZ, Z.data(), 3 * DD * sizeof(cd), hipMemcpyHostToDevice, stream));
-            HIPCHECK(hipStreamSynchronize(stream));  // G0 and Z are released at scope end
-            bcoef.alloc<double>((size_t)3 * chunk, stream);
-            bitem.alloc<int>(chunk, stream);
-            bdt.alloc<double>(chunk, stream);
-            HIPCHECK(hipMemsetAsync(bitem.p, 0, (size_t)chunk * sizeof(int), stream));  // every entry uses G0
-            lcoef.resize((size_t)3 * chunk);
-            ldt.resize(chunk);
-        } else {
+        SweepGenerators sweep;
+        if (linear)
+            sweep.build(r, D, !factor_decay, chunk, stream);
+        else
             Hs.resize((size_t)chunk * DD);
-        }
         for (int w = 0; w < per; ++w) {
             const double dt = w == 0 ? dts[0] : dts[ns - 2];
             for (int it0 = 0; it0 < n_items; it0 += chunk) {
                 const int nb = std::min(chunk, n_items - it0);
                 double nrm = 0.0;
                 if (linear) {
-                    for (int q = 0; q < nb; ++q) {
-                        const double *f = fields + 3 * (size_t)(it0 + q);
-                        double bound = g0norm;
-                        for (int ax = 0; ax < 3; ++ax) {
-                            lcoef[3 * (size_t)q + ax] = f[ax];
-                            bound += std::fabs(f[ax]) * znorm[ax];
-                        }
-                        nrm = std::max(nrm, dt * bound);
-                    }
+                    for (int q = 0; q < nb; ++q) nrm = std::max(nrm, dt * sweep.bound(fields + 3 * (size_t)(it0 + q)));
                     int sq = 0;
                     const bool rk4 = p->method == MSG_METHOD_RK4;
                     while (!rk4 && nrm > 0.5) { nrm *= 0.5; ++sq; }
-                    for (int q = 0; q < nb; ++q) ldt[q] = std::ldexp(dt, -sq);
-                    HIPCHECK(hipMemcpyAsync(bcoef.p, lcoef.data(), (size_t)3 * nb * sizeof(double), hipMemcpyHostToDevice,
-                                            stream));
-                    HIPCHECK(hipMemcpyAsync(bdt.p, ldt.data(), (size_t)nb * sizeof(double), hipMemcpyHostToDevice, stream));
-                    HIPCHECK(launch_dyn_generator(bG0.p, bZ.p, bcoef.get<double>(), bitem.get<int>(), bdt.get<double>(), 3,
-                                                  D, nb, dX, stream));
-                    HIPCHECK(hipStreamSynchronize(stream));  // lcoef / ldt are refilled for the next chunk
+                    sweep.form(fields + 3 * (size_t)it0, nb, std::ldexp(dt, -sq), D, dX, stream);
                     taylor_batch(dX, dX2, dX3, dX4, dB, dP, dQ, D, nb, rk4, stream);
                     for (int q = 0; q < sq; ++q) {
                         HIPCHECK(zgemm(dP, dP, dQ, D, nb, 1.0, stream));
@@ -713,11 +713,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                             for (int i = 0; i < 4; ++i)
                                 for (int j = 0; j < 4; ++j)
                                     H[(size_t)(nu * 4 + i) * D + nu * 4 + j] += cd(0.0, -1.0) * r.K[i * 4 + j];
-                    for (int j = 0; j < D; ++j) {  // 1-norm of A = -i dt H_eff
-                        double cs = 0.0;
-                        for (int i = 0; i < D; ++i) cs += std::abs(H[(size_t)i * D + j]);
-                        nrm = std::max(nrm, cs * dt);
-                    }
+                    nrm = std::max(nrm, norm1(H.data(), D) * dt);  // 1-norm of A = -i dt H_eff
                     for (size_t k = 0; k < H.size(); ++k) Hs[(size_t)q * D * D + k] = cd(0.0, -dt) * H[k];
                 }
                 int sq = 0;
@@ -1674,28 +1670,37 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
             }
             for (int q = 1; q < chunk; ++q)
                 HIPCHECK(hipMemcpyAsync(dR + (size_t)q * DD, dR, DD * sizeof(cd), hipMemcpyDeviceToDevice, st));
-            std::vector<cd> Gh((size_t)chunk * DD);
+            // generators: a field sweep forms G(f) = G(0) + sum_a f_a Z_a on the device
+            // (SweepGenerators); a single item or no fields builds G on the host
+            const bool linear = fields != nullptr && n_items > 1;
+            SweepGenerators sweep;
+            std::vector<cd> Gh;
+            if (linear)
+                sweep.build(r, D, true, chunk, st);
+            else
+                Gh.resize((size_t)chunk * DD);
             std::vector<double> fro(2 * (size_t)chunk);
             std::vector<cd> tr((size_t)chunk * 16);
             for (int it0 = 0; it0 < n_items; it0 += chunk) {
                 const int nb = std::min(chunk, n_items - it0);
                 double nrm = 0.0;
-                for (int q = 0; q < nb; ++q) {
-                    dense_hamiltonian(r, fields ? fields + 3 * (it0 + q) : nullptr, H);
-                    cd *g = &Gh[(size_t)q * DD];
-                    for (size_t k = 0; k < DD; ++k) g[k] = cd(H[k].imag(), -H[k].real());  // -i H
-                    for (int nu = 0; nu < D / 4; ++nu)
-                        for (int a = 0; a < 4; ++a)
-                            for (int c = 0; c < 4; ++c) g[(size_t)(4 * nu + a) * D + 4 * nu + c] -= r.K[a * 4 + c];
-                    for (int j = 0; j < D; ++j) {  // 1-norm
-                        double cs = 0.0;
-                        for (int i = 0; i < D; ++i) cs += std::abs(g[(size_t)i * D + j]);
-                        nrm = std::max(nrm, cs);
+                if (linear) {
+                    for (int q = 0; q < nb; ++q) nrm = std::max(nrm, sweep.bound(fields + 3 * (size_t)(it0 + q)));
+                    sweep.form(fields + 3 * (size_t)it0, nb, 1.0, D, dG, st);
+                } else {
+                    for (int q = 0; q < nb; ++q) {
+                        dense_hamiltonian(r, fields ? fields + 3 * (it0 + q) : nullptr, H);
+                        cd *g = &Gh[(size_t)q * DD];
+                        for (size_t k = 0; k < DD; ++k) g[k] = cd(H[k].imag(), -H[k].real());  // -i H
+                        for (int nu = 0; nu < D / 4; ++nu)
+                            for (int a = 0; a < 4; ++a)
+                                for (int c = 0; c < 4; ++c) g[(size_t)(4 * nu + a) * D + 4 * nu + c] -= r.K[a * 4 + c];
+                        nrm = std::max(nrm, norm1(g, D));
                     }
+                    HIPCHECK(hipMemcpyAsync(dG, Gh.data(), (size_t)nb * DD * sizeof(cd), hipMemcpyHostToDevice, st));
                 }
                 if (!(nrm > 0.0)) throw ArgError{MSG_NUMERIC_BREAKDOWN, diverges};
                 const double tau = 0.25 / nrm;  // ||tau G||_1 <= 1/4
-                HIPCHECK(hipMemcpyAsync(dG, Gh.data(), (size_t)nb * DD * sizeof(cd), hipMemcpyHostToDevice, st));
                 // U = e^{tau G}: Taylor(18) of tau G (no squaring needed at norm 1/4)
                 const double cs[4] = {0.0, tau, 0.0, 0.0};
                 HIPCHECK(launch_zlincomb(dG, dG, dG, dX, D, nb, cs, st));
