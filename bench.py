@@ -84,9 +84,13 @@ def main():
     else:
         run = dense_sweep(ctx, args)
 
+    def progress(what):
+        print("[bench] %s" % what, file=sys.stderr, flush=True)
+
     for i in range(args.warmup):
         run["step"](i)
     torch.cuda.synchronize() if torch.cuda.is_available() else None
+    progress("%d warmup steps done" % args.warmup)
     kms = []
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -97,16 +101,20 @@ def main():
     value = run["state_steps"] * args.steps / wall
     kernel_s = float(np.mean(kms)) * 1e-3
     roof = run["roofline"](kernel_s)
+    progress("%d timed steps done in %.2f s" % (args.steps, wall))
 
     cpu = None
     if not args.no_cpu and run.get("packed") is not None:
         cpu = cpu_baseline(run["packed"], run["cpu_prm"], run["cpu_field"], run["cpu_columns"], args.cpu_seconds)
+        progress("CPU baseline done")
     serving = None
     if not args.no_serving and args.workload == "orientations":
         serving = serve(ctx, args.requests, args.serve_intervals)
+        progress("serving stream done")
         # the same request stream through the exact uncoupled-radical method
         serving["uncoupled_method"] = serve(ctx, args.requests, args.serve_intervals, method="uncoupled",
                                             compare_plain=False)
+        progress("serving stream through the uncoupled method done")
 
     line = {
         "metric": METRIC,

@@ -48,7 +48,8 @@ hipError_t launch_herm_update(const void *Y, const void *R, void *S, int n, int 
                               hipStream_t st);
 hipError_t launch_dyn_generator(const void *G, const void *Z, const double *coef, const int *item, const double *dt,
                                int nz, int n, int batch, void *out, hipStream_t st);
-hipError_t launch_fro2(const void *U, int n, int batch, double *out, hipStream_t st);
+hipError_t launch_accumulate_fro(const void *X, const void *Delta, void *W, int n, int batch, int parts, double *part,
+                                 hipStream_t st);
 hipError_t launch_pair_block_trace(const void *X, int n, int batch, void *out, hipStream_t st);
 hipError_t launch_pauli_right(const void *M, void *Y, int n, int mu, int batch, hipStream_t st);
 hipError_t launch_corr(const void *S, size_t plane, const double *lam, const double *times, int n, int ns, int batch,
@@ -1652,7 +1653,8 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
             cd *dX3 = bX3.alloc<cd>((size_t)chunk * DD, st), *dX4 = bX4.alloc<cd>((size_t)chunk * DD, st);
             cd *dB = bB.alloc<cd>((size_t)chunk * DD, st), *dP = bP.alloc<cd>((size_t)chunk * DD, st);
             cd *dQ = bQ.alloc<cd>((size_t)chunk * DD, st);
-            double *dn = bn.alloc<double>(2 * (size_t)chunk, st);
+            const int parts = 64;  // slabs per matrix of the fused accumulation and norms
+            double *dn = bn.alloc<double>(2 * (size_t)chunk * parts, st);
             cd *dtr = bt.alloc<cd>((size_t)chunk * 16, st);
             // rho0 (device order, row-major), replicated over the chunk
             if (given) {
@@ -1679,7 +1681,7 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
                 sweep.build(r, D, true, chunk, st);
             else
                 Gh.resize((size_t)chunk * DD);
-            std::vector<double> fro(2 * (size_t)chunk);
+            std::vector<double> fro(2 * (size_t)chunk * parts);
             std::vector<cd> tr((size_t)chunk * 16);
             for (int it0 = 0; it0 < n_items; it0 += chunk) {
                 const int nb = std::min(chunk, n_items - it0);
@@ -1726,24 +1728,27 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
                 // T = tau 2^36 (drift < 1e-5) and reports divergence there.
                 const int kMaxDoublings = 36;
                 cd *dXc = dS, *dXn = dB, *dW = dY;
-                const double one_one[4] = {0.0, 1.0, 1.0, 0.0};
                 bool done = false;
                 for (int d = 0; d < kMaxDoublings && !done; ++d) {
                     HIPCHECK(zgemm(dU, dXc, dW, D, nb, 1.0, st));              // W = U X
                     HIPCHECK(zgemm(dW, dU, dXn, D, nb, 1.0, st, true));        // Delta = W U^dagger
-                    HIPCHECK(launch_fro2(dXn, D, nb, dn, st));
-                    HIPCHECK(launch_fro2(dXc, D, nb, dn + chunk, st));
-                    HIPCHECK(launch_zlincomb(dXc, dXn, dXn, dW, D, nb, one_one, st));      // X + Delta
+                    HIPCHECK(launch_accumulate_fro(dXc, dXn, dW, D, nb, parts, dn, st));  // X + Delta, norms
                     std::swap(dXc, dW);
                     HIPCHECK(zgemm(dU, dU, dU2, D, nb, 1.0, st));              // U^2
                     std::swap(dU, dU2);
-                    HIPCHECK(hipMemcpyAsync(fro.data(), dn, 2 * (size_t)chunk * sizeof(double), hipMemcpyDeviceToHost, st));
+                    HIPCHECK(hipMemcpyAsync(fro.data(), dn, 2 * (size_t)nb * parts * sizeof(double), hipMemcpyDeviceToHost,
+                                            st));
                     HIPCHECK(hipStreamSynchronize(st));
                     done = true;
                     for (int q = 0; q < nb; ++q) {
-                        if (!std::isfinite(fro[q]) || !std::isfinite(fro[chunk + q]))
+                        double fd = 0.0, fx = 0.0;  // ||Delta||_F^2, ||X||_F^2
+                        for (int k = 0; k < parts; ++k) {
+                            fd += fro[2 * ((size_t)q * parts + k)];
+                            fx += fro[2 * ((size_t)q * parts + k) + 1];
+                        }
+                        if (!std::isfinite(fd) || !std::isfinite(fx))
                             throw ArgError{MSG_NUMERIC_BREAKDOWN, "non-finite value in the time-integral doubling"};
-                        done &= fro[q] <= 1e-20 * fro[chunk + q];
+                        done &= fd <= 1e-20 * fx;
                     }
                 }
                 if (!done) {

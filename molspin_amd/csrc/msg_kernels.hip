@@ -166,6 +166,108 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
     return y;
 }
 
+// H x for the G groups nu_g = g 2^LOGT + t of one thread (t < 2^LOGT, NN nuclei,
+// NN >= LOGT).  The index bits below LOGT are the thread's and shared by its G groups,
+// so those nuclei's blocks, field sums and shift are formed once per thread; the bits
+// from LOGT up are g's and known at compile time, so their blocks need no lane select.
+template <int NN, int G, int LOGT>
+__device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *__restrict__ co,
+                                               const float *__restrict__ hd, int t, const C4 (&x)[G],
+                                               const float4 *LA, const float4 *LB, C4 (&y)[G]) {
+    float f1x = hd[0], f1y = hd[1], f1z = hd[2];
+    float f2x = hd[3], f2y = hd[4], f2z = hd[5];
+    float rr = hd[6];
+    float e1[G][3], e2[G][3], er[G];  // contributions of the nuclei on g's bits
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) y[g].c[c] = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) e1[g][k] = e2[g][k] = 0.f;
+        er[g] = 0.f;
+    }
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+        const float *cn = co + kCoefHead + n * kNucStride;
+        const int fl = a.nuc_flags[n];
+        float2 m[4];
+        if (n < LOGT) {
+            const bool down = (t >> n) & 1;
+            const float s = down ? -1.f : 1.f;
+            C4 nb[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) nb[g] = load_group(LA, LB, (g << LOGT) + (t ^ (1 << n)));
+            if (fl & NUC_E1) {
+                f1x = fmaf(s, cn[0], f1x); f1y = fmaf(s, cn[1], f1y); f1z = fmaf(s, cn[2], f1z);
+                load_block(cn + 7, down, m);
+#pragma unroll
+                for (int g = 0; g < G; ++g) apply_e1(m, nb[g], y[g]);
+            }
+            if (fl & NUC_E2) {
+                f2x = fmaf(s, cn[3], f2x); f2y = fmaf(s, cn[4], f2y); f2z = fmaf(s, cn[5], f2z);
+                load_block(cn + 15, down, m);
+#pragma unroll
+                for (int g = 0; g < G; ++g) apply_e2(m, nb[g], y[g]);
+            }
+            if (fl & NUC_T) {
+                rr = fmaf(s, cn[6], rr);
+                const float2 tt = make_float2(cn[23], down ? -cn[24] : cn[24]);
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) y[g].c[c] = cfma(tt, nb[g].c[c], y[g].c[c]);
+            }
+        } else {
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const bool down = ((g << LOGT) >> n) & 1;
+                const float s = down ? -1.f : 1.f;
+                const C4 nb = load_group(LA, LB, ((g << LOGT) ^ (1 << n)) + t);
+                if (fl & NUC_E1) {
+                    e1[g][0] = fmaf(s, cn[0], e1[g][0]); e1[g][1] = fmaf(s, cn[1], e1[g][1]);
+                    e1[g][2] = fmaf(s, cn[2], e1[g][2]);
+                    load_block(cn + 7, down, m);
+                    apply_e1(m, nb, y[g]);
+                }
+                if (fl & NUC_E2) {
+                    e2[g][0] = fmaf(s, cn[3], e2[g][0]); e2[g][1] = fmaf(s, cn[4], e2[g][1]);
+                    e2[g][2] = fmaf(s, cn[5], e2[g][2]);
+                    load_block(cn + 15, down, m);
+                    apply_e2(m, nb, y[g]);
+                }
+                if (fl & NUC_T) {
+                    er[g] = fmaf(s, cn[6], er[g]);
+                    const float2 tt = make_float2(cn[23], down ? -cn[24] : cn[24]);
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) y[g].c[c] = cfma(tt, nb.c[c], y[g].c[c]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const float rg = rr + er[g];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            y[g].c[c].x = fmaf(rg, x[g].c[c].x, y[g].c[c].x);
+            y[g].c[c].y = fmaf(rg, x[g].c[c].y, y[g].c[c].y);
+        }
+        float2 m[4];
+        field_matrix(f1x + e1[g][0], f1y + e1[g][1], f1z + e1[g][2], m);
+        apply_e1(m, x[g], y[g]);
+        field_matrix(f2x + e2[g][0], f2y + e2[g][1], f2z + e2[g][2], m);
+        apply_e2(m, x[g], y[g]);
+        if (a.op_flags & OP_EREST) {
+            const float *E = co + 8;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    y[g].c[r] = cfma(make_float2(E[8 * r + 2 * c], E[8 * r + 2 * c + 1]), x[g].c[c], y[g].c[r]);
+        }
+    }
+}
+
 // Electron-pair density of a group, accumulated as 16 reals:
 // R[0..3] = |x_r|^2, then for r < c the complex x_c conj(x_r) (6 pairs).
 __device__ __forceinline__ void accum_rho(const C4 &x, float (&R)[16]) {
@@ -464,13 +566,15 @@ __global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict_
                     const float4 *gA = reinterpret_cast<const float4 *>(bas + (size_t)(j > 0 ? j - 1 : 0) * 2 * D);
                     const float fb = (float)bprev;
                     // w = H v_j - beta_{j-1} v_{j-1}; one reduction gives alpha and |w|^2
-                    C4 w[G];
+                    C4 w[G], vjs[G];
                     double red[2] = {0.0, 0.0};
+#pragma unroll
+                    for (int g = 0; g < G; ++g) vjs[g] = load_group(cA, cB, g * T + t);
+                    apply_h_groups<Log2<ng>::value, G, Log2<T>::value>(a, co, hd, t, vjs, cA, cB, w);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
                         const int nu = g * T + t;
-                        const C4 vj = load_group(cA, cB, nu);
-                        w[g] = apply_h<Log2<ng>::value>(a, co, hd, nu, vj, cA, cB);
+                        const C4 &vj = vjs[g];
                         if (j > 0) {
                             const C4 vp = DBUF ? load_group(pA, pB, nu) : load_group(gA, gA + D4, nu);
 #pragma unroll
@@ -1493,33 +1597,60 @@ __global__ void k_dyn_generator(const double2 *__restrict__ G, const double2 *__
 // ---------------------------------------------------------------------------
 
 // S = s (R + c (Y + Y^dagger)) for every batch entry; Y = G S_prev (Hermitian S).
-__global__ void k_herm_update(const double2 *__restrict__ Y, const double2 *__restrict__ R, double2 *__restrict__ S,
-                              int n, size_t total, double c, double s) {
-    const size_t nn = (size_t)n * n;
-    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
-        const size_t e = idx % nn, base = idx - e;
-        const size_t i = e / n, j = e - i * n;
-        const double2 y = Y[idx], yt = Y[base + j * n + i], r = R[idx];
-        S[idx] = make_double2(s * (r.x + c * (y.x + yt.x)), s * (r.y + c * (y.y - yt.y)));
+// One 32 x 32 output tile per workgroup (grid (tiles, tiles, batch), 32 x 8 threads);
+// the mirrored tile of Y is staged through LDS so that both reads are coalesced.
+__global__ __launch_bounds__(256) void k_herm_update(const double2 *__restrict__ Y, const double2 *__restrict__ R,
+                                                     double2 *__restrict__ S, int n, double c, double s) {
+    __shared__ double2 tile[32][33];
+    const size_t off = (size_t)blockIdx.z * n * n;
+    const int bi = blockIdx.y * 32, bj = blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {  // tile[r][x] = Y[bj + r][bi + x]
+        const int row = bj + r, col = bi + tx;
+        if (row < n && col < n) tile[r][tx] = Y[off + (size_t)row * n + col];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int i = bi + r, j = bj + tx;
+        if (i < n && j < n) {
+            const size_t idx = off + (size_t)i * n + j;
+            const double2 y = Y[idx], yt = tile[tx][r], q = R[idx];  // yt = Y[j][i]
+            S[idx] = make_double2(s * (q.x + c * (y.x + yt.x)), s * (q.y + c * (y.y - yt.y)));
+        }
     }
 }
 
-// out[b] = sum_ij |U_b[i][j]|^2; one 1024-thread workgroup per batch entry, fixed
-// reduction order.
-__global__ __launch_bounds__(1024) void k_fro2(const double2 *__restrict__ U, int n, double *__restrict__ out) {
-    __shared__ double red[1024];
-    const size_t nn = (size_t)n * n;
-    const double2 *u = U + (size_t)blockIdx.x * nn;
+// One doubling accumulation W_b = X_b + Delta_b, with the partial Frobenius sums of
+// Delta_b and X_b over slab p of each matrix: part[2 (b P + p)] = sum |Delta|^2,
+// part[2 (b P + p) + 1] = sum |X|^2, P = gridDim.x.  Grid (P, batch); fixed
+// reduction order, so the sums do not depend on scheduling.
+__global__ __launch_bounds__(256) void k_accumulate_fro(const double2 *__restrict__ X, const double2 *__restrict__ Dl,
+                                                        double2 *__restrict__ W, size_t nn, double *__restrict__ part) {
+    __shared__ double rd[256], rx[256];
+    const size_t P = gridDim.x, p = blockIdx.x, b = blockIdx.y;
+    const size_t lo = nn * p / P, hi = nn * (p + 1) / P, off = b * nn;
     const int t = threadIdx.x;
-    double acc = 0.0;
-    for (size_t i = t; i < nn; i += blockDim.x) acc += u[i].x * u[i].x + u[i].y * u[i].y;
-    red[t] = acc;
+    double sd = 0.0, sx = 0.0;
+    for (size_t i = lo + t; i < hi; i += blockDim.x) {
+        const double2 x = X[off + i], d = Dl[off + i];
+        sd += d.x * d.x + d.y * d.y;
+        sx += x.x * x.x + x.y * x.y;
+        W[off + i] = make_double2(x.x + d.x, x.y + d.y);
+    }
+    rd[t] = sd;
+    rx[t] = sx;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (t < w) red[t] += red[t + w];
+        if (t < w) {
+            rd[t] += rd[t + w];
+            rx[t] += rx[t + w];
+        }
         __syncthreads();
     }
-    if (t == 0) out[blockIdx.x] = red[0];
+    if (t == 0) {
+        part[2 * (b * P + p)] = rd[0];
+        part[2 * (b * P + p) + 1] = rx[0];
+    }
 }
 
 // Electron-pair block of the partial trace over the nuclei (device index 4 nu + c):
@@ -1885,17 +2016,19 @@ hipError_t launch_dyn_generator(const void *G, const void *Z, const double *coef
 
 hipError_t launch_herm_update(const void *Y, const void *R, void *S, int n, int batch, double c, double s,
                               hipStream_t st) {
-    const size_t total = (size_t)batch * n * n;
-    const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_herm_update, dim3(blocks), dim3(256), 0, st, (const double2 *)Y, (const double2 *)R,
-                       (double2 *)S, n, total, c, s);
+    const int tiles = (n + 31) / 32;
+    hipLaunchKernelGGL(k_herm_update, dim3(tiles, tiles, batch), dim3(256), 0, st, (const double2 *)Y,
+                       (const double2 *)R, (double2 *)S, n, c, s);
     return hipGetLastError();
 }
 
-hipError_t launch_fro2(const void *U, int n, int batch, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_fro2, dim3(batch), dim3(1024), 0, st, (const double2 *)U, n, out);
+hipError_t launch_accumulate_fro(const void *X, const void *Delta, void *W, int n, int batch, int parts, double *part,
+                                 hipStream_t st) {
+    hipLaunchKernelGGL(k_accumulate_fro, dim3(parts, batch), dim3(256), 0, st, (const double2 *)X,
+                       (const double2 *)Delta, (double2 *)W, (size_t)n * n, part);
     return hipGetLastError();
 }
+
 
 hipError_t launch_pair_block_trace(const void *X, int n, int batch, void *out, hipStream_t st) {
     hipLaunchKernelGGL(k_pair_block_trace, dim3(batch, 16), dim3(256), 0, st, (const double2 *)X, n, (double2 *)out);
