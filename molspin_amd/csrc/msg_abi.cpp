@@ -533,7 +533,10 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         }
         if (const char *env = std::getenv("MSG_LANCZOS_G")) {  // launch-shape override (tuning runs)
             const int g = std::atoi(env);
-            if ((g == 1 && ng >= 64 && ng <= 1024) || (g == 2 && ng >= 1024 && ng <= 2048)) { b.G = g; b.T = ng / g; }
+            if ((g == 1 && ng >= 64 && ng <= 1024) || (g == 2 && ng >= 256 && ng <= 2048) || (g == 4 && ng == 1024)) {
+                b.G = g;
+                b.T = ng / g;
+            }
             if (g == 0 && ng > 2048) { b.G = 0; b.T = 1024; }
             if (g == -2 && ng > kTileGroups) { b.G = -2; b.T = 1024; }
         }
@@ -1350,10 +1353,11 @@ msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_param
             const int ng = (int)(r.D / 4);
             std::vector<msg_ctx::Shape> base;
             if (ng <= 512) base.push_back({1, ng, 0});
-            else if (ng == 1024) { base.push_back({2, 512, 0}); base.push_back({1, 1024, 0}); }
-            else if (ng == 2048) base.push_back({2, 1024, 0});
-            else if (ng == 4096) { base.push_back({-1, 1024, 0}); base.push_back({0, 1024, 0}); }
-            else { base.push_back({-2, 1024, 0}); base.push_back({0, 1024, 0}); }
+            if (ng == 256 || ng == 512) base.push_back({2, ng / 2, 0});
+            if (ng == 1024) { base.push_back({2, 512, 0}); base.push_back({1, 1024, 0}); base.push_back({4, 256, 0}); }
+            if (ng == 2048) base.push_back({2, 1024, 0});
+            if (ng == 4096) { base.push_back({-1, 1024, 0}); base.push_back({0, 1024, 0}); }
+            if (ng > 4096) { base.push_back({-2, 1024, 0}); base.push_back({0, 1024, 0}); }
             for (const auto &c : base) {
                 cand.push_back(c);  // every resident workgroup the occupancy allows
                 msg_ctx::Shape h = c;

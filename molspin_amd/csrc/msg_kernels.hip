@@ -456,7 +456,7 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 // DYN: the operator head (electron fields, shift) comes per interval from
 // a.dyn_head (time-dependent Zeeman fields); otherwise from the item's block.
 template <int G, int TMAX, bool DBUF, bool DYN = false>
-__global__ __launch_bounds__(TMAX, 4) void k_lanczos(const StepArgs *__restrict__ pa) {
+__global__ __launch_bounds__(TMAX, G >= 4 ? 2 : 4) void k_lanczos(const StepArgs *__restrict__ pa) {
     const StepArgs &a = *pa;
     extern __shared__ float4 smem[];
     constexpr int T = TMAX;  // launched with exactly TMAX threads
@@ -1837,6 +1837,8 @@ __global__ void k_to_planes(const double2 *__restrict__ U, float *__restrict__ d
 #define MSG_DENSE_VARIANTS(X) X(1) X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32)
 #define MSG_LANCZOS_VARIANTS(X) \
     X(1, 64, true) X(1, 128, true) X(1, 256, true) X(1, 512, true) X(1, 1024, true) X(2, 512, true) X(2, 1024, true)
+// shapes of the Lanczos kernel only (more groups per thread for tuning; no Arnoldi form)
+#define MSG_LANCZOS_EXTRA(X) X(2, 128, true) X(2, 256, true) X(4, 256, true)
 
 hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_thread, int threads, int slots,
                           hipStream_t st) {
@@ -1871,6 +1873,7 @@ hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_t
         return hipGetLastError();                                                                        \
     }
     MSG_LANCZOS_VARIANTS(X)
+    MSG_LANCZOS_EXTRA(X)
 #undef X
     return hipErrorInvalidValue;
 }
@@ -1887,6 +1890,11 @@ hipError_t set_lanczos_lds_limits() {
     (void)hipFuncSetAttribute((const void *)k_lanczos<G_, T_, B_, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
     (void)hipFuncSetAttribute((const void *)k_arnoldi<G_, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     MSG_LANCZOS_VARIANTS(X)
+#undef X
+#define X(G_, T_, B_)                                                                                               \
+    (void)hipFuncSetAttribute((const void *)k_lanczos<G_, T_, B_, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+    (void)hipFuncSetAttribute((const void *)k_lanczos<G_, T_, B_, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    MSG_LANCZOS_EXTRA(X)
 #undef X
 #define X(NT_) (void)hipFuncSetAttribute((const void *)k_dense<NT_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     MSG_DENSE_VARIANTS(X)
@@ -1933,6 +1941,7 @@ int lanczos_occupancy(int group_per_thread, int threads, size_t lds, bool arnold
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos<G_, T_, B_, false>, threads, lds) == hipSuccess \
                    ? nb : 1;
     MSG_LANCZOS_VARIANTS(X)
+    MSG_LANCZOS_EXTRA(X)
 #undef X
     return 1;
 }

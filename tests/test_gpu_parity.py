@@ -402,6 +402,27 @@ def test_autotune_keeps_results(ctx):
         c2.close()
 
 
+@pytest.mark.parametrize("index,shapes", [(4, ["1", "2"]), (3, ["1", "2"]), (1, ["2", "1", "4"])])
+def test_launch_shapes_match_oracle(ctx, index, shapes, monkeypatch):
+    """Every shape of the LDS Lanczos kernel that msg_autotune may pick (G groups
+    per thread, D / 4G threads; the operator forms the nucleus blocks of a
+    thread's shared index bits once for its G groups): D = 1024 and 2048 with
+    G = 1, 2 and D = 4096 with G = 1, 2, 4, on 3 fields of an uneven-batch pair,
+    each against the oracle."""
+    s, cfg = workloads.config4_pair(index)
+    p = s.pack()
+    sysd = ctx.upload(p)
+    assert sysd.dim == 4 << cfg["n"] and sysd.dim in (1024, 2048, 4096)
+    fields = cfg["fields"][::24]
+    prm = abi.run_params(total_time=20.0, time_step=2.0, method=abi.MSG_METHOD_KRYLOV)
+    yo, _ = ref.run_hsgeneral(p, prm, fields=fields, threads=8)
+    for g in shapes:
+        monkeypatch.setenv("MSG_LANCZOS_G", g)
+        y, _ = sysd.propagate(prm, fields=fields)
+        print("D", sysd.dim, "G", g, "max |dPhi|", np.abs(y - yo).max())
+        assert np.abs(y - yo).max() <= YIELD_TOL, g
+
+
 @pytest.mark.parametrize("singlet_only", [False, True])
 def test_rk4_matches_oracle(ctx, singlet_only):
     """propagationmethod=rk4 (HSPropagator.cpp:80-90): config 1 (D = 48) with the
