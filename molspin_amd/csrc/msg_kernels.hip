@@ -73,12 +73,28 @@ __device__ __forceinline__ void apply_e2(const float2 m[4], const C4 &x, C4 &y) 
     y.c[2] = cfma(m[0], x.c[2], cfma(m[1], x.c[3], y.c[2]));
     y.c[3] = cfma(m[2], x.c[2], cfma(m[3], x.c[3], y.c[3]));
 }
-// 2x2 matrix f.S (S = sigma/2) for a real field f.
-__device__ __forceinline__ void field_matrix(float fx, float fy, float fz, float2 m[4]) {
-    m[0] = make_float2(0.5f * fz, 0.f);
-    m[1] = make_float2(0.5f * fx, -0.5f * fy);
-    m[2] = make_float2(0.5f * fx, 0.5f * fy);
-    m[3] = make_float2(-0.5f * fz, 0.f);
+// y += (f.S) x on one electron for a real field f (S = sigma/2), the electron's two
+// states being x.c[i] and x.c[i + d] for i in {i0, i0 + e}:
+// f.S = [[hz, hx - i hy], [hx + i hy, -hz]] with h = f/2 (12 FMAs per pair).
+template <int D_, int E_>
+__device__ __forceinline__ void apply_field(float fx, float fy, float fz, const C4 &x, C4 &y) {
+    const float hx = 0.5f * fx, hy = 0.5f * fy, hz = 0.5f * fz;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int i = k * E_, j = i + D_;
+        const float2 a = x.c[i], b = x.c[j];
+        y.c[i].x = fmaf(hz, a.x, fmaf(hx, b.x, fmaf(hy, b.y, y.c[i].x)));
+        y.c[i].y = fmaf(hz, a.y, fmaf(hx, b.y, fmaf(-hy, b.x, y.c[i].y)));
+        y.c[j].x = fmaf(hx, a.x, fmaf(-hy, a.y, fmaf(-hz, b.x, y.c[j].x)));
+        y.c[j].y = fmaf(hx, a.y, fmaf(hy, a.x, fmaf(-hz, b.y, y.c[j].y)));
+    }
+}
+// electron 1: pairs (0,2), (1,3); electron 2: pairs (0,1), (2,3)
+__device__ __forceinline__ void apply_field_e1(float fx, float fy, float fz, const C4 &x, C4 &y) {
+    apply_field<2, 1>(fx, fy, fz, x, y);
+}
+__device__ __forceinline__ void apply_field_e2(float fx, float fy, float fz, const C4 &x, C4 &y) {
+    apply_field<1, 2>(fx, fy, fz, x, y);
 }
 // Loads a 2x2 complex block, conjugate-transposed if `herm` (lane-selected).
 __device__ __forceinline__ void load_block(const float *p, bool herm, float2 m[4]) {
@@ -150,11 +166,8 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
         y.c[c].x = fmaf(rr, x.c[c].x, y.c[c].x);
         y.c[c].y = fmaf(rr, x.c[c].y, y.c[c].y);
     }
-    float2 m[4];
-    field_matrix(f1x, f1y, f1z, m);
-    apply_e1(m, x, y);
-    field_matrix(f2x, f2y, f2z, m);
-    apply_e2(m, x, y);
+    apply_field_e1(f1x, f1y, f1z, x, y);
+    apply_field_e2(f2x, f2y, f2z, x, y);
     if (a.op_flags & OP_EREST) {
         const float *E = co + 8;
 #pragma unroll
@@ -252,11 +265,8 @@ __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *_
             y[g].c[c].x = fmaf(rg, x[g].c[c].x, y[g].c[c].x);
             y[g].c[c].y = fmaf(rg, x[g].c[c].y, y[g].c[c].y);
         }
-        float2 m[4];
-        field_matrix(f1x + e1[g][0], f1y + e1[g][1], f1z + e1[g][2], m);
-        apply_e1(m, x[g], y[g]);
-        field_matrix(f2x + e2[g][0], f2y + e2[g][1], f2z + e2[g][2], m);
-        apply_e2(m, x[g], y[g]);
+        apply_field_e1(f1x + e1[g][0], f1y + e1[g][1], f1z + e1[g][2], x[g], y[g]);
+        apply_field_e2(f2x + e2[g][0], f2y + e2[g][1], f2z + e2[g][2], x[g], y[g]);
         if (a.op_flags & OP_EREST) {
             const float *E = co + 8;
 #pragma unroll
