@@ -289,6 +289,7 @@ def variant_batch(ctx, args):
                          dynamics=rydmr)
     ncols = sysd.direct_columns()
     state_steps = B * ncols * intervals
+    shape = sysd.autotune(prm, fields=fields, sample_steps=3)
 
     def step(i):
         sysd.propagate(prm, fields=fields)
@@ -314,7 +315,9 @@ def variant_batch(ctx, args):
                 config={"workload": "orientation batch D=4096 with %s: %d orientations x %d columns x %d steps of "
                                     "2 ns, Krylov m=%d" % (what, B, ncols, intervals, m),
                         "orientations_per_step": B, "columns": ncols, "intervals": intervals,
-                        "parallelism": "single GPU"})
+                        "parallelism": "single GPU",
+                        "launch_shape": {"groups_per_thread": shape.groups_per_thread, "threads": shape.threads,
+                                         "slots": shape.slots, "chosen_by": "msg_autotune"}})
 
 
 def timeinf_batch(ctx, args):

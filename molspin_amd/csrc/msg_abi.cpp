@@ -545,6 +545,12 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
             if (ng > 2048) throw ArgError{MSG_UNSUPPORTED, "matrix-free Arnoldi is limited to D <= 8192"};
             b.G = ng <= 1024 ? 1 : 2;
             b.T = ng / b.G;
+            // a tuned, forced or requested shape with 1 or 2 groups per thread: D = 1024,
+            // 2048 and 4096 run either
+            int want = ts && ts->G > 0 ? ts->G : 0;
+            if (const char *env = std::getenv("MSG_LANCZOS_G")) want = std::atoi(env);
+            if (want == 2 && ng >= 256 && ng <= 1024) { b.G = 2; b.T = ng / 2; }
+            if (want == 1 && ng <= 1024) { b.G = 1; b.T = ng; }
             a.arnoldi = 1;
             for (int i = 0; i < 16; ++i) {
                 a.kmat[2 * i] = (float)r.K[i].real();
@@ -1349,7 +1355,12 @@ msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_param
         msg_launch_shape out{};
         out.path = path == Path::Lanczos ? 1 : 2;
         std::vector<msg_ctx::Shape> cand;
-        if (path == Path::Lanczos) {
+        if (path == Path::Lanczos && !sym_for(r, &p)) {
+            // Arnoldi (unequal recombination rates): 1 or 2 groups per thread for D = 1024-4096
+            const int ng = (int)(r.D / 4);
+            cand.push_back({ng <= 1024 ? 1 : 2, ng <= 1024 ? ng : ng / 2, 0});
+            if (ng >= 256 && ng <= 1024) cand.push_back({2, ng / 2, 0});
+        } else if (path == Path::Lanczos) {
             const int ng = (int)(r.D / 4);
             std::vector<msg_ctx::Shape> base;
             if (ng <= 512) base.push_back({1, ng, 0});

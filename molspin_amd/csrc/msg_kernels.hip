@@ -809,33 +809,41 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
                             w[g].c[r] = make_float2(hv.c[r].y - kv.x, -hv.c[r].x - kv.y);
                         }
                     }
-                    // modified Gram-Schmidt: h_ij = <v_i, w>, w -= h_ij v_i
+                    // modified Gram-Schmidt: h_ij = <v_i, w>, w -= h_ij v_i.  v_j comes from
+                    // LDS, earlier basis vectors from the slot's global scratch; each v_i is
+                    // read once into registers, and v_{i+1} is requested before the
+                    // reduction of h_ij so that its latency overlaps the reduction.
+                    auto basis_group = [&](int i, int nu) {
+                        const float4 *gA = reinterpret_cast<const float4 *>(bas + (size_t)i * 2 * D);
+                        return i == j ? load_group(LA, LB, nu) : load_group(gA, gA + D4, nu);
+                    };
+                    C4 vi[G];
+#pragma unroll
+                    for (int g = 0; g < G; ++g) vi[g] = basis_group(0, g * T + t);
                     for (int i = 0; i <= j; ++i) {
-                        // v_j from LDS, earlier basis vectors from the slot's global scratch
-                        const float4 *gA = reinterpret_cast<const float4 *>(bas + (size_t)i * 2 * D), *gB = gA + D4;
                         double h[2] = {0.0, 0.0};
 #pragma unroll
-                        for (int g = 0; g < G; ++g) {
-                            const int nu = g * T + t;
-                            const C4 vi = i == j ? load_group(LA, LB, nu) : load_group(gA, gB, nu);
+                        for (int g = 0; g < G; ++g)
 #pragma unroll
                             for (int c = 0; c < 4; ++c) {
-                                h[0] += (double)vi.c[c].x * w[g].c[c].x + (double)vi.c[c].y * w[g].c[c].y;
-                                h[1] += (double)vi.c[c].x * w[g].c[c].y - (double)vi.c[c].y * w[g].c[c].x;
+                                h[0] += (double)vi[g].c[c].x * w[g].c[c].x + (double)vi[g].c[c].y * w[g].c[c].y;
+                                h[1] += (double)vi[g].c[c].x * w[g].c[c].y - (double)vi[g].c[c].y * w[g].c[c].x;
                             }
-                        }
+                        C4 vn[G];
+                        if (i < j)
+#pragma unroll
+                            for (int g = 0; g < G; ++g) vn[g] = basis_group(i + 1, g * T + t);
                         block_sum<2>(h, SCR());
                         if (t == 0) s_h[i * kMaxKrylov + j] = make_double2(h[0], h[1]);
                         const float hr = (float)h[0], hi = (float)h[1];
 #pragma unroll
                         for (int g = 0; g < G; ++g) {
-                            const int nu = g * T + t;
-                            const C4 vi = i == j ? load_group(LA, LB, nu) : load_group(gA, gB, nu);
 #pragma unroll
                             for (int c = 0; c < 4; ++c) {
-                                w[g].c[c].x -= hr * vi.c[c].x - hi * vi.c[c].y;
-                                w[g].c[c].y -= hr * vi.c[c].y + hi * vi.c[c].x;
+                                w[g].c[c].x -= hr * vi[g].c[c].x - hi * vi[g].c[c].y;
+                                w[g].c[c].y -= hr * vi[g].c[c].y + hi * vi[g].c[c].x;
                             }
+                            if (i < j) vi[g] = vn[g];
                         }
                     }
                     if (j == meff0 - 1) break;
@@ -1849,6 +1857,8 @@ __global__ void k_to_planes(const double2 *__restrict__ U, float *__restrict__ d
     X(1, 64, true) X(1, 128, true) X(1, 256, true) X(1, 512, true) X(1, 1024, true) X(2, 512, true) X(2, 1024, true)
 // shapes of the Lanczos kernel only (more groups per thread for tuning; no Arnoldi form)
 #define MSG_LANCZOS_EXTRA(X) X(2, 128, true) X(2, 256, true) X(4, 256, true)
+// and of the Arnoldi kernel (two groups per thread at D = 1024 and 2048)
+#define MSG_ARNOLDI_EXTRA(X) X(2, 128, true) X(2, 256, true)
 
 hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_thread, int threads, int slots,
                           hipStream_t st) {
@@ -1870,6 +1880,7 @@ hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_t
             return hipGetLastError();                                                                   \
         }
         MSG_LANCZOS_VARIANTS(X)
+        MSG_ARNOLDI_EXTRA(X)
 #undef X
         return hipErrorInvalidValue;
     }
@@ -1906,6 +1917,10 @@ hipError_t set_lanczos_lds_limits() {
     (void)hipFuncSetAttribute((const void *)k_lanczos<G_, T_, B_, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     MSG_LANCZOS_EXTRA(X)
 #undef X
+#define X(G_, T_, B_) \
+    (void)hipFuncSetAttribute((const void *)k_arnoldi<G_, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    MSG_ARNOLDI_EXTRA(X)
+#undef X
 #define X(NT_) (void)hipFuncSetAttribute((const void *)k_dense<NT_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     MSG_DENSE_VARIANTS(X)
 #undef X
@@ -1936,6 +1951,7 @@ int lanczos_occupancy(int group_per_thread, int threads, size_t lds, bool arnold
         if (group_per_thread == G_ && threads == T_)                                                          \
             return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_arnoldi<G_, T_>, threads, lds) == hipSuccess ? nb : 1;
         MSG_LANCZOS_VARIANTS(X)
+        MSG_ARNOLDI_EXTRA(X)
 #undef X
         return 1;
     }

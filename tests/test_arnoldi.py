@@ -53,12 +53,14 @@ def trace_err(dev, ora):
     return np.abs(dev - ora).max() / np.abs(ora).max()
 
 
-def test_arnoldi_d1024_singlet_only_sink(ctx):
+@pytest.mark.parametrize("groups", ["1", "2"])
+def test_arnoldi_d1024_singlet_only_sink(ctx, groups, monkeypatch):
     """D = 1024: singlet-only recombination (the Haberkorn singlet sink), three fields,
-    yields and traces against the oracle."""
+    yields and traces against the oracle, with one and two groups per thread."""
     s = pair(4, 4, (3e-3, 0.0, 0.0, 0.0))
     p = s.pack()
     sysd = ctx.upload(p)
+    monkeypatch.setenv("MSG_LANCZOS_G", groups)
     prm = abi.run_params(calculation=abi.MSG_CALC_TIMEEVOLUTION, method=abi.MSG_METHOD_KRYLOV,
                          total_time=200.0, time_step=2.0)
     _, tr = sysd.propagate(prm, fields=FIELDS, trace=True)
@@ -72,15 +74,34 @@ def test_arnoldi_d1024_singlet_only_sink(ctx):
     assert np.abs(y - yo).max() <= YIELD_TOL
 
 
-def test_arnoldi_d4096_unequal_rates(ctx):
-    """D = 4096 (6 + 4 nuclei), k_S != k_T, one orientation, 20 intervals."""
+@pytest.mark.parametrize("groups", ["1", "2"])
+def test_arnoldi_d2048_singlet_only_sink(ctx, groups, monkeypatch):
+    """D = 2048 (5 + 4 nuclei), singlet-only sink, two fields, 10 intervals, with one
+    and two groups per thread."""
+    s = pair(5, 4, (2e-3, 0.0, 0.0, 0.0), seed=5)
+    p = s.pack()
+    sysd = ctx.upload(p)
+    assert sysd.dim == 2048
+    prm = abi.run_params(method=abi.MSG_METHOD_KRYLOV, total_time=20.0, time_step=2.0)
+    monkeypatch.setenv("MSG_LANCZOS_G", groups)
+    y, _ = sysd.propagate(prm, fields=FIELDS[1:])
+    yo, _ = ref.run_hsgeneral(p, prm, fields=FIELDS[1:], threads=16)
+    print("D=2048 arnoldi G", groups, y[:, 0], yo[:, 0])
+    assert np.abs(y - yo).max() <= YIELD_TOL
+
+
+@pytest.mark.parametrize("groups", ["1", "2"])
+def test_arnoldi_d4096_unequal_rates(ctx, groups, monkeypatch):
+    """D = 4096 (6 + 4 nuclei), k_S != k_T, one orientation, 20 intervals, in both
+    launch shapes the tuner may pick (1024 threads with one group each, 512 with two)."""
     s = pair(6, 4, (2e-3, 5e-4, 5e-4, 5e-4))
     p = s.pack()
     sysd = ctx.upload(p)
     prm = abi.run_params(method=abi.MSG_METHOD_KRYLOV, total_time=40.0, time_step=2.0)
+    monkeypatch.setenv("MSG_LANCZOS_G", groups)
     y, _ = sysd.propagate(prm, fields=FIELDS[2:])
     yo, _ = ref.run_hsgeneral(p, prm, fields=FIELDS[2:], threads=16)
-    print("D=4096 arnoldi", y[:, 0], yo[:, 0])
+    print("D=4096 arnoldi G", groups, y[:, 0], yo[:, 0])
     assert np.abs(y - yo).max() <= YIELD_TOL
 
 
