@@ -297,9 +297,10 @@ def variant_batch(ctx, args):
     def roof(kernel_s):
         if rydmr:
             return lanczos_roofline(D, m, state_steps, kernel_s, None)
-        # Arnoldi: per state-step m basis stores, the Gram-Schmidt reads of the earlier
-        # vectors (twice each), and m reads in the final combination: 8 D m (m + 1) bytes
-        bss = 8.0 * D * m * (m + 1)
+        # Arnoldi: per state-step m basis stores, one read of each earlier vector per
+        # Gram-Schmidt sweep (v_j itself is in LDS; m (m - 1) / 2 reads) and m reads in
+        # the final combination: 8 D (2 m + m (m - 1) / 2) bytes
+        bss = 8.0 * D * (2 * m + m * (m - 1) / 2)
         tr = MEASURED_TRAFFIC_PER_STATE_STEP["arnoldi"] * state_steps
         return {"bound": "hbm", "achieved": bss * state_steps / kernel_s / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": bss * state_steps / kernel_s / HBM_PEAK, "traffic": tr,

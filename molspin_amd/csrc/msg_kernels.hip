@@ -794,19 +794,19 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
                 int meff = meff0;
                 for (int j = 0; j < meff0; ++j) {
                     __syncthreads();  // v_j complete in LDS
-                    C4 w[G];
+                    C4 w[G], vjs[G], hv[G];
+#pragma unroll
+                    for (int g = 0; g < G; ++g) vjs[g] = load_group(LA, LB, g * T + t);
+                    apply_h_groups<Log2<ng>::value, G, Log2<T>::value>(a, co, hd, t, vjs, LA, LB, hv);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
-                        const int nu = g * T + t;
-                        const C4 vj = load_group(LA, LB, nu);
-                        const C4 hv = apply_h<Log2<ng>::value>(a, co, hd, nu, vj, LA, LB);
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {  // w = -i H v - K v
                             float2 kv = make_float2(0.f, 0.f);
 #pragma unroll
                             for (int c = 0; c < 4; ++c)
-                                kv = cfma(make_float2(a.kmat[8 * r + 2 * c], a.kmat[8 * r + 2 * c + 1]), vj.c[c], kv);
-                            w[g].c[r] = make_float2(hv.c[r].y - kv.x, -hv.c[r].x - kv.y);
+                                kv = cfma(make_float2(a.kmat[8 * r + 2 * c], a.kmat[8 * r + 2 * c + 1]), vjs[g].c[c], kv);
+                            w[g].c[r] = make_float2(hv[g].c[r].y - kv.x, -hv[g].c[r].x - kv.y);
                         }
                     }
                     // modified Gram-Schmidt: h_ij = <v_i, w>, w -= h_ij v_i.  v_j comes from
