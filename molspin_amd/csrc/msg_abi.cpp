@@ -537,7 +537,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                 b.G = g;
                 b.T = ng / g;
             }
-            if (g == 0 && ng > 2048) { b.G = 0; b.T = 1024; }
+            if ((g == 0 && ng > 2048) || (g == -1 && ng == 4096)) { b.G = g; b.T = 1024; }
             if (g == -2 && ng > kTileGroups) { b.G = -2; b.T = 1024; }
         }
         b.arnoldi = !sym_for(r, p);

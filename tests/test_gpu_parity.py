@@ -175,9 +175,13 @@ def test_step_entry_matches_oracle(ctx):
     assert np.abs(np.array(pops) - tro[0, 1]).max() < 1e-6
 
 
-def test_large_dimension_streaming_path(ctx):
+@pytest.mark.parametrize("shape", [None, "-1", "0"])
+def test_large_dimension_streaming_path(ctx, shape, monkeypatch):
     """D = 16384 (7 + 5 anisotropic spin-1/2 nuclei): the streaming matrix-free
-    variant on 6 direct columns and 2 orientations over 0-60 ns."""
+    variants on 6 direct columns and 2 orientations over 0-60 ns: the default shape,
+    the LDS-mirrored form and the plain streaming form (both tuning candidates)."""
+    if shape is not None:
+        monkeypatch.setenv("MSG_LANCZOS_G", shape)
     rng = workloads.SplitMix64(41, 3)
     hf = [workloads._aniso_hf(rng) for _ in range(12)]
     s = workloads._radical_pair("big", ["1/2"] * 7, ["1/2"] * 5, hf, 1e-3, swept=True)
