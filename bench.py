@@ -157,7 +157,7 @@ def lanczos_roofline(D, m, state_steps, kernel_s, traffic):
 # profiles/k_lanczos_big_trajectories_D65536.md, profiles/k_lanczos_big_pair_D16384.md,
 # profiles/k_dense_field_sweep_D512.md and profiles/k_arnoldi_singlet_sink_D4096.md.
 MEASURED_TRAFFIC_PER_STATE_STEP = {"orientations": 1.066e+06, "trajectories": 7.523e+07, "dense": 2.51e+04,
-                                   "pair": 6.60e+06, "arnoldi": 4.825e+06}
+                                   "pair": 6.60e+06, "arnoldi": 5.50e+06}
 
 
 def orientation_batch(ctx, args):
