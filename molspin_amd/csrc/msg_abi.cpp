@@ -150,6 +150,8 @@ struct Batch {
     int n_items = 0, ncols = 0, n_units = 0, nobs = 0, nsamples = 0;
     int G = 1, T = 64, slots = 1;
     int share = 1;  // the launch keeps 1/share of the resident workgroups it could hold (server space sharing)
+    bool no_split = false;  // no split-column shape (G = -3): its cooperative launch needs the whole device
+    DBuf xred, bar;         // split columns: partial sums and barrier counters per column slot
     std::vector<double> times, scale;
     StepArgs a{};
     DBuf coef, U, dts, wy, wt, thr, yacc, trace, state, basis, queue, status, given, norm0, yout, tout;
@@ -537,8 +539,12 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                 b.G = g;
                 b.T = ng / g;
             }
-            if ((g == 0 && ng > 2048) || (g == -1 && ng == 4096)) { b.G = g; b.T = 1024; }
+            if ((g == 0 && ng > 2048) || (g == -1 && ng == 4096) || (g == -3 && ng >= 4096)) { b.G = g; b.T = 1024; }
             if (g == -2 && ng > kTileGroups) { b.G = -2; b.T = 1024; }
+        }
+        if (b.G == -3 && (b.no_split || ng < 4096)) {  // back to the default streaming shape
+            b.G = ng <= 4096 ? -1 : -2;
+            b.T = 1024;
         }
         b.arnoldi = !sym_for(r, p);
         if (b.arnoldi) {  // G = -iH - K: Arnoldi with the Krylov vector in LDS, one group per thread up to D = 4096
@@ -773,17 +779,30 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     // launch geometry
     if (b.path == Path::Lanczos) {
         const size_t lds = b.G == 0 ? 0 : b.G == -1 ? (size_t)8 * D : b.G == -2 ? (size_t)32 * kTileGroups
+                         : b.G == -3 ? (size_t)64 * kSplitGroups
                                      : b.arnoldi ? (size_t)8 * D + 4096 : (size_t)16 * D + 4096;
         const int occ = std::max(1, lanczos_occupancy(b.G, b.T, lds, b.arnoldi));
-        b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ));
+        // split columns: a slot is S workgroups, all resident at once
+        const int S = b.G == -3 ? (int)(D / 4 / kSplitGroups) : 1;
+        if (ctx->cus * occ < S) throw ArgError{MSG_UNSUPPORTED, "split column wider than the device"};
+        b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ / S));
         if (ts && ts->slots > 0) b.slots = std::max(1, std::min(b.slots, ts->slots));
         if (b.share > 1) b.slots = std::max(1, std::min(b.slots, ctx->cus * occ / b.share));
         if (const char *env = std::getenv("MSG_LANCZOS_SLOTS")) {  // resident-workgroup override (tuning runs)
             const int sl = std::atoi(env);
             if (sl >= 1) b.slots = std::max(1, std::min(b.slots, sl));
         }
-        const int vecs = b.G <= 0 ? a.m + 2 : a.m;  // streaming variants: + residual and state
+        if (b.G == -3) {  // whole groups of 8 slots (one per XCD); slots past the units stay idle
+            const int most = ctx->cus * occ / S / 8 * 8;
+            if (most < 8) throw ArgError{MSG_UNSUPPORTED, "split column wider than the device"};
+            b.slots = std::min(most, (b.slots + 7) / 8 * 8);
+        }
+        const int vecs = b.G <= 0 && b.G != -3 ? a.m + 2 : a.m;  // streaming variants: + residual and state
         a.basis = b.basis.alloc<float>((size_t)b.slots * vecs * 2 * D, stream);
+        if (b.G == -3) {
+            a.xred = b.xred.alloc<double>((size_t)b.slots * 2 * S * 17, stream);
+            a.bar = b.bar.alloc<int>(b.slots, stream);
+        }
         a.norm0 = b.norm0.alloc<double>(b.n_units, stream);
         if (a.renorm) HIPCHECK(launch_init_norms(a, push_args(b, a), stream));
     } else {
@@ -817,6 +836,7 @@ void run_chunk(Batch &b, int steps) {
         a.init_mode = 2;
     }
     HIPCHECK(launch_zero_int(a.queue, 1, b.stream));
+    if (a.bar) HIPCHECK(launch_zero_int(a.bar, b.slots, b.stream));
     const StepArgs *da = push_args(b, a);
     if (b.timed) HIPCHECK(hipEventRecord(b.ctx->ev0, b.stream));
     if (b.path == Path::Lanczos) HIPCHECK(launch_lanczos(a, da, b.G, b.T, b.slots, b.stream));
@@ -950,6 +970,7 @@ void collect(Batch &b, double *yields, double *trace) {
     HIPCHECK(hipStreamSynchronize(b.stream));
     if (st & 1) throw ArgError{MSG_NUMERIC_BREAKDOWN, "non-finite state during propagation"};
     if (st & 2) throw ArgError{MSG_NUMERIC_BREAKDOWN, "time step too large for the Krylov exponential"};
+    if (st & 4) throw ArgError{MSG_DEVICE_ERROR, "the workgroups of a split column lost synchronisation"};
     if (yields && b.prm.calculation == MSG_CALC_YIELDS && b.prm.yield_corrections) {
         const double den = 1.0 - std::exp(-b.prm.total_time * b.sys->r.kmax);
         if (!(den > 0.0) || !std::isfinite(den))
@@ -1369,6 +1390,7 @@ msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_param
             if (ng == 2048) base.push_back({2, 1024, 0});
             if (ng == 4096) { base.push_back({-1, 1024, 0}); base.push_back({0, 1024, 0}); }
             if (ng > 4096) { base.push_back({-2, 1024, 0}); base.push_back({0, 1024, 0}); }
+            if (ng >= 4096) base.push_back({-3, 1024, 0});  // split columns (cooperative launch)
             for (const auto &c : base) {
                 cand.push_back(c);  // every resident workgroup the occupancy allows
                 msg_ctx::Shape h = c;
@@ -1917,6 +1939,7 @@ msg_status msg_server_pump(msg_server *srv, int32_t max_steps, int32_t *n_active
             }
             q->stream = slot;
             q->batch.reset(new Batch());
+            q->batch->no_split = true;  // requests share the device
             q->whole = max_steps <= 0;
             try {
                 prepare(*q->batch, ctx, q->sys, &q->prm, q->n_fields, q->fields.data(), nullptr, 0, false, !q->whole,

@@ -179,14 +179,20 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
     return y;
 }
 
-// H x for the G groups nu_g = g 2^LOGT + t of one thread (t < 2^LOGT, NN nuclei,
-// NN >= LOGT).  The index bits below LOGT are the thread's and shared by its G groups,
-// so those nuclei's blocks, field sums and shift are formed once per thread; the bits
-// from LOGT up are g's and known at compile time, so their blocks need no lane select.
+// H x for the G groups nu_g = base + g 2^LOGT + t of one thread (t < 2^LOGT, NN
+// nuclei).  The index bits below LOGT are the thread's and shared by its G groups, so
+// those nuclei's blocks, field sums and shift are formed once per thread; the bits
+// from LOGT up are g's and base's, the same for the whole workgroup, so their blocks
+// need no lane select.  The workgroup's slab of the vector (SB = LOGT + log2 G index
+// bits, local index g 2^LOGT + t) is in LDS (LA, LB); nuclei on bits from SB up
+// (a column split over several workgroups, base = slab 2^SB) read their neighbours
+// from the whole vector in global memory (GA, GB).
 template <int NN, int G, int LOGT>
 __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *__restrict__ co,
                                                const float *__restrict__ hd, int t, const C4 (&x)[G],
-                                               const float4 *LA, const float4 *LB, C4 (&y)[G]) {
+                                               const float4 *LA, const float4 *LB, C4 (&y)[G], int base = 0,
+                                               const float4 *GA = nullptr, const float4 *GB = nullptr) {
+    constexpr int SB = LOGT + Log2<G>::value;
     float f1x = hd[0], f1y = hd[1], f1z = hd[2];
     float f2x = hd[3], f2y = hd[4], f2z = hd[5];
     float rr = hd[6];
@@ -233,9 +239,11 @@ __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *_
         } else {
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                const bool down = ((g << LOGT) >> n) & 1;
+                const int hi = base + (g << LOGT);  // the group's index bits from LOGT up
+                const bool down = (hi >> n) & 1;
                 const float s = down ? -1.f : 1.f;
-                const C4 nb = load_group(LA, LB, ((g << LOGT) ^ (1 << n)) + t);
+                const C4 nb = n < SB ? load_group(LA, LB, ((g << LOGT) ^ (1 << n)) + t)
+                                     : load_group(GA, GB, (hi ^ (1 << n)) + t);
                 if (fl & NUC_E1) {
                     e1[g][0] = fmaf(s, cn[0], e1[g][0]); e1[g][1] = fmaf(s, cn[1], e1[g][1]);
                     e1[g][2] = fmaf(s, cn[2], e1[g][2]);
@@ -1266,6 +1274,301 @@ __global__ void k_reduce_columns(const double *in, double *out, int n_items, int
 }
 
 // ---------------------------------------------------------------------------
+// Matrix-free Lanczos with each column split over S = 2^(NN - kSplitBits)
+// workgroups (D = 4 2^NN >= 16384).  A workgroup of 1024 threads owns a slab of
+// 2048 groups (index bits 0-10), two per thread, and keeps the slab of v_j and
+// v_{j-1} in LDS (128 KiB); the residual stays in registers.  The nuclei on bits 11
+// and up couple slabs: their neighbours come from v_j in the slot's global basis,
+// which every workgroup of the slot writes before the slot's barrier.  Inner
+// products are reduced over the workgroup, then over the slot's workgroups in slab
+// order (slot_sum), so every workgroup of a slot sees the same alpha, beta and
+// observables and follows the same control flow.  Per state-step HBM carries the m
+// basis stores, the partner slabs of each v_j (L2 / MALL) and the m reads of the
+// final combination.  All workgroups of a launch are resident at once (cooperative
+// launch of a multiple of 8 slots); a barrier wait that does not end sets status
+// bit 2 and ends the launch.
+// ---------------------------------------------------------------------------
+constexpr int kSplitBits = 11;  // groups per slab: 2^11
+static_assert((1 << kSplitBits) == kSplitGroups, "slab size shared with the host");
+
+// Barrier over the S workgroups of a slot: counter `bar` counts arrivals over the
+// launch, `nbar` the barriers this workgroup has passed.  Every thread calls it;
+// false when the wait gave up (status bit 2).  The workgroups of a slot run on one
+// XCD (blockIdx.x mod 8 selects the XCD), whose L2 they share: every wave waits for
+// its own stores to reach L2 (L1 writes through), and after the arrival count is
+// complete one thread invalidates the CU's L1 for the whole workgroup.
+__device__ __forceinline__ bool slot_barrier(int *bar, int S, int &nbar, int *status, int *s_flag) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's global stores are in L2
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int target = S * (nbar + 1);
+        int ok = 1;
+        for (int spins = 0; __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++spins) {
+            if (spins > (1 << 26)) {  // about a second: a workgroup of the slot is gone
+                ok = 0;
+                atomicOr(status, 4);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        asm volatile("buffer_inv sc0" ::: "memory");  // drop this CU's stale L1 lines
+        *s_flag = ok;
+    }
+    ++nbar;
+    __syncthreads();
+    return *s_flag != 0;
+}
+
+// Sums K workgroup totals (every thread holds them) over the slot's S workgroups in
+// slab order; xr holds the slot's partials, 2 x S x 17 doubles used alternately (a
+// workgroup can only overwrite a half two barriers later, when every workgroup of
+// the slot has read it).  Every thread receives the sums.
+template <int K>
+__device__ __forceinline__ bool slot_sum(double (&v)[K], double *xr, int *bar, int S, int slab, int &nbar,
+                                         int *status, int *s_flag, double *s_tot) {
+    double *buf = xr + (size_t)(nbar & 1) * S * 17;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < K; ++k) buf[slab * 17 + k] = v[k];
+    if (!slot_barrier(bar, S, nbar, status, s_flag)) return false;
+    if (threadIdx.x < K) {
+        double s = 0.0;
+        for (int q = 0; q < S; ++q) s += buf[q * 17 + threadIdx.x];
+        s_tot[threadIdx.x] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = s_tot[k];
+    return true;
+}
+
+template <int NN>
+__global__ __launch_bounds__(1024, 4) void k_lanczos_split(const StepArgs *__restrict__ pa) {
+    const StepArgs &a = *pa;
+    constexpr int T = 1024, G = 2, SB = kSplitBits, SG = 1 << SB, S = 1 << (NN - SB);
+    static_assert(G * T == SG, "two groups per thread fill a slab");
+    extern __shared__ float4 smem[];  // slab of v_j and v_{j-1}: planes A and B of each buffer
+    auto LA = [&](int i) -> float4 * { return smem + (i ? 2 * SG : 0); };
+    auto LB = [&](int i) -> float4 * { return smem + SG + (i ? 2 * SG : 0); };
+    __shared__ double scr[2 * 17 * 16];  // block_sum of up to 17 values, double-buffered
+    __shared__ double s_tot[17];
+    __shared__ double ys[kMaxObs], yc[kMaxObs];
+    __shared__ float2 s_coef[kMaxKrylov];
+    __shared__ int s_flag;
+    const int t = threadIdx.x, lane = t & 63;
+    // block b runs on XCD b mod 8; the S workgroups of a slot share one XCD (and L2)
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    const int slab = local % S, slot = (local / S) * 8 + xcd, nslots = gridDim.x / S;
+    const int base = slab << SB;  // first group of the slab
+    const int D = a.dim, D4 = D >> 2;
+    float *bas = a.basis + (size_t)slot * a.m * 2 * D;
+    double *xr = a.xred + (size_t)slot * 2 * S * 17;
+    int *bar = a.bar + slot;
+    const bool lead = slab == 0 && t == 0;  // records samples and yields
+    const int meff0 = a.m < D ? a.m : D;
+    int nbar = 0, par = 0;
+    auto SCR = [&]() { par ^= 1; return scr + par * 17 * 16; };
+    auto norm2 = [&](const C4 (&x)[G]) {
+        double nn = 0.0;
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) nn += (double)x[g].c[c].x * x[g].c[c].x + (double)x[g].c[c].y * x[g].c[c].y;
+        return nn;
+    };
+
+    for (int u = slot; u < a.n_units; u += nslots) {
+        const int item = u / a.ncols, col = u - item * a.ncols;
+        const float *co = a.coef + (size_t)item * a.coef_stride;
+        C4 psi[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int nu = base + g * T + t;
+            if (a.init_mode == 0) {
+                const int which = col / a.zdim, ncol = col - which * a.zdim;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    psi[g].c[c] = (nu == ncol) ? make_float2(a.evec[which * 8 + 2 * c], a.evec[which * 8 + 2 * c + 1])
+                                               : make_float2(0.f, 0.f);
+            } else {
+                const float *src = a.init_mode == 1 ? a.given + (size_t)col * 2 * D : a.state + (size_t)u * 2 * D;
+                const float4 *sA = reinterpret_cast<const float4 *>(src);
+                psi[g] = load_group(sA, sA + D4, nu);
+            }
+        }
+        if (lead)
+            for (int c = 0; c < a.nobs; ++c) {
+                ys[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2] : 0.0;
+                yc[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2 + 1] : 0.0;
+            }
+        const double n0 = a.norm0 ? a.norm0[u] : 0.0;
+
+        for (int s = a.s_begin;; ++s) {
+            const bool end = (s == a.s_end);
+            // observables at sample s and |psi|^2, summed over the slot
+            double R[17];
+            {
+                float Rf[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) Rf[k] = 0.f;
+#pragma unroll
+                for (int g = 0; g < G; ++g) accum_rho(psi[g], Rf);
+#pragma unroll
+                for (int k = 0; k < 16; ++k) R[k] = Rf[k];
+                R[16] = norm2(psi);
+                block_sum<17>(R, SCR());
+                if (!slot_sum<17>(R, xr, bar, S, slab, nbar, a.status, &s_flag, s_tot)) return;
+                if (lead && (!end || s == a.nsamples - 1)) record_sample(a, u, s, R, ys, yc);
+            }
+            if (end) break;
+            const double dt = a.dts[s];
+            const float *hd = a.dyn_head ? a.dyn_head + ((size_t)item * (a.nsamples - 1) + s) * 8 : co;
+            double nn = R[16];
+            // relaxation half-step before (HSPropagator.cpp:137); the same decision in
+            // every workgroup of the slot
+            if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s]) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) st_flip(psi[g]);
+                double q[1] = {norm2(psi)};
+                block_sum<1>(q, SCR());
+                if (!slot_sum<1>(q, xr, bar, S, slab, nbar, a.status, &s_flag, s_tot)) return;
+                nn = q[0];
+            }
+            const double beta0 = sqrt(nn);
+            if (beta0 > 0.0) {
+                {
+                    const float inv = (float)(1.0 / beta0);
+                    float4 *bA = reinterpret_cast<float4 *>(bas);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        C4 v0;
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) v0.c[c] = make_float2(psi[g].c[c].x * inv, psi[g].c[c].y * inv);
+                        store_group(LA(0), LB(0), g * T + t, v0);
+                        store_group(bA, bA + D4, base + g * T + t, v0);
+                    }
+                }
+                if (!slot_barrier(bar, S, nbar, a.status, &s_flag)) return;  // v_0 complete in the slot
+                double my_al = 0.0, my_be = 0.0;  // lane j keeps alpha_j and beta_j
+                double bprev = 0.0;
+                int meff = meff0;
+                int cur = 0;
+                for (int j = 0; j < meff0; ++j) {
+                    const float4 *cA = LA(cur), *cB = LB(cur);
+                    float4 *pA = LA(cur ^ 1), *pB = LB(cur ^ 1);
+                    const float4 *gA = reinterpret_cast<const float4 *>(bas + (size_t)j * 2 * D);
+                    const float fb = (float)bprev;
+                    // w = H v_j - beta_{j-1} v_{j-1}; one reduction gives alpha and |w|^2
+                    C4 w[G], vjs[G];
+                    double red[2] = {0.0, 0.0};
+#pragma unroll
+                    for (int g = 0; g < G; ++g) vjs[g] = load_group(cA, cB, g * T + t);
+                    apply_h_groups<NN, G, Log2<T>::value>(a, co, hd, t, vjs, cA, cB, w, base, gA, gA + D4);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        if (j > 0) {
+                            const C4 vp = load_group(pA, pB, g * T + t);
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                w[g].c[c].x = fmaf(-fb, vp.c[c].x, w[g].c[c].x);
+                                w[g].c[c].y = fmaf(-fb, vp.c[c].y, w[g].c[c].y);
+                            }
+                        }
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            red[0] += (double)vjs[g].c[c].x * w[g].c[c].x + (double)vjs[g].c[c].y * w[g].c[c].y;
+                            red[1] += (double)w[g].c[c].x * w[g].c[c].x + (double)w[g].c[c].y * w[g].c[c].y;
+                        }
+                    }
+                    block_sum<2>(red, SCR());
+                    if (!slot_sum<2>(red, xr, bar, S, slab, nbar, a.status, &s_flag, s_tot)) return;
+                    const double alpha = red[0];
+                    if (lane == j) my_al = alpha;
+                    const float fa = (float)alpha;
+#pragma unroll
+                    for (int g = 0; g < G; ++g)
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            w[g].c[c].x = fmaf(-fa, vjs[g].c[c].x, w[g].c[c].x);
+                            w[g].c[c].y = fmaf(-fa, vjs[g].c[c].y, w[g].c[c].y);
+                        }
+                    if (j == meff0 - 1) break;
+                    // |w - alpha v|^2 = |w|^2 - alpha^2 for orthonormal v; recomputed when it cancels
+                    double beta2 = red[1] - alpha * alpha;
+                    if (!(beta2 > 0.02 * red[1])) {
+                        double bb[1] = {norm2(w)};
+                        block_sum<1>(bb, SCR());
+                        if (!slot_sum<1>(bb, xr, bar, S, slab, nbar, a.status, &s_flag, s_tot)) return;
+                        beta2 = bb[0];
+                    }
+                    const double beta = sqrt(fmax(beta2, 0.0));
+                    if (!(beta > 1e-30)) {  // invariant Krylov subspace (happy breakdown)
+                        meff = j + 1;
+                        break;
+                    }
+                    if (lane == j) my_be = beta;
+                    const float ib = (float)(1.0 / beta);
+                    float4 *bA = reinterpret_cast<float4 *>(bas + (size_t)(j + 1) * 2 * D);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) w[g].c[c] = make_float2(w[g].c[c].x * ib, w[g].c[c].y * ib);
+                        store_group(pA, pB, g * T + t, w[g]);  // v_{j+1} replaces v_{j-1}
+                        store_group(bA, bA + D4, base + g * T + t, w[g]);
+                    }
+                    bprev = beta;
+                    cur ^= 1;
+                    if (!slot_barrier(bar, S, nbar, a.status, &s_flag)) return;  // v_{j+1} complete in the slot
+                }
+                // wavefront 0 evaluates the small exponential (the same in every
+                // workgroup of the slot); psi = beta V c over the own slab
+                if (t < 64) {
+                    double cre, cim;
+                    cheb_expm_lanes(meff, dt, my_al, my_be, cre, cim, a.status);
+                    const double sc = a.renorm ? n0 : beta0;
+                    if (t < meff) s_coef[t] = make_float2((float)(sc * cre), (float)(sc * cim));
+                }
+                __syncthreads();
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) psi[g].c[c] = make_float2(0.f, 0.f);
+                for (int j = 0; j < meff; ++j) {
+                    const float2 cj = s_coef[j];
+                    const float4 *bA = reinterpret_cast<const float4 *>(bas + (size_t)j * 2 * D);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const C4 vj = load_group(bA, bA + D4, base + g * T + t);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) psi[g].c[c] = cfma(cj, vj.c[c], psi[g].c[c]);
+                    }
+                }
+            }
+            // relaxation half-step after (HSPropagator.cpp:139)
+            if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
+#pragma unroll
+                for (int g = 0; g < G; ++g) st_flip(psi[g]);
+        }
+        if (lead)
+            for (int c = 0; c < a.nobs; ++c) {
+                a.yacc[((size_t)u * a.nobs + c) * 2] = ys[c];
+                a.yacc[((size_t)u * a.nobs + c) * 2 + 1] = yc[c];
+            }
+        if (a.state) {
+            float4 *dA = reinterpret_cast<float4 *>(a.state + (size_t)u * 2 * D);
+#pragma unroll
+            for (int g = 0; g < G; ++g) store_group(dA, dA + D4, base + g * T + t, psi[g]);
+        }
+        bool bad = false;
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bad |= !isfinite(psi[g].c[c].x) || !isfinite(psi[g].c[c].y);
+        if (bad) atomicOr(a.status, 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Dense path: B <- U B on the FP32 matrix cores.
 // ---------------------------------------------------------------------------
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -1860,8 +2163,25 @@ __global__ void k_to_planes(const double2 *__restrict__ U, float *__restrict__ d
 // and of the Arnoldi kernel (two groups per thread at D = 1024 and 2048)
 #define MSG_ARNOLDI_EXTRA(X) X(2, 128, true) X(2, 256, true)
 
+// Split-column kernels for D = 4 2^NN, NN = 12..16
+#define MSG_SPLIT_VARIANTS(X) X(12) X(13) X(14) X(15) X(16)
+
 hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_thread, int threads, int slots,
                           hipStream_t st) {
+    if (group_per_thread == -3) {  // split columns: `slots` column slots of S workgroups each
+        if (threads != 1024) return hipErrorInvalidValue;
+        const int nn = 31 - __builtin_clz((unsigned)(a.dim >> 2));  // D / 4 = 2^nn
+        const int S = 1 << (nn - kSplitBits);
+        const size_t lds = (size_t)4 * (1 << kSplitBits) * sizeof(float4);
+        void *args[] = {(void *)&da};
+#define X(NN_)                                                                                                  \
+        if (nn == NN_)                                                                                          \
+            return hipLaunchCooperativeKernel((const void *)k_lanczos_split<NN_>, dim3(slots * S), dim3(1024), args, \
+                                              (unsigned)lds, st);
+        MSG_SPLIT_VARIANTS(X)
+#undef X
+        return hipErrorInvalidValue;
+    }
     if (group_per_thread <= 0) {  // streaming variants for large D (v_j mirrored in LDS when G = -1)
         if (threads != 1024) return hipErrorInvalidValue;
         if (group_per_thread == -1)
@@ -1924,6 +2244,10 @@ hipError_t set_lanczos_lds_limits() {
 #define X(NT_) (void)hipFuncSetAttribute((const void *)k_dense<NT_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     MSG_DENSE_VARIANTS(X)
 #undef X
+#define X(NN_) \
+    (void)hipFuncSetAttribute((const void *)k_lanczos_split<NN_>, hipFuncAttributeMaxDynamicSharedMemorySize, maxb - 8192);
+    MSG_SPLIT_VARIANTS(X)
+#undef X
     (void)hipFuncSetAttribute((const void *)k_lanczos_big<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               maxb - 8192);
     (void)hipFuncSetAttribute((const void *)k_lanczos_big<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1955,6 +2279,8 @@ int lanczos_occupancy(int group_per_thread, int threads, size_t lds, bool arnold
 #undef X
         return 1;
     }
+    if (group_per_thread == -3)  // the split kernels share one resource footprint
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_split<14>, 1024, lds) == hipSuccess ? nb : 1;
     if (group_per_thread == 0)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<false>, 1024, 0) == hipSuccess ? nb : 1;
     if (group_per_thread == -1)

@@ -175,11 +175,12 @@ def test_step_entry_matches_oracle(ctx):
     assert np.abs(np.array(pops) - tro[0, 1]).max() < 1e-6
 
 
-@pytest.mark.parametrize("shape", [None, "-1", "0"])
+@pytest.mark.parametrize("shape", [None, "-1", "0", "-3"])
 def test_large_dimension_streaming_path(ctx, shape, monkeypatch):
-    """D = 16384 (7 + 5 anisotropic spin-1/2 nuclei): the streaming matrix-free
+    """D = 16384 (7 + 5 anisotropic spin-1/2 nuclei): the large-D matrix-free
     variants on 6 direct columns and 2 orientations over 0-60 ns: the default shape,
-    the LDS-mirrored form and the plain streaming form (both tuning candidates)."""
+    the LDS-mirrored form, the plain streaming form and the column split over two
+    workgroups (all tuning candidates)."""
     if shape is not None:
         monkeypatch.setenv("MSG_LANCZOS_G", shape)
     rng = workloads.SplitMix64(41, 3)
@@ -198,10 +199,14 @@ def test_large_dimension_streaming_path(ctx, shape, monkeypatch):
     assert np.abs(y - yo).max() <= YIELD_TOL
 
 
-def test_config3_trajectories(ctx):
+@pytest.mark.parametrize("shape", [None, "-3"])
+def test_config3_trajectories(ctx, shape, monkeypatch):
     """Config 3 (D = 65536, 14 isotropic nuclei, ST dephasing 1e-4 /ns): 4 spin-
     coherent trajectories drawn by the backend's sampler, 0-30 ns, against the
-    oracle on the same columns and the same counter-based flips."""
+    oracle on the same columns and the same counter-based flips; the default shape
+    and the column split over eight workgroups."""
+    if shape is not None:
+        monkeypatch.setenv("MSG_LANCZOS_G", shape)
     s, cfg = workloads.config3(n_traj=4)
     p = s.pack()
     sysd = ctx.upload(p)
@@ -211,6 +216,26 @@ def test_config3_trajectories(ctx):
     y, _ = sysd.propagate(prm, columns=cols)
     yo, _ = ref.run_hsgeneral(p, prm, columns=cols, threads=8)
     print("cfg3 dev", y, "oracle", yo)
+    assert np.abs(y - yo).max() <= YIELD_TOL
+
+
+def test_split_column_d32768(ctx, monkeypatch):
+    """D = 32768 (13 anisotropic spin-1/2 nuclei) with each column split over four
+    workgroups: 4 direct columns, 2 orientations, 0-20 ns, against the oracle."""
+    rng = workloads.SplitMix64(43, 1)
+    hf = [workloads._aniso_hf(rng) for _ in range(13)]
+    s = workloads._radical_pair("split", ["1/2"] * 7, ["1/2"] * 6, hf, 1e-3, swept=True)
+    p = s.pack()
+    sysd = ctx.upload(p)
+    assert sysd.dim == 32768
+    B = ref.direct_factors(p)
+    cols = B[:, [3, 2050, 5000, 8191]] * math.sqrt(B.shape[1] / 4.0)
+    fields = [workloads.fibonacci_point(64, i, 0.05) for i in (9, 33)]
+    prm = abi.run_params(total_time=20.0, time_step=2.0, sampling=abi.MSG_SAMPLING_GIVEN)
+    monkeypatch.setenv("MSG_LANCZOS_G", "-3")
+    y, _ = sysd.propagate(prm, fields=fields, columns=cols)
+    yo, _ = ref.run_hsgeneral(p, prm, fields=fields, columns=cols, threads=8)
+    print("D=32768 split dev", y, "oracle", yo)
     assert np.abs(y - yo).max() <= YIELD_TOL
 
 
@@ -344,10 +369,14 @@ def test_config2_against_fixture(ctx):
     assert err <= YIELD_TOL
 
 
-def test_config3_against_fixture(ctx):
+@pytest.mark.parametrize("shape", [None, "-3"])
+def test_config3_against_fixture(ctx, shape, monkeypatch):
     """Config 3 (D = 65536): 32 spin-coherent trajectories drawn by the backend's
     sampler (same columns as the oracle's up to a phase per column), ST dephasing
-    with flip seed 2026, 0-200 ns, against the committed FP64 fixture."""
+    with flip seed 2026, 0-200 ns, against the committed FP64 fixture; the default
+    shape and the column split over eight workgroups."""
+    if shape is not None:
+        monkeypatch.setenv("MSG_LANCZOS_G", shape)
     fx = load_fixture("cfg3_subset")
     M = fx["trajectories"]
     s, cfg = workloads.config3(n_traj=M)
