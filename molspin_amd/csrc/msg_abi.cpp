@@ -779,11 +779,11 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     // launch geometry
     if (b.path == Path::Lanczos) {
         const size_t lds = b.G == 0 ? 0 : b.G == -1 ? (size_t)8 * D : b.G == -2 ? (size_t)32 * kTileGroups
-                         : b.G == -3 ? (size_t)64 * kSplitGroups
+                         : b.G == -3 ? (size_t)128 * b.T
                                      : b.arnoldi ? (size_t)8 * D + 4096 : (size_t)16 * D + 4096;
         const int occ = std::max(1, lanczos_occupancy(b.G, b.T, lds, b.arnoldi));
         // split columns: a slot is S workgroups, all resident at once
-        const int S = b.G == -3 ? (int)(D / 4 / kSplitGroups) : 1;
+        const int S = b.G == -3 ? (int)(D / 4 / (2 * b.T)) : 1;
         if (ctx->cus * occ < S) throw ArgError{MSG_UNSUPPORTED, "split column wider than the device"};
         b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ / S));
         if (ts && ts->slots > 0) b.slots = std::max(1, std::min(b.slots, ts->slots));

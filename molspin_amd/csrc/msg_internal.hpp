@@ -18,7 +18,6 @@ constexpr int kMaxNuclei = 16;     // matrix-free path: D = 4 * 2^N <= 262144
 constexpr int kMaxInitial = 4;
 constexpr int kChebMax = 96;
 constexpr int kTileGroups = 4096;  // groups per LDS tile of the tiled streaming Lanczos (bits 0-11)
-constexpr int kSplitGroups = 2048; // groups per workgroup slab of a split column (bits 0-10)
 
 // Per-item coefficient block of the matrix-free operator (floats):
 //   [0..2]  w1: field on electron 1 (H += w1 . S1), [3..5] w2: field on electron 2

@@ -1274,11 +1274,11 @@ __global__ void k_reduce_columns(const double *in, double *out, int n_items, int
 }
 
 // ---------------------------------------------------------------------------
-// Matrix-free Lanczos with each column split over S = 2^(NN - kSplitBits)
-// workgroups (D = 4 2^NN >= 16384).  A workgroup of 1024 threads owns a slab of
-// 2048 groups (index bits 0-10), two per thread, and keeps the slab of v_j and
-// v_{j-1} in LDS (128 KiB); the residual stays in registers.  The nuclei on bits 11
-// and up couple slabs: their neighbours come from v_j in the slot's global basis,
+// Matrix-free Lanczos with each column split over S = D / (8 T) workgroups of T
+// threads (D = 4 2^NN >= 16384).  A workgroup owns a slab of 2 T groups (the low
+// index bits), two per thread, and keeps the slab of v_j and v_{j-1} in LDS (64 T
+// bytes: one workgroup per CU at T = 1024, two at T = 512); the residual stays in
+// registers.  The nuclei on the higher bits couple slabs: their neighbours come from v_j in the slot's global basis,
 // which every workgroup of the slot writes before the slot's barrier.  Inner
 // products are reduced over the workgroup, then over the slot's workgroups in slab
 // order (slot_sum), so every workgroup of a slot sees the same alpha, beta and
@@ -1288,8 +1288,6 @@ __global__ void k_reduce_columns(const double *in, double *out, int n_items, int
 // launch of a multiple of 8 slots); a barrier wait that does not end sets status
 // bit 2 and ends the launch.
 // ---------------------------------------------------------------------------
-constexpr int kSplitBits = 11;  // groups per slab: 2^11
-static_assert((1 << kSplitBits) == kSplitGroups, "slab size shared with the host");
 
 // Barrier over the S workgroups of a slot: counter `bar` counts arrivals over the
 // launch, `nbar` the barriers this workgroup has passed.  Every thread calls it;
@@ -1342,11 +1340,10 @@ __device__ __forceinline__ bool slot_sum(double (&v)[K], double *xr, int *bar, i
     return true;
 }
 
-template <int NN>
-__global__ __launch_bounds__(1024, 4) void k_lanczos_split(const StepArgs *__restrict__ pa) {
+template <int NN, int T>
+__global__ __launch_bounds__(T, 4) void k_lanczos_split(const StepArgs *__restrict__ pa) {
     const StepArgs &a = *pa;
-    constexpr int T = 1024, G = 2, SB = kSplitBits, SG = 1 << SB, S = 1 << (NN - SB);
-    static_assert(G * T == SG, "two groups per thread fill a slab");
+    constexpr int G = 2, SG = G * T, SB = Log2<SG>::value, S = 1 << (NN - SB);
     extern __shared__ float4 smem[];  // slab of v_j and v_{j-1}: planes A and B of each buffer
     auto LA = [&](int i) -> float4 * { return smem + (i ? 2 * SG : 0); };
     auto LB = [&](int i) -> float4 * { return smem + SG + (i ? 2 * SG : 0); };
@@ -2163,20 +2160,21 @@ __global__ void k_to_planes(const double2 *__restrict__ U, float *__restrict__ d
 // and of the Arnoldi kernel (two groups per thread at D = 1024 and 2048)
 #define MSG_ARNOLDI_EXTRA(X) X(2, 128, true) X(2, 256, true)
 
-// Split-column kernels for D = 4 2^NN, NN = 12..16
-#define MSG_SPLIT_VARIANTS(X) X(12) X(13) X(14) X(15) X(16)
+// Split-column kernels for D = 4 2^NN, NN = 12..16, with 1024 threads (512-thread
+// workgroups, two per CU over twice as many slabs, measured slower at D = 16384 and
+// 65536: more partner slabs and wider barriers)
+#define MSG_SPLIT_VARIANTS(X) X(12, 1024) X(13, 1024) X(14, 1024) X(15, 1024) X(16, 1024)
 
 hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_thread, int threads, int slots,
                           hipStream_t st) {
     if (group_per_thread == -3) {  // split columns: `slots` column slots of S workgroups each
-        if (threads != 1024) return hipErrorInvalidValue;
         const int nn = 31 - __builtin_clz((unsigned)(a.dim >> 2));  // D / 4 = 2^nn
-        const int S = 1 << (nn - kSplitBits);
-        const size_t lds = (size_t)4 * (1 << kSplitBits) * sizeof(float4);
+        const int S = (a.dim >> 2) / (2 * threads);
+        const size_t lds = (size_t)4 * 2 * threads * sizeof(float4);
         void *args[] = {(void *)&da};
-#define X(NN_)                                                                                                  \
-        if (nn == NN_)                                                                                          \
-            return hipLaunchCooperativeKernel((const void *)k_lanczos_split<NN_>, dim3(slots * S), dim3(1024), args, \
+#define X(NN_, T_)                                                                                                \
+        if (nn == NN_ && threads == T_)                                                                           \
+            return hipLaunchCooperativeKernel((const void *)k_lanczos_split<NN_, T_>, dim3(slots * S), dim3(T_), args, \
                                               (unsigned)lds, st);
         MSG_SPLIT_VARIANTS(X)
 #undef X
@@ -2244,8 +2242,8 @@ hipError_t set_lanczos_lds_limits() {
 #define X(NT_) (void)hipFuncSetAttribute((const void *)k_dense<NT_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     MSG_DENSE_VARIANTS(X)
 #undef X
-#define X(NN_) \
-    (void)hipFuncSetAttribute((const void *)k_lanczos_split<NN_>, hipFuncAttributeMaxDynamicSharedMemorySize, maxb - 8192);
+#define X(NN_, T_) \
+    (void)hipFuncSetAttribute((const void *)k_lanczos_split<NN_, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, maxb - 8192);
     MSG_SPLIT_VARIANTS(X)
 #undef X
     (void)hipFuncSetAttribute((const void *)k_lanczos_big<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2279,8 +2277,9 @@ int lanczos_occupancy(int group_per_thread, int threads, size_t lds, bool arnold
 #undef X
         return 1;
     }
-    if (group_per_thread == -3)  // the split kernels share one resource footprint
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_split<14>, 1024, lds) == hipSuccess ? nb : 1;
+    if (group_per_thread == -3)  // the split kernels of one width share one resource footprint
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_split<14, 1024>, threads, lds) == hipSuccess
+                   ? nb : 1;
     if (group_per_thread == 0)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<false>, 1024, 0) == hipSuccess ? nb : 1;
     if (group_per_thread == -1)

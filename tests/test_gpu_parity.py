@@ -175,14 +175,19 @@ def test_step_entry_matches_oracle(ctx):
     assert np.abs(np.array(pops) - tro[0, 1]).max() < 1e-6
 
 
+def force_shape(monkeypatch, shape):
+    """Launch shape override (MSG_LANCZOS_G)."""
+    if shape is not None:
+        monkeypatch.setenv("MSG_LANCZOS_G", shape)
+
+
 @pytest.mark.parametrize("shape", [None, "-1", "0", "-3"])
 def test_large_dimension_streaming_path(ctx, shape, monkeypatch):
     """D = 16384 (7 + 5 anisotropic spin-1/2 nuclei): the large-D matrix-free
     variants on 6 direct columns and 2 orientations over 0-60 ns: the default shape,
     the LDS-mirrored form, the plain streaming form and the column split over two
     workgroups (all tuning candidates)."""
-    if shape is not None:
-        monkeypatch.setenv("MSG_LANCZOS_G", shape)
+    force_shape(monkeypatch, shape)
     rng = workloads.SplitMix64(41, 3)
     hf = [workloads._aniso_hf(rng) for _ in range(12)]
     s = workloads._radical_pair("big", ["1/2"] * 7, ["1/2"] * 5, hf, 1e-3, swept=True)
@@ -205,8 +210,7 @@ def test_config3_trajectories(ctx, shape, monkeypatch):
     coherent trajectories drawn by the backend's sampler, 0-30 ns, against the
     oracle on the same columns and the same counter-based flips; the default shape
     and the column split over eight workgroups."""
-    if shape is not None:
-        monkeypatch.setenv("MSG_LANCZOS_G", shape)
+    force_shape(monkeypatch, shape)
     s, cfg = workloads.config3(n_traj=4)
     p = s.pack()
     sysd = ctx.upload(p)
@@ -232,7 +236,7 @@ def test_split_column_d32768(ctx, monkeypatch):
     cols = B[:, [3, 2050, 5000, 8191]] * math.sqrt(B.shape[1] / 4.0)
     fields = [workloads.fibonacci_point(64, i, 0.05) for i in (9, 33)]
     prm = abi.run_params(total_time=20.0, time_step=2.0, sampling=abi.MSG_SAMPLING_GIVEN)
-    monkeypatch.setenv("MSG_LANCZOS_G", "-3")
+    force_shape(monkeypatch, "-3")
     y, _ = sysd.propagate(prm, fields=fields, columns=cols)
     yo, _ = ref.run_hsgeneral(p, prm, fields=fields, columns=cols, threads=8)
     print("D=32768 split dev", y, "oracle", yo)
@@ -375,8 +379,7 @@ def test_config3_against_fixture(ctx, shape, monkeypatch):
     sampler (same columns as the oracle's up to a phase per column), ST dephasing
     with flip seed 2026, 0-200 ns, against the committed FP64 fixture; the default
     shape and the column split over eight workgroups."""
-    if shape is not None:
-        monkeypatch.setenv("MSG_LANCZOS_G", shape)
+    force_shape(monkeypatch, shape)
     fx = load_fixture("cfg3_subset")
     M = fx["trajectories"]
     s, cfg = workloads.config3(n_traj=M)
