@@ -154,10 +154,12 @@ def lanczos_roofline(D, m, state_steps, kernel_s, traffic):
 # device copy: the FETCH_SIZE count is doubled, WRITE_SIZE taken as is); the
 # traffic of a launch scales with its state-steps at fixed D.  The counter runs
 # are summarised in profiles/k_lanczos_orientation_batch_D4096.md,
-# profiles/k_lanczos_big_trajectories_D65536.md, profiles/k_lanczos_big_pair_D16384.md,
+# profiles/k_lanczos_big_trajectories_D65536.md, profiles/k_lanczos_split_trajectories_D65536.md,
+# profiles/k_lanczos_big_pair_D16384.md,
 # profiles/k_dense_field_sweep_D512.md and profiles/k_arnoldi_singlet_sink_D4096.md.
-MEASURED_TRAFFIC_PER_STATE_STEP = {"orientations": 1.066e+06, "trajectories": 7.523e+07, "dense": 2.51e+04,
-                                   "pair": 6.60e+06, "arnoldi": 5.50e+06}
+MEASURED_TRAFFIC_PER_STATE_STEP = {"orientations": 1.066e+06, "dense": 2.51e+04, "pair/-1": 6.60e+06, "arnoldi": 5.50e+06,
+                                   # D = 65536 by launch shape: tiled streaming (-2), split columns (-3)
+                                   "trajectories/-2": 7.523e+07, "trajectories/-3": 3.154e+07}
 
 
 def orientation_batch(ctx, args):
@@ -212,7 +214,7 @@ def trajectory_batch(ctx, args):
         sysd.propagate(prm, columns=cols)
 
     def roof(kernel_s):
-        tr = MEASURED_TRAFFIC_PER_STATE_STEP.get("trajectories")
+        tr = MEASURED_TRAFFIC_PER_STATE_STEP.get("trajectories/%d" % shape.groups_per_thread)
         return lanczos_roofline(D, m, state_steps, kernel_s, None if tr is None else tr * state_steps)
 
     cpu_prm = abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_KRYLOV,
@@ -248,7 +250,7 @@ def pair_batch(ctx, args):
         sysd.propagate(prm, fields=fields)
 
     def roof(kernel_s):
-        tr = MEASURED_TRAFFIC_PER_STATE_STEP.get("pair")
+        tr = MEASURED_TRAFFIC_PER_STATE_STEP.get("pair/%d" % shape.groups_per_thread)
         return lanczos_roofline(D, m, state_steps, kernel_s, None if tr is None else tr * state_steps)
 
     return dict(step=step, state_steps=state_steps, roofline=roof, packed=packed,
