@@ -384,18 +384,20 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
 /* Launch shape chosen by msg_autotune. */
 typedef struct msg_launch_shape {
     int32_t path;                 /* 1 matrix-free Lanczos, 2 dense MFMA */
-    int32_t groups_per_thread;    /* Lanczos: electron-pair groups per thread (1, 2), -1 streaming with the
-                                     Krylov vector in LDS, -2 streaming with LDS tiles, 0 plain streaming */
+    int32_t groups_per_thread;    /* Lanczos: electron-pair groups per thread (1, 2, 4), -1 streaming with the
+                                     Krylov vector in LDS, -2 streaming with LDS tiles, 0 plain streaming,
+                                     -3 each column split over several workgroups */
     int32_t threads;              /* workgroup size */
-    int32_t slots;                /* resident workgroups (persistent launch) */
+    int32_t slots;                /* resident workgroups (persistent launch); columns in flight for -3 */
     double kernel_ms;             /* device time of the chosen shape on the sample */
 } msg_launch_shape;
 
 /*
  * Autotuner (no counterpart in the reference: the reference's CPU loop has no
  * launch shapes).  Times the candidate launch shapes of this system's path --
- * workgroup shape and Krylov-vector placement (LDS, LDS tiles or streamed), and
- * the number of resident workgroups -- on a sample of `params` (n_items items
+ * workgroup shape and Krylov-vector placement (LDS, LDS tiles, streamed, or a
+ * column split over several workgroups), and the number of resident workgroups --
+ * on a sample of `params` (n_items items
  * of `fields`/`columns` as for msg_propagate, `sample_steps` time steps) and
  * keeps the fastest for this context and Hilbert dimension: later
  * msg_propagate, msg_step and server launches on `ctx` use it.  Results do not
