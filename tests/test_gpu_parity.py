@@ -409,7 +409,8 @@ def test_config1_trace_against_fixture(ctx):
 def test_autotune_keeps_results(ctx):
     """msg_autotune picks a launch shape per Hilbert dimension; yields computed
     with the tuned shape agree with the default shape's to FP32 rounding, for the
-    LDS path (D = 4096) and the tiled streaming path (D = 65536)."""
+    LDS path (D = 4096) and the large-D path (D = 65536: tiled, plain streaming or
+    split columns)."""
     from molspin_amd import backend
     s, cfg = workloads.config2(n_orientations=1024)
     fields = cfg["fields"][:4]
@@ -431,7 +432,7 @@ def test_autotune_keeps_results(ctx):
         z0, _ = sys3.propagate(p3, columns=cols)
         sh3 = sys3.autotune(p3, columns=cols, sample_steps=2)
         print("D=65536 tuned", sh3.groups_per_thread, sh3.threads, sh3.slots, sh3.kernel_ms)
-        assert sh3.groups_per_thread in (-2, 0)
+        assert sh3.groups_per_thread in (-3, -2, 0)
         z1, _ = sys3.propagate(p3, columns=cols)
         assert np.abs(z1 - z0).max() <= 1e-7
     finally:
