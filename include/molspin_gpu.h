@@ -442,8 +442,10 @@ msg_status msg_server_submit(msg_server *srv, const msg_system *sys, const msg_r
  * while a slot and the HBM budget allow, shortest first (state-steps x D; arrival
  * order among equals).  A request with method MSG_METHOD_UNCOUPLED is computed
  * within the call (at most one per call).  max_steps <= 0: every admitted request
- * is propagated over its whole grid in one persistent launch on its own stream,
- * so requests overlap on the device.  max_steps > 0: resident requests advance
+ * is propagated over its whole grid on its own stream, in persistent launches of
+ * a few fields each (about 50 ms of device time: a large request does not hold the
+ * device while small ones wait), so requests overlap on the device.
+ * max_steps > 0: resident requests advance
  * in chunks of max_steps intervals per call, their states kept on the device.
  * Sets *n_active to the number of requests still queued or running. */
 msg_status msg_server_pump(msg_server *srv, int32_t max_steps, int32_t *n_active);

@@ -1013,10 +1013,18 @@ struct Request {
     std::string error;
     int64_t bytes = 0;
     int stream = -1;             // index of the server stream the request owns
-    bool whole = false;          // launched to completion in one persistent launch
+    bool whole = false;          // launched to completion in persistent launches of field chunks
     hipEvent_t done_ev = nullptr;
     double work = 0.0;           // state-steps x D: admission order (shortest first)
+    int f_begin = 0, f_count = 0;  // fields of the launch in flight (whole mode)
+    int nobs = 0;
 };
+
+// Work (D x columns x samples) of one whole-mode launch of the server: about 50 ms of
+// device time at D = 16384 (one field of 4096 columns over 6 samples), so one large
+// request holds the device for at most that long before smaller requests' launches,
+// queued on the other streams, get their turn.
+constexpr double kServerChunkWork = 4.0e8;
 
 struct msg_server {
     msg_ctx *ctx = nullptr;
@@ -1882,13 +1890,30 @@ msg_status msg_server_pump(msg_server *srv, int32_t max_steps, int32_t *n_active
     return guarded(srv->ctx, [&] {
         msg_ctx *ctx = srv->ctx;
         HIPCHECK(hipSetDevice(ctx->device));
-        auto finish = [&](std::unique_ptr<Request> &q) {
-            q->result.assign((size_t)q->n_fields * q->batch->nobs, 0.0);
+        // yields of the launch in flight into the request's rows [f_begin, f_begin + f_count)
+        auto collect_part = [&](Request &q) {
+            if (q.result.empty()) q.result.assign((size_t)q.n_fields * q.batch->nobs, 0.0);
             try {
-                collect(*q->batch, q->result.data(), nullptr);
+                collect(*q.batch, q.result.data() + (size_t)q.f_begin * q.batch->nobs, nullptr);
             } catch (const ArgError &e) {
-                q->failed = true;
-                q->error = e.what;
+                q.failed = true;
+                q.error = e.what;
+            }
+        };
+        // one persistent launch over the whole time grid for fields [f_begin, f_begin + f_count)
+        auto launch_part = [&](Request &q) {
+            q.batch.reset(new Batch());
+            q.batch->no_split = true;  // requests share the device
+            prepare(*q.batch, ctx, q.sys, &q.prm, q.f_count, q.fields.data() + 3 * (size_t)q.f_begin, nullptr, 0, false,
+                    false, srv->streams[q.stream]);
+            run_chunk(*q.batch, q.batch->nsamples);
+            if (!q.done_ev) HIPCHECK(hipEventCreateWithFlags(&q.done_ev, hipEventDisableTiming));
+            HIPCHECK(hipEventRecord(q.done_ev, srv->streams[q.stream]));
+        };
+        auto finish = [&](std::unique_ptr<Request> &q) {
+            if (!q->whole) {
+                q->f_begin = 0;
+                collect_part(*q);
             }
             srv->in_use -= q->bytes;
             q->batch.reset();
@@ -1902,6 +1927,23 @@ msg_status msg_server_pump(msg_server *srv, int32_t max_steps, int32_t *n_active
         for (auto it = srv->active.begin(); it != srv->active.end();) {
             Request &q = **it;
             const bool done = q.whole ? hipEventQuery(q.done_ev) == hipSuccess : q.batch->s_next >= q.batch->nsamples - 1;
+            if (done && q.whole) {
+                collect_part(q);
+                q.f_begin += q.f_count;
+                if (!q.failed && q.f_begin < q.n_fields) {  // next field chunk on the same stream
+                    q.f_count = std::min(q.f_count, q.n_fields - q.f_begin);
+                    try {
+                        launch_part(q);
+                    } catch (const ArgError &e) {
+                        q.failed = true;
+                        q.error = e.what;
+                    }
+                    if (!q.failed) {
+                        ++it;
+                        continue;
+                    }
+                }
+            }
             if (done) {
                 finish(*it);
                 it = srv->active.erase(it);
@@ -1938,28 +1980,32 @@ msg_status msg_server_pump(msg_server *srv, int32_t max_steps, int32_t *n_active
                 break;  // one synchronous request per call, so callers collect between them
             }
             q->stream = slot;
-            q->batch.reset(new Batch());
-            q->batch->no_split = true;  // requests share the device
             q->whole = max_steps <= 0;
+            q->f_begin = 0;
+            q->f_count = q->whole ? (int)std::max(1.0, std::min((double)q->n_fields,
+                                                                 std::floor(kServerChunkWork * q->n_fields / q->work)))
+                                  : q->n_fields;
             try {
-                prepare(*q->batch, ctx, q->sys, &q->prm, q->n_fields, q->fields.data(), nullptr, 0, false, !q->whole,
-                        srv->streams[slot]);
+                if (q->whole) {
+                    launch_part(*q);
+                } else {
+                    q->batch.reset(new Batch());
+                    q->batch->no_split = true;  // requests share the device
+                    prepare(*q->batch, ctx, q->sys, &q->prm, q->n_fields, q->fields.data(), nullptr, 0, false, true,
+                            srv->streams[slot]);
+                }
             } catch (const ArgError &e) {
                 q->failed = true;
                 q->error = e.what;
                 q->batch.reset();
+                if (q->done_ev) (void)hipEventDestroy(q->done_ev);
+                q->done_ev = nullptr;
                 const int64_t id = q->id;
                 srv->finished[id] = std::move(q);
                 continue;
             }
             srv->stream_busy[slot] = 1;
             srv->in_use += q->bytes;
-            if (q->whole) {
-                // one persistent launch over the whole time grid; completion signalled by an event
-                run_chunk(*q->batch, q->batch->nsamples);
-                HIPCHECK(hipEventCreateWithFlags(&q->done_ev, hipEventDisableTiming));
-                HIPCHECK(hipEventRecord(q->done_ev, srv->streams[slot]));
-            }
             srv->active.push_back(std::move(q));
         }
         // 3. chunked mode: one chunk of every resident request, overlapping on the streams

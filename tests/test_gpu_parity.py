@@ -311,6 +311,32 @@ def test_server_alone_equals_mixed(ctx):
         assert np.abs(m - yo).max() <= YIELD_TOL
 
 
+def test_server_whole_mode_field_chunks(ctx):
+    """Whole-mode serving (pump(0)): a D = 4096 request of 40 fields runs as
+    persistent launches of a few fields each (so it cannot hold the device for
+    long), beside a D = 512 request admitted after it; both equal their
+    msg_propagate results."""
+    import time
+    from molspin_amd import backend
+    s4, cfg4 = workloads.config4_pair(1)
+    s5, cfg5 = workloads.config4_pair(20)
+    sd4, sd5 = ctx.upload(s4.pack()), ctx.upload(s5.pack())
+    assert sd4.dim == 4096 and sd5.dim == 512
+    prm = abi.run_params(total_time=40.0, time_step=2.0)
+    f4, f5 = cfg4["fields"][:40], cfg5["fields"][::8]
+    a4, a5 = sd4.propagate(prm, fields=f4)[0], sd5.propagate(prm, fields=f5)[0]
+    srv = backend.Server(ctx, max_slots=2, hbm_budget_bytes=8 << 30)
+    i4 = srv.submit(sd4, prm, f4)
+    srv.pump(0)
+    i5 = srv.submit(sd5, prm, f5)
+    while srv.pump(0):
+        time.sleep(0.001)
+    m4, m5 = srv.collect(i4), srv.collect(i5)
+    srv.close()
+    assert m4.shape == a4.shape and m5.shape == a5.shape
+    assert np.abs(m4 - a4).max() <= 1e-12 and np.abs(m5 - a5).max() <= 1e-12
+
+
 def test_symmetric_decay_golden_values(ctx):
     """Device eigenbasis yields (TaskStaticHSSymmetricDecay) reproduce the
     reference's golden values at k = 1e-4 (tests_TaskStaticHSSymmetricDecay.cpp:64-69)
