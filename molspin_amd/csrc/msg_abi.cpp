@@ -666,7 +666,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         b.field_given = fields != nullptr;
         if (fields) b.item_fields.assign(fields, fields + 3 * (size_t)n_items);
         // propagator planes of one chunk: at most ~1 GiB
-        const size_t per = (size_t)n_items * 2 * Dp * Dp * sizeof(float);
+        const size_t per = (size_t)n_items * kDensePlanes * Dp * Dp * sizeof(float);
         b.dyn_chunk = (int)std::max<size_t>(1, std::min<size_t>(ns - 1, ((size_t)1 << 30) / per));
         if (const char *env = std::getenv("MSG_DYN_CHUNK"))  // smaller chunks (tests of the resume)
             if (std::atoi(env) >= 1) b.dyn_chunk = std::min(b.dyn_chunk, std::atoi(env));
@@ -683,7 +683,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         // exp(-i H_eff dt) on the device in FP64: Taylor(18) of X = A / 2^s evaluated by
         // Paterson-Stockmeyer in X^4 (7 complex GEMMs), then s squarings; items are
         // processed in chunks to bound the FP64 workspace
-        float *dU = b.U.alloc<float>((size_t)n_items * per * 2 * Dp * Dp, stream);
+        float *dU = b.U.alloc<float>((size_t)n_items * per * kDensePlanes * Dp * Dp, stream);
         const int chunk = std::max(1, std::min(n_items, (int)((size_t)128 << 20) / (D * D * 16)));
         DBuf X, P, Q, X2, X3, X4, Bk;
         cd *dX = X.alloc<cd>((size_t)chunk * D * D, stream);
@@ -718,8 +718,8 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                         HIPCHECK(zgemm(dP, dP, dQ, D, nb, 1.0, stream));
                         std::swap(dP, dQ);
                     }
-                    HIPCHECK(launch_to_planes(dP, dU + ((size_t)it0 * per + w) * 2 * Dp * Dp, D, Dp,
-                                              (size_t)per * 2 * Dp * Dp, nb, stream));
+                    HIPCHECK(launch_to_planes(dP, dU + ((size_t)it0 * per + w) * kDensePlanes * Dp * Dp, D, Dp,
+                                              (size_t)per * kDensePlanes * Dp * Dp, nb, stream));
                     continue;
                 }
                 for (int q = 0; q < nb; ++q) {
@@ -743,8 +743,8 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                     HIPCHECK(zgemm(dP, dP, dQ, D, nb, 1.0, stream));
                     std::swap(dP, dQ);
                 }
-                HIPCHECK(launch_to_planes(dP, dU + ((size_t)it0 * per + w) * 2 * Dp * Dp, D, Dp,
-                                          (size_t)per * 2 * Dp * Dp, nb, stream));
+                HIPCHECK(launch_to_planes(dP, dU + ((size_t)it0 * per + w) * kDensePlanes * Dp * Dp, D, Dp,
+                                          (size_t)per * kDensePlanes * Dp * Dp, nb, stream));
             }
         }
         // X, P, Q are released (stream-ordered) after the conversion
@@ -806,7 +806,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         a.norm0 = b.norm0.alloc<double>(b.n_units, stream);
         if (a.renorm) HIPCHECK(launch_init_norms(a, push_args(b, a), stream));
     } else {
-        const size_t lds = (size_t)8 * a.dpad * 16 + 4096;
+        const size_t lds = (size_t)4 * kDenseLdsPlanes * a.dpad * 16 + 4096;
         const int occ = std::max(1, dense_occupancy(a.dpad, lds));
         b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ));
         if (b.share > 1) b.slots = std::max(1, std::min(b.slots, ctx->cus * occ / b.share));
@@ -857,7 +857,7 @@ void build_dyn_chunk(Batch &b, int s0, int s1) {
     hipStream_t st = b.stream;
     const bool rk4 = b.prm.method == MSG_METHOD_RK4;
     const int nz = (int)b.dyn_terms.size(), stages = rk4 ? 3 : 1;
-    float *dU = b.U.p ? b.U.get<float>() : b.U.alloc<float>((size_t)b.n_items * b.dyn_chunk * 2 * Dp * Dp, st);
+    float *dU = b.U.p ? b.U.get<float>() : b.U.alloc<float>((size_t)b.n_items * b.dyn_chunk * kDensePlanes * Dp * Dp, st);
     b.a.U = dU;  // the launch reads the chunk's propagators from here
     const int total = b.n_items * nsc;
     const int nbuf = rk4 ? 10 : 7;
@@ -929,7 +929,8 @@ void build_dyn_chunk(Batch &b, int s0, int s1) {
             HIPCHECK(launch_zlincomb(dB, dX4, dX4, dP, D, nb, c2, st));            // + K4/6
             res = dP;
         }
-        HIPCHECK(launch_to_planes(res, dU + (size_t)q0 * 2 * Dp * Dp, D, Dp, (size_t)2 * Dp * Dp, nb, st));
+        HIPCHECK(launch_to_planes(res, dU + (size_t)q0 * kDensePlanes * Dp * Dp, D, Dp, (size_t)kDensePlanes * Dp * Dp,
+                                  nb, st));
         HIPCHECK(hipStreamSynchronize(st));  // host staging reused by the next sub-batch
     }
 }
@@ -1170,7 +1171,7 @@ msg_status msg_query_footprint(const msg_system *sys, const msg_run_params *p, i
     } else {
         const int64_t Dp = dense_padding((int)D);
         const int64_t units = (int64_t)n_items * ((ncols + 15) / 16);
-        b += (int64_t)n_items * 2 * 8 * Dp * Dp;
+        b += (int64_t)n_items * 2 * 4 * kDensePlanes * Dp * Dp;
         b += units * nobs * 16;
     }
     if (p->sampling == MSG_SAMPLING_GIVEN) b += (int64_t)ncols * 8 * D;

@@ -18,6 +18,8 @@ constexpr int kMaxNuclei = 16;     // matrix-free path: D = 4 * 2^N <= 262144
 constexpr int kMaxInitial = 4;
 constexpr int kChebMax = 96;
 constexpr int kTileGroups = 4096;  // groups per LDS tile of the tiled streaming Lanczos (bits 0-11)
+constexpr int kDensePlanes = 2;    // dense-path propagator planes: re, im
+constexpr int kDenseLdsPlanes = 3; // dense-path state tile in LDS: re, im, re + im
 
 // Per-item coefficient block of the matrix-free operator (floats):
 //   [0..2]  w1: field on electron 1 (H += w1 . S1), [3..5] w2: field on electron 2
@@ -49,7 +51,7 @@ struct StepArgs {
     const float *dyn_head;  // [n_items][nsamples-1][8] head (w1, w2, shift) of each interval of a
                             // time-dependent operator, or NULL (static: the head of `coef`)
     // dense path
-    const float *U;    // [n_items][ustride][2][Dp*Dp] planes re, im, column-major (Dp = padded dim)
+    const float *U;    // [n_items][ustride][kDensePlanes][Dp*Dp] planes re, im (Dp = padded dim)
     int dpad;
     int ustride;       // propagators per item (2 when the last interval is shorter)
     int ulast;         // use the item's second propagator for interval nsamples-2

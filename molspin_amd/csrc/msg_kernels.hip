@@ -1600,8 +1600,8 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
     __shared__ double s_col[2][64];              // per-wave column norms (double-buffered)
     constexpr int Dp = 16 * NT;
     constexpr int KT = (NT + 3) / 4;  // row tiles per wave
-    float *Bre = dsm, *Bim = dsm + Dp * 16;
-    double *scr = reinterpret_cast<double *>(dsm + 2 * Dp * 16);
+    float *Bre = dsm, *Bim = dsm + Dp * 16, *Bsm = dsm + 2 * Dp * 16;  // B: re, im, re + im
+    double *scr = reinterpret_cast<double *>(dsm + kDenseLdsPlanes * Dp * 16);
     __shared__ int s_unit;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int q = lane >> 4, cl = lane & 15;
@@ -1692,16 +1692,21 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
                     for (int i = 0; i < 4; ++i) {
                         Bre[(rt * 16 + 4 * q + i) * 16 + cl] = xr[k][i];
                         Bim[(rt * 16 + 4 * q + i) * 16 + cl] = xi[k][i];
+                        Bsm[(rt * 16 + 4 * q + i) * 16 + cl] = xr[k][i] + xi[k][i];
                     }
             }
             __syncthreads();
-            // B' = U B: real MFMA form of the complex product
+            // B' = U B as three real products on the matrix cores (Gauss):
+            // P1 = Ur Br, P2 = Ui Bi, P3 = (Ur + Ui)(Br + Bi); re = P1 - P2, im = P3 - P1 - P2
             const int uw = a.ustep ? s - a.s_begin : ((a.ulast && s == a.nsamples - 2) ? 1 : 0);
-            const float *Ur = a.U + ((size_t)item * a.ustride + uw) * 2 * Dp * Dp, *Ui = Ur + (size_t)Dp * Dp;
+            const float *Ur = a.U + ((size_t)item * a.ustride + uw) * kDensePlanes * Dp * Dp;
+            const float *Ui = Ur + (size_t)Dp * Dp;
+            floatx4 x2[KT];  // P2 (xr holds P1, xi P3)
 #pragma unroll
             for (int k = 0; k < KT; ++k) {
                 xr[k] = floatx4{0.f, 0.f, 0.f, 0.f};
                 xi[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+                x2[k] = floatx4{0.f, 0.f, 0.f, 0.f};
             }
             // k-groups of 16: one float4 per lane and plane carries four k-steps of A;
             // the A operands of group g + 1 are fetched while group g multiplies
@@ -1714,12 +1719,13 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
                 ni4[k] = rt < ntile ? *reinterpret_cast<const float4 *>(Ui + o) : float4{0.f, 0.f, 0.f, 0.f};
             }
             for (int g = 0; g < ntile; ++g) {
-                float br[4], bi[4];
+                float br[4], bi[4], bs[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int kr = 16 * g + 4 * u + q;
                     br[u] = Bre[kr * 16 + cl];
                     bi[u] = Bim[kr * 16 + cl];
+                    bs[u] = Bsm[kr * 16 + cl];
                 }
                 float4 cr4[KT], ci4[KT];
 #pragma unroll
@@ -1739,15 +1745,20 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
                     if (rt < ntile) {
                         const float4 ar4 = cr4[k], ai4 = ci4[k];
                         const float ar[4] = {ar4.x, ar4.y, ar4.z, ar4.w}, ai[4] = {ai4.x, ai4.y, ai4.z, ai4.w};
+                        const float as[4] = {ar4.x + ai4.x, ar4.y + ai4.y, ar4.z + ai4.z, ar4.w + ai4.w};
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
                             xr[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[u], br[u], xr[k], 0, 0, 0);
-                            xr[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai[u], bi[u], xr[k], 0, 0, 0);
-                            xi[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[u], bi[u], xi[k], 0, 0, 0);
-                            xi[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[u], br[u], xi[k], 0, 0, 0);
+                            x2[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[u], bi[u], x2[k], 0, 0, 0);
+                            xi[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(as[u], bs[u], xi[k], 0, 0, 0);
                         }
                     }
                 }
+            }
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                xi[k] = xi[k] - xr[k] - x2[k];
+                xr[k] = xr[k] - x2[k];
             }
             if (a.renorm) {
                 const double n2 = colnorm2(xr, xi);
@@ -2129,8 +2140,8 @@ __global__ void k_unc_combine(const double *__restrict__ CA, const double *__res
     for (int c = 0; c < nobs; ++c) yields[(size_t)item * nobs + c] = ys[c];
 }
 
-// Planes of the dense-path operand: U[b] row-major n x n (double2) ->
-// dst[b*stride + k*Dp + row] = re, dst[b*stride + Dp*Dp + k*Dp + row] = im.
+// Planes of the dense-path operand: U[b] row-major n x n (double2) -> two planes of
+// Dp*Dp floats at dst + b*stride: re, then im.
 // The planes are stored in the MFMA operand order of k_dense: for row tile rt and
 // k-group g (16 columns), lane l = 16 q + i holds rows rt*16 + i and columns
 // 16 g + 4 u + q for u = 0..3 as one float4, so a lane fetches four k-steps of
@@ -2255,7 +2266,7 @@ hipError_t set_lanczos_lds_limits() {
 }
 
 hipError_t launch_dense(const StepArgs &a, const StepArgs *da, int slots, hipStream_t st) {
-    const size_t lds = (size_t)2 * a.dpad * 16 * sizeof(float) + 2 * 256 * sizeof(double);
+    const size_t lds = (size_t)kDenseLdsPlanes * a.dpad * 16 * sizeof(float) + 2 * 256 * sizeof(double);
 #define X(NT_)                                                                  \
     if (a.dpad == 16 * NT_) {                                                   \
         hipLaunchKernelGGL(k_dense<NT_>, dim3(slots), dim3(256), lds, st, da); \
