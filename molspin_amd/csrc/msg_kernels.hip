@@ -1590,16 +1590,22 @@ __device__ __forceinline__ float2 dense_init(const StepArgs &a, int u, int col, 
     return make_float2(st[row * 16 + tcol], st[a.dpad * 16 + row * 16 + tcol]);
 }
 
+// Waves per workgroup of the dense kernel: 8 for the largest tiles (four row tiles
+// per wave keep the three products' accumulators in registers), else 4.
+constexpr int dense_waves(int NT) { return NT >= 24 ? 8 : 4; }
+
 // NT: row tiles of 16 (padded dimension Dp = 16 NT, a compile-time constant so the
-// product loop and the per-wave tile loops unroll); each wave owns tiles wave + 4k.
+// product loop and the per-wave tile loops unroll); NW = dense_waves(NT) waves, each
+// owning tiles wave + NW k.
 template <int NT>
-__global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) {
+__global__ __launch_bounds__(64 * dense_waves(NT)) void k_dense(const StepArgs *__restrict__ pa) {
     const StepArgs &a = *pa;
     extern __shared__ float dsm[];
     __shared__ double ys[kMaxObs], yc[kMaxObs];  // Kahan yield sums (thread 0)
-    __shared__ double s_col[2][64];              // per-wave column norms (double-buffered)
+    __shared__ double s_col[2][16 * 8];          // per-wave column norms (double-buffered)
     constexpr int Dp = 16 * NT;
-    constexpr int KT = (NT + 3) / 4;  // row tiles per wave
+    constexpr int NW = dense_waves(NT);
+    constexpr int KT = (NT + NW - 1) / NW;  // row tiles per wave
     float *Bre = dsm, *Bim = dsm + Dp * 16, *Bsm = dsm + 2 * Dp * 16;  // B: re, im, re + im
     double *scr = reinterpret_cast<double *>(dsm + kDenseLdsPlanes * Dp * 16);
     __shared__ int s_unit;
@@ -1622,7 +1628,10 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
         cpar ^= 1;
         if (q == 0) s_col[cpar][wave * 16 + cl] = d;
         __syncthreads();
-        return s_col[cpar][cl] + s_col[cpar][16 + cl] + s_col[cpar][32 + cl] + s_col[cpar][48 + cl];
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) sum += s_col[cpar][16 * w + cl];
+        return sum;
     };
     for (;;) {
         if (t == 0) s_unit = atomicAdd(a.queue, 1);
@@ -1636,7 +1645,7 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
         floatx4 xr[KT], xi[KT];
 #pragma unroll
         for (int k = 0; k < KT; ++k) {
-            const int rt = wave + 4 * k;
+            const int rt = wave + NW * k;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const float2 v = rt < ntile ? dense_init(a, u, col, rt * 16 + 4 * q + i) : make_float2(0.f, 0.f);
@@ -1686,7 +1695,7 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < KT; ++k) {
-                const int rt = wave + 4 * k;
+                const int rt = wave + NW * k;
                 if (rt < ntile)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
@@ -1713,7 +1722,7 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
             float4 nr4[KT], ni4[KT];
 #pragma unroll
             for (int k = 0; k < KT; ++k) {
-                const int rt = wave + 4 * k;
+                const int rt = wave + NW * k;
                 const size_t o = ((size_t)(rt * ntile) * 64 + lane) * 4;
                 nr4[k] = rt < ntile ? *reinterpret_cast<const float4 *>(Ur + o) : float4{0.f, 0.f, 0.f, 0.f};
                 ni4[k] = rt < ntile ? *reinterpret_cast<const float4 *>(Ui + o) : float4{0.f, 0.f, 0.f, 0.f};
@@ -1732,7 +1741,7 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
                 for (int k = 0; k < KT; ++k) {
                     cr4[k] = nr4[k];
                     ci4[k] = ni4[k];
-                    const int rt = wave + 4 * k;
+                    const int rt = wave + NW * k;
                     if (rt < ntile && g + 1 < ntile) {
                         const size_t o = ((size_t)(rt * ntile + g + 1) * 64 + lane) * 4;
                         nr4[k] = *reinterpret_cast<const float4 *>(Ur + o);
@@ -1741,7 +1750,7 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
                 }
 #pragma unroll
                 for (int k = 0; k < KT; ++k) {
-                    const int rt = wave + 4 * k;
+                    const int rt = wave + NW * k;
                     if (rt < ntile) {
                         const float4 ar4 = cr4[k], ai4 = ci4[k];
                         const float ar[4] = {ar4.x, ar4.y, ar4.z, ar4.w}, ai[4] = {ai4.x, ai4.y, ai4.z, ai4.w};
@@ -1788,7 +1797,7 @@ __global__ __launch_bounds__(256) void k_dense(const StepArgs *__restrict__ pa) 
         bool bad = false;
 #pragma unroll
         for (int k = 0; k < KT; ++k) {
-            const int rt = wave + 4 * k;
+            const int rt = wave + NW * k;
             if (rt < ntile)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -2269,7 +2278,7 @@ hipError_t launch_dense(const StepArgs &a, const StepArgs *da, int slots, hipStr
     const size_t lds = (size_t)kDenseLdsPlanes * a.dpad * 16 * sizeof(float) + 2 * 256 * sizeof(double);
 #define X(NT_)                                                                  \
     if (a.dpad == 16 * NT_) {                                                   \
-        hipLaunchKernelGGL(k_dense<NT_>, dim3(slots), dim3(256), lds, st, da); \
+        hipLaunchKernelGGL(k_dense<NT_>, dim3(slots), dim3(64 * dense_waves(NT_)), lds, st, da); \
         return hipGetLastError();                                               \
     }
     MSG_DENSE_VARIANTS(X)
@@ -2312,7 +2321,8 @@ int dense_occupancy(int dpad, size_t lds) {
     int nb = 0;
 #define X(NT_)                                                                                      \
     if (dpad == 16 * NT_)                                                                           \
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_dense<NT_>, 256, lds) == hipSuccess ? nb : 1;
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_dense<NT_>, 64 * dense_waves(NT_), lds) == hipSuccess \
+                   ? nb : 1;
     MSG_DENSE_VARIANTS(X)
 #undef X
     return 1;
