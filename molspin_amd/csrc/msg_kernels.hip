@@ -1590,9 +1590,9 @@ __device__ __forceinline__ float2 dense_init(const StepArgs &a, int u, int col, 
     return make_float2(st[row * 16 + tcol], st[a.dpad * 16 + row * 16 + tcol]);
 }
 
-// Waves per workgroup of the dense kernel: 8 for the largest tiles (four row tiles
+// Waves per workgroup of the dense kernel: 8 from 12 row tiles up (at most four row tiles
 // per wave keep the three products' accumulators in registers), else 4.
-constexpr int dense_waves(int NT) { return NT >= 24 ? 8 : 4; }
+constexpr int dense_waves(int NT) { return NT >= 12 ? 8 : 4; }
 
 // NT: row tiles of 16 (padded dimension Dp = 16 NT, a compile-time constant so the
 // product loop and the per-wave tile loops unroll); NW = dense_waves(NT) waves, each

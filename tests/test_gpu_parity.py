@@ -283,6 +283,24 @@ def test_legacy_stoch_yields_singlet_only_sink(ctx):
     assert np.abs(y[0] - yo).max() <= YIELD_TOL
 
 
+@pytest.mark.parametrize("index", [12, 0, 20])
+def test_dense_sweep_matches_oracle(ctx, index):
+    """Dense MFMA path (three-product complex GEMM) on uneven-batch pairs of D = 256,
+    128 and 512 (16, 8 and 32 row tiles: eight waves from 12 row tiles up, four
+    below), 4 fields over 0-40 ns, against the oracle."""
+    s, cfg = workloads.config4_pair(index)
+    p = s.pack()
+    sysd = ctx.upload(p)
+    assert sysd.dim == 4 << cfg["n"] and sysd.dim <= 512
+    fields = cfg["fields"][::16]
+    prm = abi.run_params(total_time=40.0, time_step=2.0, method=abi.MSG_METHOD_EXPONENTIAL)
+    y, _ = sysd.propagate(prm, fields=fields)
+    yo, _ = ref.run_hsgeneral(p, abi.run_params(total_time=40.0, time_step=2.0, method=abi.MSG_METHOD_KRYLOV),
+                              fields=fields, threads=8)
+    print("D", sysd.dim, "max |dPhi|", np.abs(y - yo).max())
+    assert np.abs(y - yo).max() <= YIELD_TOL
+
+
 def test_server_alone_equals_mixed(ctx):
     """Continuous batching: requests of different radical pairs (dense and
     matrix-free paths) admitted while others run give the same yields as each
