@@ -59,6 +59,7 @@ hipError_t launch_unc_combine(const double *CA, const double *CB, const double *
                               hipStream_t st);
 hipError_t launch_to_planes(const void *U, float *dst, int n, int Dp, size_t stride, int batch, hipStream_t st);
 int dense_occupancy(int dpad, size_t lds);
+int dense_threads(int dpad);
 }  // namespace msg
 
 using namespace msg;
@@ -1433,11 +1434,20 @@ msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_param
                     trial.slots = std::max(1, probe.slots / 2);
                 }
                 ctx->force = path == Path::Lanczos ? &trial : nullptr;
-                prepare(b, ctx, sys, &p, n_items, fields, columns, n_columns, false, false, ctx->stream);
-                ctx->force = nullptr;
-                b.timed = true;
-                advance(b, b.nsamples);
-                collect(b, y.data(), nullptr);
+                try {
+                    prepare(b, ctx, sys, &p, n_items, fields, columns, n_columns, false, false, ctx->stream);
+                    ctx->force = nullptr;
+                    b.timed = true;
+                    advance(b, b.nsamples);
+                    collect(b, y.data(), nullptr);
+                } catch (const DeviceError &) {
+                    // a split-column candidate whose cooperative launch the device refuses
+                    // (nothing was launched) drops out; any other failure is reported
+                    ctx->force = nullptr;
+                    if (!(path == Path::Lanczos && c.G == -3)) throw;
+                    ms_min = 1e300;
+                    break;
+                }
                 float ms = 0.f;
                 HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
                 ms_min = std::min(ms_min, (double)ms);
@@ -1448,7 +1458,7 @@ msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_param
                 best_ms = ms_min;
                 best_shape = c;
                 out.groups_per_thread = c.G;
-                out.threads = path == Path::Lanczos ? c.T : 256;
+                out.threads = path == Path::Lanczos ? c.T : dense_threads(dense_padding((int)r.D));
                 out.slots = slots_used;
             }
         }

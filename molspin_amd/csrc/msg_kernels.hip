@@ -2317,6 +2317,8 @@ int lanczos_occupancy(int group_per_thread, int threads, size_t lds, bool arnold
     return 1;
 }
 
+int dense_threads(int dpad) { return 64 * dense_waves(dpad / 16); }
+
 int dense_occupancy(int dpad, size_t lds) {
     int nb = 0;
 #define X(NT_)                                                                                      \
