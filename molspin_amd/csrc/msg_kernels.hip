@@ -819,8 +819,8 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
                     }
                     // modified Gram-Schmidt: h_ij = <v_i, w>, w -= h_ij v_i.  v_j comes from
                     // LDS, earlier basis vectors from the slot's global scratch; each v_i is
-                    // read once into registers, and v_{i+1} is requested before the
-                    // reduction of h_ij so that its latency overlaps the reduction.
+                    // read once into registers.  (Requesting v_{i+1} before the reduction of
+                    // h_ij spilled registers and measured 6 % slower.)
                     auto basis_group = [&](int i, int nu) {
                         const float4 *gA = reinterpret_cast<const float4 *>(bas + (size_t)i * 2 * D);
                         return i == j ? load_group(LA, LB, nu) : load_group(gA, gA + D4, nu);
@@ -837,21 +837,23 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
                                 h[0] += (double)vi[g].c[c].x * w[g].c[c].x + (double)vi[g].c[c].y * w[g].c[c].y;
                                 h[1] += (double)vi[g].c[c].x * w[g].c[c].y - (double)vi[g].c[c].y * w[g].c[c].x;
                             }
-                        C4 vn[G];
-                        if (i < j)
-#pragma unroll
-                            for (int g = 0; g < G; ++g) vn[g] = basis_group(i + 1, g * T + t);
                         block_sum<2>(h, SCR());
                         if (t == 0) s_h[i * kMaxKrylov + j] = make_double2(h[0], h[1]);
                         const float hr = (float)h[0], hi = (float)h[1];
 #pragma unroll
-                        for (int g = 0; g < G; ++g) {
+                        for (int g = 0; g < G; ++g)
 #pragma unroll
                             for (int c = 0; c < 4; ++c) {
                                 w[g].c[c].x -= hr * vi[g].c[c].x - hi * vi[g].c[c].y;
                                 w[g].c[c].y -= hr * vi[g].c[c].y + hi * vi[g].c[c].x;
                             }
-                            if (i < j) vi[g] = vn[g];
+                        if (i + 1 < j) {
+                            const float4 *nA = reinterpret_cast<const float4 *>(bas + (size_t)(i + 1) * 2 * D);
+#pragma unroll
+                            for (int g = 0; g < G; ++g) vi[g] = load_group(nA, nA + D4, g * T + t);
+                        } else if (i + 1 == j) {
+#pragma unroll
+                            for (int g = 0; g < G; ++g) vi[g] = load_group(LA, LB, g * T + t);
                         }
                     }
                     if (j == meff0 - 1) break;
