@@ -364,7 +364,7 @@ msg_status msg_symmetric_decay(msg_ctx *ctx, const msg_system *sys, int32_t n_it
  * n_columns <= D, MolSpin basis order, column-major).  FP64 on the device:
  *   - K = kappa 1 and D <= 4096 (method AUTO / KRYLOV): the eigenpairs of H give
  *     X_gh = rho0_gh / (2 kappa + i (lambda_g - lambda_h)) in the eigenbasis;
- *   - otherwise D <= 512 (and always with method EXPONENTIAL): X by doubling,
+ *   - otherwise D <= 2048 (and always with method EXPONENTIAL): X by doubling,
  *     X <- X + U X U^dagger, U <- U^2 from U = e^{G tau} and the Taylor-series
  *     integral over [0, tau], until the added integral over [T, 2T] is below 1e-10
  *     of X (Frobenius).

@@ -1622,9 +1622,9 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
         if (given && (!columns || n_columns < 1)) throw ArgError{MSG_INVALID_ARG, "sampling=given needs columns"};
         if (given && n_columns > D) throw ArgError{MSG_UNSUPPORTED, "time-infinity yields take at most D columns"};
         const bool eig = r.symmetric && D <= 4096 && p->method != MSG_METHOD_EXPONENTIAL;
-        if (!eig && D > 512)
+        if (!eig && D > 2048)
             throw ArgError{MSG_UNSUPPORTED, r.symmetric ? "time-infinity yields are limited to D <= 4096"
-                                                        : "time-infinity yields with unequal recombination need D <= 512"};
+                                                        : "time-infinity yields with unequal recombination need D <= 2048"};
         const char *diverges = "failed to solve the static Hilbert time-integral equation: part of the state never decays";
         if (r.symmetric && !(r.kappa > 0.0)) throw ArgError{MSG_NUMERIC_BREAKDOWN, diverges};
         hipStream_t st = ctx->stream;
@@ -1702,7 +1702,8 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
             HIPCHECK(hipStreamSynchronize(st));
         } else {
             // doubling on X = int_0^inf e^{Gt} rho0 e^{G^dagger t} dt, G = -iH - K (x) 1
-            const int chunk = std::max(1, std::min(n_items, (int)(((size_t)1536 << 20) / (10 * DD * sizeof(cd)))));
+            // ten D x D FP64 complex buffers per item of a chunk, at most ~12 GiB in all
+            const int chunk = std::max(1, std::min(n_items, (int)(((size_t)12 << 30) / (10 * DD * sizeof(cd)))));
             DBuf bG, bR, bX, bX2, bX3, bX4, bB, bP, bQ, bn, bt;
             cd *dG = bG.alloc<cd>((size_t)chunk * DD, st), *dR = bR.alloc<cd>((size_t)chunk * DD, st);
             cd *dX = bX.alloc<cd>((size_t)chunk * DD, st), *dX2 = bX2.alloc<cd>((size_t)chunk * DD, st);

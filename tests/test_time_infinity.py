@@ -213,11 +213,13 @@ def test_device_given_columns(ctx, method):
 
 
 @pytest.mark.gpu
-def test_device_d512_routes_agree_with_bartels_stewart(ctx):
-    """D = 512 (7 nuclei, equal sinks): the eigen route and the doubling route agree,
-    and both equal scipy's Bartels-Stewart solution on the oracle's H, K and rho0."""
+@pytest.mark.parametrize("n", [7, 8])
+def test_device_routes_agree_with_bartels_stewart(ctx, n):
+    """D = 512 and 1024 (7 and 8 nuclei, equal sinks): the eigen route and the
+    doubling route agree, and both equal scipy's Bartels-Stewart solution on the
+    oracle's H, K and rho0."""
     import scipy.linalg as sl
-    idx = next(i for i in range(1000) if workloads.config4_pair(i)[1]["n"] == 7)
+    idx = next(i for i in range(1000) if workloads.config4_pair(i)[1]["n"] == n)
     s, cfg = workloads.config4_pair(idx)
     p = s.pack()
     sysd = ctx.upload(p)
@@ -230,7 +232,7 @@ def test_device_d512_routes_agree_with_bartels_stewart(ctx):
     X = sl.solve_continuous_lyapunov(-1j * H - K, -(B @ B.conj().T))
     yo = np.array([p.desc.transitions[t].rate * np.trace(ref.projector(p, p.desc.transitions[t].source_state) @ X).real
                    for t in ref.packed_transitions(p)])
-    print("D=512 eigen", ye, "doubling", yd, "scipy", yo)
+    print("D", sysd.dim, "eigen", ye, "doubling", yd, "scipy", yo)
     assert np.abs(ye - yo).max() <= TI_TOL
     assert np.abs(yd - yo).max() <= TI_TOL
 
