@@ -79,6 +79,9 @@ struct msg_ctx {
     };
     std::map<int64_t, Shape> tuned;
     const Shape *force = nullptr;  // candidate under trial inside msg_autotune
+    // pinned host staging of given columns (grown on demand, freed with the context)
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
 };
 
 struct msg_system {
@@ -479,7 +482,15 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     // initial states
     if (p->sampling == MSG_SAMPLING_GIVEN) {
         a.init_mode = 1;
-        std::vector<float> g((size_t)n_columns * 2 * D);
+        const size_t gbytes = (size_t)n_columns * 2 * D * sizeof(float);
+        if (ctx->pinned_bytes < gbytes) {  // page-locked, so the copy runs at DMA speed
+            if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+            ctx->pinned = nullptr;
+            ctx->pinned_bytes = 0;
+            HIPCHECK(hipHostMalloc(&ctx->pinned, gbytes, hipHostMallocDefault));
+            ctx->pinned_bytes = gbytes;
+        }
+        float *g = static_cast<float *>(ctx->pinned);
         // MolSpin order (complex128) -> device planes (complex64); columns split over host threads
         auto convert = [&](int j0, int j1) {
             for (int j = j0; j < j1; ++j)
@@ -502,8 +513,9 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                 pool.emplace_back(convert, (int)((int64_t)n_columns * k / nth), (int)((int64_t)n_columns * (k + 1) / nth));
             for (auto &th : pool) th.join();
         }
-        float *dg = b.given.alloc<float>(g.size(), stream);
-        HIPCHECK(hipMemcpyAsync(dg, g.data(), g.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        float *dg = b.given.alloc<float>((size_t)n_columns * 2 * D, stream);
+        HIPCHECK(hipMemcpyAsync(dg, g, gbytes, hipMemcpyHostToDevice, stream));
+        HIPCHECK(hipStreamSynchronize(stream));  // the staging buffer is reused by the next call
         a.given = dg;
     } else {
         a.init_mode = 0;
@@ -1072,6 +1084,7 @@ msg_status msg_ctx_destroy(msg_ctx *ctx) {
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     delete ctx;
     return MSG_OK;
 }
