@@ -187,12 +187,27 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
 // bits, local index g 2^LOGT + t) is in LDS (LA, LB); nuclei on bits from SB up
 // (a column split over several workgroups, base = slab 2^SB) read their neighbours
 // from the whole vector in global memory (GA, GB).
-template <int NN, int G, int LOGT>
+template <int NN, int G, int LOGT, bool TBL = false>
 __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *__restrict__ co,
                                                const float *__restrict__ hd, int t, const C4 (&x)[G],
                                                const float4 *LA, const float4 *LB, C4 (&y)[G], int base = 0,
-                                               const float4 *GA = nullptr, const float4 *GB = nullptr) {
+                                               const float4 *GA = nullptr, const float4 *GB = nullptr,
+                                               const float4 *blk = nullptr) {
     constexpr int SB = LOGT + Log2<G>::value;
+    // the thread's coupling block of nucleus n on electron e; with TBL from the LDS table of
+    // both orientations, blk[((n 2 + e) 2 + down) 2 + 0..1], which replaces a select per entry
+    auto block = [&](int n, int e, bool down, const float *cb, float2 m[4]) {
+        if constexpr (TBL) {
+            const float4 *b = blk + ((n * 2 + e) * 2 + (down ? 1 : 0)) * 2;
+            const float4 u = b[0], v = b[1];
+            m[0] = make_float2(u.x, u.y);
+            m[1] = make_float2(u.z, u.w);
+            m[2] = make_float2(v.x, v.y);
+            m[3] = make_float2(v.z, v.w);
+        } else {
+            load_block(cb, down, m);
+        }
+    };
     float f1x = hd[0], f1y = hd[1], f1z = hd[2];
     float f2x = hd[3], f2y = hd[4], f2z = hd[5];
     float rr = hd[6];
@@ -218,13 +233,13 @@ __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *_
             for (int g = 0; g < G; ++g) nb[g] = load_group(LA, LB, (g << LOGT) + (t ^ (1 << n)));
             if (fl & NUC_E1) {
                 f1x = fmaf(s, cn[0], f1x); f1y = fmaf(s, cn[1], f1y); f1z = fmaf(s, cn[2], f1z);
-                load_block(cn + 7, down, m);
+                block(n, 0, down, cn + 7, m);
 #pragma unroll
                 for (int g = 0; g < G; ++g) apply_e1(m, nb[g], y[g]);
             }
             if (fl & NUC_E2) {
                 f2x = fmaf(s, cn[3], f2x); f2y = fmaf(s, cn[4], f2y); f2z = fmaf(s, cn[5], f2z);
-                load_block(cn + 15, down, m);
+                block(n, 1, down, cn + 15, m);
 #pragma unroll
                 for (int g = 0; g < G; ++g) apply_e2(m, nb[g], y[g]);
             }
@@ -486,6 +501,7 @@ __global__ __launch_bounds__(TMAX, G >= 4 ? 2 : 4) void k_lanczos(const StepArgs
     double *scr = reinterpret_cast<double *>(smem + (DBUF ? 4 : 2) * ng);  // 2 x 16 x 16 doubles
     __shared__ double ys[kMaxObs], yc[kMaxObs];                // Kahan yield sums (thread 0)
     __shared__ float2 s_coef[kMaxKrylov];                        // exp(-i dt T) e1, scaled
+    __shared__ float4 s_blk[kMaxNuclei * 8];                     // coupling blocks, both orientations
     __shared__ int s_unit;
     const int D = a.dim;
     const int D4 = D >> 2;
@@ -502,6 +518,16 @@ __global__ __launch_bounds__(TMAX, G >= 4 ? 2 : 4) void k_lanczos(const StepArgs
         if (u >= a.n_units) break;
         const int item = u / a.ncols, col = u - item * a.ncols;
         const float *co = a.coef + (size_t)item * a.coef_stride;
+        // the item's coupling blocks for both orientations of each nucleus (the operator
+        // then reads the one of each lane's index bit instead of selecting)
+        if (t < a.nnuc * 4) {
+            const int n = t >> 2, e = (t >> 1) & 1;
+            float2 m[4];
+            load_block(co + kCoefHead + n * kNucStride + (e ? 15 : 7), (t & 1) != 0, m);
+            s_blk[2 * t] = make_float4(m[0].x, m[0].y, m[1].x, m[1].y);
+            s_blk[2 * t + 1] = make_float4(m[2].x, m[2].y, m[3].x, m[3].y);
+        }
+        __syncthreads();
 
         // ---- initial state of this column
         C4 psi[G];
@@ -588,7 +614,8 @@ __global__ __launch_bounds__(TMAX, G >= 4 ? 2 : 4) void k_lanczos(const StepArgs
                     double red[2] = {0.0, 0.0};
 #pragma unroll
                     for (int g = 0; g < G; ++g) vjs[g] = load_group(cA, cB, g * T + t);
-                    apply_h_groups<Log2<ng>::value, G, Log2<T>::value>(a, co, hd, t, vjs, cA, cB, w);
+                    apply_h_groups<Log2<ng>::value, G, Log2<T>::value, true>(a, co, hd, t, vjs, cA, cB, w, 0,
+                                                                             nullptr, nullptr, s_blk);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
                         const int nu = g * T + t;
