@@ -106,6 +106,29 @@ __device__ __forceinline__ void load_block(const float *p, bool herm, float2 m[4
     m[3] = herm ? conjf2(d) : d;
 }
 
+// Table in LDS of every nucleus's coupling blocks in both orientations:
+// blk[((n 2 + e) 2 + down) 2 + 0..1] holds the block of nucleus n on electron e (e1, e2),
+// conjugate-transposed for down = 1.  A lane then reads the block of its own index bit
+// instead of selecting it entry by entry.  Filled by the first 4 nnuc threads; the caller
+// synchronises the workgroup before the first read.
+__device__ __forceinline__ void fill_block_table(const StepArgs &a, const float *co, int t, float4 *blk) {
+    if (t < a.nnuc * 4) {
+        const int n = t >> 2, e = (t >> 1) & 1;
+        float2 m[4];
+        load_block(co + kCoefHead + n * kNucStride + (e ? 15 : 7), (t & 1) != 0, m);
+        blk[2 * t] = make_float4(m[0].x, m[0].y, m[1].x, m[1].y);
+        blk[2 * t + 1] = make_float4(m[2].x, m[2].y, m[3].x, m[3].y);
+    }
+}
+__device__ __forceinline__ void table_block(const float4 *blk, int n, int e, bool down, float2 m[4]) {
+    const float4 *b = blk + ((n * 2 + e) * 2 + (down ? 1 : 0)) * 2;
+    const float4 u = b[0], v = b[1];
+    m[0] = make_float2(u.x, u.y);
+    m[1] = make_float2(u.z, u.w);
+    m[2] = make_float2(v.x, v.y);
+    m[3] = make_float2(v.z, v.w);
+}
+
 // H x for the group nu whose value is x; neighbours come from the LDS planes.
 // H = w1.S1 + w2.S2 + shift + [E] + sum_n (diagonal s_n(...) + flip-flop blocks);
 // w1, w2 and the shift come from the head `hd` (the item's block, or the interval's
@@ -119,10 +142,12 @@ static_assert(kTileG == kTileGroups, "tile size shared with the host");
 // NN < 0: a.nnuc at run time.  TILED: neighbours across index bits below
 // kTileBits come from the LDS tile (LA/LB indexed by nu mod kTileG), the others
 // from the vector in global memory (GA/GB).
-template <int NN = -1, bool TILED = false>
+// TBL: the coupling blocks come from the LDS table `blk` (fill_block_table).
+template <int NN = -1, bool TILED = false, bool TBL = false>
 __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict__ co, const float *__restrict__ hd,
                                       int nu, const C4 &x, const float4 *LA, const float4 *LB,
-                                      const float4 *GA = nullptr, const float4 *GB = nullptr) {
+                                      const float4 *GA = nullptr, const float4 *GB = nullptr,
+                                      const float4 *blk = nullptr) {
     float f1x = hd[0], f1y = hd[1], f1z = hd[2];
     float f2x = hd[3], f2y = hd[4], f2z = hd[5];
     float rr = hd[6];
@@ -140,12 +165,14 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
         float2 m[4];
         if (fl & NUC_E1) {
             f1x = fmaf(s, cn[0], f1x); f1y = fmaf(s, cn[1], f1y); f1z = fmaf(s, cn[2], f1z);
-            load_block(cn + 7, down, m);
+            if constexpr (TBL) table_block(blk, n, 0, down, m);
+            else load_block(cn + 7, down, m);
             apply_e1(m, nb, y);
         }
         if (fl & NUC_E2) {
             f2x = fmaf(s, cn[3], f2x); f2y = fmaf(s, cn[4], f2y); f2z = fmaf(s, cn[5], f2z);
-            load_block(cn + 15, down, m);
+            if constexpr (TBL) table_block(blk, n, 1, down, m);
+            else load_block(cn + 15, down, m);
             apply_e2(m, nb, y);
         }
         if (fl & NUC_T) {
@@ -194,19 +221,10 @@ __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *_
                                                const float4 *GA = nullptr, const float4 *GB = nullptr,
                                                const float4 *blk = nullptr) {
     constexpr int SB = LOGT + Log2<G>::value;
-    // the thread's coupling block of nucleus n on electron e; with TBL from the LDS table of
-    // both orientations, blk[((n 2 + e) 2 + down) 2 + 0..1], which replaces a select per entry
+    // the thread's coupling block of nucleus n on electron e (TBL: from the LDS table)
     auto block = [&](int n, int e, bool down, const float *cb, float2 m[4]) {
-        if constexpr (TBL) {
-            const float4 *b = blk + ((n * 2 + e) * 2 + (down ? 1 : 0)) * 2;
-            const float4 u = b[0], v = b[1];
-            m[0] = make_float2(u.x, u.y);
-            m[1] = make_float2(u.z, u.w);
-            m[2] = make_float2(v.x, v.y);
-            m[3] = make_float2(v.z, v.w);
-        } else {
-            load_block(cb, down, m);
-        }
+        if constexpr (TBL) table_block(blk, n, e, down, m);
+        else load_block(cb, down, m);
     };
     float f1x = hd[0], f1y = hd[1], f1z = hd[2];
     float f2x = hd[3], f2y = hd[4], f2z = hd[5];
@@ -518,15 +536,7 @@ __global__ __launch_bounds__(TMAX, G >= 4 ? 2 : 4) void k_lanczos(const StepArgs
         if (u >= a.n_units) break;
         const int item = u / a.ncols, col = u - item * a.ncols;
         const float *co = a.coef + (size_t)item * a.coef_stride;
-        // the item's coupling blocks for both orientations of each nucleus (the operator
-        // then reads the one of each lane's index bit instead of selecting)
-        if (t < a.nnuc * 4) {
-            const int n = t >> 2, e = (t >> 1) & 1;
-            float2 m[4];
-            load_block(co + kCoefHead + n * kNucStride + (e ? 15 : 7), (t & 1) != 0, m);
-            s_blk[2 * t] = make_float4(m[0].x, m[0].y, m[1].x, m[1].y);
-            s_blk[2 * t + 1] = make_float4(m[2].x, m[2].y, m[3].x, m[3].y);
-        }
+        fill_block_table(a, co, t, s_blk);  // read after the first barrier below
         __syncthreads();
 
         // ---- initial state of this column
@@ -1017,6 +1027,7 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
     __shared__ double scr[2 * 17 * 16];  // block_sum of up to 17 values, double-buffered
     __shared__ double ys[kMaxObs], yc[kMaxObs];
     __shared__ float2 s_coef[kMaxKrylov];
+    __shared__ float4 s_blk[kMaxNuclei * 8];  // coupling blocks, both orientations
     __shared__ int s_unit;
     constexpr int T = 1024;  // launched with exactly 1024 threads
     const int t = threadIdx.x, lane = t & 63;
@@ -1037,6 +1048,7 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
         if (u >= a.n_units) break;
         const int item = u / a.ncols, col = u - item * a.ncols;
         const float *co = a.coef + (size_t)item * a.coef_stride;
+        fill_block_table(a, co, t, s_blk);  // read after the workgroup's next barrier
         for (int nu = t; nu < ng; nu += T) {
             C4 x;
             if (a.init_mode == 0) {
@@ -1138,7 +1150,7 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
                                 for (int k = t; k < kTileG; k += T) {
                                     const int nu = tb + k;
                                     const C4 vj = load_group(tA, tB, k);
-                                    pass_a(nu, vj, apply_h<-1, true>(a, co, hd, nu, vj, tA, tB, vA, vB), nullptr);
+                                    pass_a(nu, vj, apply_h<-1, true, true>(a, co, hd, nu, vj, tA, tB, vA, vB, s_blk), nullptr);
                                 }
                             }
                         } else if (LV) {
@@ -1146,12 +1158,12 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
                             for (int g = 0; g < (LV ? 4 : 1); ++g) {
                                 const int nu = g * T + t;
                                 const C4 vj = load_group(vA, vB, nu);
-                                pass_a(nu, vj, apply_h(a, co, hd, nu, vj, vA, vB), &wr[g]);
+                                pass_a(nu, vj, apply_h<-1, false, true>(a, co, hd, nu, vj, vA, vB, nullptr, nullptr, s_blk), &wr[g]);
                             }
                         } else {
                             for (int nu = t; nu < ng; nu += T) {
                                 const C4 vj = load_group(vA, vB, nu);
-                                pass_a(nu, vj, apply_h(a, co, hd, nu, vj, vA, vB), nullptr);
+                                pass_a(nu, vj, apply_h<-1, false, true>(a, co, hd, nu, vj, vA, vB, nullptr, nullptr, s_blk), nullptr);
                             }
                         }
                         block_sum<2>(red, SCR());
@@ -1489,6 +1501,7 @@ __global__ __launch_bounds__(T, 4) void k_lanczos_split(const StepArgs *__restri
                     double red[2] = {0.0, 0.0};
 #pragma unroll
                     for (int g = 0; g < G; ++g) vjs[g] = load_group(cA, cB, g * T + t);
+                    // lane-selected blocks: the LDS block table measured 8 % slower in this kernel
                     apply_h_groups<NN, G, Log2<T>::value>(a, co, hd, t, vjs, cA, cB, w, base, gA, gA + D4);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
