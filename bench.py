@@ -159,7 +159,7 @@ def lanczos_roofline(D, m, state_steps, kernel_s, traffic):
 # profiles/k_dense_field_sweep_D512.md and profiles/k_arnoldi_singlet_sink_D4096.md.
 MEASURED_TRAFFIC_PER_STATE_STEP = {"orientations": 1.069e+06, "dense": 2.47e+04, "pair/-1": 6.60e+06, "arnoldi": 5.02e+06,
                                    # D = 65536 by launch shape: tiled streaming (-2), split columns (-3)
-                                   "trajectories/-2": 7.523e+07, "trajectories/-3": 3.154e+07}
+                                   "trajectories/-2": 7.54e+07, "trajectories/-3": 3.154e+07}
 
 
 def orientation_batch(ctx, args):

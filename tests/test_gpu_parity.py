@@ -204,12 +204,12 @@ def test_large_dimension_streaming_path(ctx, shape, monkeypatch):
     assert np.abs(y - yo).max() <= YIELD_TOL
 
 
-@pytest.mark.parametrize("shape", [None, "-3"])
+@pytest.mark.parametrize("shape", [None, "-2", "-3"])
 def test_config3_trajectories(ctx, shape, monkeypatch):
     """Config 3 (D = 65536, 14 isotropic nuclei, ST dephasing 1e-4 /ns): 4 spin-
     coherent trajectories drawn by the backend's sampler, 0-30 ns, against the
-    oracle on the same columns and the same counter-based flips; the default shape
-    and the column split over eight workgroups."""
+    oracle on the same columns and the same counter-based flips; the default shape,
+    the tiled streaming form and the column split over eight workgroups."""
     force_shape(monkeypatch, shape)
     s, cfg = workloads.config3(n_traj=4)
     p = s.pack()
