@@ -399,7 +399,8 @@ typedef struct msg_launch_shape {
  * column split over several workgroups), and the number of resident workgroups --
  * on a sample of `params` (n_items items
  * of `fields`/`columns` as for msg_propagate, `sample_steps` time steps) and
- * keeps the fastest for this context and Hilbert dimension: later
+ * keeps the fastest for this context and Hilbert dimension (a candidate tried later
+ * must be more than 1.5 % faster than the best earlier one to replace it): later
  * msg_propagate, msg_step and server launches on `ctx` use it.  Results do not
  * depend on the shape beyond FP32 rounding.  `best` may be NULL.
  */

@@ -1383,6 +1383,9 @@ msg_status msg_propagate(msg_ctx *ctx, const msg_system *sys, const msg_run_para
     });
 }
 
+// Relative margin by which a later autotune candidate must beat the best earlier one.
+constexpr double kTuneMargin = 0.015;
+
 msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_params *params, int32_t n_items,
                         const double *fields, const double *columns, int32_t n_columns, int32_t sample_steps,
                         msg_launch_shape *best) {
@@ -1467,7 +1470,10 @@ msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_param
                 slots_used = b.slots;
                 if (path == Path::Lanczos && c.slots == -2) c.slots = trial.slots;
             }
-            if (ms_min < best_ms) {
+            // a later candidate has to beat the earlier ones by more than the trials' noise
+            // (about 1 %): a short trial can rank shapes within that margin wrongly for the
+            // full launch, and the candidates are listed with the usual winner first
+            if (ms_min < best_ms * (1.0 - kTuneMargin)) {
                 best_ms = ms_min;
                 best_shape = c;
                 out.groups_per_thread = c.G;
