@@ -23,7 +23,11 @@ Other workloads (`--workload`), same JSON line format:
   arnoldi       the orientation batch with singlet-only recombination (unequal rates:
                 matrix-free Arnoldi on -iH - K);
   timeinf       time-infinity yields (method=timeinf) of a D = 512 pair with unequal
-                sinks over 64 field strengths (propagator doubling, FP64).
+                sinks over 64 field strengths (propagator doubling, FP64);
+  uneven        config 4's uneven batch: the first `--pairs` radical pairs (nuclear
+                counts as drawn, D = 16 ... 16384) x `--pair-fields` log-spaced field
+                strengths x all direct columns in one persistent launch
+                (msg_propagate_uneven), against the same pairs one at a time.
 """
 import argparse
 import json
@@ -38,6 +42,8 @@ if ROOT not in sys.path:
 
 HBM_PEAK = 8.0e12      # MI355X, bytes/s
 MFMA_F32_PEAK = 157.3e12  # MI355X dense FP32 matrix, flop/s
+VALU_F32_PEAK = 157.3e12  # MI355X FP32 vector (published; counts packed FP32)
+VALU_F32_UNPACKED = 78.6e12  # FP32 vector FMA rate without packed FP32 (256 CUs x 64 lanes x 2 x 2.4 GHz)
 METRIC = "propagated state-steps/s + p99 yield-request latency at |dPhiS|<=1e-5 vs FP64"
 
 
@@ -47,11 +53,13 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["orientations", "trajectories", "dense", "pair", "uncoupled", "rydmr", "arnoldi",
-                             "timeinf"],
+                             "timeinf", "uneven"],
                     default="orientations")
     ap.add_argument("--orientations", type=int, default=32)
     ap.add_argument("--trajectories", type=int, default=512)
     ap.add_argument("--fields", type=int, default=2, help="field points of the pair workload")
+    ap.add_argument("--pairs", type=int, default=24, help="radical pairs of the uneven workload")
+    ap.add_argument("--pair-fields", type=int, default=8, help="field points per pair of the uneven workload")
     ap.add_argument("--intervals", type=int, default=None, help="time steps per bench step")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -63,12 +71,25 @@ def main():
         print("bench.py runs on one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    from molspin_amd import backend
+
+    ctx = backend.Context(0)
+    try:
+        line = run_bench(ctx, args)
+    finally:
+        ctx.close()  # release the library's device state before interpreter teardown
+    print(json.dumps(line), flush=True)
+
+
+def load_fixture(name):
+    with open(os.path.join(ROOT, "tests", "golden", name + ".json")) as f:
+        return json.load(f)
+
+
+def run_bench(ctx, args):
     import numpy as np
     import torch
 
-    from molspin_amd import abi, backend, workloads
-
-    ctx = backend.Context(0)
     if args.workload == "orientations":
         run = orientation_batch(ctx, args)
     elif args.workload == "trajectories":
@@ -81,6 +102,8 @@ def main():
         run = variant_batch(ctx, args)
     elif args.workload == "timeinf":
         run = timeinf_batch(ctx, args)
+    elif args.workload == "uneven":
+        run = uneven_batch(ctx, args)
     else:
         run = dense_sweep(ctx, args)
 
@@ -91,10 +114,10 @@ def main():
         run["step"](i)
     torch.cuda.synchronize() if torch.cuda.is_available() else None
     progress("%d warmup steps done" % args.warmup)
-    kms = []
+    kms, outs = [], {}
     t0 = time.perf_counter()
     for i in range(args.steps):
-        run["step"](args.warmup + i)
+        outs[args.warmup + i] = run["step"](args.warmup + i)
         kms.append(ctx.last_kernel_ms())
     torch.cuda.synchronize() if torch.cuda.is_available() else None
     wall = time.perf_counter() - t0
@@ -102,6 +125,11 @@ def main():
     kernel_s = float(np.mean(kms)) * 1e-3
     roof = run["roofline"](kernel_s)
     progress("%d timed steps done in %.2f s" % (args.steps, wall))
+    # the error against the FP64 oracle: the timed launches' own results where a
+    # committed fixture covers them, and a post-timing run of the fixture's set
+    parity = run["parity"](outs) if run.get("parity") else None
+    progress("parity check done")
+    compare = run["compare"]() if run.get("compare") else None
 
     cpu = None
     if not args.no_cpu and run.get("packed") is not None:
@@ -116,7 +144,7 @@ def main():
                                             compare_plain=False)
         progress("serving stream through the uncoupled method done")
 
-    line = {
+    return {
         "metric": METRIC,
         "value": value,
         "unit": run.get("unit", "state-steps/s"),
@@ -131,35 +159,79 @@ def main():
         "data": run["data"],
         "config": run["config"],
         "roofline": roof,
+        "parity": parity,
+        "compare": compare,
         "cpu_baseline": cpu,
         "serving": serving,
+        "device": torch.cuda.get_device_name(0) if torch.cuda.is_available() else None,
     }
-    print(json.dumps(line), flush=True)
 
 
-def lanczos_roofline(D, m, state_steps, kernel_s, traffic):
-    """Matrix-free Lanczos: algorithmic HBM bytes per state-step 8 D (4m + 1)
-    (m iterations reading v_j, v_{j-1} and writing v_{j+1}, plus the m + 1
-    vectors of the final combination, complex FP32)."""
+def lanczos_roofline(D, m, state_steps, kernel_s, traffic, traffic_source=None):
+    """Matrix-free Lanczos at D >= 16384: algorithmic HBM bytes per state-step
+    8 D (4m + 1) (m iterations reading v_j, v_{j-1} and writing v_{j+1}, plus the
+    m + 1 vectors of the final combination, complex FP32)."""
     bytes_per_ss = 8.0 * D * (4 * m + 1)
     achieved = bytes_per_ss * state_steps / kernel_s
     return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": traffic,
+            "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_source,
             "kernel_ms": kernel_s * 1e3, "bytes_per_state_step": bytes_per_ss,
             "state_steps_per_launch": state_steps}
+
+
+def operator_flops_per_state_step(D, m, n_couplings, e_block=False):
+    """FP32 flops of one state-step of the matrix-free Lanczos kernels, counted per
+    electron-pair group (4 amplitudes, D / 4 groups) from the kernel's arithmetic:
+    per matvec 32 FMAs per electron-nucleus coupling (a 2x2 complex block on one
+    electron of the flipped neighbour group), 8 for the scalar shift, 24 per electron
+    field, 64 for a general 4x4 electron block; per iteration 20 more for the
+    three-term recurrence (w -= beta v_{j-1}, w -= alpha v_j, 1/beta scaling); and
+    16 m FMAs for the final combination psi = sum_j c_j v_j.  An FMA is 2 flops."""
+    fma_matvec = 32 * n_couplings + 8 + 48 + (64 if e_block else 0)
+    fma_group = m * (fma_matvec + 20) + 16 * m
+    return 2.0 * fma_group * (D // 4)
+
+
+def valu_roofline(D, m, n_couplings, state_steps, kernel_s, hbm_traffic, traffic_source, valu_issue):
+    """D <= 8192: the Krylov basis of a column stays in LDS / L2 and the operator's
+    FP32 VALU work binds.  `achieved` counts the kernel's FP32 flops
+    (operator_flops_per_state_step) against MI355X's published FP32 vector peak
+    (157.3 TFLOP/s, which assumes packed FP32); this build compiles with packed FP32
+    disabled, so 78.6 TFLOP/s is its attainable ceiling (`frac_of_unpacked`).  The
+    byte model is reported beside it (`hbm_model`)."""
+    fl = operator_flops_per_state_step(D, m, n_couplings)
+    achieved = fl * state_steps / kernel_s
+    hbm = lanczos_roofline(D, m, state_steps, kernel_s, hbm_traffic, traffic_source)
+    return {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_F32_PEAK / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / VALU_F32_PEAK, "frac_of_unpacked": achieved / VALU_F32_UNPACKED,
+            "traffic": hbm_traffic, "traffic_source": traffic_source,
+            "kernel_ms": kernel_s * 1e3, "flops_per_state_step": fl, "state_steps_per_launch": state_steps,
+            "valu_issue_profiled": valu_issue,
+            "hbm_model": {"achieved": hbm["achieved"], "unit": "GB/s", "frac": hbm["frac"],
+                          "bytes_per_state_step": hbm["bytes_per_state_step"]}}
 
 
 # HBM bytes per state-step of the dominant kernel, from rocprofv3 FETCH_SIZE /
 # WRITE_SIZE runs of each workload at a smaller launch (calibrated against a
 # device copy: the FETCH_SIZE count is doubled, WRITE_SIZE taken as is); the
-# traffic of a launch scales with its state-steps at fixed D.  The counter runs
-# are summarised in profiles/k_lanczos_orientation_batch_D4096.md,
-# profiles/k_lanczos_big_trajectories_D65536.md, profiles/k_lanczos_split_trajectories_D65536.md,
-# profiles/k_lanczos_big_pair_D16384.md,
-# profiles/k_dense_field_sweep_D512.md and profiles/k_arnoldi_singlet_sink_D4096.md.
-MEASURED_TRAFFIC_PER_STATE_STEP = {"orientations": 1.069e+06, "dense": 2.47e+04, "pair/-1": 6.60e+06, "arnoldi": 5.02e+06,
-                                   # D = 65536 by launch shape: tiled streaming (-2), split columns (-3)
-                                   "trajectories/-2": 7.54e+07, "trajectories/-3": 3.154e+07}
+# traffic of a launch scales with its state-steps at fixed D.  Each entry names the
+# profile summary it comes from; a kernel change needs a new counter run.
+MEASURED_TRAFFIC_PER_STATE_STEP = {
+    "orientations": (1.069e+06, "profiles/k_lanczos_orientation_batch_D4096.md"),
+    "dense": (2.47e+04, "profiles/k_dense_field_sweep_D512.md"),
+    "pair/-1": (6.60e+06, "profiles/k_lanczos_big_pair_D16384.md"),
+    "arnoldi": (5.02e+06, "profiles/k_arnoldi_singlet_sink_D4096.md"),
+    # D = 65536 by launch shape: tiled streaming (-2), split columns (-3)
+    "trajectories/-2": (7.54e+07, "profiles/k_lanczos_big_trajectories_D65536.md"),
+    "trajectories/-3": (3.154e+07, "profiles/k_lanczos_split_trajectories_D65536.md"),
+}
+# VALU issue fraction (SQ_ACTIVE_INST_VALU / SQ_BUSY_CYCLES per SIMD) of the profiled kernels
+PROFILED_VALU_ISSUE = {"orientations": (0.78, "profiles/k_lanczos_orientation_batch_D4096.md")}
+
+
+def traffic_for(key, state_steps):
+    t = MEASURED_TRAFFIC_PER_STATE_STEP.get(key)
+    return (None, None) if t is None else (t[0] * state_steps, t[1] + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)")
 
 
 def orientation_batch(ctx, args):
@@ -175,15 +247,45 @@ def orientation_batch(ctx, args):
     state_steps = B * ncols * intervals
     shape = sysd.autotune(prm, fields=cfg["fields"][:B], sample_steps=5)
 
+    def orients(i):
+        return [(i * B + j) % 1024 for j in range(B)]
+
     def step(i):
-        first = (i * B) % 1024
-        sysd.propagate(prm, fields=[cfg["fields"][(first + j) % 1024] for j in range(B)])
+        y, _ = sysd.propagate(prm, fields=[cfg["fields"][o] for o in orients(i)])
+        return y
 
     def roof(kernel_s):
-        tr = MEASURED_TRAFFIC_PER_STATE_STEP.get("orientations")
-        return lanczos_roofline(D, m, state_steps, kernel_s, None if tr is None else tr * state_steps)
+        tr, src = traffic_for("orientations", state_steps)
+        issue = PROFILED_VALU_ISSUE["orientations"]
+        return valu_roofline(D, m, 10, state_steps, kernel_s, tr, src,
+                             {"value": issue[0], "source": issue[1]})
 
-    return dict(step=step, state_steps=state_steps, roofline=roof, packed=packed,
+    def parity(outs):
+        import numpy as np
+        fx = load_fixture("cfg2_bench")
+        fy, ftr = np.array(fx["yields"]), np.array(fx["trace"])
+        res = {"fixture": "tests/golden/cfg2_bench.json (FP64 oracle: Arnoldi m=16, CSR H)",
+               "tolerance": {"yield_abs": 1e-5, "trace_max_rel": 1e-4}}
+        # orientations of the timed launches that the fixture covers (same horizon)
+        timed = []
+        if intervals * dt == fx["total_time"]:
+            for i, y in outs.items():
+                for j, o in enumerate(orients(i)):
+                    if o in fx["orientations"]:
+                        timed.append(float(np.abs(y[j] - fy[fx["orientations"].index(o)]).max()))
+        res["timed_launches"] = {"orientations_checked": len(timed),
+                                 "max_abs_dPhi": max(timed) if timed else None}
+        # the fixture's 8 orientations in one launch, yields and traces
+        prm_f = abi.run_params(total_time=fx["total_time"], time_step=fx["time_step"], method=abi.MSG_METHOD_KRYLOV,
+                               krylov_size=fx["krylov_size"])
+        y, tr = sysd.propagate(prm_f, fields=[cfg["fields"][o] for o in fx["orientations"]], trace=True)
+        terr = max(float(np.abs(tr[k] - ftr[k]).max() / np.abs(ftr[k]).max()) for k in range(len(ftr)))
+        res["fixture_launch"] = {"orientations": len(fx["orientations"]), "max_abs_dPhi": float(np.abs(y - fy).max()),
+                                 "trace_max_rel": terr}
+        res["max_abs_dPhi"] = max([res["fixture_launch"]["max_abs_dPhi"]] + timed)
+        return res
+
+    return dict(step=step, state_steps=state_steps, roofline=roof, parity=parity, packed=packed,
                 cpu_prm=abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_KRYLOV),
                 cpu_field=cfg["fields"][0], cpu_columns=None,
                 data="synthetic (seeded radical pair, Fibonacci field directions)",
@@ -191,6 +293,9 @@ def orientation_batch(ctx, args):
                                     "%d orientations x %d columns x %d steps of 2 ns per step, Lanczos m=%d"
                                     % (B, ncols, intervals, m),
                         "orientations_per_step": B, "columns": ncols, "intervals": intervals,
+                        "full_config": "1024 orientations x 1024 columns x 1500 steps (1.57e9 state-steps); "
+                                       "each bench step is a 32-orientation, 50-step slice with the same "
+                                       "per-state-step work",
                         "parallelism": "single GPU",
                         "launch_shape": {"groups_per_thread": shape.groups_per_thread, "threads": shape.threads,
                                          "slots": shape.slots, "chosen_by": "msg_autotune"}})
@@ -211,15 +316,29 @@ def trajectory_batch(ctx, args):
     shape = sysd.autotune(prm, columns=cols[:, :min(M, 256)], sample_steps=2)
 
     def step(i):
-        sysd.propagate(prm, columns=cols)
+        return sysd.propagate(prm, columns=cols)[0]
 
     def roof(kernel_s):
-        tr = MEASURED_TRAFFIC_PER_STATE_STEP.get("trajectories/%d" % shape.groups_per_thread)
-        return lanczos_roofline(D, m, state_steps, kernel_s, None if tr is None else tr * state_steps)
+        tr, src = traffic_for("trajectories/%d" % shape.groups_per_thread, state_steps)
+        return lanczos_roofline(D, m, state_steps, kernel_s, tr, src)
+
+    def parity(outs):
+        import numpy as np
+        fx = load_fixture("cfg3_subset")
+        Mf = fx["trajectories"]
+        cf = sysd.trace_samples(sysobj.state_index("S"), Mf, fx["sample_seed"]) / math.sqrt(Mf)
+        prm_f = abi.run_params(total_time=fx["total_time"], time_step=fx["time_step"], method=abi.MSG_METHOD_KRYLOV,
+                               sampling=abi.MSG_SAMPLING_GIVEN, seed=fx["flip_seed"], krylov_size=fx["krylov_size"])
+        y, _ = sysd.propagate(prm_f, columns=cf)
+        err = float(np.abs(y[0] - np.array(fx["yields"])).max())
+        return {"fixture": "tests/golden/cfg3_subset.json (FP64 oracle)", "tolerance": {"yield_abs": 1e-5},
+                "fixture_launch": {"trajectories": Mf, "steps": int(fx["total_time"] / fx["time_step"]),
+                                   "max_abs_dPhi": err},
+                "max_abs_dPhi": err}
 
     cpu_prm = abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_KRYLOV,
                              sampling=abi.MSG_SAMPLING_GIVEN, seed=2026)
-    return dict(step=step, state_steps=state_steps, roofline=roof, packed=packed, cpu_prm=cpu_prm,
+    return dict(step=step, state_steps=state_steps, roofline=roof, parity=parity, packed=packed, cpu_prm=cpu_prm,
                 cpu_field=None, cpu_columns=cols,
                 data="synthetic (seeded radical pair, host-drawn spin-coherent nuclear states)",
                 config={"workload": "stochastic trajectories: D=65536 (8+6 spin-1/2 nuclei, isotropic HF), "
@@ -247,11 +366,11 @@ def pair_batch(ctx, args):
     shape = sysd.autotune(prm, fields=fields, sample_steps=2)
 
     def step(i):
-        sysd.propagate(prm, fields=fields)
+        return sysd.propagate(prm, fields=fields)[0]
 
     def roof(kernel_s):
-        tr = MEASURED_TRAFFIC_PER_STATE_STEP.get("pair/%d" % shape.groups_per_thread)
-        return lanczos_roofline(D, m, state_steps, kernel_s, None if tr is None else tr * state_steps)
+        tr, src = traffic_for("pair/%d" % shape.groups_per_thread, state_steps)
+        return lanczos_roofline(D, m, state_steps, kernel_s, tr, src)
 
     return dict(step=step, state_steps=state_steps, roofline=roof, packed=packed,
                 cpu_prm=abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_KRYLOV),
@@ -294,7 +413,7 @@ def variant_batch(ctx, args):
     shape = sysd.autotune(prm, fields=fields, sample_steps=3)
 
     def step(i):
-        sysd.propagate(prm, fields=fields)
+        return sysd.propagate(prm, fields=fields)[0]
 
     def roof(kernel_s):
         if rydmr:
@@ -303,9 +422,9 @@ def variant_batch(ctx, args):
         # Gram-Schmidt sweep (v_j itself is in LDS; m (m - 1) / 2 reads) and m reads in
         # the final combination: 8 D (2 m + m (m - 1) / 2) bytes
         bss = 8.0 * D * (2 * m + m * (m - 1) / 2)
-        tr = MEASURED_TRAFFIC_PER_STATE_STEP["arnoldi"] * state_steps
+        tr, src = traffic_for("arnoldi", state_steps)
         return {"bound": "hbm", "achieved": bss * state_steps / kernel_s / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": bss * state_steps / kernel_s / HBM_PEAK, "traffic": tr,
+                "unit": "GB/s", "frac": bss * state_steps / kernel_s / HBM_PEAK, "traffic": tr, "traffic_source": src,
                 "kernel_ms": kernel_s * 1e3, "bytes_per_state_step": bss, "state_steps_per_launch": state_steps}
 
     what = ("linearly polarized 5 MHz, 50 uT field on both electrons (dynamics=dynamic)" if rydmr
@@ -337,7 +456,7 @@ def timeinf_batch(ctx, args):
     nf = len(cfg["fields"])
 
     def step(i):
-        sysd.time_infinity(prm, fields=cfg["fields"])
+        return sysd.time_infinity(prm, fields=cfg["fields"])
 
     return dict(step=step, state_steps=nf, roofline=lambda kernel_s: None, packed=None,
                 data="synthetic (seeded radical pair, log-spaced field strengths)",
@@ -345,6 +464,73 @@ def timeinf_batch(ctx, args):
                 config={"workload": "time-infinity yields: D=512 pair %d (7 spin-1/2 nuclei), unequal sinks "
                                     "k_S=3e-3, k_T=1e-3 /ns, %d field strengths per step (FP64 doubling)" % (idx, nf),
                         "fields": nf, "parallelism": "single GPU"})
+
+
+def uneven_batch(ctx, args):
+    """Config 4's uneven batch in one persistent launch: pairs 0 .. P-1 of the
+    config (n ~ U{2..12} nuclei as drawn), F of the 64 log-spaced field strengths
+    (every 64/F-th), all direct columns, `--intervals` steps of 2 ns."""
+    import numpy as np
+    from molspin_amd import abi, workloads
+    P, F = args.pairs, args.pair_fields
+    intervals = args.intervals or 20
+    pairs = []
+    for i in range(P):
+        s, cfg = workloads.config4_pair(i)
+        sysd = ctx.upload(s.pack())
+        fields = cfg["fields"][::max(1, 64 // F)][:F]
+        pairs.append(dict(sys=sysd, n=cfg["n"], fields=fields, cols=sysd.direct_columns()))
+    prm = abi.run_params(total_time=intervals * 2.0, time_step=2.0, method=abi.MSG_METHOD_AUTO, krylov_size=16)
+    ents = [dict(system=p["sys"], params=prm, fields=p["fields"]) for p in pairs]
+    state_steps = sum(len(p["fields"]) * p["cols"] * intervals for p in pairs)
+    mf_bytes = sum(len(p["fields"]) * p["cols"] * intervals * 8.0 * p["sys"].dim * 65 for p in pairs if p["sys"].dim > 512)
+
+    def step(i):
+        return ctx.propagate_uneven(ents)
+
+    # the same work one pair at a time (each pair's whole field sweep per call), and
+    # one pair and one field per call, timed once each outside the timed region
+    def one_at_a_time():
+        t0 = time.perf_counter()
+        for p in pairs:
+            p["sys"].propagate(prm, fields=p["fields"])
+        t_pair = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for p in pairs:
+            for f in p["fields"]:
+                p["sys"].propagate(prm, fields=[f])
+        t_field = time.perf_counter() - t0
+        return {"pair_at_a_time_state_steps_per_s": state_steps / t_pair,
+                "pair_and_field_at_a_time_state_steps_per_s": state_steps / t_field}
+
+    def roof(kernel_s):
+        return {"bound": "hbm", "achieved": mf_bytes / kernel_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": mf_bytes / kernel_s / HBM_PEAK, "traffic": None, "traffic_source": None,
+                "kernel_ms": kernel_s * 1e3, "state_steps_per_launch": state_steps,
+                "note": "byte model 8 D (4m + 1) per state-step summed over the matrix-free (D > 512) units; "
+                        "dense units counted in state-steps only"}
+
+    def parity(outs):
+        fx = load_fixture("cfg4_uneven")
+        prm_f = abi.run_params(total_time=fx["total_time"], time_step=fx["time_step"], method=abi.MSG_METHOD_AUTO,
+                               krylov_size=fx["krylov_size"])
+        ents_f = []
+        for e in fx["pairs"]:
+            s, _ = workloads.config4_pair(e["pair"])
+            ents_f.append(dict(system=ctx.upload(s.pack()), params=prm_f, fields=e["fields"]))
+        out = ctx.propagate_uneven(ents_f)
+        errs = {str(e["n"]): float(np.abs(y - np.array(e["yields"])).max()) for e, (y, _) in zip(fx["pairs"], out)}
+        return {"fixture": "tests/golden/cfg4_uneven.json (FP64 oracle; one pair per n = 2..12, 2 fields, 10 steps)",
+                "tolerance": {"yield_abs": 1e-5}, "max_abs_dPhi_by_n": errs, "max_abs_dPhi": max(errs.values())}
+
+    ns = sorted(p["n"] for p in pairs)
+    return dict(step=step, state_steps=state_steps, roofline=roof, parity=parity, packed=None, compare=one_at_a_time,
+                data="synthetic (seeded radical pairs, log-spaced field strengths)",
+                config={"workload": "uneven batch (config 4 subset): pairs 0..%d (n = %s), %d fields x all direct "
+                                    "columns x %d steps of 2 ns, one persistent launch" % (P - 1, ns, F, intervals),
+                        "pairs": P, "fields_per_pair": F, "intervals": intervals,
+                        "full_config": "256 pairs x 64 fields x 1500 steps (1.8e10 state-steps)",
+                        "parallelism": "single GPU"})
 
 
 def uncoupled_batch(ctx, args):
@@ -365,7 +551,7 @@ def uncoupled_batch(ctx, args):
     flops = B * (128 * 128 + 32 * 32) * ns * 24.0  # correlation sums: 9 products + 1 phase update per pair and sample
 
     def step(i):
-        sysd.propagate(prm, fields=cfg["fields"])
+        return sysd.propagate(prm, fields=cfg["fields"])[0]
 
     def roof(kernel_s):
         return {"bound": "fp64-valu", "achieved": flops / kernel_s / 1e12, "peak": 78.6, "unit": "TFLOP/s",
@@ -397,14 +583,14 @@ def dense_sweep(ctx, args):
     state_steps = nf * ncols * intervals
 
     def step(i):
-        sysd.propagate(prm, fields=cfg["fields"])
+        return sysd.propagate(prm, fields=cfg["fields"])[0]
 
     def roof(kernel_s):
         flops = 8.0 * D * D * state_steps
-        tr = MEASURED_TRAFFIC_PER_STATE_STEP.get("dense")
+        tr, src = traffic_for("dense", state_steps)
         return {"bound": "mfma", "achieved": flops / kernel_s / 1e12, "peak": MFMA_F32_PEAK / 1e12,
                 "unit": "TFLOP/s", "frac": flops / kernel_s / MFMA_F32_PEAK,
-                "traffic": None if tr is None else tr * state_steps,
+                "traffic": tr, "traffic_source": src,
                 "kernel_ms": kernel_s * 1e3, "flops_per_state_step": 8.0 * D * D,
                 "state_steps_per_launch": state_steps}
 
@@ -462,14 +648,28 @@ def serve(ctx, n_requests, intervals, method="auto", compare_plain=True):
             if y is not None:
                 i = live.pop(rid)
                 lat.append(time.perf_counter() - t0 - reqs[i]["arrival"])
+                reqs[i]["result"] = y
     elapsed = time.perf_counter() - t0
     srv.close()
     lat = np.array(lat)
     total = sum(r["steps"] for r in reqs)
+    # every request against the FP64 oracle: first and last field of each sweep
+    fx = load_fixture("serving40")
+    errs = []
+    if intervals == fx["intervals"]:
+        for e in fx["requests"]:
+            if e["request"] < n_requests:
+                y = reqs[e["request"]]["result"]
+                errs.append(float(np.abs(y[[0, -1]] - np.array(e["yields"])).max()))
+    check = {"fixture": "tests/golden/serving40.json (FP64 oracle, first and last field of each request)",
+             "requests_checked": len(errs), "max_abs_dPhi": max(errs) if errs else None, "tolerance": 1e-5}
+    reduced = {"requests": n_requests, "intervals_per_request": intervals,
+               "full_config": ">= 2000 requests of 1500-step sweeps (about 25 min of device time); this run is "
+                              "the stream's first %d requests over %d steps of 2 ns" % (n_requests, intervals)}
     if not compare_plain:
         return {"requests": n_requests, "p50_ms": float(np.percentile(lat, 50) * 1e3),
                 "p99_ms": float(np.percentile(lat, 99) * 1e3), "state_steps_per_s": total / elapsed,
-                "elapsed_s": elapsed, "method": method}
+                "elapsed_s": elapsed, "method": method, "parity": check, "run_size": reduced}
     plain_s = 0.0
     for r in reqs:
         r["sys"].propagate(r["prm"], fields=[r["fields"][0]])  # first call per system uploads factors
@@ -498,36 +698,58 @@ def serve(ctx, n_requests, intervals, method="auto", compare_plain=True):
     return {"requests": n_requests, "p50_ms": float(np.percentile(lat, 50) * 1e3),
             "p99_ms": float(np.percentile(lat, 99) * 1e3), "state_steps_per_s": total / elapsed,
             "plain_one_at_a_time_state_steps_per_s": plain, "served_over_plain": total / elapsed / plain,
-            "small_systems_D_le_512": by_class,
+            "small_systems_D_le_512": by_class, "parity": check, "run_size": reduced,
             "elapsed_s": elapsed, "intervals_per_request": intervals,
             "workload": "Poisson 200 req/s, seeded radical pairs (D 16..16384), 16-128 fields/request"}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(packed, prm, field, columns, seconds):
     """FP64 oracle (the reference algorithm: Arnoldi m=16 + Pade expm on the
     Krylov matrix, CSR matvecs; dense Pade propagator on the dense workload) on
-    a bounded sample of the same workload, OpenMP over columns on the host
-    threads."""
+    a bounded sample of the same workload, in the two modes of the reference:
+    one thread (HSGeneral's serial loop) and OpenMP over columns on the host's
+    cores (at most 16, the box's CPU share).  `value` is the all-core figure."""
     from oracle import binding as ref
-    threads = min(16, os.cpu_count() or 1)
-    cols = 2 * threads
-    if columns is not None:
-        cols = min(cols, columns.shape[1])
-    steps = 4
-    kw = dict(col_range=(0, cols), threads=threads)
-    if field is not None:
-        kw["field"] = field
-    if columns is not None:
-        kw["columns"] = columns
-    t0 = time.perf_counter()
-    ref.run_columns(packed, prm, max_steps=steps, **kw)
-    t1 = time.perf_counter() - t0
-    steps = max(4, min(4000, int(steps * seconds / max(t1, 1e-3))))
-    t0 = time.perf_counter()
-    ref.run_columns(packed, prm, max_steps=steps, **kw)
-    el = time.perf_counter() - t0
-    return {"value": cols * steps / el, "unit": "state-steps/s", "cores": threads, "kind": "port",
-            "sample": "1 item x %d columns x %d steps (FP64 oracle, %d OpenMP threads)" % (cols, steps, threads)}
+    nproc = os.cpu_count() or 1
+
+    def timed(threads, cols, budget):
+        if columns is not None:
+            cols = min(cols, columns.shape[1])
+        kw = dict(col_range=(0, cols), threads=threads)
+        if field is not None:
+            kw["field"] = field
+        if columns is not None:
+            kw["columns"] = columns
+        steps = 2
+        t0 = time.perf_counter()
+        ref.run_columns(packed, prm, max_steps=steps, **kw)
+        t1 = time.perf_counter() - t0
+        steps = max(2, min(4000, int(steps * budget / max(t1, 1e-3))))
+        t0 = time.perf_counter()
+        ref.run_columns(packed, prm, max_steps=steps, **kw)
+        el = time.perf_counter() - t0
+        return cols * steps / el, cols, steps
+
+    threads = min(16, nproc)
+    v_all, c_all, s_all = timed(threads, 2 * threads, 0.6 * seconds)
+    v_one, c_one, s_one = timed(1, 1, 0.4 * seconds)
+    return {"value": v_all, "unit": "state-steps/s", "cores": threads, "kind": "port",
+            "sample": "1 item x %d columns x %d steps (FP64 oracle, %d OpenMP threads over columns)"
+                      % (c_all, s_all, threads),
+            "single_thread": {"value": v_one, "unit": "state-steps/s", "cores": 1,
+                              "sample": "1 item x %d column x %d steps" % (c_one, s_one)},
+            "cpu_model": cpu_model(), "nproc": nproc}
 
 
 if __name__ == "__main__":

@@ -342,6 +342,35 @@ msg_status msg_propagate(msg_ctx *ctx, const msg_system *sys, const msg_run_para
                          double *yields, double *trace);
 
 /*
+ * One radical pair (system) of an uneven batch: what msg_propagate takes for one
+ * system, plus where its results go.
+ */
+typedef struct msg_uneven_entry {
+    const msg_system *system;
+    const msg_run_params *params;  /* static Hamiltonian, symmetric recombination on the
+                                      matrix-free path (any recombination at D <= 512) */
+    int32_t n_items;
+    const double *fields;          /* n_items x 3, or NULL */
+    const double *columns;         /* SAMPLING_GIVEN: D x n_columns complex, else NULL */
+    int32_t n_columns;
+    double *yields;                /* n_items x n_obs */
+    double *trace;                 /* n_items x n_samples x n_obs, or NULL */
+} msg_uneven_entry;
+
+/*
+ * Propagates several systems of different Hilbert dimension in one persistent
+ * launch: every (entry, item, column) of every entry goes into one work queue,
+ * largest dimension first, and each workgroup of the launch takes whatever comes
+ * next (matrix-free Lanczos bodies for 256 <= D <= 16384, dense MFMA bodies for
+ * D <= 512 under method AUTO / EXPONENTIAL / RK4).  Replaces the loop over
+ * systems and sweep points that rebuilds and reruns the task per point
+ * (RunSection/RunSection.cpp:96-140, main.cpp:634-686) around the per-system
+ * time loop (TaskHSGeneral.cpp:470-576).  An entry's results do not depend on the
+ * other entries of the batch.  Blocks until every entry's results are on the host.
+ */
+msg_status msg_propagate_uneven(msg_ctx *ctx, int32_t n_entries, const msg_uneven_entry *entries);
+
+/*
  * Infinite-time yields of every State object under symmetric decay at rate
  * `rate` (TaskStaticHSSymmetricDecay::RunLocal, TaskStaticHSSymmetricDecay.cpp:33-171):
  *   Phi_P = sum_{g,h} Re(<g|P|h><h|rho0|g>) k^2 / (k^2 + (lambda_g - lambda_h)^2)

@@ -109,6 +109,12 @@ class LaunchShape(C.Structure):
                 ("slots", C.c_int32), ("kernel_ms", C.c_double)]
 
 
+class UnevenEntry(C.Structure):
+    _fields_ = [("system", C.c_void_p), ("params", C.POINTER(RunParams)), ("n_items", C.c_int32),
+                ("fields", C.c_void_p), ("columns", C.c_void_p), ("n_columns", C.c_int32),
+                ("yields", C.c_void_p), ("trace", C.c_void_p)]
+
+
 def run_params(calculation=MSG_CALC_YIELDS, observables=MSG_OBS_TRANSITION_YIELDS,
                method=MSG_METHOD_KRYLOV, krylov_size=16, sampling=MSG_SAMPLING_DIRECT,
                yield_corrections=False, total_time=10000.0, time_step=1.0, seed=1, task=MSG_TASK_HSGENERAL,

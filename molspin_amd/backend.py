@@ -21,6 +21,7 @@ EXPORTS = [
     "msg_server_submit", "msg_server_pump", "msg_server_collect", "msg_last_kernel_time",
     "msg_trace_samples", "msg_symmetric_decay", "msg_resolve_plan", "msg_autotune", "msg_time_infinity",
     "msg_msd_compile", "msg_msd_system", "msg_msd_steps", "msg_msd_task_count", "msg_msd_task", "msg_msd_destroy",
+    "msg_propagate_uneven",
 ]
 
 
@@ -69,6 +70,7 @@ def lib():
         L.msg_resolve_plan.argtypes = [C.c_char_p, P(abi.Plan), C.c_char_p, C.c_int32]
         L.msg_autotune.argtypes = [vp, vp, P(abi.RunParams), C.c_int32, vp, vp, C.c_int32, C.c_int32,
                                    P(abi.LaunchShape)]
+        L.msg_propagate_uneven.argtypes = [vp, C.c_int32, P(abi.UnevenEntry)]
         for name in EXPORTS:
             getattr(L, name).restype = C.c_int if name != "msg_last_error" else C.c_char_p
         _LIB = L
@@ -175,6 +177,36 @@ class Context:
 
     def upload(self, packed):
         return System(self, packed)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
+    def propagate_uneven(self, entries):
+        """msg_propagate_uneven: several systems of different dimension in one
+        persistent launch.  `entries`: dicts with keys system, params and optionally
+        fields, columns, trace (bool); returns a list of (yields[n_items, n_obs],
+        trace[n_items, n_samples, n_obs] or None)."""
+        arr = (abi.UnevenEntry * len(entries))()
+        keep, outs = [], []
+        for k, ent in enumerate(entries):
+            system, params = ent["system"], ent["params"]
+            fields, columns = ent.get("fields"), ent.get("columns")
+            n_items = 1 if fields is None else len(fields)
+            f = None if fields is None else np.ascontiguousarray(fields, dtype=np.float64).reshape(-1)
+            cols, ncol = (None, 0) if columns is None else _columns(columns)
+            nobs = system.n_obs(params)
+            ns = len(time_grid(params.total_time, params.time_step))
+            y = np.zeros(n_items * nobs)
+            tr = np.zeros(n_items * ns * nobs) if ent.get("trace") else None
+            keep += [params, f, cols, y, tr]
+            arr[k] = abi.UnevenEntry(system.h, C.pointer(params), n_items, _ptr(f), _ptr(cols), ncol, _ptr(y), _ptr(tr))
+            outs.append((y.reshape(n_items, nobs), None if tr is None else tr.reshape(n_items, ns, nobs)))
+        self.check(lib().msg_propagate_uneven(self.h, len(entries), arr))
+        return outs
 
     def last_kernel_ms(self):
         """Device time of the propagation kernel of the last propagate call."""

@@ -60,6 +60,8 @@ hipError_t launch_unc_combine(const double *CA, const double *CB, const double *
 hipError_t launch_to_planes(const void *U, float *dst, int n, int Dp, size_t stride, int batch, hipStream_t st);
 int dense_occupancy(int dpad, size_t lds);
 int dense_threads(int dpad);
+hipError_t launch_uneven(const UnevenArgs *du, int workgroups, hipStream_t st);
+int uneven_occupancy();
 }  // namespace msg
 
 using namespace msg;
@@ -114,6 +116,11 @@ struct ArgError {
     msg_status st;
     std::string what;
 };
+// The device refused a cooperative launch (split columns wider than the resident
+// capacity): nothing was launched.  Only the autotuner recovers from it.
+struct CoopRefused {
+    std::string what;
+};
 
 // RAII device allocation, stream-ordered so that a batch never synchronises the
 // device (other requests keep running while this one allocates or frees).
@@ -165,7 +172,9 @@ struct Batch {
     hipStream_t stream = nullptr;
     size_t footprint = 0;
     StepArgs args_host{};  // host copy of the last launch arguments
-    bool timed = false;    // bracket the propagation kernel with the context events
+    bool timed = false;    // bracket the propagation kernels with the context events
+    int launches = 0;      // propagation kernels launched so far
+    double kernel_ms = 0.0;  // device time of the launches before the last one
     // dynamics=dynamic on the dense path: one propagator per interval, built per chunk
     // of intervals from G_item (static generator) + sum_k c_k(t) Z_k
     bool arnoldi = false;  // matrix-free path on G = -iH - K (unequal recombination rates)
@@ -175,6 +184,7 @@ struct Batch {
     std::vector<double> dyn_gnorm, dyn_znorm, item_fields;
     std::vector<std::pair<int, int>> dyn_terms;  // (spin, axis) of each Z_k
     bool field_given = false;
+    bool uneven = false;  // part of an uneven launch: the launch owns the Krylov bases
     Batch() = default;
     Batch(const Batch &) = delete;
 };
@@ -256,18 +266,25 @@ Path choose_path(const Resolved &r, const msg_run_params *p) {
 
 const StepArgs *push_args(Batch &b, const StepArgs &a);
 
+// The context's rocBLAS handle (created on first use, destroyed with the context),
+// bound to the stream of the calling batch.  A context is used by one host thread at
+// a time, so rebinding the stream per call is safe.
+rocblas_handle blas_on(msg_ctx *ctx, hipStream_t st) {
+    if (!ctx->blas && rocblas_create_handle(&ctx->blas) != rocblas_status_success) {
+        ctx->blas = nullptr;
+        throw DeviceError("rocblas_create_handle failed");
+    }
+    if (rocblas_set_stream(ctx->blas, st) != rocblas_status_success) throw DeviceError("rocblas_set_stream failed");
+    return ctx->blas;
+}
+
 // Batched complex FP64 GEMM of row-major n x n operands (contiguous batches) on
 // rocBLAS: C = coef A op(B) + E with op(B) = B or B^dagger.  A row-major array read
 // column-major is the transpose, so C^T = op(B)^T A^T is one column-major call with
-// the operands swapped (conj(B) A^T for op(B) = B^dagger).  One process-wide handle,
-// set to the caller's stream for each call.
-hipError_t zgemm(const void *A, const void *B, void *C, int n, int batch, double coef, hipStream_t st,
+// the operands swapped (conj(B) A^T for op(B) = B^dagger).
+hipError_t zgemm(msg_ctx *ctx, const void *A, const void *B, void *C, int n, int batch, double coef, hipStream_t st,
                  bool conj_b = false, const void *E = nullptr) {
-    static std::mutex mu;
-    static rocblas_handle h = nullptr;
-    std::lock_guard<std::mutex> lock(mu);
-    if (!h && rocblas_create_handle(&h) != rocblas_status_success) return hipErrorUnknown;
-    if (rocblas_set_stream(h, st) != rocblas_status_success) return hipErrorUnknown;
+    rocblas_handle h = blas_on(ctx, st);
     const size_t nn = (size_t)n * n;
     if (E && E != C) {
         const hipError_t e = hipMemcpyAsync(C, E, (size_t)batch * nn * sizeof(cd), hipMemcpyDeviceToDevice, st);
@@ -287,30 +304,30 @@ hipError_t zgemm(const void *A, const void *B, void *C, int n, int batch, double
 // then P = B_4 and P = P X^4 + B_k for k = 3..0 with B_k = sum_i X^i / (4k+i)!).
 // With rk4 the polynomial is I + X + X^2/2 + X^3/6 + X^4/24 (one RK4 step of a static
 // generator).  The result is left in *dP; dX2, dX3, dX4, dB and *dQ are workspace.
-void taylor_batch(cd *dX, cd *dX2, cd *dX3, cd *dX4, cd *dB, cd *&dP, cd *&dQ, int D, int nb, bool rk4,
-                  hipStream_t stream) {
+void taylor_batch(msg_ctx *ctx, cd *dX, cd *dX2, cd *dX3, cd *dX4, cd *dB, cd *&dP, cd *&dQ, int D, int nb,
+                  bool rk4, hipStream_t stream) {
     double fact[19];  // 1/j!
     fact[0] = 1.0;
     for (int j = 1; j <= 18; ++j) fact[j] = fact[j - 1] / j;
     auto taylor_block = [&](int k, double c[4]) {  // coefficients of X^0..X^3 in block k
         for (int i = 0; i < 4; ++i) c[i] = 4 * k + i <= 18 ? fact[4 * k + i] : 0.0;
     };
-    HIPCHECK(zgemm(dX, dX, dX2, D, nb, 1.0, stream));
-    HIPCHECK(zgemm(dX2, dX, dX3, D, nb, 1.0, stream));
+    HIPCHECK(zgemm(ctx, dX, dX, dX2, D, nb, 1.0, stream));
+    HIPCHECK(zgemm(ctx, dX2, dX, dX3, D, nb, 1.0, stream));
     double c[4];
     if (rk4) {
         taylor_block(0, c);
         HIPCHECK(launch_zlincomb(dX, dX2, dX3, dB, D, nb, c, stream));
-        HIPCHECK(zgemm(dX2, dX2, dP, D, nb, 1.0 / 24.0, stream, false, dB));
+        HIPCHECK(zgemm(ctx, dX2, dX2, dP, D, nb, 1.0 / 24.0, stream, false, dB));
         return;
     }
-    HIPCHECK(zgemm(dX2, dX2, dX4, D, nb, 1.0, stream));
+    HIPCHECK(zgemm(ctx, dX2, dX2, dX4, D, nb, 1.0, stream));
     taylor_block(4, c);
     HIPCHECK(launch_zlincomb(dX, dX2, dX3, dP, D, nb, c, stream));
     for (int k = 3; k >= 0; --k) {
         taylor_block(k, c);
         HIPCHECK(launch_zlincomb(dX, dX2, dX3, dB, D, nb, c, stream));
-        HIPCHECK(zgemm(dP, dX4, dQ, D, nb, 1.0, stream, false, dB));
+        HIPCHECK(zgemm(ctx, dP, dX4, dQ, D, nb, 1.0, stream, false, dB));
         std::swap(dP, dQ);
     }
 }
@@ -726,9 +743,9 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                     const bool rk4 = p->method == MSG_METHOD_RK4;
                     while (!rk4 && nrm > 0.5) { nrm *= 0.5; ++sq; }
                     sweep.form(fields + 3 * (size_t)it0, nb, std::ldexp(dt, -sq), D, dX, stream);
-                    taylor_batch(dX, dX2, dX3, dX4, dB, dP, dQ, D, nb, rk4, stream);
+                    taylor_batch(ctx, dX, dX2, dX3, dX4, dB, dP, dQ, D, nb, rk4, stream);
                     for (int q = 0; q < sq; ++q) {
-                        HIPCHECK(zgemm(dP, dP, dQ, D, nb, 1.0, stream));
+                        HIPCHECK(zgemm(ctx, dP, dP, dQ, D, nb, 1.0, stream));
                         std::swap(dP, dQ);
                     }
                     HIPCHECK(launch_to_planes(dP, dU + ((size_t)it0 * per + w) * kDensePlanes * Dp * Dp, D, Dp,
@@ -751,9 +768,9 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                 const double scale = std::ldexp(1.0, -sq);
                 for (auto &x : Hs) x *= scale;
                 HIPCHECK(hipMemcpyAsync(dX, Hs.data(), (size_t)nb * D * D * sizeof(cd), hipMemcpyHostToDevice, stream));
-                taylor_batch(dX, dX2, dX3, dX4, dB, dP, dQ, D, nb, rk4, stream);
+                taylor_batch(ctx, dX, dX2, dX3, dX4, dB, dP, dQ, D, nb, rk4, stream);
                 for (int q = 0; q < sq; ++q) {
-                    HIPCHECK(zgemm(dP, dP, dQ, D, nb, 1.0, stream));
+                    HIPCHECK(zgemm(ctx, dP, dP, dQ, D, nb, 1.0, stream));
                     std::swap(dP, dQ);
                 }
                 HIPCHECK(launch_to_planes(dP, dU + ((size_t)it0 * per + w) * kDensePlanes * Dp * Dp, D, Dp,
@@ -790,7 +807,10 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         a.state = b.state.alloc<float>((size_t)b.n_units * per_unit, stream);
     }
     // launch geometry
-    if (b.path == Path::Lanczos) {
+    if (b.path == Path::Lanczos && b.uneven) {
+        a.norm0 = b.norm0.alloc<double>(b.n_units, stream);
+        if (a.renorm) HIPCHECK(launch_init_norms(a, push_args(b, a), stream));
+    } else if (b.path == Path::Lanczos) {
         const size_t lds = b.G == 0 ? 0 : b.G == -1 ? (size_t)8 * D : b.G == -2 ? (size_t)32 * kTileGroups
                          : b.G == -3 ? (size_t)128 * b.T
                                      : b.arnoldi ? (size_t)8 * D + 4096 : (size_t)16 * D + 4096;
@@ -818,7 +838,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         }
         a.norm0 = b.norm0.alloc<double>(b.n_units, stream);
         if (a.renorm) HIPCHECK(launch_init_norms(a, push_args(b, a), stream));
-    } else {
+    } else if (!b.uneven) {
         const size_t lds = (size_t)4 * kDenseLdsPlanes * a.dpad * 16 + 4096;
         const int occ = std::max(1, dense_occupancy(a.dpad, lds));
         b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ));
@@ -851,11 +871,35 @@ void run_chunk(Batch &b, int steps) {
     HIPCHECK(launch_zero_int(a.queue, 1, b.stream));
     if (a.bar) HIPCHECK(launch_zero_int(a.bar, b.slots, b.stream));
     const StepArgs *da = push_args(b, a);
-    if (b.timed) HIPCHECK(hipEventRecord(b.ctx->ev0, b.stream));
-    if (b.path == Path::Lanczos) HIPCHECK(launch_lanczos(a, da, b.G, b.T, b.slots, b.stream));
-    else HIPCHECK(launch_dense(a, da, b.slots, b.stream));
+    if (b.timed) {
+        if (b.launches > 0) {  // the previous chunk's launch time, before the events are re-recorded
+            HIPCHECK(hipEventSynchronize(b.ctx->ev1));
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, b.ctx->ev0, b.ctx->ev1));
+            b.kernel_ms += ms;
+        }
+        HIPCHECK(hipEventRecord(b.ctx->ev0, b.stream));
+    }
+    if (b.path == Path::Lanczos) {
+        const hipError_t e = launch_lanczos(a, da, b.G, b.T, b.slots, b.stream);
+        if (e == hipErrorCooperativeLaunchTooLarge && b.G == -3)
+            throw CoopRefused{"cooperative launch of the split-column kernel refused by the device"};
+        HIPCHECK(e);
+    } else {
+        HIPCHECK(launch_dense(a, da, b.slots, b.stream));
+    }
     if (b.timed) HIPCHECK(hipEventRecord(b.ctx->ev1, b.stream));
+    ++b.launches;
     b.s_next = a.s_end;
+}
+
+// Device time of all propagation launches of a timed batch (call after collect()).
+double batch_kernel_ms(Batch &b) {
+    if (!b.timed || b.launches == 0) return 0.0;
+    float ms = 0.f;
+    HIPCHECK(hipEventSynchronize(b.ctx->ev1));
+    HIPCHECK(hipEventElapsedTime(&ms, b.ctx->ev0, b.ctx->ev1));
+    return b.kernel_ms + ms;
 }
 
 // Propagators of intervals [s0, s1) of a dynamic dense batch, into b.U with index
@@ -864,6 +908,7 @@ void run_chunk(Batch &b, int steps) {
 // map I + (K1 + 2 K2 + 2 K3 + K4)/6 of G at t, t + dt/2, t + dt (HSPropagator.cpp:104-129),
 // K1 = dt G0, K2 = dt Gm (1 + K1/2), K3 = dt Gm (1 + K2/2), K4 = dt G1 (1 + K3).
 void build_dyn_chunk(Batch &b, int s0, int s1) {
+    msg_ctx *ctx = b.ctx;
     const Resolved &r = b.sys->r;
     const int D = (int)r.D, Dp = b.a.dpad, nsc = s1 - s0;
     const size_t DD = (size_t)D * D;
@@ -925,17 +970,17 @@ void build_dyn_chunk(Batch &b, int s0, int s1) {
         cd *res = nullptr;
         if (!rk4) {
             cd *p = dP, *qq = dQ;
-            taylor_batch(dX, dX2, dX3, dX4, dB, p, qq, D, nb, false, st);
+            taylor_batch(ctx, dX, dX2, dX3, dX4, dB, p, qq, D, nb, false, st);
             for (int k = 0; k < sq; ++k) {
-                HIPCHECK(zgemm(p, p, qq, D, nb, 1.0, st));
+                HIPCHECK(zgemm(ctx, p, p, qq, D, nb, 1.0, st));
                 std::swap(p, qq);
             }
             res = p;
         } else {
             const cd *a0 = dX, *am = dX + (size_t)nb * DD, *a1 = dX + (size_t)2 * nb * DD;
-            HIPCHECK(zgemm(am, a0, dX2, D, nb, 0.5, st, false, am));   // K2 = am + am K1 / 2
-            HIPCHECK(zgemm(am, dX2, dX3, D, nb, 0.5, st, false, am));  // K3 = am + am K2 / 2
-            HIPCHECK(zgemm(a1, dX3, dX4, D, nb, 1.0, st, false, a1));  // K4 = a1 + a1 K3
+            HIPCHECK(zgemm(ctx, am, a0, dX2, D, nb, 0.5, st, false, am));   // K2 = am + am K1 / 2
+            HIPCHECK(zgemm(ctx, am, dX2, dX3, D, nb, 0.5, st, false, am));  // K3 = am + am K2 / 2
+            HIPCHECK(zgemm(ctx, a1, dX3, dX4, D, nb, 1.0, st, false, a1));  // K4 = a1 + a1 K3
             const double c1[4] = {1.0, 1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0};
             HIPCHECK(launch_zlincomb(a0, dX2, dX3, dB, D, nb, c1, st));            // 1 + K1/6 + K2/3 + K3/3
             const double c2[4] = {0.0, 1.0, 1.0 / 6.0, 0.0};
@@ -1003,11 +1048,43 @@ msg_status guarded(msg_ctx *ctx, F f) {
         return fail(ctx, e.st, e.what);
     } catch (const DeviceError &e) {
         return fail(ctx, MSG_DEVICE_ERROR, e.what);
+    } catch (const CoopRefused &e) {
+        return fail(ctx, MSG_DEVICE_ERROR, e.what);
     } catch (const std::bad_alloc &) {
         return fail(ctx, MSG_OOM_ADMISSION, "host allocation failed");
     } catch (const std::exception &e) {
         return fail(ctx, MSG_DEVICE_ERROR, e.what());
     }
+}
+
+// Class and packing of an entry in the uneven launch (the table of k_uneven).
+struct UnevenShape {
+    int cls = 0, per_pack = 1;
+    size_t basis_floats = 0;  // per workgroup
+};
+
+UnevenShape uneven_shape(const Batch &b) {
+    UnevenShape s;
+    const int D = b.a.dim;
+    if (b.path == Path::Lanczos) {
+        const int ng = D / 4;
+        static const int packs[7] = {16, 8, 4, 2, 2, 1, 1};
+        int lg = 0;
+        while ((1 << lg) < ng) ++lg;
+        if ((1 << lg) != ng || lg < 6 || lg > 12)
+            throw ArgError{MSG_UNSUPPORTED, "uneven batch: matrix-free entries need 256 <= D <= 16384"};
+        s.cls = UC_LZ64 + (lg - 6);
+        s.per_pack = packs[lg - 6];
+        s.basis_floats = s.cls == UC_LZ4096 ? (size_t)(b.a.m + 2) * 2 * D : (size_t)s.per_pack * b.a.m * 2 * D;
+    } else {
+        static const int nts[10] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32};
+        const int nt = b.a.dpad / 16;
+        const int i = (int)(std::find(nts, nts + 10, nt) - nts);
+        if (i == 10) throw ArgError{MSG_UNSUPPORTED, "uneven batch: no dense class for this dimension"};
+        s.cls = UC_DENSE + i;
+        s.per_pack = nt >= 12 ? 1 : nt >= 6 ? 2 : 4;
+    }
+    return s;
 }
 
 }  // namespace
@@ -1080,10 +1157,13 @@ msg_status msg_ctx_create(int device, msg_ctx **out) {
 msg_status msg_ctx_destroy(msg_ctx *ctx) {
     if (!ctx) return MSG_INVALID_ARG;
     (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    // finish the context's work first; the rocBLAS handle is bound to the context's
+    // stream, so it goes before the stream does
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     delete ctx;
     return MSG_OK;
@@ -1274,12 +1354,7 @@ void propagate_uncoupled(msg_ctx *ctx, const msg_system *sys, const msg_run_para
                         Q[(size_t)c * 256 + (mu * 4 + al) * 16 + nu * 4 + be] = pc[mu * 4 + nu] * rc[al * 4 + be] * norm;
     }
     hipStream_t st = ctx->stream;
-    if (!ctx->blas) {
-        if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
-            throw ArgError{MSG_DEVICE_ERROR, "rocblas_create_handle failed"};
-    }
-    if (rocblas_set_stream(ctx->blas, st) != rocblas_status_success)
-        throw ArgError{MSG_DEVICE_ERROR, "rocblas_set_stream failed"};
+    (void)blas_on(ctx, st);
     DBuf dQ, dwy, dwt, dtimes, dy, dtr;
     double *q = dQ.alloc<double>(Q.size(), st);
     double *wy = dwy.alloc<double>(wyh.size(), st);
@@ -1328,7 +1403,7 @@ void propagate_uncoupled(msg_ctx *ctx, const msg_system *sys, const msg_run_para
             const size_t plane = (size_t)nb * rd.n * rd.n;  // S_mu of all items of the chunk
             for (int mu = 0; mu < 3; ++mu) {
                 HIPCHECK(launch_pauli_right(rd.Hm, rd.Y, rd.n, mu, nb, st));
-                HIPCHECK(zgemm(rd.Y, rd.Hm, rd.S + mu * plane, rd.n, nb, 1.0, st, true));
+                HIPCHECK(zgemm(ctx, rd.Y, rd.Hm, rd.S + mu * plane, rd.n, nb, 1.0, st, true));
             }
             HIPCHECK(launch_corr(rd.S, plane, rd.lam, tm, rd.n, ns, nb, rd.C, st));
         }
@@ -1376,10 +1451,8 @@ msg_status msg_propagate(msg_ctx *ctx, const msg_system *sys, const msg_run_para
         b.timed = true;
         advance(b, b.nsamples);
         collect(b, yields, trace);
-        float ms = 0.f;
-        HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        ctx->kernel_ms = ms;
-        ctx->launches = 1;
+        ctx->kernel_ms = batch_kernel_ms(b);
+        ctx->launches = b.launches;
     });
 }
 
@@ -1456,17 +1529,14 @@ msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_param
                     b.timed = true;
                     advance(b, b.nsamples);
                     collect(b, y.data(), nullptr);
-                } catch (const DeviceError &) {
+                } catch (const CoopRefused &) {
                     // a split-column candidate whose cooperative launch the device refuses
-                    // (nothing was launched) drops out; any other failure is reported
+                    // (nothing was launched) drops out; every other failure is reported
                     ctx->force = nullptr;
-                    if (!(path == Path::Lanczos && c.G == -3)) throw;
                     ms_min = 1e300;
                     break;
                 }
-                float ms = 0.f;
-                HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-                ms_min = std::min(ms_min, (double)ms);
+                ms_min = std::min(ms_min, batch_kernel_ms(b));
                 slots_used = b.slots;
                 if (path == Path::Lanczos && c.slots == -2) c.slots = trial.slots;
             }
@@ -1485,6 +1555,103 @@ msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_param
         if (path == Path::Lanczos) ctx->tuned[r.D] = best_shape;
         out.kernel_ms = best_ms;
         if (best) *best = out;
+    });
+}
+
+msg_status msg_propagate_uneven(msg_ctx *ctx, int32_t n_entries, const msg_uneven_entry *entries) {
+    if (!ctx || n_entries < 1 || !entries) return fail(ctx, MSG_INVALID_ARG, "null context or no entries");
+    return guarded(ctx, [&] {
+        HIPCHECK(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        std::vector<std::unique_ptr<Batch>> bs(n_entries);
+        std::vector<StepArgs> ents(n_entries);
+        struct Ordered {
+            UnevenPack p;
+            double cost;
+        };
+        std::vector<Ordered> order;
+        size_t basis_floats = 0;
+        for (int e = 0; e < n_entries; ++e) {
+            const msg_uneven_entry &en = entries[e];
+            const std::string at = "entry " + std::to_string(e) + ": ";
+            if (!en.system || !en.params || !en.yields)
+                throw ArgError{MSG_INVALID_ARG, at + "null system, parameters or yields"};
+            if (en.params->method == MSG_METHOD_UNCOUPLED)
+                throw ArgError{MSG_UNSUPPORTED, at + "propagationmethod=uncoupled runs through msg_propagate"};
+            if (en.params->dynamics && en.system->r.has_td)
+                throw ArgError{MSG_UNSUPPORTED, at + "the uneven batch takes static Hamiltonians"};
+            bs[e].reset(new Batch());
+            Batch &b = *bs[e];
+            b.uneven = true;
+            try {
+                prepare(b, ctx, en.system, en.params, en.n_items, en.fields, en.columns, en.n_columns,
+                        en.trace != nullptr, false, st);
+            } catch (const ArgError &x) {
+                throw ArgError{x.st, at + x.what};
+            }
+            if (b.arnoldi)
+                throw ArgError{MSG_UNSUPPORTED, at + "unequal recombination rates above D = 512 run through msg_propagate"};
+            const UnevenShape sh = uneven_shape(b);
+            basis_floats = std::max(basis_floats, sh.basis_floats);
+            StepArgs a = b.a;
+            a.s_begin = 0;
+            a.s_end = b.nsamples - 1;
+            ents[e] = a;
+            // packs of consecutive units of one item, cost ~ device time of the pack
+            const int upi = b.n_units / b.n_items;
+            const double D = a.dim, steps = b.nsamples - 1;
+            for (int it = 0; it < b.n_items; ++it)
+                for (int c = 0; c < upi; c += sh.per_pack) {
+                    const int nact = std::min(sh.per_pack, upi - c);
+                    const double cost = b.path == Path::Lanczos
+                                            ? nact * D * steps * a.m * (a.nnuc + 4)
+                                            : nact * 16.0 * a.dpad * a.dpad * steps / 16.0;
+                    order.push_back({UnevenPack{e, it * upi + c, nact, sh.cls}, cost});
+                }
+        }
+        std::stable_sort(order.begin(), order.end(), [](const Ordered &x, const Ordered &y) { return x.cost > y.cost; });
+        std::vector<UnevenPack> packs(order.size());
+        for (size_t k = 0; k < order.size(); ++k) packs[k] = order[k].p;
+        const int n_packs = (int)packs.size();
+        const int occ = std::max(1, uneven_occupancy());
+        const int wgs = std::max(1, std::min(n_packs, ctx->cus * occ));
+        DBuf dents, dpacks, dqueue, dbasis, dargs;
+        StepArgs *de = dents.alloc<StepArgs>(ents.size(), st);
+        HIPCHECK(hipMemcpyAsync(de, ents.data(), ents.size() * sizeof(StepArgs), hipMemcpyHostToDevice, st));
+        UnevenPack *dp = dpacks.alloc<UnevenPack>(packs.size(), st);
+        HIPCHECK(hipMemcpyAsync(dp, packs.data(), packs.size() * sizeof(UnevenPack), hipMemcpyHostToDevice, st));
+        UnevenArgs ua{};
+        ua.ents = de;
+        ua.packs = dp;
+        ua.n_packs = n_packs;
+        ua.queue = dqueue.alloc<int>(1, st);
+        HIPCHECK(launch_zero_int(ua.queue, 1, st));
+        ua.basis_stride = (basis_floats + 63) / 64 * 64;
+        ua.basis = ua.basis_stride ? dbasis.alloc<float>((size_t)wgs * ua.basis_stride, st) : nullptr;
+        UnevenArgs *du = dargs.alloc<UnevenArgs>(1, st);
+        HIPCHECK(hipMemcpyAsync(du, &ua, sizeof(UnevenArgs), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipEventRecord(ctx->ev0, st));
+        HIPCHECK(launch_uneven(du, wgs, st));
+        HIPCHECK(hipEventRecord(ctx->ev1, st));
+        HIPCHECK(hipStreamSynchronize(st));  // ents / packs / ua are host locals
+        int bad_class = 0;
+        for (int e = 0; e < n_entries; ++e) {
+            int sflag = 0;
+            HIPCHECK(hipMemcpy(&sflag, bs[e]->a.status, sizeof(int), hipMemcpyDeviceToHost));
+            bad_class |= sflag & 8;
+        }
+        if (bad_class) throw DeviceError("uneven launch: a pack named no class");
+        for (int e = 0; e < n_entries; ++e) {
+            try {
+                collect(*bs[e], entries[e].yields, entries[e].trace);
+            } catch (const ArgError &x) {
+                throw ArgError{x.st, "entry " + std::to_string(e) + ": " + x.what};
+            }
+        }
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->kernel_ms = ms;
+        ctx->launches = 1;
     });
 }
 
@@ -1556,12 +1723,7 @@ msg_status msg_symmetric_decay(msg_ctx *ctx, const msg_system *sys, int32_t n_it
         const int D = (int)r.D;
         const int S = (int)r.P_states.size();
         hipStream_t st = ctx->stream;
-        if (!ctx->blas) {
-            if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
-                throw ArgError{MSG_DEVICE_ERROR, "rocblas_create_handle failed"};
-        }
-        if (rocblas_set_stream(ctx->blas, st) != rocblas_status_success)
-            throw ArgError{MSG_DEVICE_ERROR, "rocblas_set_stream failed"};
+        (void)blas_on(ctx, st);
         // rho0 = (sum_i P_i) (x) 1 / tr(...)  (TaskStaticHSSymmetricDecay.cpp:100-112)
         double rho[32] = {0.0};
         double tr = 0.0;
@@ -1598,7 +1760,7 @@ msg_status msg_symmetric_decay(msg_ctx *ctx, const msg_system *sys, int32_t n_it
             rocblas_int hinfo = 0;
             HIPCHECK(hipMemcpyAsync(&hinfo, di, sizeof(hinfo), hipMemcpyDeviceToHost, st));
             HIPCHECK(launch_blockdiag_right(dA, rho, rho_scale, dY, D, st));       // Y = M rho0
-            HIPCHECK(zgemm(dY, dA, dR, D, 1, 1.0, st, true));          // Rt = M rho0 M^dagger
+            HIPCHECK(zgemm(ctx, dY, dA, dR, D, 1, 1.0, st, true));          // Rt = M rho0 M^dagger
             for (int s0 = 0; s0 < S; ++s0) {
                 double P[32];
                 for (int i = 0; i < 16; ++i) {
@@ -1606,7 +1768,7 @@ msg_status msg_symmetric_decay(msg_ctx *ctx, const msg_system *sys, int32_t n_it
                     P[2 * i + 1] = r.P_states[s0][i].imag();
                 }
                 HIPCHECK(launch_blockdiag_right(dA, P, 1.0, dY, D, st));           // Y = M P
-                HIPCHECK(zgemm(dY, dA, dP, D, 1, 1.0, st, true));      // Pt = M P M^dagger
+                HIPCHECK(zgemm(ctx, dY, dA, dP, D, 1, 1.0, st, true));      // Pt = M P M^dagger
                 HIPCHECK(launch_lorentz_sum(dP, dR, dl, k, D, drows, dres + (size_t)it * S + s0, st));
             }
             HIPCHECK(hipStreamSynchronize(st));
@@ -1671,10 +1833,7 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
         std::vector<double> out((size_t)n_items * nobs, 0.0);
         if (eig) {
             // eigenbasis closed form: Re Tr[P X] = sum_gh Re(conj(Pt_gh) Rt_gh / (2 kappa + i w_gh))
-            if (!ctx->blas && rocblas_create_handle(&ctx->blas) != rocblas_status_success)
-                throw ArgError{MSG_DEVICE_ERROR, "rocblas_create_handle failed"};
-            if (rocblas_set_stream(ctx->blas, st) != rocblas_status_success)
-                throw ArgError{MSG_DEVICE_ERROR, "rocblas_set_stream failed"};
+            (void)blas_on(ctx, st);
             DBuf A, Y, Rt, Pt, lam, off, info, rows, res, Cb;
             cd *dA = A.alloc<cd>(DD, st), *dY = Y.alloc<cd>(DD, st), *dR = Rt.alloc<cd>(DD, st);
             cd *dP = Pt.alloc<cd>(DD, st);
@@ -1698,11 +1857,11 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
                 rocblas_int hinfo = 0;
                 HIPCHECK(hipMemcpyAsync(&hinfo, di, sizeof(hinfo), hipMemcpyDeviceToHost, st));
                 if (given) {
-                    HIPCHECK(zgemm(dA, dC, dY, D, 1, 1.0, st));         // Y = M C
-                    HIPCHECK(zgemm(dY, dY, dR, D, 1, 1.0, st, true));   // Rt = Y Y^dagger
+                    HIPCHECK(zgemm(ctx, dA, dC, dY, D, 1, 1.0, st));         // Y = M C
+                    HIPCHECK(zgemm(ctx, dY, dY, dR, D, 1, 1.0, st, true));   // Rt = Y Y^dagger
                 } else {
                     HIPCHECK(launch_blockdiag_right(dA, rho_e, inv_z, dY, D, st));  // Y = M rho0
-                    HIPCHECK(zgemm(dY, dA, dR, D, 1, 1.0, st, true));   // Rt = M rho0 M^dagger
+                    HIPCHECK(zgemm(ctx, dY, dA, dR, D, 1, 1.0, st, true));   // Rt = M rho0 M^dagger
                 }
                 for (int c = 0; c < nobs; ++c) {
                     double P[32];
@@ -1711,7 +1870,7 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
                         P[2 * i + 1] = Pobs[c][i].imag();
                     }
                     HIPCHECK(launch_blockdiag_right(dA, P, 1.0, dY, D, st));           // Y = M P
-                    HIPCHECK(zgemm(dY, dA, dP, D, 1, 1.0, st, true));      // Pt = M P M^dagger
+                    HIPCHECK(zgemm(ctx, dY, dA, dP, D, 1, 1.0, st, true));      // Pt = M P M^dagger
                     HIPCHECK(launch_timeinf_sum(dP, dR, dl, 2.0 * r.kappa, D, drows, dres + (size_t)it * nobs + c, st));
                 }
                 HIPCHECK(hipStreamSynchronize(st));
@@ -1735,7 +1894,7 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
             // rho0 (device order, row-major), replicated over the chunk
             if (given) {
                 HIPCHECK(hipMemcpyAsync(dX, Cpad.data(), DD * sizeof(cd), hipMemcpyHostToDevice, st));
-                HIPCHECK(zgemm(dX, dX, dR, D, 1, 1.0, st, true));  // C C^dagger
+                HIPCHECK(zgemm(ctx, dX, dX, dR, D, 1, 1.0, st, true));  // C C^dagger
             } else {
                 std::vector<cd> rho0(DD, cd(0.0));
                 for (int nu = 0; nu < D / 4; ++nu)
@@ -1783,13 +1942,13 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
                 const double cs[4] = {0.0, tau, 0.0, 0.0};
                 HIPCHECK(launch_zlincomb(dG, dG, dG, dX, D, nb, cs, st));
                 cd *dU = dP, *dU2 = dQ;
-                taylor_batch(dX, dX2, dX3, dX4, dB, dU, dU2, D, nb, false, st);
+                taylor_batch(ctx, dX, dX2, dX3, dX4, dB, dU, dU2, D, nb, false, st);
                 // X_tau = tau S_0: S_N = rho0, S_n = rho0 + tau/(n+2) (G S + (G S)^dagger), N = 16
                 const int N = 16;
                 cd *dS = dX3, *dY = dX4;
                 const cd *src = dR;
                 for (int n = N - 1; n >= 0; --n) {
-                    HIPCHECK(zgemm(dG, src, dY, D, nb, 1.0, st));
+                    HIPCHECK(zgemm(ctx, dG, src, dY, D, nb, 1.0, st));
                     HIPCHECK(launch_herm_update(dY, dR, dS, D, nb, tau / (n + 2), n == 0 ? tau : 1.0, st));
                     src = dS;
                 }
@@ -1806,11 +1965,11 @@ msg_status msg_time_infinity(msg_ctx *ctx, const msg_system *sys, const msg_run_
                 cd *dXc = dS, *dXn = dB, *dW = dY;
                 bool done = false;
                 for (int d = 0; d < kMaxDoublings && !done; ++d) {
-                    HIPCHECK(zgemm(dU, dXc, dW, D, nb, 1.0, st));              // W = U X
-                    HIPCHECK(zgemm(dW, dU, dXn, D, nb, 1.0, st, true));        // Delta = W U^dagger
+                    HIPCHECK(zgemm(ctx, dU, dXc, dW, D, nb, 1.0, st));              // W = U X
+                    HIPCHECK(zgemm(ctx, dW, dU, dXn, D, nb, 1.0, st, true));        // Delta = W U^dagger
                     HIPCHECK(launch_accumulate_fro(dXc, dXn, dW, D, nb, parts, dn, st));  // X + Delta, norms
                     std::swap(dXc, dW);
-                    HIPCHECK(zgemm(dU, dU, dU2, D, nb, 1.0, st));              // U^2
+                    HIPCHECK(zgemm(ctx, dU, dU, dU2, D, nb, 1.0, st));              // U^2
                     std::swap(dU, dU2);
                     HIPCHECK(hipMemcpyAsync(fro.data(), dn, 2 * (size_t)nb * parts * sizeof(double), hipMemcpyDeviceToHost,
                                             st));
@@ -1881,6 +2040,8 @@ msg_status msg_server_destroy(msg_server *srv) {
         if (q->done_ev) (void)hipEventDestroy(q->done_ev);
     srv->active.clear();
     srv->finished.clear();
+    // requests may have left the context's rocBLAS handle bound to a server stream
+    if (srv->ctx->blas) (void)rocblas_set_stream(srv->ctx->blas, srv->ctx->stream);
     for (hipStream_t st : srv->streams) (void)hipStreamDestroy(st);
     delete srv;
     return MSG_OK;

@@ -84,9 +84,35 @@ struct StepArgs {
     int *queue;              // work counter (zeroed before launch)
     float *basis;            // [slots][m][2D]
     int *status;             // device error flags (bit 0: non-finite, bit 1: dt too large,
-                             // bit 2: workgroups of a split column lost each other)
+                             // bit 2: workgroups of a split column lost each other,
+                             // bit 3: a pack of the uneven launch names no class)
     double *xred;            // split columns: [slots][2][S][17] partial sums
     int *bar;                // split columns: [slots] barrier counters (zeroed before launch)
+};
+
+// One persistent launch over radical pairs of uneven Hilbert dimension (k_uneven):
+// each pack is up to P columns (matrix-free classes) or 16-column tiles (dense
+// classes) of one item of one entry; the class fixes the body and its packing.
+enum UnevenClass {
+    UC_LZ64 = 0, UC_LZ128, UC_LZ256, UC_LZ512, UC_LZ1024, UC_LZ2048, UC_LZ4096,  // groups per column D / 4
+    UC_DENSE = 16,  // + index of the row-tile count in {1, 2, 3, 4, 6, 8, 12, 16, 24, 32}
+};
+constexpr int kUnevenThreads = 1024;
+constexpr int kUnevenPlanesBytes = 131072;       // per workgroup: the body's LDS planes
+constexpr int kUnevenScratchDoubles = 2 * 17 * 16;  // reductions, double-buffered
+struct UnevenPack {
+    int entry;  // index into UnevenArgs::ents
+    int u0;     // first unit of the pack (entry's unit numbering)
+    int nact;   // live units in the pack
+    int cls;    // UnevenClass
+};
+struct UnevenArgs {
+    const StepArgs *ents;     // [n_entries] device
+    const UnevenPack *packs;  // [n_packs] device, in launch order
+    int n_packs;
+    int *queue;               // pack counter (zeroed before launch)
+    float *basis;             // [workgroups][basis_stride] Krylov bases
+    size_t basis_stride;      // floats per workgroup
 };
 
 }  // namespace msg

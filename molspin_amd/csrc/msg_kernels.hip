@@ -501,180 +501,254 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// Workgroup-shared state of the propagation bodies, carved by the kernel that calls
+// them (one allocation per kernel however many bodies it instantiates).  Per
+// subgroup sg (a body may pack several columns or tiles of one item into a
+// workgroup, one per subgroup of consecutive waves): Kahan yield sums ys/yc
+// [sg * kMaxObs], Krylov combination coefficients coef [sg * kMaxKrylov]; one
+// coupling-block table blk per workgroup (every packed column belongs to one item).
+struct BodyShared {
+    double *ys, *yc;
+    float2 *coef;
+    float4 *blk;
+    double *colsum;  // dense bodies: per-wave column norms, 2 x 16 x 16 doubles
+};
+
+// Sum over the W waves of the calling thread's subgroup (waves w0 .. w0 + W - 1)
+// of K doubles per thread, in fixed order; every thread of
+// the subgroup receives its subgroup's totals.  `scr` holds K x 16 doubles.  With
+// W equal to the workgroup's wave count this is block_sum.
+template <int K>
+__device__ __forceinline__ void group_sum(double (&v)[K], double *scr, int W, int w0) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) scr[k * 16 + wave] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+#pragma unroll 1
+        for (int w = 0; w < W; ++w) s += scr[k * 16 + w0 + w];  // rolled: unrolled, the loads all go live at once
+        v[k] = s;
+    }
+}
+
+// After group_sum<2> of (alpha, |w|^2) into `scr`: whether the Lanczos cancellation
+// guard fires in any of the P subgroups (each subgroup's sums formed in the order
+// group_sum uses, so every subgroup takes the same decision).
+__device__ __forceinline__ bool any_guard(const double *scr, int W, int P) {
+    bool any = false;
+#pragma unroll 1
+    for (int q = 0; q < P; ++q) {
+        double al = 0.0, ww = 0.0;
+#pragma unroll 1
+        for (int w = 0; w < W; ++w) {
+            al += scr[q * W + w];
+            ww += scr[16 + q * W + w];
+        }
+        any |= !(ww - al * al > 0.02 * ww);
+    }
+    return any;
+}
+
+// Matrix-free Lanczos propagation of P columns of one item, one per subgroup of T
+// threads (G groups per thread, D = 4 G T), over the launch's intervals: the
+// HSGeneral time loop of those columns (observables at every sample, ST flips,
+// KrylovExpmGeneral, SpinSpace_operators.cpp:1257-1408, as symmetric Lanczos).
+// Units u0 .. u0 + nact - 1 (unit = item * ncols + column) are live; subgroups
+// from nact up repeat unit u0 and discard their results, so every subgroup meets
+// every barrier.  LDS: `smem` holds per subgroup the planes of v_j (and v_{j-1}
+// with DBUF), `scr` 2 x 256 doubles of reduction scratch.  `bas0`: the workgroup's
+// basis store, m vectors per subgroup.
 // DBUF: two LDS buffers hold v_j and v_{j-1} (swapped every iteration); without
-// DBUF one buffer holds v_j and v_{j-1} is re-read from the basis store, which
-// keeps D = 16384 (128 KiB per vector) resident in LDS.
+// DBUF one buffer holds v_j and v_{j-1} is re-read from the basis store.
 // DYN: the operator head (electron fields, shift) comes per interval from
 // a.dyn_head (time-dependent Zeeman fields); otherwise from the item's block.
-template <int G, int TMAX, bool DBUF, bool DYN = false>
-__global__ __launch_bounds__(TMAX, G >= 4 ? 2 : 4) void k_lanczos(const StepArgs *__restrict__ pa) {
-    const StepArgs &a = *pa;
-    extern __shared__ float4 smem[];
-    constexpr int T = TMAX;  // launched with exactly TMAX threads
-    const int t = threadIdx.x, lane = t & 63;
-    constexpr int ng = G * T;
-    // LDS planes of buffer i (i = 0, 1; a single buffer without DBUF)
-    auto LA = [&](int i) -> float4 * { return smem + ((DBUF && i) ? 2 * ng : 0); };
-    auto LB = [&](int i) -> float4 * { return smem + ng + ((DBUF && i) ? 2 * ng : 0); };
-    double *scr = reinterpret_cast<double *>(smem + (DBUF ? 4 : 2) * ng);  // 2 x 16 x 16 doubles
-    __shared__ double ys[kMaxObs], yc[kMaxObs];                // Kahan yield sums (thread 0)
-    __shared__ float2 s_coef[kMaxKrylov];                        // exp(-i dt T) e1, scaled
-    __shared__ float4 s_blk[kMaxNuclei * 8];                     // coupling blocks, both orientations
-    __shared__ int s_unit;
+// Packed subgroups take every decision that guards a barrier together: a
+// cancellation-guard reduction runs when any subgroup needs it (each uses it only
+// for itself), and a subgroup whose Krylov space closed early (happy breakdown)
+// idles through the remaining iterations.  A column's arithmetic is therefore the
+// same whatever P and its neighbours are.
+template <int G, int T, int P, bool DBUF, bool DYN>
+__device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int nact, float *bas0, float4 *smem,
+                                                double *scr, const BodyShared &sh) {
+    constexpr int ng = G * T, W = T / 64;
+    static_assert(T % 64 == 0 && P * T <= 1024 && P <= 16, "subgroups of whole waves");
+    const int sg = P == 1 ? 0 : (int)threadIdx.x / T;
+    const int t = (int)threadIdx.x - sg * T, lane = threadIdx.x & 63, w0 = sg * W;
+    const bool act = sg < nact;
+    const int u = u0 + (act ? sg : 0);
+    const int item = u / a.ncols, col = u - item * a.ncols;
+    const float *co = a.coef + (size_t)item * a.coef_stride;
+    float4 *pl = smem + (size_t)sg * (DBUF ? 4 : 2) * ng;
+    auto LA = [&](int i) -> float4 * { return pl + ((DBUF && i) ? 2 * ng : 0); };
+    auto LB = [&](int i) -> float4 * { return pl + ng + ((DBUF && i) ? 2 * ng : 0); };
+    double *ys = sh.ys + sg * kMaxObs, *yc = sh.yc + sg * kMaxObs;
+    float2 *s_coef = sh.coef + sg * kMaxKrylov;
     const int D = a.dim;
     const int D4 = D >> 2;
-    float *bas = a.basis + (size_t)blockIdx.x * a.m * 2 * D;
+    float *bas = bas0 + (size_t)sg * a.m * 2 * D;
     const int meff0 = a.m < D ? a.m : D;
     int par = 0;
     auto SCR = [&]() { par ^= 1; return scr + par * 256; };
+    fill_block_table(a, co, threadIdx.x, sh.blk);  // read after the first barrier below
+    __syncthreads();
 
-    for (;;) {
-        if (t == 0) s_unit = atomicAdd(a.queue, 1);
-        __syncthreads();
-        const int u = __builtin_amdgcn_readfirstlane(s_unit);
-        __syncthreads();
-        if (u >= a.n_units) break;
-        const int item = u / a.ncols, col = u - item * a.ncols;
-        const float *co = a.coef + (size_t)item * a.coef_stride;
-        fill_block_table(a, co, t, s_blk);  // read after the first barrier below
-        __syncthreads();
-
-        // ---- initial state of this column
-        C4 psi[G];
+    // ---- initial state of this column
+    C4 psi[G];
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const int nu = g * T + t;
-            if (a.init_mode == 0) {
-                const int which = col / a.zdim, ncol = col - which * a.zdim;
+    for (int g = 0; g < G; ++g) {
+        const int nu = g * T + t;
+        if (a.init_mode == 0) {
+            const int which = col / a.zdim, ncol = col - which * a.zdim;
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    psi[g].c[c] = (nu == ncol) ? make_float2(a.evec[which * 8 + 2 * c], a.evec[which * 8 + 2 * c + 1])
-                                               : make_float2(0.f, 0.f);
-            } else {
-                const float *src = a.init_mode == 1 ? a.given + (size_t)col * 2 * D : a.state + (size_t)u * 2 * D;
-                const float4 *sA = reinterpret_cast<const float4 *>(src);
-                psi[g] = load_group(sA, sA + D4, nu);
-            }
+            for (int c = 0; c < 4; ++c)
+                psi[g].c[c] = (nu == ncol) ? make_float2(a.evec[which * 8 + 2 * c], a.evec[which * 8 + 2 * c + 1])
+                                           : make_float2(0.f, 0.f);
+        } else {
+            const float *src = a.init_mode == 1 ? a.given + (size_t)col * 2 * D : a.state + (size_t)u * 2 * D;
+            const float4 *sA = reinterpret_cast<const float4 *>(src);
+            psi[g] = load_group(sA, sA + D4, nu);
         }
-        if (t == 0)
-            for (int c = 0; c < a.nobs; ++c) {
-                ys[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2] : 0.0;
-                yc[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2 + 1] : 0.0;
-            }
-        const double n0 = a.norm0 ? a.norm0[u] : 0.0;
+    }
+    if (t == 0)
+        for (int c = 0; c < a.nobs; ++c) {
+            ys[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2] : 0.0;
+            yc[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2 + 1] : 0.0;
+        }
+    const double n0 = a.norm0 ? a.norm0[u] : 0.0;
 
-        for (int s = a.s_begin;; ++s) {
-            const bool end = (s == a.s_end);
-            // ---- observables at sample s (HSObservableCollector::Evaluate)
-            if (!end || s == a.nsamples - 1) {
-                float Rf[16];
+    for (int s = a.s_begin;; ++s) {
+        const bool end = (s == a.s_end);
+        // ---- observables at sample s (HSObservableCollector::Evaluate)
+        if (!end || s == a.nsamples - 1) {
+            float Rf[16];
 #pragma unroll
-                for (int k = 0; k < 16; ++k) Rf[k] = 0.f;
+            for (int k = 0; k < 16; ++k) Rf[k] = 0.f;
 #pragma unroll
-                for (int g = 0; g < G; ++g) accum_rho(psi[g], Rf);
-                double R[16];
+            for (int g = 0; g < G; ++g) accum_rho(psi[g], Rf);
+            double R[16];
 #pragma unroll
-                for (int k = 0; k < 16; ++k) R[k] = Rf[k];
-                block_sum<16>(R, SCR());
-                if (t == 0) record_sample(a, u, s, R, ys, yc);
-            }
-            if (end) break;
-            const double dt = a.dts[s];
-            const float *hd = DYN ? a.dyn_head + ((size_t)item * (a.nsamples - 1) + s) * 8 : co;
-            // ---- relaxation half-step before (HSPropagator.cpp:137)
-            if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s])
+            for (int k = 0; k < 16; ++k) R[k] = Rf[k];
+            group_sum<16>(R, SCR(), W, w0);
+            if (t == 0 && act) record_sample(a, u, s, R, ys, yc);
+        }
+        if (end) break;
+        const double dt = a.dts[s];
+        const float *hd = DYN ? a.dyn_head + ((size_t)item * (a.nsamples - 1) + s) * 8 : co;
+        // ---- relaxation half-step before (HSPropagator.cpp:137)
+        if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s])
 #pragma unroll
-                for (int g = 0; g < G; ++g) st_flip(psi[g]);
+            for (int g = 0; g < G; ++g) st_flip(psi[g]);
 
-            // ---- Lanczos basis of H from psi (KrylovExpmGeneral, SpinSpace_operators.cpp:1257-1408)
-            double nn[1] = {0.0};
+        // ---- Lanczos basis of H from psi (KrylovExpmGeneral, SpinSpace_operators.cpp:1257-1408)
+        double nn[1] = {0.0};
 #pragma unroll
-            for (int g = 0; g < G; ++g)
+        for (int g = 0; g < G; ++g)
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    nn[0] += (double)psi[g].c[c].x * psi[g].c[c].x + (double)psi[g].c[c].y * psi[g].c[c].y;
-            block_sum<1>(nn, SCR());
-            const double beta0 = sqrt(nn[0]);
-            if (beta0 > 0.0) {
-                {
-                    const float inv = (float)(1.0 / beta0);
-                    float4 *bA = reinterpret_cast<float4 *>(bas);
+            for (int c = 0; c < 4; ++c)
+                nn[0] += (double)psi[g].c[c].x * psi[g].c[c].x + (double)psi[g].c[c].y * psi[g].c[c].y;
+        group_sum<1>(nn, SCR(), W, w0);
+        const double beta0 = sqrt(nn[0]);
+        const bool live = beta0 > 0.0;
+        bool anylive = live;
+        if constexpr (P > 1) anylive = __syncthreads_or(live) != 0;
+        if (anylive) {
+            {
+                const float inv = live ? (float)(1.0 / beta0) : 0.f;
+                float4 *bA = reinterpret_cast<float4 *>(bas);
 #pragma unroll
-                    for (int g = 0; g < G; ++g) {
-                        C4 v0;
+                for (int g = 0; g < G; ++g) {
+                    C4 v0;
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) v0.c[c] = make_float2(psi[g].c[c].x * inv, psi[g].c[c].y * inv);
-                        const int nu = g * T + t;
-                        store_group(LA(0), LB(0), nu, v0);
-                        store_group(bA, bA + D4, nu, v0);
+                    for (int c = 0; c < 4; ++c) v0.c[c] = make_float2(psi[g].c[c].x * inv, psi[g].c[c].y * inv);
+                    const int nu = g * T + t;
+                    store_group(LA(0), LB(0), nu, v0);
+                    store_group(bA, bA + D4, nu, v0);
+                }
+            }
+            double my_al = 0.0, my_be = 0.0;  // lane j keeps alpha_j and beta_j
+            double bprev = 0.0;
+            int meff = live ? meff0 : 1;
+            bool done = !live;
+            int cur = 0;
+            for (int j = 0; j < meff0; ++j) {
+                __syncthreads();  // v_j complete in buffer `cur`
+                const float4 *cA = LA(cur), *cB = LB(cur);
+                float4 *pA = LA(cur ^ 1), *pB = LB(cur ^ 1);
+                const float4 *gA = reinterpret_cast<const float4 *>(bas + (size_t)(j > 0 ? j - 1 : 0) * 2 * D);
+                const float fb = (float)bprev;
+                // w = H v_j - beta_{j-1} v_{j-1}; one reduction gives alpha and |w|^2
+                C4 w[G], vjs[G];
+                double red[2] = {0.0, 0.0};
+#pragma unroll
+                for (int g = 0; g < G; ++g) vjs[g] = load_group(cA, cB, g * T + t);
+                apply_h_groups<Log2<ng>::value, G, Log2<T>::value, true>(a, co, hd, t, vjs, cA, cB, w, 0, nullptr,
+                                                                         nullptr, sh.blk);
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    const int nu = g * T + t;
+                    const C4 &vj = vjs[g];
+                    if (j > 0) {
+                        const C4 vp = DBUF ? load_group(pA, pB, nu) : load_group(gA, gA + D4, nu);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            w[g].c[c].x = fmaf(-fb, vp.c[c].x, w[g].c[c].x);
+                            w[g].c[c].y = fmaf(-fb, vp.c[c].y, w[g].c[c].y);
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        red[0] += (double)vj.c[c].x * w[g].c[c].x + (double)vj.c[c].y * w[g].c[c].y;
+                        red[1] += (double)w[g].c[c].x * w[g].c[c].x + (double)w[g].c[c].y * w[g].c[c].y;
                     }
                 }
-                double my_al = 0.0, my_be = 0.0;  // lane j keeps alpha_j and beta_j
-                double bprev = 0.0;
-                int meff = meff0;
-                int cur = 0;
-                for (int j = 0; j < meff0; ++j) {
-                    __syncthreads();  // v_j complete in buffer `cur`
-                    const float4 *cA = LA(cur), *cB = LB(cur);
-                    float4 *pA = LA(cur ^ 1), *pB = LB(cur ^ 1);
-                    const float4 *gA = reinterpret_cast<const float4 *>(bas + (size_t)(j > 0 ? j - 1 : 0) * 2 * D);
-                    const float fb = (float)bprev;
-                    // w = H v_j - beta_{j-1} v_{j-1}; one reduction gives alpha and |w|^2
-                    C4 w[G], vjs[G];
-                    double red[2] = {0.0, 0.0};
+                double *rs = SCR();
+                group_sum<2>(red, rs, W, w0);
+                const double alpha = red[0];
+                if (lane == j && !done) my_al = alpha;
+                const float fa = (float)alpha;
 #pragma unroll
-                    for (int g = 0; g < G; ++g) vjs[g] = load_group(cA, cB, g * T + t);
-                    apply_h_groups<Log2<ng>::value, G, Log2<T>::value, true>(a, co, hd, t, vjs, cA, cB, w, 0,
-                                                                             nullptr, nullptr, s_blk);
+                for (int g = 0; g < G; ++g) {
+                    const C4 vj = load_group(cA, cB, g * T + t);
 #pragma unroll
-                    for (int g = 0; g < G; ++g) {
-                        const int nu = g * T + t;
-                        const C4 &vj = vjs[g];
-                        if (j > 0) {
-                            const C4 vp = DBUF ? load_group(pA, pB, nu) : load_group(gA, gA + D4, nu);
-#pragma unroll
-                            for (int c = 0; c < 4; ++c) {
-                                w[g].c[c].x = fmaf(-fb, vp.c[c].x, w[g].c[c].x);
-                                w[g].c[c].y = fmaf(-fb, vp.c[c].y, w[g].c[c].y);
-                            }
-                        }
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            red[0] += (double)vj.c[c].x * w[g].c[c].x + (double)vj.c[c].y * w[g].c[c].y;
-                            red[1] += (double)w[g].c[c].x * w[g].c[c].x + (double)w[g].c[c].y * w[g].c[c].y;
-                        }
+                    for (int c = 0; c < 4; ++c) {
+                        w[g].c[c].x = fmaf(-fa, vj.c[c].x, w[g].c[c].x);
+                        w[g].c[c].y = fmaf(-fa, vj.c[c].y, w[g].c[c].y);
                     }
-                    block_sum<2>(red, SCR());
-                    const double alpha = red[0];
-                    if (lane == j) my_al = alpha;
-                    const float fa = (float)alpha;
+                }
+                if (j == meff0 - 1) break;
+                // |w - alpha v|^2 = |w|^2 - alpha^2 for orthonormal v; recomputed when it cancels
+                double beta2 = red[1] - alpha * alpha;
+                const bool need = !(beta2 > 0.02 * red[1]);
+                bool anyneed = need;
+                if constexpr (P > 1) anyneed = any_guard(rs, W, P);
+                if (anyneed) {
+                    double bb[1] = {0.0};
 #pragma unroll
-                    for (int g = 0; g < G; ++g) {
-                        const C4 vj = load_group(cA, cB, g * T + t);
+                    for (int g = 0; g < G; ++g)
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            w[g].c[c].x = fmaf(-fa, vj.c[c].x, w[g].c[c].x);
-                            w[g].c[c].y = fmaf(-fa, vj.c[c].y, w[g].c[c].y);
-                        }
-                    }
-                    if (j == meff0 - 1) break;
-                    // |w - alpha v|^2 = |w|^2 - alpha^2 for orthonormal v; recomputed when it cancels
-                    double beta2 = red[1] - alpha * alpha;
-                    if (!(beta2 > 0.02 * red[1])) {
-                        double bb[1] = {0.0};
-#pragma unroll
-                        for (int g = 0; g < G; ++g)
-#pragma unroll
-                            for (int c = 0; c < 4; ++c)
-                                bb[0] += (double)w[g].c[c].x * w[g].c[c].x + (double)w[g].c[c].y * w[g].c[c].y;
-                        block_sum<1>(bb, SCR());
-                        beta2 = bb[0];
-                    }
-                    const double beta = sqrt(fmax(beta2, 0.0));
+                        for (int c = 0; c < 4; ++c)
+                            bb[0] += (double)w[g].c[c].x * w[g].c[c].x + (double)w[g].c[c].y * w[g].c[c].y;
+                    group_sum<1>(bb, SCR(), W, w0);
+                    if (need) beta2 = bb[0];
+                }
+                const double beta = sqrt(fmax(beta2, 0.0));
+                if constexpr (P == 1) {
                     if (!(beta > 1e-30)) {  // invariant Krylov subspace (happy breakdown)
                         meff = j + 1;
                         break;
                     }
+                } else if (!done && !(beta > 1e-30)) {
+                    meff = j + 1;
+                    done = true;
+                }
+                if (P == 1 || !done) {
                     if (lane == j) my_be = beta;
                     const float ib = (float)(1.0 / beta);
                     float4 *bA = reinterpret_cast<float4 *>(bas + (size_t)(j + 1) * 2 * D);
@@ -686,55 +760,79 @@ __global__ __launch_bounds__(TMAX, G >= 4 ? 2 : 4) void k_lanczos(const StepArgs
                         store_group(pA, pB, nu, w[g]);  // v_{j+1} replaces v_{j-1} (DBUF) or v_j (own elements only)
                         store_group(bA, bA + D4, nu, w[g]);
                     }
-                    bprev = beta;
-                    if (DBUF) cur ^= 1;
                 }
-                // wavefront 0 evaluates the small exponential; psi = beta * V c
-                // (beta = norm0 when renormalising the unitary part)
-                if (t < 64) {
-                    double cre, cim;
-                    cheb_expm_lanes(meff, dt, my_al, my_be, cre, cim, a.status);
-                    const double sc = a.renorm ? n0 : beta0;
-                    if (t < meff) s_coef[t] = make_float2((float)(sc * cre), (float)(sc * cim));
-                }
-                __syncthreads();
+                bprev = beta;
+                if (DBUF) cur ^= 1;
+            }
+            // wavefront 0 of the subgroup evaluates the small exponential; psi = beta V c
+            // (beta = norm0 when renormalising the unitary part)
+            if (t < 64) {
+                double cre, cim;
+                cheb_expm_lanes(meff, dt, my_al, my_be, cre, cim, a.status);
+                const double sc = live ? (a.renorm ? n0 : beta0) : 0.0;
+                if (t < meff) s_coef[t] = make_float2((float)(sc * cre), (float)(sc * cim));
+            }
+            __syncthreads();
 #pragma unroll
-                for (int g = 0; g < G; ++g)
+            for (int g = 0; g < G; ++g)
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) psi[g].c[c] = make_float2(0.f, 0.f);
-                for (int j = 0; j < meff; ++j) {
-                    const float2 cj = s_coef[j];
-                    const float4 *bA = reinterpret_cast<const float4 *>(bas + (size_t)j * 2 * D);
+                for (int c = 0; c < 4; ++c) psi[g].c[c] = make_float2(0.f, 0.f);
+            for (int j = 0; j < meff; ++j) {
+                const float2 cj = s_coef[j];
+                const float4 *bA = reinterpret_cast<const float4 *>(bas + (size_t)j * 2 * D);
 #pragma unroll
-                    for (int g = 0; g < G; ++g) {
-                        const C4 vj = load_group(bA, bA + D4, g * T + t);
+                for (int g = 0; g < G; ++g) {
+                    const C4 vj = load_group(bA, bA + D4, g * T + t);
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) psi[g].c[c] = cfma(cj, vj.c[c], psi[g].c[c]);
-                    }
+                    for (int c = 0; c < 4; ++c) psi[g].c[c] = cfma(cj, vj.c[c], psi[g].c[c]);
                 }
             }
-            // ---- relaxation half-step after (HSPropagator.cpp:139)
-            if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
-#pragma unroll
-                for (int g = 0; g < G; ++g) st_flip(psi[g]);
         }
-        // ---- results of this unit
-        if (t == 0)
-            for (int c = 0; c < a.nobs; ++c) {
-                a.yacc[((size_t)u * a.nobs + c) * 2] = ys[c];
-                a.yacc[((size_t)u * a.nobs + c) * 2 + 1] = yc[c];
-            }
-        if (a.state) {
-            float4 *dA = reinterpret_cast<float4 *>(a.state + (size_t)u * 2 * D);
+        // ---- relaxation half-step after (HSPropagator.cpp:139)
+        if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
 #pragma unroll
-            for (int g = 0; g < G; ++g) store_group(dA, dA + D4, g * T + t, psi[g]);
+            for (int g = 0; g < G; ++g) st_flip(psi[g]);
+    }
+    // ---- results of this column
+    if (t == 0 && act)
+        for (int c = 0; c < a.nobs; ++c) {
+            a.yacc[((size_t)u * a.nobs + c) * 2] = ys[c];
+            a.yacc[((size_t)u * a.nobs + c) * 2 + 1] = yc[c];
         }
-        bool bad = false;
+    if (a.state && act) {
+        float4 *dA = reinterpret_cast<float4 *>(a.state + (size_t)u * 2 * D);
 #pragma unroll
-        for (int g = 0; g < G; ++g)
+        for (int g = 0; g < G; ++g) store_group(dA, dA + D4, g * T + t, psi[g]);
+    }
+    bool bad = false;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) bad |= !isfinite(psi[g].c[c].x) || !isfinite(psi[g].c[c].y);
-        if (bad) atomicOr(a.status, 1);
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) bad |= !isfinite(psi[g].c[c].x) || !isfinite(psi[g].c[c].y);
+    if (bad && act) atomicOr(a.status, 1);
+}
+
+// One column per workgroup of TMAX threads, persistent over an atomic queue of
+// (item, column) units.
+template <int G, int TMAX, bool DBUF, bool DYN = false>
+__global__ __launch_bounds__(TMAX, G >= 4 ? 2 : 4) void k_lanczos(const StepArgs *__restrict__ pa) {
+    const StepArgs &a = *pa;
+    extern __shared__ float4 smem[];
+    constexpr int ng = G * TMAX;
+    double *scr = reinterpret_cast<double *>(smem + (DBUF ? 4 : 2) * ng);  // 2 x 16 x 16 doubles
+    __shared__ double ys[kMaxObs], yc[kMaxObs];
+    __shared__ float2 s_coef[kMaxKrylov];
+    __shared__ float4 s_blk[kMaxNuclei * 8];
+    __shared__ int s_unit;
+    const BodyShared sh{ys, yc, s_coef, s_blk, nullptr};
+    float *bas = a.basis + (size_t)blockIdx.x * a.m * 2 * a.dim;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        __syncthreads();
+        if (u >= a.n_units) break;
+        lanczos_columns<G, TMAX, 1, DBUF, DYN>(a, u, 1, bas, smem, scr, sh);
     }
 }
 
@@ -1020,6 +1118,241 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
 // global memory.
 // (Unrolling the group and nuclear loops at compile time was measured slower
 // here: these variants are bound by memory latency, not instruction issue.)
+// One column of the streaming Lanczos kernels (1024 threads; unit u = item * ncols +
+// column): the time loop with the Krylov basis, residual W and state P in the
+// workgroup's global scratch `bas` ((m + 2) vectors), `lvs` the dynamic LDS (the
+// mirror of v_j with LV, the tile of TILED), `scr` 2 x 17 x 16 doubles.
+template <bool LV, bool TILED>
+__device__ __forceinline__ void lanczos_big_column(const StepArgs &a, int u, float *bas, float4 *lvs, double *scr,
+                                                   const BodyShared &sh) {
+    constexpr int T = 1024;  // launched with exactly 1024 threads
+    const int t = threadIdx.x, lane = t & 63;
+    const int D = a.dim, D4 = D >> 2, ng = D >> 2;
+    float4 *WA = reinterpret_cast<float4 *>(bas + (size_t)a.m * 2 * D), *WB = WA + D4;
+    float4 *PA = reinterpret_cast<float4 *>(bas + (size_t)(a.m + 1) * 2 * D), *PB = PA + D4;
+    auto BA = [&](int j) { return reinterpret_cast<float4 *>(bas + (size_t)j * 2 * D); };
+    const int meff0 = a.m < D ? a.m : D;
+    int par = 0;
+    auto SCR = [&]() { par ^= 1; return scr + par * 17 * 16; };
+    const int item = u / a.ncols, col = u - item * a.ncols;
+    const float *co = a.coef + (size_t)item * a.coef_stride;
+    fill_block_table(a, co, t, sh.blk);  // read after the workgroup's next barrier
+    for (int nu = t; nu < ng; nu += T) {
+        C4 x;
+        if (a.init_mode == 0) {
+            const int which = col / a.zdim, ncol = col - which * a.zdim;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                x.c[c] = (nu == ncol) ? make_float2(a.evec[which * 8 + 2 * c], a.evec[which * 8 + 2 * c + 1])
+                                      : make_float2(0.f, 0.f);
+        } else {
+            const float *src = a.init_mode == 1 ? a.given + (size_t)col * 2 * D : a.state + (size_t)u * 2 * D;
+            const float4 *sA = reinterpret_cast<const float4 *>(src);
+            x = load_group(sA, sA + D4, nu);
+        }
+        store_group(PA, PB, nu, x);
+    }
+    if (t == 0)
+        for (int c = 0; c < a.nobs; ++c) {
+            sh.ys[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2] : 0.0;
+            sh.yc[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2 + 1] : 0.0;
+        }
+    const double n0 = a.norm0 ? a.norm0[u] : 0.0;
+    for (int s = a.s_begin;; ++s) {
+        const bool end = (s == a.s_end);
+        const bool flip0 = !end && a.relax &&
+                           philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s];
+        // observables at sample s, then the first relaxation half-step and |psi|^2
+        {
+            float Rf[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) Rf[k] = 0.f;
+            double nn = 0.0;
+            for (int nu = t; nu < ng; nu += T) {
+                C4 x = load_group(PA, PB, nu);
+                accum_rho(x, Rf);
+                if (flip0) {
+                    st_flip(x);
+                    store_group(PA, PB, nu, x);
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c) nn += (double)x.c[c].x * x.c[c].x + (double)x.c[c].y * x.c[c].y;
+            }
+            double R[17];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) R[k] = Rf[k];
+            R[16] = nn;
+            block_sum<17>(R, SCR());
+            if (t == 0 && (!end || s == a.nsamples - 1)) record_sample(a, u, s, R, sh.ys, sh.yc);
+            if (end) break;
+            const double dt = a.dts[s];
+            const float *hd = a.dyn_head ? a.dyn_head + ((size_t)item * (a.nsamples - 1) + s) * 8 : co;
+            const double beta0 = sqrt(R[16]);
+            if (beta0 > 0.0) {
+                const float inv = (float)(1.0 / beta0);
+                for (int nu = t; nu < ng; nu += T) {
+                    C4 x = load_group(PA, PB, nu);
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) x.c[c] = make_float2(x.c[c].x * inv, x.c[c].y * inv);
+                    store_group(BA(0), BA(0) + D4, nu, x);
+                    if (LV) store_group(lvs, lvs + D4, nu, x);
+                }
+                double my_al = 0.0, my_be = 0.0, bprev = 0.0;
+                int meff = meff0;
+                for (int j = 0; j < meff0; ++j) {
+                    __syncthreads();  // v_j written by the whole workgroup
+                    const float4 *vA = LV ? lvs : BA(j), *vB = vA + D4;
+                    const float4 *pA = BA(j > 0 ? j - 1 : 0), *pB = pA + D4;
+                    const float fb = (float)bprev;
+                    double red[2] = {0.0, 0.0};
+                    // LV (D = 16384, four groups per thread): the residual stays in
+                    // registers; otherwise it goes through the per-slot scratch W
+                    C4 wr[LV ? 4 : 1];
+                    auto pass_a = [&](int nu, const C4 &vj, const C4 &w0, C4 *keep) {
+                        C4 w = w0;
+                        if (j > 0) {
+                            const C4 vp = load_group(pA, pB, nu);
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                w.c[c].x = fmaf(-fb, vp.c[c].x, w.c[c].x);
+                                w.c[c].y = fmaf(-fb, vp.c[c].y, w.c[c].y);
+                            }
+                        }
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            red[0] += (double)vj.c[c].x * w.c[c].x + (double)vj.c[c].y * w.c[c].y;
+                            red[1] += (double)w.c[c].x * w.c[c].x + (double)w.c[c].y * w.c[c].y;
+                        }
+                        if (keep) *keep = w;
+                        else store_group(WA, WB, nu, w);
+                    };
+                    if (TILED) {
+                        float4 *tA = lvs, *tB = lvs + kTileG;
+                        for (int tb = 0; tb < ng; tb += kTileG) {
+                            if (tb > 0) __syncthreads();  // the previous tile is no longer read
+                            for (int k = t; k < kTileG; k += T) {
+                                tA[k] = vA[tb + k];
+                                tB[k] = vB[tb + k];
+                            }
+                            __syncthreads();
+                            for (int k = t; k < kTileG; k += T) {
+                                const int nu = tb + k;
+                                const C4 vj = load_group(tA, tB, k);
+                                pass_a(nu, vj, apply_h<-1, true, true>(a, co, hd, nu, vj, tA, tB, vA, vB, sh.blk), nullptr);
+                            }
+                        }
+                    } else if (LV) {
+#pragma unroll
+                        for (int g = 0; g < (LV ? 4 : 1); ++g) {
+                            const int nu = g * T + t;
+                            const C4 vj = load_group(vA, vB, nu);
+                            pass_a(nu, vj, apply_h<-1, false, true>(a, co, hd, nu, vj, vA, vB, nullptr, nullptr, sh.blk), &wr[g]);
+                        }
+                    } else {
+                        for (int nu = t; nu < ng; nu += T) {
+                            const C4 vj = load_group(vA, vB, nu);
+                            pass_a(nu, vj, apply_h<-1, false, true>(a, co, hd, nu, vj, vA, vB, nullptr, nullptr, sh.blk), nullptr);
+                        }
+                    }
+                    block_sum<2>(red, SCR());
+                    const double alpha = red[0];
+                    if (lane == j) my_al = alpha;
+                    if (j == meff0 - 1) break;
+                    const float fa = (float)alpha;
+                    double beta2 = red[1] - alpha * alpha;
+                    if (!(beta2 > 0.02 * red[1])) {
+                        double bb[1] = {0.0};
+                        auto acc_bb = [&](const C4 &vj, const C4 &w) {
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                const float wx = fmaf(-fa, vj.c[c].x, w.c[c].x), wy = fmaf(-fa, vj.c[c].y, w.c[c].y);
+                                bb[0] += (double)wx * wx + (double)wy * wy;
+                            }
+                        };
+                        if (LV) {
+#pragma unroll
+                            for (int g = 0; g < (LV ? 4 : 1); ++g) acc_bb(load_group(vA, vB, g * T + t), wr[g]);
+                        } else {
+                            for (int nu = t; nu < ng; nu += T) acc_bb(load_group(vA, vB, nu), load_group(WA, WB, nu));
+                        }
+                        block_sum<1>(bb, SCR());
+                        beta2 = bb[0];
+                    }
+                    const double beta = sqrt(fmax(beta2, 0.0));
+                    if (!(beta > 1e-30)) {
+                        meff = j + 1;
+                        break;
+                    }
+                    if (lane == j) my_be = beta;
+                    const float ib = (float)(1.0 / beta);
+                    float4 *nA = BA(j + 1), *nB = nA + D4;
+                    auto pass_b = [&](int nu, C4 w) {
+                        const C4 vj = load_group(vA, vB, nu);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            w.c[c] = make_float2(fmaf(-fa, vj.c[c].x, w.c[c].x) * ib, fmaf(-fa, vj.c[c].y, w.c[c].y) * ib);
+                        store_group(nA, nB, nu, w);
+                        if (LV) store_group(lvs, lvs + D4, nu, w);  // own elements: read above by this thread only
+                    };
+                    if (LV) {
+#pragma unroll
+                        for (int g = 0; g < (LV ? 4 : 1); ++g) pass_b(g * T + t, wr[g]);
+                    } else {
+                        for (int nu = t; nu < ng; nu += T) pass_b(nu, load_group(WA, WB, nu));
+                    }
+                    bprev = beta;
+                }
+                if (t < 64) {  // wavefront 0 evaluates the small exponential
+                    double cre, cim;
+                    cheb_expm_lanes(meff, dt, my_al, my_be, cre, cim, a.status);
+                    const double sc = a.renorm ? n0 : beta0;
+                    if (t < meff) sh.coef[t] = make_float2((float)(sc * cre), (float)(sc * cim));
+                }
+                __syncthreads();
+                for (int nu = t; nu < ng; nu += T) {
+                    C4 x;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) x.c[c] = make_float2(0.f, 0.f);
+                    for (int j = 0; j < meff; ++j) {
+                        const float2 cj = sh.coef[j];
+                        const C4 vj = load_group(BA(j), BA(j) + D4, nu);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) x.c[c] = cfma(cj, vj.c[c], x.c[c]);
+                    }
+                    if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
+                        st_flip(x);
+                    store_group(PA, PB, nu, x);
+                }
+            } else if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) <
+                                      a.thr[s]) {
+                for (int nu = t; nu < ng; nu += T) {
+                    C4 x = load_group(PA, PB, nu);
+                    st_flip(x);
+                    store_group(PA, PB, nu, x);
+                }
+            }
+            __syncthreads();  // psi complete before the next sample
+        }
+    }
+    if (t == 0)
+        for (int c = 0; c < a.nobs; ++c) {
+            a.yacc[((size_t)u * a.nobs + c) * 2] = sh.ys[c];
+            a.yacc[((size_t)u * a.nobs + c) * 2 + 1] = sh.yc[c];
+        }
+    bool bad = false;
+    for (int nu = t; nu < ng; nu += T) {
+        const C4 x = load_group(PA, PB, nu);
+        if (a.state) {
+            float4 *dA = reinterpret_cast<float4 *>(a.state + (size_t)u * 2 * D);
+            store_group(dA, dA + D4, nu, x);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) bad |= !isfinite(x.c[c].x) || !isfinite(x.c[c].y);
+    }
+    if (bad) atomicOr(a.status, 1);
+    __syncthreads();
+}
+
 template <bool LV, bool TILED = false>
 __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict__ pa) {
     const StepArgs &a = *pa;
@@ -1029,240 +1362,16 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
     __shared__ float2 s_coef[kMaxKrylov];
     __shared__ float4 s_blk[kMaxNuclei * 8];  // coupling blocks, both orientations
     __shared__ int s_unit;
-    constexpr int T = 1024;  // launched with exactly 1024 threads
-    const int t = threadIdx.x, lane = t & 63;
-    const int D = a.dim, D4 = D >> 2, ng = D >> 2;
+    const BodyShared sh{ys, yc, s_coef, s_blk, nullptr};
     // per-slot scratch: basis (m vectors), residual W, state P
-    float *bas = a.basis + (size_t)blockIdx.x * (a.m + 2) * 2 * D;
-    float4 *WA = reinterpret_cast<float4 *>(bas + (size_t)a.m * 2 * D), *WB = WA + D4;
-    float4 *PA = reinterpret_cast<float4 *>(bas + (size_t)(a.m + 1) * 2 * D), *PB = PA + D4;
-    auto BA = [&](int j) { return reinterpret_cast<float4 *>(bas + (size_t)j * 2 * D); };
-    const int meff0 = a.m < D ? a.m : D;
-    int par = 0;
-    auto SCR = [&]() { par ^= 1; return scr + par * 17 * 16; };
+    float *bas = a.basis + (size_t)blockIdx.x * (a.m + 2) * 2 * a.dim;
     for (;;) {
-        if (t == 0) s_unit = atomicAdd(a.queue, 1);
+        if (threadIdx.x == 0) s_unit = atomicAdd(a.queue, 1);
         __syncthreads();
         const int u = __builtin_amdgcn_readfirstlane(s_unit);
         __syncthreads();
         if (u >= a.n_units) break;
-        const int item = u / a.ncols, col = u - item * a.ncols;
-        const float *co = a.coef + (size_t)item * a.coef_stride;
-        fill_block_table(a, co, t, s_blk);  // read after the workgroup's next barrier
-        for (int nu = t; nu < ng; nu += T) {
-            C4 x;
-            if (a.init_mode == 0) {
-                const int which = col / a.zdim, ncol = col - which * a.zdim;
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    x.c[c] = (nu == ncol) ? make_float2(a.evec[which * 8 + 2 * c], a.evec[which * 8 + 2 * c + 1])
-                                          : make_float2(0.f, 0.f);
-            } else {
-                const float *src = a.init_mode == 1 ? a.given + (size_t)col * 2 * D : a.state + (size_t)u * 2 * D;
-                const float4 *sA = reinterpret_cast<const float4 *>(src);
-                x = load_group(sA, sA + D4, nu);
-            }
-            store_group(PA, PB, nu, x);
-        }
-        if (t == 0)
-            for (int c = 0; c < a.nobs; ++c) {
-                ys[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2] : 0.0;
-                yc[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2 + 1] : 0.0;
-            }
-        const double n0 = a.norm0 ? a.norm0[u] : 0.0;
-        for (int s = a.s_begin;; ++s) {
-            const bool end = (s == a.s_end);
-            const bool flip0 = !end && a.relax &&
-                               philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s];
-            // observables at sample s, then the first relaxation half-step and |psi|^2
-            {
-                float Rf[16];
-#pragma unroll
-                for (int k = 0; k < 16; ++k) Rf[k] = 0.f;
-                double nn = 0.0;
-                for (int nu = t; nu < ng; nu += T) {
-                    C4 x = load_group(PA, PB, nu);
-                    accum_rho(x, Rf);
-                    if (flip0) {
-                        st_flip(x);
-                        store_group(PA, PB, nu, x);
-                    }
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) nn += (double)x.c[c].x * x.c[c].x + (double)x.c[c].y * x.c[c].y;
-                }
-                double R[17];
-#pragma unroll
-                for (int k = 0; k < 16; ++k) R[k] = Rf[k];
-                R[16] = nn;
-                block_sum<17>(R, SCR());
-                if (t == 0 && (!end || s == a.nsamples - 1)) record_sample(a, u, s, R, ys, yc);
-                if (end) break;
-                const double dt = a.dts[s];
-                const float *hd = a.dyn_head ? a.dyn_head + ((size_t)item * (a.nsamples - 1) + s) * 8 : co;
-                const double beta0 = sqrt(R[16]);
-                if (beta0 > 0.0) {
-                    const float inv = (float)(1.0 / beta0);
-                    for (int nu = t; nu < ng; nu += T) {
-                        C4 x = load_group(PA, PB, nu);
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) x.c[c] = make_float2(x.c[c].x * inv, x.c[c].y * inv);
-                        store_group(BA(0), BA(0) + D4, nu, x);
-                        if (LV) store_group(lvs, lvs + D4, nu, x);
-                    }
-                    double my_al = 0.0, my_be = 0.0, bprev = 0.0;
-                    int meff = meff0;
-                    for (int j = 0; j < meff0; ++j) {
-                        __syncthreads();  // v_j written by the whole workgroup
-                        const float4 *vA = LV ? lvs : BA(j), *vB = vA + D4;
-                        const float4 *pA = BA(j > 0 ? j - 1 : 0), *pB = pA + D4;
-                        const float fb = (float)bprev;
-                        double red[2] = {0.0, 0.0};
-                        // LV (D = 16384, four groups per thread): the residual stays in
-                        // registers; otherwise it goes through the per-slot scratch W
-                        C4 wr[LV ? 4 : 1];
-                        auto pass_a = [&](int nu, const C4 &vj, const C4 &w0, C4 *keep) {
-                            C4 w = w0;
-                            if (j > 0) {
-                                const C4 vp = load_group(pA, pB, nu);
-#pragma unroll
-                                for (int c = 0; c < 4; ++c) {
-                                    w.c[c].x = fmaf(-fb, vp.c[c].x, w.c[c].x);
-                                    w.c[c].y = fmaf(-fb, vp.c[c].y, w.c[c].y);
-                                }
-                            }
-#pragma unroll
-                            for (int c = 0; c < 4; ++c) {
-                                red[0] += (double)vj.c[c].x * w.c[c].x + (double)vj.c[c].y * w.c[c].y;
-                                red[1] += (double)w.c[c].x * w.c[c].x + (double)w.c[c].y * w.c[c].y;
-                            }
-                            if (keep) *keep = w;
-                            else store_group(WA, WB, nu, w);
-                        };
-                        if (TILED) {
-                            float4 *tA = lvs, *tB = lvs + kTileG;
-                            for (int tb = 0; tb < ng; tb += kTileG) {
-                                if (tb > 0) __syncthreads();  // the previous tile is no longer read
-                                for (int k = t; k < kTileG; k += T) {
-                                    tA[k] = vA[tb + k];
-                                    tB[k] = vB[tb + k];
-                                }
-                                __syncthreads();
-                                for (int k = t; k < kTileG; k += T) {
-                                    const int nu = tb + k;
-                                    const C4 vj = load_group(tA, tB, k);
-                                    pass_a(nu, vj, apply_h<-1, true, true>(a, co, hd, nu, vj, tA, tB, vA, vB, s_blk), nullptr);
-                                }
-                            }
-                        } else if (LV) {
-#pragma unroll
-                            for (int g = 0; g < (LV ? 4 : 1); ++g) {
-                                const int nu = g * T + t;
-                                const C4 vj = load_group(vA, vB, nu);
-                                pass_a(nu, vj, apply_h<-1, false, true>(a, co, hd, nu, vj, vA, vB, nullptr, nullptr, s_blk), &wr[g]);
-                            }
-                        } else {
-                            for (int nu = t; nu < ng; nu += T) {
-                                const C4 vj = load_group(vA, vB, nu);
-                                pass_a(nu, vj, apply_h<-1, false, true>(a, co, hd, nu, vj, vA, vB, nullptr, nullptr, s_blk), nullptr);
-                            }
-                        }
-                        block_sum<2>(red, SCR());
-                        const double alpha = red[0];
-                        if (lane == j) my_al = alpha;
-                        if (j == meff0 - 1) break;
-                        const float fa = (float)alpha;
-                        double beta2 = red[1] - alpha * alpha;
-                        if (!(beta2 > 0.02 * red[1])) {
-                            double bb[1] = {0.0};
-                            auto acc_bb = [&](const C4 &vj, const C4 &w) {
-#pragma unroll
-                                for (int c = 0; c < 4; ++c) {
-                                    const float wx = fmaf(-fa, vj.c[c].x, w.c[c].x), wy = fmaf(-fa, vj.c[c].y, w.c[c].y);
-                                    bb[0] += (double)wx * wx + (double)wy * wy;
-                                }
-                            };
-                            if (LV) {
-#pragma unroll
-                                for (int g = 0; g < (LV ? 4 : 1); ++g) acc_bb(load_group(vA, vB, g * T + t), wr[g]);
-                            } else {
-                                for (int nu = t; nu < ng; nu += T) acc_bb(load_group(vA, vB, nu), load_group(WA, WB, nu));
-                            }
-                            block_sum<1>(bb, SCR());
-                            beta2 = bb[0];
-                        }
-                        const double beta = sqrt(fmax(beta2, 0.0));
-                        if (!(beta > 1e-30)) {
-                            meff = j + 1;
-                            break;
-                        }
-                        if (lane == j) my_be = beta;
-                        const float ib = (float)(1.0 / beta);
-                        float4 *nA = BA(j + 1), *nB = nA + D4;
-                        auto pass_b = [&](int nu, C4 w) {
-                            const C4 vj = load_group(vA, vB, nu);
-#pragma unroll
-                            for (int c = 0; c < 4; ++c)
-                                w.c[c] = make_float2(fmaf(-fa, vj.c[c].x, w.c[c].x) * ib, fmaf(-fa, vj.c[c].y, w.c[c].y) * ib);
-                            store_group(nA, nB, nu, w);
-                            if (LV) store_group(lvs, lvs + D4, nu, w);  // own elements: read above by this thread only
-                        };
-                        if (LV) {
-#pragma unroll
-                            for (int g = 0; g < (LV ? 4 : 1); ++g) pass_b(g * T + t, wr[g]);
-                        } else {
-                            for (int nu = t; nu < ng; nu += T) pass_b(nu, load_group(WA, WB, nu));
-                        }
-                        bprev = beta;
-                    }
-                    if (t < 64) {  // wavefront 0 evaluates the small exponential
-                        double cre, cim;
-                        cheb_expm_lanes(meff, dt, my_al, my_be, cre, cim, a.status);
-                        const double sc = a.renorm ? n0 : beta0;
-                        if (t < meff) s_coef[t] = make_float2((float)(sc * cre), (float)(sc * cim));
-                    }
-                    __syncthreads();
-                    for (int nu = t; nu < ng; nu += T) {
-                        C4 x;
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) x.c[c] = make_float2(0.f, 0.f);
-                        for (int j = 0; j < meff; ++j) {
-                            const float2 cj = s_coef[j];
-                            const C4 vj = load_group(BA(j), BA(j) + D4, nu);
-#pragma unroll
-                            for (int c = 0; c < 4; ++c) x.c[c] = cfma(cj, vj.c[c], x.c[c]);
-                        }
-                        if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
-                            st_flip(x);
-                        store_group(PA, PB, nu, x);
-                    }
-                } else if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) <
-                                          a.thr[s]) {
-                    for (int nu = t; nu < ng; nu += T) {
-                        C4 x = load_group(PA, PB, nu);
-                        st_flip(x);
-                        store_group(PA, PB, nu, x);
-                    }
-                }
-                __syncthreads();  // psi complete before the next sample
-            }
-        }
-        if (t == 0)
-            for (int c = 0; c < a.nobs; ++c) {
-                a.yacc[((size_t)u * a.nobs + c) * 2] = ys[c];
-                a.yacc[((size_t)u * a.nobs + c) * 2 + 1] = yc[c];
-            }
-        bool bad = false;
-        for (int nu = t; nu < ng; nu += T) {
-            const C4 x = load_group(PA, PB, nu);
-            if (a.state) {
-                float4 *dA = reinterpret_cast<float4 *>(a.state + (size_t)u * 2 * D);
-                store_group(dA, dA + D4, nu, x);
-            }
-#pragma unroll
-            for (int c = 0; c < 4; ++c) bad |= !isfinite(x.c[c].x) || !isfinite(x.c[c].y);
-        }
-        if (bad) atomicOr(a.status, 1);
-        __syncthreads();
+        lanczos_big_column<LV, TILED>(a, u, bas, lvs, scr, sh);
     }
 }
 
@@ -1636,28 +1745,32 @@ __device__ __forceinline__ float2 dense_init(const StepArgs &a, int u, int col, 
 // per wave keep the three products' accumulators in registers), else 4.
 constexpr int dense_waves(int NT) { return NT >= 12 ? 8 : 4; }
 
-// NT: row tiles of 16 (padded dimension Dp = 16 NT, a compile-time constant so the
-// product loop and the per-wave tile loops unroll); NW = dense_waves(NT) waves, each
-// owning tiles wave + NW k.
-template <int NT>
-__global__ __launch_bounds__(64 * dense_waves(NT)) void k_dense(const StepArgs *__restrict__ pa) {
-    const StepArgs &a = *pa;
-    extern __shared__ float dsm[];
-    __shared__ double ys[kMaxObs], yc[kMaxObs];  // Kahan yield sums (thread 0)
-    __shared__ double s_col[2][16 * 8];          // per-wave column norms (double-buffered)
+// Dense-path propagation of P 16-column tiles of one item, one per subgroup of NW
+// waves (unit u = item * tiles_per_item + tile; units u0 .. u0 + nact - 1 live,
+// subgroups from nact up repeat u0 and discard their results).  NT: row tiles of 16
+// (padded dimension Dp = 16 NT, a compile-time constant so the product loop and the
+// per-wave tile loops unroll); wave w of a subgroup owns row tiles w + NW k.  `dsm`
+// holds per subgroup the B tile planes (re, im, re + im), `scr` 2 x 256 doubles.
+template <int NT, int NW, int P>
+__device__ __forceinline__ void dense_tiles(const StepArgs &a, int u0, int nact, float *dsm, double *scr,
+                                            const BodyShared &sh) {
     constexpr int Dp = 16 * NT;
-    constexpr int NW = dense_waves(NT);
     constexpr int KT = (NT + NW - 1) / NW;  // row tiles per wave
-    float *Bre = dsm, *Bim = dsm + Dp * 16, *Bsm = dsm + 2 * Dp * 16;  // B: re, im, re + im
-    double *scr = reinterpret_cast<double *>(dsm + kDenseLdsPlanes * Dp * 16);
-    __shared__ int s_unit;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    static_assert(NW * P <= 16, "at most 16 waves per workgroup");
+    const int sg = P == 1 ? 0 : (int)threadIdx.x / (64 * NW);
+    const int t = (int)threadIdx.x - sg * 64 * NW, lane = t & 63, wave = t >> 6, gwave = threadIdx.x >> 6;
+    const int w0 = sg * NW;
+    const bool act = sg < nact;
+    const int u = u0 + (act ? sg : 0);
+    float *Bre = dsm + (size_t)sg * kDenseLdsPlanes * Dp * 16;
+    float *Bim = Bre + Dp * 16, *Bsm = Bre + 2 * Dp * 16;  // B: re, im, re + im
+    double *ys = sh.ys + sg * kMaxObs, *yc = sh.yc + sg * kMaxObs;
     const int q = lane >> 4, cl = lane & 15;
     constexpr int ntile = NT;
     const int tiles_per_item = (a.ncols + 15) >> 4;
     int par = 0, cpar = 0;
     auto SCR = [&]() { par ^= 1; return scr + par * 256; };
-    // squared norm of this lane's column: lanes cl + 16 q of every wave hold its rows
+    // squared norm of this lane's column: lanes cl + 16 q of every wave of the subgroup hold its rows
     auto colnorm2 = [&](const floatx4 *xr_, const floatx4 *xi_) -> double {
         float p = 0.f;
 #pragma unroll
@@ -1668,190 +1781,204 @@ __global__ __launch_bounds__(64 * dense_waves(NT)) void k_dense(const StepArgs *
         d += __shfl_xor(d, 16, 64);
         d += __shfl_xor(d, 32, 64);
         cpar ^= 1;
-        if (q == 0) s_col[cpar][wave * 16 + cl] = d;
+        if (q == 0) sh.colsum[cpar * 256 + gwave * 16 + cl] = d;
         __syncthreads();
         double sum = 0.0;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) sum += s_col[cpar][16 * w + cl];
+        for (int w = 0; w < NW; ++w) sum += sh.colsum[cpar * 256 + (w0 + w) * 16 + cl];
         return sum;
     };
-    for (;;) {
-        if (t == 0) s_unit = atomicAdd(a.queue, 1);
-        __syncthreads();
-        const int u = __builtin_amdgcn_readfirstlane(s_unit);
-        __syncthreads();
-        if (u >= a.n_units) break;
-        const int item = u / tiles_per_item, tile = u - item * tiles_per_item;
-        const int col = tile * 16 + cl;  // the column this lane's accumulators hold
-        // register copy of the state in MFMA output layout
-        floatx4 xr[KT], xi[KT];
+    const int item = u / tiles_per_item, tile = u - item * tiles_per_item;
+    const int col = tile * 16 + cl;  // the column this lane's accumulators hold
+    // register copy of the state in MFMA output layout
+    floatx4 xr[KT], xi[KT];
 #pragma unroll
-        for (int k = 0; k < KT; ++k) {
-            const int rt = wave + NW * k;
+    for (int k = 0; k < KT; ++k) {
+        const int rt = wave + NW * k;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const float2 v = rt < ntile ? dense_init(a, u, col, rt * 16 + 4 * q + i) : make_float2(0.f, 0.f);
-                xr[k][i] = v.x;
-                xi[k][i] = v.y;
-            }
+        for (int i = 0; i < 4; ++i) {
+            const float2 v = rt < ntile ? dense_init(a, u, col, rt * 16 + 4 * q + i) : make_float2(0.f, 0.f);
+            xr[k][i] = v.x;
+            xi[k][i] = v.y;
         }
-        if (t == 0)
-            for (int c = 0; c < a.nobs; ++c) {
-                ys[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2] : 0.0;
-                yc[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2 + 1] : 0.0;
-            }
-        const bool colok = col < a.ncols;
-        // target norm of the column for the renormalised unitary part (symmetric recombination)
-        const double cn0 = a.renorm ? sqrt(colnorm2(xr, xi)) : 0.0;
-        for (int s = a.s_begin;; ++s) {
-            const bool end = s == a.s_end;
-            if (!end || s == a.nsamples - 1) {
-                float Rf[16];
-#pragma unroll
-                for (int k = 0; k < 16; ++k) Rf[k] = 0.f;
-#pragma unroll
-                for (int k = 0; k < KT; ++k) {
-                    C4 g;
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) g.c[i] = make_float2(xr[k][i], xi[k][i]);
-                    accum_rho(g, Rf);
-                }
-                double R[16];
-#pragma unroll
-                for (int k = 0; k < 16; ++k) R[k] = Rf[k];
-                block_sum<16>(R, SCR());
-                if (t == 0) record_sample(a, u, s, R, ys, yc);
-            }
-            if (end) break;
-            if (a.relax && colok && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s])
-#pragma unroll
-                for (int k = 0; k < KT; ++k) {
-                    C4 g;
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) g.c[i] = make_float2(xr[k][i], xi[k][i]);
-                    st_flip(g);
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) { xr[k][i] = g.c[i].x; xi[k][i] = g.c[i].y; }
-                }
-            // publish B to LDS ([row][16 columns] planes)
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < KT; ++k) {
-                const int rt = wave + NW * k;
-                if (rt < ntile)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        Bre[(rt * 16 + 4 * q + i) * 16 + cl] = xr[k][i];
-                        Bim[(rt * 16 + 4 * q + i) * 16 + cl] = xi[k][i];
-                        Bsm[(rt * 16 + 4 * q + i) * 16 + cl] = xr[k][i] + xi[k][i];
-                    }
-            }
-            __syncthreads();
-            // B' = U B as three real products on the matrix cores (Gauss):
-            // P1 = Ur Br, P2 = Ui Bi, P3 = (Ur + Ui)(Br + Bi); re = P1 - P2, im = P3 - P1 - P2
-            const int uw = a.ustep ? s - a.s_begin : ((a.ulast && s == a.nsamples - 2) ? 1 : 0);
-            const float *Ur = a.U + ((size_t)item * a.ustride + uw) * kDensePlanes * Dp * Dp;
-            const float *Ui = Ur + (size_t)Dp * Dp;
-            floatx4 x2[KT];  // P2 (xr holds P1, xi P3)
-#pragma unroll
-            for (int k = 0; k < KT; ++k) {
-                xr[k] = floatx4{0.f, 0.f, 0.f, 0.f};
-                xi[k] = floatx4{0.f, 0.f, 0.f, 0.f};
-                x2[k] = floatx4{0.f, 0.f, 0.f, 0.f};
-            }
-            // k-groups of 16: one float4 per lane and plane carries four k-steps of A;
-            // the A operands of group g + 1 are fetched while group g multiplies
-            float4 nr4[KT], ni4[KT];
-#pragma unroll
-            for (int k = 0; k < KT; ++k) {
-                const int rt = wave + NW * k;
-                const size_t o = ((size_t)(rt * ntile) * 64 + lane) * 4;
-                nr4[k] = rt < ntile ? *reinterpret_cast<const float4 *>(Ur + o) : float4{0.f, 0.f, 0.f, 0.f};
-                ni4[k] = rt < ntile ? *reinterpret_cast<const float4 *>(Ui + o) : float4{0.f, 0.f, 0.f, 0.f};
-            }
-            for (int g = 0; g < ntile; ++g) {
-                float br[4], bi[4], bs[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int kr = 16 * g + 4 * u + q;
-                    br[u] = Bre[kr * 16 + cl];
-                    bi[u] = Bim[kr * 16 + cl];
-                    bs[u] = Bsm[kr * 16 + cl];
-                }
-                float4 cr4[KT], ci4[KT];
-#pragma unroll
-                for (int k = 0; k < KT; ++k) {
-                    cr4[k] = nr4[k];
-                    ci4[k] = ni4[k];
-                    const int rt = wave + NW * k;
-                    if (rt < ntile && g + 1 < ntile) {
-                        const size_t o = ((size_t)(rt * ntile + g + 1) * 64 + lane) * 4;
-                        nr4[k] = *reinterpret_cast<const float4 *>(Ur + o);
-                        ni4[k] = *reinterpret_cast<const float4 *>(Ui + o);
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < KT; ++k) {
-                    const int rt = wave + NW * k;
-                    if (rt < ntile) {
-                        const float4 ar4 = cr4[k], ai4 = ci4[k];
-                        const float ar[4] = {ar4.x, ar4.y, ar4.z, ar4.w}, ai[4] = {ai4.x, ai4.y, ai4.z, ai4.w};
-                        const float as[4] = {ar4.x + ai4.x, ar4.y + ai4.y, ar4.z + ai4.z, ar4.w + ai4.w};
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            xr[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[u], br[u], xr[k], 0, 0, 0);
-                            x2[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[u], bi[u], x2[k], 0, 0, 0);
-                            xi[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(as[u], bs[u], xi[k], 0, 0, 0);
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < KT; ++k) {
-                xi[k] = xi[k] - xr[k] - x2[k];
-                xr[k] = xr[k] - x2[k];
-            }
-            if (a.renorm) {
-                const double n2 = colnorm2(xr, xi);
-                const float f = n2 > 0.0 ? (float)(cn0 / sqrt(n2)) : 0.f;
-#pragma unroll
-                for (int k = 0; k < KT; ++k) {
-                    xr[k] *= f;
-                    xi[k] *= f;
-                }
-            }
-            if (a.relax && colok && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
-#pragma unroll
-                for (int k = 0; k < KT; ++k) {
-                    C4 g;
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) g.c[i] = make_float2(xr[k][i], xi[k][i]);
-                    st_flip(g);
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) { xr[k][i] = g.c[i].x; xi[k][i] = g.c[i].y; }
-                }
+    }
+    if (t == 0)
+        for (int c = 0; c < a.nobs; ++c) {
+            ys[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2] : 0.0;
+            yc[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2 + 1] : 0.0;
         }
-        if (t == 0)
-            for (int c = 0; c < a.nobs; ++c) {
-                a.yacc[((size_t)u * a.nobs + c) * 2] = ys[c];
-                a.yacc[((size_t)u * a.nobs + c) * 2 + 1] = yc[c];
+    const bool colok = col < a.ncols;
+    // target norm of the column for the renormalised unitary part (symmetric recombination)
+    const double cn0 = a.renorm ? sqrt(colnorm2(xr, xi)) : 0.0;
+    for (int s = a.s_begin;; ++s) {
+        const bool end = s == a.s_end;
+        if (!end || s == a.nsamples - 1) {
+            float Rf[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) Rf[k] = 0.f;
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                C4 g;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) g.c[i] = make_float2(xr[k][i], xi[k][i]);
+                accum_rho(g, Rf);
             }
-        bool bad = false;
+            double R[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) R[k] = Rf[k];
+            group_sum<16>(R, SCR(), NW, w0);
+            if (t == 0 && act) record_sample(a, u, s, R, ys, yc);
+        }
+        if (end) break;
+        if (a.relax && colok && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s])
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                C4 g;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) g.c[i] = make_float2(xr[k][i], xi[k][i]);
+                st_flip(g);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) { xr[k][i] = g.c[i].x; xi[k][i] = g.c[i].y; }
+            }
+        // publish B to LDS ([row][16 columns] planes)
+        __syncthreads();
 #pragma unroll
         for (int k = 0; k < KT; ++k) {
             const int rt = wave + NW * k;
             if (rt < ntile)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    bad |= !isfinite(xr[k][i]) || !isfinite(xi[k][i]);
-                    if (a.state) {
-                        float *st = a.state + (size_t)u * 2 * Dp * 16;
-                        st[(rt * 16 + 4 * q + i) * 16 + cl] = xr[k][i];
-                        st[Dp * 16 + (rt * 16 + 4 * q + i) * 16 + cl] = xi[k][i];
-                    }
+                    Bre[(rt * 16 + 4 * q + i) * 16 + cl] = xr[k][i];
+                    Bim[(rt * 16 + 4 * q + i) * 16 + cl] = xi[k][i];
+                    Bsm[(rt * 16 + 4 * q + i) * 16 + cl] = xr[k][i] + xi[k][i];
                 }
         }
-        if (bad) atomicOr(a.status, 1);
+        __syncthreads();
+        // B' = U B as three real products on the matrix cores (Gauss):
+        // P1 = Ur Br, P2 = Ui Bi, P3 = (Ur + Ui)(Br + Bi); re = P1 - P2, im = P3 - P1 - P2
+        const int uw = a.ustep ? s - a.s_begin : ((a.ulast && s == a.nsamples - 2) ? 1 : 0);
+        const float *Ur = a.U + ((size_t)item * a.ustride + uw) * kDensePlanes * Dp * Dp;
+        const float *Ui = Ur + (size_t)Dp * Dp;
+        floatx4 x2[KT];  // P2 (xr holds P1, xi P3)
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+            xr[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+            xi[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+            x2[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+        // k-groups of 16: one float4 per lane and plane carries four k-steps of A;
+        // the A operands of group g + 1 are fetched while group g multiplies
+        float4 nr4[KT], ni4[KT];
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+            const int rt = wave + NW * k;
+            const size_t o = ((size_t)(rt * ntile) * 64 + lane) * 4;
+            nr4[k] = rt < ntile ? *reinterpret_cast<const float4 *>(Ur + o) : float4{0.f, 0.f, 0.f, 0.f};
+            ni4[k] = rt < ntile ? *reinterpret_cast<const float4 *>(Ui + o) : float4{0.f, 0.f, 0.f, 0.f};
+        }
+        for (int g = 0; g < ntile; ++g) {
+            float br[4], bi[4], bs[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int kr = 16 * g + 4 * v + q;
+                br[v] = Bre[kr * 16 + cl];
+                bi[v] = Bim[kr * 16 + cl];
+                bs[v] = Bsm[kr * 16 + cl];
+            }
+            float4 cr4[KT], ci4[KT];
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                cr4[k] = nr4[k];
+                ci4[k] = ni4[k];
+                const int rt = wave + NW * k;
+                if (rt < ntile && g + 1 < ntile) {
+                    const size_t o = ((size_t)(rt * ntile + g + 1) * 64 + lane) * 4;
+                    nr4[k] = *reinterpret_cast<const float4 *>(Ur + o);
+                    ni4[k] = *reinterpret_cast<const float4 *>(Ui + o);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                const int rt = wave + NW * k;
+                if (rt < ntile) {
+                    const float4 ar4 = cr4[k], ai4 = ci4[k];
+                    const float ar[4] = {ar4.x, ar4.y, ar4.z, ar4.w}, ai[4] = {ai4.x, ai4.y, ai4.z, ai4.w};
+                    const float as[4] = {ar4.x + ai4.x, ar4.y + ai4.y, ar4.z + ai4.z, ar4.w + ai4.w};
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        xr[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[v], br[v], xr[k], 0, 0, 0);
+                        x2[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[v], bi[v], x2[k], 0, 0, 0);
+                        xi[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(as[v], bs[v], xi[k], 0, 0, 0);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+            xi[k] = xi[k] - xr[k] - x2[k];
+            xr[k] = xr[k] - x2[k];
+        }
+        if (a.renorm) {
+            const double n2 = colnorm2(xr, xi);
+            const float f = n2 > 0.0 ? (float)(cn0 / sqrt(n2)) : 0.f;
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                xr[k] *= f;
+                xi[k] *= f;
+            }
+        }
+        if (a.relax && colok && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                C4 g;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) g.c[i] = make_float2(xr[k][i], xi[k][i]);
+                st_flip(g);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) { xr[k][i] = g.c[i].x; xi[k][i] = g.c[i].y; }
+            }
+    }
+    if (t == 0 && act)
+        for (int c = 0; c < a.nobs; ++c) {
+            a.yacc[((size_t)u * a.nobs + c) * 2] = ys[c];
+            a.yacc[((size_t)u * a.nobs + c) * 2 + 1] = yc[c];
+        }
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+        const int rt = wave + NW * k;
+        if (rt < ntile)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                bad |= !isfinite(xr[k][i]) || !isfinite(xi[k][i]);
+                if (a.state && act) {
+                    float *st = a.state + (size_t)u * 2 * Dp * 16;
+                    st[(rt * 16 + 4 * q + i) * 16 + cl] = xr[k][i];
+                    st[Dp * 16 + (rt * 16 + 4 * q + i) * 16 + cl] = xi[k][i];
+                }
+            }
+    }
+    if (bad && act) atomicOr(a.status, 1);
+}
+
+// One tile per workgroup of dense_waves(NT) waves, persistent over an atomic queue
+// of (item, tile) units.
+template <int NT>
+__global__ __launch_bounds__(64 * dense_waves(NT)) void k_dense(const StepArgs *__restrict__ pa) {
+    const StepArgs &a = *pa;
+    extern __shared__ float dsm[];
+    __shared__ double ys[kMaxObs], yc[kMaxObs];  // Kahan yield sums (thread 0)
+    __shared__ double s_col[2 * 256];             // per-wave column norms (double-buffered)
+    __shared__ int s_unit;
+    double *scr = reinterpret_cast<double *>(dsm + kDenseLdsPlanes * 16 * NT * 16);
+    const BodyShared sh{ys, yc, nullptr, nullptr, s_col};
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        __syncthreads();
+        if (u >= a.n_units) break;
+        dense_tiles<NT, dense_waves(NT), 1>(a, u, 1, dsm, scr, sh);
     }
 }
 
@@ -2214,6 +2341,86 @@ __global__ void k_to_planes(const double2 *__restrict__ U, float *__restrict__ d
 // ---------------------------------------------------------------------------
 // Launchers (called from the host part of the library)
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Uneven batch: radical pairs of different Hilbert dimension in one persistent
+// launch (the loop over systems of RunSection.cpp:96-140 / main.cpp:634-686 and the
+// per-system time loop of TaskHSGeneral.cpp:470-576, for every entry at once).
+// Every workgroup has 1024 threads and the same LDS, and pops packs from one
+// queue; the pack's class selects the body and its packing:
+//   matrix-free, D = 4 ng:   ng  64  128  256  512 1024 2048 4096
+//     groups per thread G:        1    1    1    1    2    2    4 (v_j mirrored in LDS)
+//     threads per column T:      64  128  256  512  512 1024 1024
+//     columns per workgroup P:   16    8    4    2    2    1    1
+//   dense MFMA (D <= 512, NT row tiles of 16): 16 waves on one tile from NT = 12 up,
+//     8 waves x 2 tiles at NT = 6, 8 and 4 waves x 4 tiles below.
+// A pack's result does not depend on the other packs of the launch.  The packs
+// are ordered by the host (largest first) so that the small ones fill the tail.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kUnevenThreads) void k_uneven(const UnevenArgs *__restrict__ pu) {
+    extern __shared__ float4 usm[];  // kUnevenPlanesBytes of planes, then the reduction scratch
+    __shared__ double ys[16 * kMaxObs], yc[16 * kMaxObs];
+    __shared__ float2 s_coef[16 * kMaxKrylov];
+    __shared__ float4 s_blk[kMaxNuclei * 8];
+    __shared__ double s_col[2 * 256];
+    __shared__ int s_pack;
+    double *scr = reinterpret_cast<double *>(reinterpret_cast<char *>(usm) + kUnevenPlanesBytes);
+    const BodyShared sh{ys, yc, s_coef, s_blk, s_col};
+    float *bas = pu->basis + (size_t)blockIdx.x * pu->basis_stride;
+    for (;;) {
+        if (threadIdx.x == 0) s_pack = atomicAdd(pu->queue, 1);
+        __syncthreads();
+        const int k = __builtin_amdgcn_readfirstlane(s_pack);
+        __syncthreads();
+        if (k >= pu->n_packs) break;
+        const UnevenPack pk = pu->packs[k];
+        const StepArgs &a = pu->ents[pk.entry];
+        float *dl = reinterpret_cast<float *>(usm);
+        switch (pk.cls) {
+            case UC_LZ64: lanczos_columns<1, 64, 16, true, false>(a, pk.u0, pk.nact, bas, usm, scr, sh); break;
+            case UC_LZ128: lanczos_columns<1, 128, 8, true, false>(a, pk.u0, pk.nact, bas, usm, scr, sh); break;
+            case UC_LZ256: lanczos_columns<1, 256, 4, true, false>(a, pk.u0, pk.nact, bas, usm, scr, sh); break;
+            case UC_LZ512: lanczos_columns<1, 512, 2, true, false>(a, pk.u0, pk.nact, bas, usm, scr, sh); break;
+            case UC_LZ1024: lanczos_columns<2, 512, 2, true, false>(a, pk.u0, pk.nact, bas, usm, scr, sh); break;
+            case UC_LZ2048: lanczos_columns<2, 1024, 1, true, false>(a, pk.u0, pk.nact, bas, usm, scr, sh); break;
+            case UC_LZ4096: lanczos_big_column<true, false>(a, pk.u0, bas, usm, scr, sh); break;
+            case UC_DENSE + 0: dense_tiles<1, 4, 4>(a, pk.u0, pk.nact, dl, scr, sh); break;
+            case UC_DENSE + 1: dense_tiles<2, 4, 4>(a, pk.u0, pk.nact, dl, scr, sh); break;
+            case UC_DENSE + 2: dense_tiles<3, 4, 4>(a, pk.u0, pk.nact, dl, scr, sh); break;
+            case UC_DENSE + 3: dense_tiles<4, 4, 4>(a, pk.u0, pk.nact, dl, scr, sh); break;
+            case UC_DENSE + 4: dense_tiles<6, 8, 2>(a, pk.u0, pk.nact, dl, scr, sh); break;
+            case UC_DENSE + 5: dense_tiles<8, 8, 2>(a, pk.u0, pk.nact, dl, scr, sh); break;
+            case UC_DENSE + 6: dense_tiles<12, 16, 1>(a, pk.u0, pk.nact, dl, scr, sh); break;
+            case UC_DENSE + 7: dense_tiles<16, 16, 1>(a, pk.u0, pk.nact, dl, scr, sh); break;
+            case UC_DENSE + 8: dense_tiles<24, 16, 1>(a, pk.u0, pk.nact, dl, scr, sh); break;
+            case UC_DENSE + 9: dense_tiles<32, 16, 1>(a, pk.u0, pk.nact, dl, scr, sh); break;
+            default:
+                if (threadIdx.x == 0) atomicOr(a.status, 8);
+                break;
+        }
+    }
+}
+
+// LDS planes each class uses (bytes), for the host's checks.
+static_assert(16 * 4 * 64 * 16 <= kUnevenPlanesBytes && 2 * 4 * 1024 * 16 <= kUnevenPlanesBytes &&
+                  4 * 2048 * 16 <= kUnevenPlanesBytes && 2 * 4096 * 16 <= kUnevenPlanesBytes &&
+                  3 * 512 * 16 * 4 <= kUnevenPlanesBytes && 2 * 3 * 128 * 16 * 4 <= kUnevenPlanesBytes &&
+                  4 * 3 * 64 * 16 * 4 <= kUnevenPlanesBytes,
+              "every class fits the uneven launch's LDS planes");
+
+size_t uneven_lds_bytes() { return (size_t)kUnevenPlanesBytes + kUnevenScratchDoubles * sizeof(double); }
+
+hipError_t launch_uneven(const UnevenArgs *du, int workgroups, hipStream_t st) {
+    hipLaunchKernelGGL(k_uneven, dim3(workgroups), dim3(kUnevenThreads), uneven_lds_bytes(), st, du);
+    return hipGetLastError();
+}
+
+int uneven_occupancy() {
+    int nb = 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_uneven, kUnevenThreads, uneven_lds_bytes()) == hipSuccess
+               ? nb
+               : 1;
+}
+
 #define MSG_DENSE_VARIANTS(X) X(1) X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32)
 #define MSG_LANCZOS_VARIANTS(X) \
     X(1, 64, true) X(1, 128, true) X(1, 256, true) X(1, 512, true) X(1, 1024, true) X(2, 512, true) X(2, 1024, true)
@@ -2312,6 +2519,8 @@ hipError_t set_lanczos_lds_limits() {
                               maxb - 8192);
     (void)hipFuncSetAttribute((const void *)k_lanczos_big<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               maxb - 8192);
+    (void)hipFuncSetAttribute((const void *)k_uneven, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)uneven_lds_bytes());
     (void)hipGetLastError();
     return hipSuccess;
 }

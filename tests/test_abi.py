@@ -33,7 +33,8 @@ def test_library_exports_every_declared_symbol():
 STRUCTS = [("msg_spin_desc", "SpinDesc"), ("msg_interaction_desc", "InteractionDesc"),
            ("msg_state_desc", "StateDesc"), ("msg_transition_desc", "TransitionDesc"),
            ("msg_relaxation_desc", "RelaxationDesc"), ("msg_system_desc", "SystemDesc"),
-           ("msg_run_params", "RunParams"), ("msg_plan", "Plan"), ("msg_launch_shape", "LaunchShape")]
+           ("msg_run_params", "RunParams"), ("msg_plan", "Plan"), ("msg_launch_shape", "LaunchShape"),
+           ("msg_uneven_entry", "UnevenEntry")]
 
 
 def test_struct_layouts_match_header(tmp_path):

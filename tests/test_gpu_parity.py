@@ -608,3 +608,63 @@ def test_server_uncoupled_requests(ctx):
     for (p, sd, f), a in zip(reqs, alone):
         yo, _ = ref.run_hsgeneral(p, pk, fields=f, threads=8)
         assert np.abs(a - yo).max() <= YIELD_TOL
+
+
+@pytest.mark.parametrize("shape", [None, "-3"])
+def test_config3_full_horizon(ctx, shape, monkeypatch):
+    """Config 3 (D = 65536) over its whole 0-2000 ns grid (2000 steps) on 8
+    spin-coherent trajectories with the ST flips of seed 2026, against the committed
+    FP64 fixture: the FP32 drift of the matrix-free path over the full horizon stays
+    inside the yield tolerance, and the singlet/triplet traces (every 10th sample)
+    inside the trace tolerance.  Default (tuner-free) shape and split columns."""
+    force_shape(monkeypatch, shape)
+    fx = load_fixture("cfg3_full")
+    M = fx["trajectories"]
+    s, cfg = workloads.config3(n_traj=M)
+    sysd = ctx.upload(s.pack())
+    cols = sysd.trace_samples(s.state_index("S"), M, fx["sample_seed"]) / math.sqrt(M)
+    prm = abi.run_params(total_time=fx["total_time"], time_step=fx["time_step"], method=abi.MSG_METHOD_KRYLOV,
+                         sampling=abi.MSG_SAMPLING_GIVEN, seed=fx["flip_seed"], krylov_size=fx["krylov_size"])
+    y, tr = sysd.propagate(prm, columns=cols, trace=True)
+    err = np.abs(y[0] - np.array(fx["yields"])).max()
+    terr = trace_err(tr[0, ::10], np.array(fx["trace_every_10"]))
+    print("cfg3 full horizon max |dPhi|", err, "trace max-rel", terr)
+    assert err <= YIELD_TOL
+    assert terr <= TRACE_TOL
+
+
+def test_config2_bench_fixture_traces(ctx):
+    """Config 2 (D = 4096) on the bench's 100 ns horizon: 8 orientations, all 1024
+    direct columns, through the static k_lanczos kernel at the default shape; yields
+    and per-sample traces of every transition observable against the fixture."""
+    fx = load_fixture("cfg2_bench")
+    s, cfg = workloads.config2(n_orientations=1024)
+    sysd = ctx.upload(s.pack())
+    prm = abi.run_params(total_time=fx["total_time"], time_step=fx["time_step"], method=abi.MSG_METHOD_KRYLOV,
+                         krylov_size=fx["krylov_size"])
+    y, tr = sysd.propagate(prm, fields=[cfg["fields"][i] for i in fx["orientations"]], trace=True)
+    ftr = np.array(fx["trace"])
+    err = np.abs(y - np.array(fx["yields"])).max()
+    terr = max(trace_err(tr[k], ftr[k]) for k in range(len(ftr)))
+    print("cfg2 bench fixture max |dPhi|", err, "trace max-rel", terr)
+    assert err <= YIELD_TOL
+    assert terr <= TRACE_TOL
+
+
+def test_legacy_direct_yields_lanczos_d1024(ctx):
+    """statichs-direct-yields at D = 1024 (config-4 pair, 8 anisotropic nuclei): the
+    task mode on the matrix-free Lanczos kernel (a fresh Krylov basis every step,
+    the reference's rule at krylovtol = 0) against the oracle's legacy loop."""
+    idx = next(i for i in range(1000) if workloads.config4_pair(i)[1]["n"] == 8)
+    s, cfg = workloads.config4_pair(idx)
+    p = s.pack()
+    sysd = ctx.upload(p)
+    assert sysd.dim == 1024
+    T, dt = 200.0, 2.0
+    field = cfg["fields"][30]
+    prm = abi.run_params(total_time=T, time_step=dt, task=abi.MSG_TASK_LEGACY_DIRECT_YIELDS, yield_corrections=True,
+                         method=abi.MSG_METHOD_KRYLOV)
+    y, _ = sysd.propagate(prm, fields=[field])
+    yo = ref.legacy_direct_yields(p, T, dt, correction=True, field=field)
+    print("legacy direct D=1024 dev", y[0], "oracle", yo)
+    assert np.abs(y[0] - yo).max() <= YIELD_TOL

@@ -1,0 +1,132 @@
+"""msg_propagate_uneven: radical pairs of uneven Hilbert dimension (config 4: one
+pair per nuclear count n = 2 ... 12, D = 16 ... 16384) in one persistent launch,
+against the committed FP64 oracle fixture (tests/golden/cfg4_uneven.json): singlet
+and triplet yields to |dPhi| <= 1e-5, traces to 1e-4 max-relative (against the
+trace maximum).  A pair's result must not depend on the rest of the batch, so the
+mixed launch is compared with each pair alone and with the batch reordered, to
+1e-12."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from molspin_amd import abi, workloads
+
+pytestmark = pytest.mark.gpu
+
+YIELD_TOL = 1e-5
+TRACE_TOL = 1e-4
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from molspin_amd import backend
+    c = backend.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def cfg4(ctx):
+    with open(os.path.join(HERE, "golden", "cfg4_uneven.json")) as f:
+        fx = json.load(f)
+    prm = abi.run_params(total_time=fx["total_time"], time_step=fx["time_step"], method=abi.MSG_METHOD_AUTO,
+                         krylov_size=fx["krylov_size"])
+    pairs = []
+    for e in fx["pairs"]:
+        s, _ = workloads.config4_pair(e["pair"])
+        pairs.append(dict(fx=e, sys=ctx.upload(s.pack())))
+    return fx, prm, pairs
+
+
+def entries(cfg4, order=None, trace=True):
+    _, prm, pairs = cfg4
+    order = range(len(pairs)) if order is None else order
+    return [dict(system=pairs[i]["sys"], params=prm, fields=pairs[i]["fx"]["fields"], trace=trace) for i in order]
+
+
+def trace_err(dev, ora):
+    return float(np.abs(dev - ora).max() / np.abs(ora).max())
+
+
+def test_uneven_batch_matches_oracle(ctx, cfg4):
+    """Every D class of config 4 (dense MFMA bodies for D <= 512, matrix-free Lanczos
+    bodies from D = 1024 to 16384) in one launch."""
+    _, _, pairs = cfg4
+    out = ctx.propagate_uneven(entries(cfg4))
+    for p, (y, tr) in zip(pairs, out):
+        e = p["fx"]
+        dy = float(np.abs(y - np.array(e["yields"])).max())
+        dt = max(trace_err(tr[k], np.array(e["trace"][k])) for k in range(len(e["fields"])))
+        print("n = %2d (D = %5d): max |dPhi| %.2e, trace max-rel %.2e" % (e["n"], 4 << e["n"], dy, dt))
+        assert dy <= YIELD_TOL, e["n"]
+        assert dt <= TRACE_TOL, e["n"]
+
+
+def test_uneven_alone_equals_mixed(ctx, cfg4):
+    """The same pair alone, in the mixed batch and in the reversed batch: one result."""
+    _, _, pairs = cfg4
+    mixed = ctx.propagate_uneven(entries(cfg4, trace=False))
+    rev_order = list(range(len(pairs)))[::-1]
+    rev = ctx.propagate_uneven(entries(cfg4, order=rev_order, trace=False))
+    for k in range(len(pairs)):
+        alone = ctx.propagate_uneven(entries(cfg4, order=[k], trace=False))[0][0]
+        assert np.abs(alone - mixed[k][0]).max() <= 1e-12, pairs[k]["fx"]["n"]
+        assert np.abs(rev[len(pairs) - 1 - k][0] - mixed[k][0]).max() <= 1e-12, pairs[k]["fx"]["n"]
+
+
+def test_uneven_against_per_pair_launches(ctx, cfg4):
+    """The per-pair kernels (msg_propagate) on the same pairs: the matrix-free classes
+    run the same column arithmetic as the dedicated kernels' default shapes."""
+    _, prm, pairs = cfg4
+    out = ctx.propagate_uneven(entries(cfg4, trace=False))
+    for p, (y, _) in zip(pairs, out):
+        yp, _ = p["sys"].propagate(prm, fields=p["fx"]["fields"])
+        tol = 1e-12 if p["sys"].dim > 512 else 1e-6
+        assert np.abs(yp - y).max() <= tol, p["fx"]["n"]
+
+
+def test_lanczos_d8192_default_shape(ctx, cfg4, monkeypatch):
+    """D = 8192 through msg_propagate's default symmetric-K shape, k_lanczos<2,1024>
+    with both Krylov vectors in LDS (the shape config 4's n = 11 pairs take),
+    against the oracle fixture: yields and traces."""
+    monkeypatch.setenv("MSG_LANCZOS_G", "2")
+    _, prm, pairs = cfg4
+    p = next(q for q in pairs if q["fx"]["n"] == 11)
+    assert p["sys"].dim == 8192
+    y, tr = p["sys"].propagate(prm, fields=p["fx"]["fields"], trace=True)
+    e = p["fx"]
+    dy = float(np.abs(y - np.array(e["yields"])).max())
+    dt = max(trace_err(tr[k], np.array(e["trace"][k])) for k in range(2))
+    print("D = 8192 k_lanczos<2,1024>: max |dPhi| %.2e, trace max-rel %.2e" % (dy, dt))
+    assert dy <= YIELD_TOL and dt <= TRACE_TOL
+
+
+def test_uneven_given_columns_and_rejections(ctx, cfg4):
+    """Given columns (trajectory sampling) of a D = 4096 pair beside direct entries;
+    unequal recombination above D = 512 and time-dependent fields are refused with
+    the entry named."""
+    from molspin_amd import backend
+    from oracle import binding as ref
+    _, prm, pairs = cfg4
+    p10 = next(q for q in pairs if q["fx"]["n"] == 10)
+    s, _ = workloads.config4_pair(p10["fx"]["pair"])
+    packed = s.pack()
+    cols = ref.coherent_samples(packed, s.state_index("S"), 77, 24) / np.sqrt(24)
+    prm_g = abi.run_params(total_time=prm.total_time, time_step=prm.time_step, sampling=abi.MSG_SAMPLING_GIVEN,
+                           method=abi.MSG_METHOD_KRYLOV)
+    ents = entries(cfg4, order=[0, 8], trace=False)
+    ents.append(dict(system=p10["sys"], params=prm_g, fields=p10["fx"]["fields"], columns=cols))
+    out = ctx.propagate_uneven(ents)
+    yo, _ = ref.run_hsgeneral(packed, prm_g, fields=p10["fx"]["fields"], columns=cols, threads=8)
+    assert np.abs(out[2][0] - yo).max() <= YIELD_TOL
+    # unequal rates at D = 4096: Arnoldi runs through msg_propagate only
+    for t in s.transitions:
+        t["rate"] = 2e-3 if s.states[t["source"]]["name"] == "S" else 0.0
+    bad = ctx.upload(s.pack())
+    with pytest.raises(backend.BackendError) as exc:
+        ctx.propagate_uneven(entries(cfg4, order=[0], trace=False) +
+                             [dict(system=bad, params=prm, fields=p10["fx"]["fields"])])
+    assert exc.value.status == abi.MSG_UNSUPPORTED and "entry 1" in str(exc.value)
