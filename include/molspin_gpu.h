@@ -342,6 +342,24 @@ msg_status msg_propagate(msg_ctx *ctx, const msg_system *sys, const msg_run_para
                          double *yields, double *trace);
 
 /*
+ * msg_propagate without any device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Every device buffer of the run
+ * (operator factors, grids, Krylov bases, partial sums, results) comes from a
+ * library-owned scratch of 64 MiB per GPU, held until the process ends; runs that
+ * do not fit return MSG_OOM_ADMISSION, and the number of resident columns shrinks to
+ * what fits beside the rest.  Matrix-free path only (symmetric recombination with
+ * 256 <= D <= 262144, or unequal rates up to D = 8192); the dense path returns
+ * MSG_UNSUPPORTED.  Calls on one GPU share the scratch: calls that overlap on one
+ * GPU (other host threads, other contexts) race.  Arguments and results as
+ * msg_propagate; the same results.
+ */
+msg_status msg_propagate_noalloc(msg_ctx *ctx, const msg_system *sys, const msg_run_params *params,
+                                 int32_t n_items, const double *fields,
+                                 const double *columns, int32_t n_columns,
+                                 double *yields, double *trace);
+
+/*
  * One radical pair (system) of an uneven batch: what msg_propagate takes for one
  * system, plus where its results go.
  */

@@ -21,7 +21,7 @@ EXPORTS = [
     "msg_server_submit", "msg_server_pump", "msg_server_collect", "msg_last_kernel_time",
     "msg_trace_samples", "msg_symmetric_decay", "msg_resolve_plan", "msg_autotune", "msg_time_infinity",
     "msg_msd_compile", "msg_msd_system", "msg_msd_steps", "msg_msd_task_count", "msg_msd_task", "msg_msd_destroy",
-    "msg_propagate_uneven",
+    "msg_propagate_uneven", "msg_propagate_noalloc",
 ]
 
 
@@ -71,6 +71,7 @@ def lib():
         L.msg_autotune.argtypes = [vp, vp, P(abi.RunParams), C.c_int32, vp, vp, C.c_int32, C.c_int32,
                                    P(abi.LaunchShape)]
         L.msg_propagate_uneven.argtypes = [vp, C.c_int32, P(abi.UnevenEntry)]
+        L.msg_propagate_noalloc.argtypes = [vp, vp, P(abi.RunParams), C.c_int32, vp, vp, C.c_int32, vp, vp]
         for name in EXPORTS:
             getattr(L, name).restype = C.c_int if name != "msg_last_error" else C.c_char_p
         _LIB = L
@@ -273,6 +274,20 @@ class System:
         tr = np.zeros(n_items * ns * nobs) if trace else None
         self.ctx.check(lib().msg_propagate(self.ctx.h, self.h, C.byref(params), n_items, _ptr(f), _ptr(cols), ncol,
                                            _ptr(y), _ptr(tr)))
+        return y.reshape(n_items, nobs), (tr.reshape(n_items, ns, nobs) if trace else None)
+
+    def propagate_noalloc(self, params, fields=None, columns=None, trace=False):
+        """msg_propagate_noalloc: as propagate, with every device buffer in the
+        library's fixed scratch (no device allocation)."""
+        n_items = 1 if fields is None else len(fields)
+        f = None if fields is None else np.ascontiguousarray(fields, dtype=np.float64).reshape(-1)
+        cols, ncol = (None, 0) if columns is None else _columns(columns)
+        nobs = self.n_obs(params)
+        ns = len(time_grid(params.total_time, params.time_step))
+        y = np.zeros(n_items * nobs)
+        tr = np.zeros(n_items * ns * nobs) if trace else None
+        self.ctx.check(lib().msg_propagate_noalloc(self.ctx.h, self.h, C.byref(params), n_items, _ptr(f), _ptr(cols),
+                                                   ncol, _ptr(y), _ptr(tr)))
         return y.reshape(n_items, nobs), (tr.reshape(n_items, ns, nobs) if trace else None)
 
     def autotune(self, params, fields=None, columns=None, sample_steps=4):

@@ -21,8 +21,8 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Wall", "
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 LIBS = ["-L" + os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas", "-Wl,-rpath," + os.path.join(ROCM, "lib")]
 
-SOURCES = [("msg_kernels.hip", "hip"), ("msg_abi.cpp", "host"), ("msg_resolve.cpp", "host"), ("msg_plan.cpp", "host"),
-           ("msg_msd.cpp", "host")]
+SOURCES = [("msg_kernels.hip", "hip"), ("msg_noalloc.hip", "hip"), ("msg_abi.cpp", "host"), ("msg_resolve.cpp", "host"),
+           ("msg_plan.cpp", "host"), ("msg_msd.cpp", "host")]
 
 
 def build(verbose=False):

@@ -62,6 +62,7 @@ int dense_occupancy(int dpad, size_t lds);
 int dense_threads(int dpad);
 hipError_t launch_uneven(const UnevenArgs *du, int workgroups, hipStream_t st);
 int uneven_occupancy();
+hipError_t scratch_address(int device, void **ptr);
 }  // namespace msg
 
 using namespace msg;
@@ -84,6 +85,9 @@ struct msg_ctx {
     // pinned host staging of given columns (grown on demand, freed with the context)
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
+    // this device's copy of the allocation-free entry point's scratch (kScratchBytes),
+    // looked up once at set-up
+    void *scratch = nullptr;
 };
 
 struct msg_system {
@@ -122,27 +126,54 @@ struct CoopRefused {
     std::string what;
 };
 
+// Bump allocator over the allocation-free entry point's scratch: while an Arena is
+// installed on the calling thread (ArenaScope), DBuf carves its buffers out of it
+// instead of allocating, and never frees them.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, off = 0;
+    size_t left() const { return cap - off; }
+};
+thread_local Arena *t_arena = nullptr;
+struct ArenaScope {
+    explicit ArenaScope(Arena *a) { t_arena = a; }
+    ~ArenaScope() { t_arena = nullptr; }
+};
+
 // RAII device allocation, stream-ordered so that a batch never synchronises the
 // device (other requests keep running while this one allocates or frees).
 struct DBuf {
     void *p = nullptr;
     size_t n = 0;
     hipStream_t st = nullptr;
+    bool carved = false;  // from the arena: nothing to free
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { reset(); }
     void reset() {
-        if (p) (void)hipFreeAsync(p, st);
+        if (p && !carved) (void)hipFreeAsync(p, st);
         p = nullptr;
         n = 0;
+        carved = false;
     }
     template <class T>
     T *alloc(size_t count, hipStream_t stream) {
         reset();
         if (count == 0) return nullptr;
         st = stream;
-        HIPCHECK(hipMallocAsync(&p, count * sizeof(T), stream));
+        if (t_arena) {
+            const size_t bytes = (count * sizeof(T) + 255) / 256 * 256;
+            if (bytes > t_arena->left())
+                throw ArgError{MSG_OOM_ADMISSION, "the run needs more than the " +
+                                                      std::to_string(t_arena->cap >> 20) +
+                                                      " MiB of the allocation-free scratch"};
+            p = t_arena->base + t_arena->off;
+            t_arena->off += bytes;
+            carved = true;
+        } else {
+            HIPCHECK(hipMallocAsync(&p, count * sizeof(T), stream));
+        }
         n = count * sizeof(T);
         return static_cast<T *>(p);
     }
@@ -500,14 +531,17 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     if (p->sampling == MSG_SAMPLING_GIVEN) {
         a.init_mode = 1;
         const size_t gbytes = (size_t)n_columns * 2 * D * sizeof(float);
-        if (ctx->pinned_bytes < gbytes) {  // page-locked, so the copy runs at DMA speed
+        std::vector<float> pageable;  // allocation-free runs: no pinned staging
+        if (t_arena) {
+            pageable.resize(gbytes / sizeof(float));
+        } else if (ctx->pinned_bytes < gbytes) {  // page-locked, so the copy runs at DMA speed
             if (ctx->pinned) (void)hipHostFree(ctx->pinned);
             ctx->pinned = nullptr;
             ctx->pinned_bytes = 0;
             HIPCHECK(hipHostMalloc(&ctx->pinned, gbytes, hipHostMallocDefault));
             ctx->pinned_bytes = gbytes;
         }
-        float *g = static_cast<float *>(ctx->pinned);
+        float *g = t_arena ? pageable.data() : static_cast<float *>(ctx->pinned);
         // MolSpin order (complex128) -> device planes (complex64); columns split over host threads
         auto convert = [&](int j0, int j1) {
             for (int j = j0; j < j1; ++j)
@@ -801,7 +835,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     a.trace = want_trace ? b.trace.alloc<double>((size_t)b.n_units * ns * b.nobs, stream) : nullptr;
     a.queue = b.queue.alloc<int>(1, stream);
     a.status = b.status.alloc<int>(1, stream);
-    HIPCHECK(hipMemsetAsync(a.status, 0, sizeof(int), stream));
+    HIPCHECK(launch_zero_int(a.status, 1, stream));  // a fill kernel: no memset node under graph capture
     if (b.keep_state) {
         const size_t per_unit = b.path == Path::Lanczos ? (size_t)2 * D : (size_t)2 * a.dpad * 16;
         a.state = b.state.alloc<float>((size_t)b.n_units * per_unit, stream);
@@ -831,6 +865,16 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
             b.slots = std::min(most, (b.slots + 7) / 8 * 8);
         }
         const int vecs = b.G <= 0 && b.G != -3 ? a.m + 2 : a.m;  // streaming variants: + residual and state
+        if (t_arena && b.G != -3) {  // as many resident columns as the scratch holds beside the rest of the run
+            const size_t rest = (size_t)b.n_units * 8 + sizeof(StepArgs) + (size_t)n_items * (b.nobs + 1) * 8 +
+                                (want_trace ? (size_t)n_items * ns * b.nobs * 8 : 0) + 4 * 256;
+            const size_t per = (size_t)vecs * 2 * D * sizeof(float);
+            const size_t room = t_arena->left() > rest ? t_arena->left() - rest : 0;
+            if (room < per)
+                throw ArgError{MSG_OOM_ADMISSION, "the run needs more than the " + std::to_string(t_arena->cap >> 20) +
+                                                      " MiB of the allocation-free scratch"};
+            b.slots = std::max(1, std::min<int>(b.slots, (int)(room / per)));
+        }
         a.basis = b.basis.alloc<float>((size_t)b.slots * vecs * 2 * D, stream);
         if (b.G == -3) {
             a.xred = b.xred.alloc<double>((size_t)b.slots * 2 * S * 17, stream);
@@ -1150,6 +1194,8 @@ msg_status msg_ctx_create(int device, msg_ctx **out) {
     if (set_lanczos_lds_limits() != hipSuccess) return fail(nullptr, MSG_DEVICE_ERROR, "cannot raise the LDS limit");
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return fail(nullptr, MSG_DEVICE_ERROR, "hipEventCreate failed");
+    if (scratch_address(device, &c->scratch) != hipSuccess)
+        return fail(nullptr, MSG_DEVICE_ERROR, "cannot locate the allocation-free scratch on this device");
     *out = c.release();
     return MSG_OK;
 }
@@ -1447,6 +1493,31 @@ msg_status msg_propagate(msg_ctx *ctx, const msg_system *sys, const msg_run_para
             return;
         }
         Batch b;
+        prepare(b, ctx, sys, params, n_items, fields, columns, n_columns, trace != nullptr, false, ctx->stream);
+        b.timed = true;
+        advance(b, b.nsamples);
+        collect(b, yields, trace);
+        ctx->kernel_ms = batch_kernel_ms(b);
+        ctx->launches = b.launches;
+    });
+}
+
+msg_status msg_propagate_noalloc(msg_ctx *ctx, const msg_system *sys, const msg_run_params *params, int32_t n_items,
+                                 const double *fields, const double *columns, int32_t n_columns, double *yields,
+                                 double *trace) {
+    if (!ctx || !sys) return fail(ctx, MSG_INVALID_ARG, "null context or system");
+    return guarded(ctx, [&] {
+        HIPCHECK(hipSetDevice(ctx->device));
+        if (!ctx->scratch) throw DeviceError("no allocation-free scratch on this device");
+        check_params(sys, params);
+        if (params->method == MSG_METHOD_UNCOUPLED || choose_path(sys->r, params) != Path::Lanczos)
+            throw ArgError{MSG_UNSUPPORTED, "the allocation-free entry point runs the matrix-free path only (the "
+                                            "dense and uncoupled paths build their operators with rocBLAS / "
+                                            "rocSOLVER, which allocate)"};
+        Arena arena{static_cast<char *>(ctx->scratch), (size_t)kScratchBytes, 0};
+        ArenaScope scope(&arena);
+        Batch b;
+        b.no_split = true;  // the split-column shape needs its cooperative launch's own sizing
         prepare(b, ctx, sys, params, n_items, fields, columns, n_columns, trace != nullptr, false, ctx->stream);
         b.timed = true;
         advance(b, b.nsamples);

@@ -20,6 +20,8 @@ constexpr int kChebMax = 96;
 constexpr int kTileGroups = 4096;  // groups per LDS tile of the tiled streaming Lanczos (bits 0-11)
 constexpr int kDensePlanes = 2;    // dense-path propagator planes: re, im
 constexpr int kDenseLdsPlanes = 3; // dense-path state tile in LDS: re, im, re + im
+// Device scratch of the allocation-free entry point, per GPU (msg_noalloc.hip)
+constexpr unsigned long long kScratchBytes = 64ull << 20;
 
 // Per-item coefficient block of the matrix-free operator (floats):
 //   [0..2]  w1: field on electron 1 (H += w1 . S1), [3..5] w2: field on electron 2
