@@ -1112,13 +1112,12 @@ UnevenShape uneven_shape(const Batch &b) {
     const int D = b.a.dim;
     if (b.path == Path::Lanczos) {
         const int ng = D / 4;
-        static const int packs[7] = {16, 8, 4, 2, 2, 1, 1};
         int lg = 0;
         while ((1 << lg) < ng) ++lg;
         if ((1 << lg) != ng || lg < 6 || lg > 12)
             throw ArgError{MSG_UNSUPPORTED, "uneven batch: matrix-free entries need 256 <= D <= 16384"};
         s.cls = UC_LZ64 + (lg - 6);
-        s.per_pack = packs[lg - 6];
+        s.per_pack = uneven_pack_width(s.cls);
         s.basis_floats = s.cls == UC_LZ4096 ? (size_t)(b.a.m + 2) * 2 * D : (size_t)s.per_pack * b.a.m * 2 * D;
     } else {
         static const int nts[10] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32};
@@ -1126,7 +1125,7 @@ UnevenShape uneven_shape(const Batch &b) {
         const int i = (int)(std::find(nts, nts + 10, nt) - nts);
         if (i == 10) throw ArgError{MSG_UNSUPPORTED, "uneven batch: no dense class for this dimension"};
         s.cls = UC_DENSE + i;
-        s.per_pack = nt >= 12 ? 1 : nt >= 6 ? 2 : 4;
+        s.per_pack = uneven_pack_width(s.cls);
     }
     return s;
 }
@@ -1711,7 +1710,7 @@ msg_status msg_propagate_uneven(msg_ctx *ctx, int32_t n_entries, const msg_uneve
             HIPCHECK(hipMemcpy(&sflag, bs[e]->a.status, sizeof(int), hipMemcpyDeviceToHost));
             bad_class |= sflag & 8;
         }
-        if (bad_class) throw DeviceError("uneven launch: a pack named no class");
+        if (bad_class) throw DeviceError("uneven launch: a pack named no class or did not fit its class");
         for (int e = 0; e < n_entries; ++e) {
             try {
                 collect(*bs[e], entries[e].yields, entries[e].trace);

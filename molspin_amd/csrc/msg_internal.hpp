@@ -87,7 +87,8 @@ struct StepArgs {
     float *basis;            // [slots][m][2D]
     int *status;             // device error flags (bit 0: non-finite, bit 1: dt too large,
                              // bit 2: workgroups of a split column lost each other,
-                             // bit 3: a pack of the uneven launch names no class)
+                             // bit 3: a pack of the uneven launch names no class or
+                             // holds more units than its class packs)
     double *xred;            // split columns: [slots][2][S][17] partial sums
     int *bar;                // split columns: [slots] barrier counters (zeroed before launch)
 };
@@ -99,6 +100,15 @@ enum UnevenClass {
     UC_LZ64 = 0, UC_LZ128, UC_LZ256, UC_LZ512, UC_LZ1024, UC_LZ2048, UC_LZ4096,  // groups per column D / 4
     UC_DENSE = 16,  // + index of the row-tile count in {1, 2, 3, 4, 6, 8, 12, 16, 24, 32}
 };
+// Columns (matrix-free classes) or 16-column tiles (dense classes) per pack of each
+// class: what the kernel's body for the class packs into one workgroup.  Shared by the
+// host, which forms the packs, and the kernel, which checks them.
+constexpr int uneven_pack_width(int cls) {
+    return cls >= UC_LZ64 && cls <= UC_LZ4096 ? (cls <= UC_LZ64 ? 16 : cls == UC_LZ128 ? 8 : cls == UC_LZ256 ? 4
+                                                                    : cls <= UC_LZ1024 ? 2 : 1)
+         : cls >= UC_DENSE && cls < UC_DENSE + 10 ? (cls < UC_DENSE + 4 ? 4 : cls < UC_DENSE + 6 ? 2 : 1)
+                                                  : 0;
+}
 constexpr int kUnevenThreads = 1024;
 constexpr int kUnevenPlanesBytes = 131072;       // per workgroup: the body's LDS planes
 constexpr int kUnevenScratchDoubles = 2 * 17 * 16;  // reductions, double-buffered

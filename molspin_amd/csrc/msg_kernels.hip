@@ -2356,6 +2356,51 @@ __global__ void k_to_planes(const double2 *__restrict__ U, float *__restrict__ d
 // A pack's result does not depend on the other packs of the launch.  The packs
 // are ordered by the host (largest first) so that the small ones fill the tail.
 // ---------------------------------------------------------------------------
+// The uneven kernel calls each body through a function of its own, so that every
+// body gets the register allocation it has in its dedicated kernel (inlined into one
+// switch, the bodies shared one allocation and spilled).  LDS pointers cross the
+// call in the LDS address space, so the body's accesses stay LDS instructions.
+#define MSG_LDS __attribute__((address_space(3)))
+// Function arguments arrive in vector registers, and loads through them are vector
+// loads; the entry's arguments are the same for the whole workgroup, so the body works
+// on a copy whose words are made uniform (scalar registers), as in a dedicated kernel.
+__device__ __forceinline__ void uniform_copy(const StepArgs *src, StepArgs &dst) {
+    constexpr int W = sizeof(StepArgs) / 4;
+    static_assert(sizeof(StepArgs) % 4 == 0, "whole words");
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(src);
+    uint32_t *d = reinterpret_cast<uint32_t *>(&dst);
+#pragma unroll
+    for (int i = 0; i < W; ++i) d[i] = __builtin_amdgcn_readfirstlane(s[i]);
+}
+template <int G, int T, int P>
+__device__ __noinline__ void uneven_lz(const StepArgs *__restrict__ a, int u0, int nact, float *bas, MSG_LDS float4 *lds,
+                                       MSG_LDS double *scr, MSG_LDS double *ys, MSG_LDS double *yc,
+                                       MSG_LDS float2 *coef, MSG_LDS float4 *blk) {
+    const BodyShared sh{(double *)ys, (double *)yc, (float2 *)coef, (float4 *)blk, nullptr};
+    StepArgs ua;
+    uniform_copy(a, ua);
+    lanczos_columns<G, T, P, true, false>(ua, __builtin_amdgcn_readfirstlane(u0),
+                                          __builtin_amdgcn_readfirstlane(nact), bas, (float4 *)lds, (double *)scr, sh);
+}
+__device__ __noinline__ void uneven_big(const StepArgs *__restrict__ a, int u, float *bas, MSG_LDS float4 *lds, MSG_LDS double *scr,
+                                        MSG_LDS double *ys, MSG_LDS double *yc, MSG_LDS float2 *coef,
+                                        MSG_LDS float4 *blk) {
+    const BodyShared sh{(double *)ys, (double *)yc, (float2 *)coef, (float4 *)blk, nullptr};
+    StepArgs ua;
+    uniform_copy(a, ua);
+    lanczos_big_column<true, false>(ua, __builtin_amdgcn_readfirstlane(u), bas, (float4 *)lds,
+                                    (double *)scr, sh);
+}
+template <int NT, int NW, int P>
+__device__ __noinline__ void uneven_dense(const StepArgs *__restrict__ a, int u0, int nact, MSG_LDS float *lds, MSG_LDS double *scr,
+                                          MSG_LDS double *ys, MSG_LDS double *yc, MSG_LDS double *colsum) {
+    const BodyShared sh{(double *)ys, (double *)yc, nullptr, nullptr, (double *)colsum};
+    StepArgs ua;
+    uniform_copy(a, ua);
+    dense_tiles<NT, NW, P>(ua, __builtin_amdgcn_readfirstlane(u0), __builtin_amdgcn_readfirstlane(nact),
+                           (float *)lds, (double *)scr, sh);
+}
+
 __global__ __launch_bounds__(kUnevenThreads) void k_uneven(const UnevenArgs *__restrict__ pu) {
     extern __shared__ float4 usm[];  // kUnevenPlanesBytes of planes, then the reduction scratch
     __shared__ double ys[16 * kMaxObs], yc[16 * kMaxObs];
@@ -2364,7 +2409,6 @@ __global__ __launch_bounds__(kUnevenThreads) void k_uneven(const UnevenArgs *__r
     __shared__ double s_col[2 * 256];
     __shared__ int s_pack;
     double *scr = reinterpret_cast<double *>(reinterpret_cast<char *>(usm) + kUnevenPlanesBytes);
-    const BodyShared sh{ys, yc, s_coef, s_blk, s_col};
     float *bas = pu->basis + (size_t)blockIdx.x * pu->basis_stride;
     for (;;) {
         if (threadIdx.x == 0) s_pack = atomicAdd(pu->queue, 1);
@@ -2373,34 +2417,50 @@ __global__ __launch_bounds__(kUnevenThreads) void k_uneven(const UnevenArgs *__r
         __syncthreads();
         if (k >= pu->n_packs) break;
         const UnevenPack pk = pu->packs[k];
-        const StepArgs &a = pu->ents[pk.entry];
-        float *dl = reinterpret_cast<float *>(usm);
+        const StepArgs *a = pu->ents + pk.entry;
+        if (pk.nact < 1 || pk.nact > uneven_pack_width(pk.cls)) {  // a pack the class's body cannot hold
+            if (threadIdx.x == 0) atomicOr(a->status, 8);
+            continue;
+        }
+        MSG_LDS float4 *L = (MSG_LDS float4 *)usm;
+        MSG_LDS double *R = (MSG_LDS double *)scr;
+        MSG_LDS double *Y = (MSG_LDS double *)ys, *Yc = (MSG_LDS double *)yc;
+        MSG_LDS float2 *Cf = (MSG_LDS float2 *)s_coef;
+        MSG_LDS float4 *Bk = (MSG_LDS float4 *)s_blk;
+        MSG_LDS float *Lf = (MSG_LDS float *)usm;
+        MSG_LDS double *Cs = (MSG_LDS double *)s_col;
         switch (pk.cls) {
-            case UC_LZ64: lanczos_columns<1, 64, 16, true, false>(a, pk.u0, pk.nact, bas, usm, scr, sh); break;
-            case UC_LZ128: lanczos_columns<1, 128, 8, true, false>(a, pk.u0, pk.nact, bas, usm, scr, sh); break;
-            case UC_LZ256: lanczos_columns<1, 256, 4, true, false>(a, pk.u0, pk.nact, bas, usm, scr, sh); break;
-            case UC_LZ512: lanczos_columns<1, 512, 2, true, false>(a, pk.u0, pk.nact, bas, usm, scr, sh); break;
-            case UC_LZ1024: lanczos_columns<2, 512, 2, true, false>(a, pk.u0, pk.nact, bas, usm, scr, sh); break;
-            case UC_LZ2048: lanczos_columns<2, 1024, 1, true, false>(a, pk.u0, pk.nact, bas, usm, scr, sh); break;
-            case UC_LZ4096: lanczos_big_column<true, false>(a, pk.u0, bas, usm, scr, sh); break;
-            case UC_DENSE + 0: dense_tiles<1, 4, 4>(a, pk.u0, pk.nact, dl, scr, sh); break;
-            case UC_DENSE + 1: dense_tiles<2, 4, 4>(a, pk.u0, pk.nact, dl, scr, sh); break;
-            case UC_DENSE + 2: dense_tiles<3, 4, 4>(a, pk.u0, pk.nact, dl, scr, sh); break;
-            case UC_DENSE + 3: dense_tiles<4, 4, 4>(a, pk.u0, pk.nact, dl, scr, sh); break;
-            case UC_DENSE + 4: dense_tiles<6, 8, 2>(a, pk.u0, pk.nact, dl, scr, sh); break;
-            case UC_DENSE + 5: dense_tiles<8, 8, 2>(a, pk.u0, pk.nact, dl, scr, sh); break;
-            case UC_DENSE + 6: dense_tiles<12, 16, 1>(a, pk.u0, pk.nact, dl, scr, sh); break;
-            case UC_DENSE + 7: dense_tiles<16, 16, 1>(a, pk.u0, pk.nact, dl, scr, sh); break;
-            case UC_DENSE + 8: dense_tiles<24, 16, 1>(a, pk.u0, pk.nact, dl, scr, sh); break;
-            case UC_DENSE + 9: dense_tiles<32, 16, 1>(a, pk.u0, pk.nact, dl, scr, sh); break;
+            case UC_LZ64: uneven_lz<1, 64, 16>(a, pk.u0, pk.nact, bas, L, R, Y, Yc, Cf, Bk); break;
+            case UC_LZ128: uneven_lz<1, 128, 8>(a, pk.u0, pk.nact, bas, L, R, Y, Yc, Cf, Bk); break;
+            case UC_LZ256: uneven_lz<1, 256, 4>(a, pk.u0, pk.nact, bas, L, R, Y, Yc, Cf, Bk); break;
+            case UC_LZ512: uneven_lz<1, 512, 2>(a, pk.u0, pk.nact, bas, L, R, Y, Yc, Cf, Bk); break;
+            case UC_LZ1024: uneven_lz<2, 512, 2>(a, pk.u0, pk.nact, bas, L, R, Y, Yc, Cf, Bk); break;
+            case UC_LZ2048: uneven_lz<2, 1024, 1>(a, pk.u0, pk.nact, bas, L, R, Y, Yc, Cf, Bk); break;
+            case UC_LZ4096: uneven_big(a, pk.u0, bas, L, R, Y, Yc, Cf, Bk); break;
+            case UC_DENSE + 0: uneven_dense<1, 4, 4>(a, pk.u0, pk.nact, Lf, R, Y, Yc, Cs); break;
+            case UC_DENSE + 1: uneven_dense<2, 4, 4>(a, pk.u0, pk.nact, Lf, R, Y, Yc, Cs); break;
+            case UC_DENSE + 2: uneven_dense<3, 4, 4>(a, pk.u0, pk.nact, Lf, R, Y, Yc, Cs); break;
+            case UC_DENSE + 3: uneven_dense<4, 4, 4>(a, pk.u0, pk.nact, Lf, R, Y, Yc, Cs); break;
+            case UC_DENSE + 4: uneven_dense<6, 8, 2>(a, pk.u0, pk.nact, Lf, R, Y, Yc, Cs); break;
+            case UC_DENSE + 5: uneven_dense<8, 8, 2>(a, pk.u0, pk.nact, Lf, R, Y, Yc, Cs); break;
+            case UC_DENSE + 6: uneven_dense<12, 16, 1>(a, pk.u0, pk.nact, Lf, R, Y, Yc, Cs); break;
+            case UC_DENSE + 7: uneven_dense<16, 16, 1>(a, pk.u0, pk.nact, Lf, R, Y, Yc, Cs); break;
+            case UC_DENSE + 8: uneven_dense<24, 16, 1>(a, pk.u0, pk.nact, Lf, R, Y, Yc, Cs); break;
+            case UC_DENSE + 9: uneven_dense<32, 16, 1>(a, pk.u0, pk.nact, Lf, R, Y, Yc, Cs); break;
             default:
-                if (threadIdx.x == 0) atomicOr(a.status, 8);
+                if (threadIdx.x == 0) atomicOr(a->status, 8);
                 break;
         }
     }
 }
 
 // LDS planes each class uses (bytes), for the host's checks.
+static_assert(uneven_pack_width(UC_LZ64) == 16 && uneven_pack_width(UC_LZ128) == 8 &&
+                  uneven_pack_width(UC_LZ256) == 4 && uneven_pack_width(UC_LZ512) == 2 &&
+                  uneven_pack_width(UC_LZ1024) == 2 && uneven_pack_width(UC_LZ2048) == 1 &&
+                  uneven_pack_width(UC_LZ4096) == 1 && uneven_pack_width(UC_DENSE + 3) == 4 &&
+                  uneven_pack_width(UC_DENSE + 5) == 2 && uneven_pack_width(UC_DENSE + 6) == 1,
+              "pack widths match the bodies of k_uneven");
 static_assert(16 * 4 * 64 * 16 <= kUnevenPlanesBytes && 2 * 4 * 1024 * 16 <= kUnevenPlanesBytes &&
                   4 * 2048 * 16 <= kUnevenPlanesBytes && 2 * 4096 * 16 <= kUnevenPlanesBytes &&
                   3 * 512 * 16 * 4 <= kUnevenPlanesBytes && 2 * 3 * 128 * 16 * 4 <= kUnevenPlanesBytes &&
