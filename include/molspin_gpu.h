@@ -200,6 +200,10 @@ typedef struct msg_run_params {
                                      Zeeman fields at their time-dependent value; each interval
                                      uses H at its midpoint (TaskHSGeneral.cpp:527-537), or the RK4
                                      stages at t, t+dt/2, t+dt with method RK4 (:493-519) */
+    double   krylov_tol;          /* krylovtol of statichs-direct-yields (TaskStaticHSDirectYields.cpp:
+                                     373-410, 569-652): a column keeps its Krylov basis while
+                                     h_{m+1,m} |c_{m-1}| <= krylov_tol; 0 builds a fresh basis every
+                                     step (the reference's default 1e-16 in effect) */
 } msg_run_params;
 
 /* propagationmethod as written in the plan (HSExecutionPlan.h:29) */

@@ -95,7 +95,8 @@ class RunParams(C.Structure):
                 ("method", C.c_int32), ("krylov_size", C.c_int32),
                 ("sampling", C.c_int32), ("yield_corrections", C.c_int32),
                 ("total_time", C.c_double), ("time_step", C.c_double),
-                ("seed", C.c_uint64), ("task", C.c_int32), ("dynamics", C.c_int32)]
+                ("seed", C.c_uint64), ("task", C.c_int32), ("dynamics", C.c_int32),
+                ("krylov_tol", C.c_double)]
 
 
 class Plan(C.Structure):
@@ -118,8 +119,8 @@ class UnevenEntry(C.Structure):
 def run_params(calculation=MSG_CALC_YIELDS, observables=MSG_OBS_TRANSITION_YIELDS,
                method=MSG_METHOD_KRYLOV, krylov_size=16, sampling=MSG_SAMPLING_DIRECT,
                yield_corrections=False, total_time=10000.0, time_step=1.0, seed=1, task=MSG_TASK_HSGENERAL,
-               dynamics=False):
+               dynamics=False, krylov_tol=0.0):
     """RunParams with HSExecutionPlan defaults (HSExecutionPlan.h:60-79)."""
     return RunParams(calculation, observables, method, krylov_size, sampling,
                      int(bool(yield_corrections)), float(total_time), float(time_step), int(seed), int(task),
-                     int(bool(dynamics)))
+                     int(bool(dynamics)), float(krylov_tol))

@@ -484,17 +484,14 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     b.nsamples = (int)b.times.size();
     const int ns = b.nsamples;
     std::vector<double> dts(std::max(ns - 1, 1), 0.0), wyh((size_t)ns * b.nobs), wth((size_t)ns * b.nobs);
-    std::vector<uint32_t> thr(std::max(ns - 1, 1), 0u);
+    std::vector<uint32_t> thr(4 * (size_t)std::max(ns - 1, 1), 0u);
     // symmetric K = kappa 1 is factored out as e^{-2 kappa t} -- exact for the exponential
     // and Krylov propagators, not for RK4's polynomial, which keeps K inside
     const bool factor_decay = sym_for(r, p) && p->method != MSG_METHOD_RK4;
     const double kappa = factor_decay ? kappa_for(r, p) : 0.0;
     for (int s = 0; s + 1 < ns; ++s) {
         dts[s] = b.times[s + 1] - b.times[s];
-        if (r.relax) {
-            const double pr = -0.5 * std::expm1(-2.0 * (0.5 * r.relax_rate) * (0.5 * dts[s]));
-            thr[s] = (uint32_t)std::floor(pr * 4294967296.0);
-        }
+        if (r.relax) relax_thresholds(r.relax_rates, 0.5 * dts[s], &thr[4 * (size_t)s]);
     }
     for (int s = 0; s < ns; ++s) {
         double w = 0.0;
@@ -525,6 +522,10 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     }
     a.relax = r.relax ? 1 : 0;
     a.seed = p->seed;
+    if (!(p->krylov_tol >= 0.0) || !std::isfinite(p->krylov_tol))
+        throw ArgError{MSG_INVALID_ARG, "krylov_tol must be finite and non-negative"};
+    // basis reuse: legacy direct yields with symmetric recombination (the decay factored out)
+    a.ktol = p->task == MSG_TASK_LEGACY_DIRECT_YIELDS && factor_decay && !r.relax ? p->krylov_tol : 0.0;
     a.m = p->krylov_size > 0 ? p->krylov_size : 16;
     a.renorm = 0;
     // initial states
@@ -1661,6 +1662,8 @@ msg_status msg_propagate_uneven(msg_ctx *ctx, int32_t n_entries, const msg_uneve
             }
             if (b.arnoldi)
                 throw ArgError{MSG_UNSUPPORTED, at + "unequal recombination rates above D = 512 run through msg_propagate"};
+            if (b.a.ktol > 0.0)
+                throw ArgError{MSG_UNSUPPORTED, at + "krylovtol basis reuse runs through msg_propagate"};
             const UnevenShape sh = uneven_shape(b);
             basis_floats = std::max(basis_floats, sh.basis_floats);
             StepArgs a = b.a;

@@ -73,6 +73,8 @@ struct StepArgs {
     uint64_t seed;
     const uint32_t *thr;     // [nsamples-1] flip thresholds of one half-step
     int rng_step0;           // RNG counter of interval s is s + rng_step0
+    double ktol;             // > 0: keep a column's Krylov basis while h_{m+1,m} |c_{m-1}| <= ktol
+                             // (legacy direct yields' krylovtol; one column per workgroup only)
     // state source / sink
     int init_mode;           // 0 direct (generated), 1 given columns, 2 resume from state
     int n_init;              // direct: number of electron vectors

@@ -409,19 +409,32 @@ __device__ __forceinline__ void st_flip(C4 &x) {
     x.c[3] = make_float2(-x.c[3].x, -x.c[3].y);
 }
 
-// exp(-i dt T) e1 for the real symmetric tridiagonal T of order meff, in FP64,
-// by a Chebyshev expansion on the Gershgorin interval of T, summed with
-// Clenshaw's recurrence while Miller's backward recurrence generates the Bessel
+// Whether the relaxation half-step `half` of interval s flips the column: the first
+// Philox word x of counter (column, step, half, item) against the interval's two
+// half-open ranges [t0, t1) and [t2, t3) (a[4 s .. 4 s + 3]): a single channel's
+// Bernoulli draw is x < t1 with t0 = 0, an odd count of the superposed Poisson
+// process of several channels on the electron pair is the union of two ranges.
+__device__ __forceinline__ bool relax_flip(const StepArgs &a, int col, int s, uint32_t half, int item) {
+    const uint32_t x = philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), half, (uint32_t)item, a.seed);
+    const uint32_t *t = a.thr + 4 * (size_t)s;
+    return (x >= t[0] && x < t[1]) || (x >= t[2] && x < t[3]);
+}
+
+// exp(-i dt T) v for the real symmetric tridiagonal T of order meff and a complex
+// vector v, in FP64, by a Chebyshev expansion on the Gershgorin interval of T, summed
+// with Clenshaw's recurrence while Miller's backward recurrence generates the Bessel
 // coefficients J_k(dt*rho) in the same order.  Every wavefront evaluates it
-// redundantly: lane i holds ai = alpha_i, bi = beta_i (coupling of i and i+1),
-// and receives component i of the result in (re, im).  Lanes >= meff are zero.
-__device__ __forceinline__ void cheb_expm_lanes(int meff, double dt, double ai_in, double bi_in, double &re,
-                                                double &im, int *status) {
+// redundantly: lane i holds ai = alpha_i, bi = beta_i (coupling of i and i+1) and
+// v_i = (vr, vi), and receives component i of the result in (re, im).  Lanes >= meff
+// are zero.
+__device__ __forceinline__ void cheb_expm_vec(int meff, double dt, double ai_in, double bi_in, double vr_in,
+                                              double vi_in, double &re, double &im, int *status) {
     const int i = threadIdx.x & 63;
     const bool on = i < meff;
     const double ai = on ? ai_in : 0.0;
     const double br = (i < meff - 1) ? bi_in : 0.0;
     const double bl = __shfl_up(br, 1, 64) * (i > 0 ? 1.0 : 0.0);
+    const double er = on ? vr_in : 0.0, ei = on ? vi_in : 0.0;
     double lo = on ? ai - fabs(bl) - fabs(br) : 1e300;
     double hi = on ? ai + fabs(bl) + fabs(br) : -1e300;
 #pragma unroll
@@ -432,15 +445,18 @@ __device__ __forceinline__ void cheb_expm_lanes(int meff, double dt, double ai_i
     const double sigma = 0.5 * (lo + hi);
     const double rho = 0.5 * (hi - lo);
     const double z = dt * rho;
-    const double e = (i == 0) ? 1.0 : 0.0;
+    // (T - sigma) u for a lane-distributed real vector u
+    auto tmul = [&](double u) {
+        const double ul = __shfl_up(u, 1, 64), ur = __shfl_down(u, 1, 64);
+        return (on ? (ai - sigma) * u : 0.0) + (i > 0 ? bl * ul : 0.0) + br * ur;
+    };
     double rr, ri;
     if (z < 1e-6) {
-        // second order Taylor of exp(-i dt (T - sigma)) e1; error < z^3/6
-        const double te = (i == 0) ? (ai - sigma) : (i == 1 ? bl : 0.0);
-        const double tl = __shfl_up(te, 1, 64), tr = __shfl_down(te, 1, 64);
-        const double t2 = (on ? (ai - sigma) * te : 0.0) + (i > 0 ? bl * tl : 0.0) + br * tr;
-        rr = e - 0.5 * dt * dt * t2;
-        ri = -dt * te;
+        // second order Taylor of exp(-i dt (T - sigma)) v; error < z^3/6
+        const double tr = tmul(er), ti = tmul(ei);
+        const double t2r = tmul(tr), t2i = tmul(ti);
+        rr = er + dt * ti - 0.5 * dt * dt * t2r;
+        ri = ei - dt * tr - 0.5 * dt * dt * t2i;
     } else {
         int K = (int)ceil(1.3 * z) + 18;
         if (K > kChebMax) {
@@ -462,16 +478,17 @@ __device__ __forceinline__ void cheb_expm_lanes(int meff, double dt, double ai_i
                     case 2: cr = -mag; break;
                     default: ci = mag; break;
                 }
+                const double cvr = cr * er - ci * ei, cvi = cr * ei + ci * er;  // c_k v
                 const double lr = __shfl_up(b1r, 1, 64), li = __shfl_up(b1i, 1, 64);
                 const double qr = __shfl_down(b1r, 1, 64), qi = __shfl_down(b1i, 1, 64);
                 const double xbr = xd * b1r + (i > 0 ? xl * lr : 0.0) + xr * qr;
                 const double xbi = xd * b1i + (i > 0 ? xl * li : 0.0) + xr * qi;
-                if (k > 0) {  // b_k = c_k e + 2 x b_{k+1} - b_{k+2}
-                    const double nr = cr * e + 2.0 * xbr - b2r, ni = ci * e + 2.0 * xbi - b2i;
+                if (k > 0) {  // b_k = c_k v + 2 x b_{k+1} - b_{k+2}
+                    const double nr = cvr + 2.0 * xbr - b2r, ni = cvi + 2.0 * xbi - b2i;
                     b2r = b1r; b2i = b1i; b1r = nr; b1i = ni;
-                } else {      // f = c_0 e + x b_1 - b_2
-                    b1r = cr * e + xbr - b2r;
-                    b1i = ci * e + xbi - b2i;
+                } else {      // f = c_0 v + x b_1 - b_2
+                    b1r = cvr + xbr - b2r;
+                    b1i = cvi + xbi - b2i;
                 }
             }
             S += (k == 0) ? jn : ((k & 1) ? 0.0 : 2.0 * jn);
@@ -492,6 +509,11 @@ __device__ __forceinline__ void cheb_expm_lanes(int meff, double dt, double ai_i
     const double pr = cos(dt * sigma), pi = -sin(dt * sigma);
     re = on ? rr * pr - ri * pi : 0.0;
     im = on ? rr * pi + ri * pr : 0.0;
+}
+// exp(-i dt T) e1 (the fresh-basis step of KrylovExpmGeneral / LanczosProcess).
+__device__ __forceinline__ void cheb_expm_lanes(int meff, double dt, double ai_in, double bi_in, double &re,
+                                                double &im, int *status) {
+    cheb_expm_vec(meff, dt, ai_in, bi_in, (threadIdx.x & 63) == 0 ? 1.0 : 0.0, 0.0, re, im, status);
 }
 
 __device__ __forceinline__ double readlane_f64(double v, int lane) {
@@ -622,6 +644,14 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
             yc[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2 + 1] : 0.0;
         }
     const double n0 = a.norm0 ? a.norm0[u] : 0.0;
+    // Basis reuse (one column per workgroup): the tridiagonal of the column's current
+    // basis (lane j: alpha_j, beta_j), its order, h_{m+1,m}, and (wavefront 0) the
+    // coefficients c of the state in that basis, advanced by exp(-i dt T) while kept.
+    const bool reuse_on = P == 1 && a.ktol > 0.0 && a.renorm && !a.relax;
+    double my_al = 0.0, my_be = 0.0, kc_r = 0.0, kc_i = 0.0, hlast = 0.0;
+    int meff = meff0;
+    bool have_basis = false;
+    __shared__ int s_keep;
 
     for (int s = a.s_begin;; ++s) {
         const bool end = (s == a.s_end);
@@ -642,10 +672,45 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
         const double dt = a.dts[s];
         const float *hd = DYN ? a.dyn_head + ((size_t)item * (a.nsamples - 1) + s) * 8 : co;
         // ---- relaxation half-step before (HSPropagator.cpp:137)
-        if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s])
+        if (a.relax && relax_flip(a, col, s, 0u, item))
 #pragma unroll
             for (int g = 0; g < G; ++g) st_flip(psi[g]);
 
+        // ---- keep the basis while h_{m+1,m} |c_{m-1}| <= krylovtol (TaskStaticHSDirectYields.cpp:
+        // 569-652): psi = |psi| V exp(-i dt T) c, with |psi| = the column's initial norm
+        if (reuse_on && have_basis) {
+            if (t < 64) {
+                const double cl = readlane_f64(sqrt(kc_r * kc_r + kc_i * kc_i), meff0 - 1);
+                if (t == 0) s_keep = hlast * (meff0 - 1 < meff ? cl : 0.0) <= a.ktol;
+            }
+            __syncthreads();
+            if (s_keep) {
+                if (t < 64) {
+                    double nr, ni;
+                    cheb_expm_vec(meff, dt, my_al, my_be, kc_r, kc_i, nr, ni, a.status);
+                    kc_r = nr;
+                    kc_i = ni;
+                    if (t < meff) s_coef[t] = make_float2((float)(n0 * nr), (float)(n0 * ni));
+                }
+                __syncthreads();
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) psi[g].c[c] = make_float2(0.f, 0.f);
+                for (int j = 0; j < meff; ++j) {
+                    const float2 cj = s_coef[j];
+                    const float4 *bA = reinterpret_cast<const float4 *>(bas + (size_t)j * 2 * D);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const C4 vj = load_group(bA, bA + D4, g * T + t);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) psi[g].c[c] = cfma(cj, vj.c[c], psi[g].c[c]);
+                    }
+                }
+                __syncthreads();  // s_coef is rewritten at the next step
+                continue;
+            }
+        }
         // ---- Lanczos basis of H from psi (KrylovExpmGeneral, SpinSpace_operators.cpp:1257-1408)
         double nn[1] = {0.0};
 #pragma unroll
@@ -672,9 +737,10 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
                     store_group(bA, bA + D4, nu, v0);
                 }
             }
-            double my_al = 0.0, my_be = 0.0;  // lane j keeps alpha_j and beta_j
+            my_al = 0.0;  // lane j keeps alpha_j and beta_j
+            my_be = 0.0;
             double bprev = 0.0;
-            int meff = live ? meff0 : 1;
+            meff = live ? meff0 : 1;
             bool done = !live;
             int cur = 0;
             for (int j = 0; j < meff0; ++j) {
@@ -722,7 +788,10 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
                         w[g].c[c].y = fmaf(-fa, vj.c[c].y, w[g].c[c].y);
                     }
                 }
-                if (j == meff0 - 1) break;
+                if (j == meff0 - 1) {
+                    hlast = sqrt(fmax(red[1] - alpha * alpha, 0.0));  // h_{m+1,m} of the basis-reuse test
+                    break;
+                }
                 // |w - alpha v|^2 = |w|^2 - alpha^2 for orthonormal v; recomputed when it cancels
                 double beta2 = red[1] - alpha * alpha;
                 const bool need = !(beta2 > 0.02 * red[1]);
@@ -742,6 +811,7 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
                 if constexpr (P == 1) {
                     if (!(beta > 1e-30)) {  // invariant Krylov subspace (happy breakdown)
                         meff = j + 1;
+                        hlast = 0.0;
                         break;
                     }
                 } else if (!done && !(beta > 1e-30)) {
@@ -771,7 +841,10 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
                 cheb_expm_lanes(meff, dt, my_al, my_be, cre, cim, a.status);
                 const double sc = live ? (a.renorm ? n0 : beta0) : 0.0;
                 if (t < meff) s_coef[t] = make_float2((float)(sc * cre), (float)(sc * cim));
+                kc_r = cre;
+                kc_i = cim;
             }
+            have_basis = live;
             __syncthreads();
 #pragma unroll
             for (int g = 0; g < G; ++g)
@@ -789,7 +862,7 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
             }
         }
         // ---- relaxation half-step after (HSPropagator.cpp:139)
-        if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
+        if (a.relax && relax_flip(a, col, s, 1u, item))
 #pragma unroll
             for (int g = 0; g < G; ++g) st_flip(psi[g]);
     }
@@ -908,7 +981,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
             if (end) break;
             const double dt = a.dts[s];
             const float *hd = a.dyn_head ? a.dyn_head + ((size_t)item * (a.nsamples - 1) + s) * 8 : co;
-            if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s])
+            if (a.relax && relax_flip(a, col, s, 0u, item))
 #pragma unroll
                 for (int g = 0; g < G; ++g) st_flip(psi[g]);
             double nn[1] = {0.0};
@@ -1081,7 +1154,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
                     }
                 }
             }
-            if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
+            if (a.relax && relax_flip(a, col, s, 1u, item))
 #pragma unroll
                 for (int g = 0; g < G; ++g) st_flip(psi[g]);
         }
@@ -1161,7 +1234,7 @@ __device__ __forceinline__ void lanczos_big_column(const StepArgs &a, int u, flo
     for (int s = a.s_begin;; ++s) {
         const bool end = (s == a.s_end);
         const bool flip0 = !end && a.relax &&
-                           philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s];
+                           relax_flip(a, col, s, 0u, item);
         // observables at sample s, then the first relaxation half-step and |psi|^2
         {
             float Rf[16];
@@ -1319,12 +1392,11 @@ __device__ __forceinline__ void lanczos_big_column(const StepArgs &a, int u, flo
 #pragma unroll
                         for (int c = 0; c < 4; ++c) x.c[c] = cfma(cj, vj.c[c], x.c[c]);
                     }
-                    if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
+                    if (a.relax && relax_flip(a, col, s, 1u, item))
                         st_flip(x);
                     store_group(PA, PB, nu, x);
                 }
-            } else if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) <
-                                      a.thr[s]) {
+            } else if (a.relax && relax_flip(a, col, s, 1u, item)) {
                 for (int nu = t; nu < ng; nu += T) {
                     C4 x = load_group(PA, PB, nu);
                     st_flip(x);
@@ -1573,7 +1645,7 @@ __global__ __launch_bounds__(T, 4) void k_lanczos_split(const StepArgs *__restri
             double nn = R[16];
             // relaxation half-step before (HSPropagator.cpp:137); the same decision in
             // every workgroup of the slot
-            if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s]) {
+            if (a.relax && relax_flip(a, col, s, 0u, item)) {
 #pragma unroll
                 for (int g = 0; g < G; ++g) st_flip(psi[g]);
                 double q[1] = {norm2(psi)};
@@ -1693,7 +1765,7 @@ __global__ __launch_bounds__(T, 4) void k_lanczos_split(const StepArgs *__restri
                 }
             }
             // relaxation half-step after (HSPropagator.cpp:139)
-            if (a.relax && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
+            if (a.relax && relax_flip(a, col, s, 1u, item))
 #pragma unroll
                 for (int g = 0; g < G; ++g) st_flip(psi[g]);
         }
@@ -1830,7 +1902,7 @@ __device__ __forceinline__ void dense_tiles(const StepArgs &a, int u0, int nact,
             if (t == 0 && act) record_sample(a, u, s, R, ys, yc);
         }
         if (end) break;
-        if (a.relax && colok && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 0u, (uint32_t)item, a.seed) < a.thr[s])
+        if (a.relax && colok && relax_flip(a, col, s, 0u, item))
 #pragma unroll
             for (int k = 0; k < KT; ++k) {
                 C4 g;
@@ -1927,7 +1999,7 @@ __device__ __forceinline__ void dense_tiles(const StepArgs &a, int u0, int nact,
                 xi[k] *= f;
             }
         }
-        if (a.relax && colok && philox_x0((uint32_t)col, (uint32_t)(s + a.rng_step0), 1u, (uint32_t)item, a.seed) < a.thr[s])
+        if (a.relax && colok && relax_flip(a, col, s, 1u, item))
 #pragma unroll
             for (int k = 0; k < KT; ++k) {
                 C4 g;

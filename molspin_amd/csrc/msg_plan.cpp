@@ -85,6 +85,10 @@ void resolve(const Props &p, msg_plan &out) {
         }
         int ks = 16;
         if (p.integer("krylovsize", ks) && ks >= 1) r.krylov_size = ks;
+        // krylovtol (TaskStaticHSDirectYields.cpp:373-410): the Krylov basis is kept while
+        // h_{m+1,m} |c_{m-1}| <= krylovtol; unset or not positive, the reference's 1e-16
+        double kt = 0.0;
+        if (p.num("krylovtol", kt) && std::isfinite(kt) && kt > 0.0) r.krylov_tol = kt;
         int mc = 1;
         if (p.integer("montecarlosamples", mc)) {
             if (mc < 1) throw invalid("montecarlosamples must be at least one");

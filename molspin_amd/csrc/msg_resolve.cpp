@@ -459,15 +459,20 @@ std::string resolve_system(const msg_system_desc *d, Resolved &r) {
         r.symmetric = dev <= 1e-13 * std::max(1.0, std::abs(kap)) && std::fabs(kap.imag()) < 1e-300 + 1e-13;
         r.kappa = kap.real();
     }
-    // relaxation
-    if (d->n_relaxations > 1) return "only one relaxationdephasing operator is supported";
-    if (d->n_relaxations == 1 && d->relaxations[0].rate != 0.0) {
-        const auto &rl = d->relaxations[0];
+    // relaxation: relaxationdephasing channels on the radical-pair electrons; a zero
+    // rate is no channel (HasNonzeroRelaxationRate, SpinSpace_relaxation.cpp:1110-1125)
+    r.relax_rates.clear();
+    r.relax_rate = 0.0;
+    for (int i = 0; i < d->n_relaxations; ++i) {
+        const auto &rl = d->relaxations[i];
+        if (rl.rate == 0.0) continue;
         if (!((rl.spin1 == r.e1 && rl.spin2 == r.e2) || (rl.spin1 == r.e2 && rl.spin2 == r.e1)))
             return "relaxationdephasing must act on the radical-pair electrons";
-        r.relax = true;
-        r.relax_rate = rl.rate;
+        if (!(rl.rate > 0.0) || !std::isfinite(rl.rate)) return "relaxationdephasing rate must be positive and finite";
+        r.relax_rates.push_back(rl.rate);
+        r.relax_rate += rl.rate;
     }
+    r.relax = !r.relax_rates.empty();
     if (d->n_transitions > 0) r.first_rate = d->transitions[0].rate;
     for (const Term1 &t : r.t1)
         if (t.td && t.s != r.e1 && t.s != r.e2) r.td_nuclear = true;
@@ -835,6 +840,26 @@ void expm_dense(int n, const std::vector<cd> &A, std::vector<cd> &E) {
         matmul(n, E, E, tmp);
         E.swap(tmp);
     }
+}
+
+void relax_thresholds(const std::vector<double> &rates, double h, uint32_t t[4]) {
+    t[0] = t[1] = t[2] = t[3] = 0u;
+    if (rates.empty() || !(h > 0.0)) return;
+    if (rates.size() == 1) {  // the single-channel branch (:1224-1232)
+        const double p = -0.5 * std::expm1(-2.0 * (0.5 * rates[0]) * h);
+        t[1] = (uint32_t)std::floor(p * 4294967296.0);
+        return;
+    }
+    double mu = 0.0;  // superposed Poisson process (:1233-1252)
+    for (double r : rates) mu += 0.5 * r * h;
+    // T_n = ceil(F(n) 2^32) (count > n exactly when x >= T_n), held in 32 bits
+    double term = std::exp(-mu), F = term;
+    for (int n = 0; n < 3; ++n) {
+        t[n] = (uint32_t)std::min(4294967295.0, std::ceil(F * 4294967296.0));
+        term *= mu / (n + 1);
+        F += term;
+    }
+    t[3] = 0xFFFFFFFFu;  // count 1 on [T_0, T_1), count 3 on [T_2, 2^32 - 1)
 }
 
 }  // namespace msg

@@ -73,7 +73,8 @@ struct Resolved {
     double kappa = 0.0;
     // relaxation
     bool relax = false;
-    double relax_rate = 0.0;
+    double relax_rate = 0.0;               // sum of the channels' rates
+    std::vector<double> relax_rates;       // relaxationdephasing channels on the electron pair
     // direct factors: rho0 = rho_e (x) 1/Z; columns evec_i (x) e_nu / sqrt(Z)
     std::vector<std::array<cd, 4>> evec;
     int64_t zdim = 0;
@@ -126,5 +127,17 @@ std::string free_times(double total, double dt, std::vector<double> &t);
 
 // Legacy StaticHS grid: k dt for k < ceil(T/dt) (TaskStaticHSDirectYields.cpp:347).
 std::string legacy_times(double total, double dt, std::vector<double> &t);
+
+// Flip ranges of one relaxation half-step of length h for relaxationdephasing
+// channels of rates `rates` on the electron pair (ApplyStochasticRelaxationHilbert,
+// SpinSpace_relaxation.cpp:1211-1254); a column flips when the half-step's uniform
+// word x (Philox) lies in [t[0], t[1]) or [t[2], t[3]).  One channel: the parity of
+// its events, p = -expm1(-2 lambda h) / 2 with lambda = rate / 2, as x < floor(p 2^32).
+// Several channels: the event count of the superposed Poisson process of mean
+// mu = sum lambda_j h, drawn by inversion (count >= n exactly when x >= ceil(F(n-1)
+// 2^32), F the Poisson CDF); every channel applies the same U = P_S - P_T with U^2 = 1,
+// so the column flips when the count is odd (counts above 3 have probability below
+// mu^4 / 24 and are not distinguished).
+void relax_thresholds(const std::vector<double> &rates, double h, uint32_t t[4]);
 
 }  // namespace msg

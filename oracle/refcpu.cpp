@@ -630,6 +630,42 @@ bool flip_draw(uint64_t seed, uint32_t item, uint32_t column, uint32_t step, uin
     return o[0] < thr;
 }
 
+// Several relaxation channels (SpinSpace_relaxation.cpp:1237-1252): the event count of
+// the superposed Poisson process of mean mu = sum_j lambda_j h, lambda_j = rate_j / 2, by
+// inversion of the Philox word x (count > n exactly when x >= ceil(F(n) 2^32), F the
+// Poisson CDF), then per event a channel drawn from the next words with probabilities
+// lambda_j / sum lambda (cumulative search, word / 2^32); applies the events in order.
+void relax_events(const std::vector<std::pair<double, CSR>> &ch, double h, uint64_t seed, uint32_t item,
+                  uint32_t column, uint32_t step, uint32_t half, cd *b, std::vector<cd> &tmp) {
+    const uint32_t ctr[4] = {column, step, half, item};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    philox(ctr, key, o);
+    double tot = 0.0;
+    for (const auto &c : ch) tot += 0.5 * c.first;
+    const double mu = tot * h;
+    // thresholds T_n = ceil(F(n) 2^32) held in 32 bits (2^32 - 1 at most): count 1 on
+    // [T_0, T_1), 3 on [T_2, 2^32 - 1), 0 below T_0 and 2 otherwise (counts above 3 have
+    // probability below mu^4 / 24; the word 2^32 - 1 counts as even)
+    uint32_t T[3];
+    double term = std::exp(-mu), F = term;
+    for (int n = 0; n < 3; ++n) {
+        T[n] = (uint32_t)std::min(4294967295.0, std::ceil(F * 4294967296.0));
+        term *= mu / (n + 1);
+        F += term;
+    }
+    const uint32_t x = o[0];
+    const int count = x < T[0] ? 0 : x < T[1] ? 1 : (x >= T[2] && x < 0xFFFFFFFFu) ? 3 : 2;
+    for (int e = 0; e < count; ++e) {
+        const double v = (double)o[1 + e] / 4294967296.0 * tot;
+        size_t j = 0;
+        double cum = 0.5 * ch[0].first;
+        while (j + 1 < ch.size() && v >= cum) cum += 0.5 * ch[++j].first;
+        ch[j].second.matvec(b, tmp.data());
+        std::copy(tmp.begin(), tmp.begin() + ch[j].second.n, b);
+    }
+}
+
 // U = P_S - P_T for the relaxationdephasing pair (SpinSpace_relaxation.cpp:1144-1199).
 CSR st_unitary(const System &S, int s1, int s2) {
     if (S.mult[s1] != 2 || S.mult[s2] != 2) throw std::runtime_error("relaxationdephasing needs spin-1/2 spins");
@@ -686,6 +722,7 @@ struct ItemModel {
     bool has_relax = false;
     CSR U;
     double relax_rate = 0.0;
+    std::vector<std::pair<double, CSR>> channels;  // (rate, U) of every relaxation channel
 };
 
 ItemModel make_model(const System &S, const double *field, int method, const double *time = nullptr) {
@@ -700,11 +737,18 @@ ItemModel make_model(const System &S, const double *field, int method, const dou
     m.G = T.csr();
     if (method == MSG_METHOD_EXPONENTIAL) m.Gdense = dense_of(m.G);
     const msg_system_desc *d = S.d;
-    if (d->n_relaxations > 1) throw std::runtime_error("the oracle supports one relaxationdephasing operator");
-    if (d->n_relaxations == 1 && d->relaxations[0].rate != 0.0) {
+    // relaxationdephasing channels with a nonzero rate (PrepareStochasticRelaxationHilbert,
+    // SpinSpace_relaxation.cpp:1171-1199)
+    for (int i = 0; i < d->n_relaxations; ++i) {
+        const auto &rl = d->relaxations[i];
+        if (rl.rate == 0.0) continue;
+        if (!(rl.rate > 0.0)) throw std::runtime_error("invalid stochastic relaxation rate");
+        m.channels.push_back({rl.rate, st_unitary(S, rl.spin1, rl.spin2)});
+    }
+    if (!m.channels.empty()) {
         m.has_relax = true;
-        m.relax_rate = d->relaxations[0].rate;
-        m.U = st_unitary(S, d->relaxations[0].spin1, d->relaxations[0].spin2);
+        m.relax_rate = m.channels[0].first;
+        m.U = m.channels[0].second;
     }
     return m;
 }
@@ -755,7 +799,9 @@ void run_block(const System &S, const ItemModel &M, const ObsSet &O, const std::
         for (int c = 0; c < ncol; ++c) {
             cd *b = &B[(size_t)c * D];
             // StepStochastic: R(dt/2) Step(dt) R(dt/2) (HSPropagator.cpp:131-140)
-            if (M.has_relax && flip_draw(seed, item, (uint32_t)(c0 + c), (uint32_t)s, 0u, thr)) {
+            if (M.channels.size() > 1) {
+                relax_events(M.channels, 0.5 * dt, seed, item, (uint32_t)(c0 + c), (uint32_t)s, 0u, b, tmp);
+            } else if (M.has_relax && flip_draw(seed, item, (uint32_t)(c0 + c), (uint32_t)s, 0u, thr)) {
                 M.U.matvec(b, tmp.data()); std::copy(tmp.begin(), tmp.end(), b);
             }
             if (method == MSG_METHOD_KRYLOV) {
@@ -782,7 +828,9 @@ void run_block(const System &S, const ItemModel &M, const ObsSet &O, const std::
                 Gs[2]->G.matvec(tmp.data(), k4.data());
                 for (int64_t i = 0; i < D; ++i) b[i] += (dt / 6.0) * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
             }
-            if (M.has_relax && flip_draw(seed, item, (uint32_t)(c0 + c), (uint32_t)s, 1u, thr)) {
+            if (M.channels.size() > 1) {
+                relax_events(M.channels, 0.5 * dt, seed, item, (uint32_t)(c0 + c), (uint32_t)s, 1u, b, tmp);
+            } else if (M.has_relax && flip_draw(seed, item, (uint32_t)(c0 + c), (uint32_t)s, 1u, thr)) {
                 M.U.matvec(b, tmp.data()); std::copy(tmp.begin(), tmp.end(), b);
             }
         }

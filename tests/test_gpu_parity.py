@@ -668,3 +668,54 @@ def test_legacy_direct_yields_lanczos_d1024(ctx):
     yo = ref.legacy_direct_yields(p, T, dt, correction=True, field=field)
     print("legacy direct D=1024 dev", y[0], "oracle", yo)
     assert np.abs(y[0] - yo).max() <= YIELD_TOL
+
+
+def test_legacy_direct_yields_basis_reuse(ctx):
+    """statichs-direct-yields with krylovtol at D = 1024: each column keeps its Krylov
+    basis while h_{m+1,m} |c_{m-1}| <= krylovtol and advances its coefficients by
+    exp(-i dt T) (TaskStaticHSDirectYields.cpp:569-652), against the oracle's loop with
+    the same tolerance.  At 1e-4 the reuse moves the yields by about 4e-6 from the
+    fresh-basis result (on both sides), so the device follows the oracle's restarts."""
+    idx = next(i for i in range(1000) if workloads.config4_pair(i)[1]["n"] == 8)
+    s, cfg = workloads.config4_pair(idx)
+    p = s.pack()
+    sysd = ctx.upload(p)
+    T, dt = 200.0, 2.0
+    field = cfg["fields"][30]
+    out = {}
+    for tol in (0.0, 1e-6, 1e-4):
+        prm = abi.run_params(total_time=T, time_step=dt, task=abi.MSG_TASK_LEGACY_DIRECT_YIELDS,
+                             yield_corrections=True, method=abi.MSG_METHOD_KRYLOV, krylov_tol=tol)
+        y, _ = sysd.propagate(prm, fields=[field])
+        yo = ref.legacy_direct_yields(p, T, dt, tol=tol, correction=True, field=field)
+        print("krylovtol", tol, "dev", y[0], "oracle", yo, "diff", np.abs(y[0] - yo).max())
+        assert np.abs(y[0] - yo).max() <= YIELD_TOL
+        out[tol] = (y[0], yo)
+    # the reuse changes the result on both sides by the same amount
+    dev_shift, ora_shift = out[1e-4][0] - out[0.0][0], out[1e-4][1] - out[0.0][1]
+    assert np.abs(ora_shift).max() > 1e-6
+    assert np.abs(dev_shift - ora_shift).max() <= 0.5 * np.abs(ora_shift).max()
+
+
+@pytest.mark.parametrize("na,nb", [(2, 2), (5, 3)])
+def test_stochastic_trajectories_two_channels(ctx, na, nb):
+    """Two relaxationdephasing channels on the electron pair (rates 0.015 and 0.01 /ns):
+    the superposed Poisson count (SpinSpace_relaxation.cpp:1237-1252), drawn by
+    inversion of the same Philox word on the device and in the oracle, so trajectories
+    agree one by one; dense path (D = 64) and matrix-free Lanczos (D = 1024)."""
+    s = st_system(na, nb, 0.015, na * 10 + nb + 1)
+    s.add_st_dephasing("E1", "E2", 0.01)
+    p = s.pack()
+    sysd = ctx.upload(p)
+    M = 64
+    cols = ref.coherent_samples(p, s.state_index("S"), 4243, M) / math.sqrt(M)
+    prm = abi.run_params(total_time=400.0, time_step=1.0, sampling=abi.MSG_SAMPLING_GIVEN, seed=101,
+                         method=abi.MSG_METHOD_KRYLOV)
+    y, _ = sysd.propagate(prm, columns=cols)
+    yo, _ = ref.run_hsgeneral(p, prm, columns=cols, threads=8)
+    # the same columns without relaxation: the channels change the yields
+    s0 = st_system(na, nb, 0.0, na * 10 + nb + 1)
+    y0, _ = ref.run_hsgeneral(s0.pack(), prm, columns=cols, threads=8)
+    print("two channels dev", y, "oracle", yo, "no relaxation", y0)
+    assert np.abs(y - yo).max() <= YIELD_TOL
+    assert np.abs(yo - y0).max() > 1e-3

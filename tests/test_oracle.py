@@ -295,6 +295,60 @@ def test_st_dephasing_statistics():
     assert abs((2 * pops[4] - 0.36) / 2 - 0.18 * math.exp(-k * t)) < 6 * sigma
 
 
+def test_st_dephasing_channels_superpose():
+    """Several relaxationdephasing channels on the electron pair
+    (SpinSpace_relaxation.cpp:1237-1252): the superposed Poisson process with the same
+    U = P_S - P_T in every channel flips a column with the single-channel probability of
+    the summed rate."""
+    k1, k2, t, N = 0.4, 0.3, 1.3, 32768
+    s = electron_pair(k1)
+    s.add_st_dephasing("e1", "e2", k2)
+    p = s.pack()
+    cols = np.zeros((4, N), dtype=complex)
+    cols[1, :] = 0.6
+    prm = abi.run_params(calculation=abi.MSG_CALC_TIMEEVOLUTION, observables=abi.MSG_OBS_STATE_POPULATIONS,
+                         sampling=abi.MSG_SAMPLING_GIVEN, total_time=t, time_step=t / 2, seed=77)
+    _, tr = ref.run_hsgeneral(p, prm, columns=cols, trace=True)
+    pops = tr[0, -1] / N
+    k = k1 + k2
+    ptot = -math.expm1(-k * t) / 2.0
+    sigma = 0.36 * math.sqrt(ptot * (1 - ptot) / N)
+    assert abs(pops[0] - 0.18) < 1e-12 and abs(pops[1] - 0.18) < 1e-12
+    assert abs(0.36 * (1 - ptot) - pops[4]) < 6 * sigma
+
+
+def test_relaxation_channels_on_different_pairs():
+    """Channels on two spin pairs that share a spin, A = (e1, e2) and B = (e1, n):
+    U = -SWAP of the pair, so a basis state hops udd <-> dud at rate lambda_A = k_A / 2
+    and udd <-> ddu at rate lambda_B = k_B / 2.  The trajectory average of the udd
+    population follows that three-state Markov chain, which pins the event count and
+    the channel draw (probabilities lambda_j / sum lambda) of the superposed process."""
+    from scipy.linalg import expm
+    kA, kB, t, N = 0.5, 0.2, 2.0, 32768
+    s = SpinSystem("three")
+    s.add_spin("e1", "1/2", electron=True)
+    s.add_spin("e2", "1/2", electron=True)
+    s.add_spin("n", "1/2", tensor=1.0)
+    amp = lambda i: [1.0 if j == i else 0.0 for j in range(8)]
+    for name, i in (("udd", 3), ("dud", 5), ("ddu", 6)):
+        s.add_state(name, ["e1", "e2", "n"], amp(i))
+    s.add_st_dephasing("e1", "e2", kA)
+    s.add_st_dephasing("e1", "n", kB)
+    p = s.pack()
+    cols = np.zeros((8, N), dtype=complex)
+    cols[3, :] = 1.0
+    prm = abi.run_params(calculation=abi.MSG_CALC_TIMEEVOLUTION, observables=abi.MSG_OBS_STATE_POPULATIONS,
+                         sampling=abi.MSG_SAMPLING_GIVEN, total_time=t, time_step=0.25, seed=5)
+    _, tr = ref.run_hsgeneral(p, prm, columns=cols, trace=True)
+    pops = tr[0, -1] / N
+    lA, lB = kA / 2, kB / 2
+    Q = np.array([[-(lA + lB), lA, lB], [lA, -lA, 0.0], [lB, 0.0, -lB]])
+    want = expm(Q * t)[0]
+    sigma = math.sqrt(0.25 / N)
+    assert np.abs(pops - want).max() < 6 * sigma
+    assert abs(pops.sum() - 1.0) < 1e-12
+
+
 def test_coherent_samples_statistics():
     """Spin-coherent nuclear states projected on the singlet: unit norm, singlet
     electronic part, and E|Omega><Omega| = I/(2I+1) per nucleus."""

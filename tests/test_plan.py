@@ -61,6 +61,9 @@ def test_legacy_task_types():
     assert p.params.sampling == abi.MSG_SAMPLING_GIVEN
     assert resolve("type=statichs-symmetricdecay;").params.task == abi.MSG_TASK_SYMMETRIC_DECAY
     assert resolve("type=StaticHS;").params.task == abi.MSG_TASK_SYMMETRIC_DECAY
+    # krylovtol of the legacy direct-yields task (TaskStaticHSDirectYields.cpp:373-410)
+    assert resolve("type=statichs-direct-yields; krylovtol=1e-9;").params.krylov_tol == 1e-9
+    assert resolve("type=statichs-direct-yields;").params.krylov_tol == 0.0
     with pytest.raises(backend.BackendError) as e:
         resolve("type=StaticSS;")
     assert e.value.status == abi.MSG_UNSUPPORTED
