@@ -157,7 +157,7 @@ enum {
 enum {
     MSG_METHOD_AUTO = 0,          /* device chooses: dense U for D<=512, matrix-free Krylov above */
     MSG_METHOD_KRYLOV = 1,        /* propagationmethod=krylov, fixed krylovsize: Lanczos for K = kappa 1,
-                                     Arnoldi on -iH - K for unequal rates (D <= 8192)             */
+                                     Arnoldi on -iH - K for unequal rates (D <= 16384)            */
     MSG_METHOD_EXPONENTIAL = 2,   /* propagationmethod=normal: exact exp(dt G) applied as a GEMM  */
     MSG_METHOD_RK4 = 3,           /* propagationmethod=rk4 (HSPropagator.cpp:80-90): for a static G
                                      one RK4 step is the degree-4 Taylor polynomial of dt G, applied
@@ -353,7 +353,7 @@ msg_status msg_propagate(msg_ctx *ctx, const msg_system *sys, const msg_run_para
  * library-owned scratch of 64 MiB per GPU, held until the process ends; runs that
  * do not fit return MSG_OOM_ADMISSION, and the number of resident columns shrinks to
  * what fits beside the rest.  Matrix-free path only (symmetric recombination with
- * 256 <= D <= 262144, or unequal rates up to D = 8192); the dense path returns
+ * 256 <= D <= 262144, or unequal rates up to D = 16384); the dense path returns
  * MSG_UNSUPPORTED.  Calls on one GPU share the scratch: calls that overlap on one
  * GPU (other host threads, other contexts) race.  Arguments and results as
  * msg_propagate; the same results.

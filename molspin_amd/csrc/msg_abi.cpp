@@ -37,6 +37,7 @@ hipError_t launch_zero_int(int *p, int n, hipStream_t st);
 hipError_t launch_reduce_columns(const double *in, double *out, int n_items, int ncols, int nk, int in_stride, int pick,
                                  hipStream_t st);
 int lanczos_occupancy(int group_per_thread, int threads, size_t lds, bool arnoldi);
+int rk4_occupancy(int group_per_thread, int threads, size_t lds);
 hipError_t launch_zlincomb(const void *X, const void *X2, const void *X3, void *out, int n, int batch,
                            const double *c, hipStream_t st);
 hipError_t launch_blockdiag_right(const void *M, const double *B4, double scale, void *Y, int n, hipStream_t st);
@@ -277,19 +278,27 @@ Path choose_path(const Resolved &r, const msg_run_params *p) {
         r.fast_ok && sym_for(r, p) && r.D >= 256 && r.D <= (int64_t)4 << kMaxNuclei && !dyn_nuclear;
     if (dyn_nuclear && !dense_ok)
         throw ArgError{MSG_UNSUPPORTED, "a time-dependent field on a nucleus needs D <= 512 (dense path)"};
+    if (p->method == MSG_METHOD_RK4 && !dense_ok) {
+        // matrix-free RK4 stages of -iH - K for a static Hamiltonian
+        if (!r.fast_ok) throw ArgError{MSG_UNSUPPORTED, "matrix-free RK4: " + r.why_not_fast};
+        if (p->dynamics && r.has_td)
+            throw ArgError{MSG_UNSUPPORTED, "propagationmethod=rk4 with a time-dependent field needs D <= 512"};
+        if (r.D > 16384) throw ArgError{MSG_UNSUPPORTED, "matrix-free RK4 is limited to D <= 16384"};
+        return Path::Lanczos;
+    }
     if (p->method == MSG_METHOD_EXPONENTIAL || p->method == MSG_METHOD_RK4) {
         if (!dense_ok)
-            throw ArgError{MSG_UNSUPPORTED, "propagationmethod=exponential/rk4 needs D <= 512 on the device"};
+            throw ArgError{MSG_UNSUPPORTED, "propagationmethod=exponential needs D <= 512 on the device"};
         return Path::Dense;
     }
     if (p->method == MSG_METHOD_AUTO && dense_ok) return Path::Dense;
     if (lanczos_ok) return Path::Lanczos;
     if (dense_ok) return Path::Dense;  // exact exponential: same result as Krylov within truncation
     // unequal rates: matrix-free Arnoldi on G = -iH - K while the Krylov vector fits in LDS
-    const bool arnoldi_ok = r.fast_ok && !sym_for(r, p) && r.D <= 8192 && !dyn_nuclear;
+    const bool arnoldi_ok = r.fast_ok && !sym_for(r, p) && r.D <= 16384 && !dyn_nuclear;
     if (arnoldi_ok) return Path::Lanczos;
     if (!sym_for(r, p))
-        throw ArgError{MSG_UNSUPPORTED, "unequal recombination rates need D <= 8192 (matrix-free Arnoldi) with "
+        throw ArgError{MSG_UNSUPPORTED, "unequal recombination rates need D <= 16384 (matrix-free Arnoldi) with "
                                         "spin-1/2 nuclei, or D <= 512 (dense exponential path)"};
     if (!r.fast_ok) throw ArgError{MSG_UNSUPPORTED, "matrix-free path: " + r.why_not_fast};
     throw ArgError{MSG_UNSUPPORTED, "Hilbert dimension outside the matrix-free range"};
@@ -611,10 +620,20 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
             b.G = ng <= 4096 ? -1 : -2;
             b.T = 1024;
         }
-        b.arnoldi = !sym_for(r, p);
+        const bool rk4 = p->method == MSG_METHOD_RK4;
+        b.arnoldi = !sym_for(r, p) && !rk4;
+        if (rk4) {  // one column per workgroup, the stage argument in LDS: 1, 2 or 4 groups per thread
+            b.G = ng <= 512 ? 1 : ng <= 2048 ? 2 : 4;
+            b.T = ng / b.G;
+            a.rk4 = 1;
+            for (int i = 0; i < 16; ++i) {
+                a.kmat[2 * i] = (float)r.K[i].real();
+                a.kmat[2 * i + 1] = (float)r.K[i].imag();
+            }
+        }
         if (b.arnoldi) {  // G = -iH - K: Arnoldi with the Krylov vector in LDS, one group per thread up to D = 4096
-            if (ng > 2048) throw ArgError{MSG_UNSUPPORTED, "matrix-free Arnoldi is limited to D <= 8192"};
-            b.G = ng <= 1024 ? 1 : 2;
+            if (ng > 4096) throw ArgError{MSG_UNSUPPORTED, "matrix-free Arnoldi is limited to D <= 16384"};
+            b.G = ng <= 1024 ? 1 : ng <= 2048 ? 2 : 4;
             b.T = ng / b.G;
             // a tuned, forced or requested shape with 1 or 2 groups per thread: D = 1024,
             // 2048 and 4096 run either
@@ -848,8 +867,8 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     } else if (b.path == Path::Lanczos) {
         const size_t lds = b.G == 0 ? 0 : b.G == -1 ? (size_t)8 * D : b.G == -2 ? (size_t)32 * kTileGroups
                          : b.G == -3 ? (size_t)128 * b.T
-                                     : b.arnoldi ? (size_t)8 * D + 4096 : (size_t)16 * D + 4096;
-        const int occ = std::max(1, lanczos_occupancy(b.G, b.T, lds, b.arnoldi));
+                                     : (b.arnoldi || a.rk4) ? (size_t)8 * D + 4096 : (size_t)16 * D + 4096;
+        const int occ = std::max(1, a.rk4 ? rk4_occupancy(b.G, b.T, lds) : lanczos_occupancy(b.G, b.T, lds, b.arnoldi));
         // split columns: a slot is S workgroups, all resident at once
         const int S = b.G == -3 ? (int)(D / 4 / (2 * b.T)) : 1;
         if (ctx->cus * occ < S) throw ArgError{MSG_UNSUPPORTED, "split column wider than the device"};
@@ -866,7 +885,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
             b.slots = std::min(most, (b.slots + 7) / 8 * 8);
         }
         const int vecs = b.G <= 0 && b.G != -3 ? a.m + 2 : a.m;  // streaming variants: + residual and state
-        if (t_arena && b.G != -3) {  // as many resident columns as the scratch holds beside the rest of the run
+        if (t_arena && b.G != -3 && !a.rk4) {  // as many resident columns as the scratch holds beside the rest of the run
             const size_t rest = (size_t)b.n_units * 8 + sizeof(StepArgs) + (size_t)n_items * (b.nobs + 1) * 8 +
                                 (want_trace ? (size_t)n_items * ns * b.nobs * 8 : 0) + 4 * 256;
             const size_t per = (size_t)vecs * 2 * D * sizeof(float);
@@ -876,7 +895,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                                                       " MiB of the allocation-free scratch"};
             b.slots = std::max(1, std::min<int>(b.slots, (int)(room / per)));
         }
-        a.basis = b.basis.alloc<float>((size_t)b.slots * vecs * 2 * D, stream);
+        a.basis = a.rk4 ? nullptr : b.basis.alloc<float>((size_t)b.slots * vecs * 2 * D, stream);  // RK4: no basis
         if (b.G == -3) {
             a.xred = b.xred.alloc<double>((size_t)b.slots * 2 * S * 17, stream);
             a.bar = b.bar.alloc<int>(b.slots, stream);
@@ -1546,10 +1565,14 @@ msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_param
         msg_launch_shape out{};
         out.path = path == Path::Lanczos ? 1 : 2;
         std::vector<msg_ctx::Shape> cand;
-        if (path == Path::Lanczos && !sym_for(r, &p)) {
-            // Arnoldi (unequal recombination rates): 1 or 2 groups per thread for D = 1024-4096
+        if (path == Path::Lanczos && p.method == MSG_METHOD_RK4) {
+            cand.push_back({0, 0, 0});  // the RK4 kernel has a single launch shape per dimension
+        } else if (path == Path::Lanczos && !sym_for(r, &p)) {
+            // Arnoldi (unequal recombination rates): 1 or 2 groups per thread for D = 1024-4096,
+            // 2 at D = 8192, 4 at D = 16384
             const int ng = (int)(r.D / 4);
-            cand.push_back({ng <= 1024 ? 1 : 2, ng <= 1024 ? ng : ng / 2, 0});
+            const int g0 = ng <= 1024 ? 1 : ng <= 2048 ? 2 : 4;
+            cand.push_back({g0, ng / g0, 0});
             if (ng >= 256 && ng <= 1024) cand.push_back({2, ng / 2, 0});
         } else if (path == Path::Lanczos) {
             const int ng = (int)(r.D / 4);
@@ -1623,7 +1646,7 @@ msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_param
             }
         }
         ctx->force = nullptr;
-        if (path == Path::Lanczos) ctx->tuned[r.D] = best_shape;
+        if (path == Path::Lanczos && p.method != MSG_METHOD_RK4) ctx->tuned[r.D] = best_shape;
         out.kernel_ms = best_ms;
         if (best) *best = out;
     });
@@ -1664,6 +1687,7 @@ msg_status msg_propagate_uneven(msg_ctx *ctx, int32_t n_entries, const msg_uneve
                 throw ArgError{MSG_UNSUPPORTED, at + "unequal recombination rates above D = 512 run through msg_propagate"};
             if (b.a.ktol > 0.0)
                 throw ArgError{MSG_UNSUPPORTED, at + "krylovtol basis reuse runs through msg_propagate"};
+            if (b.a.rk4) throw ArgError{MSG_UNSUPPORTED, at + "matrix-free RK4 runs through msg_propagate"};
             const UnevenShape sh = uneven_shape(b);
             basis_floats = std::max(basis_floats, sh.basis_floats);
             StepArgs a = b.a;

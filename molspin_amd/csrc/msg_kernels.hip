@@ -909,6 +909,133 @@ __global__ __launch_bounds__(TMAX, G >= 4 ? 2 : 4) void k_lanczos(const StepArgs
     }
 }
 
+// Matrix-free RK4 (HSPropagator::Step with propagationmethod=rk4, HSPropagator.cpp:80-90)
+// for one column per workgroup of T threads (G groups per thread, D = 4 G T):
+// k1 = A y, k2 = A (y + dt/2 k1), k3 = A (y + dt/2 k2), k4 = A (y + dt k3),
+// y <- y + dt/6 (k1 + 2 k2 + 2 k3 + k4) with A = -iH - K (x) 1 (K kept inside, as the
+// reference's RK4 does).  Each stage's argument is published to LDS for the operator's
+// neighbour reads; the state and the stage sum stay in registers.
+template <int G, int T>
+__global__ __launch_bounds__(T, G >= 4 ? 1 : 2) void k_rk4(const StepArgs *__restrict__ pa) {
+    const StepArgs &a = *pa;
+    extern __shared__ float4 smem[];
+    constexpr int ng = G * T;
+    float4 *LA = smem, *LB = smem + ng;
+    double *scr = reinterpret_cast<double *>(smem + 2 * ng);  // 2 x 256 doubles
+    __shared__ double ys[kMaxObs], yc[kMaxObs];
+    __shared__ float4 s_blk[kMaxNuclei * 8];
+    __shared__ int s_unit;
+    const int t = threadIdx.x;
+    const int D = a.dim, D4 = D >> 2;
+    int par = 0;
+    auto SCR = [&]() { par ^= 1; return scr + par * 256; };
+    for (;;) {
+        if (t == 0) s_unit = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        __syncthreads();
+        if (u >= a.n_units) break;
+        const int item = u / a.ncols, col = u - item * a.ncols;
+        const float *co = a.coef + (size_t)item * a.coef_stride;
+        fill_block_table(a, co, t, s_blk);  // read after the first barrier below
+        C4 psi[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int nu = g * T + t;
+            if (a.init_mode == 0) {
+                const int which = col / a.zdim, ncol = col - which * a.zdim;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    psi[g].c[c] = (nu == ncol) ? make_float2(a.evec[which * 8 + 2 * c], a.evec[which * 8 + 2 * c + 1])
+                                               : make_float2(0.f, 0.f);
+            } else {
+                const float *src = a.init_mode == 1 ? a.given + (size_t)col * 2 * D : a.state + (size_t)u * 2 * D;
+                const float4 *sA = reinterpret_cast<const float4 *>(src);
+                psi[g] = load_group(sA, sA + D4, nu);
+            }
+        }
+        if (t == 0)
+            for (int c = 0; c < a.nobs; ++c) {
+                ys[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2] : 0.0;
+                yc[c] = a.s_begin > 0 ? a.yacc[((size_t)u * a.nobs + c) * 2 + 1] : 0.0;
+            }
+        for (int s = a.s_begin;; ++s) {
+            const bool end = (s == a.s_end);
+            if (!end || s == a.nsamples - 1) {  // observables at sample s
+                float Rf[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) Rf[k] = 0.f;
+#pragma unroll
+                for (int g = 0; g < G; ++g) accum_rho(psi[g], Rf);
+                double R[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) R[k] = Rf[k];
+                block_sum<16>(R, SCR());
+                if (t == 0) record_sample(a, u, s, R, ys, yc);
+            }
+            if (end) break;
+            const float dt = (float)a.dts[s];
+            if (a.relax && relax_flip(a, col, s, 0u, item))
+#pragma unroll
+                for (int g = 0; g < G; ++g) st_flip(psi[g]);
+            C4 acc[G], arg[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) arg[g] = psi[g];
+            for (int st = 0; st < 4; ++st) {
+                const float wst = (st == 0 || st == 3) ? 1.f : 2.f, ast = st == 2 ? 1.f : 0.5f;
+                __syncthreads();  // the previous stage's neighbour reads are done
+#pragma unroll
+                for (int g = 0; g < G; ++g) store_group(LA, LB, g * T + t, arg[g]);
+                __syncthreads();
+                C4 hv[G];
+                apply_h_groups<Log2<ng>::value, G, Log2<T>::value, true>(a, co, co, t, arg, LA, LB, hv, 0, nullptr,
+                                                                         nullptr, s_blk);
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {  // k = -i H x - K x
+                        float2 kv = make_float2(0.f, 0.f);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            kv = cfma(make_float2(a.kmat[8 * r + 2 * c], a.kmat[8 * r + 2 * c + 1]), arg[g].c[c], kv);
+                        const float2 k = make_float2(hv[g].c[r].y - kv.x, -hv[g].c[r].x - kv.y);
+                        acc[g].c[r] = st == 0 ? make_float2(k.x, k.y)
+                                              : make_float2(fmaf(wst, k.x, acc[g].c[r].x),
+                                                            fmaf(wst, k.y, acc[g].c[r].y));
+                        if (st < 3)
+                            arg[g].c[r] = make_float2(fmaf(ast * dt, k.x, psi[g].c[r].x),
+                                                      fmaf(ast * dt, k.y, psi[g].c[r].y));
+                    }
+            }
+            const float h6 = dt / 6.f;
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    psi[g].c[c] = make_float2(fmaf(h6, acc[g].c[c].x, psi[g].c[c].x), fmaf(h6, acc[g].c[c].y, psi[g].c[c].y));
+            if (a.relax && relax_flip(a, col, s, 1u, item))
+#pragma unroll
+                for (int g = 0; g < G; ++g) st_flip(psi[g]);
+        }
+        if (t == 0)
+            for (int c = 0; c < a.nobs; ++c) {
+                a.yacc[((size_t)u * a.nobs + c) * 2] = ys[c];
+                a.yacc[((size_t)u * a.nobs + c) * 2 + 1] = yc[c];
+            }
+        if (a.state) {
+            float4 *dA = reinterpret_cast<float4 *>(a.state + (size_t)u * 2 * D);
+#pragma unroll
+            for (int g = 0; g < G; ++g) store_group(dA, dA + D4, g * T + t, psi[g]);
+        }
+        bool bad = false;
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bad |= !isfinite(psi[g].c[c].x) || !isfinite(psi[g].c[c].y);
+        if (bad) atomicOr(a.status, 1);
+    }
+}
+
 // Matrix-free Arnoldi for the non-Hermitian generator G = -iH - K (x) 1 of unequal
 // recombination rates (KrylovExpmGeneral / ArnoldiProcess, SpinSpace_operators.cpp:
 // 1257-1408): psi <- beta V_m exp(dt H_m) e1 with the m x m upper Hessenberg H_m of
@@ -2558,8 +2685,11 @@ int uneven_occupancy() {
     X(1, 64, true) X(1, 128, true) X(1, 256, true) X(1, 512, true) X(1, 1024, true) X(2, 512, true) X(2, 1024, true)
 // shapes of the Lanczos kernel only (more groups per thread for tuning; no Arnoldi form)
 #define MSG_LANCZOS_EXTRA(X) X(2, 128, true) X(2, 256, true) X(4, 256, true)
-// and of the Arnoldi kernel (two groups per thread at D = 1024 and 2048)
-#define MSG_ARNOLDI_EXTRA(X) X(2, 128, true) X(2, 256, true)
+// and of the Arnoldi kernel (two groups per thread at D = 1024 and 2048, four at
+// D = 16384: the Krylov vector takes 128 KiB of LDS)
+#define MSG_ARNOLDI_EXTRA(X) X(2, 128, true) X(2, 256, true) X(4, 1024, true)
+// shapes of the RK4 kernel (D = 1024 ... 16384)
+#define MSG_RK4_VARIANTS(X) X(1, 256) X(1, 512) X(2, 512) X(2, 1024) X(4, 1024)
 
 // Split-column kernels for D = 4 2^NN, NN = 12..16, with 1024 threads (512-thread
 // workgroups, two per CU over twice as many slabs, measured slower at D = 16384 and
@@ -2590,6 +2720,17 @@ hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_t
         else
             hipLaunchKernelGGL(k_lanczos_big<false>, dim3(slots), dim3(1024), 0, st, da);
         return hipGetLastError();
+    }
+    if (a.rk4) {
+        const size_t lds_r = (size_t)2 * a.dim * sizeof(float) + 2 * 256 * sizeof(double);
+#define X(G_, T_)                                                                                 \
+        if (group_per_thread == G_ && threads == T_) {                                            \
+            hipLaunchKernelGGL((k_rk4<G_, T_>), dim3(slots), dim3(threads), lds_r, st, da);       \
+            return hipGetLastError();                                                              \
+        }
+        MSG_RK4_VARIANTS(X)
+#undef X
+        return hipErrorInvalidValue;
     }
     if (a.arnoldi) {
         const size_t lds_a = (size_t)2 * a.dim * sizeof(float) + 2 * 256 * sizeof(double);
@@ -2653,6 +2794,9 @@ hipError_t set_lanczos_lds_limits() {
                               maxb - 8192);
     (void)hipFuncSetAttribute((const void *)k_uneven, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)uneven_lds_bytes());
+#define X(G_, T_) (void)hipFuncSetAttribute((const void *)k_rk4<G_, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    MSG_RK4_VARIANTS(X)
+#undef X
     (void)hipGetLastError();
     return hipSuccess;
 }
@@ -2667,6 +2811,16 @@ hipError_t launch_dense(const StepArgs &a, const StepArgs *da, int slots, hipStr
     MSG_DENSE_VARIANTS(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+int rk4_occupancy(int group_per_thread, int threads, size_t lds) {
+    int nb = 0;
+#define X(G_, T_)                                                                                           \
+    if (group_per_thread == G_ && threads == T_)                                                            \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_rk4<G_, T_>, threads, lds) == hipSuccess ? nb : 1;
+    MSG_RK4_VARIANTS(X)
+#undef X
+    return 1;
 }
 
 int lanczos_occupancy(int group_per_thread, int threads, size_t lds, bool arnoldi) {

@@ -1,4 +1,4 @@
-"""Matrix-free Arnoldi on G = -iH - K (unequal recombination rates, D up to 8192)
+"""Matrix-free Arnoldi on G = -iH - K (unequal recombination rates, D up to 16384)
 against the oracle's KrylovExpmGeneral restatement (Arnoldi, modified Gram-Schmidt,
 exp(dt H_m) e1), same Krylov order.  Tolerances as for the Lanczos path:
 |dPhi| <= 1e-5 on yields, 1e-4 max-relative on traces."""
@@ -137,11 +137,30 @@ def test_arnoldi_dynamic_field(ctx):
     assert np.abs(y - yo).max() <= YIELD_TOL
 
 
-def test_arnoldi_limit(ctx):
-    """Unequal rates above D = 8192 stay refused, naming the limit."""
+def test_arnoldi_d16384_given_columns(ctx):
+    """D = 16384 (four groups per thread, the Krylov vector in 128 KiB of LDS):
+    singlet-only recombination on trace-sample columns, yields and traces against
+    the oracle."""
     s = pair(8, 4, (2e-3, 0.0, 0.0, 0.0))
+    p = s.pack()
+    sysd = ctx.upload(p)
+    assert sysd.dim == 16384
+    M = 4
+    cols = ref.coherent_samples(p, s.state_index("S"), 13, M) / math.sqrt(M)
+    prm = abi.run_params(calculation=abi.MSG_CALC_TIMEEVOLUTION, method=abi.MSG_METHOD_KRYLOV,
+                         sampling=abi.MSG_SAMPLING_GIVEN, total_time=30.0, time_step=2.0)
+    y, tr = sysd.propagate(prm, fields=FIELDS[1:2], columns=cols, trace=True)
+    yo, tro = ref.run_hsgeneral(p, prm, fields=FIELDS[1:2], columns=cols, trace=True, threads=8)
+    print("D=16384 arnoldi", y[0], yo[0], "trace err", trace_err(tr, tro))
+    assert np.abs(y - yo).max() <= YIELD_TOL
+    assert trace_err(tr, tro) <= TRACE_TOL
+
+
+def test_arnoldi_limit(ctx):
+    """Unequal rates above D = 16384 stay refused, naming the limit."""
+    s = pair(9, 4, (2e-3, 0.0, 0.0, 0.0))
     sysd = ctx.upload(s.pack())
     with pytest.raises(backend.BackendError) as e:
         sysd.propagate(abi.run_params(method=abi.MSG_METHOD_KRYLOV, total_time=4.0, time_step=2.0),
                        fields=FIELDS[:1])
-    assert e.value.status == abi.MSG_UNSUPPORTED and "8192" in str(e.value)
+    assert e.value.status == abi.MSG_UNSUPPORTED and "16384" in str(e.value)

@@ -719,3 +719,30 @@ def test_stochastic_trajectories_two_channels(ctx, na, nb):
     print("two channels dev", y, "oracle", yo, "no relaxation", y0)
     assert np.abs(y - yo).max() <= YIELD_TOL
     assert np.abs(yo - y0).max() > 1e-3
+
+
+@pytest.mark.parametrize("n,cols", [(8, None), (10, 8), (12, 4)])
+def test_rk4_matrix_free(ctx, n, cols):
+    """propagationmethod=rk4 above D = 512 (HSPropagator.cpp:80-90): the matrix-free
+    RK4 stages of -iH - K with K inside (singlet sink faster than the triplet ones),
+    D = 1024 on all direct columns, D = 4096 and 16384 on trace-sample columns,
+    against the oracle's RK4 loop: yields and traces."""
+    idx = next(i for i in range(1000) if workloads.config4_pair(i)[1]["n"] == n)
+    s, cfg = workloads.config4_pair(idx)
+    for t in s.transitions:
+        t["rate"] = 4e-3 if s.states[t["source"]]["name"] == "S" else 1e-3
+    p = s.pack()
+    sysd = ctx.upload(p)
+    kw = {}
+    sampling = abi.MSG_SAMPLING_DIRECT
+    if cols is not None:
+        kw["columns"] = ref.coherent_samples(p, s.state_index("S"), 21, cols) / math.sqrt(cols)
+        sampling = abi.MSG_SAMPLING_GIVEN
+    prm = abi.run_params(calculation=abi.MSG_CALC_TIMEEVOLUTION, method=abi.MSG_METHOD_RK4, total_time=40.0,
+                         time_step=0.25, sampling=sampling)
+    fields = [cfg["fields"][40]]
+    y, tr = sysd.propagate(prm, fields=fields, trace=True, **kw)
+    yo, tro = ref.run_hsgeneral(p, prm, fields=fields, trace=True, threads=8, **kw)
+    print("rk4 D", sysd.dim, "dev", y[0], "oracle", yo[0], "trace err", trace_err(tr, tro))
+    assert np.abs(y - yo).max() <= YIELD_TOL
+    assert trace_err(tr, tro) <= TRACE_TOL
