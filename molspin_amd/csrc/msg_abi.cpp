@@ -38,6 +38,8 @@ hipError_t launch_reduce_columns(const double *in, double *out, int n_items, int
                                  hipStream_t st);
 int lanczos_occupancy(int group_per_thread, int threads, size_t lds, bool arnoldi);
 int rk4_occupancy(int group_per_thread, int threads, size_t lds);
+int mix_occupancy(int mult, int threads, size_t lds);
+bool mix_dbuf(int mult, int threads);
 hipError_t launch_zlincomb(const void *X, const void *X2, const void *X3, void *out, int n, int batch,
                            const double *c, hipStream_t st);
 hipError_t launch_blockdiag_right(const void *M, const double *B4, double scale, void *Y, int n, hipStream_t st);
@@ -274,13 +276,15 @@ Path choose_path(const Resolved &r, const msg_run_params *p) {
     // a time-dependent field on a nucleus changes per-nucleus coefficients, which the
     // matrix-free head cannot carry; the dense path builds each interval's propagator
     const bool dyn_nuclear = p->dynamics && r.has_td && r.td_nuclear;
-    const bool lanczos_ok =
-        r.fast_ok && sym_for(r, p) && r.D >= 256 && r.D <= (int64_t)4 << kMaxNuclei && !dyn_nuclear;
+    const bool mixed = r.mix_mult > 0;  // one nucleus of spin > 1/2 on the group index (k_lanczos_mix)
+    const bool lanczos_ok = r.fast_ok && sym_for(r, p) && r.D >= 256 && r.D <= (int64_t)4 << kMaxNuclei &&
+                            !dyn_nuclear && !(mixed && p->dynamics && r.has_td);
     if (dyn_nuclear && !dense_ok)
         throw ArgError{MSG_UNSUPPORTED, "a time-dependent field on a nucleus needs D <= 512 (dense path)"};
     if (p->method == MSG_METHOD_RK4 && !dense_ok) {
         // matrix-free RK4 stages of -iH - K for a static Hamiltonian
         if (!r.fast_ok) throw ArgError{MSG_UNSUPPORTED, "matrix-free RK4: " + r.why_not_fast};
+        if (mixed) throw ArgError{MSG_UNSUPPORTED, "matrix-free RK4 takes spin-1/2 nuclei only"};
         if (p->dynamics && r.has_td)
             throw ArgError{MSG_UNSUPPORTED, "propagationmethod=rk4 with a time-dependent field needs D <= 512"};
         if (r.D > 16384) throw ArgError{MSG_UNSUPPORTED, "matrix-free RK4 is limited to D <= 16384"};
@@ -295,7 +299,7 @@ Path choose_path(const Resolved &r, const msg_run_params *p) {
     if (lanczos_ok) return Path::Lanczos;
     if (dense_ok) return Path::Dense;  // exact exponential: same result as Krylov within truncation
     // unequal rates: matrix-free Arnoldi on G = -iH - K while the Krylov vector fits in LDS
-    const bool arnoldi_ok = r.fast_ok && !sym_for(r, p) && r.D <= 16384 && !dyn_nuclear;
+    const bool arnoldi_ok = r.fast_ok && !mixed && !sym_for(r, p) && r.D <= 16384 && !dyn_nuclear;
     if (arnoldi_ok) return Path::Lanczos;
     if (!sym_for(r, p))
         throw ArgError{MSG_UNSUPPORTED, "unequal recombination rates need D <= 16384 (matrix-free Arnoldi) with "
@@ -622,6 +626,11 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         }
         const bool rk4 = p->method == MSG_METHOD_RK4;
         b.arnoldi = !sym_for(r, p) && !rk4;
+        if (r.mix_mult > 0) {  // spin-1/2 nuclei on the thread index, the other nucleus on the group index
+            b.G = r.mix_mult;
+            b.T = ng / r.mix_mult;
+            a.mixm = r.mix_mult;
+        }
         if (rk4) {  // one column per workgroup, the stage argument in LDS: 1, 2 or 4 groups per thread
             b.G = ng <= 512 ? 1 : ng <= 2048 ? 2 : 4;
             b.T = ng / b.G;
@@ -867,8 +876,11 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
     } else if (b.path == Path::Lanczos) {
         const size_t lds = b.G == 0 ? 0 : b.G == -1 ? (size_t)8 * D : b.G == -2 ? (size_t)32 * kTileGroups
                          : b.G == -3 ? (size_t)128 * b.T
-                                     : (b.arnoldi || a.rk4) ? (size_t)8 * D + 4096 : (size_t)16 * D + 4096;
-        const int occ = std::max(1, a.rk4 ? rk4_occupancy(b.G, b.T, lds) : lanczos_occupancy(b.G, b.T, lds, b.arnoldi));
+                         : a.mixm ? (size_t)(mix_dbuf(b.G, b.T) ? 16 : 8) * D + 4096
+                         : (b.arnoldi || a.rk4) ? (size_t)8 * D + 4096 : (size_t)16 * D + 4096;
+        const int occ = std::max(1, a.mixm  ? mix_occupancy(b.G, b.T, lds)
+                                    : a.rk4 ? rk4_occupancy(b.G, b.T, lds)
+                                            : lanczos_occupancy(b.G, b.T, lds, b.arnoldi));
         // split columns: a slot is S workgroups, all resident at once
         const int S = b.G == -3 ? (int)(D / 4 / (2 * b.T)) : 1;
         if (ctx->cus * occ < S) throw ArgError{MSG_UNSUPPORTED, "split column wider than the device"};
@@ -1565,7 +1577,7 @@ msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_param
         msg_launch_shape out{};
         out.path = path == Path::Lanczos ? 1 : 2;
         std::vector<msg_ctx::Shape> cand;
-        if (path == Path::Lanczos && p.method == MSG_METHOD_RK4) {
+        if (path == Path::Lanczos && (p.method == MSG_METHOD_RK4 || r.mix_mult > 0)) {
             cand.push_back({0, 0, 0});  // the RK4 kernel has a single launch shape per dimension
         } else if (path == Path::Lanczos && !sym_for(r, &p)) {
             // Arnoldi (unequal recombination rates): 1 or 2 groups per thread for D = 1024-4096,
@@ -1646,7 +1658,7 @@ msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_param
             }
         }
         ctx->force = nullptr;
-        if (path == Path::Lanczos && p.method != MSG_METHOD_RK4) ctx->tuned[r.D] = best_shape;
+        if (path == Path::Lanczos && p.method != MSG_METHOD_RK4 && r.mix_mult == 0) ctx->tuned[r.D] = best_shape;
         out.kernel_ms = best_ms;
         if (best) *best = out;
     });
@@ -1688,6 +1700,8 @@ msg_status msg_propagate_uneven(msg_ctx *ctx, int32_t n_entries, const msg_uneve
             if (b.a.ktol > 0.0)
                 throw ArgError{MSG_UNSUPPORTED, at + "krylovtol basis reuse runs through msg_propagate"};
             if (b.a.rk4) throw ArgError{MSG_UNSUPPORTED, at + "matrix-free RK4 runs through msg_propagate"};
+            if (b.a.mixm)
+                throw ArgError{MSG_UNSUPPORTED, at + "a nucleus with spin > 1/2 above D = 512 runs through msg_propagate"};
             const UnevenShape sh = uneven_shape(b);
             basis_floats = std::max(basis_floats, sh.basis_floats);
             StepArgs a = b.a;

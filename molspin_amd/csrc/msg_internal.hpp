@@ -50,6 +50,8 @@ struct StepArgs {
     int coef_stride;
     int arnoldi;            // 1: Arnoldi on G = -iH - K (unequal recombination rates)
     int rk4;                // 1: RK4 stages of -iH - K instead of the Krylov step (k_rk4)
+    int mixm;               // > 0: the last nucleus has this multiplicity (3 or 4) and is the
+                            // group index of k_lanczos_mix; the others are spin-1/2
     float kmat[32];         // K on the electron pair, 4x4 complex row-major (Arnoldi)
     const float *dyn_head;  // [n_items][nsamples-1][8] head (w1, w2, shift) of each interval of a
                             // time-dependent operator, or NULL (static: the head of `coef`)

@@ -214,7 +214,12 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
 // bits, local index g 2^LOGT + t) is in LDS (LA, LB); nuclei on bits from SB up
 // (a column split over several workgroups, base = slab 2^SB) read their neighbours
 // from the whole vector in global memory (GA, GB).
-template <int NN, int G, int LOGT, bool TBL = false>
+// MIXM > 0: the last nucleus (n = LOGT, NN = LOGT + 1) has multiplicity MIXM = G and is
+// the thread's group index g, with magnetic quantum number m_g = (G - 1)/2 - g; its
+// diagonal terms scale by 2 m_g and its blocks between g and g + 1 by
+// <m_g|I+|m_g - 1> = sqrt(I(I+1) - m_g (m_g - 1)) (Spin.cpp:330-410), applied within
+// the thread's registers.
+template <int NN, int G, int LOGT, bool TBL = false, int MIXM = 0>
 __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *__restrict__ co,
                                                const float *__restrict__ hd, int t, const C4 (&x)[G],
                                                const float4 *LA, const float4 *LB, C4 (&y)[G], int base = 0,
@@ -268,6 +273,53 @@ __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *_
                 for (int g = 0; g < G; ++g)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) y[g].c[c] = cfma(tt, nb[g].c[c], y[g].c[c]);
+            }
+        } else if constexpr (MIXM > 0) {
+            static_assert(MIXM == G && NN == LOGT + 1, "the nucleus of multiplicity G is the group index");
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const float sg = (float)(G - 1 - 2 * g);  // 2 m_g
+                if (fl & NUC_E1) {
+                    e1[g][0] = fmaf(sg, cn[0], e1[g][0]); e1[g][1] = fmaf(sg, cn[1], e1[g][1]);
+                    e1[g][2] = fmaf(sg, cn[2], e1[g][2]);
+                }
+                if (fl & NUC_E2) {
+                    e2[g][0] = fmaf(sg, cn[3], e2[g][0]); e2[g][1] = fmaf(sg, cn[4], e2[g][1]);
+                    e2[g][2] = fmaf(sg, cn[5], e2[g][2]);
+                }
+                if (fl & NUC_T) er[g] = fmaf(sg, cn[6], er[g]);
+            }
+#pragma unroll
+            for (int g = 0; g + 1 < G; ++g) {
+                // <m_g| I+ |m_g - 1> with m_g = (G - 1)/2 - g: sqrt(I(I+1) - m_g (m_g - 1))
+                constexpr float ii = 0.25f * (G * G - 1);
+                const float mg = 0.5f * (G - 1) - g;
+                const float cp = sqrtf(ii - mg * (mg - 1.f));
+                float2 up[4], dn[4];
+                auto scale = [&](float2 (&q)[4]) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) q[k] = make_float2(cp * q[k].x, cp * q[k].y);
+                };
+                if (fl & NUC_E1) {
+                    load_block(cn + 7, false, up); load_block(cn + 7, true, dn);
+                    scale(up); scale(dn);
+                    apply_e1(up, x[g + 1], y[g]);
+                    apply_e1(dn, x[g], y[g + 1]);
+                }
+                if (fl & NUC_E2) {
+                    load_block(cn + 15, false, up); load_block(cn + 15, true, dn);
+                    scale(up); scale(dn);
+                    apply_e2(up, x[g + 1], y[g]);
+                    apply_e2(dn, x[g], y[g + 1]);
+                }
+                if (fl & NUC_T) {
+                    const float2 tu = make_float2(cp * cn[23], cp * cn[24]), td = make_float2(cp * cn[23], -cp * cn[24]);
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        y[g].c[c] = cfma(tu, x[g + 1].c[c], y[g].c[c]);
+                        y[g + 1].c[c] = cfma(td, x[g].c[c], y[g + 1].c[c]);
+                    }
+                }
             }
         } else {
 #pragma unroll
@@ -596,10 +648,13 @@ __device__ __forceinline__ bool any_guard(const double *scr, int W, int P) {
 // for itself), and a subgroup whose Krylov space closed early (happy breakdown)
 // idles through the remaining iterations.  A column's arithmetic is therefore the
 // same whatever P and its neighbours are.
-template <int G, int T, int P, bool DBUF, bool DYN>
+template <int G, int T, int P, bool DBUF, bool DYN, bool MIX = false>
 __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int nact, float *bas0, float4 *smem,
                                                 double *scr, const BodyShared &sh) {
     constexpr int ng = G * T, W = T / 64;
+    // nuclei: log2(D / 4), or with MIX the spin-1/2 nuclei on the thread index plus the
+    // nucleus of multiplicity G on the group index
+    constexpr int NNUC = MIX ? Log2<T>::value + 1 : Log2<ng>::value;
     static_assert(T % 64 == 0 && P * T <= 1024 && P <= 16, "subgroups of whole waves");
     const int sg = P == 1 ? 0 : (int)threadIdx.x / T;
     const int t = (int)threadIdx.x - sg * T, lane = threadIdx.x & 63, w0 = sg * W;
@@ -754,8 +809,8 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
                 double red[2] = {0.0, 0.0};
 #pragma unroll
                 for (int g = 0; g < G; ++g) vjs[g] = load_group(cA, cB, g * T + t);
-                apply_h_groups<Log2<ng>::value, G, Log2<T>::value, true>(a, co, hd, t, vjs, cA, cB, w, 0, nullptr,
-                                                                         nullptr, sh.blk);
+                apply_h_groups<NNUC, G, Log2<T>::value, true, MIX ? G : 0>(a, co, hd, t, vjs, cA, cB, w, 0, nullptr,
+                                                                           nullptr, sh.blk);
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
                     const int nu = g * T + t;
@@ -906,6 +961,31 @@ __global__ __launch_bounds__(TMAX, G >= 4 ? 2 : 4) void k_lanczos(const StepArgs
         __syncthreads();
         if (u >= a.n_units) break;
         lanczos_columns<G, TMAX, 1, DBUF, DYN>(a, u, 1, bas, smem, scr, sh);
+    }
+}
+
+// One nucleus of spin above 1/2 (multiplicity M = 3 or 4) beside spin-1/2 nuclei:
+// D = 4 M T, the spin-1/2 nuclei on the thread index (T = 2^n threads), the other
+// nucleus on the group index (M groups per thread).
+template <int M, int T, bool DBUF>
+__global__ __launch_bounds__(T, 1) void k_lanczos_mix(const StepArgs *__restrict__ pa) {
+    const StepArgs &a = *pa;
+    extern __shared__ float4 smem[];
+    constexpr int ng = M * T;
+    double *scr = reinterpret_cast<double *>(smem + (DBUF ? 4 : 2) * ng);  // 2 x 16 x 16 doubles
+    __shared__ double ys[kMaxObs], yc[kMaxObs];
+    __shared__ float2 s_coef[kMaxKrylov];
+    __shared__ float4 s_blk[kMaxNuclei * 8];
+    __shared__ int s_unit;
+    const BodyShared sh{ys, yc, s_coef, s_blk, nullptr};
+    float *bas = a.basis + (size_t)blockIdx.x * a.m * 2 * a.dim;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        __syncthreads();
+        if (u >= a.n_units) break;
+        lanczos_columns<M, T, 1, DBUF, false, true>(a, u, 1, bas, smem, scr, sh);
     }
 }
 
@@ -2688,6 +2768,11 @@ int uneven_occupancy() {
 // and of the Arnoldi kernel (two groups per thread at D = 1024 and 2048, four at
 // D = 16384: the Krylov vector takes 128 KiB of LDS)
 #define MSG_ARNOLDI_EXTRA(X) X(2, 128, true) X(2, 256, true) X(4, 1024, true)
+// shapes of the kernel with one nucleus of spin above 1/2 (M groups per thread, T threads;
+// both Krylov vectors in LDS up to 96 KiB)
+#define MSG_MIX_VARIANTS(X) \
+    X(3, 64, true) X(3, 128, true) X(3, 256, true) X(3, 512, true) X(3, 1024, false) \
+    X(4, 64, true) X(4, 128, true) X(4, 256, true) X(4, 512, false) X(4, 1024, false)
 // shapes of the RK4 kernel (D = 1024 ... 16384)
 #define MSG_RK4_VARIANTS(X) X(1, 256) X(1, 512) X(2, 512) X(2, 1024) X(4, 1024)
 
@@ -2720,6 +2805,17 @@ hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_t
         else
             hipLaunchKernelGGL(k_lanczos_big<false>, dim3(slots), dim3(1024), 0, st, da);
         return hipGetLastError();
+    }
+    if (a.mixm) {
+#define X(M_, T_, B_)                                                                                          \
+        if (a.mixm == M_ && threads == T_) {                                                                   \
+            const size_t lds_m = (size_t)(B_ ? 4 : 2) * a.dim * sizeof(float) + 2 * 256 * sizeof(double);     \
+            hipLaunchKernelGGL((k_lanczos_mix<M_, T_, B_>), dim3(slots), dim3(threads), lds_m, st, da);        \
+            return hipGetLastError();                                                                           \
+        }
+        MSG_MIX_VARIANTS(X)
+#undef X
+        return hipErrorInvalidValue;
     }
     if (a.rk4) {
         const size_t lds_r = (size_t)2 * a.dim * sizeof(float) + 2 * 256 * sizeof(double);
@@ -2797,6 +2893,10 @@ hipError_t set_lanczos_lds_limits() {
 #define X(G_, T_) (void)hipFuncSetAttribute((const void *)k_rk4<G_, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     MSG_RK4_VARIANTS(X)
 #undef X
+#define X(M_, T_, B_) \
+    (void)hipFuncSetAttribute((const void *)k_lanczos_mix<M_, T_, B_>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    MSG_MIX_VARIANTS(X)
+#undef X
     (void)hipGetLastError();
     return hipSuccess;
 }
@@ -2811,6 +2911,23 @@ hipError_t launch_dense(const StepArgs &a, const StepArgs *da, int slots, hipStr
     MSG_DENSE_VARIANTS(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+int mix_occupancy(int mult, int threads, size_t lds) {
+    int nb = 0;
+#define X(M_, T_, B_)                                                                                           \
+    if (mult == M_ && threads == T_)                                                                            \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_mix<M_, T_, B_>, threads, lds) == hipSuccess ? nb : 1;
+    MSG_MIX_VARIANTS(X)
+#undef X
+    return 1;
+}
+bool mix_dbuf(int mult, int threads) {
+#define X(M_, T_, B_) \
+    if (mult == M_ && threads == T_) return B_;
+    MSG_MIX_VARIANTS(X)
+#undef X
+    return false;
 }
 
 int rk4_occupancy(int group_per_thread, int threads, size_t lds) {

@@ -331,8 +331,11 @@ std::string resolve_system(const msg_system_desc *d, Resolved &r) {
         r.state_amps.push_back(amps);
     }
     if (!r.pair_ok) return "";  // structure not representable on the device paths; caller reports
-    for (int i = 0; i < r.nspins; ++i)
-        if (i != r.e1 && i != r.e2) r.nuc.push_back(i);
+    // nuclei in device digit order: the spin-1/2 nuclei first (system order), then the
+    // others, so that a single nucleus of higher spin is the most significant digit
+    for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < r.nspins; ++i)
+            if (i != r.e1 && i != r.e2 && ((r.mult[i] == 2) == (pass == 0))) r.nuc.push_back(i);
     r.zdim = r.D / 4;
     // MolSpin index -> device index (nu * 4 + 2a + b, nucleus 0 least significant)
     r.perm.resize(r.D);
@@ -424,8 +427,22 @@ std::string resolve_system(const msg_system_desc *d, Resolved &r) {
     }
     // structure of the matrix-free operator
     r.fast_ok = true;
-    for (int k : r.nuc)
-        if (r.mult[k] != 2) { r.fast_ok = false; r.why_not_fast = "nucleus with spin > 1/2"; }
+    r.mix_mult = 0;
+    {
+        int nhalf = 0, nbig = 0;
+        for (int k : r.nuc) {
+            if (r.mult[k] == 2) ++nhalf;
+            else { ++nbig; r.mix_mult = r.mult[k]; }
+        }
+        // one nucleus of spin 1 or 3/2 rides on the group index of the matrix-free kernel,
+        // beside 6 to 10 spin-1/2 nuclei on the thread index
+        if (nbig > 1) { r.fast_ok = false; r.why_not_fast = "more than one nucleus with spin > 1/2"; }
+        else if (nbig == 1 && r.mix_mult > 4) { r.fast_ok = false; r.why_not_fast = "nucleus with spin > 3/2"; }
+        else if (nbig == 1 && (nhalf < 6 || nhalf > 10)) {
+            r.fast_ok = false;
+            r.why_not_fast = "a nucleus with spin > 1/2 needs 6 to 10 spin-1/2 nuclei beside it";
+        }
+    }
     if ((int)r.nuc.size() > kMaxNuclei) { r.fast_ok = false; r.why_not_fast = "too many nuclei"; }
     for (const Term2 &t : r.t2) {
         const bool e_a = t.s1 == r.e1 || t.s1 == r.e2, e_b = t.s2 == r.e1 || t.s2 == r.e2;

@@ -72,6 +72,7 @@ struct Resolved {
     bool symmetric = false;            // K = kappa * 1
     double kappa = 0.0;
     // relaxation
+    int mix_mult = 0;  // multiplicity of the one nucleus with spin > 1/2 on the matrix-free path (0: none)
     bool relax = false;
     double relax_rate = 0.0;               // sum of the channels' rates
     std::vector<double> relax_rates;       // relaxationdephasing channels on the electron pair

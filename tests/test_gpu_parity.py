@@ -746,3 +746,39 @@ def test_rk4_matrix_free(ctx, n, cols):
     print("rk4 D", sysd.dim, "dev", y[0], "oracle", yo[0], "trace err", trace_err(tr, tro))
     assert np.abs(y - yo).max() <= YIELD_TOL
     assert trace_err(tr, tro) <= TRACE_TOL
+
+
+def mixed_system(n_half, mult, seed):
+    """Radical pair with n_half spin-1/2 nuclei and one nucleus of multiplicity `mult`
+    (spin 1 or 3/2, like FAD's N5, example_2_5_ps.msd) with anisotropic hyperfine."""
+    rng = workloads.SplitMix64(91, seed)
+    spin = {3: "1", 4: "3/2"}[mult]
+    hf = [workloads._aniso_hf(rng) for _ in range(n_half + 1)]
+    na = (n_half + 1) // 2
+    s = workloads._radical_pair("mix", [spin] + ["1/2"] * (na - 1), ["1/2"] * (n_half + 1 - na), hf, 1e-3,
+                                swept=True)
+    return s
+
+
+@pytest.mark.parametrize("n_half,mult,cols", [(7, 3, None), (6, 4, None), (10, 3, 4)])
+def test_matrix_free_spin1_nucleus(ctx, n_half, mult, cols):
+    """A nucleus of spin 1 or 3/2 beside 6-10 spin-1/2 nuclei on the matrix-free
+    Lanczos path (Spin.cpp:330-410: I+ elements sqrt(I(I+1) - m(m-1))): D = 1536 and
+    1024 on all direct columns at two field orientations, D = 12288 on trace-sample
+    columns, against the oracle."""
+    s = mixed_system(n_half, mult, n_half * 10 + mult)
+    p = s.pack()
+    sysd = ctx.upload(p)
+    assert sysd.dim == 4 * mult * 2 ** n_half
+    fields = [workloads.fibonacci_point(64, i, 0.05) for i in (5, 40)]
+    kw = {}
+    sampling = abi.MSG_SAMPLING_DIRECT
+    if cols is not None:
+        kw["columns"] = ref.coherent_samples(p, s.state_index("S"), 3, cols) / math.sqrt(cols)
+        sampling = abi.MSG_SAMPLING_GIVEN
+        fields = fields[:1]
+    prm = abi.run_params(total_time=40.0, time_step=2.0, method=abi.MSG_METHOD_KRYLOV, sampling=sampling)
+    y, _ = sysd.propagate(prm, fields=fields, **kw)
+    yo, _ = ref.run_hsgeneral(p, prm, fields=fields, threads=8, **kw)
+    print("mixed D", sysd.dim, "dev", y[:, 0], "oracle", yo[:, 0], "max diff", np.abs(y - yo).max())
+    assert np.abs(y - yo).max() <= YIELD_TOL
