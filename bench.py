@@ -66,10 +66,16 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--requests", type=int, default=40)
     ap.add_argument("--serve-intervals", type=int, default=5)
+    ap.add_argument("--copy-gib", type=float, default=None,
+                    help="only time a device-to-device copy of this many GiB (three times) and exit: the kernel of "
+                         "known size that rocprofv3's FETCH_SIZE and WRITE_SIZE are calibrated on")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_gib is not None:
+        device_copy(args.copy_gib)
+        return
 
     from molspin_amd import backend
 
@@ -79,6 +85,24 @@ def main():
     finally:
         ctx.close()  # release the library's device state before interpreter teardown
     print(json.dumps(line), flush=True)
+
+
+def device_copy(gib):
+    """Three device-to-device copies of `gib` GiB (read + write of that many bytes
+    each); prints the copy rate as one JSON line."""
+    import torch
+    n = int(gib * (1 << 30))
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    ms = []
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    print(json.dumps({"copy_bytes": n, "ms": ms, "read_plus_write_GBps": [2 * n / (t * 1e6) for t in ms]}), flush=True)
 
 
 def load_fixture(name):

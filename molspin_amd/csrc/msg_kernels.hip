@@ -96,8 +96,20 @@ __device__ __forceinline__ void apply_field_e1(float fx, float fy, float fz, con
 __device__ __forceinline__ void apply_field_e2(float fx, float fy, float fz, const C4 &x, C4 &y) {
     apply_field<1, 2>(fx, fy, fz, x, y);
 }
+// Read-only data that is the same for a whole wavefront (an item's operator
+// coefficients, the nucleus flags) is read through the constant address space: with
+// a wave-uniform address the compiler then loads it with scalar loads into SGPRs,
+// which the FMAs take as operands, instead of vector loads into VGPRs.
+template <typename T>
+using ConstPtr = const __attribute__((address_space(4))) T *;
+template <typename T>
+__device__ __forceinline__ ConstPtr<T> as_const(const T *p) {
+    return (ConstPtr<T>)p;
+}
+
 // Loads a 2x2 complex block, conjugate-transposed if `herm` (lane-selected).
-__device__ __forceinline__ void load_block(const float *p, bool herm, float2 m[4]) {
+template <typename P>
+__device__ __forceinline__ void load_block(P p, bool herm, float2 m[4]) {
     const float2 a = make_float2(p[0], p[1]), b = make_float2(p[2], p[3]);
     const float2 c = make_float2(p[4], p[5]), d = make_float2(p[6], p[7]);
     m[0] = herm ? conjf2(a) : a;
@@ -148,15 +160,17 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
                                       int nu, const C4 &x, const float4 *LA, const float4 *LB,
                                       const float4 *GA = nullptr, const float4 *GB = nullptr,
                                       const float4 *blk = nullptr) {
-    float f1x = hd[0], f1y = hd[1], f1z = hd[2];
-    float f2x = hd[3], f2y = hd[4], f2z = hd[5];
-    float rr = hd[6];
+    const auto hc = as_const(hd), cc = as_const(co);
+    const auto flags = as_const(a.nuc_flags);
+    float f1x = hc[0], f1y = hc[1], f1z = hc[2];
+    float f2x = hc[3], f2y = hc[4], f2z = hc[5];
+    float rr = hc[6];
     C4 y;
 #pragma unroll
     for (int c = 0; c < 4; ++c) y.c[c] = make_float2(0.f, 0.f);
     auto nucleus = [&](int n) {
-        const float *cn = co + kCoefHead + n * kNucStride;
-        const int fl = a.nuc_flags[n];
+        const auto cn = cc + kCoefHead + n * kNucStride;
+        const int fl = flags[n];
         const bool down = (nu >> n) & 1;
         const float s = down ? -1.f : 1.f;
         const C4 nb = !TILED ? load_group(LA, LB, nu ^ (1 << n))
@@ -196,7 +210,7 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
     apply_field_e1(f1x, f1y, f1z, x, y);
     apply_field_e2(f2x, f2y, f2z, x, y);
     if (a.op_flags & OP_EREST) {
-        const float *E = co + 8;
+        const auto E = cc + 8;
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
@@ -226,14 +240,16 @@ __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *_
                                                const float4 *GA = nullptr, const float4 *GB = nullptr,
                                                const float4 *blk = nullptr) {
     constexpr int SB = LOGT + Log2<G>::value;
+    const auto hc = as_const(hd), cc = as_const(co);
+    const auto flags = as_const(a.nuc_flags);
     // the thread's coupling block of nucleus n on electron e (TBL: from the LDS table)
-    auto block = [&](int n, int e, bool down, const float *cb, float2 m[4]) {
+    auto block = [&](int n, int e, bool down, ConstPtr<float> cb, float2 m[4]) {
         if constexpr (TBL) table_block(blk, n, e, down, m);
         else load_block(cb, down, m);
     };
-    float f1x = hd[0], f1y = hd[1], f1z = hd[2];
-    float f2x = hd[3], f2y = hd[4], f2z = hd[5];
-    float rr = hd[6];
+    float f1x = hc[0], f1y = hc[1], f1z = hc[2];
+    float f2x = hc[3], f2y = hc[4], f2z = hc[5];
+    float rr = hc[6];
     float e1[G][3], e2[G][3], er[G];  // contributions of the nuclei on g's bits
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -245,8 +261,8 @@ __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *_
     }
 #pragma unroll
     for (int n = 0; n < NN; ++n) {
-        const float *cn = co + kCoefHead + n * kNucStride;
-        const int fl = a.nuc_flags[n];
+        const auto cn = cc + kCoefHead + n * kNucStride;
+        const int fl = flags[n];
         float2 m[4];
         if (n < LOGT) {
             const bool down = (t >> n) & 1;
@@ -361,7 +377,7 @@ __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *_
         apply_field_e1(f1x + e1[g][0], f1y + e1[g][1], f1z + e1[g][2], x[g], y[g]);
         apply_field_e2(f2x + e2[g][0], f2y + e2[g][1], f2z + e2[g][2], x[g], y[g]);
         if (a.op_flags & OP_EREST) {
-            const float *E = co + 8;
+            const auto E = cc + 8;
 #pragma unroll
             for (int r = 0; r < 4; ++r)
 #pragma unroll
