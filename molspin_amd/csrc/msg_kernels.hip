@@ -260,7 +260,10 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
 // diagonal terms scale by 2 m_g and its blocks between g and g + 1 by
 // <m_g|I+|m_g - 1> = sqrt(I(I+1) - m_g (m_g - 1)) (Spin.cpp:330-410), applied within
 // the thread's registers.
-template <int NN, int G, int LOGT, bool TBL = false, int MIXM = 0>
+// PART 1 forms y from the workgroup's slab alone: every diagonal term and the flips of
+// the nuclei on bits below SB; PART 2 adds the flips of the nuclei on bits from SB up
+// (neighbours from GA, GB) to y.  PART 0 does both.
+template <int NN, int G, int LOGT, bool TBL = false, int MIXM = 0, int PART = 0>
 __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *__restrict__ co,
                                                const float *__restrict__ hd, int t, const C4 (&x)[G],
                                                const float4 *LA, const float4 *LB, C4 (&y)[G], int base = 0,
@@ -280,14 +283,16 @@ __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *_
     float e1[G][3], e2[G][3], er[G];  // contributions of the nuclei on g's bits
 #pragma unroll
     for (int g = 0; g < G; ++g) {
+        if constexpr (PART != 2)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) y[g].c[c] = make_float2(0.f, 0.f);
+            for (int c = 0; c < 4; ++c) y[g].c[c] = make_float2(0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 3; ++k) e1[g][k] = e2[g][k] = 0.f;
         er[g] = 0.f;
     }
 #pragma unroll
     for (int n = 0; n < NN; ++n) {
+        if (PART == 2 && n < SB) continue;
         const auto cn = cc + kCoefHead + n * kNucStride;
         const int fl = flags[n];
         float2 m[4];
@@ -370,29 +375,42 @@ __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *_
                 const int hi = base + (g << LOGT);  // the group's index bits from LOGT up
                 const bool down = (hi >> n) & 1;
                 const float s = down ? -1.f : 1.f;
-                const C4 nb = n < SB ? load_group(LA, LB, ((g << LOGT) ^ (1 << n)) + t)
-                                     : load_group(GA, GB, (hi ^ (1 << n)) + t);
+                const bool diag = PART != 2, flips = PART == 0 || (PART == 1) == (n < SB);
+                C4 nb;
+                if (flips)
+                    nb = n < SB ? load_group(LA, LB, ((g << LOGT) ^ (1 << n)) + t) : load_group(GA, GB, (hi ^ (1 << n)) + t);
                 if (fl & NUC_E1) {
-                    e1[g][0] = fmaf(s, cn[0], e1[g][0]); e1[g][1] = fmaf(s, cn[1], e1[g][1]);
-                    e1[g][2] = fmaf(s, cn[2], e1[g][2]);
-                    load_block(cn + 7, down, m);
-                    apply_e1(m, nb, y[g]);
+                    if (diag) {
+                        e1[g][0] = fmaf(s, cn[0], e1[g][0]); e1[g][1] = fmaf(s, cn[1], e1[g][1]);
+                        e1[g][2] = fmaf(s, cn[2], e1[g][2]);
+                    }
+                    if (flips) {
+                        load_block(cn + 7, down, m);
+                        apply_e1(m, nb, y[g]);
+                    }
                 }
                 if (fl & NUC_E2) {
-                    e2[g][0] = fmaf(s, cn[3], e2[g][0]); e2[g][1] = fmaf(s, cn[4], e2[g][1]);
-                    e2[g][2] = fmaf(s, cn[5], e2[g][2]);
-                    load_block(cn + 15, down, m);
-                    apply_e2(m, nb, y[g]);
+                    if (diag) {
+                        e2[g][0] = fmaf(s, cn[3], e2[g][0]); e2[g][1] = fmaf(s, cn[4], e2[g][1]);
+                        e2[g][2] = fmaf(s, cn[5], e2[g][2]);
+                    }
+                    if (flips) {
+                        load_block(cn + 15, down, m);
+                        apply_e2(m, nb, y[g]);
+                    }
                 }
                 if (fl & NUC_T) {
-                    er[g] = fmaf(s, cn[6], er[g]);
-                    const float2 tt = make_float2(cn[23], down ? -cn[24] : cn[24]);
+                    if (diag) er[g] = fmaf(s, cn[6], er[g]);
+                    if (flips) {
+                        const float2 tt = make_float2(cn[23], down ? -cn[24] : cn[24]);
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) y[g].c[c] = cfma(tt, nb.c[c], y[g].c[c]);
+                        for (int c = 0; c < 4; ++c) y[g].c[c] = cfma(tt, nb.c[c], y[g].c[c]);
+                    }
                 }
             }
         }
     }
+    if constexpr (PART == 2) return;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const float rg = rr + er[g];
@@ -1773,11 +1791,16 @@ __global__ void k_reduce_columns(const double *in, double *out, int n_items, int
 // XCD (blockIdx.x mod 8 selects the XCD), whose L2 they share: every wave waits for
 // its own stores to reach L2 (L1 writes through), and after the arrival count is
 // complete one thread invalidates the CU's L1 for the whole workgroup.
-__device__ __forceinline__ bool slot_barrier(int *bar, int S, int &nbar, int *status, int *s_flag) {
+// The barrier in two halves: slot_arrive counts this workgroup in once its stores are
+// in L2; slot_wait waits for the whole slot.  Work between the halves that reads only
+// the workgroup's own data overlaps the other workgroups' arrival.
+__device__ __forceinline__ void slot_arrive(int *bar) {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's global stores are in L2
     __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool slot_wait(int *bar, int S, int &nbar, int *status, int *s_flag) {
     if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const int target = S * (nbar + 1);
         int ok = 1;
         for (int spins = 0; __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++spins) {
@@ -1794,6 +1817,10 @@ __device__ __forceinline__ bool slot_barrier(int *bar, int S, int &nbar, int *st
     ++nbar;
     __syncthreads();
     return *s_flag != 0;
+}
+__device__ __forceinline__ bool slot_barrier(int *bar, int S, int &nbar, int *status, int *s_flag) {
+    slot_arrive(bar);
+    return slot_wait(bar, S, nbar, status, s_flag);
 }
 
 // Sums K workgroup totals (every thread holds them) over the slot's S workgroups in
@@ -1923,7 +1950,7 @@ __global__ __launch_bounds__(T, 4) void k_lanczos_split(const StepArgs *__restri
                         store_group(bA, bA + D4, base + g * T + t, v0);
                     }
                 }
-                if (!slot_barrier(bar, S, nbar, a.status, &s_flag)) return;  // v_0 complete in the slot
+                slot_arrive(bar);  // v_0 stored; the slot waits before the first cross-slab read
                 double my_al = 0.0, my_be = 0.0;  // lane j keeps alpha_j and beta_j
                 double bprev = 0.0;
                 int meff = meff0;
@@ -1938,8 +1965,13 @@ __global__ __launch_bounds__(T, 4) void k_lanczos_split(const StepArgs *__restri
                     double red[2] = {0.0, 0.0};
 #pragma unroll
                     for (int g = 0; g < G; ++g) vjs[g] = load_group(cA, cB, g * T + t);
-                    // lane-selected blocks: the LDS block table measured 8 % slower in this kernel
-                    apply_h_groups<NN, G, Log2<T>::value>(a, co, hd, t, vjs, cA, cB, w, base, gA, gA + D4);
+                    // the slab's part of H v_j while the slot's other workgroups store
+                    // theirs, then the nuclei across slabs once v_j is complete in the slot
+                    // (lane-selected blocks: the LDS block table measured 8 % slower here)
+                    apply_h_groups<NN, G, Log2<T>::value, false, 0, 1>(a, co, hd, t, vjs, cA, cB, w, base);
+                    if (!slot_wait(bar, S, nbar, a.status, &s_flag)) return;
+                    apply_h_groups<NN, G, Log2<T>::value, false, 0, 2>(a, co, hd, t, vjs, cA, cB, w, base, gA,
+                                                                       gA + D4);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
                         if (j > 0) {
@@ -1994,7 +2026,7 @@ __global__ __launch_bounds__(T, 4) void k_lanczos_split(const StepArgs *__restri
                     }
                     bprev = beta;
                     cur ^= 1;
-                    if (!slot_barrier(bar, S, nbar, a.status, &s_flag)) return;  // v_{j+1} complete in the slot
+                    slot_arrive(bar);  // v_{j+1} stored
                 }
                 // wavefront 0 evaluates the small exponential (the same in every
                 // workgroup of the slot); psi = beta V c over the own slab
