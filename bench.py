@@ -674,6 +674,7 @@ def serve(ctx, n_requests, intervals, method="auto", compare_plain=True):
                 lat.append(time.perf_counter() - t0 - reqs[i]["arrival"])
                 reqs[i]["result"] = y
     elapsed = time.perf_counter() - t0
+    shared = srv.stats()
     srv.close()
     lat = np.array(lat)
     total = sum(r["steps"] for r in reqs)
@@ -693,7 +694,8 @@ def serve(ctx, n_requests, intervals, method="auto", compare_plain=True):
     if not compare_plain:
         return {"requests": n_requests, "p50_ms": float(np.percentile(lat, 50) * 1e3),
                 "p99_ms": float(np.percentile(lat, 99) * 1e3), "state_steps_per_s": total / elapsed,
-                "elapsed_s": elapsed, "method": method, "parity": check, "run_size": reduced}
+                "elapsed_s": elapsed, "method": method, "parity": check, "run_size": reduced,
+                "shared_launches": shared[0], "requests_in_shared_launches": shared[1]}
     plain_s = 0.0
     for r in reqs:
         r["sys"].propagate(r["prm"], fields=[r["fields"][0]])  # first call per system uploads factors
@@ -723,6 +725,7 @@ def serve(ctx, n_requests, intervals, method="auto", compare_plain=True):
             "p99_ms": float(np.percentile(lat, 99) * 1e3), "state_steps_per_s": total / elapsed,
             "plain_one_at_a_time_state_steps_per_s": plain, "served_over_plain": total / elapsed / plain,
             "small_systems_D_le_512": by_class, "parity": check, "run_size": reduced,
+            "shared_launches": shared[0], "requests_in_shared_launches": shared[1],
             "elapsed_s": elapsed, "intervals_per_request": intervals,
             "workload": "Poisson 200 req/s, seeded radical pairs (D 16..16384), 16-128 fields/request"}
 

@@ -493,7 +493,11 @@ msg_status msg_server_submit(msg_server *srv, const msg_system *sys, const msg_r
 /* Collects finished requests (without blocking), then admits queued requests
  * while a slot and the HBM budget allow, shortest first (state-steps x D; arrival
  * order among equals).  A request with method MSG_METHOD_UNCOUPLED is computed
- * within the call (at most one per call).  max_steps <= 0: every admitted request
+ * within the call (at most one per call).  max_steps <= 0: queued requests that
+ * the uneven launch takes (msg_propagate_uneven: static Hamiltonian, symmetric
+ * recombination above D = 512) and that are small enough to fit about 50 ms of
+ * device time together share one persistent launch on one stream, smallest first;
+ * every other admitted request
  * is propagated over its whole grid on its own stream, in persistent launches of
  * a few fields each (about 50 ms of device time: a large request does not hold the
  * device while small ones wait), so requests overlap on the device.
@@ -504,6 +508,9 @@ msg_status msg_server_pump(msg_server *srv, int32_t max_steps, int32_t *n_active
 /* Copies the yields (n_fields x n_obs) of a finished request and forgets it.
  * Returns MSG_INVALID_ARG if the request is unknown, *done = 0 if unfinished. */
 msg_status msg_server_collect(msg_server *srv, int64_t request_id, double *yields, int32_t *done);
+/* Counts since the server was created: shared launches and the requests they held.
+ * (No counterpart in the reference: it reports how the server batched the stream.) */
+msg_status msg_server_stats(const msg_server *srv, int64_t *shared_launches, int64_t *shared_requests);
 
 #ifdef __cplusplus
 }

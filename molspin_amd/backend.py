@@ -21,7 +21,7 @@ EXPORTS = [
     "msg_server_submit", "msg_server_pump", "msg_server_collect", "msg_last_kernel_time",
     "msg_trace_samples", "msg_symmetric_decay", "msg_resolve_plan", "msg_autotune", "msg_time_infinity",
     "msg_msd_compile", "msg_msd_system", "msg_msd_steps", "msg_msd_task_count", "msg_msd_task", "msg_msd_destroy",
-    "msg_propagate_uneven", "msg_propagate_noalloc",
+    "msg_propagate_uneven", "msg_propagate_noalloc", "msg_server_stats",
 ]
 
 
@@ -57,6 +57,7 @@ def lib():
         L.msg_server_submit.argtypes = [vp, vp, P(abi.RunParams), C.c_int32, vp, P(C.c_int64)]
         L.msg_server_pump.argtypes = [vp, C.c_int32, P(C.c_int32)]
         L.msg_server_collect.argtypes = [vp, C.c_int64, vp, P(C.c_int32)]
+        L.msg_server_stats.argtypes = [vp, P(C.c_int64), P(C.c_int64)]
         L.msg_last_kernel_time.argtypes = [vp, P(C.c_double), P(C.c_int32)]
         L.msg_trace_samples.argtypes = [vp, C.c_int32, C.c_int32, C.c_uint64, C.c_int32, vp]
         L.msg_symmetric_decay.argtypes = [vp, vp, C.c_int32, vp, C.c_double, vp]
@@ -356,6 +357,12 @@ class Server:
         n = C.c_int32()
         self.ctx.check(lib().msg_server_pump(self.h, int(max_steps), C.byref(n)))
         return n.value
+
+    def stats(self):
+        """(shared launches, requests they held) since the server was created."""
+        a, b = C.c_int64(), C.c_int64()
+        self.ctx.check(lib().msg_server_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def collect(self, rid):
         system, params, nf, nobs = self._keep[rid]

@@ -617,7 +617,12 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                 b.G = g;
                 b.T = ng / g;
             }
-            if ((g == 0 && ng > 2048) || (g == -1 && ng == 4096) || (g == -3 && ng >= 4096)) { b.G = g; b.T = 1024; }
+            if ((g == 0 && ng > 2048) || (g == -1 && ng == 4096)) { b.G = g; b.T = 1024; }
+            if (g == -3 && ng >= 4096) {  // split columns: 512 threads, 1024 with MSG_SPLIT_T=1024
+                const char *st = std::getenv("MSG_SPLIT_T");
+                b.G = -3;
+                b.T = st && std::atoi(st) == 1024 ? 1024 : 512;
+            }
             if (g == -2 && ng > kTileGroups) { b.G = -2; b.T = 1024; }
         }
         if (b.G == -3 && (b.no_split || ng < 4096)) {  // back to the default streaming shape
@@ -875,14 +880,14 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         if (a.renorm) HIPCHECK(launch_init_norms(a, push_args(b, a), stream));
     } else if (b.path == Path::Lanczos) {
         const size_t lds = b.G == 0 ? 0 : b.G == -1 ? (size_t)8 * D : b.G == -2 ? (size_t)32 * kTileGroups
-                         : b.G == -3 ? (size_t)128 * b.T
+                         : b.G == -3 ? (size_t)4 * kSplitSlab * 16
                          : a.mixm ? (size_t)(mix_dbuf(b.G, b.T) ? 16 : 8) * D + 4096
                          : (b.arnoldi || a.rk4) ? (size_t)8 * D + 4096 : (size_t)16 * D + 4096;
         const int occ = std::max(1, a.mixm  ? mix_occupancy(b.G, b.T, lds)
                                     : a.rk4 ? rk4_occupancy(b.G, b.T, lds)
                                             : lanczos_occupancy(b.G, b.T, lds, b.arnoldi));
         // split columns: a slot is S workgroups, all resident at once
-        const int S = b.G == -3 ? (int)(D / 4 / (2 * b.T)) : 1;
+        const int S = b.G == -3 ? (int)(D / 4 / kSplitSlab) : 1;
         if (ctx->cus * occ < S) throw ArgError{MSG_UNSUPPORTED, "split column wider than the device"};
         b.slots = std::max(1, std::min(b.n_units, ctx->cus * occ / S));
         if (ts && ts->slots > 0) b.slots = std::max(1, std::min(b.slots, ts->slots));
@@ -1162,6 +1167,132 @@ UnevenShape uneven_shape(const Batch &b) {
     return s;
 }
 
+// One uneven launch in flight: the entries' batches, the launch's own buffers and
+// the host copies its asynchronous uploads read.
+struct UnevenRun {
+    std::vector<std::unique_ptr<Batch>> bs;
+    std::vector<StepArgs> ents;
+    std::vector<UnevenPack> packs;
+    UnevenArgs ua{};
+    DBuf dents, dpacks, dqueue, dbasis, dargs;
+};
+
+// Whether a request of `sys` under `p` can go into an uneven launch (the checks of
+// uneven_begin made before anything is allocated).
+bool uneven_eligible(const msg_system *sys, const msg_run_params *p) {
+    const Resolved &r = sys->r;
+    if (p->method == MSG_METHOD_UNCOUPLED || p->calculation != MSG_CALC_YIELDS) return false;
+    if (p->dynamics && r.has_td) return false;
+    Path path;
+    try {
+        path = choose_path(r, p);
+    } catch (const ArgError &) {
+        return false;
+    }
+    if (path == Path::Dense) return r.D <= 512;
+    const int64_t ng = r.D / 4;
+    return sym_for(r, p) && p->method != MSG_METHOD_RK4 && !(p->krylov_tol > 0.0) && r.mix_mult == 0 && ng >= 64 &&
+           ng <= 4096 && (ng & (ng - 1)) == 0;
+}
+
+// Prepares every entry on stream st and queues one persistent k_uneven launch over all
+// of their (item, column) units, packed per class and ordered largest first.  Nothing
+// is synchronised: the results are valid once the stream has passed the launch
+// (uneven_finish).  `timed`: HIP events of the context around the launch.
+void uneven_begin(UnevenRun &run, msg_ctx *ctx, int n_entries, const msg_uneven_entry *entries, hipStream_t st,
+                  bool timed) {
+    run.bs.resize(n_entries);
+    run.ents.resize(n_entries);
+    struct Ordered {
+        UnevenPack p;
+        double cost;
+    };
+    std::vector<Ordered> order;
+    size_t basis_floats = 0;
+    for (int e = 0; e < n_entries; ++e) {
+        const msg_uneven_entry &en = entries[e];
+        const std::string at = "entry " + std::to_string(e) + ": ";
+        if (!en.system || !en.params || !en.yields) throw ArgError{MSG_INVALID_ARG, at + "null system, parameters or yields"};
+        if (en.params->method == MSG_METHOD_UNCOUPLED)
+            throw ArgError{MSG_UNSUPPORTED, at + "propagationmethod=uncoupled runs through msg_propagate"};
+        if (en.params->dynamics && en.system->r.has_td)
+            throw ArgError{MSG_UNSUPPORTED, at + "the uneven batch takes static Hamiltonians"};
+        run.bs[e].reset(new Batch());
+        Batch &b = *run.bs[e];
+        b.uneven = true;
+        try {
+            prepare(b, ctx, en.system, en.params, en.n_items, en.fields, en.columns, en.n_columns, en.trace != nullptr,
+                    false, st);
+        } catch (const ArgError &x) {
+            throw ArgError{x.st, at + x.what};
+        }
+        if (b.arnoldi)
+            throw ArgError{MSG_UNSUPPORTED, at + "unequal recombination rates above D = 512 run through msg_propagate"};
+        if (b.a.ktol > 0.0) throw ArgError{MSG_UNSUPPORTED, at + "krylovtol basis reuse runs through msg_propagate"};
+        if (b.a.rk4) throw ArgError{MSG_UNSUPPORTED, at + "matrix-free RK4 runs through msg_propagate"};
+        if (b.a.mixm)
+            throw ArgError{MSG_UNSUPPORTED, at + "a nucleus with spin > 1/2 above D = 512 runs through msg_propagate"};
+        const UnevenShape sh = uneven_shape(b);
+        basis_floats = std::max(basis_floats, sh.basis_floats);
+        StepArgs a = b.a;
+        a.s_begin = 0;
+        a.s_end = b.nsamples - 1;
+        run.ents[e] = a;
+        // packs of consecutive units of one item, cost ~ device time of the pack
+        const int upi = b.n_units / b.n_items;
+        const double D = a.dim, steps = b.nsamples - 1;
+        for (int it = 0; it < b.n_items; ++it)
+            for (int c = 0; c < upi; c += sh.per_pack) {
+                const int nact = std::min(sh.per_pack, upi - c);
+                const double cost = b.path == Path::Lanczos ? nact * D * steps * a.m * (a.nnuc + 4)
+                                                            : nact * 16.0 * a.dpad * a.dpad * steps / 16.0;
+                order.push_back({UnevenPack{e, it * upi + c, nact, sh.cls}, cost});
+            }
+    }
+    std::stable_sort(order.begin(), order.end(), [](const Ordered &x, const Ordered &y) { return x.cost > y.cost; });
+    run.packs.resize(order.size());
+    for (size_t k = 0; k < order.size(); ++k) run.packs[k] = order[k].p;
+    const int n_packs = (int)run.packs.size();
+    const int occ = std::max(1, uneven_occupancy());
+    const int wgs = std::max(1, std::min(n_packs, ctx->cus * occ));
+    StepArgs *de = run.dents.alloc<StepArgs>(run.ents.size(), st);
+    HIPCHECK(hipMemcpyAsync(de, run.ents.data(), run.ents.size() * sizeof(StepArgs), hipMemcpyHostToDevice, st));
+    UnevenPack *dp = run.dpacks.alloc<UnevenPack>(run.packs.size(), st);
+    HIPCHECK(hipMemcpyAsync(dp, run.packs.data(), run.packs.size() * sizeof(UnevenPack), hipMemcpyHostToDevice, st));
+    UnevenArgs &ua = run.ua;
+    ua.ents = de;
+    ua.packs = dp;
+    ua.n_packs = n_packs;
+    ua.queue = run.dqueue.alloc<int>(1, st);
+    HIPCHECK(launch_zero_int(ua.queue, 1, st));
+    ua.basis_stride = (basis_floats + 63) / 64 * 64;
+    ua.basis = ua.basis_stride ? run.dbasis.alloc<float>((size_t)wgs * ua.basis_stride, st) : nullptr;
+    UnevenArgs *du = run.dargs.alloc<UnevenArgs>(1, st);
+    HIPCHECK(hipMemcpyAsync(du, &ua, sizeof(UnevenArgs), hipMemcpyHostToDevice, st));
+    if (timed) HIPCHECK(hipEventRecord(ctx->ev0, st));
+    HIPCHECK(launch_uneven(du, wgs, st));
+    if (timed) HIPCHECK(hipEventRecord(ctx->ev1, st));
+}
+
+// Status and results of a launch the stream has passed (uneven_begin).
+void uneven_finish(UnevenRun &run, const msg_uneven_entry *entries) {
+    const int n = (int)run.bs.size();
+    int bad_class = 0;
+    for (int e = 0; e < n; ++e) {
+        int sflag = 0;
+        HIPCHECK(hipMemcpy(&sflag, run.bs[e]->a.status, sizeof(int), hipMemcpyDeviceToHost));
+        bad_class |= sflag & 8;
+    }
+    if (bad_class) throw DeviceError("uneven launch: a pack named no class or did not fit its class");
+    for (int e = 0; e < n; ++e) {
+        try {
+            collect(*run.bs[e], entries[e].yields, entries[e].trace);
+        } catch (const ArgError &x) {
+            throw ArgError{x.st, "entry " + std::to_string(e) + ": " + x.what};
+        }
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1184,6 +1315,19 @@ struct Request {
     double work = 0.0;           // state-steps x D: admission order (shortest first)
     int f_begin = 0, f_count = 0;  // fields of the launch in flight (whole mode)
     int nobs = 0;
+    bool no_group = false;       // admitted on its own (its shared launch was refused)
+};
+
+// Requests that share one uneven launch (whole mode): small requests of different
+// systems and dimensions go into one persistent launch together instead of one
+// launch each, which alone could not fill the device.
+struct RequestGroup {
+    std::vector<std::unique_ptr<Request>> members;
+    std::vector<msg_uneven_entry> ents;
+    std::unique_ptr<UnevenRun> run;
+    hipEvent_t done = nullptr;
+    int stream = -1;
+    int64_t bytes = 0;
 };
 
 // Work (D x columns x samples) of one whole-mode launch of the server: about 50 ms of
@@ -1191,6 +1335,12 @@ struct Request {
 // request holds the device for at most that long before smaller requests' launches,
 // queued on the other streams, get their turn.
 constexpr double kServerChunkWork = 4.0e8;
+// Work of one shared launch of small requests (about 50 ms of device time, as one
+// whole-mode launch): queued requests below it that the uneven launch takes are
+// admitted together, smallest first.  On the bench's 40-request stream larger groups
+// raised the median latency (0.95 s at 1e9, 1.5 s at 4e9 against 0.71 s here and
+// 0.73 s without groups) at the same throughput.
+constexpr double kServerGroupWork = 4.0e8;
 
 struct msg_server {
     msg_ctx *ctx = nullptr;
@@ -1200,6 +1350,9 @@ struct msg_server {
     std::deque<std::unique_ptr<Request>> queued;
     std::vector<std::unique_ptr<Request>> active;
     std::map<int64_t, std::unique_ptr<Request>> finished;
+    std::vector<std::unique_ptr<RequestGroup>> groups;
+    int64_t shared_launches = 0, shared_requests = 0;
+    double group_work = kServerGroupWork;  // MSG_SERVER_GROUP_WORK overrides (tuning runs; 0 disables)
     std::vector<hipStream_t> streams;
     std::vector<char> stream_busy;
 };
@@ -1595,7 +1748,10 @@ msg_status msg_autotune(msg_ctx *ctx, const msg_system *sys, const msg_run_param
             if (ng == 2048) base.push_back({2, 1024, 0});
             if (ng == 4096) { base.push_back({-1, 1024, 0}); base.push_back({0, 1024, 0}); }
             if (ng > 4096) { base.push_back({-2, 1024, 0}); base.push_back({0, 1024, 0}); }
-            if (ng >= 4096) base.push_back({-3, 1024, 0});  // split columns (cooperative launch)
+            if (ng >= 4096) {  // split columns (cooperative launch), 2 or 4 groups per thread
+                base.push_back({-3, 1024, 0});
+                base.push_back({-3, 512, 0});
+            }
             for (const auto &c : base) {
                 cand.push_back(c);  // every resident workgroup the occupancy allows
                 msg_ctx::Shape h = c;
@@ -1668,97 +1824,10 @@ msg_status msg_propagate_uneven(msg_ctx *ctx, int32_t n_entries, const msg_uneve
     if (!ctx || n_entries < 1 || !entries) return fail(ctx, MSG_INVALID_ARG, "null context or no entries");
     return guarded(ctx, [&] {
         HIPCHECK(hipSetDevice(ctx->device));
-        hipStream_t st = ctx->stream;
-        std::vector<std::unique_ptr<Batch>> bs(n_entries);
-        std::vector<StepArgs> ents(n_entries);
-        struct Ordered {
-            UnevenPack p;
-            double cost;
-        };
-        std::vector<Ordered> order;
-        size_t basis_floats = 0;
-        for (int e = 0; e < n_entries; ++e) {
-            const msg_uneven_entry &en = entries[e];
-            const std::string at = "entry " + std::to_string(e) + ": ";
-            if (!en.system || !en.params || !en.yields)
-                throw ArgError{MSG_INVALID_ARG, at + "null system, parameters or yields"};
-            if (en.params->method == MSG_METHOD_UNCOUPLED)
-                throw ArgError{MSG_UNSUPPORTED, at + "propagationmethod=uncoupled runs through msg_propagate"};
-            if (en.params->dynamics && en.system->r.has_td)
-                throw ArgError{MSG_UNSUPPORTED, at + "the uneven batch takes static Hamiltonians"};
-            bs[e].reset(new Batch());
-            Batch &b = *bs[e];
-            b.uneven = true;
-            try {
-                prepare(b, ctx, en.system, en.params, en.n_items, en.fields, en.columns, en.n_columns,
-                        en.trace != nullptr, false, st);
-            } catch (const ArgError &x) {
-                throw ArgError{x.st, at + x.what};
-            }
-            if (b.arnoldi)
-                throw ArgError{MSG_UNSUPPORTED, at + "unequal recombination rates above D = 512 run through msg_propagate"};
-            if (b.a.ktol > 0.0)
-                throw ArgError{MSG_UNSUPPORTED, at + "krylovtol basis reuse runs through msg_propagate"};
-            if (b.a.rk4) throw ArgError{MSG_UNSUPPORTED, at + "matrix-free RK4 runs through msg_propagate"};
-            if (b.a.mixm)
-                throw ArgError{MSG_UNSUPPORTED, at + "a nucleus with spin > 1/2 above D = 512 runs through msg_propagate"};
-            const UnevenShape sh = uneven_shape(b);
-            basis_floats = std::max(basis_floats, sh.basis_floats);
-            StepArgs a = b.a;
-            a.s_begin = 0;
-            a.s_end = b.nsamples - 1;
-            ents[e] = a;
-            // packs of consecutive units of one item, cost ~ device time of the pack
-            const int upi = b.n_units / b.n_items;
-            const double D = a.dim, steps = b.nsamples - 1;
-            for (int it = 0; it < b.n_items; ++it)
-                for (int c = 0; c < upi; c += sh.per_pack) {
-                    const int nact = std::min(sh.per_pack, upi - c);
-                    const double cost = b.path == Path::Lanczos
-                                            ? nact * D * steps * a.m * (a.nnuc + 4)
-                                            : nact * 16.0 * a.dpad * a.dpad * steps / 16.0;
-                    order.push_back({UnevenPack{e, it * upi + c, nact, sh.cls}, cost});
-                }
-        }
-        std::stable_sort(order.begin(), order.end(), [](const Ordered &x, const Ordered &y) { return x.cost > y.cost; });
-        std::vector<UnevenPack> packs(order.size());
-        for (size_t k = 0; k < order.size(); ++k) packs[k] = order[k].p;
-        const int n_packs = (int)packs.size();
-        const int occ = std::max(1, uneven_occupancy());
-        const int wgs = std::max(1, std::min(n_packs, ctx->cus * occ));
-        DBuf dents, dpacks, dqueue, dbasis, dargs;
-        StepArgs *de = dents.alloc<StepArgs>(ents.size(), st);
-        HIPCHECK(hipMemcpyAsync(de, ents.data(), ents.size() * sizeof(StepArgs), hipMemcpyHostToDevice, st));
-        UnevenPack *dp = dpacks.alloc<UnevenPack>(packs.size(), st);
-        HIPCHECK(hipMemcpyAsync(dp, packs.data(), packs.size() * sizeof(UnevenPack), hipMemcpyHostToDevice, st));
-        UnevenArgs ua{};
-        ua.ents = de;
-        ua.packs = dp;
-        ua.n_packs = n_packs;
-        ua.queue = dqueue.alloc<int>(1, st);
-        HIPCHECK(launch_zero_int(ua.queue, 1, st));
-        ua.basis_stride = (basis_floats + 63) / 64 * 64;
-        ua.basis = ua.basis_stride ? dbasis.alloc<float>((size_t)wgs * ua.basis_stride, st) : nullptr;
-        UnevenArgs *du = dargs.alloc<UnevenArgs>(1, st);
-        HIPCHECK(hipMemcpyAsync(du, &ua, sizeof(UnevenArgs), hipMemcpyHostToDevice, st));
-        HIPCHECK(hipEventRecord(ctx->ev0, st));
-        HIPCHECK(launch_uneven(du, wgs, st));
-        HIPCHECK(hipEventRecord(ctx->ev1, st));
-        HIPCHECK(hipStreamSynchronize(st));  // ents / packs / ua are host locals
-        int bad_class = 0;
-        for (int e = 0; e < n_entries; ++e) {
-            int sflag = 0;
-            HIPCHECK(hipMemcpy(&sflag, bs[e]->a.status, sizeof(int), hipMemcpyDeviceToHost));
-            bad_class |= sflag & 8;
-        }
-        if (bad_class) throw DeviceError("uneven launch: a pack named no class or did not fit its class");
-        for (int e = 0; e < n_entries; ++e) {
-            try {
-                collect(*bs[e], entries[e].yields, entries[e].trace);
-            } catch (const ArgError &x) {
-                throw ArgError{x.st, "entry " + std::to_string(e) + ": " + x.what};
-            }
-        }
+        UnevenRun run;
+        uneven_begin(run, ctx, n_entries, entries, ctx->stream, true);
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+        uneven_finish(run, entries);
         float ms = 0.f;
         HIPCHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ctx->kernel_ms = ms;
@@ -2140,6 +2209,7 @@ msg_status msg_server_create(msg_ctx *ctx, int32_t max_slots, int64_t hbm_budget
             s->streams.push_back(st);
         }
         s->stream_busy.assign(max_slots, 0);
+        if (const char *env = std::getenv("MSG_SERVER_GROUP_WORK")) s->group_work = std::max(0.0, std::atof(env));
         *out = s.release();
     });
 }
@@ -2149,6 +2219,9 @@ msg_status msg_server_destroy(msg_server *srv) {
     for (hipStream_t st : srv->streams) (void)hipStreamSynchronize(st);
     for (auto &q : srv->active)
         if (q->done_ev) (void)hipEventDestroy(q->done_ev);
+    for (auto &g : srv->groups)
+        if (g->done) (void)hipEventDestroy(g->done);
+    srv->groups.clear();
     srv->active.clear();
     srv->finished.clear();
     // requests may have left the context's rocBLAS handle bound to a server stream
@@ -2226,7 +2299,33 @@ msg_status msg_server_pump(msg_server *srv, int32_t max_steps, int32_t *n_active
             const int64_t id = q->id;
             srv->finished[id] = std::move(q);
         };
-        // 1. completions (non-blocking)
+        // 1. completions (non-blocking): shared launches, then requests of their own
+        for (auto it = srv->groups.begin(); it != srv->groups.end();) {
+            RequestGroup &g = **it;
+            if (hipEventQuery(g.done) != hipSuccess) {
+                ++it;
+                continue;
+            }
+            std::string err;
+            try {
+                uneven_finish(*g.run, g.ents.data());
+            } catch (const ArgError &e) {
+                err = e.what;
+            }
+            g.run.reset();  // releases the launch's buffers (stream-ordered)
+            (void)hipEventDestroy(g.done);
+            srv->stream_busy[g.stream] = 0;
+            srv->in_use -= g.bytes;
+            for (auto &q : g.members) {
+                if (!err.empty()) {
+                    q->failed = true;
+                    q->error = err;
+                }
+                const int64_t id = q->id;
+                srv->finished[id] = std::move(q);
+            }
+            it = srv->groups.erase(it);
+        }
         for (auto it = srv->active.begin(); it != srv->active.end();) {
             Request &q = **it;
             const bool done = q.whole ? hipEventQuery(q.done_ev) == hipSuccess : q.batch->s_next >= q.batch->nsamples - 1;
@@ -2254,9 +2353,77 @@ msg_status msg_server_pump(msg_server *srv, int32_t max_steps, int32_t *n_active
                 ++it;
             }
         }
-        // 2. admission by free stream and HBM footprint, shortest job first (arrival order
-        //    among equals).  (Capping each launch's resident workgroups while others wait
-        //    was measured slower: a persistent launch cannot grow once the others finish.)
+        // 2a. whole mode: the queued requests below kServerGroupWork that the uneven
+        //     launch takes share one launch on a free stream, smallest first, while their
+        //     summed work stays below kServerGroupWork and their footprint fits
+        while (max_steps <= 0) {
+            int slot = -1;
+            for (int i = 0; i < (int)srv->streams.size(); ++i)
+                if (!srv->stream_busy[i]) { slot = i; break; }
+            if (slot < 0) break;
+            std::vector<size_t> cand;
+            for (size_t i = 0; i < srv->queued.size(); ++i) {
+                const Request &q = *srv->queued[i];
+                if (!q.no_group && q.work <= srv->group_work && uneven_eligible(q.sys, &q.prm)) cand.push_back(i);
+            }
+            std::stable_sort(cand.begin(), cand.end(),
+                             [&](size_t x, size_t y) { return srv->queued[x]->work < srv->queued[y]->work; });
+            std::vector<size_t> take;
+            double work = 0.0;
+            int64_t bytes = 0;
+            for (size_t i : cand) {
+                const Request &q = *srv->queued[i];
+                if (work + q.work > srv->group_work || srv->in_use + bytes + q.bytes > srv->budget) continue;
+                take.push_back(i);
+                work += q.work;
+                bytes += q.bytes;
+            }
+            if (take.size() < 2) break;  // a single request runs on its own below
+            std::sort(take.begin(), take.end());
+            std::unique_ptr<RequestGroup> g(new RequestGroup());
+            for (auto k = take.rbegin(); k != take.rend(); ++k) {
+                g->members.push_back(std::move(srv->queued[*k]));
+                srv->queued.erase(srv->queued.begin() + (std::ptrdiff_t)*k);
+            }
+            std::reverse(g->members.begin(), g->members.end());
+            for (auto &q : g->members) {
+                q->stream = slot;
+                q->whole = true;
+                q->result.assign((size_t)q->n_fields * std::max(1, observable_count(q->sys->r, &q->prm)), 0.0);
+                msg_uneven_entry en{};
+                en.system = q->sys;
+                en.params = &q->prm;
+                en.n_items = q->n_fields;
+                en.fields = q->fields.data();
+                en.yields = q->result.data();
+                g->ents.push_back(en);
+            }
+            g->run.reset(new UnevenRun());
+            try {
+                uneven_begin(*g->run, ctx, (int)g->ents.size(), g->ents.data(), srv->streams[slot], false);
+            } catch (const ArgError &) {
+                // the shared launch refused a member: every member goes back to the
+                // queue and runs on its own (nothing of the refused launch ran)
+                g->run.reset();
+                for (auto &q : g->members) {
+                    q->no_group = true;
+                    srv->queued.push_back(std::move(q));
+                }
+                continue;
+            }
+            HIPCHECK(hipEventCreateWithFlags(&g->done, hipEventDisableTiming));
+            HIPCHECK(hipEventRecord(g->done, srv->streams[slot]));
+            g->stream = slot;
+            g->bytes = bytes;
+            srv->stream_busy[slot] = 1;
+            srv->in_use += bytes;
+            ++srv->shared_launches;
+            srv->shared_requests += (int64_t)g->members.size();
+            srv->groups.push_back(std::move(g));
+        }
+        // 2b. admission by free stream and HBM footprint, shortest job first (arrival order
+        //     among equals).  (Capping each launch's resident workgroups while others wait
+        //     was measured slower: a persistent launch cannot grow once the others finish.)
         for (;;) {
             int slot = -1;
             for (int i = 0; i < (int)srv->streams.size(); ++i)
@@ -2324,8 +2491,17 @@ msg_status msg_server_pump(msg_server *srv, int32_t max_steps, int32_t *n_active
                 }
             }
         }
-        if (n_active) *n_active = (int32_t)(srv->queued.size() + srv->active.size());
+        size_t grouped = 0;
+        for (auto &g : srv->groups) grouped += g->members.size();
+        if (n_active) *n_active = (int32_t)(srv->queued.size() + srv->active.size() + grouped);
     });
+}
+
+msg_status msg_server_stats(const msg_server *srv, int64_t *shared_launches, int64_t *shared_requests) {
+    if (!srv) return MSG_INVALID_ARG;
+    if (shared_launches) *shared_launches = srv->shared_launches;
+    if (shared_requests) *shared_requests = srv->shared_requests;
+    return MSG_OK;
 }
 
 msg_status msg_server_collect(msg_server *srv, int64_t request_id, double *yields, int32_t *done) {
@@ -2336,6 +2512,9 @@ msg_status msg_server_collect(msg_server *srv, int64_t request_id, double *yield
             if (q->id == request_id) { *done = 0; return MSG_OK; }
         for (auto &q : srv->queued)
             if (q->id == request_id) { *done = 0; return MSG_OK; }
+        for (auto &g : srv->groups)
+            for (auto &q : g->members)
+                if (q->id == request_id) { *done = 0; return MSG_OK; }
         return fail(srv->ctx, MSG_INVALID_ARG, "unknown request id");
     }
     Request &q = *it->second;

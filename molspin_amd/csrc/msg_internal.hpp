@@ -18,6 +18,7 @@ constexpr int kMaxNuclei = 16;     // matrix-free path: D = 4 * 2^N <= 262144
 constexpr int kMaxInitial = 4;
 constexpr int kChebMax = 96;
 constexpr int kTileGroups = 4096;  // groups per LDS tile of the tiled streaming Lanczos (bits 0-11)
+constexpr int kSplitSlab = 2048;   // groups per workgroup slab of the split-column Lanczos (bits 0-10)
 constexpr int kDensePlanes = 2;    // dense-path propagator planes: re, im
 constexpr int kDenseLdsPlanes = 3; // dense-path state tile in LDS: re, im, re + im
 // Device scratch of the allocation-free entry point, per GPU (msg_noalloc.hip)

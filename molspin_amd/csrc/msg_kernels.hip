@@ -1770,12 +1770,13 @@ __global__ void k_reduce_columns(const double *in, double *out, int n_items, int
 }
 
 // ---------------------------------------------------------------------------
-// Matrix-free Lanczos with each column split over S = D / (8 T) workgroups of T
-// threads (D = 4 2^NN >= 16384).  A workgroup owns a slab of 2 T groups (the low
-// index bits), two per thread, and keeps the slab of v_j and v_{j-1} in LDS (64 T
-// bytes: one workgroup per CU at T = 1024, two at T = 512); the residual stays in
-// registers.  The nuclei on the higher bits couple slabs: their neighbours come from v_j in the slot's global basis,
-// which every workgroup of the slot writes before the slot's barrier.  Inner
+// Matrix-free Lanczos with each column split over S = D / 8192 workgroups of T
+// threads (D = 4 2^NN >= 16384).  A workgroup owns a slab of 2048 groups (the low
+// index bits), G = 2048 / T per thread (2 at T = 1024 with four waves per SIMD and
+// 128 VGPRs, 4 at T = 512 with two waves per SIMD and 256 VGPRs), and keeps the slab
+// of v_j and v_{j-1} in LDS (128 KiB: one workgroup per CU); the residual stays in
+// registers.  The nuclei on the higher bits couple slabs: their neighbours come from
+// the slot's global vectors, complete at the slot's barrier.  Inner
 // products are reduced over the workgroup, then over the slot's workgroups in slab
 // order (slot_sum), so every workgroup of a slot sees the same alpha, beta and
 // observables and follows the same control flow.  Per state-step HBM carries the m
@@ -1846,9 +1847,9 @@ __device__ __forceinline__ bool slot_sum(double (&v)[K], double *xr, int *bar, i
 }
 
 template <int NN, int T>
-__global__ __launch_bounds__(T, 4) void k_lanczos_split(const StepArgs *__restrict__ pa) {
+__global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const StepArgs *__restrict__ pa) {
     const StepArgs &a = *pa;
-    constexpr int G = 2, SG = G * T, SB = Log2<SG>::value, S = 1 << (NN - SB);
+    constexpr int SG = kSplitSlab, G = SG / T, SB = Log2<SG>::value, S = 1 << (NN - SB);
     extern __shared__ float4 smem[];  // slab of v_j and v_{j-1}: planes A and B of each buffer
     auto LA = [&](int i) -> float4 * { return smem + (i ? 2 * SG : 0); };
     auto LB = [&](int i) -> float4 * { return smem + SG + (i ? 2 * SG : 0); };
@@ -1967,10 +1968,16 @@ __global__ __launch_bounds__(T, 4) void k_lanczos_split(const StepArgs *__restri
                     for (int g = 0; g < G; ++g) vjs[g] = load_group(cA, cB, g * T + t);
                     // the slab's part of H v_j while the slot's other workgroups store
                     // theirs, then the nuclei across slabs once v_j is complete in the slot
-                    // (lane-selected blocks: the LDS block table measured 8 % slower here)
-                    apply_h_groups<NN, G, Log2<T>::value, false, 0, 1>(a, co, hd, t, vjs, cA, cB, w, base);
+                    // (lane-selected blocks: the LDS block table measured 8 % slower here).
+                    // The coefficient pointers pass through an empty asm statement in every
+                    // iteration, so the coefficients are reloaded (scalar loads) where they
+                    // are used instead of being held in SGPRs across the loop (5 % faster
+                    // at D = 65536: 115 ms against 122 ms per launch).
+                    const float *coj = co, *hdj = hd;
+                    asm volatile("" : "+s"(coj), "+s"(hdj));
+                    apply_h_groups<NN, G, Log2<T>::value, false, 0, 1>(a, coj, hdj, t, vjs, cA, cB, w, base);
                     if (!slot_wait(bar, S, nbar, a.status, &s_flag)) return;
-                    apply_h_groups<NN, G, Log2<T>::value, false, 0, 2>(a, co, hd, t, vjs, cA, cB, w, base, gA,
+                    apply_h_groups<NN, G, Log2<T>::value, false, 0, 2>(a, coj, hdj, t, vjs, cA, cB, w, base, gA,
                                                                        gA + D4);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
@@ -2857,17 +2864,20 @@ int uneven_occupancy() {
 // shapes of the RK4 kernel (D = 1024 ... 16384)
 #define MSG_RK4_VARIANTS(X) X(1, 256) X(1, 512) X(2, 512) X(2, 1024) X(4, 1024)
 
-// Split-column kernels for D = 4 2^NN, NN = 12..16, with 1024 threads (512-thread
-// workgroups, two per CU over twice as many slabs, measured slower at D = 16384 and
-// 65536: more partner slabs and wider barriers)
-#define MSG_SPLIT_VARIANTS(X) X(12, 1024) X(13, 1024) X(14, 1024) X(15, 1024) X(16, 1024)
+// Split-column kernels for D = 4 2^NN, NN = 12..16, with 1024 or 512 threads over a
+// 2048-group slab.  At D = 65536 the 512-thread form (four groups per thread, 256
+// VGPRs, 27 spilled against 71 at 1024 threads) takes 115 ms per trajectory launch
+// against 148 ms.  (512-thread workgroups over 1024-group slabs, two per CU, measured
+// slower at D = 16384 and 65536: more partner slabs and wider barriers.)
+#define MSG_SPLIT_VARIANTS(X) X(12, 1024) X(13, 1024) X(14, 1024) X(15, 1024) X(16, 1024) \
+    X(12, 512) X(13, 512) X(14, 512) X(15, 512) X(16, 512)
 
 hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_thread, int threads, int slots,
                           hipStream_t st) {
     if (group_per_thread == -3) {  // split columns: `slots` column slots of S workgroups each
         const int nn = 31 - __builtin_clz((unsigned)(a.dim >> 2));  // D / 4 = 2^nn
-        const int S = (a.dim >> 2) / (2 * threads);
-        const size_t lds = (size_t)4 * 2 * threads * sizeof(float4);
+        const int S = (a.dim >> 2) / kSplitSlab;
+        const size_t lds = (size_t)4 * kSplitSlab * sizeof(float4);
         void *args[] = {(void *)&da};
 #define X(NN_, T_)                                                                                                \
         if (nn == NN_ && threads == T_)                                                                           \
@@ -3033,7 +3043,8 @@ int lanczos_occupancy(int group_per_thread, int threads, size_t lds, bool arnold
         return 1;
     }
     if (group_per_thread == -3)  // the split kernels of one width share one resource footprint
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_split<14, 1024>, threads, lds) == hipSuccess
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                   &nb, threads == 512 ? k_lanczos_split<14, 512> : k_lanczos_split<14, 1024>, threads, lds) == hipSuccess
                    ? nb : 1;
     if (group_per_thread == 0)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<false>, 1024, 0) == hipSuccess ? nb : 1;

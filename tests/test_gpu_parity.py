@@ -176,12 +176,17 @@ def test_step_entry_matches_oracle(ctx):
 
 
 def force_shape(monkeypatch, shape):
-    """Launch shape override (MSG_LANCZOS_G)."""
+    """Launch shape override (MSG_LANCZOS_G); "-3" is the split-column shape with
+    512-thread workgroups (four groups per thread) and "-3/1024" the one with
+    1024-thread workgroups (two groups per thread, MSG_SPLIT_T)."""
     if shape is not None:
-        monkeypatch.setenv("MSG_LANCZOS_G", shape)
+        g, _, t = shape.partition("/")
+        monkeypatch.setenv("MSG_LANCZOS_G", g)
+        if t:
+            monkeypatch.setenv("MSG_SPLIT_T", t)
 
 
-@pytest.mark.parametrize("shape", [None, "-1", "0", "-3"])
+@pytest.mark.parametrize("shape", [None, "-1", "0", "-3", "-3/1024"])
 def test_large_dimension_streaming_path(ctx, shape, monkeypatch):
     """D = 16384 (7 + 5 anisotropic spin-1/2 nuclei): the large-D matrix-free
     variants on 6 direct columns and 2 orientations over 0-60 ns: the default shape,
@@ -204,7 +209,7 @@ def test_large_dimension_streaming_path(ctx, shape, monkeypatch):
     assert np.abs(y - yo).max() <= YIELD_TOL
 
 
-@pytest.mark.parametrize("shape", [None, "-2", "-3"])
+@pytest.mark.parametrize("shape", [None, "-2", "-3", "-3/1024"])
 def test_config3_trajectories(ctx, shape, monkeypatch):
     """Config 3 (D = 65536, 14 isotropic nuclei, ST dephasing 1e-4 /ns): 4 spin-
     coherent trajectories drawn by the backend's sampler, 0-30 ns, against the
@@ -223,7 +228,8 @@ def test_config3_trajectories(ctx, shape, monkeypatch):
     assert np.abs(y - yo).max() <= YIELD_TOL
 
 
-def test_split_column_d32768(ctx, monkeypatch):
+@pytest.mark.parametrize("shape", ["-3", "-3/1024"])
+def test_split_column_d32768(ctx, shape, monkeypatch):
     """D = 32768 (13 anisotropic spin-1/2 nuclei) with each column split over four
     workgroups: 4 direct columns, 2 orientations, 0-20 ns, against the oracle."""
     rng = workloads.SplitMix64(43, 1)
@@ -236,7 +242,7 @@ def test_split_column_d32768(ctx, monkeypatch):
     cols = B[:, [3, 2050, 5000, 8191]] * math.sqrt(B.shape[1] / 4.0)
     fields = [workloads.fibonacci_point(64, i, 0.05) for i in (9, 33)]
     prm = abi.run_params(total_time=20.0, time_step=2.0, sampling=abi.MSG_SAMPLING_GIVEN)
-    force_shape(monkeypatch, "-3")
+    force_shape(monkeypatch, shape)
     y, _ = sysd.propagate(prm, fields=fields, columns=cols)
     yo, _ = ref.run_hsgeneral(p, prm, fields=fields, columns=cols, threads=8)
     print("D=32768 split dev", y, "oracle", yo)
@@ -611,7 +617,7 @@ def test_server_uncoupled_requests(ctx):
         assert np.abs(a - yo).max() <= YIELD_TOL
 
 
-@pytest.mark.parametrize("shape", [None, "-3"])
+@pytest.mark.parametrize("shape", [None, "-3", "-3/1024"])
 def test_config3_full_horizon(ctx, shape, monkeypatch):
     """Config 3 (D = 65536) over its whole 0-2000 ns grid (2000 steps) on 8
     spin-coherent trajectories with the ST flips of seed 2026, against the committed

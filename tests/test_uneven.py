@@ -130,3 +130,35 @@ def test_uneven_given_columns_and_rejections(ctx, cfg4):
         ctx.propagate_uneven(entries(cfg4, order=[0], trace=False) +
                              [dict(system=bad, params=prm, fields=p10["fx"]["fields"])])
     assert exc.value.status == abi.MSG_UNSUPPORTED and "entry 1" in str(exc.value)
+
+
+def test_server_shares_uneven_launches(ctx, cfg4):
+    """The whole-mode server (msg_server_pump(0)) puts the config 4 pairs submitted
+    together into shared uneven launches: every request's yields equal its pair alone
+    in an uneven launch (1e-12) and the oracle fixture (1e-5)."""
+    import time
+    from molspin_amd import backend
+    _, prm, pairs = cfg4
+    srv = backend.Server(ctx, max_slots=4, hbm_budget_bytes=16 << 30)
+    try:
+        ids = [srv.submit(p["sys"], prm, p["fx"]["fields"]) for p in pairs]
+        res = {}
+        deadline = time.time() + 60.0
+        while len(res) < len(ids) and time.time() < deadline:
+            srv.pump(0)
+            for rid in ids:
+                if rid not in res:
+                    y = srv.collect(rid)
+                    if y is not None:
+                        res[rid] = y
+            time.sleep(0.001)
+        launches, held = srv.stats()
+    finally:
+        srv.close()
+    assert len(res) == len(ids)
+    print("shared launches %d holding %d of %d requests" % (launches, held, len(ids)))
+    assert launches >= 1 and held >= 2
+    for k, (rid, p) in enumerate(zip(ids, pairs)):
+        alone = ctx.propagate_uneven(entries(cfg4, order=[k], trace=False))[0][0]
+        assert np.abs(res[rid] - alone).max() <= 1e-12, p["fx"]["n"]
+        assert np.abs(res[rid] - np.array(p["fx"]["yields"])).max() <= YIELD_TOL, p["fx"]["n"]
