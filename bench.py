@@ -191,16 +191,21 @@ def run_bench(ctx, args):
     }
 
 
-def lanczos_roofline(D, m, state_steps, kernel_s, traffic, traffic_source=None):
+def lanczos_roofline(D, m, state_steps, kernel_s, traffic, traffic_source=None, issue_key=None):
     """Matrix-free Lanczos at D >= 16384: algorithmic HBM bytes per state-step
     8 D (4m + 1) (m iterations reading v_j, v_{j-1} and writing v_{j+1}, plus the
-    m + 1 vectors of the final combination, complex FP32)."""
+    m + 1 vectors of the final combination, complex FP32).  `issue_key` names the
+    profiled VALU issue fraction of the kernel, reported beside the byte model."""
     bytes_per_ss = 8.0 * D * (4 * m + 1)
     achieved = bytes_per_ss * state_steps / kernel_s
-    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_source,
-            "kernel_ms": kernel_s * 1e3, "bytes_per_state_step": bytes_per_ss,
-            "state_steps_per_launch": state_steps}
+    out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_source,
+           "kernel_ms": kernel_s * 1e3, "bytes_per_state_step": bytes_per_ss,
+           "state_steps_per_launch": state_steps}
+    if issue_key in PROFILED_VALU_ISSUE:
+        v = PROFILED_VALU_ISSUE[issue_key]
+        out["valu_issue_profiled"] = {"value": v[0], "source": v[1]}
+    return out
 
 
 def operator_flops_per_state_step(D, m, n_couplings, e_block=False):
@@ -241,16 +246,22 @@ def valu_roofline(D, m, n_couplings, state_steps, kernel_s, hbm_traffic, traffic
 # traffic of a launch scales with its state-steps at fixed D.  Each entry names the
 # profile summary it comes from; a kernel change needs a new counter run.
 MEASURED_TRAFFIC_PER_STATE_STEP = {
-    "orientations": (1.069e+06, "profiles/k_lanczos_orientation_batch_D4096.md"),
+    "orientations": (1.0715e+06, "profiles/k_lanczos_orientation_batch_D4096.md"),
     "dense": (2.47e+04, "profiles/k_dense_field_sweep_D512.md"),
-    "pair/-1": (6.60e+06, "profiles/k_lanczos_big_pair_D16384.md"),
-    "arnoldi": (5.02e+06, "profiles/k_arnoldi_singlet_sink_D4096.md"),
+    "pair/-1": (6.613e+06, "profiles/k_lanczos_big_pair_D16384.md"),
+    "arnoldi": (5.097e+06, "profiles/k_arnoldi_singlet_sink_D4096.md"),
     # D = 65536 by launch shape: tiled streaming (-2), split columns (-3)
     "trajectories/-2": (7.54e+07, "profiles/k_lanczos_big_trajectories_D65536.md"),
-    "trajectories/-3": (3.154e+07, "profiles/k_lanczos_split_trajectories_D65536.md"),
+    "trajectories/-3": (2.545e+07, "profiles/k_lanczos_split_trajectories_D65536.md"),
+    # config 4's first 24 pairs x 8 fields x 20 steps: mixed D, so only this launch size
+    "uneven": (3.477e+06, "profiles/k_uneven_config4_pairs0_23.md"),
 }
-# VALU issue fraction (SQ_ACTIVE_INST_VALU / SQ_BUSY_CYCLES per SIMD) of the profiled kernels
-PROFILED_VALU_ISSUE = {"orientations": (0.78, "profiles/k_lanczos_orientation_batch_D4096.md")}
+# VALU issue fraction (SQ_INSTS_VALU / (GRBM_GUI_ACTIVE / 8 XCDs x 256 CUs)) of the profiled kernels
+PROFILED_VALU_ISSUE = {"orientations": (0.86, "profiles/k_lanczos_orientation_batch_D4096.md"),
+                       "pair/-1": (0.77, "profiles/k_lanczos_big_pair_D16384.md"),
+                       "trajectories/-3": (0.46, "profiles/k_lanczos_split_trajectories_D65536.md"),
+                       "arnoldi": (0.65, "profiles/k_arnoldi_singlet_sink_D4096.md"),
+                       "uneven": (0.76, "profiles/k_uneven_config4_pairs0_23.md")}
 
 
 def traffic_for(key, state_steps):
@@ -343,8 +354,9 @@ def trajectory_batch(ctx, args):
         return sysd.propagate(prm, columns=cols)[0]
 
     def roof(kernel_s):
-        tr, src = traffic_for("trajectories/%d" % shape.groups_per_thread, state_steps)
-        return lanczos_roofline(D, m, state_steps, kernel_s, tr, src)
+        key = "trajectories/%d" % shape.groups_per_thread
+        tr, src = traffic_for(key, state_steps)
+        return lanczos_roofline(D, m, state_steps, kernel_s, tr, src, key)
 
     def parity(outs):
         import numpy as np
@@ -393,8 +405,9 @@ def pair_batch(ctx, args):
         return sysd.propagate(prm, fields=fields)[0]
 
     def roof(kernel_s):
-        tr, src = traffic_for("pair/%d" % shape.groups_per_thread, state_steps)
-        return lanczos_roofline(D, m, state_steps, kernel_s, tr, src)
+        key = "pair/%d" % shape.groups_per_thread
+        tr, src = traffic_for(key, state_steps)
+        return lanczos_roofline(D, m, state_steps, kernel_s, tr, src, key)
 
     return dict(step=step, state_steps=state_steps, roofline=roof, packed=packed,
                 cpu_prm=abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_KRYLOV),
@@ -528,9 +541,13 @@ def uneven_batch(ctx, args):
                 "pair_and_field_at_a_time_state_steps_per_s": state_steps / t_field}
 
     def roof(kernel_s):
+        # measured traffic applies to the profiled launch size only (mixed D)
+        tr, src = traffic_for("uneven", state_steps) if (P, F, intervals) == (24, 8, 20) else (None, None)
+        v = PROFILED_VALU_ISSUE["uneven"]
         return {"bound": "hbm", "achieved": mf_bytes / kernel_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": mf_bytes / kernel_s / HBM_PEAK, "traffic": None, "traffic_source": None,
+                "frac": mf_bytes / kernel_s / HBM_PEAK, "traffic": tr, "traffic_source": src,
                 "kernel_ms": kernel_s * 1e3, "state_steps_per_launch": state_steps,
+                "valu_issue_profiled": {"value": v[0], "source": v[1]} if tr else None,
                 "note": "byte model 8 D (4m + 1) per state-step summed over the matrix-free (D > 512) units; "
                         "dense units counted in state-steps only"}
 

@@ -107,6 +107,19 @@ __device__ __forceinline__ ConstPtr<T> as_const(const T *p) {
     return (ConstPtr<T>)p;
 }
 
+// A wave-uniform pointer the compiler must treat as unknown at this point (an empty
+// asm statement on its scalar registers): the coefficient loads through it are issued
+// where they are used, once per Krylov iteration, instead of being hoisted out of the
+// iteration loop and held in SGPRs, which spilled SGPRs into VGPRs and those to
+// scratch (k_arnoldi<2,512>: 440 -> 172 bytes of scratch per lane, 6 % faster;
+// k_lanczos<2,512>: SGPR spills 115 -> 37, 2 % faster).
+__device__ __forceinline__ const float *opaque_uniform(const float *p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    asm volatile("" : "+s"(lo), "+s"(hi));
+    return reinterpret_cast<const float *>(((uint64_t)hi << 32) | lo);
+}
+
 // y += (E + f1.S1 + f2.S2 + r) x on the electron pair, E the item's 4x4 complex
 // electron-pair block (row-major, interleaved; index 2 b1 + b2 with b1, b2 the
 // electrons' states), f1, f2 real fields: the fields and the shift are added into the
@@ -876,7 +889,8 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
                 double red[2] = {0.0, 0.0};
 #pragma unroll
                 for (int g = 0; g < G; ++g) vjs[g] = load_group(cA, cB, g * T + t);
-                apply_h_groups<NNUC, G, Log2<T>::value, true, MIX ? G : 0>(a, co, hd, t, vjs, cA, cB, w, 0, nullptr,
+                const float *coj = opaque_uniform(co), *hdj = opaque_uniform(hd);
+                apply_h_groups<NNUC, G, Log2<T>::value, true, MIX ? G : 0>(a, coj, hdj, t, vjs, cA, cB, w, 0, nullptr,
                                                                            nullptr, sh.blk);
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
@@ -1287,7 +1301,8 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
                     C4 w[G], vjs[G], hv[G];
 #pragma unroll
                     for (int g = 0; g < G; ++g) vjs[g] = load_group(LA, LB, g * T + t);
-                    apply_h_groups<Log2<ng>::value, G, Log2<T>::value>(a, co, hd, t, vjs, LA, LB, hv);
+                    const float *coj = opaque_uniform(co), *hdj = opaque_uniform(hd);
+                    apply_h_groups<Log2<ng>::value, G, Log2<T>::value>(a, coj, hdj, t, vjs, LA, LB, hv);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
 #pragma unroll
@@ -1968,13 +1983,10 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
                     for (int g = 0; g < G; ++g) vjs[g] = load_group(cA, cB, g * T + t);
                     // the slab's part of H v_j while the slot's other workgroups store
                     // theirs, then the nuclei across slabs once v_j is complete in the slot
-                    // (lane-selected blocks: the LDS block table measured 8 % slower here).
-                    // The coefficient pointers pass through an empty asm statement in every
-                    // iteration, so the coefficients are reloaded (scalar loads) where they
-                    // are used instead of being held in SGPRs across the loop (5 % faster
-                    // at D = 65536: 115 ms against 122 ms per launch).
-                    const float *coj = co, *hdj = hd;
-                    asm volatile("" : "+s"(coj), "+s"(hdj));
+                    // (lane-selected blocks: the LDS block table measured 8 % slower here);
+                    // coefficients reloaded per iteration (5 % faster at D = 65536: 115 ms
+                    // against 122 ms per launch)
+                    const float *coj = opaque_uniform(co), *hdj = opaque_uniform(hd);
                     apply_h_groups<NN, G, Log2<T>::value, false, 0, 1>(a, coj, hdj, t, vjs, cA, cB, w, base);
                     if (!slot_wait(bar, S, nbar, a.status, &s_flag)) return;
                     apply_h_groups<NN, G, Log2<T>::value, false, 0, 2>(a, coj, hdj, t, vjs, cA, cB, w, base, gA,
