@@ -951,6 +951,7 @@ void run_chunk(Batch &b, int steps) {
     }
     HIPCHECK(launch_zero_int(a.queue, 1, b.stream));
     if (a.bar) HIPCHECK(launch_zero_int(a.bar, b.slots, b.stream));
+    if (a.xred) HIPCHECK(launch_zero_int(reinterpret_cast<int *>(a.xred), (int)(b.xred.n / sizeof(int)), b.stream));
     const StepArgs *da = push_args(b, a);
     if (b.timed) {
         if (b.launches > 0) {  // the previous chunk's launch time, before the events are re-recorded

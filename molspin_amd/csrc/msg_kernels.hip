@@ -1840,25 +1840,71 @@ __device__ __forceinline__ bool slot_barrier(int *bar, int S, int &nbar, int *st
 }
 
 // Sums K workgroup totals (every thread holds them) over the slot's S workgroups in
-// slab order; xr holds the slot's partials, 2 x S x 17 doubles used alternately (a
-// workgroup can only overwrite a half two barriers later, when every workgroup of
-// the slot has read it).  Every thread receives the sums.
+// slab order, without an arrival counter: each workgroup stores its K totals as 64-bit
+// words whose low 8 bits carry the sum's tag (1 ... 255 in turn; each total keeps 44
+// mantissa bits, a relative change below 6e-14), one single-copy-atomic store per word
+// (thread 0 stores them all: selecting a thread's word by its index put the totals in
+// scratch), and wavefront 0 of every workgroup polls the slot's words in L2 until every tag
+// matches, then adds them in slab order, so every workgroup gets the same bits.  The
+// sum is complete one L2 round trip after the last word lands (the counter barrier took
+// a store, a wait for it, an atomic, the poll, an L1 invalidation and a second read).
+// xr holds 2 x S x 17 words, zeroed before the launch and used alternately: a workgroup
+// rewrites a half two sums later, after every workgroup has passed the sum in between
+// and so has read it; the stale words in a half carry a tag two steps back (255 is odd).
+// `nsum` counts the sums this workgroup has passed.  Every thread calls it; false when
+// the wait gave up (status bit 2).
 template <int K>
-__device__ __forceinline__ bool slot_sum(double (&v)[K], double *xr, int *bar, int S, int slab, int &nbar,
-                                         int *status, int *s_flag, double *s_tot) {
-    double *buf = xr + (size_t)(nbar & 1) * S * 17;
+__device__ __forceinline__ bool slot_sum(double (&v)[K], uint64_t *xr, int S, int slab, int &nsum, int *status,
+                                         int *s_flag, double *s_tot) {
+    constexpr uint64_t kTagBits = 0xFF;
+    const uint64_t tag = 1 + (uint64_t)(nsum % 255);
+    uint64_t *buf = xr + (size_t)(nsum & 1) * S * 17;
+    ++nsum;
     if (threadIdx.x == 0)
-        for (int k = 0; k < K; ++k) buf[slab * 17 + k] = v[k];
-    if (!slot_barrier(bar, S, nbar, status, s_flag)) return false;
-    if (threadIdx.x < K) {
-        double s = 0.0;
-        for (int q = 0; q < S; ++q) s += buf[q * 17 + threadIdx.x];
-        s_tot[threadIdx.x] = s;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            __hip_atomic_store(buf + slab * 17 + k, ((uint64_t)__double_as_longlong(v[k]) & ~kTagBits) | tag,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < 64) {
+        const int l = threadIdx.x;
+        const bool on = l < S;  // lane q polls the words of slab q
+        // the few words of the per-iteration sums stay in registers from the poll; the
+        // 17 of the per-step sum are read once more after it (fewer live registers)
+        constexpr bool keep = K <= 2;
+        uint64_t w[keep ? K : 1];
+        int ok = 1;
+        for (int spins = 0;; ++spins) {
+            bool good = true;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const uint64_t x = on ? __hip_atomic_load(buf + l * 17 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : tag;
+                if constexpr (keep) w[k] = x;
+                good = good && (x & kTagBits) == tag;
+            }
+            if (__all(good)) break;
+            if (spins > (1 << 26)) {  // about a second: a workgroup of the slot is gone
+                ok = 0;
+                if (l == 0) atomicOr(status, 4);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            uint64_t x;
+            if constexpr (keep) x = w[k];
+            else x = on ? __hip_atomic_load(buf + l * 17 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+            const double d = on ? __longlong_as_double((long long)(x & ~kTagBits)) : 0.0;
+            double sum = 0.0;
+            for (int q = 0; q < S; ++q) sum += __shfl(d, q, 64);
+            if (l == 0) s_tot[k] = sum;
+        }
+        if (l == 0) *s_flag = ok;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = s_tot[k];
-    return true;
+    return *s_flag != 0;
 }
 
 template <int NN, int T>
@@ -1880,11 +1926,11 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
     const int base = slab << SB;  // first group of the slab
     const int D = a.dim, D4 = D >> 2;
     float *bas = a.basis + (size_t)slot * a.m * 2 * D;
-    double *xr = a.xred + (size_t)slot * 2 * S * 17;
+    uint64_t *xr = reinterpret_cast<uint64_t *>(a.xred) + (size_t)slot * 2 * S * 17;
     int *bar = a.bar + slot;
     const bool lead = slab == 0 && t == 0;  // records samples and yields
     const int meff0 = a.m < D ? a.m : D;
-    int nbar = 0, par = 0;
+    int nbar = 0, nsum = 0, par = 0;
     auto SCR = [&]() { par ^= 1; return scr + par * 17 * 16; };
     auto norm2 = [&](const C4 (&x)[G]) {
         double nn = 0.0;
@@ -1935,7 +1981,7 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
                 for (int k = 0; k < 16; ++k) R[k] = Rf[k];
                 R[16] = norm2(psi);
                 block_sum<17>(R, SCR());
-                if (!slot_sum<17>(R, xr, bar, S, slab, nbar, a.status, &s_flag, s_tot)) return;
+                if (!slot_sum<17>(R, xr, S, slab, nsum, a.status, &s_flag, s_tot)) return;
                 if (lead && (!end || s == a.nsamples - 1)) record_sample(a, u, s, R, ys, yc);
             }
             if (end) break;
@@ -1949,7 +1995,7 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
                 for (int g = 0; g < G; ++g) st_flip(psi[g]);
                 double q[1] = {norm2(psi)};
                 block_sum<1>(q, SCR());
-                if (!slot_sum<1>(q, xr, bar, S, slab, nbar, a.status, &s_flag, s_tot)) return;
+                if (!slot_sum<1>(q, xr, S, slab, nsum, a.status, &s_flag, s_tot)) return;
                 nn = q[0];
             }
             const double beta0 = sqrt(nn);
@@ -2008,7 +2054,7 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
                         }
                     }
                     block_sum<2>(red, SCR());
-                    if (!slot_sum<2>(red, xr, bar, S, slab, nbar, a.status, &s_flag, s_tot)) return;
+                    if (!slot_sum<2>(red, xr, S, slab, nsum, a.status, &s_flag, s_tot)) return;
                     const double alpha = red[0];
                     if (lane == j) my_al = alpha;
                     const float fa = (float)alpha;
@@ -2025,7 +2071,7 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
                     if (!(beta2 > 0.02 * red[1])) {
                         double bb[1] = {norm2(w)};
                         block_sum<1>(bb, SCR());
-                        if (!slot_sum<1>(bb, xr, bar, S, slab, nbar, a.status, &s_flag, s_tot)) return;
+                        if (!slot_sum<1>(bb, xr, S, slab, nsum, a.status, &s_flag, s_tot)) return;
                         beta2 = bb[0];
                     }
                     const double beta = sqrt(fmax(beta2, 0.0));
