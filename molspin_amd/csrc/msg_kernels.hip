@@ -58,6 +58,25 @@ __device__ __forceinline__ void store_group(float4 *A, float4 *B, int nu, const 
     A[nu] = make_float4(x.c[0].x, x.c[0].y, x.c[1].x, x.c[1].y);
     B[nu] = make_float4(x.c[2].x, x.c[2].y, x.c[3].x, x.c[3].y);
 }
+// The same on planes in device memory: the pointers are cast to the global address
+// space at the access, so the accesses are global instead of flat instructions (a flat
+// access also counts against lgkmcnt, so every wait for an LDS result would wait for
+// it as well, and it needs a 64-bit vector address).
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) f4v gf4v;
+__device__ __forceinline__ C4 load_group_g(const float4 *A, const float4 *B, int nu) {
+    const f4v p = ((const gf4v *)A)[nu], q = ((const gf4v *)B)[nu];
+    C4 r;
+    r.c[0] = make_float2(p.x, p.y);
+    r.c[1] = make_float2(p.z, p.w);
+    r.c[2] = make_float2(q.x, q.y);
+    r.c[3] = make_float2(q.z, q.w);
+    return r;
+}
+__device__ __forceinline__ void store_group_g(float4 *A, float4 *B, int nu, const C4 &x) {
+    ((gf4v *)A)[nu] = f4v{x.c[0].x, x.c[0].y, x.c[1].x, x.c[1].y};
+    ((gf4v *)B)[nu] = f4v{x.c[2].x, x.c[2].y, x.c[3].x, x.c[3].y};
+}
 
 // y += M (x) 1 on electron 1 (pairs (0,2), (1,3)); M row-major 2x2 complex.
 __device__ __forceinline__ void apply_e1(const float2 m[4], const C4 &x, C4 &y) {
@@ -219,7 +238,7 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
         const float s = down ? -1.f : 1.f;
         const C4 nb = !TILED ? load_group(LA, LB, nu ^ (1 << n))
                              : (n < kTileBits ? load_group(LA, LB, (nu ^ (1 << n)) & (kTileG - 1))
-                                              : load_group(GA, GB, nu ^ (1 << n)));
+                                              : load_group_g(GA, GB, nu ^ (1 << n)));
         float2 m[4];
         if (fl & NUC_E1) {
             f1x = fmaf(s, cn[0], f1x); f1y = fmaf(s, cn[1], f1y); f1z = fmaf(s, cn[2], f1z);
@@ -391,7 +410,7 @@ __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *_
                 const bool diag = PART != 2, flips = PART == 0 || (PART == 1) == (n < SB);
                 C4 nb;
                 if (flips)
-                    nb = n < SB ? load_group(LA, LB, ((g << LOGT) ^ (1 << n)) + t) : load_group(GA, GB, (hi ^ (1 << n)) + t);
+                    nb = n < SB ? load_group(LA, LB, ((g << LOGT) ^ (1 << n)) + t) : load_group_g(GA, GB, (hi ^ (1 << n)) + t);
                 if (fl & NUC_E1) {
                     if (diag) {
                         e1[g][0] = fmaf(s, cn[0], e1[g][0]); e1[g][1] = fmaf(s, cn[1], e1[g][1]);
@@ -770,7 +789,7 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
         } else {
             const float *src = a.init_mode == 1 ? a.given + (size_t)col * 2 * D : a.state + (size_t)u * 2 * D;
             const float4 *sA = reinterpret_cast<const float4 *>(src);
-            psi[g] = load_group(sA, sA + D4, nu);
+            psi[g] = load_group_g(sA, sA + D4, nu);
         }
     }
     if (t == 0)
@@ -837,7 +856,7 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
                     const float4 *bA = reinterpret_cast<const float4 *>(bas + (size_t)j * 2 * D);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
-                        const C4 vj = load_group(bA, bA + D4, g * T + t);
+                        const C4 vj = load_group_g(bA, bA + D4, g * T + t);
 #pragma unroll
                         for (int c = 0; c < 4; ++c) psi[g].c[c] = cfma(cj, vj.c[c], psi[g].c[c]);
                     }
@@ -869,7 +888,7 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
                     for (int c = 0; c < 4; ++c) v0.c[c] = make_float2(psi[g].c[c].x * inv, psi[g].c[c].y * inv);
                     const int nu = g * T + t;
                     store_group(LA(0), LB(0), nu, v0);
-                    store_group(bA, bA + D4, nu, v0);
+                    store_group_g(bA, bA + D4, nu, v0);
                 }
             }
             my_al = 0.0;  // lane j keeps alpha_j and beta_j
@@ -897,7 +916,7 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
                     const int nu = g * T + t;
                     const C4 &vj = vjs[g];
                     if (j > 0) {
-                        const C4 vp = DBUF ? load_group(pA, pB, nu) : load_group(gA, gA + D4, nu);
+                        const C4 vp = DBUF ? load_group(pA, pB, nu) : load_group_g(gA, gA + D4, nu);
 #pragma unroll
                         for (int c = 0; c < 4; ++c) {
                             w[g].c[c].x = fmaf(-fb, vp.c[c].x, w[g].c[c].x);
@@ -964,7 +983,7 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
                         for (int c = 0; c < 4; ++c) w[g].c[c] = make_float2(w[g].c[c].x * ib, w[g].c[c].y * ib);
                         const int nu = g * T + t;
                         store_group(pA, pB, nu, w[g]);  // v_{j+1} replaces v_{j-1} (DBUF) or v_j (own elements only)
-                        store_group(bA, bA + D4, nu, w[g]);
+                        store_group_g(bA, bA + D4, nu, w[g]);
                     }
                 }
                 bprev = beta;
@@ -991,7 +1010,7 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
                 const float4 *bA = reinterpret_cast<const float4 *>(bas + (size_t)j * 2 * D);
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
-                    const C4 vj = load_group(bA, bA + D4, g * T + t);
+                    const C4 vj = load_group_g(bA, bA + D4, g * T + t);
 #pragma unroll
                     for (int c = 0; c < 4; ++c) psi[g].c[c] = cfma(cj, vj.c[c], psi[g].c[c]);
                 }
@@ -1011,7 +1030,7 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
     if (a.state && act) {
         float4 *dA = reinterpret_cast<float4 *>(a.state + (size_t)u * 2 * D);
 #pragma unroll
-        for (int g = 0; g < G; ++g) store_group(dA, dA + D4, g * T + t, psi[g]);
+        for (int g = 0; g < G; ++g) store_group_g(dA, dA + D4, g * T + t, psi[g]);
     }
     bool bad = false;
 #pragma unroll
@@ -1112,7 +1131,7 @@ __global__ __launch_bounds__(T, G >= 4 ? 1 : 2) void k_rk4(const StepArgs *__res
             } else {
                 const float *src = a.init_mode == 1 ? a.given + (size_t)col * 2 * D : a.state + (size_t)u * 2 * D;
                 const float4 *sA = reinterpret_cast<const float4 *>(src);
-                psi[g] = load_group(sA, sA + D4, nu);
+                psi[g] = load_group_g(sA, sA + D4, nu);
             }
         }
         if (t == 0)
@@ -1186,7 +1205,7 @@ __global__ __launch_bounds__(T, G >= 4 ? 1 : 2) void k_rk4(const StepArgs *__res
         if (a.state) {
             float4 *dA = reinterpret_cast<float4 *>(a.state + (size_t)u * 2 * D);
 #pragma unroll
-            for (int g = 0; g < G; ++g) store_group(dA, dA + D4, g * T + t, psi[g]);
+            for (int g = 0; g < G; ++g) store_group_g(dA, dA + D4, g * T + t, psi[g]);
         }
         bool bad = false;
 #pragma unroll
@@ -1244,7 +1263,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
             } else {
                 const float *src = a.init_mode == 1 ? a.given + (size_t)col * 2 * D : a.state + (size_t)u * 2 * D;
                 const float4 *sA = reinterpret_cast<const float4 *>(src);
-                psi[g] = load_group(sA, sA + D4, nu);
+                psi[g] = load_group_g(sA, sA + D4, nu);
             }
         }
         if (t == 0)
@@ -1292,7 +1311,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
                         for (int c = 0; c < 4; ++c) v0.c[c] = make_float2(psi[g].c[c].x * inv, psi[g].c[c].y * inv);
                         const int nu = g * T + t;
                         store_group(LA, LB, nu, v0);
-                        store_group(bA, bA + D4, nu, v0);
+                        store_group_g(bA, bA + D4, nu, v0);
                     }
                 }
                 int meff = meff0;
@@ -1320,7 +1339,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
                     // h_ij spilled registers and measured 6 % slower.)
                     auto basis_group = [&](int i, int nu) {
                         const float4 *gA = reinterpret_cast<const float4 *>(bas + (size_t)i * 2 * D);
-                        return i == j ? load_group(LA, LB, nu) : load_group(gA, gA + D4, nu);
+                        return i == j ? load_group(LA, LB, nu) : load_group_g(gA, gA + D4, nu);
                     };
                     C4 vi[G];
 #pragma unroll
@@ -1347,7 +1366,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
                         if (i + 1 < j) {
                             const float4 *nA = reinterpret_cast<const float4 *>(bas + (size_t)(i + 1) * 2 * D);
 #pragma unroll
-                            for (int g = 0; g < G; ++g) vi[g] = load_group(nA, nA + D4, g * T + t);
+                            for (int g = 0; g < G; ++g) vi[g] = load_group_g(nA, nA + D4, g * T + t);
                         } else if (i + 1 == j) {
 #pragma unroll
                             for (int g = 0; g < G; ++g) vi[g] = load_group(LA, LB, g * T + t);
@@ -1377,7 +1396,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
                         for (int c = 0; c < 4; ++c) w[g].c[c] = make_float2(w[g].c[c].x * ib, w[g].c[c].y * ib);
                         const int nu = g * T + t;
                         store_group(LA, LB, nu, w[g]);
-                        store_group(bA, bA + D4, nu, w[g]);
+                        store_group_g(bA, bA + D4, nu, w[g]);
                     }
                 }
                 __syncthreads();  // H_m complete
@@ -1437,7 +1456,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
                     const float4 *bA = reinterpret_cast<const float4 *>(bas + (size_t)j * 2 * D);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
-                        const C4 vj = load_group(bA, bA + D4, g * T + t);
+                        const C4 vj = load_group_g(bA, bA + D4, g * T + t);
 #pragma unroll
                         for (int c = 0; c < 4; ++c) psi[g].c[c] = cfma(cj, vj.c[c], psi[g].c[c]);
                     }
@@ -1455,7 +1474,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
         if (a.state) {
             float4 *dA = reinterpret_cast<float4 *>(a.state + (size_t)u * 2 * D);
 #pragma unroll
-            for (int g = 0; g < G; ++g) store_group(dA, dA + D4, g * T + t, psi[g]);
+            for (int g = 0; g < G; ++g) store_group_g(dA, dA + D4, g * T + t, psi[g]);
         }
         bool bad = false;
 #pragma unroll
@@ -1510,9 +1529,9 @@ __device__ __forceinline__ void lanczos_big_column(const StepArgs &a, int u, flo
         } else {
             const float *src = a.init_mode == 1 ? a.given + (size_t)col * 2 * D : a.state + (size_t)u * 2 * D;
             const float4 *sA = reinterpret_cast<const float4 *>(src);
-            x = load_group(sA, sA + D4, nu);
+            x = load_group_g(sA, sA + D4, nu);
         }
-        store_group(PA, PB, nu, x);
+        store_group_g(PA, PB, nu, x);
     }
     if (t == 0)
         for (int c = 0; c < a.nobs; ++c) {
@@ -1531,11 +1550,11 @@ __device__ __forceinline__ void lanczos_big_column(const StepArgs &a, int u, flo
             for (int k = 0; k < 16; ++k) Rf[k] = 0.f;
             double nn = 0.0;
             for (int nu = t; nu < ng; nu += T) {
-                C4 x = load_group(PA, PB, nu);
+                C4 x = load_group_g(PA, PB, nu);
                 accum_rho(x, Rf);
                 if (flip0) {
                     st_flip(x);
-                    store_group(PA, PB, nu, x);
+                    store_group_g(PA, PB, nu, x);
                 }
 #pragma unroll
                 for (int c = 0; c < 4; ++c) nn += (double)x.c[c].x * x.c[c].x + (double)x.c[c].y * x.c[c].y;
@@ -1553,10 +1572,10 @@ __device__ __forceinline__ void lanczos_big_column(const StepArgs &a, int u, flo
             if (beta0 > 0.0) {
                 const float inv = (float)(1.0 / beta0);
                 for (int nu = t; nu < ng; nu += T) {
-                    C4 x = load_group(PA, PB, nu);
+                    C4 x = load_group_g(PA, PB, nu);
 #pragma unroll
                     for (int c = 0; c < 4; ++c) x.c[c] = make_float2(x.c[c].x * inv, x.c[c].y * inv);
-                    store_group(BA(0), BA(0) + D4, nu, x);
+                    store_group_g(BA(0), BA(0) + D4, nu, x);
                     if (LV) store_group(lvs, lvs + D4, nu, x);
                 }
                 double my_al = 0.0, my_be = 0.0, bprev = 0.0;
@@ -1573,7 +1592,7 @@ __device__ __forceinline__ void lanczos_big_column(const StepArgs &a, int u, flo
                     auto pass_a = [&](int nu, const C4 &vj, const C4 &w0, C4 *keep) {
                         C4 w = w0;
                         if (j > 0) {
-                            const C4 vp = load_group(pA, pB, nu);
+                            const C4 vp = load_group_g(pA, pB, nu);
 #pragma unroll
                             for (int c = 0; c < 4; ++c) {
                                 w.c[c].x = fmaf(-fb, vp.c[c].x, w.c[c].x);
@@ -1586,16 +1605,13 @@ __device__ __forceinline__ void lanczos_big_column(const StepArgs &a, int u, flo
                             red[1] += (double)w.c[c].x * w.c[c].x + (double)w.c[c].y * w.c[c].y;
                         }
                         if (keep) *keep = w;
-                        else store_group(WA, WB, nu, w);
+                        else store_group_g(WA, WB, nu, w);
                     };
                     if (TILED) {
                         float4 *tA = lvs, *tB = lvs + kTileG;
                         for (int tb = 0; tb < ng; tb += kTileG) {
                             if (tb > 0) __syncthreads();  // the previous tile is no longer read
-                            for (int k = t; k < kTileG; k += T) {
-                                tA[k] = vA[tb + k];
-                                tB[k] = vB[tb + k];
-                            }
+                            for (int k = t; k < kTileG; k += T) store_group(tA, tB, k, load_group_g(vA, vB, tb + k));
                             __syncthreads();
                             for (int k = t; k < kTileG; k += T) {
                                 const int nu = tb + k;
@@ -1612,7 +1628,7 @@ __device__ __forceinline__ void lanczos_big_column(const StepArgs &a, int u, flo
                         }
                     } else {
                         for (int nu = t; nu < ng; nu += T) {
-                            const C4 vj = load_group(vA, vB, nu);
+                            const C4 vj = load_group_g(vA, vB, nu);
                             pass_a(nu, vj, apply_h<-1, false, true>(a, co, hd, nu, vj, vA, vB, nullptr, nullptr, sh.blk), nullptr);
                         }
                     }
@@ -1635,7 +1651,7 @@ __device__ __forceinline__ void lanczos_big_column(const StepArgs &a, int u, flo
 #pragma unroll
                             for (int g = 0; g < (LV ? 4 : 1); ++g) acc_bb(load_group(vA, vB, g * T + t), wr[g]);
                         } else {
-                            for (int nu = t; nu < ng; nu += T) acc_bb(load_group(vA, vB, nu), load_group(WA, WB, nu));
+                            for (int nu = t; nu < ng; nu += T) acc_bb(load_group_g(vA, vB, nu), load_group_g(WA, WB, nu));
                         }
                         block_sum<1>(bb, SCR());
                         beta2 = bb[0];
@@ -1649,18 +1665,18 @@ __device__ __forceinline__ void lanczos_big_column(const StepArgs &a, int u, flo
                     const float ib = (float)(1.0 / beta);
                     float4 *nA = BA(j + 1), *nB = nA + D4;
                     auto pass_b = [&](int nu, C4 w) {
-                        const C4 vj = load_group(vA, vB, nu);
+                        const C4 vj = LV ? load_group(vA, vB, nu) : load_group_g(vA, vB, nu);
 #pragma unroll
                         for (int c = 0; c < 4; ++c)
                             w.c[c] = make_float2(fmaf(-fa, vj.c[c].x, w.c[c].x) * ib, fmaf(-fa, vj.c[c].y, w.c[c].y) * ib);
-                        store_group(nA, nB, nu, w);
+                        store_group_g(nA, nB, nu, w);
                         if (LV) store_group(lvs, lvs + D4, nu, w);  // own elements: read above by this thread only
                     };
                     if (LV) {
 #pragma unroll
                         for (int g = 0; g < (LV ? 4 : 1); ++g) pass_b(g * T + t, wr[g]);
                     } else {
-                        for (int nu = t; nu < ng; nu += T) pass_b(nu, load_group(WA, WB, nu));
+                        for (int nu = t; nu < ng; nu += T) pass_b(nu, load_group_g(WA, WB, nu));
                     }
                     bprev = beta;
                 }
@@ -1677,19 +1693,19 @@ __device__ __forceinline__ void lanczos_big_column(const StepArgs &a, int u, flo
                     for (int c = 0; c < 4; ++c) x.c[c] = make_float2(0.f, 0.f);
                     for (int j = 0; j < meff; ++j) {
                         const float2 cj = sh.coef[j];
-                        const C4 vj = load_group(BA(j), BA(j) + D4, nu);
+                        const C4 vj = load_group_g(BA(j), BA(j) + D4, nu);
 #pragma unroll
                         for (int c = 0; c < 4; ++c) x.c[c] = cfma(cj, vj.c[c], x.c[c]);
                     }
                     if (a.relax && relax_flip(a, col, s, 1u, item))
                         st_flip(x);
-                    store_group(PA, PB, nu, x);
+                    store_group_g(PA, PB, nu, x);
                 }
             } else if (a.relax && relax_flip(a, col, s, 1u, item)) {
                 for (int nu = t; nu < ng; nu += T) {
-                    C4 x = load_group(PA, PB, nu);
+                    C4 x = load_group_g(PA, PB, nu);
                     st_flip(x);
-                    store_group(PA, PB, nu, x);
+                    store_group_g(PA, PB, nu, x);
                 }
             }
             __syncthreads();  // psi complete before the next sample
@@ -1702,10 +1718,10 @@ __device__ __forceinline__ void lanczos_big_column(const StepArgs &a, int u, flo
         }
     bool bad = false;
     for (int nu = t; nu < ng; nu += T) {
-        const C4 x = load_group(PA, PB, nu);
+        const C4 x = load_group_g(PA, PB, nu);
         if (a.state) {
             float4 *dA = reinterpret_cast<float4 *>(a.state + (size_t)u * 2 * D);
-            store_group(dA, dA + D4, nu, x);
+            store_group_g(dA, dA + D4, nu, x);
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c) bad |= !isfinite(x.c[c].x) || !isfinite(x.c[c].y);
@@ -1813,13 +1829,15 @@ __global__ void k_reduce_columns(const double *in, double *out, int n_items, int
 __device__ __forceinline__ void slot_arrive(int *bar) {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's global stores are in L2
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    typedef __attribute__((address_space(1))) int gint;
+    if (threadIdx.x == 0) __hip_atomic_fetch_add((gint *)bar, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ bool slot_wait(int *bar, int S, int &nbar, int *status, int *s_flag) {
     if (threadIdx.x == 0) {
         const int target = S * (nbar + 1);
         int ok = 1;
-        for (int spins = 0; __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++spins) {
+        typedef __attribute__((address_space(1))) int gint;
+        for (int spins = 0; __hip_atomic_load((gint *)bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++spins) {
             if (spins > (1 << 26)) {  // about a second: a workgroup of the slot is gone
                 ok = 0;
                 atomicOr(status, 4);
@@ -1858,7 +1876,8 @@ __device__ __forceinline__ bool slot_sum(double (&v)[K], uint64_t *xr, int S, in
                                          int *s_flag, double *s_tot) {
     constexpr uint64_t kTagBits = 0xFF;
     const uint64_t tag = 1 + (uint64_t)(nsum % 255);
-    uint64_t *buf = xr + (size_t)(nsum & 1) * S * 17;
+    typedef __attribute__((address_space(1))) uint64_t guint64;  // global, not flat, accesses
+    guint64 *buf = (guint64 *)(xr + (size_t)(nsum & 1) * S * 17);
     ++nsum;
     if (threadIdx.x == 0)
 #pragma unroll
@@ -1957,7 +1976,7 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
             } else {
                 const float *src = a.init_mode == 1 ? a.given + (size_t)col * 2 * D : a.state + (size_t)u * 2 * D;
                 const float4 *sA = reinterpret_cast<const float4 *>(src);
-                psi[g] = load_group(sA, sA + D4, nu);
+                psi[g] = load_group_g(sA, sA + D4, nu);
             }
         }
         if (lead)
@@ -2009,7 +2028,7 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
 #pragma unroll
                         for (int c = 0; c < 4; ++c) v0.c[c] = make_float2(psi[g].c[c].x * inv, psi[g].c[c].y * inv);
                         store_group(LA(0), LB(0), g * T + t, v0);
-                        store_group(bA, bA + D4, base + g * T + t, v0);
+                        store_group_g(bA, bA + D4, base + g * T + t, v0);
                     }
                 }
                 slot_arrive(bar);  // v_0 stored; the slot waits before the first cross-slab read
@@ -2087,7 +2106,7 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
 #pragma unroll
                         for (int c = 0; c < 4; ++c) w[g].c[c] = make_float2(w[g].c[c].x * ib, w[g].c[c].y * ib);
                         store_group(pA, pB, g * T + t, w[g]);  // v_{j+1} replaces v_{j-1}
-                        store_group(bA, bA + D4, base + g * T + t, w[g]);
+                        store_group_g(bA, bA + D4, base + g * T + t, w[g]);
                     }
                     bprev = beta;
                     cur ^= 1;
@@ -2111,7 +2130,7 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
                     const float4 *bA = reinterpret_cast<const float4 *>(bas + (size_t)j * 2 * D);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
-                        const C4 vj = load_group(bA, bA + D4, base + g * T + t);
+                        const C4 vj = load_group_g(bA, bA + D4, base + g * T + t);
 #pragma unroll
                         for (int c = 0; c < 4; ++c) psi[g].c[c] = cfma(cj, vj.c[c], psi[g].c[c]);
                     }
@@ -2130,7 +2149,7 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
         if (a.state) {
             float4 *dA = reinterpret_cast<float4 *>(a.state + (size_t)u * 2 * D);
 #pragma unroll
-            for (int g = 0; g < G; ++g) store_group(dA, dA + D4, base + g * T + t, psi[g]);
+            for (int g = 0; g < G; ++g) store_group_g(dA, dA + D4, base + g * T + t, psi[g]);
         }
         bool bad = false;
 #pragma unroll
