@@ -73,6 +73,10 @@ __device__ __forceinline__ C4 load_group_g(const float4 *A, const float4 *B, int
     r.c[3] = make_float2(q.z, q.w);
     return r;
 }
+__device__ __forceinline__ float4 load_f4_g(const float *p) {
+    const f4v v = *(const gf4v *)p;
+    return float4{v.x, v.y, v.z, v.w};
+}
 __device__ __forceinline__ void store_group_g(float4 *A, float4 *B, int nu, const C4 &x) {
     ((gf4v *)A)[nu] = f4v{x.c[0].x, x.c[0].y, x.c[1].x, x.c[1].y};
     ((gf4v *)B)[nu] = f4v{x.c[2].x, x.c[2].y, x.c[3].x, x.c[3].y};
@@ -2317,8 +2321,8 @@ __device__ __forceinline__ void dense_tiles(const StepArgs &a, int u0, int nact,
         for (int k = 0; k < KT; ++k) {
             const int rt = wave + NW * k;
             const size_t o = ((size_t)(rt * ntile) * 64 + lane) * 4;
-            nr4[k] = rt < ntile ? *reinterpret_cast<const float4 *>(Ur + o) : float4{0.f, 0.f, 0.f, 0.f};
-            ni4[k] = rt < ntile ? *reinterpret_cast<const float4 *>(Ui + o) : float4{0.f, 0.f, 0.f, 0.f};
+            nr4[k] = rt < ntile ? load_f4_g(Ur + o) : float4{0.f, 0.f, 0.f, 0.f};
+            ni4[k] = rt < ntile ? load_f4_g(Ui + o) : float4{0.f, 0.f, 0.f, 0.f};
         }
         for (int g = 0; g < ntile; ++g) {
             float br[4], bi[4], bs[4];
@@ -2337,8 +2341,8 @@ __device__ __forceinline__ void dense_tiles(const StepArgs &a, int u0, int nact,
                 const int rt = wave + NW * k;
                 if (rt < ntile && g + 1 < ntile) {
                     const size_t o = ((size_t)(rt * ntile + g + 1) * 64 + lane) * 4;
-                    nr4[k] = *reinterpret_cast<const float4 *>(Ur + o);
-                    ni4[k] = *reinterpret_cast<const float4 *>(Ui + o);
+                    nr4[k] = load_f4_g(Ur + o);  // global loads (no measurable change against flat here)
+                    ni4[k] = load_f4_g(Ui + o);
                 }
             }
 #pragma unroll
