@@ -663,6 +663,11 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         }
         if (b.G > 0 && (b.T < 64 || b.T > 1024 || b.G * b.T != ng))
             throw ArgError{MSG_UNSUPPORTED, "no matrix-free launch shape for this dimension"};
+        // nucleus-nucleus couplings read three partner groups across any two index bits:
+        // the shapes that hold the whole Krylov vector in LDS (the grouped operator)
+        if (!r.nn_pairs.empty() && b.G <= 0)
+            throw ArgError{MSG_UNSUPPORTED, "nucleus-nucleus couplings on the matrix-free path need D <= 8192 "
+                                            "(D <= 16384 with unequal rates or RK4), or D <= 512 (dense path)"};
         a.nnuc = (int)r.nuc.size();
         std::vector<float> all, one;
         int flags = 0;
@@ -681,9 +686,15 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
             a.op_flags |= flags;
             std::copy(one.begin(), one.end(), all.begin() + (size_t)it * a.coef_stride);
         }
+        for (const auto &pr : r.nn_pairs) {  // the couplings' nucleus bits follow the flags
+            nfl.push_back(pr[0]);
+            nfl.push_back(pr[1]);
+        }
         int *dfl = b.nflags.alloc<int>(nfl.size(), stream);
         HIPCHECK(hipMemcpyAsync(dfl, nfl.data(), nfl.size() * sizeof(int), hipMemcpyHostToDevice, stream));
         a.nuc_flags = dfl;
+        a.npairs = (int)r.nn_pairs.size();
+        a.pair_bits = a.npairs ? dfl + kMaxNuclei : nullptr;
         float *dc = b.coef.alloc<float>(all.size(), stream);
         HIPCHECK(hipMemcpyAsync(dc, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice, stream));
         if (dyn) {
@@ -1154,6 +1165,8 @@ UnevenShape uneven_shape(const Batch &b) {
         while ((1 << lg) < ng) ++lg;
         if ((1 << lg) != ng || lg < 6 || lg > 12)
             throw ArgError{MSG_UNSUPPORTED, "uneven batch: matrix-free entries need 256 <= D <= 16384"};
+        if (lg == 12 && b.a.npairs > 0)
+            throw ArgError{MSG_UNSUPPORTED, "uneven batch: nucleus-nucleus couplings need D <= 8192"};
         s.cls = UC_LZ64 + (lg - 6);
         s.per_pack = uneven_pack_width(s.cls);
         s.basis_floats = s.cls == UC_LZ4096 ? (size_t)(b.a.m + 2) * 2 * D : (size_t)s.per_pack * b.a.m * 2 * D;
@@ -1193,7 +1206,7 @@ bool uneven_eligible(const msg_system *sys, const msg_run_params *p) {
     if (path == Path::Dense) return r.D <= 512;
     const int64_t ng = r.D / 4;
     return sym_for(r, p) && p->method != MSG_METHOD_RK4 && !(p->krylov_tol > 0.0) && r.mix_mult == 0 && ng >= 64 &&
-           ng <= 4096 && (ng & (ng - 1)) == 0;
+           ng <= (r.nn_pairs.empty() ? 4096 : 2048) && (ng & (ng - 1)) == 0;
 }
 
 // Prepares every entry on stream st and queues one persistent k_uneven launch over all
@@ -1489,7 +1502,7 @@ msg_status msg_query_footprint(const msg_system *sys, const msg_run_params *p, i
     const int m = p->krylov_size > 0 ? p->krylov_size : 16;
     if (path == Path::Lanczos) {
         const int64_t units = (int64_t)n_items * ncols;
-        b += (int64_t)n_items * (kCoefHead + (int64_t)r.nuc.size() * kNucStride) * 4;
+        b += (int64_t)n_items * (kCoefHead + (int64_t)r.nuc.size() * kNucStride + (int64_t)r.nn_pairs.size() * kPairStride) * 4;
         b += units * (nobs * 16 + 8);
         const int vecs = D > 8192 ? m + 2 : m;  // streaming variants keep residual and state beside the basis
         b += std::min<int64_t>(units, 2048) * vecs * 8 * D;  // Krylov bases of the resident workgroups

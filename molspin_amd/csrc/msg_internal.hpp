@@ -15,6 +15,7 @@ namespace msg {
 constexpr int kMaxKrylov = 32;
 constexpr int kMaxObs = 8;
 constexpr int kMaxNuclei = 16;     // matrix-free path: D = 4 * 2^N <= 262144
+constexpr int kMaxPairs = 16;      // matrix-free path: nucleus-nucleus couplings
 constexpr int kMaxInitial = 4;
 constexpr int kChebMax = 96;
 constexpr int kTileGroups = 4096;  // groups per LDS tile of the tiled streaming Lanczos (bits 0-11)
@@ -32,8 +33,16 @@ constexpr unsigned long long kScratchBytes = 64ull << 20;
 //     [0..2] h1, [3..5] h2, [6] r         diagonal: s_n (h1.S1 + h2.S2 + r), s_n = +1 up / -1 down
 //     [7..14] C1 (2x2 complex on e1), [15..22] C2 (2x2 complex on e2), [23..24] t (complex)
 //     off-diagonal <up_n| H |down_n> = C1 (x) 1 + 1 (x) C2 + t 1 on the electron pair
+//   then per nucleus-nucleus coupling c (kPairStride floats): the coupling as a 4x4 complex
+//     matrix on the two nuclei's states (row-major, interleaved; index 2 b1 + b2 with b = 0
+//     up, 1 down, b1 the nucleus pair_bits[2 c]), times the identity on the electron pair
 constexpr int kCoefHead = 40;
 constexpr int kNucStride = 28;
+constexpr int kPairStride = 32;
+// LDS block table of the matrix-free kernels (float4 entries): per nucleus 8 (its 2x2
+// blocks on both electrons in both orientations), then per nucleus-nucleus coupling 8
+// (for each of the 4 rows: the diagonal, and the entries of the three partners)
+constexpr int kBlkTable = 8 * kMaxNuclei + 8 * kMaxPairs;
 enum NucFlags { NUC_E1 = 1, NUC_E2 = 2, NUC_T = 4 };
 enum OpFlags { OP_EREST = 1 };
 
@@ -47,6 +56,8 @@ struct StepArgs {
     int ncols;         // columns per item
     int op_flags;
     const int *nuc_flags;  // [nnuc] device
+    int npairs;            // nucleus-nucleus couplings (0: none)
+    const int *pair_bits;  // [npairs][2] device: the two nuclei (index bits) of each coupling
     const float *coef; // [n_items][coef_stride]
     int coef_stride;
     int arnoldi;            // 1: Arnoldi on G = -iH - K (unequal recombination rates)

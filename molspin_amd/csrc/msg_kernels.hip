@@ -190,6 +190,9 @@ __device__ __forceinline__ void load_block(P p, bool herm, float2 m[4]) {
 // conjugate-transposed for down = 1.  A lane then reads the block of its own index bit
 // instead of selecting it entry by entry.  Filled by the first 4 nnuc threads; the caller
 // synchronises the workgroup before the first read.
+// The rows of the nucleus-nucleus couplings follow at blk[8 kMaxNuclei + ((c 4 + r) 2 + 0..1)]:
+// for row r = 2 b1 + b2 of coupling c, (Re H_rr, H_{r,r^2}, H_{r,r^1}, H_{r,r^3}) with
+// the three partners flipping b1, b2 and both.
 __device__ __forceinline__ void fill_block_table(const StepArgs &a, const float *co, int t, float4 *blk) {
     if (t < a.nnuc * 4) {
         const int n = t >> 2, e = (t >> 1) & 1;
@@ -197,6 +200,12 @@ __device__ __forceinline__ void fill_block_table(const StepArgs &a, const float 
         load_block(co + kCoefHead + n * kNucStride + (e ? 15 : 7), (t & 1) != 0, m);
         blk[2 * t] = make_float4(m[0].x, m[0].y, m[1].x, m[1].y);
         blk[2 * t + 1] = make_float4(m[2].x, m[2].y, m[3].x, m[3].y);
+    }
+    if (t < a.npairs * 4) {
+        const int c = t >> 2, r = t & 3;
+        const float *h = co + kCoefHead + a.nnuc * kNucStride + c * kPairStride + 8 * r;
+        blk[8 * kMaxNuclei + 2 * t] = make_float4(h[2 * r], h[2 * (r ^ 2)], h[2 * (r ^ 2) + 1], h[2 * (r ^ 1)]);
+        blk[8 * kMaxNuclei + 2 * t + 1] = make_float4(h[2 * (r ^ 1) + 1], h[2 * (r ^ 3)], h[2 * (r ^ 3) + 1], 0.f);
     }
 }
 __device__ __forceinline__ void table_block(const float4 *blk, int n, int e, bool down, float2 m[4]) {
@@ -447,6 +456,31 @@ __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *_
         }
     }
     if constexpr (PART == 2) return;
+    if constexpr (PART == 0) {
+        // nucleus-nucleus couplings (every index bit of the vector is in LDS): row
+        // 2 b1 + b2 of the coupling's 4x4 block on the group and its three partners
+        const auto pb = as_const(a.pair_bits);
+        const float4 *ptab = blk + 8 * kMaxNuclei;
+        for (int c = 0; c < a.npairs; ++c) {
+            const int k1 = pb[2 * c], k2 = pb[2 * c + 1];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int nl = (g << LOGT) + t, hi = base + nl;
+                const int r = (((hi >> k1) & 1) << 1) | ((hi >> k2) & 1);
+                const float4 q0 = ptab[(c * 4 + r) * 2], q1 = ptab[(c * 4 + r) * 2 + 1];
+                const C4 x1 = load_group(LA, LB, nl ^ (1 << k1)), x2 = load_group(LA, LB, nl ^ (1 << k2)),
+                         x12 = load_group(LA, LB, nl ^ (1 << k1) ^ (1 << k2));
+                const float2 f1 = make_float2(q0.y, q0.z), f2 = make_float2(q0.w, q1.x), f12 = make_float2(q1.y, q1.z);
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    float2 v = cfma(f1, x1.c[cc], cfma(f2, x2.c[cc], cfma(f12, x12.c[cc], y[g].c[cc])));
+                    v.x = fmaf(q0.x, x[g].c[cc].x, v.x);
+                    v.y = fmaf(q0.x, x[g].c[cc].y, v.y);
+                    y[g].c[cc] = v;
+                }
+            }
+        }
+    }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const float rg = rr + er[g];
@@ -1054,7 +1088,7 @@ __global__ __launch_bounds__(TMAX, G >= 4 ? 2 : 4) void k_lanczos(const StepArgs
     double *scr = reinterpret_cast<double *>(smem + (DBUF ? 4 : 2) * ng);  // 2 x 16 x 16 doubles
     __shared__ double ys[kMaxObs], yc[kMaxObs];
     __shared__ float2 s_coef[kMaxKrylov];
-    __shared__ float4 s_blk[kMaxNuclei * 8];
+    __shared__ float4 s_blk[kBlkTable];
     __shared__ int s_unit;
     const BodyShared sh{ys, yc, s_coef, s_blk, nullptr};
     float *bas = a.basis + (size_t)blockIdx.x * a.m * 2 * a.dim;
@@ -1079,7 +1113,7 @@ __global__ __launch_bounds__(T, 1) void k_lanczos_mix(const StepArgs *__restrict
     double *scr = reinterpret_cast<double *>(smem + (DBUF ? 4 : 2) * ng);  // 2 x 16 x 16 doubles
     __shared__ double ys[kMaxObs], yc[kMaxObs];
     __shared__ float2 s_coef[kMaxKrylov];
-    __shared__ float4 s_blk[kMaxNuclei * 8];
+    __shared__ float4 s_blk[kBlkTable];
     __shared__ int s_unit;
     const BodyShared sh{ys, yc, s_coef, s_blk, nullptr};
     float *bas = a.basis + (size_t)blockIdx.x * a.m * 2 * a.dim;
@@ -1107,7 +1141,7 @@ __global__ __launch_bounds__(T, G >= 4 ? 1 : 2) void k_rk4(const StepArgs *__res
     float4 *LA = smem, *LB = smem + ng;
     double *scr = reinterpret_cast<double *>(smem + 2 * ng);  // 2 x 256 doubles
     __shared__ double ys[kMaxObs], yc[kMaxObs];
-    __shared__ float4 s_blk[kMaxNuclei * 8];
+    __shared__ float4 s_blk[kBlkTable];
     __shared__ int s_unit;
     const int t = threadIdx.x;
     const int D = a.dim, D4 = D >> 2;
@@ -1241,6 +1275,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
     __shared__ float2 s_coef[kMaxKrylov];
     __shared__ double ys[kMaxObs], yc[kMaxObs];
     __shared__ int s_unit;
+    __shared__ float4 s_blk[kBlkTable];  // the nucleus-nucleus rows (fill_block_table)
     const int D = a.dim, D4 = D >> 2;
     float *bas = a.basis + (size_t)blockIdx.x * a.m * 2 * D;
     const int meff0 = a.m < D ? a.m : D;
@@ -1254,6 +1289,7 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
         if (u >= a.n_units) break;
         const int item = u / a.ncols, col = u - item * a.ncols;
         const float *co = a.coef + (size_t)item * a.coef_stride;
+        fill_block_table(a, co, t, s_blk);  // read after the workgroup's next barrier
         C4 psi[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
@@ -1325,7 +1361,8 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
 #pragma unroll
                     for (int g = 0; g < G; ++g) vjs[g] = load_group(LA, LB, g * T + t);
                     const float *coj = opaque_uniform(co), *hdj = opaque_uniform(hd);
-                    apply_h_groups<Log2<ng>::value, G, Log2<T>::value>(a, coj, hdj, t, vjs, LA, LB, hv);
+                    apply_h_groups<Log2<ng>::value, G, Log2<T>::value>(a, coj, hdj, t, vjs, LA, LB, hv, 0, nullptr, nullptr,
+                                                                       s_blk);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
 #pragma unroll
@@ -1741,7 +1778,7 @@ __global__ __launch_bounds__(1024) void k_lanczos_big(const StepArgs *__restrict
     __shared__ double scr[2 * 17 * 16];  // block_sum of up to 17 values, double-buffered
     __shared__ double ys[kMaxObs], yc[kMaxObs];
     __shared__ float2 s_coef[kMaxKrylov];
-    __shared__ float4 s_blk[kMaxNuclei * 8];  // coupling blocks, both orientations
+    __shared__ float4 s_blk[kBlkTable];  // coupling blocks, both orientations
     __shared__ int s_unit;
     const BodyShared sh{ys, yc, s_coef, s_blk, nullptr};
     // per-slot scratch: basis (m vectors), residual W, state P
@@ -2853,7 +2890,7 @@ __global__ __launch_bounds__(kUnevenThreads) void k_uneven(const UnevenArgs *__r
     extern __shared__ float4 usm[];  // kUnevenPlanesBytes of planes, then the reduction scratch
     __shared__ double ys[16 * kMaxObs], yc[16 * kMaxObs];
     __shared__ float2 s_coef[16 * kMaxKrylov];
-    __shared__ float4 s_blk[kMaxNuclei * 8];
+    __shared__ float4 s_blk[kBlkTable];
     __shared__ double s_col[2 * 256];
     __shared__ int s_pack;
     double *scr = reinterpret_cast<double *>(reinterpret_cast<char *>(usm) + kUnevenPlanesBytes);

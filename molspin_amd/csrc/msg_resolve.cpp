@@ -444,10 +444,28 @@ std::string resolve_system(const msg_system_desc *d, Resolved &r) {
         }
     }
     if ((int)r.nuc.size() > kMaxNuclei) { r.fast_ok = false; r.why_not_fast = "too many nuclei"; }
-    for (const Term2 &t : r.t2) {
-        const bool e_a = t.s1 == r.e1 || t.s1 == r.e2, e_b = t.s2 == r.e1 || t.s2 == r.e2;
-        if (e_a && e_b) r.has_ee = true;
-        if (!e_a && !e_b) { r.fast_ok = false; r.why_not_fast = "nucleus-nucleus coupling"; }
+    r.nn_pairs.clear();
+    {
+        std::vector<int> nucpos(r.nspins, -1);
+        for (size_t k = 0; k < r.nuc.size(); ++k) nucpos[r.nuc[k]] = (int)k;
+        for (const Term2 &t : r.t2) {
+            const bool e_a = t.s1 == r.e1 || t.s1 == r.e2, e_b = t.s2 == r.e1 || t.s2 == r.e2;
+            if (e_a && e_b) r.has_ee = true;
+            if (e_a || e_b) continue;
+            // nucleus-nucleus coupling: a 4x4 block on the two nuclei's index bits
+            if (r.mult[t.s1] != 2 || r.mult[t.s2] != 2) {
+                r.fast_ok = false;
+                r.why_not_fast = "nucleus-nucleus coupling with a nucleus of spin > 1/2";
+                continue;
+            }
+            const int k1 = std::min(nucpos[t.s1], nucpos[t.s2]), k2 = std::max(nucpos[t.s1], nucpos[t.s2]);
+            const std::array<int, 2> pr{k1, k2};
+            if (std::find(r.nn_pairs.begin(), r.nn_pairs.end(), pr) == r.nn_pairs.end()) r.nn_pairs.push_back(pr);
+        }
+        if ((int)r.nn_pairs.size() > kMaxPairs) {
+            r.fast_ok = false;
+            r.why_not_fast = "more than " + std::to_string(kMaxPairs) + " nucleus-nucleus couplings";
+        }
     }
     // states, transitions, reaction operator
     for (int s = 0; s < d->n_states; ++s) {
@@ -571,8 +589,8 @@ bool term1_value(const Term1 &t, const double *field, const TimePoint *tp, doubl
 
 void fast_coefficients(const Resolved &r, const double *field, std::vector<float> &out, int &op_flags,
                        int nuc_flags[kMaxNuclei], const TimePoint *tp) {
-    const int N = (int)r.nuc.size();
-    std::vector<double> c((size_t)kCoefHead + (size_t)N * kNucStride, 0.0);
+    const int N = (int)r.nuc.size(), P = (int)r.nn_pairs.size();
+    std::vector<double> c((size_t)kCoefHead + (size_t)N * kNucStride + (size_t)P * kPairStride, 0.0);
     std::vector<int> nucpos(r.nspins, -1);
     for (int k = 0; k < N; ++k) nucpos[r.nuc[k]] = k;
     for (int k = 0; k < kMaxNuclei; ++k) nuc_flags[k] = 0;
@@ -611,6 +629,28 @@ void fast_coefficients(const Resolved &r, const double *field, std::vector<float
                             E[i * 4 + j] += m * A1[(i >> 1) * 2 + (j >> 1)] * A2[(i & 1) * 2 + (j & 1)];
                 }
             op_flags |= OP_EREST;
+            continue;
+        }
+        if (!e1a && !e2b) {
+            // nucleus-nucleus: sum_ab M_ab I_{k1,a} I_{k2,b} as kron(k1 op, k2 op)
+            const int k1 = nucpos[t.s1], k2 = nucpos[t.s2];
+            const std::array<int, 2> pr{std::min(k1, k2), std::max(k1, k2)};
+            const int pc = (int)(std::find(r.nn_pairs.begin(), r.nn_pairs.end(), pr) - r.nn_pairs.begin());
+            double *ch = &c[kCoefHead + (size_t)N * kNucStride + (size_t)pc * kPairStride];
+            const bool first = k1 == pr[0];  // t.s1 is the pair's first nucleus
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    const double m = t.M[3 * a + b];
+                    if (m == 0.0) continue;
+                    const std::vector<cd> &A1 = first ? *S[a] : *S[b];
+                    const std::vector<cd> &A2 = first ? *S[b] : *S[a];
+                    for (int i = 0; i < 4; ++i)
+                        for (int j = 0; j < 4; ++j) {
+                            const cd v = m * A1[(i >> 1) * 2 + (j >> 1)] * A2[(i & 1) * 2 + (j & 1)];
+                            ch[8 * i + 2 * j] += v.real();
+                            ch[8 * i + 2 * j + 1] += v.imag();
+                        }
+                }
             continue;
         }
         // electron-nucleus: M_ab with a on the electron, b on the nucleus

@@ -58,7 +58,10 @@ struct Resolved {
     std::vector<Term2> t2;
     double shift = 0.0;
     bool pair_ok = false;         // two spin-1/2 electrons, electron-only states
-    bool fast_ok = false;         // + all nuclei spin-1/2 and no nucleus-nucleus terms
+    bool fast_ok = false;         // + matrix-free operator (nuclei spin-1/2 but at most one,
+                                  //   nucleus-nucleus terms between spin-1/2 nuclei only)
+    std::vector<std::array<int, 2>> nn_pairs;  // nucleus-nucleus couplings: device nucleus
+                                               // indices (k1 < k2), one entry per pair
     bool has_ee = false;          // electron-electron couplings present
     bool has_td = false;          // time-dependent Zeeman terms present
     bool td_nuclear = false;      // ... acting on a nucleus

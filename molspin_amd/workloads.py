@@ -67,6 +67,23 @@ def _radical_pair(name, nuc_a, nuc_b, hyperfine, rate, field=(0.0, 0.0, 0.05), s
     return s
 
 
+def add_nuclear_couplings(s, na, nb, seed=0):
+    """Couplings between nuclei of a `_radical_pair` system with na + nb spin-1/2
+    nuclei (mT, hyperfine-type two-spin interactions): an anisotropic one between the
+    first two nuclei of A, an isotropic one between the last nuclei of A and B (the
+    most significant index bits), and an anisotropic one across the radicals between A0
+    and the last B nucleus (needs na >= 2, nb >= 1)."""
+    rng = SplitMix64(53, seed)
+
+    def tensor(scale):
+        m = _aniso_hf(rng)
+        return [[scale * m[i][j] for j in range(3)] for i in range(3)]
+    s.add_hyperfine(["A0"], ["A1"], tensor=tensor(0.2), prefactor=0.001, name="nn_a01")
+    s.add_hyperfine(["A%d" % (na - 1)], ["B%d" % (nb - 1)], tensor=0.15, prefactor=0.001, name="nn_ab")
+    s.add_hyperfine(["B%d" % (nb - 1)], ["A0"], tensor=tensor(0.1), prefactor=0.001, name="nn_ba")
+    return s
+
+
 def config1():
     """2 x spin-1/2 on A (0.5, 1.2 mT), 1 x spin-1 on B (0.8 mT); D = 48;
     B = 0.05 mT along z; k = 1e-3 /ns; 0-5000 ns at dt = 1 ns."""

@@ -94,6 +94,17 @@ int shim_fast_apply(const msg_system_desc *d, const double *field, const float *
             }
         }
         for (int c = 0; c < 4; ++c) out[c] += rr * X(nu, c);
+        // nucleus-nucleus couplings: row 2 b1 + b2 of each pair's 4x4 block
+        for (size_t pc = 0; pc < r.nn_pairs.size(); ++pc) {
+            const int k1 = r.nn_pairs[pc][0], k2 = r.nn_pairs[pc][1];
+            const float *h = &co[kCoefHead + N * kNucStride + pc * kPairStride];
+            const int row = (((nu >> k1) & 1) << 1) | ((nu >> k2) & 1);
+            for (int col = 0; col < 4; ++col) {
+                const int np = (nu & ~((1 << k1) | (1 << k2))) | ((col >> 1) << k1) | ((col & 1) << k2);
+                const std::complex<float> hv(h[8 * row + 2 * col], h[8 * row + 2 * col + 1]);
+                for (int c = 0; c < 4; ++c) out[c] += hv * X(np, c);
+            }
+        }
         auto fm = [](const float f[3], std::complex<float> m[4]) {
             m[0] = {0.5f * f[2], 0.f}; m[1] = {0.5f * f[0], -0.5f * f[1]};
             m[2] = {0.5f * f[0], 0.5f * f[1]}; m[3] = {-0.5f * f[2], 0.f};

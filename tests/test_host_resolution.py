@@ -57,7 +57,7 @@ def device_dense(shim, packed, field=None):
     return out.view(np.complex128).reshape(D, D)  # row-major
 
 
-def small_aniso_pair(na, nb, seed=0, exchange=None, nuclear_zeeman=False):
+def small_aniso_pair(na, nb, seed=0, exchange=None, nuclear_zeeman=False, nuclear_pairs=False):
     rng = workloads.SplitMix64(31, seed)
     hf = [workloads._aniso_hf(rng) for _ in range(na + nb)]
     s = workloads._radical_pair("t", ["1/2"] * na, ["1/2"] * nb, hf, 1e-3, field=(0.01, -0.02, 0.05), swept=True)
@@ -65,6 +65,8 @@ def small_aniso_pair(na, nb, seed=0, exchange=None, nuclear_zeeman=False):
         s.add_exchange(["E1"], ["E2"], exchange, prefactor=0.001)
     if nuclear_zeeman:
         s.add_zeeman(["A0", "B0"], (0.0, 0.0, 50.0), prefactor=0.001, swept=False)
+    if nuclear_pairs:
+        workloads.add_nuclear_couplings(s, na, nb, seed)
     return s
 
 
@@ -84,11 +86,12 @@ def test_device_order_dense_hamiltonian_matches_oracle(shim, name):
     assert np.abs(Hd[np.ix_(perm, perm)] - Hr).max() < 1e-14
 
 
-@pytest.mark.parametrize("variant", [dict(), dict(exchange=0.4), dict(nuclear_zeeman=True)])
+@pytest.mark.parametrize("variant", [dict(), dict(exchange=0.4), dict(nuclear_zeeman=True), dict(nuclear_pairs=True)])
 def test_matrix_free_operator_matches_oracle(shim, variant):
     """The per-item Kronecker factors reproduce H x of the oracle's CSR
     Hamiltonian (float accuracy) for anisotropic hyperfine on both radicals,
-    off-axis fields, exchange and nuclear Zeeman terms."""
+    off-axis fields, exchange, nuclear Zeeman terms and couplings between nuclei
+    (isotropic and anisotropic, within a radical and across the two)."""
     s = small_aniso_pair(4, 3, 7, **variant)
     p = s.pack()
     field = (0.03, -0.01, 0.04)
