@@ -246,22 +246,22 @@ def valu_roofline(D, m, n_couplings, state_steps, kernel_s, hbm_traffic, traffic
 # traffic of a launch scales with its state-steps at fixed D.  Each entry names the
 # profile summary it comes from; a kernel change needs a new counter run.
 MEASURED_TRAFFIC_PER_STATE_STEP = {
-    "orientations": (1.0715e+06, "profiles/k_lanczos_orientation_batch_D4096.md"),
+    "orientations": (1.049e+06, "profiles/k_lanczos_orientation_batch_D4096.md"),
     "dense": (2.47e+04, "profiles/k_dense_field_sweep_D512.md"),
-    "pair/-1": (6.613e+06, "profiles/k_lanczos_big_pair_D16384.md"),
-    "arnoldi": (5.097e+06, "profiles/k_arnoldi_singlet_sink_D4096.md"),
+    "pair/-1": (6.630e+06, "profiles/k_lanczos_big_pair_D16384.md"),
+    "arnoldi": (5.330e+06, "profiles/k_arnoldi_singlet_sink_D4096.md"),
     # D = 65536 by launch shape: tiled streaming (-2), split columns (-3)
     "trajectories/-2": (7.54e+07, "profiles/k_lanczos_big_trajectories_D65536.md"),
-    "trajectories/-3": (2.556e+07, "profiles/k_lanczos_split_trajectories_D65536.md"),
+    "trajectories/-3": (2.543e+07, "profiles/k_lanczos_split_trajectories_D65536.md"),
     # config 4's first 24 pairs x 8 fields x 20 steps: mixed D, so only this launch size
-    "uneven": (3.477e+06, "profiles/k_uneven_config4_pairs0_23.md"),
+    "uneven": (3.522e+06, "profiles/k_uneven_config4_pairs0_23.md"),
 }
 # VALU issue fraction (SQ_INSTS_VALU / (GRBM_GUI_ACTIVE / 8 XCDs x 256 CUs)) of the profiled kernels
-PROFILED_VALU_ISSUE = {"orientations": (0.86, "profiles/k_lanczos_orientation_batch_D4096.md"),
-                       "pair/-1": (0.77, "profiles/k_lanczos_big_pair_D16384.md"),
-                       "trajectories/-3": (0.52, "profiles/k_lanczos_split_trajectories_D65536.md"),
+PROFILED_VALU_ISSUE = {"orientations": (0.78, "profiles/k_lanczos_orientation_batch_D4096.md"),
+                       "pair/-1": (0.82, "profiles/k_lanczos_big_pair_D16384.md"),
+                       "trajectories/-3": (0.56, "profiles/k_lanczos_split_trajectories_D65536.md"),
                        "arnoldi": (0.65, "profiles/k_arnoldi_singlet_sink_D4096.md"),
-                       "uneven": (0.76, "profiles/k_uneven_config4_pairs0_23.md")}
+                       "uneven": (0.78, "profiles/k_uneven_config4_pairs0_23.md")}
 
 
 def traffic_for(key, state_steps):
