@@ -671,7 +671,7 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
         a.nnuc = (int)r.nuc.size();
         std::vector<float> all, one;
         int flags = 0;
-        std::vector<int> nfl(kMaxNuclei, 0);
+        std::vector<int> nfl(kMaxNuclei, 0), axial(kMaxNuclei, NUC_A1 | NUC_A2);
         const bool dyn = p->dynamics && r.has_td;
         TimePoint static_part;
         static_part.skip = true;
@@ -682,10 +682,20 @@ void prepare(Batch &b, msg_ctx *ctx, const msg_system *sys, const msg_run_params
                 a.coef_stride = (int)((one.size() + 15) / 16 * 16);
                 all.assign((size_t)n_items * a.coef_stride, 0.f);
             }
-            for (int k = 0; k < kMaxNuclei; ++k) nfl[k] |= nf[k];
+            for (int k = 0; k < kMaxNuclei; ++k) {
+                nfl[k] |= nf[k] & ~(NUC_A1 | NUC_A2);
+                axial[k] &= nf[k];  // the short form only where every item's coupling is axial
+            }
             a.op_flags |= flags;
             std::copy(one.begin(), one.end(), all.begin() + (size_t)it * a.coef_stride);
         }
+        bool all_axial = true;
+        for (int k = 0; k < kMaxNuclei; ++k) {
+            nfl[k] |= axial[k];
+            if (((nfl[k] & NUC_E1) && !(nfl[k] & NUC_A1)) || ((nfl[k] & NUC_E2) && !(nfl[k] & NUC_A2)))
+                all_axial = false;
+        }
+        if (all_axial) a.op_flags |= OP_AXIAL;
         for (const auto &pr : r.nn_pairs) {  // the couplings' nucleus bits follow the flags
             nfl.push_back(pr[0]);
             nfl.push_back(pr[1]);

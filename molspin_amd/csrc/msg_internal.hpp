@@ -43,8 +43,12 @@ constexpr int kPairStride = 32;
 // blocks on both electrons in both orientations), then per nucleus-nucleus coupling 8
 // (for each of the 4 rows: the diagonal, and the entries of the three partners)
 constexpr int kBlkTable = 8 * kMaxNuclei + 8 * kMaxPairs;
-enum NucFlags { NUC_E1 = 1, NUC_E2 = 2, NUC_T = 4 };
-enum OpFlags { OP_EREST = 1 };
+// NUC_A1 / NUC_A2: the nucleus's coupling to electron 1 / 2 is axial about z (isotropic
+// hyperfine among others): h1 (h2) has no x and y part, and C1 (C2) = [[0, 0], [b, 0]]
+// with b = C[2] real.  OP_AXIAL: every electron-nucleus coupling of the system is, and
+// the kernels apply the blocks' two real entries instead of the full 2x2 blocks.
+enum NucFlags { NUC_E1 = 1, NUC_E2 = 2, NUC_T = 4, NUC_A1 = 8, NUC_A2 = 16 };
+enum OpFlags { OP_EREST = 1, OP_AXIAL = 2 };
 
 struct StepArgs {
     // geometry

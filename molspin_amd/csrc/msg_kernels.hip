@@ -119,6 +119,20 @@ __device__ __forceinline__ void apply_field_e1(float fx, float fy, float fz, con
 __device__ __forceinline__ void apply_field_e2(float fx, float fy, float fz, const C4 &x, C4 &y) {
     apply_field<1, 2>(fx, fy, fz, x, y);
 }
+// y += [[0, pd], [pu, 0]] on one electron (pairs as in apply_field), pu and pd real: the
+// flip block of an axial coupling, [[0, 0], [b, 0]] for the nucleus up (pu = b, pd = 0)
+// and its transpose for the nucleus down (8 FMAs instead of the full block's 32).
+template <int D_, int E_>
+__device__ __forceinline__ void apply_axial(float pu, float pd, const C4 &x, C4 &y) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int i = k * E_, j = i + D_;
+        y.c[j].x = fmaf(pu, x.c[i].x, y.c[j].x);
+        y.c[j].y = fmaf(pu, x.c[i].y, y.c[j].y);
+        y.c[i].x = fmaf(pd, x.c[j].x, y.c[i].x);
+        y.c[i].y = fmaf(pd, x.c[j].y, y.c[i].y);
+    }
+}
 // Read-only data that is the same for a whole wavefront (an item's operator
 // coefficients, the nucleus flags) is read through the constant address space: with
 // a wave-uniform address the compiler then loads it with scalar loads into SGPRs,
@@ -231,11 +245,11 @@ static_assert(kTileG == kTileGroups, "tile size shared with the host");
 // kTileBits come from the LDS tile (LA/LB indexed by nu mod kTileG), the others
 // from the vector in global memory (GA/GB).
 // TBL: the coupling blocks come from the LDS table `blk` (fill_block_table).
-template <int NN = -1, bool TILED = false, bool TBL = false>
-__device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict__ co, const float *__restrict__ hd,
-                                      int nu, const C4 &x, const float4 *LA, const float4 *LB,
-                                      const float4 *GA = nullptr, const float4 *GB = nullptr,
-                                      const float4 *blk = nullptr) {
+// AX: every electron-nucleus coupling is axial (OP_AXIAL), applied in its short form.
+template <int NN, bool TILED, bool TBL, bool AX>
+__device__ __forceinline__ C4 apply_h_k(const StepArgs &a, const float *__restrict__ co, const float *__restrict__ hd,
+                                        int nu, const C4 &x, const float4 *LA, const float4 *LB, const float4 *GA,
+                                        const float4 *GB, const float4 *blk) {
     const auto hc = as_const(hd), cc = as_const(co);
     const auto flags = as_const(a.nuc_flags);
     float f1x = hc[0], f1y = hc[1], f1z = hc[2];
@@ -254,16 +268,26 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
                                               : load_group_g(GA, GB, nu ^ (1 << n)));
         float2 m[4];
         if (fl & NUC_E1) {
-            f1x = fmaf(s, cn[0], f1x); f1y = fmaf(s, cn[1], f1y); f1z = fmaf(s, cn[2], f1z);
-            if constexpr (TBL) table_block(blk, n, 0, down, m);
-            else load_block(cn + 7, down, m);
-            apply_e1(m, nb, y);
+            if constexpr (AX) {
+                f1z = fmaf(s, cn[2], f1z);
+                apply_axial<2, 1>(down ? 0.f : cn[11], down ? cn[11] : 0.f, nb, y);
+            } else {
+                f1x = fmaf(s, cn[0], f1x); f1y = fmaf(s, cn[1], f1y); f1z = fmaf(s, cn[2], f1z);
+                if constexpr (TBL) table_block(blk, n, 0, down, m);
+                else load_block(cn + 7, down, m);
+                apply_e1(m, nb, y);
+            }
         }
         if (fl & NUC_E2) {
-            f2x = fmaf(s, cn[3], f2x); f2y = fmaf(s, cn[4], f2y); f2z = fmaf(s, cn[5], f2z);
-            if constexpr (TBL) table_block(blk, n, 1, down, m);
-            else load_block(cn + 15, down, m);
-            apply_e2(m, nb, y);
+            if constexpr (AX) {
+                f2z = fmaf(s, cn[5], f2z);
+                apply_axial<1, 2>(down ? 0.f : cn[19], down ? cn[19] : 0.f, nb, y);
+            } else {
+                f2x = fmaf(s, cn[3], f2x); f2y = fmaf(s, cn[4], f2y); f2z = fmaf(s, cn[5], f2z);
+                if constexpr (TBL) table_block(blk, n, 1, down, m);
+                else load_block(cn + 15, down, m);
+                apply_e2(m, nb, y);
+            }
         }
         if (fl & NUC_T) {
             rr = fmaf(s, cn[6], rr);
@@ -291,6 +315,14 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
     }
     return y;
 }
+template <int NN = -1, bool TILED = false, bool TBL = false>
+__device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict__ co, const float *__restrict__ hd,
+                                      int nu, const C4 &x, const float4 *LA, const float4 *LB,
+                                      const float4 *GA = nullptr, const float4 *GB = nullptr,
+                                      const float4 *blk = nullptr) {
+    if (a.op_flags & OP_AXIAL) return apply_h_k<NN, TILED, TBL, true>(a, co, hd, nu, x, LA, LB, GA, GB, blk);
+    return apply_h_k<NN, TILED, TBL, false>(a, co, hd, nu, x, LA, LB, GA, GB, blk);
+}
 
 // H x for the G groups nu_g = base + g 2^LOGT + t of one thread (t < 2^LOGT, NN
 // nuclei).  The index bits below LOGT are the thread's and shared by its G groups, so
@@ -308,12 +340,14 @@ __device__ __forceinline__ C4 apply_h(const StepArgs &a, const float *__restrict
 // PART 1 forms y from the workgroup's slab alone: every diagonal term and the flips of
 // the nuclei on bits below SB; PART 2 adds the flips of the nuclei on bits from SB up
 // (neighbours from GA, GB) to y.  PART 0 does both.
-template <int NN, int G, int LOGT, bool TBL = false, int MIXM = 0, int PART = 0>
-__device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *__restrict__ co,
-                                               const float *__restrict__ hd, int t, const C4 (&x)[G],
-                                               const float4 *LA, const float4 *LB, C4 (&y)[G], int base = 0,
-                                               const float4 *GA = nullptr, const float4 *GB = nullptr,
-                                               const float4 *blk = nullptr) {
+// AX: every electron-nucleus coupling is axial (OP_AXIAL), applied in its short form.
+// The neighbours of a nucleus on the group bits are loaded for all G groups before they
+// are applied, so their load latencies overlap.
+template <int NN, int G, int LOGT, bool TBL, int MIXM, int PART, bool AX>
+__device__ __forceinline__ void apply_h_groups_k(const StepArgs &a, const float *__restrict__ co,
+                                                 const float *__restrict__ hd, int t, const C4 (&x)[G],
+                                                 const float4 *LA, const float4 *LB, C4 (&y)[G], int base,
+                                                 const float4 *GA, const float4 *GB, const float4 *blk) {
     constexpr int SB = LOGT + Log2<G>::value;
     const auto hc = as_const(hd), cc = as_const(co);
     const auto flags = as_const(a.nuc_flags);
@@ -348,16 +382,30 @@ __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *_
 #pragma unroll
             for (int g = 0; g < G; ++g) nb[g] = load_group(LA, LB, (g << LOGT) + (t ^ (1 << n)));
             if (fl & NUC_E1) {
-                f1x = fmaf(s, cn[0], f1x); f1y = fmaf(s, cn[1], f1y); f1z = fmaf(s, cn[2], f1z);
-                block(n, 0, down, cn + 7, m);
+                if constexpr (AX) {
+                    f1z = fmaf(s, cn[2], f1z);
+                    const float pu = down ? 0.f : cn[11], pd = down ? cn[11] : 0.f;
 #pragma unroll
-                for (int g = 0; g < G; ++g) apply_e1(m, nb[g], y[g]);
+                    for (int g = 0; g < G; ++g) apply_axial<2, 1>(pu, pd, nb[g], y[g]);
+                } else {
+                    f1x = fmaf(s, cn[0], f1x); f1y = fmaf(s, cn[1], f1y); f1z = fmaf(s, cn[2], f1z);
+                    block(n, 0, down, cn + 7, m);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) apply_e1(m, nb[g], y[g]);
+                }
             }
             if (fl & NUC_E2) {
-                f2x = fmaf(s, cn[3], f2x); f2y = fmaf(s, cn[4], f2y); f2z = fmaf(s, cn[5], f2z);
-                block(n, 1, down, cn + 15, m);
+                if constexpr (AX) {
+                    f2z = fmaf(s, cn[5], f2z);
+                    const float pu = down ? 0.f : cn[19], pd = down ? cn[19] : 0.f;
 #pragma unroll
-                for (int g = 0; g < G; ++g) apply_e2(m, nb[g], y[g]);
+                    for (int g = 0; g < G; ++g) apply_axial<1, 2>(pu, pd, nb[g], y[g]);
+                } else {
+                    f2x = fmaf(s, cn[3], f2x); f2y = fmaf(s, cn[4], f2y); f2z = fmaf(s, cn[5], f2z);
+                    block(n, 1, down, cn + 15, m);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) apply_e2(m, nb[g], y[g]);
+                }
             }
             if (fl & NUC_T) {
                 rr = fmaf(s, cn[6], rr);
@@ -415,33 +463,47 @@ __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *_
                 }
             }
         } else {
+            const bool diag = PART != 2, flips = PART == 0 || (PART == 1) == (n < SB);
+            C4 nbg[G];
+            if (flips)
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+                    nbg[g] = n < SB ? load_group(LA, LB, ((g << LOGT) ^ (1 << n)) + t)
+                                    : load_group_g(GA, GB, ((base + (g << LOGT)) ^ (1 << n)) + t);
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 const int hi = base + (g << LOGT);  // the group's index bits from LOGT up
                 const bool down = (hi >> n) & 1;
                 const float s = down ? -1.f : 1.f;
-                const bool diag = PART != 2, flips = PART == 0 || (PART == 1) == (n < SB);
-                C4 nb;
-                if (flips)
-                    nb = n < SB ? load_group(LA, LB, ((g << LOGT) ^ (1 << n)) + t) : load_group_g(GA, GB, (hi ^ (1 << n)) + t);
+                const C4 &nb = nbg[g];
                 if (fl & NUC_E1) {
-                    if (diag) {
-                        e1[g][0] = fmaf(s, cn[0], e1[g][0]); e1[g][1] = fmaf(s, cn[1], e1[g][1]);
-                        e1[g][2] = fmaf(s, cn[2], e1[g][2]);
-                    }
-                    if (flips) {
-                        load_block(cn + 7, down, m);
-                        apply_e1(m, nb, y[g]);
+                    if constexpr (AX) {
+                        if (diag) e1[g][2] = fmaf(s, cn[2], e1[g][2]);
+                        if (flips) apply_axial<2, 1>(down ? 0.f : cn[11], down ? cn[11] : 0.f, nb, y[g]);
+                    } else {
+                        if (diag) {
+                            e1[g][0] = fmaf(s, cn[0], e1[g][0]); e1[g][1] = fmaf(s, cn[1], e1[g][1]);
+                            e1[g][2] = fmaf(s, cn[2], e1[g][2]);
+                        }
+                        if (flips) {
+                            load_block(cn + 7, down, m);
+                            apply_e1(m, nb, y[g]);
+                        }
                     }
                 }
                 if (fl & NUC_E2) {
-                    if (diag) {
-                        e2[g][0] = fmaf(s, cn[3], e2[g][0]); e2[g][1] = fmaf(s, cn[4], e2[g][1]);
-                        e2[g][2] = fmaf(s, cn[5], e2[g][2]);
-                    }
-                    if (flips) {
-                        load_block(cn + 15, down, m);
-                        apply_e2(m, nb, y[g]);
+                    if constexpr (AX) {
+                        if (diag) e2[g][2] = fmaf(s, cn[5], e2[g][2]);
+                        if (flips) apply_axial<1, 2>(down ? 0.f : cn[19], down ? cn[19] : 0.f, nb, y[g]);
+                    } else {
+                        if (diag) {
+                            e2[g][0] = fmaf(s, cn[3], e2[g][0]); e2[g][1] = fmaf(s, cn[4], e2[g][1]);
+                            e2[g][2] = fmaf(s, cn[5], e2[g][2]);
+                        }
+                        if (flips) {
+                            load_block(cn + 15, down, m);
+                            apply_e2(m, nb, y[g]);
+                        }
                     }
                 }
                 if (fl & NUC_T) {
@@ -497,6 +559,18 @@ __device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *_
             apply_field_e2(f2x + e2[g][0], f2y + e2[g][1], f2z + e2[g][2], x[g], y[g]);
         }
     }
+}
+
+template <int NN, int G, int LOGT, bool TBL = false, int MIXM = 0, int PART = 0>
+__device__ __forceinline__ void apply_h_groups(const StepArgs &a, const float *__restrict__ co,
+                                               const float *__restrict__ hd, int t, const C4 (&x)[G],
+                                               const float4 *LA, const float4 *LB, C4 (&y)[G], int base = 0,
+                                               const float4 *GA = nullptr, const float4 *GB = nullptr,
+                                               const float4 *blk = nullptr) {
+    if (a.op_flags & OP_AXIAL)
+        apply_h_groups_k<NN, G, LOGT, TBL, MIXM, PART, true>(a, co, hd, t, x, LA, LB, y, base, GA, GB, blk);
+    else
+        apply_h_groups_k<NN, G, LOGT, TBL, MIXM, PART, false>(a, co, hd, t, x, LA, LB, y, base, GA, GB, blk);
 }
 
 // Electron-pair density of a group, accumulated as 16 reals:
@@ -2093,10 +2167,13 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
                     // coefficients reloaded per iteration (5 % faster at D = 65536: 115 ms
                     // against 122 ms per launch)
                     const float *coj = opaque_uniform(co), *hdj = opaque_uniform(hd);
-                    apply_h_groups<NN, G, Log2<T>::value, false, 0, 1>(a, coj, hdj, t, vjs, cA, cB, w, base);
+                    // the general form only: both forms inlined here push the residual's
+                    // registers to scratch
+                    apply_h_groups_k<NN, G, Log2<T>::value, false, 0, 1, false>(a, coj, hdj, t, vjs, cA, cB, w, base,
+                                                                                nullptr, nullptr, nullptr);
                     if (!slot_wait(bar, S, nbar, a.status, &s_flag)) return;
-                    apply_h_groups<NN, G, Log2<T>::value, false, 0, 2>(a, coj, hdj, t, vjs, cA, cB, w, base, gA,
-                                                                       gA + D4);
+                    apply_h_groups_k<NN, G, Log2<T>::value, false, 0, 2, false>(a, coj, hdj, t, vjs, cA, cB, w, base,
+                                                                                gA, gA + D4, nullptr);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
                         if (j > 0) {

@@ -678,6 +678,27 @@ void fast_coefficients(const Resolved &r, const double *field, std::vector<float
         c[8 + 2 * i] = E[i].real();
         c[8 + 2 * i + 1] = E[i].imag();
     }
+    // axial couplings: the block C = [[0, 0], [b, 0]] with b real and h along z; entries
+    // below 1e-12 of the block's largest (the rounding of a rotated isotropic tensor) are
+    // cleared, far below the FP32 rounding of the kernels' sums
+    for (int k = 0; k < N; ++k) {
+        double *cn = &c[kCoefHead + (size_t)k * kNucStride];
+        for (int e = 0; e < 2; ++e) {
+            if (!(nuc_flags[k] & (e ? NUC_E2 : NUC_E1))) continue;
+            double *h = cn + 3 * e, *b = cn + (e ? 15 : 7);
+            double big = 0.0;
+            for (int q = 0; q < 3; ++q) big = std::max(big, std::abs(h[q]));
+            for (int q = 0; q < 8; ++q) big = std::max(big, std::abs(b[q]));
+            const double eps = 1e-12 * big;
+            const int zero[] = {0, 1, 2, 3, 5, 6, 7};  // C[0], C[1], Im C[2], C[3]
+            bool axial = std::abs(h[0]) <= eps && std::abs(h[1]) <= eps;
+            for (int q : zero) axial = axial && std::abs(b[q]) <= eps;
+            if (!axial) continue;
+            h[0] = h[1] = 0.0;
+            for (int q : zero) b[q] = 0.0;
+            nuc_flags[k] |= e ? NUC_A2 : NUC_A1;
+        }
+    }
     out.resize(c.size());
     for (size_t i = 0; i < c.size(); ++i) out[i] = (float)c[i];
 }

@@ -51,6 +51,16 @@ int shim_dense_hamiltonian(const msg_system_desc *d, const double *field, double
     return 0;
 }
 
+// The per-nucleus flags of the matrix-free coefficient blocks at one field.
+int shim_nuc_flags(const msg_system_desc *d, const double *field, int *flags) {
+    Resolved r;
+    if (!resolve_system(d, r).empty() || !r.fast_ok) return 1;
+    std::vector<float> co;
+    int opf = 0;
+    fast_coefficients(r, field, co, opf, flags);
+    return 0;
+}
+
 // y = H x with the matrix-free coefficient blocks (same algebra as apply_h in
 // msg_kernels.hip); x, y in device order, interleaved complex float.
 int shim_fast_apply(const msg_system_desc *d, const double *field, const float *x, float *y) {
@@ -71,17 +81,25 @@ int shim_fast_apply(const msg_system_desc *d, const double *field, const float *
             const float s = down ? -1.f : 1.f;
             const int nb = nu ^ (1 << n);
             std::complex<float> m[4];
+            // an axial coupling's block: [[0, 0], [b, 0]] with b = Re C[2], transposed for down
+            auto axial = [&](float b) { m[0] = m[3] = 0.f; m[1] = down ? b : 0.f; m[2] = down ? 0.f : b; };
             if (nf[n] & NUC_E1) {
-                for (int a = 0; a < 3; ++a) f1[a] += s * cn[a];
-                apply_block(cn + 7, down, m);
+                if (nf[n] & NUC_A1) { f1[2] += s * cn[2]; axial(cn[11]); }
+                else {
+                    for (int a = 0; a < 3; ++a) f1[a] += s * cn[a];
+                    apply_block(cn + 7, down, m);
+                }
                 for (int b = 0; b < 2; ++b) {
                     out[b] += m[0] * X(nb, b) + m[1] * X(nb, 2 + b);
                     out[2 + b] += m[2] * X(nb, b) + m[3] * X(nb, 2 + b);
                 }
             }
             if (nf[n] & NUC_E2) {
-                for (int a = 0; a < 3; ++a) f2[a] += s * cn[3 + a];
-                apply_block(cn + 15, down, m);
+                if (nf[n] & NUC_A2) { f2[2] += s * cn[5]; axial(cn[19]); }
+                else {
+                    for (int a = 0; a < 3; ++a) f2[a] += s * cn[3 + a];
+                    apply_block(cn + 15, down, m);
+                }
                 for (int a = 0; a < 2; ++a) {
                     out[2 * a] += m[0] * X(nb, 2 * a) + m[1] * X(nb, 2 * a + 1);
                     out[2 * a + 1] += m[2] * X(nb, 2 * a) + m[3] * X(nb, 2 * a + 1);

@@ -35,6 +35,7 @@ def shim():
     L.shim_dense_hamiltonian.argtypes = [P(abi.SystemDesc), C.c_void_p, C.c_void_p]
     L.shim_fast_apply.argtypes = [P(abi.SystemDesc), C.c_void_p, C.c_void_p, C.c_void_p]
     L.shim_trace_samples.argtypes = [P(abi.SystemDesc), C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_void_p]
+    L.shim_nuc_flags.argtypes = [P(abi.SystemDesc), C.c_void_p, C.c_void_p]
     return L
 
 
@@ -104,6 +105,39 @@ def test_matrix_free_operator_matches_oracle(shim, variant):
     yd = np.zeros(p.dim, dtype=np.complex64)
     f = np.array(field)
     assert shim.shim_fast_apply(p.ref(), _p(f), _p(xd.view(np.float32)), _p(yd.view(np.float32))) == 0
+    want = Hr @ x
+    got = yd[perm].astype(np.complex128)
+    assert np.abs(got - want).max() <= 2e-6 * np.abs(want).max()
+
+
+NUC_E1, NUC_E2, NUC_A1, NUC_A2 = 1, 2, 8, 16
+
+
+def test_axial_couplings_take_the_short_form(shim):
+    """Isotropic and axial (diag(a, a, c)) hyperfine couplings get the short flip block
+    (two real entries), the anisotropic ones keep the full 2x2 block; with the short form
+    on, the matrix-free operator still reproduces the oracle's H x, in an off-axis field
+    with a nuclear Zeeman term on an isotropic nucleus."""
+    rng = workloads.SplitMix64(37, 0)
+    axial = [[0.6, 0.0, 0.0], [0.0, 0.6, 0.0], [0.0, 0.0, -0.9]]
+    hf = [0.8, workloads._aniso_hf(rng), axial, 1.3, 0.4, workloads._aniso_hf(rng), 0.07]
+    s = workloads._radical_pair("ax", ["1/2"] * 4, ["1/2"] * 3, hf, 1e-3, field=(0.01, -0.02, 0.05), swept=True)
+    s.add_zeeman(["A0"], (0.0, 0.0, 50.0), prefactor=0.001, swept=False)
+    p = s.pack()
+    field = np.array([0.03, -0.01, 0.04])
+    fl = np.zeros(16, dtype=np.int32)
+    assert shim.shim_nuc_flags(p.ref(), _p(field), _p(fl)) == 0
+    perm = perm_of(shim, p)
+    Hr = ref.hamiltonian(p, tuple(field))
+    # the short form only beside the coupling it describes; 5 of the 7 couplings are axial
+    assert all((not f & NUC_A1 or f & NUC_E1) and (not f & NUC_A2 or f & NUC_E2) for f in fl)
+    assert sum(bool(f & (NUC_A1 | NUC_A2)) for f in fl) == 5
+    rng2 = np.random.default_rng(9)
+    x = rng2.normal(size=p.dim) + 1j * rng2.normal(size=p.dim)
+    xd = np.zeros(p.dim, dtype=np.complex64)
+    xd[perm] = x
+    yd = np.zeros(p.dim, dtype=np.complex64)
+    assert shim.shim_fast_apply(p.ref(), _p(field), _p(xd.view(np.float32)), _p(yd.view(np.float32))) == 0
     want = Hr @ x
     got = yd[perm].astype(np.complex128)
     assert np.abs(got - want).max() <= 2e-6 * np.abs(want).max()
