@@ -11,6 +11,8 @@ host).  A state-step is one column advanced by dt including its observable
 sample.
 
 Other workloads (`--workload`), same JSON line format:
+  isotropic     the orientation batch with each hyperfine tensor replaced by its
+                isotropic part (the kernels' short form of axial couplings);
   trajectories  8 + 6 nuclei (D = 65536), `--trajectories` spin-coherent columns,
                 singlet-triplet dephasing 1e-4 /ns, dt = 1 ns (streaming Lanczos);
   dense         a 7-nucleus pair (D = 512) swept over 64 field strengths on the
@@ -52,8 +54,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["orientations", "trajectories", "dense", "pair", "uncoupled", "rydmr", "arnoldi",
-                             "timeinf", "uneven"],
+    ap.add_argument("--workload", choices=["orientations", "isotropic", "trajectories", "dense", "pair", "uncoupled",
+                                           "rydmr", "arnoldi", "timeinf", "uneven"],
                     default="orientations")
     ap.add_argument("--orientations", type=int, default=32)
     ap.add_argument("--trajectories", type=int, default=512)
@@ -114,7 +116,7 @@ def run_bench(ctx, args):
     import numpy as np
     import torch
 
-    if args.workload == "orientations":
+    if args.workload in ("orientations", "isotropic"):
         run = orientation_batch(ctx, args)
     elif args.workload == "trajectories":
         run = trajectory_batch(ctx, args)
@@ -208,27 +210,28 @@ def lanczos_roofline(D, m, state_steps, kernel_s, traffic, traffic_source=None, 
     return out
 
 
-def operator_flops_per_state_step(D, m, n_couplings, e_block=False):
+def operator_flops_per_state_step(D, m, n_couplings, e_block=False, axial=False):
     """FP32 flops of one state-step of the matrix-free Lanczos kernels, counted per
     electron-pair group (4 amplitudes, D / 4 groups) from the kernel's arithmetic:
     per matvec 32 FMAs per electron-nucleus coupling (a 2x2 complex block on one
-    electron of the flipped neighbour group), 8 for the scalar shift, 24 per electron
-    field, 64 for a general 4x4 electron block; per iteration 20 more for the
-    three-term recurrence (w -= beta v_{j-1}, w -= alpha v_j, 1/beta scaling); and
-    16 m FMAs for the final combination psi = sum_j c_j v_j.  An FMA is 2 flops."""
-    fma_matvec = 32 * n_couplings + 8 + 48 + (64 if e_block else 0)
+    electron of the flipped neighbour group; 8 for an axial coupling, whose block has
+    one real entry per orientation), 8 for the scalar shift, 24 per electron field,
+    64 for a general 4x4 electron block; per iteration 20 more for the three-term
+    recurrence (w -= beta v_{j-1}, w -= alpha v_j, 1/beta scaling); and 16 m FMAs for
+    the final combination psi = sum_j c_j v_j.  An FMA is 2 flops."""
+    fma_matvec = (8 if axial else 32) * n_couplings + 8 + 48 + (64 if e_block else 0)
     fma_group = m * (fma_matvec + 20) + 16 * m
     return 2.0 * fma_group * (D // 4)
 
 
-def valu_roofline(D, m, n_couplings, state_steps, kernel_s, hbm_traffic, traffic_source, valu_issue):
+def valu_roofline(D, m, n_couplings, state_steps, kernel_s, hbm_traffic, traffic_source, valu_issue, axial=False):
     """D <= 8192: the Krylov basis of a column stays in LDS / L2 and the operator's
     FP32 VALU work binds.  `achieved` counts the kernel's FP32 flops
     (operator_flops_per_state_step) against MI355X's published FP32 vector peak
     (157.3 TFLOP/s, which assumes packed FP32); this build compiles with packed FP32
     disabled, so 78.6 TFLOP/s is its attainable ceiling (`frac_of_unpacked`).  The
     byte model is reported beside it (`hbm_model`)."""
-    fl = operator_flops_per_state_step(D, m, n_couplings)
+    fl = operator_flops_per_state_step(D, m, n_couplings, axial=axial)
     achieved = fl * state_steps / kernel_s
     hbm = lanczos_roofline(D, m, state_steps, kernel_s, hbm_traffic, traffic_source)
     return {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_F32_PEAK / 1e12, "unit": "TFLOP/s",
@@ -270,8 +273,12 @@ def traffic_for(key, state_steps):
 
 
 def orientation_batch(ctx, args):
+    """The headline orientation batch; `--workload isotropic` runs it with each
+    hyperfine tensor replaced by its isotropic part (the kernels' short form of axial
+    couplings), checked against the oracle on a short run after timing."""
     from molspin_amd import abi, workloads
-    sysobj, cfg = workloads.config2(n_orientations=1024)
+    iso = args.workload == "isotropic"
+    sysobj, cfg = workloads.config2(n_orientations=1024, isotropic=iso)
     packed = sysobj.pack()
     sysd = ctx.upload(packed)
     D, m, dt = sysd.dim, cfg["krylov_size"], cfg["time_step"]
@@ -290,13 +297,28 @@ def orientation_batch(ctx, args):
         return y
 
     def roof(kernel_s):
+        if iso:
+            return valu_roofline(D, m, 10, state_steps, kernel_s, None, None, None, axial=True)
         tr, src = traffic_for("orientations", state_steps)
         issue = PROFILED_VALU_ISSUE["orientations"]
         return valu_roofline(D, m, 10, state_steps, kernel_s, tr, src,
                              {"value": issue[0], "source": issue[1]})
 
+    def parity_iso(outs):
+        import numpy as np
+        from oracle import binding as ref
+        # 2 orientations, all columns, 10 steps: the device against the FP64 oracle
+        prm_s = abi.run_params(total_time=10 * dt, time_step=dt, method=abi.MSG_METHOD_KRYLOV, krylov_size=m)
+        fs = [cfg["fields"][o] for o in (0, 517)]
+        y, _ = sysd.propagate(prm_s, fields=fs)
+        yo, _ = ref.run_hsgeneral(packed, prm_s, fields=fs, threads=16)
+        return {"oracle_run": "FP64 oracle, 2 orientations x %d columns x 10 steps" % ncols,
+                "tolerance": {"yield_abs": 1e-5}, "max_abs_dPhi": float(np.abs(y - yo).max())}
+
     def parity(outs):
         import numpy as np
+        if iso:
+            return parity_iso(outs)
         fx = load_fixture("cfg2_bench")
         fy, ftr = np.array(fx["yields"]), np.array(fx["trace"])
         res = {"fixture": "tests/golden/cfg2_bench.json (FP64 oracle: Arnoldi m=16, CSR H)",
@@ -324,9 +346,9 @@ def orientation_batch(ctx, args):
                 cpu_prm=abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_KRYLOV),
                 cpu_field=cfg["fields"][0], cpu_columns=None,
                 data="synthetic (seeded radical pair, Fibonacci field directions)",
-                config={"workload": "orientation batch: D=4096 (6+4 spin-1/2 nuclei, anisotropic HF), "
+                config={"workload": "orientation batch: D=4096 (6+4 spin-1/2 nuclei, %s HF), "
                                     "%d orientations x %d columns x %d steps of 2 ns per step, Lanczos m=%d"
-                                    % (B, ncols, intervals, m),
+                                    % ("isotropic" if iso else "anisotropic", B, ncols, intervals, m),
                         "orientations_per_step": B, "columns": ncols, "intervals": intervals,
                         "full_config": "1024 orientations x 1024 columns x 1500 steps (1.57e9 state-steps); "
                                        "each bench step is a 32-orientation, 50-step slice with the same "

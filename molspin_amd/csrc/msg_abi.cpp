@@ -56,10 +56,10 @@ hipError_t launch_accumulate_fro(const void *X, const void *Delta, void *W, int 
 hipError_t launch_pair_block_trace(const void *X, int n, int batch, void *out, hipStream_t st);
 hipError_t launch_pauli_right(const void *M, void *Y, int n, int mu, int batch, hipStream_t st);
 hipError_t launch_corr(const void *S, size_t plane, const double *lam, const double *times, int n, int ns, int batch,
-                       double *C, hipStream_t st);
-hipError_t launch_unc_combine(const double *CA, const double *CB, const double *Q, const double *wy, const double *wt,
-                              int n_items, int ns, int nobs, double dimA, double dimB, double *yields, double *trace,
-                              hipStream_t st);
+                       int nparts, size_t pstride, double *C, hipStream_t st);
+hipError_t launch_unc_combine(const double *CA, const double *CB, int nparts, size_t pstride, const double *Q,
+                              const double *wy, const double *wt, int n_items, int ns, int nobs, double dimA,
+                              double dimB, double *yields, double *trace, hipStream_t st);
 hipError_t launch_to_planes(const void *U, float *dst, int n, int Dp, size_t stride, int batch, hipStream_t st);
 int dense_occupancy(int dpad, size_t lds);
 int dense_threads(int dpad);
@@ -1630,7 +1630,10 @@ void propagate_uncoupled(msg_ctx *ctx, const msg_system *sys, const msg_run_para
     double *dlA = lA.alloc<double>((size_t)chunk * nA, st), *dlB = lB.alloc<double>((size_t)chunk * nB, st);
     double *deA = eA.alloc<double>((size_t)chunk * nA, st), *deB = eB.alloc<double>((size_t)chunk * nB, st);
     rocblas_int *diA = iA.alloc<rocblas_int>(chunk, st), *diB = iB.alloc<rocblas_int>(chunk, st);
-    double *dCA = cA.alloc<double>((size_t)chunk * ns * 9, st), *dCB = cB.alloc<double>((size_t)chunk * ns * 9, st);
+    // k_corr splits each radical's pairs into kCorrParts ranges (more workgroups in
+    // flight); the partial correlations are summed in order by k_unc_combine
+    const size_t pstride = (size_t)chunk * ns * 9;
+    double *dCA = cA.alloc<double>(kCorrParts * pstride, st), *dCB = cB.alloc<double>(kCorrParts * pstride, st);
     std::vector<cd> hostA((size_t)chunk * nA * nA), hostB((size_t)chunk * nB * nB), H;
     std::vector<rocblas_int> info(2 * (size_t)chunk);
     HIPCHECK(hipEventRecord(ctx->ev0, st));
@@ -1659,9 +1662,9 @@ void propagate_uncoupled(msg_ctx *ctx, const msg_system *sys, const msg_run_para
                 HIPCHECK(launch_pauli_right(rd.Hm, rd.Y, rd.n, mu, nb, st));
                 HIPCHECK(zgemm(ctx, rd.Y, rd.Hm, rd.S + mu * plane, rd.n, nb, 1.0, st, true));
             }
-            HIPCHECK(launch_corr(rd.S, plane, rd.lam, tm, rd.n, ns, nb, rd.C, st));
+            HIPCHECK(launch_corr(rd.S, plane, rd.lam, tm, rd.n, ns, nb, kCorrParts, pstride, rd.C, st));
         }
-        HIPCHECK(launch_unc_combine(dCA, dCB, q, wy, wt, nb, ns, nobs, (double)nA, (double)nB,
+        HIPCHECK(launch_unc_combine(dCA, dCB, kCorrParts, pstride, q, wy, wt, nb, ns, nobs, (double)nA, (double)nB,
                                     yo + (size_t)it0 * nobs, to ? to + (size_t)it0 * ns * nobs : nullptr, st));
         HIPCHECK(hipMemcpyAsync(info.data(), diA, nb * sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipMemcpyAsync(info.data() + chunk, diB, nb * sizeof(rocblas_int), hipMemcpyDeviceToHost, st));

@@ -14,6 +14,7 @@ namespace msg {
 
 constexpr int kMaxKrylov = 32;
 constexpr int kMaxObs = 8;
+constexpr int kCorrParts = 4;     // uncoupled method: pair partitions of the correlation sums
 constexpr int kMaxNuclei = 16;     // matrix-free path: D = 4 * 2^N <= 262144
 constexpr int kMaxPairs = 16;      // matrix-free path: nucleus-nucleus couplings
 constexpr int kMaxInitial = 4;

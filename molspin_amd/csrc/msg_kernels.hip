@@ -2771,23 +2771,35 @@ __global__ void k_pauli_right(const double2 *__restrict__ M, double2 *__restrict
     }
 }
 
-constexpr int kCorrSamples = 4;  // samples per lane in k_corr
+constexpr int kCorrSamples = 8;   // samples per lane in k_corr
+constexpr int kCorrThreads = 64;  // lanes per k_corr workgroup
 
-// C[item][s][3 mu + alpha] for the samples times[0..ns): every lane owns
-// kCorrSamples consecutive samples and runs over the pairs g <= h in a fixed order
-// (the pair (h, g) adds the conjugate of (g, h), so off-diagonal pairs count
-// twice); the pair data are the same in every lane.  Phases: e^{i w t} of the
-// lane's first sample by sincos, then the step factor e^{i w dt} (recomputed
-// when a step differs from the uniform one).  Grid: (sample blocks of
-// 256 kCorrSamples, items); no cross-lane traffic, deterministic sums.
-__global__ __launch_bounds__(256) void k_corr(const double2 *__restrict__ S, size_t plane, const double *__restrict__ lam,
-                                              const double *__restrict__ times, int n, int ns,
-                                              double *__restrict__ C) {
-    const int item = blockIdx.y;
+// Partial correlations C_p[item][s][3 mu + alpha] of pair partition p for the samples
+// times[0..ns): the pairs g <= h, in row-major order, are split into gridDim.y equal
+// ranges; every lane owns kCorrSamples consecutive samples and runs over its range's
+// pairs in order (the pair (h, g) adds the conjugate of (g, h), so off-diagonal pairs
+// count twice); the pair data are the same in every lane.  F = S_mu conj(S_alpha) is
+// Hermitian in (mu, alpha), so its 9 terms take 15 FMAs per sample: 3 diagonal (real
+// F) and 3 off-diagonal pairs Re(F e^{iwt}), Re(conj(F) e^{iwt}).  Phases: e^{i w t}
+// of the lane's first sample by sincos, then the step factor e^{i w dt} (recomputed
+// when a step differs from the uniform one).  Grid: (sample blocks, partitions,
+// items); partition p writes at C + p pstride; no cross-lane traffic; the partitions
+// are summed in order by k_unc_combine.
+__global__ __launch_bounds__(kCorrThreads) void k_corr(const double2 *__restrict__ S, size_t plane,
+                                                       const double *__restrict__ lam, const double *__restrict__ times,
+                                                       int n, int ns, size_t pstride, double *__restrict__ C) {
+    const int item = blockIdx.z, part = blockIdx.y, nparts = gridDim.y;
     const int s_first = (blockIdx.x * blockDim.x + threadIdx.x) * kCorrSamples;
     const size_t nn = (size_t)n * n;
     const double2 *Sx = S + (size_t)item * nn, *Sy = Sx + plane, *Sz = Sy + plane;
     const double *l = lam + (size_t)item * n;
+    // this partition's range [k0, k1) of the n (n + 1) / 2 pairs, and its first (g, h)
+    const long long npairs = (long long)n * (n + 1) / 2;
+    const long long k0 = npairs * part / nparts, k1 = npairs * (part + 1) / nparts;
+    int g = 0;
+    long long row0 = 0;  // pair index of (g, g)
+    while (g < n && row0 + (n - g) <= k0) { row0 += n - g; ++g; }
+    int h = g + (int)(k0 - row0);
     double t_s[kCorrSamples];
 #pragma unroll
     for (int j = 0; j < kCorrSamples; ++j) t_s[j] = times[min(s_first + j, ns - 1)];
@@ -2797,70 +2809,85 @@ __global__ __launch_bounds__(256) void k_corr(const double2 *__restrict__ S, siz
     for (int j = 0; j < kCorrSamples; ++j)
 #pragma unroll
         for (int q = 0; q < 9; ++q) acc[j][q] = 0.0;
-    for (int g = 0; g < n; ++g) {
-        const double lg = l[g];
-        for (int h = g; h < n; ++h) {
-            const size_t k = (size_t)g * n + h;
-            const double wt2 = h > g ? 2.0 : 1.0;
-            const double2 sv[3] = {Sx[k], Sy[k], Sz[k]};
-            double2 F[9];
+    for (long long k = k0; k < k1; ++k) {
+        const size_t e = (size_t)g * n + h;
+        const double wt2 = h > g ? 2.0 : 1.0;
+        const double2 sv[3] = {Sx[e], Sy[e], Sz[e]};
+        double fd[3], fr[3], fi[3];  // diagonal |S_m|^2; off-diagonal S_m conj(S_q), (m, q) = (0,1), (0,2), (1,2)
 #pragma unroll
-            for (int m = 0; m < 3; ++m)
+        for (int m = 0; m < 3; ++m) fd[m] = wt2 * (sv[m].x * sv[m].x + sv[m].y * sv[m].y);
 #pragma unroll
-                for (int q = 0; q < 3; ++q)  // S_m conj(S_q)
-                    F[3 * m + q] = make_double2(wt2 * (sv[m].x * sv[q].x + sv[m].y * sv[q].y),
-                                                wt2 * (sv[m].y * sv[q].x - sv[m].x * sv[q].y));
-            const double w = lg - l[h];
-            double ar, ai, zr, zi;
-            sincos(w * t_s[0], &ai, &ar);
-            sincos(w * dt0, &zi, &zr);
+        for (int o = 0; o < 3; ++o) {
+            const int m = o < 2 ? 0 : 1, q = o < 1 ? 1 : 2;
+            fr[o] = wt2 * (sv[m].x * sv[q].x + sv[m].y * sv[q].y);
+            fi[o] = wt2 * (sv[m].y * sv[q].x - sv[m].x * sv[q].y);
+        }
+        const double w = l[g] - l[h];
+        double ar, ai, zr, zi;
+        sincos(w * t_s[0], &ai, &ar);
+        sincos(w * dt0, &zi, &zr);
 #pragma unroll
-            for (int j = 0; j < kCorrSamples; ++j) {
+        for (int j = 0; j < kCorrSamples; ++j) {
 #pragma unroll
-                for (int q = 0; q < 9; ++q) acc[j][q] = fma(F[q].x, ar, fma(-F[q].y, ai, acc[j][q]));
-                if (j + 1 < kCorrSamples) {
-                    const double dtj = t_s[j + 1] - t_s[j];
-                    if (dtj == dt0) {
-                        const double nr = ar * zr - ai * zi, ni = ar * zi + ai * zr;
-                        ar = nr;
-                        ai = ni;
-                    } else {
-                        sincos(w * t_s[j + 1], &ai, &ar);
-                    }
+            for (int m = 0; m < 3; ++m) acc[j][4 * m] = fma(fd[m], ar, acc[j][4 * m]);
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
+                const int m = o < 2 ? 0 : 1, q = o < 1 ? 1 : 2;
+                acc[j][3 * m + q] = fma(fr[o], ar, fma(-fi[o], ai, acc[j][3 * m + q]));
+                acc[j][3 * q + m] = fma(fr[o], ar, fma(fi[o], ai, acc[j][3 * q + m]));
+            }
+            if (j + 1 < kCorrSamples) {
+                const double dtj = t_s[j + 1] - t_s[j];
+                if (dtj == dt0) {
+                    const double nr = ar * zr - ai * zi, ni = ar * zi + ai * zr;
+                    ar = nr;
+                    ai = ni;
+                } else {
+                    sincos(w * t_s[j + 1], &ai, &ar);
                 }
             }
         }
+        if (++h == n) { ++g; h = g; }
     }
+    double *Cp = C + part * pstride;
 #pragma unroll
     for (int j = 0; j < kCorrSamples; ++j)
         if (s_first + j < ns)
 #pragma unroll
-            for (int q = 0; q < 9; ++q) C[((size_t)item * ns + s_first + j) * 9 + q] = acc[j][q];
+            for (int q = 0; q < 9; ++q) Cp[((size_t)item * ns + s_first + j) * 9 + q] = acc[j][q];
 }
 
 // Observables of the uncoupled pair from the radicals' correlations:
 //   T_c(s) = sum_{mu nu alpha beta} Q_c[mu][alpha][nu][beta] CA_{mu alpha}(s) CB_{nu beta}(s)
 // with the identity rows/columns of C filled in (C_00 = dimension, C_0a = C_m0 = 0);
 // Q carries p^c_{mu nu} r_{alpha beta} / (16 Z_A Z_B) and the rate of observable c.
-// One thread per item sums the Kahan-compensated yields in sample order.
-__global__ void k_unc_combine(const double *__restrict__ CA, const double *__restrict__ CB, const double *__restrict__ Q,
-                              const double *__restrict__ wy, const double *__restrict__ wt, int n_items, int ns,
-                              int nobs, double dimA, double dimB, double *__restrict__ yields, double *__restrict__ trace) {
-    const int item = blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= n_items) return;
+// CA, CB hold nparts partial correlations (k_corr's pair partitions, `pstride` apart),
+// summed in partition order.  One workgroup of kCombineThreads per item: lane t takes
+// the samples t, t + kCombineThreads, ... (Kahan-compensated yield sums in sample
+// order), then the lanes' sums are added in a fixed tree order.
+constexpr int kCombineThreads = 256;
+__global__ __launch_bounds__(kCombineThreads) void k_unc_combine(
+    const double *__restrict__ CA, const double *__restrict__ CB, int nparts, size_t pstride,
+    const double *__restrict__ Q, const double *__restrict__ wy, const double *__restrict__ wt, int ns, int nobs,
+    double dimA, double dimB, double *__restrict__ yields, double *__restrict__ trace) {
+    const int item = blockIdx.x, t = threadIdx.x;
+    __shared__ double red[kCombineThreads][kMaxObs];
     double ys[kMaxObs], yc[kMaxObs];
     for (int c = 0; c < nobs; ++c) ys[c] = yc[c] = 0.0;
-    for (int s = 0; s < ns; ++s) {
+    for (int s = t; s < ns; s += kCombineThreads) {
         double a[16], b[16];
-        const double *ca = CA + ((size_t)item * ns + s) * 9, *cb = CB + ((size_t)item * ns + s) * 9;
         for (int i = 0; i < 16; ++i) a[i] = b[i] = 0.0;
         a[0] = dimA;
         b[0] = dimB;
-        for (int m = 0; m < 3; ++m)
-            for (int q = 0; q < 3; ++q) {
-                a[(m + 1) * 4 + q + 1] = ca[3 * m + q];
-                b[(m + 1) * 4 + q + 1] = cb[3 * m + q];
-            }
+        for (int p = 0; p < nparts; ++p) {
+            const double *ca = CA + p * pstride + ((size_t)item * ns + s) * 9;
+            const double *cb = CB + p * pstride + ((size_t)item * ns + s) * 9;
+            for (int m = 0; m < 3; ++m)
+                for (int q = 0; q < 3; ++q) {
+                    a[(m + 1) * 4 + q + 1] += ca[3 * m + q];
+                    b[(m + 1) * 4 + q + 1] += cb[3 * m + q];
+                }
+        }
         for (int c = 0; c < nobs; ++c) {
             const double *Qc = Q + (size_t)c * 256;
             double v = 0.0;
@@ -2877,7 +2904,15 @@ __global__ void k_unc_combine(const double *__restrict__ CA, const double *__res
             if (trace) trace[((size_t)item * ns + s) * nobs + c] = wt[(size_t)s * nobs + c] * v;
         }
     }
-    for (int c = 0; c < nobs; ++c) yields[(size_t)item * nobs + c] = ys[c];
+    for (int c = 0; c < nobs; ++c) red[t][c] = ys[c];
+    __syncthreads();
+    for (int half = kCombineThreads / 2; half > 0; half >>= 1) {
+        if (t < half)
+            for (int c = 0; c < nobs; ++c) red[t][c] += red[t + half][c];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int c = 0; c < nobs; ++c) yields[(size_t)item * nobs + c] = red[0][c];
 }
 
 // Planes of the dense-path operand: U[b] row-major n x n (double2) -> two planes of
@@ -3298,18 +3333,18 @@ hipError_t launch_pauli_right(const void *M, void *Y, int n, int mu, int batch, 
 }
 
 hipError_t launch_corr(const void *S, size_t plane, const double *lam, const double *times, int n, int ns, int batch,
-                       double *C, hipStream_t st) {
-    const int per_block = 256 * kCorrSamples;
-    hipLaunchKernelGGL(k_corr, dim3((ns + per_block - 1) / per_block, batch), dim3(256), 0, st, (const double2 *)S, plane,
-                       lam, times, n, ns, C);
+                       int nparts, size_t pstride, double *C, hipStream_t st) {
+    const int per_block = kCorrThreads * kCorrSamples;
+    hipLaunchKernelGGL(k_corr, dim3((ns + per_block - 1) / per_block, nparts, batch), dim3(kCorrThreads), 0, st,
+                       (const double2 *)S, plane, lam, times, n, ns, pstride, C);
     return hipGetLastError();
 }
 
-hipError_t launch_unc_combine(const double *CA, const double *CB, const double *Q, const double *wy, const double *wt,
-                              int n_items, int ns, int nobs, double dimA, double dimB, double *yields, double *trace,
-                              hipStream_t st) {
-    hipLaunchKernelGGL(k_unc_combine, dim3((n_items + 63) / 64), dim3(64), 0, st, CA, CB, Q, wy, wt, n_items, ns, nobs,
-                       dimA, dimB, yields, trace);
+hipError_t launch_unc_combine(const double *CA, const double *CB, int nparts, size_t pstride, const double *Q,
+                              const double *wy, const double *wt, int n_items, int ns, int nobs, double dimA,
+                              double dimB, double *yields, double *trace, hipStream_t st) {
+    hipLaunchKernelGGL(k_unc_combine, dim3(n_items), dim3(kCombineThreads), 0, st, CA, CB, nparts, pstride, Q, wy, wt,
+                       ns, nobs, dimA, dimB, yields, trace);
     return hipGetLastError();
 }
 

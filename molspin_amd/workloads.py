@@ -106,12 +106,15 @@ def _aniso_hf(rng):
     return m
 
 
-def config2(n_orientations=1024):
+def config2(n_orientations=1024, isotropic=False):
     """6 + 4 spin-1/2 nuclei with anisotropic hyperfine, D = 4096; field
     directions from the Fibonacci sphere (|B| = 0.05 mT); k = 1e-3; 0-3000 ns at
-    dt = 2 ns; Lanczos order 16."""
+    dt = 2 ns; Lanczos order 16.  isotropic: each tensor replaced by its isotropic
+    part (a third of its trace)."""
     rng = SplitMix64(2, 0)
     hf = [_aniso_hf(rng) for _ in range(10)]
+    if isotropic:
+        hf = [(t[0][0] + t[1][1] + t[2][2]) / 3.0 for t in hf]
     s = _radical_pair("cfg2", ["1/2"] * 6, ["1/2"] * 4, hf, 1e-3, swept=True)
     fields = [fibonacci_point(n_orientations, i, 0.05) for i in range(n_orientations)]
     return s, dict(total_time=3000.0, time_step=2.0, krylov_size=16, fields=fields)
