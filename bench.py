@@ -272,6 +272,17 @@ def traffic_for(key, state_steps):
     return (None, None) if t is None else (t[0] * state_steps, t[1] + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)")
 
 
+def oracle_parity(sysd, packed, prm, fields, what, columns=None):
+    """After timing: the device and the FP64 oracle on the same short run (`what`
+    names it), max |dPhi| over every yield."""
+    import numpy as np
+    from oracle import binding as ref
+    y, _ = sysd.propagate(prm, fields=fields, columns=columns)
+    yo, _ = ref.run_hsgeneral(packed, prm, fields=fields, columns=columns, threads=16)
+    return {"oracle_run": "FP64 oracle, " + what, "tolerance": {"yield_abs": 1e-5},
+            "max_abs_dPhi": float(np.abs(y - yo).max())}
+
+
 def orientation_batch(ctx, args):
     """The headline orientation batch; `--workload isotropic` runs it with each
     hyperfine tensor replaced by its isotropic part (the kernels' short form of axial
@@ -431,7 +442,15 @@ def pair_batch(ctx, args):
         tr, src = traffic_for(key, state_steps)
         return lanczos_roofline(D, m, state_steps, kernel_s, tr, src, key)
 
-    return dict(step=step, state_steps=state_steps, roofline=roof, packed=packed,
+    def parity(outs):
+        from oracle import binding as ref
+        cols = ref.coherent_samples(packed, sysobj.state_index("S"), 5, 4) / 2.0
+        prm_s = abi.run_params(total_time=10 * dt, time_step=dt, method=abi.MSG_METHOD_KRYLOV, krylov_size=m,
+                               sampling=abi.MSG_SAMPLING_GIVEN)
+        return oracle_parity(sysd, packed, prm_s, fields, "%d fields x 4 spin-coherent columns x 10 steps"
+                             % len(fields), columns=cols)
+
+    return dict(step=step, state_steps=state_steps, roofline=roof, parity=parity, packed=packed,
                 cpu_prm=abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_KRYLOV),
                 cpu_field=fields[0], cpu_columns=None,
                 data="synthetic (seeded radical pair, log-spaced field strengths)",
@@ -486,9 +505,17 @@ def variant_batch(ctx, args):
                 "unit": "GB/s", "frac": bss * state_steps / kernel_s / HBM_PEAK, "traffic": tr, "traffic_source": src,
                 "kernel_ms": kernel_s * 1e3, "bytes_per_state_step": bss, "state_steps_per_launch": state_steps}
 
+    def parity(outs):
+        from oracle import binding as ref
+        cols = ref.coherent_samples(packed, sysobj.state_index("S"), 7, 16) / 4.0
+        prm_s = abi.run_params(total_time=10 * dt, time_step=dt, method=abi.MSG_METHOD_KRYLOV, krylov_size=m,
+                               dynamics=rydmr, sampling=abi.MSG_SAMPLING_GIVEN)
+        return oracle_parity(sysd, packed, prm_s, fields[:2], "2 orientations x 16 spin-coherent columns x 10 steps",
+                             columns=cols)
+
     what = ("linearly polarized 5 MHz, 50 uT field on both electrons (dynamics=dynamic)" if rydmr
             else "singlet-only recombination 2e-3/ns (Arnoldi on -iH - K)")
-    return dict(step=step, state_steps=state_steps, roofline=roof, packed=packed,
+    return dict(step=step, state_steps=state_steps, roofline=roof, parity=parity, packed=packed,
                 cpu_prm=abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_KRYLOV,
                                        dynamics=rydmr),
                 cpu_field=fields[0], cpu_columns=None,
@@ -517,7 +544,27 @@ def timeinf_batch(ctx, args):
     def step(i):
         return sysd.time_infinity(prm, fields=cfg["fields"])
 
-    return dict(step=step, state_steps=nf, roofline=lambda kernel_s: None, packed=None,
+    def parity(outs):
+        # the oracle's H, K and rho0 at 2 field strengths through scipy's Bartels-Stewart
+        # solver (the oracle's own Kronecker-form solve is D^2 x D^2)
+        import numpy as np
+        import scipy.linalg as sl
+        from oracle import binding as ref
+        picks = (0, 40)
+        y = sysd.time_infinity(prm, fields=[cfg["fields"][i] for i in picks])
+        K, B = ref.reaction(packed), ref.direct_factors(packed)
+        rho0 = B @ B.conj().T
+        err = 0.0
+        for k, i in enumerate(picks):
+            X = sl.solve_continuous_lyapunov(-1j * ref.hamiltonian(packed, cfg["fields"][i]) - K, -rho0)
+            yo = np.array([packed.desc.transitions[t].rate *
+                           np.trace(ref.projector(packed, packed.desc.transitions[t].source_state) @ X).real
+                           for t in ref.packed_transitions(packed)])
+            err = max(err, float(np.abs(y[k] - yo).max()))
+        return {"oracle_run": "scipy Bartels-Stewart on the oracle's H, K and rho0, 2 field strengths",
+                "tolerance": {"yield_abs": 1e-9}, "max_abs_dPhi": err}
+
+    return dict(step=step, state_steps=nf, roofline=lambda kernel_s: None, parity=parity, packed=None,
                 data="synthetic (seeded radical pair, log-spaced field strengths)",
                 unit="yield sets/s",
                 config={"workload": "time-infinity yields: D=512 pair %d (7 spin-1/2 nuclei), unequal sinks "
@@ -611,7 +658,9 @@ def uncoupled_batch(ctx, args):
     ncols = sysd.direct_columns()
     state_steps = B * ncols * intervals
     ns = intervals + 1
-    flops = B * (128 * 128 + 32 * 32) * ns * 24.0  # correlation sums: 9 products + 1 phase update per pair and sample
+    # correlation sums (k_corr): per eigenvalue pair g <= h and sample, 15 FMAs for the 9
+    # Hermitian products and the phase rotation (4 multiplies, 2 adds): 36 flops
+    flops = B * (128 * 129 // 2 + 32 * 33 // 2) * ns * 36.0
 
     def step(i):
         return sysd.propagate(prm, fields=cfg["fields"])[0]
@@ -620,9 +669,19 @@ def uncoupled_batch(ctx, args):
         return {"bound": "fp64-valu", "achieved": flops / kernel_s / 1e12, "peak": 78.6, "unit": "TFLOP/s",
                 "frac": flops / kernel_s / 78.6e12, "traffic": None, "kernel_ms": kernel_s * 1e3,
                 "state_steps_per_launch": state_steps,
-                "note": "device time of the whole method (eigensolver, GEMMs, correlation and combination kernels)"}
+                "note": "k_corr's flops over the device time of the whole method (eigensolver, GEMMs, "
+                        "correlation and combination kernels)"}
 
-    return dict(step=step, state_steps=state_steps, roofline=roof, packed=packed,
+    def parity(outs):
+        import numpy as np
+        fx = load_fixture("cfg2_subset")
+        prm_f = abi.run_params(total_time=fx["total_time"], time_step=fx["time_step"], method=abi.MSG_METHOD_UNCOUPLED)
+        y, _ = sysd.propagate(prm_f, fields=[cfg["fields"][o] for o in fx["orientations"]])
+        return {"fixture": "tests/golden/cfg2_subset.json (FP64 oracle)", "tolerance": {"yield_abs": 1e-5},
+                "orientations": len(fx["orientations"]),
+                "max_abs_dPhi": float(np.abs(y - np.array(fx["yields"])).max())}
+
+    return dict(step=step, state_steps=state_steps, roofline=roof, parity=parity, packed=packed,
                 cpu_prm=abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_KRYLOV),
                 cpu_field=cfg["fields"][0], cpu_columns=None,
                 data="synthetic (seeded radical pair, Fibonacci field directions)",
@@ -657,7 +716,12 @@ def dense_sweep(ctx, args):
                 "kernel_ms": kernel_s * 1e3, "flops_per_state_step": 8.0 * D * D,
                 "state_steps_per_launch": state_steps}
 
-    return dict(step=step, state_steps=state_steps, roofline=roof, packed=packed,
+    def parity(outs):
+        prm_s = abi.run_params(total_time=10 * dt, time_step=dt, method=abi.MSG_METHOD_EXPONENTIAL)
+        return oracle_parity(sysd, packed, prm_s, [cfg["fields"][i] for i in (0, 63)],
+                             "2 field strengths x %d columns x 10 steps" % ncols)
+
+    return dict(step=step, state_steps=state_steps, roofline=roof, parity=parity, packed=packed,
                 cpu_prm=abi.run_params(total_time=dt * 4000, time_step=dt, method=abi.MSG_METHOD_EXPONENTIAL),
                 cpu_field=cfg["fields"][0], cpu_columns=None,
                 data="synthetic (seeded radical pair, log-spaced field strengths)",

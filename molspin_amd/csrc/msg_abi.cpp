@@ -1629,29 +1629,55 @@ void propagate_uncoupled(msg_ctx *ctx, const msg_system *sys, const msg_run_para
     cd *dSA = sA.alloc<cd>((size_t)chunk * 3 * nA * nA, st), *dSB = sB.alloc<cd>((size_t)chunk * 3 * nB * nB, st);
     double *dlA = lA.alloc<double>((size_t)chunk * nA, st), *dlB = lB.alloc<double>((size_t)chunk * nB, st);
     double *deA = eA.alloc<double>((size_t)chunk * nA, st), *deB = eB.alloc<double>((size_t)chunk * nB, st);
-    rocblas_int *diA = iA.alloc<rocblas_int>(chunk, st), *diB = iB.alloc<rocblas_int>(chunk, st);
+    rocblas_int *diA = iA.alloc<rocblas_int>(n_items, st), *diB = iB.alloc<rocblas_int>(n_items, st);
     // k_corr splits each radical's pairs into kCorrParts ranges (more workgroups in
     // flight); the partial correlations are summed in order by k_unc_combine
     const size_t pstride = (size_t)chunk * ns * 9;
     double *dCA = cA.alloc<double>(kCorrParts * pstride, st), *dCB = cB.alloc<double>(kCorrParts * pstride, st);
-    std::vector<cd> hostA((size_t)chunk * nA * nA), hostB((size_t)chunk * nB * nB), H;
-    std::vector<rocblas_int> info(2 * (size_t)chunk);
+    // The radicals' Hamiltonians are linear in the swept field, H(f) = H(0) + sum_a f_a Z_a:
+    // the host builds H(0) and Z_a = H(e_a) - H(0) once per radical, and the device forms
+    // every item's H(f) (k_dyn_generator); without fields every item has the system's own
+    // field, H = H(that field).
+    const int nz = fields ? 3 : 0;
+    DBuf h0A, h0B, zA, zB, dfl, ditem, done;
+    auto unit_parts = [&](const std::vector<int> &spins, int n, DBuf &h0, DBuf &z) {
+        const size_t nn = (size_t)n * n;
+        std::vector<cd> H0, H1, Z((size_t)3 * nn);
+        const double zero[3] = {0.0, 0.0, 0.0};
+        subsystem_hamiltonian(r, spins, fields ? zero : nullptr, H0);
+        for (int ax = 0; ax < nz; ++ax) {
+            double e[3] = {0.0, 0.0, 0.0};
+            e[ax] = 1.0;
+            subsystem_hamiltonian(r, spins, e, H1);
+            for (size_t k = 0; k < nn; ++k) Z[(size_t)ax * nn + k] = H1[k] - H0[k];
+        }
+        HIPCHECK(hipMemcpyAsync(h0.alloc<cd>(nn, st), H0.data(), nn * sizeof(cd), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(z.alloc<cd>(3 * nn, st), Z.data(), 3 * nn * sizeof(cd), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipStreamSynchronize(st));  // H0 and Z are released at scope end
+    };
+    unit_parts(SA, nA, h0A, zA);
+    unit_parts(SB, nB, h0B, zB);
+    {
+        std::vector<double> fh((size_t)3 * n_items, 0.0), one(chunk, 1.0);
+        if (fields) std::copy(fields, fields + (size_t)3 * n_items, fh.begin());
+        std::vector<int> zeros(chunk, 0);
+        HIPCHECK(hipMemcpyAsync(dfl.alloc<double>(fh.size(), st), fh.data(), fh.size() * sizeof(double),
+                                hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(done.alloc<double>(chunk, st), one.data(), chunk * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(ditem.alloc<int>(chunk, st), zeros.data(), chunk * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    std::vector<rocblas_int> info(2 * (size_t)n_items);
     HIPCHECK(hipEventRecord(ctx->ev0, st));
     for (int it0 = 0; it0 < n_items; it0 += chunk) {
         const int nb = std::min(chunk, n_items - it0);
-        for (int q0 = 0; q0 < nb; ++q0) {
-            const double *f = fields ? fields + 3 * (size_t)(it0 + q0) : nullptr;
-            subsystem_hamiltonian(r, SA, f, H);
-            std::copy(H.begin(), H.end(), hostA.begin() + (size_t)q0 * nA * nA);
-            subsystem_hamiltonian(r, SB, f, H);
-            std::copy(H.begin(), H.end(), hostB.begin() + (size_t)q0 * nB * nB);
-        }
-        HIPCHECK(hipMemcpyAsync(dHA, hostA.data(), (size_t)nb * nA * nA * sizeof(cd), hipMemcpyHostToDevice, st));
-        HIPCHECK(hipMemcpyAsync(dHB, hostB.data(), (size_t)nb * nB * nB * sizeof(cd), hipMemcpyHostToDevice, st));
+        const double *fc = dfl.get<double>() + (size_t)3 * it0;
+        HIPCHECK(launch_dyn_generator(h0A.p, zA.p, fc, ditem.get<int>(), done.get<double>(), nz, nA, nb, dHA, st));
+        HIPCHECK(launch_dyn_generator(h0B.p, zB.p, fc, ditem.get<int>(), done.get<double>(), nz, nB, nb, dHB, st));
         // eigenpairs (the row-major view of the solver's output is M = V^dagger, as in
         // msg_symmetric_decay), then S_mu = M (sigma_mu (x) 1) M^dagger and the correlations
         struct Rad { int n; cd *Hm, *Y, *S; double *lam, *off, *C; rocblas_int *inf; };
-        for (const Rad &rd : {Rad{nA, dHA, dYA, dSA, dlA, deA, dCA, diA}, Rad{nB, dHB, dYB, dSB, dlB, deB, dCB, diB}}) {
+        for (const Rad &rd : {Rad{nA, dHA, dYA, dSA, dlA, deA, dCA, diA + it0}, Rad{nB, dHB, dYB, dSB, dlB, deB, dCB, diB + it0}}) {
             if (rocsolver_zheevd_strided_batched(ctx->blas, rocblas_evect_original, rocblas_fill_upper, rd.n,
                                                  reinterpret_cast<rocblas_double_complex *>(rd.Hm), rd.n,
                                                  (rocblas_stride)rd.n * rd.n, rd.lam, rd.n, rd.off, rd.n, rd.inf,
@@ -1666,13 +1692,13 @@ void propagate_uncoupled(msg_ctx *ctx, const msg_system *sys, const msg_run_para
         }
         HIPCHECK(launch_unc_combine(dCA, dCB, kCorrParts, pstride, q, wy, wt, nb, ns, nobs, (double)nA, (double)nB,
                                     yo + (size_t)it0 * nobs, to ? to + (size_t)it0 * ns * nobs : nullptr, st));
-        HIPCHECK(hipMemcpyAsync(info.data(), diA, nb * sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipMemcpyAsync(info.data() + chunk, diB, nb * sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));  // host buffers are refilled by the next chunk
-        for (int q0 = 0; q0 < nb; ++q0)
-            if (info[q0] != 0 || info[chunk + q0] != 0) throw ArgError{MSG_NUMERIC_BREAKDOWN, "eigensolver did not converge"};
     }
     HIPCHECK(hipEventRecord(ctx->ev1, st));
+    HIPCHECK(hipMemcpyAsync(info.data(), diA, n_items * sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(info.data() + n_items, diB, n_items * sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    for (int q0 = 0; q0 < 2 * n_items; ++q0)
+        if (info[q0] != 0) throw ArgError{MSG_NUMERIC_BREAKDOWN, "eigensolver did not converge"};
     if (yields)
         HIPCHECK(hipMemcpyAsync(yields, yo, (size_t)n_items * nobs * sizeof(double), hipMemcpyDeviceToHost, st));
     if (trace)
