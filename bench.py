@@ -250,21 +250,23 @@ def valu_roofline(D, m, n_couplings, state_steps, kernel_s, hbm_traffic, traffic
 # profile summary it comes from; a kernel change needs a new counter run.
 MEASURED_TRAFFIC_PER_STATE_STEP = {
     "orientations": (1.049e+06, "profiles/k_lanczos_orientation_batch_D4096.md"),
+    "isotropic": (1.049e+06, "profiles/k_lanczos_isotropic_orientation_batch_D4096.md"),
     "dense": (2.47e+04, "profiles/k_dense_field_sweep_D512.md"),
-    "pair/-1": (6.630e+06, "profiles/k_lanczos_big_pair_D16384.md"),
-    "arnoldi": (5.330e+06, "profiles/k_arnoldi_singlet_sink_D4096.md"),
+    "pair/-1": (6.622e+06, "profiles/k_lanczos_big_pair_D16384.md"),
+    "arnoldi": (5.470e+06, "profiles/k_arnoldi_singlet_sink_D4096.md"),
     # D = 65536 by launch shape: tiled streaming (-2), split columns (-3)
     "trajectories/-2": (7.54e+07, "profiles/k_lanczos_big_trajectories_D65536.md"),
-    "trajectories/-3": (2.543e+07, "profiles/k_lanczos_split_trajectories_D65536.md"),
+    "trajectories/-3": (2.538e+07, "profiles/k_lanczos_split_trajectories_D65536.md"),
     # config 4's first 24 pairs x 8 fields x 20 steps: mixed D, so only this launch size
-    "uneven": (3.522e+06, "profiles/k_uneven_config4_pairs0_23.md"),
+    "uneven": (3.647e+06, "profiles/k_uneven_config4_pairs0_23.md"),
 }
 # VALU issue fraction (SQ_INSTS_VALU / (GRBM_GUI_ACTIVE / 8 XCDs x 256 CUs)) of the profiled kernels
 PROFILED_VALU_ISSUE = {"orientations": (0.78, "profiles/k_lanczos_orientation_batch_D4096.md"),
+                       "isotropic": (0.50, "profiles/k_lanczos_isotropic_orientation_batch_D4096.md"),
                        "pair/-1": (0.82, "profiles/k_lanczos_big_pair_D16384.md"),
-                       "trajectories/-3": (0.56, "profiles/k_lanczos_split_trajectories_D65536.md"),
-                       "arnoldi": (0.65, "profiles/k_arnoldi_singlet_sink_D4096.md"),
-                       "uneven": (0.78, "profiles/k_uneven_config4_pairs0_23.md")}
+                       "trajectories/-3": (0.55, "profiles/k_lanczos_split_trajectories_D65536.md"),
+                       "arnoldi": (0.60, "profiles/k_arnoldi_singlet_sink_D4096.md"),
+                       "uneven": (0.75, "profiles/k_uneven_config4_pairs0_23.md")}
 
 
 def traffic_for(key, state_steps):
@@ -308,12 +310,11 @@ def orientation_batch(ctx, args):
         return y
 
     def roof(kernel_s):
-        if iso:
-            return valu_roofline(D, m, 10, state_steps, kernel_s, None, None, None, axial=True)
-        tr, src = traffic_for("orientations", state_steps)
-        issue = PROFILED_VALU_ISSUE["orientations"]
+        key = "isotropic" if iso else "orientations"
+        tr, src = traffic_for(key, state_steps)
+        issue = PROFILED_VALU_ISSUE[key]
         return valu_roofline(D, m, 10, state_steps, kernel_s, tr, src,
-                             {"value": issue[0], "source": issue[1]})
+                             {"value": issue[0], "source": issue[1]}, axial=iso)
 
     def parity_iso(outs):
         import numpy as np
