@@ -2054,7 +2054,10 @@ __device__ __forceinline__ bool slot_sum(double (&v)[K], uint64_t *xr, int S, in
     return *s_flag != 0;
 }
 
-template <int NN, int T>
+// AX: the system's electron-nucleus couplings are all axial (OP_AXIAL); each instance
+// inlines one operator form (both forms in one kernel pushed the residual's registers to
+// scratch).
+template <int NN, int T, bool AX>
 __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const StepArgs *__restrict__ pa) {
     const StepArgs &a = *pa;
     constexpr int SG = kSplitSlab, G = SG / T, SB = Log2<SG>::value, S = 1 << (NN - SB);
@@ -2180,13 +2183,11 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
                     // coefficients reloaded per iteration (5 % faster at D = 65536: 115 ms
                     // against 122 ms per launch)
                     const float *coj = opaque_uniform(co), *hdj = opaque_uniform(hd);
-                    // the general form only: both forms inlined here push the residual's
-                    // registers to scratch
-                    apply_h_groups_k<NN, G, Log2<T>::value, false, 0, 1, false>(a, coj, hdj, t, vjs, cA, cB, w, base,
-                                                                                nullptr, nullptr, nullptr);
+                    apply_h_groups_k<NN, G, Log2<T>::value, false, 0, 1, AX>(a, coj, hdj, t, vjs, cA, cB, w, base,
+                                                                             nullptr, nullptr, nullptr);
                     if (!slot_wait(bar, S, nbar, a.status, &s_flag)) return;
-                    apply_h_groups_k<NN, G, Log2<T>::value, false, 0, 2, false>(a, coj, hdj, t, vjs, cA, cB, w, base,
-                                                                                gA, gA + D4, nullptr);
+                    apply_h_groups_k<NN, G, Log2<T>::value, false, 0, 2, AX>(a, coj, hdj, t, vjs, cA, cB, w, base,
+                                                                             gA, gA + D4, nullptr);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
                         if (j > 0) {
@@ -3128,8 +3129,9 @@ hipError_t launch_lanczos(const StepArgs &a, const StepArgs *da, int group_per_t
         void *args[] = {(void *)&da};
 #define X(NN_, T_)                                                                                                \
         if (nn == NN_ && threads == T_)                                                                           \
-            return hipLaunchCooperativeKernel((const void *)k_lanczos_split<NN_, T_>, dim3(slots * S), dim3(T_), args, \
-                                              (unsigned)lds, st);
+            return hipLaunchCooperativeKernel((a.op_flags & OP_AXIAL) ? (const void *)k_lanczos_split<NN_, T_, true>   \
+                                                                      : (const void *)k_lanczos_split<NN_, T_, false>, \
+                                              dim3(slots * S), dim3(T_), args, (unsigned)lds, st);
         MSG_SPLIT_VARIANTS(X)
 #undef X
         return hipErrorInvalidValue;
@@ -3219,7 +3221,10 @@ hipError_t set_lanczos_lds_limits() {
     MSG_DENSE_VARIANTS(X)
 #undef X
 #define X(NN_, T_) \
-    (void)hipFuncSetAttribute((const void *)k_lanczos_split<NN_, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, maxb - 8192);
+    (void)hipFuncSetAttribute((const void *)k_lanczos_split<NN_, T_, false>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                              maxb - 8192);                                                                         \
+    (void)hipFuncSetAttribute((const void *)k_lanczos_split<NN_, T_, true>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                              maxb - 8192);
     MSG_SPLIT_VARIANTS(X)
 #undef X
     (void)hipFuncSetAttribute((const void *)k_lanczos_big<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -3291,7 +3296,8 @@ int lanczos_occupancy(int group_per_thread, int threads, size_t lds, bool arnold
     }
     if (group_per_thread == -3)  // the split kernels of one width share one resource footprint
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                   &nb, threads == 512 ? k_lanczos_split<14, 512> : k_lanczos_split<14, 1024>, threads, lds) == hipSuccess
+                   &nb, threads == 512 ? k_lanczos_split<14, 512, false> : k_lanczos_split<14, 1024, false>, threads,
+                   lds) == hipSuccess
                    ? nb : 1;
     if (group_per_thread == 0)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_lanczos_big<false>, 1024, 0) == hipSuccess ? nb : 1;
