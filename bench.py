@@ -256,7 +256,7 @@ MEASURED_TRAFFIC_PER_STATE_STEP = {
     "arnoldi": (5.470e+06, "profiles/k_arnoldi_singlet_sink_D4096.md"),
     # D = 65536 by launch shape: tiled streaming (-2), split columns (-3)
     "trajectories/-2": (7.54e+07, "profiles/k_lanczos_big_trajectories_D65536.md"),
-    "trajectories/-3": (2.538e+07, "profiles/k_lanczos_split_trajectories_D65536.md"),
+    "trajectories/-3": (2.527e+07, "profiles/k_lanczos_split_trajectories_D65536.md"),
     # config 4's first 24 pairs x 8 fields x 20 steps: mixed D, so only this launch size
     "uneven": (3.647e+06, "profiles/k_uneven_config4_pairs0_23.md"),
 }
@@ -264,7 +264,7 @@ MEASURED_TRAFFIC_PER_STATE_STEP = {
 PROFILED_VALU_ISSUE = {"orientations": (0.78, "profiles/k_lanczos_orientation_batch_D4096.md"),
                        "isotropic": (0.50, "profiles/k_lanczos_isotropic_orientation_batch_D4096.md"),
                        "pair/-1": (0.82, "profiles/k_lanczos_big_pair_D16384.md"),
-                       "trajectories/-3": (0.55, "profiles/k_lanczos_split_trajectories_D65536.md"),
+                       "trajectories/-3": (0.31, "profiles/k_lanczos_split_trajectories_D65536.md"),
                        "arnoldi": (0.60, "profiles/k_arnoldi_singlet_sink_D4096.md"),
                        "uneven": (0.75, "profiles/k_uneven_config4_pairs0_23.md")}
 
