@@ -2977,23 +2977,22 @@ __global__ void k_to_planes(const double2 *__restrict__ U, float *__restrict__ d
 // call in the LDS address space, so the body's accesses stay LDS instructions.
 #define MSG_LDS __attribute__((address_space(3)))
 // Function arguments arrive in vector registers, and loads through them are vector
-// loads; the entry's arguments are the same for the whole workgroup, so the body works
-// on a copy whose words are made uniform (scalar registers), as in a dedicated kernel.
-__device__ __forceinline__ void uniform_copy(const StepArgs *src, StepArgs &dst) {
-    constexpr int W = sizeof(StepArgs) / 4;
-    static_assert(sizeof(StepArgs) % 4 == 0, "whole words");
-    const uint32_t *s = reinterpret_cast<const uint32_t *>(src);
-    uint32_t *d = reinterpret_cast<uint32_t *>(&dst);
-#pragma unroll
-    for (int i = 0; i < W; ++i) d[i] = __builtin_amdgcn_readfirstlane(s[i]);
+// loads; the entry's arguments are the same for the whole workgroup, so the body reads
+// them through a wave-uniform pointer in the constant address space: scalar loads where
+// a field is used, as in a dedicated kernel.  (A copy of all their words made uniform
+// and held in scalar registers spilled: the launch took 1418 ms against 1377 ms.)
+__device__ __forceinline__ const StepArgs &uniform_args(const StepArgs *a) {
+    const uint64_t v = reinterpret_cast<uint64_t>(a);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    const ConstPtr<StepArgs> c = (ConstPtr<StepArgs>)(((uint64_t)hi << 32) | lo);
+    return *(const StepArgs *)c;
 }
 template <int G, int T, int P>
 __device__ __noinline__ void uneven_lz(const StepArgs *__restrict__ a, int u0, int nact, float *bas, MSG_LDS float4 *lds,
                                        MSG_LDS double *scr, MSG_LDS double *ys, MSG_LDS double *yc,
                                        MSG_LDS float2 *coef, MSG_LDS float4 *blk) {
     const BodyShared sh{(double *)ys, (double *)yc, (float2 *)coef, (float4 *)blk, nullptr};
-    StepArgs ua;
-    uniform_copy(a, ua);
+    const StepArgs &ua = uniform_args(a);
     lanczos_columns<G, T, P, true, false>(ua, __builtin_amdgcn_readfirstlane(u0),
                                           __builtin_amdgcn_readfirstlane(nact), bas, (float4 *)lds, (double *)scr, sh);
 }
@@ -3001,8 +3000,7 @@ __device__ __noinline__ void uneven_big(const StepArgs *__restrict__ a, int u, f
                                         MSG_LDS double *ys, MSG_LDS double *yc, MSG_LDS float2 *coef,
                                         MSG_LDS float4 *blk) {
     const BodyShared sh{(double *)ys, (double *)yc, (float2 *)coef, (float4 *)blk, nullptr};
-    StepArgs ua;
-    uniform_copy(a, ua);
+    const StepArgs &ua = uniform_args(a);
     lanczos_big_column<true, false>(ua, __builtin_amdgcn_readfirstlane(u), bas, (float4 *)lds,
                                     (double *)scr, sh);
 }
@@ -3010,8 +3008,7 @@ template <int NT, int NW, int P>
 __device__ __noinline__ void uneven_dense(const StepArgs *__restrict__ a, int u0, int nact, MSG_LDS float *lds, MSG_LDS double *scr,
                                           MSG_LDS double *ys, MSG_LDS double *yc, MSG_LDS double *colsum) {
     const BodyShared sh{(double *)ys, (double *)yc, nullptr, nullptr, (double *)colsum};
-    StepArgs ua;
-    uniform_copy(a, ua);
+    const StepArgs &ua = uniform_args(a);
     dense_tiles<NT, NW, P>(ua, __builtin_amdgcn_readfirstlane(u0), __builtin_amdgcn_readfirstlane(nact),
                            (float *)lds, (double *)scr, sh);
 }
