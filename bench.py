@@ -258,7 +258,7 @@ MEASURED_TRAFFIC_PER_STATE_STEP = {
     "trajectories/-2": (7.54e+07, "profiles/k_lanczos_big_trajectories_D65536.md"),
     "trajectories/-3": (2.527e+07, "profiles/k_lanczos_split_trajectories_D65536.md"),
     # config 4's first 24 pairs x 8 fields x 20 steps: mixed D, so only this launch size
-    "uneven": (3.647e+06, "profiles/k_uneven_config4_pairs0_23.md"),
+    "uneven": (3.555e+06, "profiles/k_uneven_config4_pairs0_23.md"),
 }
 # VALU issue fraction (SQ_INSTS_VALU / (GRBM_GUI_ACTIVE / 8 XCDs x 256 CUs)) of the profiled kernels
 PROFILED_VALU_ISSUE = {"orientations": (0.78, "profiles/k_lanczos_orientation_batch_D4096.md"),
@@ -266,7 +266,7 @@ PROFILED_VALU_ISSUE = {"orientations": (0.78, "profiles/k_lanczos_orientation_ba
                        "pair/-1": (0.82, "profiles/k_lanczos_big_pair_D16384.md"),
                        "trajectories/-3": (0.31, "profiles/k_lanczos_split_trajectories_D65536.md"),
                        "arnoldi": (0.60, "profiles/k_arnoldi_singlet_sink_D4096.md"),
-                       "uneven": (0.75, "profiles/k_uneven_config4_pairs0_23.md")}
+                       "uneven": (0.79, "profiles/k_uneven_config4_pairs0_23.md")}
 
 
 def traffic_for(key, state_steps):
