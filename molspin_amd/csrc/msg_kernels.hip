@@ -2185,12 +2185,10 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
                     const float *coj = opaque_uniform(co), *hdj = opaque_uniform(hd);
                     apply_h_groups_k<NN, G, Log2<T>::value, false, 0, 1, AX>(a, coj, hdj, t, vjs, cA, cB, w, base,
                                                                              nullptr, nullptr, nullptr);
-                    if (!slot_wait(bar, S, nbar, a.status, &s_flag)) return;
-                    apply_h_groups_k<NN, G, Log2<T>::value, false, 0, 2, AX>(a, coj, hdj, t, vjs, cA, cB, w, base,
-                                                                             gA, gA + D4, nullptr);
+                    // - beta_{j-1} v_{j-1} from the own slab, also before the wait
+                    if (j > 0)
 #pragma unroll
-                    for (int g = 0; g < G; ++g) {
-                        if (j > 0) {
+                        for (int g = 0; g < G; ++g) {
                             const C4 vp = load_group(pA, pB, g * T + t);
 #pragma unroll
                             for (int c = 0; c < 4; ++c) {
@@ -2198,6 +2196,11 @@ __global__ __launch_bounds__(T, T >= 1024 ? 4 : 2) void k_lanczos_split(const St
                                 w[g].c[c].y = fmaf(-fb, vp.c[c].y, w[g].c[c].y);
                             }
                         }
+                    if (!slot_wait(bar, S, nbar, a.status, &s_flag)) return;
+                    apply_h_groups_k<NN, G, Log2<T>::value, false, 0, 2, AX>(a, coj, hdj, t, vjs, cA, cB, w, base,
+                                                                             gA, gA + D4, nullptr);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
                         // the group's 8 products in FP32, accumulated over groups in FP64
                         float ra = 0.f, rb = 0.f;
 #pragma unroll
