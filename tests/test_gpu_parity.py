@@ -557,6 +557,26 @@ def test_uncoupled_config1_matches_oracle(ctx):
         assert trace_err(tr[0, :, c], want[:, c]) <= TRACE_TOL, c
 
 
+def test_uncoupled_bare_radical(ctx):
+    """Radical B without nuclei (dimension 2: its 3 eigenvalue pairs leave one of the
+    correlation kernel's 4 pair ranges empty) beside 3 nuclei on A, at 3 field
+    orientations, with traces: the uncoupled method against the oracle's propagation."""
+    rng = workloads.SplitMix64(79, 0)
+    s = workloads._radical_pair("bare", ["1/2"] * 3, [], [workloads._aniso_hf(rng) for _ in range(3)], 2e-3,
+                                swept=True)
+    p = s.pack()
+    sysd = ctx.upload(p)
+    fields = [workloads.fibonacci_point(16, i, 0.05) for i in (1, 6, 11)]
+    prm = abi.run_params(total_time=600.0, time_step=2.0, method=abi.MSG_METHOD_UNCOUPLED)
+    y, tr = sysd.propagate(prm, fields=fields, trace=True)
+    yo, tro = ref.run_hsgeneral(p, abi.run_params(total_time=600.0, time_step=2.0, method=abi.MSG_METHOD_KRYLOV),
+                                fields=fields, trace=True)
+    print("uncoupled bare", y[:, 0], "oracle", yo[:, 0])
+    assert np.abs(y - yo).max() <= YIELD_TOL
+    for k in range(len(fields)):
+        assert trace_err(tr[k], tro[k]) <= TRACE_TOL, k
+
+
 def test_uncoupled_config2_against_fixture(ctx):
     """Config 2 (D = 4096 as radicals of 128 and 32): the 16 fixture orientations."""
     fx = load_fixture("cfg2_subset")
