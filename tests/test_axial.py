@@ -3,9 +3,9 @@ when every such coupling of a system is axial about z, the kernels apply each fl
 as its two real entries (SpinSpace_hamiltonians.cpp:438-535 builds the same operator from
 the full tensors).  Isotropic radical pairs, some with an axial tensor diag(a, a, c) and a
 nuclear Zeeman term in an off-axis field, on every kernel that takes the short form:
-Lanczos in every launch shape, the LDS-mirrored and plain streaming kernels at
-D = 16384, Arnoldi (unequal rates), RK4, the mixed-spin kernel and the uneven launch,
-against the FP64 oracle.  |dPhi| <= 1e-5 on yields, 1e-4 max-relative on traces."""
+Lanczos in every launch shape (also beside couplings between nuclei), the LDS-mirrored
+and plain streaming kernels at D = 16384, Arnoldi (unequal rates), RK4, the mixed-spin
+kernel and the uneven launch, against the FP64 oracle.  |dPhi| <= 1e-5 on yields, 1e-4 max-relative on traces."""
 import math
 
 import numpy as np
@@ -96,6 +96,16 @@ def test_streaming_d16384(ctx, shape, monkeypatch):
                          sampling=abi.MSG_SAMPLING_GIVEN)
     sysd = check(ctx, s, prm, columns=cols, trace=False)
     assert sysd.dim == 16384
+
+
+@pytest.mark.parametrize("groups", ["1", "2"])
+def test_with_nuclear_couplings(ctx, groups, monkeypatch):
+    """The short form beside couplings between nuclei (isotropic and anisotropic, within
+    a radical and across the two): D = 1024, one and two groups per thread."""
+    monkeypatch.setenv("MSG_LANCZOS_G", groups)
+    s = workloads.add_nuclear_couplings(iso_pair(5, 3, 9), 5, 3, 9)
+    prm = abi.run_params(total_time=40.0, time_step=2.0, method=abi.MSG_METHOD_KRYLOV)
+    check(ctx, s, prm)
 
 
 def test_arnoldi_unequal_rates(ctx):
